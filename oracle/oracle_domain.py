@@ -1,0 +1,307 @@
+"""Host orchestration of the oracle: a minimal restatement of the reference's
+evolve loop for the DE flow algorithms, driving either CPU kernel set.
+
+TEST INFRASTRUCTURE ONLY (imported by tests, the smoke check and the CPU
+baseline of the benchmark; never by the product package).
+
+Restates (paths relative to the anuga_core tree):
+  * DE presets            anuga/shallow_water/shallow_water_domain.py:583-707,
+                          config constants anuga/config.py
+  * evolve / _evolve_base shallow_water_domain.py:2300-2407,
+                          anuga/abstract_2d_finite_volumes/generic_domain.py:1715-1912
+  * euler / rk2 steps     generic_domain.py:1914-2051
+  * update_timestep       generic_domain.py:2349-2415
+  * update_boundary       generic_domain.py:2288-2306 with
+                          Reflective_boundary.evaluate_segment (boundaries.py:235-292),
+                          Transmissive_boundary / Dirichlet_boundary.evaluate_segment
+                          (generic_boundary_conditions.py:173-264)
+  * update_conserved_quantities shallow_water_domain.py:2089-2163
+  * boundary flux integral anuga/operators/boundary_flux_integral_operator.py
+  * get_water_volume      shallow_water_domain.py:1626-1653
+Boundary objects are duck-typed: `.kind` in {'reflective', 'transmissive',
+'dirichlet'} and, for Dirichlet, `.dirichlet_values`.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+EPSILON = 1.0e-12          # anuga/config.py:12
+G = 9.8                    # anuga/config.py:45
+MAX_TIMESTEP = 1.0e+3      # anuga/config.py:154
+MIN_TIMESTEP = 1.0e-6      # anuga/config.py:153
+MAX_SMALLSTEPS = 50        # anuga/config.py:155
+
+
+class OracleDomain:
+    def __init__(self, mesh, kernels, flow_algorithm="DE0"):
+        self.mesh = mesh
+        N = self.N = mesh.number_of_triangles
+        M = self.M = mesh.boundary_length
+        self.neighbours = np.ascontiguousarray(mesh.neighbours.ravel())
+        self.neighbour_edges = np.ascontiguousarray(mesh.neighbour_edges.ravel())
+        self.surrogate_neighbours = np.ascontiguousarray(mesh.surrogate_neighbours.ravel())
+        self.number_of_boundaries = np.ascontiguousarray(mesh.number_of_boundaries)
+        self.tri_full_flag = np.ones(N, np.int64)
+        self.normals = np.ascontiguousarray(mesh.normals.ravel())
+        self.edgelengths = np.ascontiguousarray(mesh.edgelengths.ravel())
+        self.radii = np.ascontiguousarray(mesh.radii)
+        self.areas = np.ascontiguousarray(mesh.areas)
+        self.centroids = np.ascontiguousarray(mesh.centroid_coordinates.ravel())
+        self.edge_midpoints = np.ascontiguousarray(mesh.edge_midpoint_coordinates.ravel())
+        self.vertex_coordinates = np.ascontiguousarray(mesh.vertex_coordinates.ravel())
+        z = lambda n: np.zeros(n)
+        for q in ("stage", "xmom", "ymom", "elev", "height", "friction"):
+            setattr(self, q + "_c", z(N))
+            setattr(self, q + "_e", z(3 * N))
+            setattr(self, q + "_v", z(3 * N))
+        for q in ("stage", "xmom", "ymom"):
+            setattr(self, q + "_b", z(max(M, 1)))
+            setattr(self, q + "_ex", z(N))
+            setattr(self, q + "_si", z(N))
+            setattr(self, q + "_bk", z(N))
+        self.x_centroid_work = z(N)
+        self.y_centroid_work = z(N)
+        self.max_speed = z(N)
+        self.edge_timestep = z(3 * N) + 1.0e100
+        self.boundary_flux_sum = z(3)
+        self.boundary_flux_integral = 0.0
+
+        self.g = G
+        self.epsilon = EPSILON
+        self.H0 = 1.0e-3
+        self.low_froude = 0
+        self.extrapolate_velocity_second_order = 1
+        self.evolve_max_timestep = MAX_TIMESTEP
+        self.evolve_min_timestep = MIN_TIMESTEP
+        self.max_smallsteps = MAX_SMALLSTEPS
+        self.fixed_flux_timestep = None
+        self.boundary_map = None
+        self.kernels = kernels
+        kernels.bind(self)
+        self.set_flow_algorithm(flow_algorithm)
+
+        self.starttime = 0.0
+        self.relative_time = 0.0
+        self.evolved_called = False
+        self.smallsteps = 0
+        self.timestep = 0.0
+        self.flux_timestep = 0.0
+        self.dt_history = []
+        self.mass_error = 0.0
+        self.negative_cells = 0
+
+    # -- presets ------------------------------------------------------------
+    def set_flow_algorithm(self, algorithm="DE0"):
+        algorithm = str(algorithm).replace(".", "_")
+        if algorithm == "DE0":
+            self.CFL = 0.9
+            self.timestepping_method = "euler"
+            self.minimum_allowed_height = 1.0e-12
+            b = 0.5
+        elif algorithm == "DE1":
+            self.CFL = 1.0
+            self.timestepping_method = "rk2"
+            self.minimum_allowed_height = 1.0e-5
+            b = 1.0
+        else:
+            raise ValueError("flow algorithm %s is not supported by the oracle" % algorithm)
+        self.flow_algorithm = algorithm
+        self.beta_w = self.beta_uh = self.beta_vh = b
+        self.beta_w_dry = self.beta_uh_dry = self.beta_vh_dry = 0.0
+        self.default_order = 2
+
+    # -- quantities -----------------------------------------------------------
+    _NAMES = {"stage": "stage", "xmomentum": "xmom", "ymomentum": "ymom",
+              "elevation": "elev", "friction": "friction", "height": "height"}
+
+    def set_centroid_values(self, name, values):
+        getattr(self, self._NAMES[name] + "_c")[:] = values
+
+    def set_vertex_values(self, name, vertex_values):
+        """(N,3) vertex values -> vertex, edge and centroid values
+        (quantity.c:665-688, interpolate)."""
+        q = self._NAMES[name]
+        v = np.asarray(vertex_values, np.float64).reshape(self.N, 3)
+        getattr(self, q + "_v")[:] = v.ravel()
+        q0, q1, q2 = v[:, 0], v[:, 1], v[:, 2]
+        getattr(self, q + "_c")[:] = (q0 + q1 + q2) / 3.0
+        e = getattr(self, q + "_e").reshape(self.N, 3)
+        e[:, 0] = 0.5 * (q1 + q2)
+        e[:, 1] = 0.5 * (q0 + q2)
+        e[:, 2] = 0.5 * (q0 + q1)
+
+    def get_centroid_values(self, name):
+        return getattr(self, self._NAMES[name] + "_c").copy()
+
+    def set_boundary(self, boundary_map):
+        self.boundary_map = dict(boundary_map)
+
+    # -- boundary conditions ---------------------------------------------------
+    def update_boundary(self):
+        m = self.mesh
+        nrm = self.normals.reshape(self.N, 6)
+        for tag, ids in m.tag_boundary_cells.items():
+            B = self.boundary_map.get(tag) if self.boundary_map else None
+            if B is None:
+                continue
+            ids = np.asarray(ids, np.int64)
+            vol = m.boundary_cells[ids]
+            edge = m.boundary_edges[ids]
+            se = self.stage_e.reshape(self.N, 3)
+            xe = self.xmom_e.reshape(self.N, 3)
+            ye = self.ymom_e.reshape(self.N, 3)
+            if B.kind == "reflective":
+                n1 = nrm[vol, 2 * edge]
+                n2 = nrm[vol, 2 * edge + 1]
+                self.stage_b[ids] = se[vol, edge]
+                q1 = xe[vol, edge]
+                q2 = ye[vol, edge]
+                r1 = -q1 * n1 - q2 * n2
+                r2 = -q1 * n2 + q2 * n1
+                self.xmom_b[ids] = n1 * r1 - n2 * r2
+                self.ymom_b[ids] = n2 * r1 + n1 * r2
+            elif B.kind == "transmissive":
+                self.stage_b[ids] = se[vol, edge]
+                self.xmom_b[ids] = xe[vol, edge]
+                self.ymom_b[ids] = ye[vol, edge]
+            elif B.kind == "dirichlet":
+                v = np.asarray(B.dirichlet_values, np.float64)
+                self.stage_b[ids] = v[0]
+                self.xmom_b[ids] = v[1]
+                self.ymom_b[ids] = v[2]
+            else:
+                raise ValueError("unsupported boundary kind %r" % B.kind)
+
+    # -- step pieces --------------------------------------------------------------
+    def distribute_to_vertices_and_edges(self):
+        self.mass_error += self.kernels.protect()
+        if self.kernels.extrapolate() == -1:
+            raise RuntimeError("Internal neighbour not found")
+
+    def compute_fluxes(self, substep):
+        self.flux_timestep = self.kernels.compute_fluxes(substep, self.evolve_max_timestep)
+
+    def compute_forcing_terms(self):
+        self.kernels.friction()
+
+    def update_conserved_quantities(self):
+        self.kernels.update(self.timestep)
+        self.negative_cells += self.kernels.fix_negative()
+
+    def update_timestep(self, yieldstep, finaltime):
+        if self.fixed_flux_timestep is not None:
+            self.flux_timestep = self.fixed_flux_timestep
+        timestep = min(self.CFL * self.flux_timestep, self.evolve_max_timestep)
+        self.recorded_max_timestep = max(timestep, self.recorded_max_timestep)
+        self.recorded_min_timestep = min(timestep, self.recorded_min_timestep)
+        if timestep < self.evolve_min_timestep:
+            self.smallsteps += 1
+            if self.smallsteps > self.max_smallsteps:
+                self.smallsteps = 0
+                if self._order_ == 1:
+                    raise Exception("WARNING: Too small timestep %.16f reached even after "
+                                    "%d steps of 1 order scheme" % (timestep, self.max_smallsteps))
+                self._order_ = 1
+        else:
+            self.smallsteps = 0
+            if self._order_ == 1 and self.default_order == 2:
+                self._order_ = 2
+        if self.relative_finaltime is not None and \
+                self.relative_time + timestep > self.relative_finaltime:
+            timestep = self.relative_finaltime - self.relative_time
+        if self.relative_time + timestep > self.relative_yieldtime:
+            timestep = self.relative_yieldtime - self.relative_time
+        self.timestep = timestep
+
+    def evolve_one_euler_step(self, yieldstep, finaltime):
+        self.distribute_to_vertices_and_edges()
+        self.update_boundary()
+        self.compute_fluxes(0)
+        self.compute_forcing_terms()
+        self.update_timestep(yieldstep, finaltime)
+        self.update_conserved_quantities()
+
+    def evolve_one_rk2_step(self, yieldstep, finaltime):
+        self.kernels.backup()
+        self.distribute_to_vertices_and_edges()
+        self.update_boundary()
+        self.compute_fluxes(0)
+        self.compute_forcing_terms()
+        self.update_timestep(yieldstep, finaltime)
+        self.update_conserved_quantities()
+        self.relative_time = self.relative_time + self.timestep
+        self.distribute_to_vertices_and_edges()
+        self.update_boundary()
+        self.compute_fluxes(1)
+        self.compute_forcing_terms()
+        self.update_conserved_quantities()
+        self.kernels.saxpy(0.5, 0.5)
+
+    def _apply_boundary_flux_integral(self):
+        dt = self.timestep
+        if self.timestepping_method == "euler":
+            self.boundary_flux_integral = self.boundary_flux_integral + dt * self.boundary_flux_sum[0]
+        else:
+            self.boundary_flux_integral = (self.boundary_flux_integral
+                                           + 0.5 * dt * self.boundary_flux_sum[0:2].sum())
+        self.boundary_flux_sum[:] = 0.0
+
+    # -- evolve ---------------------------------------------------------------------
+    def get_time(self):
+        return self.starttime + self.relative_time
+
+    def evolve(self, yieldstep=None, finaltime=None, max_steps=None):
+        """Generator over yield times; `max_steps` stops after that many steps
+        (a test convenience, not in the reference)."""
+        if self.boundary_map is None:
+            raise AssertionError("Boundary tags must be bound to boundary objects before evolving")
+        self.distribute_to_vertices_and_edges()
+        skip_initial_step = self.evolved_called
+        self.evolved_called = True
+        yieldstep = self.evolve_max_timestep if yieldstep is None else float(yieldstep)
+        self._order_ = self.default_order
+        self.relative_finaltime = float(finaltime) - self.starttime
+        if self.relative_finaltime < self.relative_time:
+            return
+        self.relative_yieldtime = self.relative_time + yieldstep
+        self.recorded_min_timestep = self.evolve_max_timestep
+        self.recorded_max_timestep = self.evolve_min_timestep
+        self.number_of_steps = 0
+        if not skip_initial_step:
+            self.distribute_to_vertices_and_edges()
+            self.update_boundary()
+            yield self.get_time()
+        steps = 0
+        while True:
+            initial = self.relative_time
+            if self.timestepping_method == "euler":
+                self.evolve_one_euler_step(yieldstep, self.relative_finaltime)
+            else:
+                self.evolve_one_rk2_step(yieldstep, self.relative_finaltime)
+            self._apply_boundary_flux_integral()
+            self.relative_time = initial + self.timestep
+            self.dt_history.append(self.timestep)
+            self.number_of_steps += 1
+            steps += 1
+            if self.relative_time >= self.relative_finaltime - EPSILON:
+                if self.relative_time > self.relative_finaltime:
+                    raise Exception("WARNING (domain.py): time overshot finaltime. ")
+                self.relative_time = self.relative_finaltime
+                self.distribute_to_vertices_and_edges()
+                self.update_boundary()
+                yield self.get_time()
+                break
+            if max_steps is not None and steps >= max_steps:
+                break
+            if self.relative_time >= self.relative_yieldtime:
+                self.distribute_to_vertices_and_edges()
+                self.update_boundary()
+                yield self.get_time()
+                self.relative_yieldtime += yieldstep
+                self.recorded_min_timestep = self.evolve_max_timestep
+                self.recorded_max_timestep = self.evolve_min_timestep
+                self.number_of_steps = 0
+
+    def get_water_volume(self):
+        return float(np.sum(self.areas * self.height_c))
