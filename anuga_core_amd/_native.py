@@ -1,0 +1,111 @@
+"""ctypes binding of the device library libanuga_hip.so (include/anuga_hip.h).
+
+The library is built in-tree by anuga_core_amd.build (or __graft_entry__.build())
+and loaded from this package directory. There is no fallback: if the library is
+missing or cannot be loaded, `lib()` raises, and so does every Domain method that
+needs the device.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libanuga_hip.so")
+
+ANUGA_OK = 0
+ANUGA_E_ARG = -1
+ANUGA_E_HIP = -2
+ANUGA_E_SEMI_IMPLICIT = -3
+ANUGA_E_NO_NEIGHBOUR = -4
+ANUGA_E_STATE = -5
+
+STAGE, XMOMENTUM, YMOMENTUM, ELEVATION, FRICTION, HEIGHT = range(6)
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int64)
+
+
+class AnugaMesh(C.Structure):
+    _fields_ = [("number_of_elements", C.c_int64), ("boundary_length", C.c_int64),
+                ("neighbours", _ip), ("neighbour_edges", _ip),
+                ("surrogate_neighbours", _ip), ("number_of_boundaries", _ip),
+                ("normals", _dp), ("edgelengths", _dp), ("radii", _dp), ("areas", _dp),
+                ("centroid_coordinates", _dp), ("edge_coordinates", _dp)]
+
+
+class AnugaParams(C.Structure):
+    _fields_ = [("g", C.c_double), ("epsilon", C.c_double),
+                ("minimum_allowed_height", C.c_double),
+                ("beta_w", C.c_double), ("beta_w_dry", C.c_double),
+                ("beta_uh", C.c_double), ("beta_uh_dry", C.c_double),
+                ("beta_vh", C.c_double), ("beta_vh_dry", C.c_double),
+                ("low_froude", C.c_int64), ("extrapolate_velocity_second_order", C.c_int64),
+                ("timestep_fluxcalls", C.c_int64), ("precision", C.c_int64),
+                ("reorder", C.c_int64)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/anuga_hip.h
+SIGNATURES = {
+    "anuga_create": (C.c_int, [C.POINTER(AnugaMesh), C.POINTER(AnugaParams),
+                               C.POINTER(C.c_void_p)]),
+    "anuga_destroy": (None, [C.c_void_p]),
+    "anuga_last_error": (C.c_char_p, [C.c_void_p]),
+    "anuga_set_params": (C.c_int, [C.c_void_p, C.POINTER(AnugaParams)]),
+    "anuga_set_centroid_values": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "anuga_get_centroid_values": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "anuga_get_edge_values": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "anuga_set_boundary": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _dp]),
+    "anuga_distribute_to_vertices_and_edges": (C.c_int, [C.c_void_p, _dp]),
+    "anuga_compute_fluxes": (C.c_int, [C.c_void_p, _dp]),
+    "anuga_update_conserved_quantities": (C.c_int, [C.c_void_p, C.c_double, _ip]),
+    "anuga_rk2_second_substep": (C.c_int, [C.c_void_p, C.c_double, _ip]),
+    "anuga_get_boundary_flux_sums": (C.c_int, [C.c_void_p, _dp]),
+    "anuga_get_boundary_flux_integral": (C.c_int, [C.c_void_p, _dp]),
+    "anuga_get_negative_cells": (C.c_int, [C.c_void_p, _ip]),
+    "anuga_get_water_volume": (C.c_int, [C.c_void_p, _dp]),
+    "anuga_synchronize": (C.c_int, [C.c_void_p]),
+    "anuga_kernel_timing": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _ip]),
+    "anuga_reset_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class DeviceError(RuntimeError):
+    """A failure reported by the device library."""
+
+
+def lib():
+    """Load libanuga_hip.so (once). Raises if it is missing or does not load."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    "%s is not built: run `python -m anuga_core_amd.build` (hipcc, gfx950)"
+                    % LIB_PATH)
+            L = C.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = L
+        return _lib
+
+
+def check(handle, rc, what):
+    """Turn a library status into the reference's exception types."""
+    if rc == ANUGA_OK:
+        return
+    msg = lib().anuga_last_error(handle)
+    msg = msg.decode() if msg else ""
+    if rc == ANUGA_E_SEMI_IMPLICIT:
+        raise AssertionError(msg or "division by zero in semi implicit update")
+    if rc == ANUGA_E_NO_NEIGHBOUR:
+        raise Exception(msg or "Internal neighbour not found")
+    raise DeviceError("%s failed (%d): %s" % (what, rc, msg))

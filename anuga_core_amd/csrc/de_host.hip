@@ -1,0 +1,821 @@
+// Host runtime and C ABI of the DE step (see include/anuga_hip.h).
+//
+// The handle owns all device buffers. Triangles are renumbered once at
+// creation (Reverse-Cuthill-McKee on the triangle adjacency) so that neighbour
+// gathers stay within a narrow index window; the permutation is applied on
+// upload and undone on download, so callers only ever see the reference's
+// order. Per-edge and per-triangle arrays are stored structure-of-arrays
+// ([edge][triangle]) for coalesced 64-lane loads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "anuga_hip.h"
+#include "de_kernels.hip"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct KernelTimer {
+    std::vector<hipEvent_t> start, stop;
+    int pending = 0;
+    double total_ms = 0.0;
+    int64_t launches = 0;
+};
+
+const char *kTimerNames[] = {"extrapolate", "flux0", "update", "flux1"};
+enum { T_EXTRAP = 0, T_FLUX0, T_UPDATE, T_FLUX1, T_COUNT };
+const int kTimerRing = 64;
+
+}  // namespace
+
+struct anuga_domain {
+    int N = 0, M = 0, nblocks = 0;
+    anuga_params P{};
+    hipStream_t stream = nullptr;
+    std::vector<int> perm;   // perm[new] = old
+    std::vector<int> iperm;  // iperm[old] = new
+
+    // static mesh, [3][N] or [N], new order
+    int *nbr = nullptr;
+    double *nx = nullptr, *ny = nullptr, *len = nullptr, *dxv = nullptr, *dyv = nullptr;
+    double *cx = nullptr, *cy = nullptr, *area = nullptr, *radius = nullptr;
+    // quantities
+    double *z = nullptr, *eta = nullptr;
+    double *A[3] = {nullptr, nullptr, nullptr};  // step state / rk2 backup
+    double *B[3] = {nullptr, nullptr, nullptr};  // rk2 substep state
+    double *E[4] = {nullptr, nullptr, nullptr, nullptr};  // we, he, uhe, vhe
+    unsigned char *flags = nullptr;
+    double *ex[3] = {nullptr, nullptr, nullptr};
+    signed char *bkind = nullptr;
+    double *bval = nullptr;
+    double *part = nullptr;   // [3][nblocks]: dt, bflux, mass
+    double *dscal = nullptr;  // [8] device scalars
+    int *derr = nullptr;
+    unsigned long long *dneg = nullptr;
+    double *hpin = nullptr;   // pinned host mirror of dscal + counters
+
+    int substate = 0;         // 0: at a step boundary (state in A); 1: rk2 substep state in B
+    bool boundary_set = false;
+    unsigned long long neg_seen = 0;
+    std::string err;
+
+    bool timing = false;
+    KernelTimer timers[T_COUNT];
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static int fail(anuga_domain *d, int code, const std::string &msg)
+{
+    if (d) d->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIPCHK(d, call)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail((d), ANUGA_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+static hipError_t dalloc(T **p, size_t n)
+{
+    return hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+// Reverse Cuthill-McKee over the triangle adjacency (interior edges). Each
+// connected component starts from a pseudo-peripheral triangle (George-Liu);
+// neighbours are queued by increasing degree, ties by index.
+static std::vector<int> rcm_order(int N, const int64_t *neighbours)
+{
+    std::vector<int> deg(N, 0);
+    for (int k = 0; k < N; k++)
+        for (int i = 0; i < 3; i++)
+            if (neighbours[3 * (int64_t)k + i] >= 0) deg[k]++;
+    auto nb = [&](int k, int i) { return (int)neighbours[3 * (int64_t)k + i]; };
+
+    std::vector<int> level(N, -1);
+    std::vector<char> placed(N, 0);
+    std::vector<int> order;
+    order.reserve(N);
+    std::vector<int> queue;
+    queue.reserve(N);
+
+    auto bfs_last_level = [&](int root, int &ecc) {
+        // BFS restricted to the component; returns the min-degree node of the last level
+        queue.clear();
+        queue.push_back(root);
+        level[root] = 0;
+        size_t head = 0;
+        int last = root;
+        ecc = 0;
+        while (head < queue.size()) {
+            int u = queue[head++];
+            for (int i = 0; i < 3; i++) {
+                int v = nb(u, i);
+                if (v >= 0 && level[v] < 0) {
+                    level[v] = level[u] + 1;
+                    queue.push_back(v);
+                }
+            }
+        }
+        ecc = level[queue.back()];
+        int best = queue.back();
+        for (size_t q = queue.size(); q-- > 0;) {
+            int u = queue[q];
+            if (level[u] != ecc) break;
+            if (deg[u] < deg[best] || (deg[u] == deg[best] && u < best)) best = u;
+        }
+        for (int u : queue) level[u] = -1;
+        last = best;
+        return last;
+    };
+
+    for (int seed = 0; seed < N; seed++) {
+        if (placed[seed]) continue;
+        int root = seed, ecc = 0;
+        int cand = bfs_last_level(root, ecc);
+        for (int it = 0; it < 8; it++) {
+            int ecc2 = 0;
+            int c2 = bfs_last_level(cand, ecc2);
+            if (ecc2 <= ecc) break;
+            root = cand;
+            cand = c2;
+            ecc = ecc2;
+        }
+        root = cand;
+        // Cuthill-McKee from root
+        size_t head = order.size();
+        order.push_back(root);
+        placed[root] = 1;
+        while (head < order.size()) {
+            int u = order[head++];
+            int cs[3], nc = 0;
+            for (int i = 0; i < 3; i++) {
+                int v = nb(u, i);
+                if (v >= 0 && !placed[v]) {
+                    placed[v] = 1;
+                    cs[nc++] = v;
+                }
+            }
+            std::sort(cs, cs + nc, [&](int a, int b) {
+                return deg[a] < deg[b] || (deg[a] == deg[b] && a < b);
+            });
+            for (int q = 0; q < nc; q++) order.push_back(cs[q]);
+        }
+    }
+    std::reverse(order.begin(), order.end());
+    return order;
+}
+
+static DeParams dev_params(const anuga_params &p)
+{
+    DeParams q;
+    q.g = p.g;
+    q.eps = p.epsilon;
+    q.mah = p.minimum_allowed_height;
+    q.bw = p.beta_w;
+    q.bwd = p.beta_w_dry;
+    q.buh = p.beta_uh;
+    q.buhd = p.beta_uh_dry;
+    q.bvh = p.beta_vh;
+    q.bvhd = p.beta_vh_dry;
+    return q;
+}
+
+static int check_params(anuga_domain *d, const anuga_params *p)
+{
+    if (!p) return fail(d, ANUGA_E_ARG, "params is NULL");
+    if (p->low_froude != 0)
+        return fail(d, ANUGA_E_ARG, "low_froude != 0 is not supported on the device path");
+    if (p->extrapolate_velocity_second_order != 1)
+        return fail(d, ANUGA_E_ARG, "extrapolate_velocity_second_order must be 1 (DE algorithms)");
+    if (p->timestep_fluxcalls != 1 && p->timestep_fluxcalls != 2)
+        return fail(d, ANUGA_E_ARG, "timestep_fluxcalls must be 1 (euler) or 2 (rk2)");
+    if (p->precision != ANUGA_PRECISION_FP64 && p->precision != ANUGA_PRECISION_MIXED)
+        return fail(d, ANUGA_E_ARG, "unknown precision");
+    return ANUGA_OK;
+}
+
+static void timer_begin(anuga_domain *d, int t)
+{
+    if (!d->timing) return;
+    KernelTimer &T = d->timers[t];
+    if (T.start.empty()) {
+        T.start.resize(kTimerRing);
+        T.stop.resize(kTimerRing);
+        for (int i = 0; i < kTimerRing; i++) {
+            hipEventCreate(&T.start[i]);
+            hipEventCreate(&T.stop[i]);
+        }
+    }
+    if (T.pending == kTimerRing) {
+        hipEventSynchronize(T.stop[T.pending - 1]);
+        for (int i = 0; i < T.pending; i++) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, T.start[i], T.stop[i]);
+            T.total_ms += ms;
+        }
+        T.launches += T.pending;
+        T.pending = 0;
+    }
+    hipEventRecord(T.start[T.pending], d->stream);
+}
+
+static void timer_end(anuga_domain *d, int t)
+{
+    if (!d->timing) return;
+    KernelTimer &T = d->timers[t];
+    hipEventRecord(T.stop[T.pending], d->stream);
+    T.pending++;
+}
+
+static void timer_flush(anuga_domain *d)
+{
+    for (int t = 0; t < T_COUNT; t++) {
+        KernelTimer &T = d->timers[t];
+        if (!T.pending) continue;
+        hipEventSynchronize(T.stop[T.pending - 1]);
+        for (int i = 0; i < T.pending; i++) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, T.start[i], T.stop[i]);
+            T.total_ms += ms;
+        }
+        T.launches += T.pending;
+        T.pending = 0;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// kernel launches
+// ---------------------------------------------------------------------------
+static DeMesh dev_mesh(const anuga_domain *d)
+{
+    DeMesh m;
+    m.N = d->N;
+    m.nbr = d->nbr;
+    m.nx = d->nx;
+    m.ny = d->ny;
+    m.len = d->len;
+    m.dxv = d->dxv;
+    m.dyv = d->dyv;
+    m.cx = d->cx;
+    m.cy = d->cy;
+    m.area = d->area;
+    m.radius = d->radius;
+    return m;
+}
+
+static int launch_extrapolate(anuga_domain *d, double *const *S)
+{
+    timer_begin(d, T_EXTRAP);
+    hipLaunchKernelGGL(de_extrapolate, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, dev_mesh(d),
+                       dev_params(d->P), DeState{S[0], S[1], S[2]}, d->z,
+                       DeEdges{d->E[0], d->E[1], d->E[2], d->E[3]}, d->flags,
+                       d->part + 2 * (size_t)d->nblocks, d->derr);
+    timer_end(d, T_EXTRAP);
+    return ANUGA_OK;
+}
+
+static void launch_materialize(anuga_domain *d, double *const *S)
+{
+    hipLaunchKernelGGL(de_materialize, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N, S[0],
+                       S[1], S[2], d->z, d->flags);
+}
+
+// substep 0: explicit updates; substep 1: advance S with dt into the backup `bk`
+static void launch_flux(anuga_domain *d, int substep, double *const *S, double *const *bk, double dt)
+{
+    DeBoundary bc{d->bkind, d->bval, d->M};
+    DeEdges e{d->E[0], d->E[1], d->E[2], d->E[3]};
+    if (substep == 0) {
+        timer_begin(d, T_FLUX0);
+        hipLaunchKernelGGL(de_flux<0>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, dev_mesh(d),
+                           dev_params(d->P), DeState{S[0], S[1], S[2]}, d->z, d->eta, e, bc,
+                           d->flags, d->ex[0], d->ex[1], d->ex[2], nullptr, nullptr, nullptr, 0.0,
+                           d->part, d->part + d->nblocks, d->derr, d->dneg);
+        timer_end(d, T_FLUX0);
+    } else {
+        timer_begin(d, T_FLUX1);
+        hipLaunchKernelGGL(de_flux<1>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, dev_mesh(d),
+                           dev_params(d->P), DeState{S[0], S[1], S[2]}, d->z, d->eta, e, bc,
+                           d->flags, nullptr, nullptr, nullptr, bk[0], bk[1], bk[2], dt, d->part,
+                           d->part + d->nblocks, d->derr, d->dneg);
+        timer_end(d, T_FLUX1);
+    }
+}
+
+static void launch_update(anuga_domain *d, double *const *S, double *const *out, double dt)
+{
+    timer_begin(d, T_UPDATE);
+    hipLaunchKernelGGL(de_update, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N,
+                       dev_params(d->P), DeState{S[0], S[1], S[2]}, d->z, d->eta, d->flags,
+                       d->ex[0], d->ex[1], d->ex[2], dt, out[0], out[1], out[2], d->derr, d->dneg);
+    timer_end(d, T_UPDATE);
+}
+
+// out = dscal (slots 0..2) or dscal + 3 (slots 3..5)
+static void launch_reduce(anuga_domain *d, const double *pdt, const double *pbf, const double *pmass,
+                          int slot = 0)
+{
+    hipLaunchKernelGGL(de_reduce, dim3(1), dim3(1024), 0, d->stream, d->nblocks, pdt, pbf, pmass,
+                       d->dscal + slot);
+}
+
+static void launch_integrate(anuga_domain *d, double dt, int rk2)
+{
+    hipLaunchKernelGGL(de_integrate, dim3(1), dim3(64), 0, d->stream, d->dscal, dt, rk2);
+}
+
+static void launch_volume(anuga_domain *d, double *const *S)
+{
+    hipLaunchKernelGGL(de_volume, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N, S[0], d->z,
+                       d->area, d->part + 2 * (size_t)d->nblocks);
+}
+
+// read the device scalars and counters; raise a pending device error
+static int sync_scalars(anuga_domain *d)
+{
+    HIPCHK(d, hipMemcpyAsync(d->hpin, d->dscal, 8 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(d->hpin + 8, d->derr, sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(d->hpin + 9, d->dneg, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             d->stream));
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    int e = 0;
+    std::memcpy(&e, d->hpin + 8, sizeof(int));
+    if (e & 2) {
+        hipMemsetAsync(d->derr, 0, sizeof(int), d->stream);
+        return fail(d, ANUGA_E_SEMI_IMPLICIT, "update: division by zero in semi implicit update");
+    }
+    if (e & 1) {
+        hipMemsetAsync(d->derr, 0, sizeof(int), d->stream);
+        return fail(d, ANUGA_E_NO_NEIGHBOUR, "Internal neighbour not found");
+    }
+    return ANUGA_OK;
+}
+
+static int launch_check(anuga_domain *d)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(d, ANUGA_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return ANUGA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *anuga_last_error(const anuga_domain *d)
+{
+    return d ? d->err.c_str() : g_create_error.c_str();
+}
+
+int anuga_create(const anuga_mesh *mesh, const anuga_params *params, anuga_domain **out)
+{
+    if (!out) return fail(nullptr, ANUGA_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (!mesh) return fail(nullptr, ANUGA_E_ARG, "mesh is NULL");
+    int rc = check_params(nullptr, params);
+    if (rc) return rc;
+    if (params->precision != ANUGA_PRECISION_FP64)
+        return fail(nullptr, ANUGA_E_ARG, "mixed precision is not built into this library");
+    const int64_t N64 = mesh->number_of_elements, M64 = mesh->boundary_length;
+    if (N64 <= 0 || N64 >= (int64_t(1) << 29))
+        return fail(nullptr, ANUGA_E_ARG, "number_of_elements must be in [1, 2^29)");
+    if (M64 < 0 || M64 >= (int64_t(1) << 30)) return fail(nullptr, ANUGA_E_ARG, "bad boundary_length");
+    const int N = (int)N64, M = (int)M64;
+    const int64_t *nbr = mesh->neighbours, *nbe = mesh->neighbour_edges;
+    const int64_t *sur = mesh->surrogate_neighbours, *nob = mesh->number_of_boundaries;
+    if (!nbr || !nbe || !sur || !nob || !mesh->normals || !mesh->edgelengths || !mesh->radii ||
+        !mesh->areas || !mesh->centroid_coordinates || !mesh->edge_coordinates)
+        return fail(nullptr, ANUGA_E_ARG, "a mesh array is NULL");
+
+    // validate connectivity: the device derives surrogate neighbours and the
+    // boundary count from `neighbours`, which requires no internal boundaries
+    for (int k = 0; k < N; k++) {
+        int nb = 0;
+        for (int i = 0; i < 3; i++) {
+            const int64_t ki = 3 * (int64_t)k + i, n = nbr[ki];
+            if (n >= 0) {
+                if (n >= N) return fail(nullptr, ANUGA_E_ARG, "neighbour index out of range");
+                const int64_t m = nbe[ki];
+                if (m < 0 || m > 2 || nbr[3 * n + m] != k)
+                    return fail(nullptr, ANUGA_E_ARG, "inconsistent neighbour_edges");
+                if (sur[ki] != n)
+                    return fail(nullptr, ANUGA_E_ARG, "internal boundary edges are not supported");
+            } else {
+                if (-n - 1 >= M) return fail(nullptr, ANUGA_E_ARG, "boundary index out of range");
+                if (sur[ki] != k)
+                    return fail(nullptr, ANUGA_E_ARG, "internal boundary edges are not supported");
+                nb++;
+            }
+        }
+        if (nob[k] != nb) return fail(nullptr, ANUGA_E_ARG, "number_of_boundaries inconsistent");
+    }
+
+    anuga_domain *d = new anuga_domain();
+    d->N = N;
+    d->M = M;
+    d->P = *params;
+    d->nblocks = (N + DE_BLOCK - 1) / DE_BLOCK;
+
+    d->perm.resize(N);
+    d->iperm.resize(N);
+    if (params->reorder) {
+        d->perm = rcm_order(N, nbr);
+    } else {
+        for (int k = 0; k < N; k++) d->perm[k] = k;
+    }
+    for (int p = 0; p < N; p++) d->iperm[d->perm[p]] = p;
+
+#define DCHK(call)                                                                       \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            std::string m_ = std::string(#call) + ": " + hipGetErrorString(e_);          \
+            anuga_destroy(d);                                                            \
+            return fail(nullptr, ANUGA_E_HIP, m_);                                       \
+        }                                                                                \
+    } while (0)
+
+    DCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    const size_t n3 = 3 * (size_t)N;
+    DCHK(dalloc(&d->nbr, n3));
+    DCHK(dalloc(&d->nx, n3));
+    DCHK(dalloc(&d->ny, n3));
+    DCHK(dalloc(&d->len, n3));
+    DCHK(dalloc(&d->dxv, n3));
+    DCHK(dalloc(&d->dyv, n3));
+    DCHK(dalloc(&d->cx, N));
+    DCHK(dalloc(&d->cy, N));
+    DCHK(dalloc(&d->area, N));
+    DCHK(dalloc(&d->radius, N));
+    DCHK(dalloc(&d->z, N));
+    DCHK(dalloc(&d->eta, N));
+    for (int q = 0; q < 3; q++) {
+        DCHK(dalloc(&d->A[q], N));
+        DCHK(dalloc(&d->B[q], N));
+        DCHK(dalloc(&d->ex[q], N));
+    }
+    for (int q = 0; q < 4; q++) DCHK(dalloc(&d->E[q], n3));
+    DCHK(dalloc(&d->flags, N));
+    DCHK(dalloc(&d->bkind, M));
+    DCHK(dalloc(&d->bval, 3 * (size_t)M));
+    DCHK(dalloc(&d->part, 3 * (size_t)d->nblocks));
+    DCHK(dalloc(&d->dscal, 8));
+    DCHK(dalloc(&d->derr, 1));
+    DCHK(dalloc(&d->dneg, 1));
+    DCHK(hipHostMalloc((void **)&d->hpin, 16 * sizeof(double), hipHostMallocDefault));
+
+    {
+        std::vector<int> hn(n3);
+        std::vector<double> a(n3), b(n3), c(n3), dx(n3), dy(n3);
+        const double *nrm = mesh->normals, *el = mesh->edgelengths;
+        const double *cen = mesh->centroid_coordinates, *em = mesh->edge_coordinates;
+        for (int p = 0; p < N; p++) {
+            const int k = d->perm[p];
+            for (int i = 0; i < 3; i++) {
+                const int64_t ki = 3 * (int64_t)k + i;
+                const int64_t n = nbr[ki];
+                hn[(size_t)i * N + p] = n >= 0 ? ((d->iperm[n] << 2) | (int)nbe[ki]) : (int)n;
+                a[(size_t)i * N + p] = nrm[2 * ki];
+                b[(size_t)i * N + p] = nrm[2 * ki + 1];
+                c[(size_t)i * N + p] = el[ki];
+                // xv - x and yv - y exactly as sw_domain_orig.c:2130-2135 forms them
+                dx[(size_t)i * N + p] = em[6 * (int64_t)k + 2 * i] - cen[2 * (int64_t)k];
+                dy[(size_t)i * N + p] = em[6 * (int64_t)k + 2 * i + 1] - cen[2 * (int64_t)k + 1];
+            }
+        }
+        DCHK(hipMemcpy(d->nbr, hn.data(), n3 * sizeof(int), hipMemcpyHostToDevice));
+        DCHK(hipMemcpy(d->nx, a.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+        DCHK(hipMemcpy(d->ny, b.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+        DCHK(hipMemcpy(d->len, c.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+        DCHK(hipMemcpy(d->dxv, dx.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+        DCHK(hipMemcpy(d->dyv, dy.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+        std::vector<double> t(N);
+        for (int p = 0; p < N; p++) t[p] = cen[2 * (int64_t)d->perm[p]];
+        DCHK(hipMemcpy(d->cx, t.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        for (int p = 0; p < N; p++) t[p] = cen[2 * (int64_t)d->perm[p] + 1];
+        DCHK(hipMemcpy(d->cy, t.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        for (int p = 0; p < N; p++) t[p] = mesh->areas[d->perm[p]];
+        DCHK(hipMemcpy(d->area, t.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        for (int p = 0; p < N; p++) t[p] = mesh->radii[d->perm[p]];
+        DCHK(hipMemcpy(d->radius, t.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    }
+    DCHK(hipMemset(d->z, 0, N * sizeof(double)));
+    DCHK(hipMemset(d->eta, 0, N * sizeof(double)));
+    for (int q = 0; q < 3; q++) {
+        DCHK(hipMemset(d->A[q], 0, N * sizeof(double)));
+        DCHK(hipMemset(d->B[q], 0, N * sizeof(double)));
+        DCHK(hipMemset(d->ex[q], 0, N * sizeof(double)));
+    }
+    for (int q = 0; q < 4; q++) DCHK(hipMemset(d->E[q], 0, n3 * sizeof(double)));
+    DCHK(hipMemset(d->flags, DE_F_KEEP, N));
+    DCHK(hipMemset(d->bkind, 0, std::max(M, 1)));
+    DCHK(hipMemset(d->bval, 0, 3 * (size_t)std::max(M, 1) * sizeof(double)));
+    DCHK(hipMemset(d->dscal, 0, 8 * sizeof(double)));
+    DCHK(hipMemset(d->derr, 0, sizeof(int)));
+    DCHK(hipMemset(d->dneg, 0, sizeof(unsigned long long)));
+    DCHK(hipDeviceSynchronize());
+#undef DCHK
+    *out = d;
+    return ANUGA_OK;
+}
+
+void anuga_destroy(anuga_domain *d)
+{
+    if (!d) return;
+    if (d->stream) hipStreamSynchronize(d->stream);
+    void *ptrs[] = {d->nbr, d->nx, d->ny, d->len, d->dxv, d->dyv, d->cx, d->cy, d->area,
+                    d->radius, d->z, d->eta, d->A[0], d->A[1], d->A[2], d->B[0], d->B[1],
+                    d->B[2], d->E[0], d->E[1], d->E[2], d->E[3], d->flags, d->ex[0],
+                    d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->dscal, d->derr,
+                    d->dneg};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (d->hpin) hipHostFree(d->hpin);
+    for (int t = 0; t < T_COUNT; t++) {
+        for (auto e : d->timers[t].start) hipEventDestroy(e);
+        for (auto e : d->timers[t].stop) hipEventDestroy(e);
+    }
+    if (d->stream) hipStreamDestroy(d->stream);
+    delete d;
+}
+
+int anuga_set_params(anuga_domain *d, const anuga_params *params)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = check_params(d, params);
+    if (rc) return rc;
+    if (params->precision != d->P.precision || params->reorder != d->P.reorder)
+        return fail(d, ANUGA_E_ARG, "precision and reorder are fixed at creation");
+    if (params->timestep_fluxcalls != d->P.timestep_fluxcalls && d->substate != 0)
+        return fail(d, ANUGA_E_STATE, "cannot change the timestepping method inside a step");
+    d->P = *params;
+    return ANUGA_OK;
+}
+
+static double *quantity_ptr(anuga_domain *d, int q)
+{
+    double *const *cur = d->substate ? d->B : d->A;
+    switch (q) {
+    case ANUGA_STAGE: return cur[0];
+    case ANUGA_XMOMENTUM: return cur[1];
+    case ANUGA_YMOMENTUM: return cur[2];
+    case ANUGA_ELEVATION: return d->z;
+    case ANUGA_FRICTION: return d->eta;
+    default: return nullptr;
+    }
+}
+
+int anuga_set_centroid_values(anuga_domain *d, int quantity, const double *values)
+{
+    if (!d || !values) return ANUGA_E_ARG;
+    double *dst = quantity_ptr(d, quantity);
+    if (!dst) return fail(d, ANUGA_E_ARG, "quantity cannot be set");
+    std::vector<double> t(d->N);
+    for (int p = 0; p < d->N; p++) t[p] = values[d->perm[p]];
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    HIPCHK(d, hipMemcpy(dst, t.data(), d->N * sizeof(double), hipMemcpyHostToDevice));
+    return ANUGA_OK;
+}
+
+int anuga_get_centroid_values(anuga_domain *d, int quantity, double *values)
+{
+    if (!d || !values) return ANUGA_E_ARG;
+    std::vector<double> t(d->N);
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    if (quantity == ANUGA_HEIGHT) {
+        std::vector<double> w(d->N), z(d->N);
+        HIPCHK(d, hipMemcpy(w.data(), quantity_ptr(d, ANUGA_STAGE), d->N * sizeof(double),
+                            hipMemcpyDeviceToHost));
+        HIPCHK(d, hipMemcpy(z.data(), d->z, d->N * sizeof(double), hipMemcpyDeviceToHost));
+        for (int p = 0; p < d->N; p++) values[d->perm[p]] = std::fmax(w[p] - z[p], 0.0);
+        return ANUGA_OK;
+    }
+    double *src = quantity_ptr(d, quantity);
+    if (!src) return fail(d, ANUGA_E_ARG, "unknown quantity");
+    HIPCHK(d, hipMemcpy(t.data(), src, d->N * sizeof(double), hipMemcpyDeviceToHost));
+    for (int p = 0; p < d->N; p++) values[d->perm[p]] = t[p];
+    return ANUGA_OK;
+}
+
+int anuga_get_edge_values(anuga_domain *d, int quantity, double *values)
+{
+    if (!d || !values) return ANUGA_E_ARG;
+    const size_t n3 = 3 * (size_t)d->N;
+    std::vector<double> a(n3), b;
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    int src;
+    switch (quantity) {
+    case ANUGA_STAGE: src = 0; break;
+    case ANUGA_HEIGHT: src = 1; break;
+    case ANUGA_XMOMENTUM: src = 2; break;
+    case ANUGA_YMOMENTUM: src = 3; break;
+    case ANUGA_ELEVATION: src = 0; break;
+    default: return fail(d, ANUGA_E_ARG, "unknown edge quantity");
+    }
+    HIPCHK(d, hipMemcpy(a.data(), d->E[src], n3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (quantity == ANUGA_ELEVATION) {
+        b.resize(n3);
+        HIPCHK(d, hipMemcpy(b.data(), d->E[1], n3 * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n3; i++) a[i] = a[i] - b[i];  // bed_edge = stage_edge - height_edge
+    }
+    const int N = d->N;
+    for (int p = 0; p < N; p++) {
+        const int64_t k = d->perm[p];
+        for (int i = 0; i < 3; i++) values[3 * k + i] = a[(size_t)i * N + p];
+    }
+    return ANUGA_OK;
+}
+
+int anuga_set_boundary(anuga_domain *d, const int32_t *kind, const double *dirichlet_values)
+{
+    if (!d) return ANUGA_E_ARG;
+    const int M = d->M;
+    if (M > 0 && !kind) return fail(d, ANUGA_E_ARG, "kind is NULL");
+    std::vector<signed char> k(std::max(M, 1), 0);
+    std::vector<double> v(3 * (size_t)std::max(M, 1), 0.0);
+    for (int j = 0; j < M; j++) {
+        if (kind[j] < 0 || kind[j] > 3) return fail(d, ANUGA_E_ARG, "unknown boundary kind");
+        k[j] = (signed char)kind[j];
+        if (kind[j] == ANUGA_BC_DIRICHLET) {
+            if (!dirichlet_values) return fail(d, ANUGA_E_ARG, "Dirichlet values are NULL");
+            for (int q = 0; q < 3; q++) v[(size_t)q * M + j] = dirichlet_values[3 * (size_t)j + q];
+        }
+    }
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    HIPCHK(d, hipMemcpy(d->bkind, k.data(), k.size(), hipMemcpyHostToDevice));
+    HIPCHK(d, hipMemcpy(d->bval, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    d->boundary_set = true;
+    return ANUGA_OK;
+}
+
+int anuga_distribute_to_vertices_and_edges(anuga_domain *d, double *mass_error)
+{
+    if (!d) return ANUGA_E_ARG;
+    double **S = d->substate ? d->B : d->A;
+    int rc = launch_extrapolate(d, S);
+    if (rc) return rc;
+    launch_materialize(d, S);
+    launch_reduce(d, nullptr, nullptr, d->part + 2 * (size_t)d->nblocks);
+    rc = launch_check(d);
+    if (rc) return rc;
+    rc = sync_scalars(d);
+    if (rc) return rc;
+    if (mass_error) *mass_error = d->hpin[2];
+    return ANUGA_OK;
+}
+
+int anuga_compute_fluxes(anuga_domain *d, double *flux_timestep)
+{
+    if (!d) return ANUGA_E_ARG;
+    if (d->substate != 0) return fail(d, ANUGA_E_STATE, "compute_fluxes called inside an rk2 step");
+    if (!d->boundary_set && d->M > 0) return fail(d, ANUGA_E_STATE, "boundary conditions not set");
+    int rc = launch_extrapolate(d, d->A);
+    if (rc) return rc;
+    launch_flux(d, 0, d->A, nullptr, 0.0);
+    launch_reduce(d, d->part, d->part + d->nblocks, d->part + 2 * (size_t)d->nblocks);
+    rc = launch_check(d);
+    if (rc) return rc;
+    rc = sync_scalars(d);
+    if (rc) return rc;
+    if (d->timing) timer_flush(d);
+    if (flux_timestep) *flux_timestep = d->hpin[0];
+    return ANUGA_OK;
+}
+
+int anuga_update_conserved_quantities(anuga_domain *d, double dt, int64_t *num_negative)
+{
+    if (!d) return ANUGA_E_ARG;
+    if (d->substate != 0) return fail(d, ANUGA_E_STATE, "update called twice in a step");
+    double **out = d->P.timestep_fluxcalls == 2 ? d->B : d->A;
+    launch_update(d, d->A, out, dt);
+    if (d->P.timestep_fluxcalls == 1) launch_integrate(d, dt, 0);
+    int rc = launch_check(d);
+    if (rc) return rc;
+    if (d->P.timestep_fluxcalls == 2) d->substate = 1;
+    if (num_negative) {
+        rc = sync_scalars(d);
+        if (rc) return rc;
+        unsigned long long n;
+        std::memcpy(&n, d->hpin + 9, sizeof(n));
+        *num_negative = (int64_t)(n - d->neg_seen);
+        d->neg_seen = n;
+    }
+    return ANUGA_OK;
+}
+
+int anuga_rk2_second_substep(anuga_domain *d, double dt, int64_t *num_negative)
+{
+    if (!d) return ANUGA_E_ARG;
+    if (d->substate != 1) return fail(d, ANUGA_E_STATE, "second substep without a first");
+    int rc = launch_extrapolate(d, d->B);
+    if (rc) return rc;
+    launch_flux(d, 1, d->B, d->A, dt);
+    launch_reduce(d, nullptr, d->part + d->nblocks, nullptr, 3);
+    launch_integrate(d, dt, 1);
+    rc = launch_check(d);
+    if (rc) return rc;
+    d->substate = 0;
+    if (num_negative) {
+        rc = sync_scalars(d);
+        if (rc) return rc;
+        unsigned long long n;
+        std::memcpy(&n, d->hpin + 9, sizeof(n));
+        *num_negative = (int64_t)(n - d->neg_seen);
+        d->neg_seen = n;
+    }
+    return ANUGA_OK;
+}
+
+int anuga_get_boundary_flux_sums(anuga_domain *d, double *sums)
+{
+    if (!d || !sums) return ANUGA_E_ARG;
+    int rc = sync_scalars(d);
+    if (rc) return rc;
+    sums[0] = d->hpin[1];
+    sums[1] = d->P.timestep_fluxcalls == 2 ? d->hpin[4] : 0.0;
+    return ANUGA_OK;
+}
+
+int anuga_get_boundary_flux_integral(anuga_domain *d, double *integral)
+{
+    if (!d || !integral) return ANUGA_E_ARG;
+    int rc = sync_scalars(d);
+    if (rc) return rc;
+    *integral = d->hpin[5];
+    return ANUGA_OK;
+}
+
+int anuga_get_negative_cells(anuga_domain *d, int64_t *total)
+{
+    if (!d || !total) return ANUGA_E_ARG;
+    unsigned long long n;
+    std::memcpy(&n, d->hpin + 9, sizeof(n));
+    *total = (int64_t)n;
+    return ANUGA_OK;
+}
+
+int anuga_get_water_volume(anuga_domain *d, double *volume)
+{
+    if (!d || !volume) return ANUGA_E_ARG;
+    double **S = d->substate ? d->B : d->A;
+    launch_volume(d, S);
+    launch_reduce(d, nullptr, nullptr, d->part + 2 * (size_t)d->nblocks);
+    int rc = launch_check(d);
+    if (rc) return rc;
+    rc = sync_scalars(d);
+    if (rc) return rc;
+    *volume = d->hpin[2];
+    return ANUGA_OK;
+}
+
+int anuga_synchronize(anuga_domain *d)
+{
+    if (!d) return ANUGA_E_ARG;
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    return ANUGA_OK;
+}
+
+int anuga_reset_kernel_timing(anuga_domain *d, int enable)
+{
+    if (!d) return ANUGA_E_ARG;
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    for (int t = 0; t < T_COUNT; t++) {
+        d->timers[t].pending = 0;
+        d->timers[t].total_ms = 0.0;
+        d->timers[t].launches = 0;
+    }
+    d->timing = enable != 0;
+    return ANUGA_OK;
+}
+
+int anuga_kernel_timing(anuga_domain *d, const char *name, double *avg_ms, int64_t *launches)
+{
+    if (!d || !name) return ANUGA_E_ARG;
+    timer_flush(d);
+    for (int t = 0; t < T_COUNT; t++) {
+        if (std::strcmp(name, kTimerNames[t]) == 0) {
+            const KernelTimer &T = d->timers[t];
+            if (avg_ms) *avg_ms = T.launches ? T.total_ms / (double)T.launches : 0.0;
+            if (launches) *launches = T.launches;
+            return ANUGA_OK;
+        }
+    }
+    return fail(d, ANUGA_E_ARG, std::string("unknown kernel timer ") + name);
+}
+
+}  // extern "C"

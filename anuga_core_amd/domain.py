@@ -1,0 +1,576 @@
+"""`Domain`: the anuga.Domain interface of the DE step, running on one MI355X.
+
+This is the host side of the drop-in boundary. It mirrors the parts of
+anuga.Domain a DE scenario uses (anuga/shallow_water/shallow_water_domain.py,
+anuga/abstract_2d_finite_volumes/generic_domain.py):
+
+  set_flow_algorithm('DE0'|'DE1'), set_quantity, get_quantity,
+  set_boundary({tag: boundary}), evolve(yieldstep, finaltime), get_time,
+  get_water_volume, timestep bookkeeping (recorded_min/max_timestep,
+  number_of_steps), set_fixed_flux_timestep, boundary_flux_integral.
+
+The evolve control flow (yield cadence, update_timestep's CFL / clamping /
+small-step logic, generic_domain.py:1715-2051, :2349-2415) runs here on the
+host exactly as in the reference, one library call per substep; the state
+stays in HBM and only the CFL timestep (8 bytes) crosses per step.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import warnings
+
+import numpy as np
+
+from . import _native
+from .boundaries import Boundary
+from .mesh import Mesh
+
+# anuga/config.py
+G = 9.8
+EPSILON = 1.0e-12
+MAX_TIMESTEP = 1.0e+3
+MIN_TIMESTEP = 1.0e-6
+MAX_SMALLSTEPS = 50
+H0 = 1.0e-3
+
+_QIDS = {"stage": _native.STAGE, "xmomentum": _native.XMOMENTUM,
+         "ymomentum": _native.YMOMENTUM, "elevation": _native.ELEVATION,
+         "friction": _native.FRICTION, "height": _native.HEIGHT}
+
+_dp = C.POINTER(C.c_double)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Quantity:
+    """Read view of one quantity of a device-resident Domain.
+
+    centroid_values / edge_values download from the device; vertex_values are
+    rebuilt from the edge values as the extrapolation does
+    (sw_domain_orig.c:2634-2676, v_i = e_{i+1} + e_{i+2} - e_i).
+    """
+
+    def __init__(self, domain, name):
+        self.domain = domain
+        self.name = name
+
+    @property
+    def centroid_values(self):
+        return self.domain._get_centroids(self.name)
+
+    @property
+    def edge_values(self):
+        return self.domain._get_edges(self.name)
+
+    @property
+    def vertex_values(self):
+        if self.name == "friction":
+            c = self.centroid_values
+            return np.repeat(c[:, None], 3, axis=1)
+        if self.name == "elevation" and not self.domain._distributed:
+            return self.domain._elev_vertex.copy()
+        e = self.edge_values
+        v = np.empty_like(e)
+        v[:, 0] = e[:, 1] + e[:, 2] - e[:, 0]
+        v[:, 1] = e[:, 0] + e[:, 2] - e[:, 1]
+        v[:, 2] = e[:, 0] + e[:, 1] - e[:, 2]
+        return v
+
+    def get_integral(self):
+        return float(np.sum(self.domain.mesh.areas * self.centroid_values))
+
+
+class Domain:
+    """Shallow-water domain whose DE step runs on the GPU.
+
+    Domain(coordinates, vertices, boundary) builds the mesh like anuga.Domain;
+    `mesh=` accepts a prebuilt anuga_core_amd.mesh.Mesh. `precision` is 'fp64'
+    (bit-comparable with the reference) or 'mixed'; `reorder` enables the
+    Reverse-Cuthill-McKee renumbering inside the library.
+    """
+
+    def __init__(self, coordinates=None, vertices=None, boundary=None, mesh=None,
+                 use_inscribed_circle=False, precision="fp64", reorder=True,
+                 starttime=0.0, verbose=False):
+        if mesh is None:
+            mesh = Mesh(coordinates, vertices, boundary, use_inscribed_circle)
+        self.mesh = mesh
+        self.number_of_elements = self.number_of_triangles = N = mesh.number_of_triangles
+        self.boundary_length = mesh.boundary_length
+        self.verbose = verbose
+        self.precision = precision
+        self._reorder = bool(reorder)
+
+        self.g = G
+        self.epsilon = EPSILON
+        self.H0 = H0
+        self.low_froude = 0
+        self.extrapolate_velocity_second_order = 1
+        self.evolve_max_timestep = MAX_TIMESTEP
+        self.evolve_min_timestep = MIN_TIMESTEP
+        self.max_smallsteps = MAX_SMALLSTEPS
+        self.fixed_flux_timestep = None
+        self.centroid_transmissive_bc = False
+        self.boundary_map = None
+        self.starttime = float(starttime)
+        self.relative_time = 0.0
+        self.evolved_called = False
+        self.smallsteps = 0
+        self.timestep = 0.0
+        self.flux_timestep = 0.0
+        self.number_of_steps = 0
+        self.recorded_min_timestep = self.recorded_max_timestep = 0.0
+        self.dt_history = []
+        self.record_dt_history = True
+        self._distributed = False
+        self._last_negative_total = 0
+        self.mass_error = 0.0
+        self._elev_vertex = np.zeros((N, 3))
+
+        self._handle = C.c_void_p()
+        self._create_device()
+        self.set_flow_algorithm("DE0")
+
+    # -- device handle -----------------------------------------------------------
+    def _create_device(self):
+        L = _native.lib()
+        m = self.mesh
+        self._mesh_keep = dict(
+            nb=np.ascontiguousarray(m.neighbours, np.int64),
+            nbe=np.ascontiguousarray(m.neighbour_edges, np.int64),
+            sur=np.ascontiguousarray(m.surrogate_neighbours, np.int64),
+            nob=np.ascontiguousarray(m.number_of_boundaries, np.int64),
+            nrm=np.ascontiguousarray(m.normals), el=np.ascontiguousarray(m.edgelengths),
+            rad=np.ascontiguousarray(m.radii), area=np.ascontiguousarray(m.areas),
+            cen=np.ascontiguousarray(m.centroid_coordinates),
+            em=np.ascontiguousarray(m.edge_midpoint_coordinates))
+        k = self._mesh_keep
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
+        desc = _native.AnugaMesh(self.number_of_elements, self.boundary_length,
+                                 ip(k["nb"]), ip(k["nbe"]), ip(k["sur"]), ip(k["nob"]),
+                                 _dptr(k["nrm"]), _dptr(k["el"]), _dptr(k["rad"]),
+                                 _dptr(k["area"]), _dptr(k["cen"]), _dptr(k["em"]))
+        self._timestep_fluxcalls = 1
+        self._params = self._make_params(fp64_defaults=True)
+        rc = L.anuga_create(C.byref(desc), C.byref(self._params), C.byref(self._handle))
+        if rc != 0:
+            msg = L.anuga_last_error(None)
+            raise _native.DeviceError("anuga_create failed (%d): %s"
+                                      % (rc, msg.decode() if msg else ""))
+        del self._mesh_keep
+
+    def _make_params(self, fp64_defaults=False):
+        prec = {"fp64": _native.PRECISION_FP64, "mixed": _native.PRECISION_MIXED}[self.precision]
+        if fp64_defaults:
+            return _native.AnugaParams(G, EPSILON, 1.0e-12, 0.5, 0.0, 0.5, 0.0, 0.5, 0.0,
+                                       0, 1, 1, prec, int(self._reorder))
+        return _native.AnugaParams(self.g, self.epsilon, self.minimum_allowed_height,
+                                   self.beta_w, self.beta_w_dry, self.beta_uh, self.beta_uh_dry,
+                                   self.beta_vh, self.beta_vh_dry, self.low_froude,
+                                   self.extrapolate_velocity_second_order,
+                                   self._timestep_fluxcalls, prec, int(self._reorder))
+
+    def _sync_params(self):
+        p = self._make_params()
+        self._call("anuga_set_params", C.byref(p))
+
+    def _call(self, name, *args):
+        L = _native.lib()
+        rc = getattr(L, name)(self._handle, *args)
+        _native.check(self._handle, rc, name)
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                _native.lib().anuga_destroy(h)
+            except Exception:
+                pass
+            self._handle = C.c_void_p()
+
+    def __len__(self):
+        return self.number_of_elements
+
+    # -- algorithm presets ---------------------------------------------------------
+    def set_flow_algorithm(self, algorithm="DE0"):
+        """DE0 / DE1 presets (shallow_water_domain.py:583-707)."""
+        algorithm = str(algorithm).replace(".", "_")
+        if algorithm == "DE0":
+            self.CFL = 0.9
+            self.set_timestepping_method("euler")
+            self.minimum_allowed_height = 1.0e-12
+            b = 0.5
+        elif algorithm == "DE1":
+            self.CFL = 1.0
+            self.set_timestepping_method("rk2")
+            self.minimum_allowed_height = 1.0e-5
+            b = 1.0
+        else:
+            raise Exception("Unknown or unsupported flow_algorithm %s; supported: DE0, DE1"
+                            % algorithm)
+        self.flow_algorithm = algorithm
+        self.beta_w = self.beta_uh = self.beta_vh = b
+        self.beta_w_dry = self.beta_uh_dry = self.beta_vh_dry = 0.0
+        self.default_order = 2
+        self.maximum_allowed_speed = 0.0
+
+    def get_flow_algorithm(self):
+        return self.flow_algorithm
+
+    def set_timestepping_method(self, method):
+        methods = {"euler": 1, "rk2": 2, 1: 1, 2: 2}
+        if method not in methods:
+            raise Exception("%s is an incorrect timestepping type" % method)
+        self._timestep_fluxcalls = methods[method]
+        self.timestepping_method = "euler" if self._timestep_fluxcalls == 1 else "rk2"
+
+    def get_timestepping_method(self):
+        return self.timestepping_method
+
+    def set_CFL(self, cfl):
+        self.CFL = float(cfl)
+
+    def get_CFL(self):
+        return self.CFL
+
+    def set_minimum_allowed_height(self, h):
+        self.minimum_allowed_height = float(h)
+
+    def set_evolve_max_timestep(self, t):
+        self.evolve_max_timestep = float(t)
+
+    def set_evolve_min_timestep(self, t):
+        self.evolve_min_timestep = float(t)
+
+    def set_fixed_flux_timestep(self, flux_timestep=None):
+        """generic_domain.py:2332-2347"""
+        if flux_timestep is None:
+            self.fixed_flux_timestep = None
+        elif flux_timestep > 0.0:
+            self.fixed_flux_timestep = float(flux_timestep)
+        else:
+            raise Exception("flux_timestep needs to be greater than 0.0")
+
+    def set_centroid_transmissive_bc(self, flag):
+        self.centroid_transmissive_bc = bool(flag)
+        if self.boundary_map is not None:
+            self.set_boundary(self.boundary_map)
+
+    def set_store(self, flag=True):
+        if flag:
+            warnings.warn("SWW storage is not part of this package; set_store ignored")
+
+    def set_name(self, name):
+        self.simulation_name = name
+
+    def get_name(self):
+        return getattr(self, "simulation_name", "domain")
+
+    # -- quantities -----------------------------------------------------------------
+    def set_quantity(self, name, numeric=None, function=None, location="vertices",
+                     indices=None):
+        """Quantity.set_values for constants, arrays and f(x, y) (quantity.py:709-1175).
+
+        location='vertices' (the reference default) sets vertex values and
+        derives centroids as (v0 + v1 + v2) / 3.0 (quantity.c:665-688);
+        location='centroids' sets centroid values directly.
+        """
+        if name not in ("stage", "xmomentum", "ymomentum", "elevation", "friction"):
+            raise Exception("Unknown quantity %s" % name)
+        if indices is not None:
+            raise Exception("indices are not supported")
+        value = function if function is not None else numeric
+        if value is None:
+            raise Exception("one of numeric or function must be given")
+        N = self.number_of_elements
+        if location == "centroids":
+            if callable(value):
+                cen = self.mesh.centroid_coordinates
+                c = np.asarray(value(cen[:, 0], cen[:, 1]), np.float64)
+                c = np.broadcast_to(c, (N,)).copy()
+            else:
+                c = np.broadcast_to(np.asarray(value, np.float64), (N,)).copy()
+            vtx = np.repeat(c[:, None], 3, axis=1)
+        elif location == "vertices":
+            if callable(value):
+                V = self.mesh.vertex_coordinates
+                v = np.asarray(value(V[:, 0], V[:, 1]), np.float64)
+                if v.ndim == 0:
+                    vtx = np.full((N, 3), float(v))
+                else:
+                    vtx = v.reshape(N, 3)
+            else:
+                a = np.asarray(value, np.float64)
+                if a.ndim == 0:
+                    vtx = np.full((N, 3), float(a))
+                elif a.shape == (N, 3):
+                    vtx = a.copy()
+                else:
+                    raise Exception("vertex values must be a constant or an (N,3) array")
+            c = (vtx[:, 0] + vtx[:, 1] + vtx[:, 2]) / 3.0
+        else:
+            raise Exception("Invalid location: %s" % location)
+        if name == "elevation":
+            self._elev_vertex = np.ascontiguousarray(vtx)
+        c = np.ascontiguousarray(c, np.float64)
+        self._call("anuga_set_centroid_values", _QIDS[name], _dptr(c))
+        self._distributed = False
+
+    def get_quantity(self, name):
+        if name not in _QIDS:
+            raise KeyError(name)
+        return Quantity(self, name)
+
+    @property
+    def quantities(self):
+        return {n: Quantity(self, n) for n in _QIDS}
+
+    def _get_centroids(self, name):
+        out = np.empty(self.number_of_elements)
+        self._call("anuga_get_centroid_values", _QIDS[name], _dptr(out))
+        return out
+
+    def _get_edges(self, name):
+        if name == "friction":
+            raise Exception("friction has no edge values")
+        out = np.empty((self.number_of_elements, 3))
+        self._call("anuga_get_edge_values", _QIDS[name], _dptr(out))
+        return out
+
+    def get_centroid_coordinates(self):
+        return self.mesh.centroid_coordinates
+
+    def get_vertex_coordinates(self):
+        return self.mesh.vertex_coordinates
+
+    # -- boundaries -------------------------------------------------------------------
+    def get_boundary_tags(self):
+        return self.mesh.get_boundary_tags()
+
+    def set_boundary(self, boundary_map):
+        """Bind boundary objects to tags (generic_domain.py set_boundary). Tags
+        mapped to None keep their previous kind (the reference skips them)."""
+        tags = self.mesh.get_boundary_tags()
+        for tag in boundary_map:
+            if tag not in tags:
+                raise Exception("Tag %s provided does not exist in the domain. "
+                                "All boundary tags are %s" % (tag, tags))
+        for tag in tags:
+            if tag not in boundary_map:
+                raise Exception("Tag %s has not been bound to a boundary object" % tag)
+        M = self.boundary_length
+        kind = np.zeros(max(M, 1), np.int32)
+        vals = np.zeros((max(M, 1), 3))
+        for tag, B in boundary_map.items():
+            if B is None:
+                continue
+            if not isinstance(B, Boundary):
+                raise Exception("unsupported boundary object %r" % (B,))
+            ids = np.asarray(self.mesh.tag_boundary_cells[tag], np.int64)
+            code = B.code
+            if code == 1 and self.centroid_transmissive_bc:
+                code = 3
+            kind[ids] = code
+            if code == 2:
+                vals[ids] = B.dirichlet_values[:3]
+        self.boundary_map = dict(boundary_map)
+        self._call("anuga_set_boundary", kind.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(vals))
+
+    # -- step -------------------------------------------------------------------------
+    def distribute_to_vertices_and_edges(self):
+        """protect + extrapolation on the device, written back to the centroids."""
+        self._sync_params()
+        me = C.c_double()
+        self._call("anuga_distribute_to_vertices_and_edges", C.byref(me))
+        self.mass_error += me.value
+        if me.value > 0.0 and self.verbose:
+            print("Cumulative mass protection: {0} m^3".format(me.value))
+        self._distributed = True
+
+    def update_boundary(self):
+        """Boundary states are evaluated inside the flux kernel; nothing to do."""
+
+    def update_timestep(self, yieldstep, finaltime):
+        """generic_domain.py:2349-2415"""
+        if self.fixed_flux_timestep is not None:
+            self.flux_timestep = self.fixed_flux_timestep
+        timestep = min(self.CFL * self.flux_timestep, self.evolve_max_timestep)
+        self.recorded_max_timestep = max(timestep, self.recorded_max_timestep)
+        self.recorded_min_timestep = min(timestep, self.recorded_min_timestep)
+        if timestep < self.evolve_min_timestep:
+            self.smallsteps += 1
+            if self.smallsteps > self.max_smallsteps:
+                self.smallsteps = 0
+                if self._order_ == 1:
+                    raise Exception("WARNING: Too small timestep %.16f reached even after %d "
+                                    "steps of 1 order scheme" % (timestep, self.max_smallsteps))
+                self._order_ = 1
+        else:
+            self.smallsteps = 0
+            if self._order_ == 1 and self.default_order == 2:
+                self._order_ = 2
+        if self.relative_finaltime is not None and \
+                self.relative_time + timestep > self.relative_finaltime:
+            timestep = self.relative_finaltime - self.relative_time
+        if self.relative_time + timestep > self.relative_yieldtime:
+            timestep = self.relative_yieldtime - self.relative_time
+        self.timestep = timestep
+
+    def _compute_fluxes(self):
+        ft = C.c_double()
+        self._call("anuga_compute_fluxes", C.byref(ft))
+        self.flux_timestep = ft.value
+        self._check_negative_cells()
+
+    def _check_negative_cells(self):
+        tot = C.c_int64()
+        self._call("anuga_get_negative_cells", C.byref(tot))
+        if tot.value > self._last_negative_total:
+            self._last_negative_total = tot.value
+            warnings.warn("Negative cells being set to zero depth, possible loss of "
+                          "conservation. \nConsider using "
+                          "domain.report_water_volume_statistics() to check the extent "
+                          "of the problem")
+
+    def evolve_one_euler_step(self, yieldstep, finaltime):
+        """generic_domain.py:1914-1972"""
+        self._compute_fluxes()
+        self.update_timestep(yieldstep, finaltime)
+        self._call("anuga_update_conserved_quantities", C.c_double(self.timestep), None)
+
+    def evolve_one_rk2_step(self, yieldstep, finaltime):
+        """generic_domain.py:1974-2051; backup is the untouched step state."""
+        self._compute_fluxes()
+        self.update_timestep(yieldstep, finaltime)
+        self._call("anuga_update_conserved_quantities", C.c_double(self.timestep), None)
+        self.relative_time = self.relative_time + self.timestep
+        self._call("anuga_rk2_second_substep", C.c_double(self.timestep), None)
+
+    def get_time(self):
+        return self.starttime + self.relative_time
+
+    def get_relative_time(self):
+        return self.relative_time
+
+    def set_time(self, time=0.0):
+        self.relative_time = float(time) - self.starttime
+
+    def evolve(self, yieldstep=None, finaltime=None, duration=None, skip_initial_step=False):
+        """Generator over yield times (shallow_water_domain.py:2300-2407 and
+        generic_domain.py:1715-1912 without SWW storage and checkpointing)."""
+        if self.boundary_map is None:
+            raise AssertionError("Boundary tags must be bound to boundary objects before "
+                                 "evolving system, e.g. using the method set_boundary.\n"
+                                 "This system has the boundary tags %s "
+                                 % self.get_boundary_tags())
+        if not (0 <= self.beta_w <= 2.0):
+            raise AssertionError("Attribute self.beta_w must be in the interval [0, 2]")
+        self._sync_params()
+        self.distribute_to_vertices_and_edges()
+        if self.evolved_called:
+            skip_initial_step = True
+        self.evolved_called = True
+        yieldstep = self.evolve_max_timestep if yieldstep is None else float(yieldstep)
+        self.yieldstep = yieldstep
+        self._order_ = self.default_order
+        if finaltime is not None and duration is not None:
+            raise Exception("Only one of finaltime and duration may be specified")
+        self.relative_finaltime = None
+        if finaltime is not None:
+            self.finaltime = float(finaltime)
+            self.relative_finaltime = self.finaltime - self.starttime
+        if duration is not None:
+            self.finaltime = float(duration) + self.get_time()
+            self.relative_finaltime = float(duration) + self.relative_time
+        if self.relative_finaltime is not None and self.relative_finaltime < self.relative_time:
+            warnings.warn("\n finaltime %g is less than current time %g! finaltime set to "
+                          "current time" % (self.finaltime, self.get_time()))
+            self.relative_finaltime = self.relative_time
+            return
+        self.relative_yieldtime = self.relative_time + yieldstep
+        self.recorded_min_timestep = self.evolve_max_timestep
+        self.recorded_max_timestep = self.evolve_min_timestep
+        self.number_of_steps = 0
+        if not skip_initial_step:
+            self.distribute_to_vertices_and_edges()
+            yield self.get_time()
+        step = self.evolve_one_euler_step if self._timestep_fluxcalls == 1 \
+            else self.evolve_one_rk2_step
+        while True:
+            initial = self.relative_time
+            self._distributed = False
+            step(yieldstep, self.relative_finaltime)
+            self.relative_time = initial + self.timestep
+            if self.record_dt_history:
+                self.dt_history.append(self.timestep)
+            self.number_of_steps += 1
+            if self.relative_finaltime is not None and \
+                    self.relative_time >= self.relative_finaltime - EPSILON:
+                if self.relative_time > self.relative_finaltime:
+                    raise Exception("WARNING (domain.py): time overshot finaltime. ")
+                self.relative_time = self.relative_finaltime
+                self.distribute_to_vertices_and_edges()
+                self._check_negative_cells()
+                yield self.get_time()
+                break
+            if self.relative_time >= self.relative_yieldtime:
+                self.distribute_to_vertices_and_edges()
+                self._check_negative_cells()
+                yield self.get_time()
+                self.relative_yieldtime += yieldstep
+                self.recorded_min_timestep = self.evolve_max_timestep
+                self.recorded_max_timestep = self.evolve_min_timestep
+                self.number_of_steps = 0
+
+    def evolve_steps(self, nsteps, yieldstep=None, finaltime=None):
+        """Run exactly `nsteps` steps with no intermediate yields (a convenience
+        for fixed-step-count runs; dt still follows update_timestep)."""
+        self._sync_params()
+        if not self.evolved_called:
+            self.distribute_to_vertices_and_edges()
+            self.evolved_called = True
+        self._order_ = self.default_order
+        self.relative_finaltime = None if finaltime is None else float(finaltime) - self.starttime
+        self.relative_yieldtime = self.relative_time + (1.0e300 if yieldstep is None
+                                                        else float(yieldstep))
+        self.recorded_min_timestep = self.evolve_max_timestep
+        self.recorded_max_timestep = self.evolve_min_timestep
+        step = self.evolve_one_euler_step if self._timestep_fluxcalls == 1 \
+            else self.evolve_one_rk2_step
+        for _ in range(int(nsteps)):
+            initial = self.relative_time
+            step(1.0e300, self.relative_finaltime)
+            self.relative_time = initial + self.timestep
+            if self.record_dt_history:
+                self.dt_history.append(self.timestep)
+            self.number_of_steps += 1
+
+    # -- diagnostics --------------------------------------------------------------------
+    def get_water_volume(self):
+        v = C.c_double()
+        self._call("anuga_get_water_volume", C.byref(v))
+        return v.value
+
+    @property
+    def boundary_flux_integral(self):
+        v = C.c_double()
+        self._call("anuga_get_boundary_flux_integral", C.byref(v))
+        return v.value
+
+    def get_boundary_flux_integral(self):
+        return self.boundary_flux_integral
+
+    def synchronize(self):
+        self._call("anuga_synchronize")
+
+    def kernel_timing(self, name):
+        ms = C.c_double()
+        n = C.c_int64()
+        self._call("anuga_kernel_timing", name.encode(), C.byref(ms), C.byref(n))
+        return ms.value, n.value
+
+    def reset_kernel_timing(self, enable=True):
+        self._call("anuga_reset_kernel_timing", int(enable))

@@ -1,0 +1,169 @@
+/*
+ * anuga_hip.h - C ABI of the MI355X (gfx950) DE shallow-water step.
+ *
+ * One `anuga_domain` handle owns every device buffer of one mesh: the conserved
+ * state (stage, xmomentum, ymomentum), bed elevation, friction and the static
+ * mesh arrays stay resident in HBM for the whole evolve loop. Host pointers are
+ * borrowed only for the duration of a call. Arrays crossing this boundary use
+ * the reference's layout and triangle order (FP64, int64 indices, (N,3)
+ * row-major); any internal renumbering is hidden behind it.
+ *
+ * Every entry point returns ANUGA_OK (0) or a negative ANUGA_E* code; the
+ * message of the last failure is available from anuga_last_error().
+ * Handles are independent; one handle must not be used from two threads at once.
+ *
+ * The reference binds the same work per call through Cython shims that rebuild
+ * a `struct domain` of borrowed numpy pointers on every call
+ * (anuga/shallow_water/sw_domain_orig_ext.pyx:104-355); the entry points below
+ * replace those functions as cited on each one.
+ */
+#ifndef ANUGA_HIP_H
+#define ANUGA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ANUGA_OK 0
+#define ANUGA_E_ARG (-1)             /* invalid argument or unsupported option */
+#define ANUGA_E_HIP (-2)             /* HIP runtime failure (allocation, launch) */
+#define ANUGA_E_SEMI_IMPLICIT (-3)   /* semi-implicit denominator <= 0 (quantity.c:805-808) */
+#define ANUGA_E_NO_NEIGHBOUR (-4)    /* two-boundary triangle without an interior neighbour
+                                        (sw_domain_orig.c:2432-2437) */
+#define ANUGA_E_STATE (-5)           /* call out of sequence */
+
+/* quantity ids */
+#define ANUGA_STAGE 0
+#define ANUGA_XMOMENTUM 1
+#define ANUGA_YMOMENTUM 2
+#define ANUGA_ELEVATION 3
+#define ANUGA_FRICTION 4
+#define ANUGA_HEIGHT 5
+
+/* boundary kinds, one per boundary edge j (neighbours = -(j+1)) */
+#define ANUGA_BC_REFLECTIVE 0         /* anuga/shallow_water/boundaries.py:235-292 */
+#define ANUGA_BC_TRANSMISSIVE 1       /* generic_boundary_conditions.py:173-195 (edge values) */
+#define ANUGA_BC_DIRICHLET 2          /* generic_boundary_conditions.py:221-264 */
+#define ANUGA_BC_TRANSMISSIVE_CENTROID 3 /* transmissive with centroid_transmissive_bc */
+
+/* arithmetic of the step */
+#define ANUGA_PRECISION_FP64 0   /* everything FP64, bit-comparable with the reference */
+#define ANUGA_PRECISION_MIXED 1  /* FP32 extrapolation/reconstruction/flux; FP64 state,
+                                    update, CFL reduction, wet/dry thresholds, volume */
+
+typedef struct anuga_domain anuga_domain;
+
+/* Static mesh arrays in the reference's layout (anuga/abstract_2d_finite_volumes/
+ * general_mesh.py, neighbour_mesh.py). Boundary edges hold neighbours = -(j+1). */
+typedef struct anuga_mesh {
+    int64_t number_of_elements;            /* N */
+    int64_t boundary_length;               /* M */
+    const int64_t *neighbours;             /* (N,3) */
+    const int64_t *neighbour_edges;        /* (N,3) */
+    const int64_t *surrogate_neighbours;   /* (N,3) */
+    const int64_t *number_of_boundaries;   /* (N) */
+    const double *normals;                 /* (N,6) */
+    const double *edgelengths;             /* (N,3) */
+    const double *radii;                   /* (N) */
+    const double *areas;                   /* (N) */
+    const double *centroid_coordinates;    /* (N,2) */
+    const double *edge_coordinates;        /* (N,6) edge midpoints */
+} anuga_mesh;
+
+/* Algorithm parameters; the DE presets are in shallow_water_domain.py:583-707. */
+typedef struct anuga_params {
+    double g;
+    double epsilon;
+    double minimum_allowed_height;
+    double beta_w, beta_w_dry;
+    double beta_uh, beta_uh_dry;
+    double beta_vh, beta_vh_dry;
+    int64_t low_froude;                        /* only 0 (the DE presets) */
+    int64_t extrapolate_velocity_second_order; /* only 1 (the DE presets) */
+    int64_t timestep_fluxcalls;                /* 1 = euler (DE0), 2 = rk2 (DE1) */
+    int64_t precision;                         /* ANUGA_PRECISION_* */
+    int64_t reorder;                           /* 1 = Reverse-Cuthill-McKee renumbering */
+} anuga_params;
+
+/* Create a handle: uploads the mesh, allocates all device buffers.
+ * Replaces the per-call marshalling of get_python_domain_pointers
+ * (sw_domain_orig_ext.pyx:143-355) with a one-off upload. */
+int anuga_create(const anuga_mesh *mesh, const anuga_params *params, anuga_domain **out);
+void anuga_destroy(anuga_domain *d);
+/* Message of the last failure on this handle (or of the last failed create when d is NULL). */
+const char *anuga_last_error(const anuga_domain *d);
+/* Change algorithm parameters (flow algorithm presets, betas) between steps. */
+int anuga_set_params(anuga_domain *d, const anuga_params *params);
+
+/* Centroid values in the caller's triangle order (Quantity.centroid_values). */
+int anuga_set_centroid_values(anuga_domain *d, int quantity, const double *values);
+int anuga_get_centroid_values(anuga_domain *d, int quantity, double *values);
+/* Edge values (N,3) from the last extrapolation; quantity in
+ * STAGE, XMOMENTUM, YMOMENTUM, ELEVATION, HEIGHT (Quantity.edge_values). */
+int anuga_get_edge_values(anuga_domain *d, int quantity, double *values);
+
+/* Boundary condition of every boundary edge: kind (M) and, for Dirichlet edges,
+ * [stage, xmom, ymom] (M,3) (may be NULL when no edge is Dirichlet).
+ * Replaces Generic_Domain.update_boundary (generic_domain.py:2288-2306): the
+ * boundary state is evaluated on the device inside the flux kernel. */
+int anuga_set_boundary(anuga_domain *d, const int32_t *kind, const double *dirichlet_values);
+
+/* protect + second-order edge extrapolation, with the result written back to
+ * the centroid state. Replaces protect_new + extrapolate_second_order_edge_sw
+ * (sw_domain_orig_ext.pyx:449-475; sw_domain_orig.c:1215-1281, :1994-2680).
+ * mass_error (may be NULL) receives protect's added mass. */
+int anuga_distribute_to_vertices_and_edges(anuga_domain *d, double *mass_error);
+
+/* First substep of a step: protect + extrapolate + boundary + edge fluxes.
+ * Replaces compute_fluxes_ext_central on substep 0 (sw_domain_orig_ext.pyx:419-432;
+ * sw_domain_orig.c:765-1150). flux_timestep receives the CFL timestep
+ * (1e100 when no edge carries a wave speed above epsilon). */
+int anuga_compute_fluxes(anuga_domain *d, double *flux_timestep);
+
+/* Manning friction + conserved update + fix negative cells with timestep dt:
+ * replaces manning_friction_flat, Quantity.update x3 and fix_negative_cells
+ * (friction.py:22-76, quantity_ext.pyx update, sw_domain_orig_ext.pyx:538-553).
+ * For rk2 the result goes to the substep buffer and the step's initial state is
+ * kept as the RK2 backup (backup_centroid_values, quantity.c:735-749).
+ * num_negative (may be NULL) receives the fix_negative_cells count. */
+int anuga_update_conserved_quantities(anuga_domain *d, double dt, int64_t *num_negative);
+
+/* Second rk2 substep with the same dt: protect + extrapolate + boundary + fluxes
+ * + friction + update + fix negative + saxpy(0.5, 0.5) with the backup
+ * (generic_domain.py:2018-2045; quantity.c:752-769). */
+int anuga_rk2_second_substep(anuga_domain *d, double dt, int64_t *num_negative);
+
+/* boundary_flux_sum of the last step's substeps (sums[0..1]): the boundary
+ * inflow rate (sw_domain_orig.c:1119-1126). Synchronizes. */
+int anuga_get_boundary_flux_sums(anuga_domain *d, double *sums);
+
+/* Time integral of the boundary inflow accumulated on the device after every
+ * update, as boundary_flux_integral_operator.__call__ does per step
+ * (anuga/operators/boundary_flux_integral_operator.py:39-49). Synchronizes. */
+int anuga_get_boundary_flux_integral(anuga_domain *d, double *integral);
+
+/* Total fix_negative_cells count since creation, as of the last synchronizing
+ * call (anuga_compute_fluxes synchronizes every step). */
+int anuga_get_negative_cells(anuga_domain *d, int64_t *total);
+
+/* sum(area * height_centroid) with height from the last extrapolation
+ * (get_water_volume, shallow_water_domain.py:1626-1653), fixed-order FP64 reduction. */
+int anuga_get_water_volume(anuga_domain *d, double *volume);
+
+/* Wait for all work on the handle's stream. */
+int anuga_synchronize(anuga_domain *d);
+
+/* Device-side timing of the last anuga_compute_fluxes / anuga_rk2_second_substep
+ * flux kernels: average milliseconds per launch of each kernel since the last
+ * reset (event-timed on the handle's stream). names: "extrapolate", "flux0",
+ * "update", "flux1". */
+int anuga_kernel_timing(anuga_domain *d, const char *name, double *avg_ms, int64_t *launches);
+int anuga_reset_kernel_timing(anuga_domain *d, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ANUGA_HIP_H */

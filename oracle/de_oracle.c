@@ -30,8 +30,31 @@
  * reference's default build has none).
  */
 #include <math.h>
+#include <quadmath.h>
 #include <stdint.h>
 #include <string.h>
+
+/*
+ * pow(h, 7/3) of Manning friction. Mode 0 calls the C library's pow exactly as
+ * the reference does. Mode 1 returns the correctly rounded value: glibc's pow
+ * is within 0.52 ulp, so its result is the correctly rounded one unless the
+ * exact value is within 0.02 ulp of a rounding boundary; that is tested with
+ * an 80-bit powl residual, and such rare cases are settled with 113-bit powq.
+ */
+static int g_pow_mode = 0;
+
+void oracle_set_pow_mode(int mode) { g_pow_mode = mode; }
+
+double oracle_pow_7_3(double h)
+{
+    const double y = 7.0 / 3.0;
+    double p = pow(h, y);
+    long double e;
+    if (g_pow_mode == 0) return p;
+    e = powl((long double)h, (long double)y) - (long double)p;
+    if (fabsl(e) < 0.49L * (long double)(nextafter(p, INFINITY) - p)) return p;
+    return (double)powq((__float128)h, (__float128)y);
+}
 
 typedef struct {
     int64_t N;                       /* triangles */
@@ -375,6 +398,7 @@ double oracle_protect(const oracle_mesh *m, const oracle_params *p, oracle_state
 {
     double mass_error = 0.0;
     int64_t k;
+#pragma omp parallel for schedule(static) reduction(+:mass_error)
     for (k = 0; k < m->N; k++) {
         double hc = s->w[k] - s->z[k];
         if (hc < p->minimum_allowed_height * 1.0) {
@@ -484,6 +508,7 @@ int64_t oracle_extrapolate(const oracle_mesh *m, const oracle_params *p, oracle_
     memset(s->ywork, 0, N * sizeof(double));
 
     if (p->extrapolate_velocity_second_order == 1) {
+#pragma omp parallel for schedule(static)
         for (k = 0; k < N; k++) {
             double dk;
             s->h[k] = fmax(s->w[k] - s->z[k], 0.0);
@@ -503,6 +528,7 @@ int64_t oracle_extrapolate(const oracle_mesh *m, const oracle_params *p, oracle_
         }
     }
 
+#pragma omp parallel for schedule(static)
     for (k = 0; k < N; k++) {
         int64_t k0 = m->surrogate[3 * k], k1 = m->surrogate[3 * k + 1], k2 = m->surrogate[3 * k + 2];
         if (((s->h[k0] < mah) | (k0 == k)) & ((s->h[k1] < mah) | (k1 == k)) &
@@ -514,6 +540,8 @@ int64_t oracle_extrapolate(const oracle_mesh *m, const oracle_params *p, oracle_
         }
     }
 
+    int64_t status = 0;
+#pragma omp parallel for schedule(static) reduction(min:status)
     for (k = 0; k < N; k++) {
         int64_t k3 = 3 * k, k6 = 6 * k;
         double x, y, dxv0, dxv1, dxv2, dyv0, dyv1, dyv2;
@@ -580,7 +608,10 @@ int64_t oracle_extrapolate(const oracle_mesh *m, const oracle_params *p, oracle_
             double dx1, dy1, area2, dx2, dy2;
             for (e = k3; e < k3 + 3; e++)
                 if (m->surrogate[e] != k) break;
-            if (e == k3 + 3) return -1;
+            if (e == k3 + 3) {
+                status = -1;
+                continue;
+            }
             k1 = m->surrogate[e];
             dx1 = m->centroids[2 * k1] - x;
             dy1 = m->centroids[2 * k1 + 1] - y;
@@ -595,6 +626,8 @@ int64_t oracle_extrapolate(const oracle_mesh *m, const oracle_params *p, oracle_
         }
     }
 
+    if (status != 0) return status;
+#pragma omp parallel for schedule(static)
     for (k = 0; k < N; k++) {
         int64_t k3 = 3 * k;
         int i;
@@ -642,12 +675,13 @@ void oracle_manning_friction_flat(int64_t N, double g, double eps,
 {
     const double seven_thirds = 7.0 / 3.0;
     int64_t k;
+#pragma omp parallel for schedule(static)
     for (k = 0; k < N; k++) {
         if (eta[k] > eps) {
             double h = w[k] - z[k];
             if (h >= eps) {
                 double S = -g * eta[k] * eta[k] * sqrt((uh[k] * uh[k] + vh[k] * vh[k]));
-                S /= pow(h, seven_thirds);
+                S /= (g_pow_mode == 0) ? pow(h, seven_thirds) : oracle_pow_7_3(h);
                 xmom_si[k] += S * uh[k];
                 ymom_si[k] += S * vh[k];
             }
@@ -658,18 +692,18 @@ void oracle_manning_friction_flat(int64_t N, double g, double eps,
 /* Returns 0, or -1 on a non-positive semi-implicit denominator. */
 int64_t oracle_update(int64_t N, double dt, double *q, const double *ex, double *si)
 {
-    int64_t k;
+    int64_t k, status = 0;
+#pragma omp parallel for schedule(static) reduction(min:status)
     for (k = 0; k < N; k++) {
-        double x = q[k];
+        double x = q[k], den;
         if (x == 0.0) si[k] = 0.0;
         else si[k] /= x;
+        q[k] += dt * ex[k];
+        den = 1.0 - dt * si[k];
+        if (den <= 0.0) status = -1;
+        else q[k] /= den;
     }
-    for (k = 0; k < N; k++) q[k] += dt * ex[k];
-    for (k = 0; k < N; k++) {
-        double den = 1.0 - dt * si[k];
-        if (den <= 0.0) return -1;
-        q[k] /= den;
-    }
+    if (status != 0) return -1;
     memset(si, 0, N * sizeof(double));
     return 0;
 }
@@ -678,6 +712,7 @@ int64_t oracle_fix_negative_cells(int64_t N, const int64_t *tri_full_flag,
                                   double *w, const double *z, double *uh, double *vh)
 {
     int64_t k, count = 0;
+#pragma omp parallel for schedule(static) reduction(+:count)
     for (k = 0; k < N; k++) {
         if ((w[k] - z[k] < 0.0) & (tri_full_flag[k] > 0)) {
             count++;
@@ -697,5 +732,6 @@ void oracle_backup(int64_t N, const double *q, double *backup)
 void oracle_saxpy(int64_t N, double a, double b, double *q, const double *backup)
 {
     int64_t k;
+#pragma omp parallel for schedule(static)
     for (k = 0; k < N; k++) q[k] = a * q[k] + b * backup[k];
 }

@@ -94,10 +94,16 @@ class OracleKernels:
     """Kernels of oracle/de_oracle.c. `gather=True` uses the per-triangle
     (OpenMP) flux loop instead of the edge-once-and-mirror loop."""
 
-    def __init__(self, gather=False):
+    def __init__(self, gather=False, pow_mode=1):
+        """pow_mode 0: the C library's pow, as the reference; 1: correctly
+        rounded pow(h, 7/3) (see oracle_pow_7_3 in de_oracle.c)."""
         lib = _load(os.path.join(HERE, "liboracle.so"))
         self.lib = lib
         self.gather = gather
+        self.pow_mode = pow_mode
+        lib.oracle_set_pow_mode.argtypes = [C.c_int]
+        lib.oracle_pow_7_3.restype = C.c_double
+        lib.oracle_pow_7_3.argtypes = [C.c_double]
         lib.oracle_protect.restype = C.c_double
         lib.oracle_extrapolate.restype = C.c_int64
         lib.oracle_compute_fluxes.restype = C.c_double
@@ -162,6 +168,7 @@ class OracleKernels:
 
     def friction(self):
         d = self.d
+        self.lib.oracle_set_pow_mode(self.pow_mode)
         self.lib.oracle_manning_friction_flat(d.N, d.g, d.minimum_allowed_height,
                                               _d(d.stage_c), _d(d.elev_c), _d(d.xmom_c),
                                               _d(d.ymom_c), _d(d.friction_c),
