@@ -67,6 +67,7 @@ SIGNATURES = {
     "anuga_get_boundary_flux_integral": (C.c_int, [C.c_void_p, _dp]),
     "anuga_get_negative_cells": (C.c_int, [C.c_void_p, _ip]),
     "anuga_get_water_volume": (C.c_int, [C.c_void_p, _dp]),
+    "anuga_rcm_order": (C.c_int, [C.c_int64, _ip, _ip]),
     "anuga_synchronize": (C.c_int, [C.c_void_p]),
     "anuga_kernel_timing": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _ip]),
     "anuga_reset_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),

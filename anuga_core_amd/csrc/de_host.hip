@@ -783,6 +783,16 @@ int anuga_get_water_volume(anuga_domain *d, double *volume)
     return ANUGA_OK;
 }
 
+int anuga_rcm_order(int64_t N, const int64_t *neighbours, int64_t *order)
+{
+    if (N <= 0 || N >= (int64_t(1) << 29) || !neighbours || !order) return ANUGA_E_ARG;
+    for (int64_t i = 0; i < 3 * N; i++)
+        if (neighbours[i] >= N) return ANUGA_E_ARG;
+    const std::vector<int> o = rcm_order((int)N, neighbours);
+    for (int64_t i = 0; i < N; i++) order[i] = o[i];
+    return ANUGA_OK;
+}
+
 int anuga_synchronize(anuga_domain *d)
 {
     if (!d) return ANUGA_E_ARG;

@@ -152,6 +152,11 @@ int anuga_get_negative_cells(anuga_domain *d, int64_t *total);
  * (get_water_volume, shallow_water_domain.py:1626-1653), fixed-order FP64 reduction. */
 int anuga_get_water_volume(anuga_domain *d, double *volume);
 
+/* The Reverse-Cuthill-McKee triangle order applied at creation when
+ * params->reorder is set: order[new] = old. Host-only (needs no device).
+ * neighbours is (N,3) with boundary edges negative. */
+int anuga_rcm_order(int64_t N, const int64_t *neighbours, int64_t *order);
+
 /* Wait for all work on the handle's stream. */
 int anuga_synchronize(anuga_domain *d);
 

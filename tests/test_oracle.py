@@ -1,0 +1,279 @@
+"""Pinning the CPU oracle (CPU only).
+
+* flux function: the reference's known-answer tests
+  (anuga/shallow_water/tests/test_shallow_water_domain.py:274-428);
+* orchestration + kernels: the reference's golden trajectories
+  (test_second_order_flat_bed_onestep :5088-5161,
+   test_bedslope_problem_second_order_one_step :5626-5715);
+* every kernel, bit for bit, against the reference's own C compiled from its
+  sources (oracle/_ref), on seeded DE0 and DE1 wet/dry runs;
+* friction and update: closed forms of test_flat_manning_friction (:3276) and
+  test_quantity.py test_update_* (:3213-3275);
+* the correctly rounded pow mode against 80-digit decimal arithmetic.
+"""
+
+import math
+import os
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+from anuga_core_amd.boundaries import Reflective_boundary
+from anuga_core_amd.mesh import Mesh, rectangular
+from oracle.kernels import HERE as ORACLE_DIR
+from oracle.kernels import OracleKernels, RefKernels
+from oracle.oracle_domain import G, OracleDomain
+from tests import scenarios as S
+
+HAVE_REF = os.path.exists(os.path.join(ORACLE_DIR, "_ref", "libref_sw.so"))
+need_ref = pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref not built (no reference tree)")
+
+EPS = 1.0e-12
+
+
+def _kernel_sets():
+    ks = [("oracle", lambda: OracleKernels(pow_mode=0)),
+          ("oracle-gather", lambda: OracleKernels(gather=True, pow_mode=0))]
+    if HAVE_REF:
+        ks.append(("reference", RefKernels))
+    return ks
+
+
+KSETS = _kernel_sets()
+
+
+@pytest.mark.parametrize("name,K", KSETS, ids=[k[0] for k in KSETS])
+def test_flux_known_answers(name, K):
+    k = K()
+    # test_flux_zero_case
+    ef, ms, pf = k.flux_function_central([0.0, 0.0], np.zeros(3), np.zeros(3), 0, 0, 0, 0,
+                                         EPS, 0.0, G, 1)
+    assert np.allclose(ef, 0) and pf == 0.0 and ms == 0.0
+    # test_flux_constants
+    for lf in (1, 2):
+        w = 2.0
+        ef, ms, pf = k.flux_function_central([1.0, 0.0], [w, 0, 0], [w, 0, 0], w, w, w, w,
+                                             EPS, 0.0, G, lf)
+        assert ms == math.sqrt(G * w) and np.allclose(pf, 0.5 * G * w * w)
+        assert np.allclose(ef, 0)
+    # test_flux1
+    h = 0.3
+    ef, ms, pf = k.flux_function_central([1.0, 0.0], [-0.2, 2, 3], [-0.2, 2, 3], h, h, h, h,
+                                         EPS, -0.5, G, 2)
+    assert np.allclose(ms, 8.38130948661)
+    assert np.allclose(ef, [2.0, 13.3333333, 20.0])
+    # test_flux2
+    ql = np.array([-0.075, 2, 3])
+    hl = ql[0] + 0.375
+    ef, ms, pf = k.flux_function_central([0.0, -1.0], ql, ql, hl, hl, hl, hl, EPS, -0.375, G, 2)
+    assert np.allclose(ef, [-3.0, -20.0, -30.0])
+    assert np.allclose(ms, 11.7146428199) and np.allclose(pf, 0.441)
+    # test_flux3
+    s2 = math.sqrt(2) / 2
+    ef, ms, pf = k.flux_function_central([-s2, s2], ql, ql, hl, hl, hl, hl, EPS, -0.375, G, 2)
+    assert np.allclose(ef, [s2, 4.71404521, 7.07106781])
+    assert np.allclose(ms, 4.0716654239) and np.allclose(pf, 0.441)
+    # test_flux4
+    ql = np.array([-0.34319278, 0.10254161, 0.07273855])
+    qr = np.array([-0.30683287, 0.1071986, 0.05930515])
+    hl, hr = ql[0] + 0.375, qr[0] + 0.375
+    ef, ms, pf = k.flux_function_central([-s2, s2], ql, qr, hl, hr, hl, hr, EPS, -0.375, G, 2)
+    assert np.allclose(ms, 1.31414103233)
+    assert np.allclose(ef, [-0.04072676, -0.05733579, -0.02967421])
+    assert np.allclose(pf, 0.0192765311)
+
+
+@pytest.mark.parametrize("name,K", KSETS, ids=[k[0] for k in KSETS])
+def test_rotate(name, K):
+    """test_rotate (test_shallow_water_domain.py:274-293)."""
+    k = K()
+    r = k.rotate([1.0, 2.0, 3.0], [0.0, -1.0], 1)
+    assert list(r) == [1, -3, 2]
+    assert np.allclose(k.rotate(r, [0.0, -1.0], -1), [1, 2, 3])
+
+
+def _flat_bed(K):
+    mesh, alg, q, bc = S.flat_bed_dirichlet()
+    d = S.setup(OracleDomain(mesh, K), alg, q, bc, betas=0.9)
+    for _ in d.evolve(yieldstep=0.05, finaltime=0.05):
+        pass
+    return d
+
+
+GOLDEN_KS = KSETS + [("oracle-crpow", lambda: OracleKernels(gather=True, pow_mode=1))]
+
+
+@pytest.mark.parametrize("name,K", GOLDEN_KS, ids=[k[0] for k in GOLDEN_KS])
+def test_golden_flat_bed_onestep(name, K):
+    d = _flat_bed(K())
+    assert np.allclose(d.recorded_min_timestep, 0.03571428571428564)
+    assert np.allclose(d.recorded_max_timestep, 0.03571428571428564)
+    assert np.allclose(d.stage_c[:12], [0.00117244, 0.025897, 0.00200148, 0.025897, 0.00200148,
+                                        0.025897, 0.00200148, 0.025897, 0.00200148, 0.025897,
+                                        0.00200148, 0.02672604])
+    d.distribute_to_vertices_and_edges()
+    v = d.stage_v.reshape(-1, 3)
+    assert np.allclose(v[:12, 0], [-0.00104301, 0.025897, 0.00020015, 0.025897, 0.00020015,
+                                   0.025897, 0.00020015, 0.025897, 0.00020015, 0.025897,
+                                   0.00011501, 0.02672604])
+    assert np.allclose(v[:12, 1], [0.00328283, 0.025897, 0.00020015, 0.025897, 0.00020015,
+                                   0.025897, 0.00020015, 0.025897, 0.00020015, 0.025897,
+                                   0.00028528, 0.02672604])
+    assert np.allclose(v[:12, 2], [0.0012775, 0.025897, 0.00560414, 0.025897, 0.00560414,
+                                   0.025897, 0.00560414, 0.025897, 0.00560414, 0.025897,
+                                   0.00560414, 0.02672604])
+    assert np.allclose(d.xmom_c[:12], [0.000189, 0.0042] * 6)
+    assert np.allclose(d.ymom_c[:12], [-1.89e-04, 0.0, 0.0, 0.0, -5.57589689e-20,
+                                       -2.78794844e-20, 8.36384533e-20, 0.0, -3.06674329e-19,
+                                       0.0, 2.23035875e-19, -1.89e-04])
+
+
+W_EX_BEDSLOPE = [
+    0.01571282, 0.02678718, 0.01765727, 0.02678718, 0.01765727, 0.02678718, 0.01765727,
+    0.02678718, 0.01765727, 0.02678718, 0.01765727, 0.02873162, -0.04259259, -0.02407407,
+    -0.04259259, -0.02407407, -0.04259259, -0.02407407, -0.04259259, -0.02407407, -0.04259259,
+    -0.02407407, -0.04259259, -0.02407407, -0.09814815, -0.07962963, -0.09814815, -0.07962963,
+    -0.09814815, -0.07962963, -0.09814815, -0.07962963, -0.09814815, -0.07962963, -0.09814815,
+    -0.07962963, -0.1537037, -0.13518519, -0.1537037, -0.13518519, -0.1537037, -0.13518519,
+    -0.1537037, -0.13518519, -0.1537037, -0.13518519, -0.1537037, -0.13518519, -0.20925926,
+    -0.19074074, -0.20925926, -0.19074074, -0.20925926, -0.19074074, -0.20925926, -0.19074074,
+    -0.20925926, -0.19074074, -0.20925926, -0.19074074, -0.26206496, -0.2509906, -0.26012051,
+    -0.2509906, -0.26012051, -0.2509906, -0.26012051, -0.2509906, -0.26012051, -0.2509906,
+    -0.26012051, -0.24904616]
+
+
+@pytest.mark.parametrize("name,K", GOLDEN_KS, ids=[k[0] for k in GOLDEN_KS])
+def test_golden_bedslope_one_step(name, K):
+    """test_bedslope_problem_second_order_one_step (:5626-5715): bed and stage
+    given at vertices (centroid = mean of vertices), DE0 default."""
+    p, v, b = rectangular(6, 6)
+    mesh = Mesh(p, v, b)
+    d = OracleDomain(mesh, K(), "DE0")
+    for k in ("beta_w", "beta_w_dry", "beta_uh", "beta_uh_dry", "beta_vh", "beta_vh_dry"):
+        setattr(d, k, 0.9)
+    V = mesh.vertex_coordinates
+    zv = (-V[:, 0] / 3).reshape(-1, 3)
+    d.set_vertex_values("elevation", zv)
+    d.set_vertex_values("stage", zv + 0.05)
+    d.set_boundary({t: Reflective_boundary(d) for t in ("left", "right", "top", "bottom")})
+    assert np.allclose(d.stage_c[:3], [0.01296296, 0.03148148, 0.01296296])
+    for _ in d.evolve(yieldstep=0.05, finaltime=0.05):
+        pass
+    assert np.allclose(d.stage_c, W_EX_BEDSLOPE)
+
+
+@need_ref
+@pytest.mark.parametrize("builder,kw,yieldstep,final", [
+    (S.dam_break_de0, dict(m=24, n=20), 1.0, 3.0),
+    (S.hills_de1, dict(m=18, n=18, seed=3), 5.0, 30.0),
+    (S.hills_de1, dict(m=12, n=16, seed=7), 2.0, 12.0),
+    (S.flat_bed_dirichlet, {}, 0.05, 0.2),
+])
+@pytest.mark.parametrize("gather", [False, True])
+def test_oracle_bitwise_vs_reference_c(builder, kw, yieldstep, final, gather):
+    """The restatement equals the reference's compiled C bit for bit, every
+    state array and the whole timestep sequence (libm pow on both sides)."""
+    res = []
+    for K in (RefKernels(), OracleKernels(gather=gather, pow_mode=0)):
+        mesh, alg, q, bc = builder(**kw)
+        d = S.setup(OracleDomain(mesh, K), alg, q, bc)
+        for _ in d.evolve(yieldstep=yieldstep, finaltime=final):
+            pass
+        res.append(d)
+    r, o = res
+    assert r.dt_history == o.dt_history
+    for a in ("stage_c", "xmom_c", "ymom_c", "height_c", "stage_e", "xmom_e", "ymom_e",
+              "elev_e", "height_e", "stage_v", "elev_v"):
+        assert np.array_equal(getattr(r, a), getattr(o, a)), a
+    assert r.boundary_flux_integral == pytest.approx(o.boundary_flux_integral, rel=1e-12,
+                                                     abs=1e-15)
+
+
+@need_ref
+def test_reference_explicit_update_single_call():
+    """One distribute + compute_fluxes from the same state: explicit updates and
+    flux timestep match (the cross-backend pattern of test_DE_openmp.py:32-150)."""
+    out = []
+    for K in (RefKernels(), OracleKernels(gather=True)):
+        mesh, alg, q, bc = S.hills_de1(m=14, n=14, seed=11)
+        d = S.setup(OracleDomain(mesh, K), alg, q, bc)
+        d.distribute_to_vertices_and_edges()
+        d.update_boundary()
+        d.compute_fluxes(0)
+        out.append((d.flux_timestep, d.stage_ex.copy(), d.xmom_ex.copy(), d.ymom_ex.copy()))
+    assert out[0][0] == out[1][0]
+    for i in (1, 2, 3):
+        assert np.array_equal(out[0][i], out[1][i])
+
+
+def _four_tri_domain(K):
+    m = Mesh([[0.0, 0.0], [0.0, 2.0], [2.0, 0.0], [0.0, 4.0], [2.0, 2.0], [4.0, 0.0]],
+             [[1, 0, 2], [1, 2, 4], [4, 2, 5], [3, 1, 4]])
+    return OracleDomain(m, K, "DE0")
+
+
+@pytest.mark.parametrize("pow_mode", [0, 1])
+def test_flat_manning_friction_closed_form(pow_mode):
+    """test_flat_manning_friction (:3276-3356): S = -g eta^2 |uh| / h^(7/3)."""
+    d = _four_tri_domain(OracleKernels(pow_mode=pow_mode))
+    d.minimum_allowed_height = 1e-5
+    V = d.mesh.vertex_coordinates
+    zv = (3 * V[:, 0]).reshape(-1, 3)
+    d.set_vertex_values("elevation", zv)
+    d.set_vertex_values("stage", zv + 0.1)
+    eta = 0.07
+    d.friction_c[:] = eta
+    d.xmom_c[:] = 1.0
+    d.kernels.friction()
+    S_ = -G * eta ** 2 / 0.1 ** (7.0 / 3)
+    assert np.allclose(d.xmom_si, S_) and np.allclose(d.ymom_si, 0)
+    d.xmom_si[:] = 0
+    d.xmom_c[:] = 3.0
+    d.ymom_c[:] = 4.0
+    d.kernels.friction()
+    S_ = -G * eta ** 2 * 5 / 0.1 ** (7.0 / 3)
+    assert np.allclose(d.xmom_si, 3 * S_) and np.allclose(d.ymom_si, 4 * S_)
+
+
+def test_update_closed_forms():
+    """test_update_explicit / semi_implicit / both (test_quantity.py:3213-3275)."""
+    d = _four_tri_domain(OracleKernels())
+    c0 = np.array([1.0, 2.0, 3.0, 4.0])
+    dt = 0.1
+    d.stage_c[:] = c0
+    d.stage_ex[:] = [1.0, 1.0, 1.0, 1.0]
+    d.xmom_c[:] = c0
+    d.xmom_si[:] = 1.0
+    d.ymom_c[:] = c0
+    d.ymom_ex[:] = [4.0, 3.0, 2.0, 1.0]
+    d.ymom_si[:] = 1.0
+    d.kernels.update(dt)
+    assert np.allclose(d.stage_c, c0 + 0.1)
+    denom = 1.0 - dt * (1.0 / c0)
+    assert np.allclose(d.xmom_c, c0 / denom)
+    assert np.allclose(d.ymom_c, (c0 + dt * np.array([4.0, 3.0, 2.0, 1.0])) / denom)
+    assert np.all(d.xmom_si == 0)
+    d.xmom_si[:] = 100.0
+    with pytest.raises(AssertionError):
+        d.kernels.update(dt)
+
+
+def test_pow_modes():
+    """Mode 0 is the C library's pow; mode 1 is correctly rounded (80-digit decimal)."""
+    k = OracleKernels()
+    lib = k.lib
+    getcontext().prec = 80
+    y = Decimal(7.0 / 3.0)
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([10 ** rng.uniform(-12, 4, 300), rng.uniform(1e-5, 50, 300),
+                         [1.0, 2.0, 1e-5, 0.5, 1e-12, 1e4]])
+    lib.oracle_set_pow_mode(0)
+    assert all(lib.oracle_pow_7_3(x) == x ** (7.0 / 3.0) for x in xs)
+    lib.oracle_set_pow_mode(1)
+    for x in xs:
+        r = lib.oracle_pow_7_3(float(x))
+        exact = (y * Decimal(float(x)).ln()).exp()
+        err = abs(Decimal(r) - exact) / Decimal(math.ulp(r))
+        assert err <= Decimal("0.5"), (x, float(err))
+    lib.oracle_set_pow_mode(0)

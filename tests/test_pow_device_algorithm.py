@@ -1,0 +1,68 @@
+"""The device's correctly rounded pow(h, 7/3) (anuga_core_amd/csrc/de_pow.h),
+compiled for the host with g++, against the oracle's independently computed
+correctly rounded pow (80-bit residual check + 113-bit powq) and against
+80-digit decimal arithmetic (CPU only)."""
+
+import ctypes as C
+import math
+import os
+import subprocess
+import tempfile
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+from oracle.kernels import OracleKernels
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+SRC = r'''
+#include "de_pow.h"
+extern "C" void pow73_many(long n, const double *x, double *y) {
+    for (long i = 0; i < n; i++) y[i] = de_pow_7_3(x[i]);
+}
+'''
+
+
+@pytest.fixture(scope="module")
+def pow73():
+    tmp = tempfile.mkdtemp()
+    src = os.path.join(tmp, "p.cpp")
+    so = os.path.join(tmp, "libp.so")
+    open(src, "w").write(SRC)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-shared", "-fPIC",
+                    "-I" + os.path.join(ROOT, "anuga_core_amd", "csrc"), "-o", so, src],
+                   check=True)
+    lib = C.CDLL(so)
+    lib.pow73_many.argtypes = [C.c_long, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+
+    def f(x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty_like(x)
+        lib.pow73_many(x.size, x.ctypes.data_as(C.POINTER(C.c_double)),
+                       y.ctypes.data_as(C.POINTER(C.c_double)))
+        return y
+    return f
+
+
+def test_matches_oracle_correctly_rounded_pow(pow73):
+    k = OracleKernels()
+    k.lib.oracle_set_pow_mode(1)
+    rng = np.random.default_rng(17)
+    xs = np.concatenate([10 ** rng.uniform(-12, 4.5, 100000), rng.uniform(1e-5, 80, 100000)])
+    y = pow73(xs)
+    ref = np.array([k.lib.oracle_pow_7_3(float(x)) for x in xs])
+    k.lib.oracle_set_pow_mode(0)
+    assert np.array_equal(y, ref)
+
+
+def test_correct_rounding_against_decimal(pow73):
+    getcontext().prec = 80
+    e = Decimal(7.0 / 3.0)
+    rng = np.random.default_rng(23)
+    xs = np.concatenate([10 ** rng.uniform(-12, 4.5, 400), [1.0, 2.0, 0.5, 1e-5, 1e-12, 8.0]])
+    y = pow73(xs)
+    for x, r in zip(xs, y):
+        exact = (e * Decimal(float(x)).ln()).exp()
+        assert abs(Decimal(float(r)) - exact) <= Decimal(math.ulp(float(r))) / 2, x
