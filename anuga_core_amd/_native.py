@@ -21,6 +21,7 @@ ANUGA_E_HIP = -2
 ANUGA_E_SEMI_IMPLICIT = -3
 ANUGA_E_NO_NEIGHBOUR = -4
 ANUGA_E_STATE = -5
+ANUGA_E_TIMESTEP = -6
 
 STAGE, XMOMENTUM, YMOMENTUM, ELEVATION, FRICTION, HEIGHT = range(6)
 PRECISION_FP64, PRECISION_MIXED = 0, 1
@@ -48,6 +49,21 @@ class AnugaParams(C.Structure):
                 ("reorder", C.c_int64)]
 
 
+class TimeControl(C.Structure):
+    _fields_ = [("CFL", C.c_double), ("evolve_max_timestep", C.c_double),
+                ("evolve_min_timestep", C.c_double), ("relative_finaltime", C.c_double),
+                ("relative_yieldtime", C.c_double), ("fixed_flux_timestep", C.c_double),
+                ("epsilon", C.c_double), ("max_smallsteps", C.c_int64),
+                ("default_order", C.c_int64)]
+
+
+class StepState(C.Structure):
+    _fields_ = [("relative_time", C.c_double), ("timestep", C.c_double),
+                ("flux_timestep", C.c_double), ("recorded_min_timestep", C.c_double),
+                ("recorded_max_timestep", C.c_double), ("smallsteps", C.c_int64),
+                ("order", C.c_int64), ("steps_taken", C.c_int64), ("reached", C.c_int64)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/anuga_hip.h
 SIGNATURES = {
     "anuga_create": (C.c_int, [C.POINTER(AnugaMesh), C.POINTER(AnugaParams),
@@ -63,6 +79,8 @@ SIGNATURES = {
     "anuga_compute_fluxes": (C.c_int, [C.c_void_p, _dp]),
     "anuga_update_conserved_quantities": (C.c_int, [C.c_void_p, C.c_double, _ip]),
     "anuga_rk2_second_substep": (C.c_int, [C.c_void_p, C.c_double, _ip]),
+    "anuga_evolve_steps": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(TimeControl),
+                                     C.POINTER(StepState), _dp]),
     "anuga_get_boundary_flux_sums": (C.c_int, [C.c_void_p, _dp]),
     "anuga_get_boundary_flux_integral": (C.c_int, [C.c_void_p, _dp]),
     "anuga_get_negative_cells": (C.c_int, [C.c_void_p, _ip]),
@@ -109,4 +127,6 @@ def check(handle, rc, what):
         raise AssertionError(msg or "division by zero in semi implicit update")
     if rc == ANUGA_E_NO_NEIGHBOUR:
         raise Exception(msg or "Internal neighbour not found")
+    if rc == ANUGA_E_TIMESTEP:
+        raise Exception(msg)
     raise DeviceError("%s failed (%d): %s" % (what, rc, msg))

@@ -51,6 +51,32 @@ struct DeBoundary {
     int M;
 };
 
+// Device-resident step control: reduction results, the timestep, the
+// relative time and the update_timestep state, so that a sequence of steps
+// (or a captured graph of one step) needs no host round trip.
+struct DeControl {
+    double flux_dt;        // CFL timestep of the last first substep
+    double bflux0, bflux1; // boundary inflow of the two substeps (boundary_flux_sum)
+    double mass;           // protect's added mass (last reduction)
+    double volume;         // last water volume reduction
+    double dt;             // timestep of the current step
+    double t;              // relative time
+    double integral;       // boundary flux integral
+    double rec_min, rec_max;
+    // update_timestep parameters (generic_domain.py:2349-2415)
+    double cfl, max_dt, min_dt, finaltime, yieldtime, fixed_flux_dt, eps;
+    long long nsteps;      // steps completed since the last set-up
+    int smallsteps, order, default_order, max_smallsteps;
+    int done;              // 1: finaltime reached, 2: yield time reached (steps become no-ops)
+    int status;            // error bits, DE_ERR_*
+    int hist_cap, rk2;
+};
+
+#define DE_ERR_NO_NEIGHBOUR 1
+#define DE_ERR_SEMI_IMPLICIT 2
+#define DE_ERR_SMALL_TIMESTEP 4
+#define DE_ERR_OVERSHOOT 8
+
 // flags written by the extrapolation, per triangle
 #define DE_F_CLAMP 1u   // protect raised the stage to the bed (sw_domain_orig.c:1262-1274)
 #define DE_F_KEEP 2u    // centroid momentum survives protect + extrapolation

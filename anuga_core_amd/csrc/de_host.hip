@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -56,10 +57,16 @@ struct anuga_domain {
     signed char *bkind = nullptr;
     double *bval = nullptr;
     double *part = nullptr;   // [3][nblocks]: dt, bflux, mass
-    double *dscal = nullptr;  // [8] device scalars
-    int *derr = nullptr;
+    DeControl *ctl = nullptr; // device step control
     unsigned long long *dneg = nullptr;
-    double *hpin = nullptr;   // pinned host mirror of dscal + counters
+    DeControl *hctl = nullptr;          // pinned host mirror of *ctl
+    unsigned long long *hneg = nullptr; // pinned host mirror of *dneg
+    double *hist = nullptr;             // device timestep history of a step run
+    int64_t hist_cap = 0;
+    bool done_on_device = false;        // a step run may have left ctl->done set
+    hipGraph_t graph = nullptr;         // one captured device-controlled step
+    hipGraphExec_t graph_exec = nullptr;
+    bool use_graph = true;
 
     int substate = 0;         // 0: at a step boundary (state in A); 1: rk2 substep state in B
     bool boundary_set = false;
@@ -283,7 +290,7 @@ static int launch_extrapolate(anuga_domain *d, double *const *S)
     hipLaunchKernelGGL(de_extrapolate, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, dev_mesh(d),
                        dev_params(d->P), DeState{S[0], S[1], S[2]}, d->z,
                        DeEdges{d->E[0], d->E[1], d->E[2], d->E[3]}, d->flags,
-                       d->part + 2 * (size_t)d->nblocks, d->derr);
+                       d->part + 2 * (size_t)d->nblocks, d->ctl);
     timer_end(d, T_EXTRAP);
     return ANUGA_OK;
 }
@@ -294,8 +301,8 @@ static void launch_materialize(anuga_domain *d, double *const *S)
                        S[1], S[2], d->z, d->flags);
 }
 
-// substep 0: explicit updates; substep 1: advance S with dt into the backup `bk`
-static void launch_flux(anuga_domain *d, int substep, double *const *S, double *const *bk, double dt)
+// substep 0: explicit updates; substep 1: advance S with ctl->dt into the backup `bk`
+static void launch_flux(anuga_domain *d, int substep, double *const *S, double *const *bk)
 {
     DeBoundary bc{d->bkind, d->bval, d->M};
     DeEdges e{d->E[0], d->E[1], d->E[2], d->E[3]};
@@ -303,39 +310,40 @@ static void launch_flux(anuga_domain *d, int substep, double *const *S, double *
         timer_begin(d, T_FLUX0);
         hipLaunchKernelGGL(de_flux<0>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, dev_mesh(d),
                            dev_params(d->P), DeState{S[0], S[1], S[2]}, d->z, d->eta, e, bc,
-                           d->flags, d->ex[0], d->ex[1], d->ex[2], nullptr, nullptr, nullptr, 0.0,
-                           d->part, d->part + d->nblocks, d->derr, d->dneg);
+                           d->flags, d->ex[0], d->ex[1], d->ex[2], nullptr, nullptr, nullptr,
+                           d->ctl, d->part, d->part + d->nblocks, d->dneg);
         timer_end(d, T_FLUX0);
     } else {
         timer_begin(d, T_FLUX1);
         hipLaunchKernelGGL(de_flux<1>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, dev_mesh(d),
                            dev_params(d->P), DeState{S[0], S[1], S[2]}, d->z, d->eta, e, bc,
-                           d->flags, nullptr, nullptr, nullptr, bk[0], bk[1], bk[2], dt, d->part,
-                           d->part + d->nblocks, d->derr, d->dneg);
+                           d->flags, nullptr, nullptr, nullptr, bk[0], bk[1], bk[2], d->ctl,
+                           d->part, d->part + d->nblocks, d->dneg);
         timer_end(d, T_FLUX1);
     }
 }
 
-static void launch_update(anuga_domain *d, double *const *S, double *const *out, double dt)
+static void launch_update(anuga_domain *d, double *const *S, double *const *out)
 {
     timer_begin(d, T_UPDATE);
     hipLaunchKernelGGL(de_update, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N,
                        dev_params(d->P), DeState{S[0], S[1], S[2]}, d->z, d->eta, d->flags,
-                       d->ex[0], d->ex[1], d->ex[2], dt, out[0], out[1], out[2], d->derr, d->dneg);
+                       d->ex[0], d->ex[1], d->ex[2], d->ctl, out[0], out[1], out[2], d->dneg);
     timer_end(d, T_UPDATE);
 }
 
-// out = dscal (slots 0..2) or dscal + 3 (slots 3..5)
-static void launch_reduce(anuga_domain *d, const double *pdt, const double *pbf, const double *pmass,
-                          int slot = 0)
+static void launch_reduce(anuga_domain *d, int mode)
 {
-    hipLaunchKernelGGL(de_reduce, dim3(1), dim3(1024), 0, d->stream, d->nblocks, pdt, pbf, pmass,
-                       d->dscal + slot);
+    const double *pdt = mode == 0 ? d->part : nullptr;
+    const double *pbf = mode <= 1 ? d->part + d->nblocks : nullptr;
+    const double *pm = (mode == 0 || mode >= 2) ? d->part + 2 * (size_t)d->nblocks : nullptr;
+    hipLaunchKernelGGL(de_reduce, dim3(1), dim3(1024), 0, d->stream, d->nblocks, pdt, pbf, pm,
+                       d->ctl, mode);
 }
 
-static void launch_integrate(anuga_domain *d, double dt, int rk2)
+static void launch_single(anuga_domain *d, void (*k)(DeControl *))
 {
-    hipLaunchKernelGGL(de_integrate, dim3(1), dim3(64), 0, d->stream, d->dscal, dt, rk2);
+    hipLaunchKernelGGL(k, dim3(1), dim3(64), 0, d->stream, d->ctl);
 }
 
 static void launch_volume(anuga_domain *d, double *const *S)
@@ -344,23 +352,61 @@ static void launch_volume(anuga_domain *d, double *const *S)
                        d->area, d->part + 2 * (size_t)d->nblocks);
 }
 
-// read the device scalars and counters; raise a pending device error
+// leave device-controlled mode before host-controlled launches
+static void ensure_live(anuga_domain *d)
+{
+    if (d->done_on_device) {
+        launch_single(d, de_clear_done);
+        d->done_on_device = false;
+    }
+}
+
+// One device-controlled step: both substeps, the timestep on the device and
+// the end-of-step bookkeeping. Every kernel returns at once when ctl->done.
+static void enqueue_device_step(anuga_domain *d)
+{
+    const bool rk2 = d->P.timestep_fluxcalls == 2;
+    launch_extrapolate(d, d->A);
+    launch_flux(d, 0, d->A, nullptr);
+    launch_reduce(d, 0);
+    launch_single(d, de_timestep);
+    launch_update(d, d->A, rk2 ? d->B : d->A);
+    if (rk2) {
+        launch_extrapolate(d, d->B);
+        launch_flux(d, 1, d->B, d->A);
+        launch_reduce(d, 1);
+    }
+    hipLaunchKernelGGL(de_finish_step, dim3(1), dim3(64), 0, d->stream, d->ctl, d->hist);
+}
+
+static void drop_graph(anuga_domain *d)
+{
+    if (d->graph_exec) hipGraphExecDestroy(d->graph_exec);
+    if (d->graph) hipGraphDestroy(d->graph);
+    d->graph_exec = nullptr;
+    d->graph = nullptr;
+}
+
+// read the control block and counters; raise a pending device error
 static int sync_scalars(anuga_domain *d)
 {
-    HIPCHK(d, hipMemcpyAsync(d->hpin, d->dscal, 8 * sizeof(double), hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(d, hipMemcpyAsync(d->hpin + 8, d->derr, sizeof(int), hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(d, hipMemcpyAsync(d->hpin + 9, d->dneg, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+    HIPCHK(d, hipMemcpyAsync(d->hctl, d->ctl, sizeof(DeControl), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(d->hneg, d->dneg, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                              d->stream));
     HIPCHK(d, hipStreamSynchronize(d->stream));
-    int e = 0;
-    std::memcpy(&e, d->hpin + 8, sizeof(int));
-    if (e & 2) {
-        hipMemsetAsync(d->derr, 0, sizeof(int), d->stream);
-        return fail(d, ANUGA_E_SEMI_IMPLICIT, "update: division by zero in semi implicit update");
-    }
-    if (e & 1) {
-        hipMemsetAsync(d->derr, 0, sizeof(int), d->stream);
-        return fail(d, ANUGA_E_NO_NEIGHBOUR, "Internal neighbour not found");
+    const int e = d->hctl->status;
+    if (e) {
+        hipMemsetAsync(&d->ctl->status, 0, sizeof(int), d->stream);
+        hipStreamSynchronize(d->stream);
+        d->hctl->status = 0;
+        if (e & DE_ERR_SEMI_IMPLICIT)
+            return fail(d, ANUGA_E_SEMI_IMPLICIT, "update: division by zero in semi implicit update");
+        if (e & DE_ERR_NO_NEIGHBOUR) return fail(d, ANUGA_E_NO_NEIGHBOUR, "Internal neighbour not found");
+        if (e & DE_ERR_SMALL_TIMESTEP)
+            return fail(d, ANUGA_E_TIMESTEP, "WARNING: Too small timestep reached even after "
+                                             "max_smallsteps steps of 1 order scheme");
+        if (e & DE_ERR_OVERSHOOT)
+            return fail(d, ANUGA_E_TIMESTEP, "WARNING (domain.py): time overshot finaltime. ");
     }
     return ANUGA_OK;
 }
@@ -474,10 +520,10 @@ int anuga_create(const anuga_mesh *mesh, const anuga_params *params, anuga_domai
     DCHK(dalloc(&d->bkind, M));
     DCHK(dalloc(&d->bval, 3 * (size_t)M));
     DCHK(dalloc(&d->part, 3 * (size_t)d->nblocks));
-    DCHK(dalloc(&d->dscal, 8));
-    DCHK(dalloc(&d->derr, 1));
+    DCHK(dalloc(&d->ctl, 1));
     DCHK(dalloc(&d->dneg, 1));
-    DCHK(hipHostMalloc((void **)&d->hpin, 16 * sizeof(double), hipHostMallocDefault));
+    DCHK(hipHostMalloc((void **)&d->hctl, sizeof(DeControl), hipHostMallocDefault));
+    DCHK(hipHostMalloc((void **)&d->hneg, sizeof(unsigned long long), hipHostMallocDefault));
 
     {
         std::vector<int> hn(n3);
@@ -525,9 +571,14 @@ int anuga_create(const anuga_mesh *mesh, const anuga_params *params, anuga_domai
     DCHK(hipMemset(d->flags, DE_F_KEEP, N));
     DCHK(hipMemset(d->bkind, 0, std::max(M, 1)));
     DCHK(hipMemset(d->bval, 0, 3 * (size_t)std::max(M, 1) * sizeof(double)));
-    DCHK(hipMemset(d->dscal, 0, 8 * sizeof(double)));
-    DCHK(hipMemset(d->derr, 0, sizeof(int)));
+    DCHK(hipMemset(d->ctl, 0, sizeof(DeControl)));
     DCHK(hipMemset(d->dneg, 0, sizeof(unsigned long long)));
+    std::memset(d->hctl, 0, sizeof(DeControl));
+    *d->hneg = 0;
+    {
+        const char *g = std::getenv("ANUGA_HIP_GRAPH");
+        d->use_graph = !(g && g[0] == '0');
+    }
     DCHK(hipDeviceSynchronize());
 #undef DCHK
     *out = d;
@@ -538,14 +589,15 @@ void anuga_destroy(anuga_domain *d)
 {
     if (!d) return;
     if (d->stream) hipStreamSynchronize(d->stream);
+    drop_graph(d);
     void *ptrs[] = {d->nbr, d->nx, d->ny, d->len, d->dxv, d->dyv, d->cx, d->cy, d->area,
                     d->radius, d->z, d->eta, d->A[0], d->A[1], d->A[2], d->B[0], d->B[1],
                     d->B[2], d->E[0], d->E[1], d->E[2], d->E[3], d->flags, d->ex[0],
-                    d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->dscal, d->derr,
-                    d->dneg};
+                    d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->ctl, d->dneg, d->hist};
     for (void *p : ptrs)
         if (p) hipFree(p);
-    if (d->hpin) hipHostFree(d->hpin);
+    if (d->hctl) hipHostFree(d->hctl);
+    if (d->hneg) hipHostFree(d->hneg);
     for (int t = 0; t < T_COUNT; t++) {
         for (auto e : d->timers[t].start) hipEventDestroy(e);
         for (auto e : d->timers[t].stop) hipEventDestroy(e);
@@ -563,6 +615,7 @@ int anuga_set_params(anuga_domain *d, const anuga_params *params)
         return fail(d, ANUGA_E_ARG, "precision and reorder are fixed at creation");
     if (params->timestep_fluxcalls != d->P.timestep_fluxcalls && d->substate != 0)
         return fail(d, ANUGA_E_STATE, "cannot change the timestepping method inside a step");
+    if (std::memcmp(params, &d->P, sizeof(anuga_params)) != 0) drop_graph(d);
     d->P = *params;
     return ANUGA_OK;
 }
@@ -666,16 +719,17 @@ int anuga_set_boundary(anuga_domain *d, const int32_t *kind, const double *diric
 int anuga_distribute_to_vertices_and_edges(anuga_domain *d, double *mass_error)
 {
     if (!d) return ANUGA_E_ARG;
+    ensure_live(d);
     double **S = d->substate ? d->B : d->A;
     int rc = launch_extrapolate(d, S);
     if (rc) return rc;
     launch_materialize(d, S);
-    launch_reduce(d, nullptr, nullptr, d->part + 2 * (size_t)d->nblocks);
+    launch_reduce(d, 2);
     rc = launch_check(d);
     if (rc) return rc;
     rc = sync_scalars(d);
     if (rc) return rc;
-    if (mass_error) *mass_error = d->hpin[2];
+    if (mass_error) *mass_error = d->hctl->mass;
     return ANUGA_OK;
 }
 
@@ -684,16 +738,27 @@ int anuga_compute_fluxes(anuga_domain *d, double *flux_timestep)
     if (!d) return ANUGA_E_ARG;
     if (d->substate != 0) return fail(d, ANUGA_E_STATE, "compute_fluxes called inside an rk2 step");
     if (!d->boundary_set && d->M > 0) return fail(d, ANUGA_E_STATE, "boundary conditions not set");
+    ensure_live(d);
     int rc = launch_extrapolate(d, d->A);
     if (rc) return rc;
-    launch_flux(d, 0, d->A, nullptr, 0.0);
-    launch_reduce(d, d->part, d->part + d->nblocks, d->part + 2 * (size_t)d->nblocks);
+    launch_flux(d, 0, d->A, nullptr);
+    launch_reduce(d, 0);
     rc = launch_check(d);
     if (rc) return rc;
     rc = sync_scalars(d);
     if (rc) return rc;
     if (d->timing) timer_flush(d);
-    if (flux_timestep) *flux_timestep = d->hpin[0];
+    if (flux_timestep) *flux_timestep = d->hctl->flux_dt;
+    return ANUGA_OK;
+}
+
+static int read_negative(anuga_domain *d, int64_t *num_negative)
+{
+    int rc = sync_scalars(d);
+    if (rc) return rc;
+    const unsigned long long n = *d->hneg;
+    *num_negative = (int64_t)(n - d->neg_seen);
+    d->neg_seen = n;
     return ANUGA_OK;
 }
 
@@ -701,44 +766,99 @@ int anuga_update_conserved_quantities(anuga_domain *d, double dt, int64_t *num_n
 {
     if (!d) return ANUGA_E_ARG;
     if (d->substate != 0) return fail(d, ANUGA_E_STATE, "update called twice in a step");
-    double **out = d->P.timestep_fluxcalls == 2 ? d->B : d->A;
-    launch_update(d, d->A, out, dt);
-    if (d->P.timestep_fluxcalls == 1) launch_integrate(d, dt, 0);
+    ensure_live(d);
+    const bool rk2 = d->P.timestep_fluxcalls == 2;
+    hipLaunchKernelGGL(de_set_dt, dim3(1), dim3(64), 0, d->stream, d->ctl, dt);
+    launch_update(d, d->A, rk2 ? d->B : d->A);
+    if (!rk2) hipLaunchKernelGGL(de_integrate, dim3(1), dim3(64), 0, d->stream, d->ctl, 0);
     int rc = launch_check(d);
     if (rc) return rc;
-    if (d->P.timestep_fluxcalls == 2) d->substate = 1;
-    if (num_negative) {
-        rc = sync_scalars(d);
-        if (rc) return rc;
-        unsigned long long n;
-        std::memcpy(&n, d->hpin + 9, sizeof(n));
-        *num_negative = (int64_t)(n - d->neg_seen);
-        d->neg_seen = n;
-    }
-    return ANUGA_OK;
+    if (rk2) d->substate = 1;
+    return num_negative ? read_negative(d, num_negative) : ANUGA_OK;
 }
 
 int anuga_rk2_second_substep(anuga_domain *d, double dt, int64_t *num_negative)
 {
     if (!d) return ANUGA_E_ARG;
     if (d->substate != 1) return fail(d, ANUGA_E_STATE, "second substep without a first");
+    hipLaunchKernelGGL(de_set_dt, dim3(1), dim3(64), 0, d->stream, d->ctl, dt);
     int rc = launch_extrapolate(d, d->B);
     if (rc) return rc;
-    launch_flux(d, 1, d->B, d->A, dt);
-    launch_reduce(d, nullptr, d->part + d->nblocks, nullptr, 3);
-    launch_integrate(d, dt, 1);
+    launch_flux(d, 1, d->B, d->A);
+    launch_reduce(d, 1);
+    hipLaunchKernelGGL(de_integrate, dim3(1), dim3(64), 0, d->stream, d->ctl, 1);
     rc = launch_check(d);
     if (rc) return rc;
     d->substate = 0;
-    if (num_negative) {
-        rc = sync_scalars(d);
-        if (rc) return rc;
-        unsigned long long n;
-        std::memcpy(&n, d->hpin + 9, sizeof(n));
-        *num_negative = (int64_t)(n - d->neg_seen);
-        d->neg_seen = n;
+    return num_negative ? read_negative(d, num_negative) : ANUGA_OK;
+}
+
+int anuga_evolve_steps(anuga_domain *d, int64_t max_steps, const anuga_time_control *tc,
+                       anuga_step_state *st, double *dt_history)
+{
+    if (!d || !tc || !st || max_steps < 0) return ANUGA_E_ARG;
+    if (d->substate != 0) return fail(d, ANUGA_E_STATE, "evolve_steps called inside an rk2 step");
+    if (!d->boundary_set && d->M > 0) return fail(d, ANUGA_E_STATE, "boundary conditions not set");
+    if (dt_history && max_steps > d->hist_cap) {
+        if (d->hist) hipFree(d->hist);
+        d->hist = nullptr;
+        d->hist_cap = 0;
+        drop_graph(d);
+        HIPCHK(d, dalloc(&d->hist, (size_t)max_steps));
+        d->hist_cap = max_steps;
     }
-    return ANUGA_OK;
+    DeBatchSetup b;
+    b.t = st->relative_time;
+    b.rec_min = st->recorded_min_timestep;
+    b.rec_max = st->recorded_max_timestep;
+    b.cfl = tc->CFL;
+    b.max_dt = tc->evolve_max_timestep;
+    b.min_dt = tc->evolve_min_timestep;
+    b.finaltime = tc->relative_finaltime;
+    b.yieldtime = tc->relative_yieldtime;
+    b.fixed_flux_dt = tc->fixed_flux_timestep;
+    b.eps = tc->epsilon;
+    b.smallsteps = (int)st->smallsteps;
+    b.order = (int)st->order;
+    b.default_order = (int)tc->default_order;
+    b.max_smallsteps = (int)tc->max_smallsteps;
+    b.hist_cap = dt_history ? (int)std::min<int64_t>(max_steps, d->hist_cap) : 0;
+    b.rk2 = d->P.timestep_fluxcalls == 2;
+    hipLaunchKernelGGL(de_setup_batch, dim3(1), dim3(64), 0, d->stream, d->ctl, b);
+    d->done_on_device = true;
+
+    const bool graph = d->use_graph && !d->timing;
+    if (graph && !d->graph_exec && max_steps > 0) {
+        HIPCHK(d, hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_device_step(d);
+        hipGraph_t g = nullptr;
+        HIPCHK(d, hipStreamEndCapture(d->stream, &g));
+        d->graph = g;
+        HIPCHK(d, hipGraphInstantiate(&d->graph_exec, d->graph, nullptr, nullptr, 0));
+    }
+    for (int64_t i = 0; i < max_steps; i++) {
+        if (graph) HIPCHK(d, hipGraphLaunch(d->graph_exec, d->stream));
+        else enqueue_device_step(d);
+    }
+    int rc = launch_check(d);
+    if (rc) return rc;
+    rc = sync_scalars(d);
+    const DeControl &c = *d->hctl;
+    st->relative_time = c.t;
+    st->timestep = c.dt;
+    st->flux_timestep = c.flux_dt;
+    st->recorded_min_timestep = c.rec_min;
+    st->recorded_max_timestep = c.rec_max;
+    st->smallsteps = c.smallsteps;
+    st->order = c.order;
+    st->steps_taken = c.nsteps;
+    st->reached = c.done;
+    if (dt_history && c.nsteps > 0) {
+        HIPCHK(d, hipMemcpy(dt_history, d->hist,
+                            (size_t)std::min<int64_t>(c.nsteps, max_steps) * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    }
+    return rc;
 }
 
 int anuga_get_boundary_flux_sums(anuga_domain *d, double *sums)
@@ -746,8 +866,8 @@ int anuga_get_boundary_flux_sums(anuga_domain *d, double *sums)
     if (!d || !sums) return ANUGA_E_ARG;
     int rc = sync_scalars(d);
     if (rc) return rc;
-    sums[0] = d->hpin[1];
-    sums[1] = d->P.timestep_fluxcalls == 2 ? d->hpin[4] : 0.0;
+    sums[0] = d->hctl->bflux0;
+    sums[1] = d->P.timestep_fluxcalls == 2 ? d->hctl->bflux1 : 0.0;
     return ANUGA_OK;
 }
 
@@ -756,16 +876,14 @@ int anuga_get_boundary_flux_integral(anuga_domain *d, double *integral)
     if (!d || !integral) return ANUGA_E_ARG;
     int rc = sync_scalars(d);
     if (rc) return rc;
-    *integral = d->hpin[5];
+    *integral = d->hctl->integral;
     return ANUGA_OK;
 }
 
 int anuga_get_negative_cells(anuga_domain *d, int64_t *total)
 {
     if (!d || !total) return ANUGA_E_ARG;
-    unsigned long long n;
-    std::memcpy(&n, d->hpin + 9, sizeof(n));
-    *total = (int64_t)n;
+    *total = (int64_t)*d->hneg;
     return ANUGA_OK;
 }
 
@@ -774,12 +892,12 @@ int anuga_get_water_volume(anuga_domain *d, double *volume)
     if (!d || !volume) return ANUGA_E_ARG;
     double **S = d->substate ? d->B : d->A;
     launch_volume(d, S);
-    launch_reduce(d, nullptr, nullptr, d->part + 2 * (size_t)d->nblocks);
+    launch_reduce(d, 3);
     int rc = launch_check(d);
     if (rc) return rc;
     rc = sync_scalars(d);
     if (rc) return rc;
-    *volume = d->hpin[2];
+    *volume = d->hctl->volume;
     return ANUGA_OK;
 }
 

@@ -35,31 +35,30 @@ struct DeState {
 __global__ void __launch_bounds__(DE_BLOCK)
 de_extrapolate(DeMesh m, DeParams p, DeState s, const double *__restrict__ z, DeEdges e,
                unsigned char *__restrict__ flags, double *__restrict__ part_mass,
-               int *__restrict__ err)
+               DeControl *ctl)
 {
     __shared__ double red[DE_BLOCK / 64];
+    if (ctl->done) return;
+    int *err = &ctl->status;
     const int N = m.N;
     const int k = xcd_block() * DE_BLOCK + threadIdx.x;
     double mass = 0.0;
 
     if (k < N) {
         const double mah = p.mah;
-        bool keep, clamp;
-        const DeCell c = de_prepare(m, s.w, s.uh, s.vh, z, k, mah, keep, clamp);
-        if (clamp) mass = (z[k] - s.w[k]) * m.area[k];
-        flags[k] = (clamp ? DE_F_CLAMP : 0u) | (keep ? DE_F_KEEP : 0u);
-
-        int sur[3];
+        // Loads are issued in three unconditional rounds (own cell, the three
+        // surrogate neighbours with their connectivity, the neighbours'
+        // neighbours' heights) so that each round's gathers overlap.
+        // round 1: own cell
+        int v[3], sur[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) v[i] = m.nbr[i * N + k];
+        const double wk = s.w[k], zk = z[k], uhk = s.uh[k], vhk = s.vh[k];
         int nbound = 0;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const int v = m.nbr[i * N + k];
-            if (v >= 0) {
-                sur[i] = v >> 2;
-            } else {
-                sur[i] = k;
-                nbound++;
-            }
+            sur[i] = v[i] >= 0 ? (v[i] >> 2) : k;
+            nbound += v[i] < 0;
         }
         double dxv[3], dyv[3];
 #pragma unroll
@@ -67,25 +66,77 @@ de_extrapolate(DeMesh m, DeParams p, DeState s, const double *__restrict__ z, De
             dxv[i] = m.dxv[i * N + k];
             dyv[i] = m.dyv[i * N + k];
         }
+        // round 2: surrogate neighbours
+        double ws[3], zs[3], uhs[3], vhs[3], xs[3], ys[3];
+        int vs[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int j = sur[i];
+            ws[i] = s.w[j];
+            zs[i] = z[j];
+            uhs[i] = s.uh[j];
+            vhs[i] = s.vh[j];
+            xs[i] = m.cx[j];
+            ys[i] = m.cy[j];
+#pragma unroll
+            for (int q = 0; q < 3; q++) vs[i][q] = m.nbr[q * N + j];
+        }
+        // round 3: heights of the neighbours' neighbours
+        double h2[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int t = vs[i][q] >= 0 ? (vs[i][q] >> 2) : sur[i];
+                h2[i][q] = de_height(s.w[t], z[t]);
+            }
+
+        // protect + velocity of triangle k (sw_domain_orig.c:1245-1276, :2014-2064)
+        const bool clamp = de_clamped(wk, zk, mah);
+        DeCell c;
+        c.w = clamp ? zk : wk;
+        c.h = de_height(c.w, zk);
+        double hs[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) hs[i] = de_height(ws[i], zs[i]);
+        bool dry_k = true;
+#pragma unroll
+        for (int i = 0; i < 3; i++) dry_k = dry_k && (v[i] < 0 || hs[i] < mah);
+        const bool keep = (c.h > mah) && !dry_k;
+        c.u = 0.0;
+        c.v = 0.0;
+        if (keep) {
+            const double inv = 1.0 / c.h;
+            c.u = uhk * inv;
+            c.v = vhk * inv;
+        }
+        if (clamp) mass = (zk - wk) * m.area[k];
+        flags[k] = (clamp ? DE_F_CLAMP : 0u) | (keep ? DE_F_KEEP : 0u);
+
+        // the same for the surrogate neighbours
+        DeCell nb[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const bool cl = de_clamped(ws[i], zs[i], mah);
+            nb[i].w = cl ? zs[i] : ws[i];
+            nb[i].h = hs[i];
+            bool dry = true;
+#pragma unroll
+            for (int q = 0; q < 3; q++) dry = dry && (vs[i][q] < 0 || h2[i][q] < mah);
+            nb[i].u = 0.0;
+            nb[i].v = 0.0;
+            if (hs[i] > mah && !dry) {
+                const double inv = 1.0 / hs[i];
+                nb[i].u = uhs[i] * inv;
+                nb[i].v = vhs[i] * inv;
+            }
+        }
 
         double ow[3], oh[3], ou[3], ov[3];
         bool constant = (nbound == 3);
         if (nbound <= 1) {
-            DeCell nb[3];
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                if (sur[i] == k) {
-                    nb[i] = c;
-                } else {
-                    bool kp, cl;
-                    nb[i] = de_prepare(m, s.w, s.uh, s.vh, z, sur[i], mah, kp, cl);
-                }
-            }
-            const double x0 = m.cx[sur[0]], y0 = m.cy[sur[0]];
-            const double x1 = m.cx[sur[1]], y1 = m.cy[sur[1]];
-            const double x2 = m.cx[sur[2]], y2 = m.cy[sur[2]];
-            const double dx1 = x1 - x0, dx2 = x2 - x0;
-            const double dy1 = y1 - y0, dy2 = y2 - y0;
+            const double dx1 = xs[1] - xs[0], dx2 = xs[2] - xs[0];
+            const double dy1 = ys[1] - ys[0], dy2 = ys[2] - ys[0];
             const double area2 = dy2 * dx1 - dy1 * dx2;
             if (area2 <= 0.0) {
                 constant = true;
@@ -93,9 +144,9 @@ de_extrapolate(DeMesh m, DeParams p, DeState s, const double *__restrict__ z, De
                 const double a_tmp = 0.3, b_tmp = 0.1;
                 const double c_tmp = 1.0 / (a_tmp - b_tmp);
                 const double d_tmp = 1.0 - (c_tmp * a_tmp);
-                const double hc = c.h, h0 = nb[0].h, h1 = nb[1].h, h2 = nb[2].h;
-                const double hmin = fmin(fmin(h0, fmin(h1, h2)), hc);
-                const double hmax = fmax(fmax(h0, fmax(h1, h2)), hc);
+                const double hc = c.h, h0 = nb[0].h, h1 = nb[1].h, h2_ = nb[2].h;
+                const double hmin = fmin(fmin(h0, fmin(h1, h2_)), hc);
+                const double hmax = fmax(fmax(h0, fmax(h1, h2_)), hc);
                 double hfactor = fmax(0.0, fmin(c_tmp * fmax(hmin, 0.0) / fmax(hc, 1.0e-06) + d_tmp,
                                                 fmin(c_tmp * fmax(hc, 0.0) / fmax(hmax, 1.0e-06) + d_tmp,
                                                      1.0)));
@@ -128,26 +179,22 @@ de_extrapolate(DeMesh m, DeParams p, DeState s, const double *__restrict__ z, De
                 }
             }
         } else if (nbound == 2) {
-            int k1 = -1;
-#pragma unroll
-            for (int i = 2; i >= 0; i--)
-                if (sur[i] != k) k1 = sur[i];
-            if (k1 < 0) {
-                atomicOr(err, 1);
+            // the first edge whose surrogate is not k itself (sw_domain_orig.c:2421-2439)
+            const int i1 = (v[0] >= 0) ? 0 : ((v[1] >= 0) ? 1 : 2);
+            if (v[i1] < 0) {
+                atomicOr(err, DE_ERR_NO_NEIGHBOUR);
                 constant = true;
             } else {
-                bool kp, cl;
-                const DeCell n1 = de_prepare(m, s.w, s.uh, s.vh, z, k1, mah, kp, cl);
-                const double dx1 = m.cx[k1] - m.cx[k];
-                const double dy1 = m.cy[k1] - m.cy[k];
+                const double dx1 = xs[i1] - m.cx[k];
+                const double dy1 = ys[i1] - m.cy[k];
                 const double area2 = dx1 * dx1 + dy1 * dy1;
                 double dx2 = 1.0 / area2;
                 const double dy2 = dx2 * dy1;
                 dx2 *= dx1;
-                de_line_edges(c.w, n1.w, dx2, dy2, dxv, dyv, p.bw, ow);
-                de_line_edges(c.h, n1.h, dx2, dy2, dxv, dyv, p.bw, oh);
-                de_line_edges(c.u, n1.u, dx2, dy2, dxv, dyv, p.bw, ou);
-                de_line_edges(c.v, n1.v, dx2, dy2, dxv, dyv, p.bw, ov);
+                de_line_edges(c.w, nb[i1].w, dx2, dy2, dxv, dyv, p.bw, ow);
+                de_line_edges(c.h, nb[i1].h, dx2, dy2, dxv, dyv, p.bw, oh);
+                de_line_edges(c.u, nb[i1].u, dx2, dy2, dxv, dyv, p.bw, ou);
+                de_line_edges(c.v, nb[i1].v, dx2, dy2, dxv, dyv, p.bw, ov);
             }
         }
         if (constant) {
@@ -212,7 +259,7 @@ __device__ __forceinline__ double de_update_one(double q, double ex, double si, 
     q += dt * ex;
     const double den = 1.0 - dt * si;
     if (den <= 0.0) {
-        atomicOr(err, 2);
+        atomicOr(err, DE_ERR_SEMI_IMPLICIT);
         return q;
     }
     return q / den;
@@ -259,12 +306,15 @@ de_flux(DeMesh m, DeParams p, DeState s, const double *__restrict__ z,
         const double *__restrict__ eta, DeEdges ec, DeBoundary bc,
         const unsigned char *__restrict__ flags, double *__restrict__ exw,
         double *__restrict__ exuh, double *__restrict__ exvh, double *__restrict__ bw,
-        double *__restrict__ buh, double *__restrict__ bvh, double dt,
-        double *__restrict__ part_dt, double *__restrict__ part_bf, int *__restrict__ err,
+        double *__restrict__ buh, double *__restrict__ bvh, DeControl *ctl,
+        double *__restrict__ part_dt, double *__restrict__ part_bf,
         unsigned long long *__restrict__ nneg)
 {
     __shared__ double red_dt[DE_BLOCK / 64];
     __shared__ double red_bf[DE_BLOCK / 64];
+    if (ctl->done) return;
+    int *err = &ctl->status;
+    const double dt = SUBSTEP ? ctl->dt : 0.0;
     const int N = m.N;
     const int k = xcd_block() * DE_BLOCK + threadIdx.x;
     double dtmin = 1.0e100, bflux = 0.0;
@@ -384,12 +434,14 @@ __global__ void __launch_bounds__(DE_BLOCK)
 de_update(int N, DeParams p, DeState s, const double *__restrict__ z,
           const double *__restrict__ eta, const unsigned char *__restrict__ flags,
           const double *__restrict__ exw, const double *__restrict__ exuh,
-          const double *__restrict__ exvh, double dt, double *ow, double *ouh, double *ovh,
-          int *__restrict__ err,
+          const double *__restrict__ exvh, DeControl *ctl, double *ow, double *ouh, double *ovh,
           unsigned long long *__restrict__ nneg)
 {
+    if (ctl->done) return;
     const int k = xcd_block() * DE_BLOCK + threadIdx.x;
     if (k >= N) return;
+    const double dt = ctl->dt;
+    int *err = &ctl->status;
     const unsigned f = flags[k];
     const double zc = z[k];
     const double wc = (f & DE_F_CLAMP) ? zc : s.w[k];
@@ -404,13 +456,15 @@ de_update(int N, DeParams p, DeState s, const double *__restrict__ z,
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
-// out[0] = min(part_dt), out[1] = sum(part_bf), out[2] = sum(part_mass)
-// (any pointer may be null); fixed-order, one block of 1024 threads.
+// Fixed-order reduction of the per-block partials into the control block,
+// one block of 1024 threads. mode 0: flux_dt, bflux0, mass (first substep);
+// 1: bflux1 (second substep); 2: mass (distribute); 3: volume.
 __global__ void __launch_bounds__(1024)
 de_reduce(int nparts, const double *__restrict__ part_dt, const double *__restrict__ part_bf,
-          const double *__restrict__ part_mass, double *__restrict__ out)
+          const double *__restrict__ part_mass, DeControl *ctl, int mode)
 {
     __shared__ double s0[16], s1[16], s2[16];
+    if (mode < 2 && ctl->done) return;
     double a = 1.0e100, b = 0.0, c = 0.0;
     for (int i = threadIdx.x; i < nparts; i += 1024) {
         if (part_dt) a = fmin(a, part_dt[i]);
@@ -429,27 +483,119 @@ de_reduce(int nparts, const double *__restrict__ part_dt, const double *__restri
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int i = 1; i < 16; i++) {
-            a = fmin(s0[0], s0[i]);
-            s0[0] = a;
+            s0[0] = fmin(s0[0], s0[i]);
             s1[0] += s1[i];
             s2[0] += s2[i];
         }
-        if (part_dt) out[0] = s0[0];
-        if (part_bf) out[1] = s1[0];
-        if (part_mass) out[2] = s2[0];
+        if (mode == 0) {
+            ctl->flux_dt = s0[0];
+            ctl->bflux0 = s1[0];
+            ctl->mass = s2[0];
+        } else if (mode == 1) {
+            ctl->bflux1 = s1[0];
+        } else if (mode == 2) {
+            ctl->mass = s2[0];
+        } else {
+            ctl->volume = s2[0];
+        }
     }
 }
 
-// boundary_flux_integral_operator.__call__ on the device
-// (anuga/operators/boundary_flux_integral_operator.py:39-49):
-// euler: I += dt * s0;  rk2: I += 0.5 * dt * (s0 + s1).
-// dscal[1] = s0, dscal[4] = s1, dscal[5] = I.
-__global__ void de_integrate(double *dscal, double dt, int rk2)
+// Generic_Domain.update_timestep (generic_domain.py:2349-2415) on the device:
+// CFL and max-timestep limit, recorded extremes, the small-step counter with
+// its first-order switch and its exception (reported as DE_ERR_SMALL_TIMESTEP),
+// and the clamps to the final time and the next yield time.
+__global__ void de_timestep(DeControl *c)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        if (rk2) dscal[5] = dscal[5] + 0.5 * dt * (dscal[1] + dscal[4]);
-        else dscal[5] = dscal[5] + dt * dscal[1];
+    if (threadIdx.x != 0 || blockIdx.x != 0 || c->done) return;
+    const double flux_dt = c->fixed_flux_dt > 0.0 ? c->fixed_flux_dt : c->flux_dt;
+    const double a = c->cfl * flux_dt;
+    double ts = (c->max_dt < a) ? c->max_dt : a;           // min(CFL * flux_dt, max_dt)
+    c->rec_max = (c->rec_max > ts) ? c->rec_max : ts;      // max(timestep, recorded_max)
+    c->rec_min = (c->rec_min < ts) ? c->rec_min : ts;      // min(timestep, recorded_min)
+    if (ts < c->min_dt) {
+        c->smallsteps += 1;
+        if (c->smallsteps > c->max_smallsteps) {
+            c->smallsteps = 0;
+            if (c->order == 1) c->status |= DE_ERR_SMALL_TIMESTEP;
+            else c->order = 1;
+        }
+    } else {
+        c->smallsteps = 0;
+        if (c->order == 1 && c->default_order == 2) c->order = 2;
     }
+    if (c->t + ts > c->finaltime) ts = c->finaltime - c->t;
+    if (c->t + ts > c->yieldtime) ts = c->yieldtime - c->t;
+    c->dt = ts;
+}
+
+// End of a step: boundary flux integral (boundary_flux_integral_operator.py:39-49),
+// relative_time += dt, timestep history, and the yield / final-time test of
+// _evolve_base (generic_domain.py:1855-1895) that turns the next steps into no-ops.
+__global__ void de_finish_step(DeControl *c, double *hist)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0 || c->done) return;
+    const double dt = c->dt;
+    if (c->rk2) c->integral = c->integral + 0.5 * dt * (c->bflux0 + c->bflux1);
+    else c->integral = c->integral + dt * c->bflux0;
+    c->t = c->t + dt;
+    if (hist && c->nsteps < c->hist_cap) hist[c->nsteps] = dt;
+    c->nsteps += 1;
+    if (c->t >= c->finaltime - c->eps) {
+        if (c->t > c->finaltime) c->status |= DE_ERR_OVERSHOOT;
+        c->done = 1;
+    } else if (c->t >= c->yieldtime) {
+        c->done = 2;
+    }
+}
+
+// host-controlled steps: the host computed dt
+__global__ void de_set_dt(DeControl *c, double dt)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->dt = dt;
+}
+
+// host-controlled steps: boundary flux integral after the update
+__global__ void de_integrate(DeControl *c, int rk2)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (rk2) c->integral = c->integral + 0.5 * c->dt * (c->bflux0 + c->bflux1);
+    else c->integral = c->integral + c->dt * c->bflux0;
+}
+
+// time-control parameters and state for a run of device-controlled steps
+struct DeBatchSetup {
+    double t, rec_min, rec_max, cfl, max_dt, min_dt, finaltime, yieldtime, fixed_flux_dt, eps;
+    int smallsteps, order, default_order, max_smallsteps, hist_cap, rk2;
+};
+
+__global__ void de_setup_batch(DeControl *c, DeBatchSetup b)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c->t = b.t;
+    c->rec_min = b.rec_min;
+    c->rec_max = b.rec_max;
+    c->cfl = b.cfl;
+    c->max_dt = b.max_dt;
+    c->min_dt = b.min_dt;
+    c->finaltime = b.finaltime;
+    c->yieldtime = b.yieldtime;
+    c->fixed_flux_dt = b.fixed_flux_dt;
+    c->eps = b.eps;
+    c->smallsteps = b.smallsteps;
+    c->order = b.order;
+    c->default_order = b.default_order;
+    c->max_smallsteps = b.max_smallsteps;
+    c->hist_cap = b.hist_cap;
+    c->rk2 = b.rk2;
+    c->nsteps = 0;
+    c->done = 0;
+}
+
+// leave device-controlled mode: steps are live again
+__global__ void de_clear_done(DeControl *c)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->done = 0;
 }
 
 // per-block partial of sum(area * max(w - z, 0)) over the current state
@@ -474,9 +620,9 @@ de_volume(int N, const double *__restrict__ w, const double *__restrict__ z,
 
 template __global__ void de_flux<0>(DeMesh, DeParams, DeState, const double *, const double *,
                                     DeEdges, DeBoundary, const unsigned char *, double *,
-                                    double *, double *, double *, double *, double *, double,
-                                    double *, double *, int *, unsigned long long *);
+                                    double *, double *, double *, double *, double *,
+                                    DeControl *, double *, double *, unsigned long long *);
 template __global__ void de_flux<1>(DeMesh, DeParams, DeState, const double *, const double *,
                                     DeEdges, DeBoundary, const unsigned char *, double *,
-                                    double *, double *, double *, double *, double *, double,
-                                    double *, double *, int *, unsigned long long *);
+                                    double *, double *, double *, double *, double *,
+                                    DeControl *, double *, double *, unsigned long long *);

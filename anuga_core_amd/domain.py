@@ -125,6 +125,10 @@ class Domain:
         self.recorded_min_timestep = self.recorded_max_timestep = 0.0
         self.dt_history = []
         self.record_dt_history = True
+        # True: update_timestep runs on the device and runs of steps need one
+        # host synchronisation each; False: one library call per substep with
+        # update_timestep on the host (the reference's sequencing, for checks)
+        self.device_timestep = True
         self._distributed = False
         self._last_negative_total = 0
         self.mass_error = 0.0
@@ -497,26 +501,18 @@ class Domain:
         if not skip_initial_step:
             self.distribute_to_vertices_and_edges()
             yield self.get_time()
-        step = self.evolve_one_euler_step if self._timestep_fluxcalls == 1 \
-            else self.evolve_one_rk2_step
         while True:
-            initial = self.relative_time
-            self._distributed = False
-            step(yieldstep, self.relative_finaltime)
-            self.relative_time = initial + self.timestep
-            if self.record_dt_history:
-                self.dt_history.append(self.timestep)
-            self.number_of_steps += 1
-            if self.relative_finaltime is not None and \
-                    self.relative_time >= self.relative_finaltime - EPSILON:
-                if self.relative_time > self.relative_finaltime:
-                    raise Exception("WARNING (domain.py): time overshot finaltime. ")
+            if self.device_timestep:
+                reached = self._device_steps(self._chunk_estimate())
+            else:
+                reached = self._host_step(yieldstep)
+            if reached == 1:
                 self.relative_time = self.relative_finaltime
                 self.distribute_to_vertices_and_edges()
                 self._check_negative_cells()
                 yield self.get_time()
                 break
-            if self.relative_time >= self.relative_yieldtime:
+            if reached == 2:
                 self.distribute_to_vertices_and_edges()
                 self._check_negative_cells()
                 yield self.get_time()
@@ -525,28 +521,93 @@ class Domain:
                 self.recorded_max_timestep = self.evolve_min_timestep
                 self.number_of_steps = 0
 
-    def evolve_steps(self, nsteps, yieldstep=None, finaltime=None):
-        """Run exactly `nsteps` steps with no intermediate yields (a convenience
-        for fixed-step-count runs; dt still follows update_timestep)."""
+    def _host_step(self, yieldstep):
+        """One step with update_timestep on the host; returns 1 / 2 / 0 as the
+        step ends at the final time / a yield time / neither."""
+        initial = self.relative_time
+        self._distributed = False
+        if self._timestep_fluxcalls == 1:
+            self.evolve_one_euler_step(yieldstep, self.relative_finaltime)
+        else:
+            self.evolve_one_rk2_step(yieldstep, self.relative_finaltime)
+        self.relative_time = initial + self.timestep
+        if self.record_dt_history:
+            self.dt_history.append(self.timestep)
+        self.number_of_steps += 1
+        if self.relative_finaltime is not None and \
+                self.relative_time >= self.relative_finaltime - EPSILON:
+            if self.relative_time > self.relative_finaltime:
+                raise Exception("WARNING (domain.py): time overshot finaltime. ")
+            return 1
+        if self.relative_time >= self.relative_yieldtime:
+            return 2
+        return 0
+
+    def _chunk_estimate(self):
+        """Steps to the next yield or final time from the last timestep, so that
+        few launches of a run become no-ops."""
+        target = self.relative_yieldtime
+        if self.relative_finaltime is not None:
+            target = min(target, self.relative_finaltime)
+        if self.timestep > 0.0:
+            est = (target - self.relative_time) / self.timestep
+            return int(max(1, min(256, est + 2)))
+        return 8
+
+    def _device_steps(self, nsteps):
+        """Up to nsteps steps with update_timestep on the device; returns the
+        `reached` code of anuga_evolve_steps."""
+        inf = float("inf")
+        tc = _native.TimeControl(self.CFL, self.evolve_max_timestep, self.evolve_min_timestep,
+                                 inf if self.relative_finaltime is None else self.relative_finaltime,
+                                 self.relative_yieldtime,
+                                 -1.0 if self.fixed_flux_timestep is None
+                                 else self.fixed_flux_timestep,
+                                 EPSILON, self.max_smallsteps, self.default_order)
+        st = _native.StepState(self.relative_time, self.timestep, self.flux_timestep,
+                               self.recorded_min_timestep, self.recorded_max_timestep,
+                               self.smallsteps, self._order_, 0, 0)
+        hist = np.empty(max(int(nsteps), 1)) if self.record_dt_history else None
+        self._distributed = False
+        self._call("anuga_evolve_steps", int(nsteps), C.byref(tc), C.byref(st),
+                   None if hist is None else _dptr(hist))
+        n = st.steps_taken
+        self.relative_time = st.relative_time
+        self.timestep = st.timestep
+        self.flux_timestep = st.flux_timestep
+        self.recorded_min_timestep = st.recorded_min_timestep
+        self.recorded_max_timestep = st.recorded_max_timestep
+        self.smallsteps = st.smallsteps
+        self._order_ = st.order
+        self.number_of_steps += n
+        if hist is not None:
+            self.dt_history.extend(hist[:n].tolist())
+        self._check_negative_cells()
+        return st.reached
+
+    def evolve_steps(self, nsteps, finaltime=None):
+        """Run exactly `nsteps` steps with no yields (dt still follows
+        update_timestep; only finaltime clamps it)."""
         self._sync_params()
         if not self.evolved_called:
             self.distribute_to_vertices_and_edges()
             self.evolved_called = True
         self._order_ = self.default_order
         self.relative_finaltime = None if finaltime is None else float(finaltime) - self.starttime
-        self.relative_yieldtime = self.relative_time + (1.0e300 if yieldstep is None
-                                                        else float(yieldstep))
+        self.relative_yieldtime = float("inf")
         self.recorded_min_timestep = self.evolve_max_timestep
         self.recorded_max_timestep = self.evolve_min_timestep
-        step = self.evolve_one_euler_step if self._timestep_fluxcalls == 1 \
-            else self.evolve_one_rk2_step
-        for _ in range(int(nsteps)):
-            initial = self.relative_time
-            step(1.0e300, self.relative_finaltime)
-            self.relative_time = initial + self.timestep
-            if self.record_dt_history:
-                self.dt_history.append(self.timestep)
-            self.number_of_steps += 1
+        done = 0
+        while done < nsteps:
+            if self.device_timestep:
+                before = self.number_of_steps
+                reached = self._device_steps(nsteps - done)
+                done += self.number_of_steps - before
+            else:
+                reached = self._host_step(float("inf"))
+                done += 1
+            if reached:
+                break
 
     # -- diagnostics --------------------------------------------------------------------
     def get_water_volume(self):

@@ -111,8 +111,13 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--m", type=int, default=2000)
     ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--timing-steps", type=int, default=50,
+                    help="steps of the eager pass that times each kernel with HIP events")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-reorder", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run 3 device copies of 1 GiB (a known-traffic kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -121,6 +126,15 @@ def main():
         sys.exit(2)
 
     from anuga_core_amd.domain import Domain
+
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     t0 = time.perf_counter()
     mesh, q, bc = build_c1(args.m)
@@ -141,11 +155,16 @@ def main():
     v0 = d.get_water_volume()
     d.evolve_steps(args.warmup)
     d.synchronize()
-    d.reset_kernel_timing(True)
+    # timed region: device-controlled steps, one HIP graph per step
     t1 = time.perf_counter()
     d.evolve_steps(args.steps)
     d.synchronize()
     elapsed = time.perf_counter() - t1
+    # per-kernel durations: HIP events around every launch on the library's
+    # stream, over an eager (non-graph) run of the same steps
+    d.reset_kernel_timing(True)
+    d.evolve_steps(args.timing_steps)
+    d.synchronize()
     kt = {n: d.kernel_timing(n) for n in ("extrapolate", "flux0", "update", "flux1")}
     d.reset_kernel_timing(False)
     d.distribute_to_vertices_and_edges()

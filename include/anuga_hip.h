@@ -33,6 +33,9 @@ extern "C" {
 #define ANUGA_E_NO_NEIGHBOUR (-4)    /* two-boundary triangle without an interior neighbour
                                         (sw_domain_orig.c:2432-2437) */
 #define ANUGA_E_STATE (-5)           /* call out of sequence */
+#define ANUGA_E_TIMESTEP (-6)        /* too small timestep after the first-order fallback,
+                                        or time overshot finaltime (generic_domain.py:2370-2391,
+                                        :1873-1878) */
 
 /* quantity ids */
 #define ANUGA_STAGE 0
@@ -134,6 +137,42 @@ int anuga_update_conserved_quantities(anuga_domain *d, double dt, int64_t *num_n
  * + friction + update + fix negative + saxpy(0.5, 0.5) with the backup
  * (generic_domain.py:2018-2045; quantity.c:752-769). */
 int anuga_rk2_second_substep(anuga_domain *d, double dt, int64_t *num_negative);
+
+/* Timestep control of Generic_Domain.update_timestep (generic_domain.py:2349-2415). */
+typedef struct anuga_time_control {
+    double CFL;
+    double evolve_max_timestep;
+    double evolve_min_timestep;
+    double relative_finaltime;   /* HUGE_VAL when there is none */
+    double relative_yieldtime;   /* next yield time; HUGE_VAL when there is none */
+    double fixed_flux_timestep;  /* > 0 replaces the CFL timestep (set_fixed_flux_timestep) */
+    double epsilon;              /* finaltime tolerance (anuga/config.py epsilon) */
+    int64_t max_smallsteps;
+    int64_t default_order;
+} anuga_time_control;
+
+/* Timestep-control state carried from one run of steps to the next. */
+typedef struct anuga_step_state {
+    double relative_time;          /* in/out */
+    double timestep;               /* out: dt of the last step */
+    double flux_timestep;          /* out: CFL timestep of the last step */
+    double recorded_min_timestep;  /* in/out */
+    double recorded_max_timestep;  /* in/out */
+    int64_t smallsteps;            /* in/out */
+    int64_t order;                 /* in/out: 2, or 1 after the small-step fallback */
+    int64_t steps_taken;           /* out */
+    int64_t reached;               /* out: 1 final time, 2 yield time, 0 neither */
+} anuga_step_state;
+
+/* Up to max_steps full steps (evolve_one_euler_step / evolve_one_rk2_step,
+ * generic_domain.py:1914-2051) with update_timestep evaluated on the device,
+ * so the run needs one host synchronisation at its end. Stops early, exactly
+ * where _evolve_base would yield (relative_yieldtime) or finish
+ * (relative_finaltime - epsilon); the remaining launches are no-ops. One step
+ * is captured once into a HIP graph and replayed (unless ANUGA_HIP_GRAPH=0 or
+ * kernel timing is on). dt_history (may be NULL) receives the timesteps taken. */
+int anuga_evolve_steps(anuga_domain *d, int64_t max_steps, const anuga_time_control *tc,
+                       anuga_step_state *state, double *dt_history);
 
 /* boundary_flux_sum of the last step's substeps (sums[0..1]): the boundary
  * inflow rate (sw_domain_orig.c:1119-1126). Synchronizes. */
