@@ -22,27 +22,66 @@ __device__ __forceinline__ int xcd_block()
     return xcd * per + (xcd < rem ? xcd : rem) + slot;
 }
 
+// Per-triangle records, packed so that gathering a neighbour's fields touches
+// one cache line. Triangles are in the library's internal (RCM) order.
+struct alignas(32) DeQ {   // conserved state + bed: one record per triangle
+    double w, uh, vh, z;
+};
+
+// FP64 geometry (bit-comparable path)
+struct alignas(16) DeGeoX64 {  // extrapolation: edge midpoint - centroid
+    double dxv[3], dyv[3];
+};
+struct alignas(16) DeGeoF64 {  // flux: outward normals, edge lengths, radius
+    double nx[3], ny[3], len[3], radius, pad[2];
+};
+// mixed-precision geometry: offsets relative to the centroid, so FP32 keeps
+// full relative precision on a large domain
+struct alignas(16) DeGeoX32 {
+    float dxv[3], dyv[3];
+    float rx[3], ry[3];   // surrogate-neighbour centroid - own centroid (0 on boundary edges)
+};
+struct alignas(16) DeGeoF32 {
+    float nx[3], ny[3], len[3], radius;
+    float dzc[3];   // neighbour centroid bed - own centroid bed (0 on boundary edges)
+    float pad[3];
+};
+
+// mixed-precision edge record of one triangle: per edge the stage and depth
+// jumps from the FP64 centroid values and the edge momenta, plus the FP64
+// centroid depth, so that the flux never subtracts two FP32 numbers of the
+// size of the water depth
+struct alignas(64) DeEdgeRec32 {
+    float4 e[3];     // dw, dh, uh, vh
+    double hc;
+    double pad;
+};
+
+template <class T>
+struct alignas(16) DeEdge4 {  // extrapolated values on one edge
+    T w, h, uh, vh;
+};
+
 struct DeMesh {
     int N;
-    const int *__restrict__ nbr;       // [3][N]: (n << 2) | m, or -(j+1) on boundary edge j
-    const double *__restrict__ nx;     // [3][N] outward unit normal
+    const int4 *__restrict__ nb;      // (n << 2) | m per edge, or -(j+1) on boundary edge j;
+                                      // .w = number of boundary edges
+    const double2 *__restrict__ cen;  // FP64 centroids
+    const void *__restrict__ gx;      // DeGeoX64 or DeGeoX32
+    const void *__restrict__ gf;      // DeGeoF64 or DeGeoF32
+    const double *__restrict__ area;  // FP64 area (both paths)
+    // FP64 flux geometry as separate arrays, [3][N] (radius [N]): every field
+    // is streamed once per triangle, and separate arrays keep the flux
+    // kernel's register footprint (and so its occupancy) low
+    const double *__restrict__ nx;
     const double *__restrict__ ny;
-    const double *__restrict__ len;    // [3][N] edge length
-    const double *__restrict__ dxv;    // [3][N] edge midpoint - centroid
-    const double *__restrict__ dyv;
-    const double *__restrict__ cx;     // [N] centroid
-    const double *__restrict__ cy;
-    const double *__restrict__ area;   // [N]
-    const double *__restrict__ radius; // [N]
+    const double *__restrict__ len;
+    const double *__restrict__ radius;
 };
 
 struct DeParams {
     double g, eps, mah;
     double bw, bwd, buh, buhd, bvh, bvhd;
-};
-
-struct DeEdges {        // [3][N] each, written by the extrapolation
-    double *we, *he, *uhe, *vhe;
 };
 
 struct DeBoundary {
@@ -95,199 +134,210 @@ __device__ __forceinline__ double de_height(double w, double z)
     return fmax(w - z, 0.0);
 }
 
-// "surrounded by dry cells" test (sw_domain_orig.c:2046-2064) of triangle j,
-// whose surrogate neighbour on a boundary edge is j itself.
-__device__ __forceinline__ bool de_dry_surrounded(const DeMesh &m, const double *__restrict__ w,
-                                                  const double *__restrict__ z, int j, double mah)
-{
-    bool dry = true;
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const int v = m.nbr[i * m.N + j];
-        if (v >= 0) {
-            const int s = v >> 2;
-            dry = dry && (de_height(w[s], z[s]) < mah);
-        }
-    }
-    return dry;
-}
-
-// Centroid stage / height / velocity of triangle j as seen by the
-// extrapolation after protect and the velocity conversion
-// (sw_domain_orig.c:2014-2064).
+// Centroid stage / height / velocity as seen by the extrapolation after
+// protect and the velocity conversion (sw_domain_orig.c:2014-2064).
 struct DeCell {
     double w, h, u, v;
 };
 
-__device__ __forceinline__ DeCell de_prepare(const DeMesh &m, const double *__restrict__ w,
-                                             const double *__restrict__ uh,
-                                             const double *__restrict__ vh,
-                                             const double *__restrict__ z, int j, double mah,
-                                             bool &keep, bool &clamp)
-{
-    DeCell c;
-    const double wr = w[j], zj = z[j];
-    clamp = de_clamped(wr, zj, mah);
-    c.w = clamp ? zj : wr;
-    c.h = de_height(c.w, zj);
-    keep = false;
-    c.u = 0.0;
-    c.v = 0.0;
-    if (c.h > mah) {
-        keep = !de_dry_surrounded(m, w, z, j, mah);
-        if (keep) {
-            const double inv = 1.0 / c.h;
-            c.u = uh[j] * inv;
-            c.v = vh[j] * inv;
-        }
-    }
-    return c;
-}
-
 // limit_gradient (sw_domain_orig.c:1306-1333) applied to provisional jumps d
-__device__ __forceinline__ void de_limit(double d[3], double qmin, double qmax, double beta)
+template <class T>
+__device__ __forceinline__ void de_limit(T d[3], T qmin, T qmax, T beta)
 {
-    double r = 1000.0, r0 = 1.0;
-    const double tiny = 1.0e-100;
+    T r = T(1000.0), r0 = T(1.0);
+    const T tiny = T(1.0e-100);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         if (d[i] < -tiny) r0 = qmin / d[i];
         if (d[i] > tiny) r0 = qmax / d[i];
         r = fmin(r0, r);
     }
-    const double phi = fmin(r * beta, 1.0);
+    const T phi = fmin(r * beta, T(1.0));
     d[0] = d[0] * phi;
     d[1] = d[1] * phi;
     d[2] = d[2] * phi;
 }
 
 // Auxiliary-triangle gradient, min/max bounds and limiter for one quantity
-// (sw_domain_orig.c:2238-2277 and its xmom/ymom/height twins)
-__device__ __forceinline__ void de_aux_edges(double qk, double q0, double q1, double q2,
-                                             double dx1, double dx2, double dy1, double dy2,
-                                             double inv_area2, const double dxv[3],
-                                             const double dyv[3], double beta, double out[3])
+// (sw_domain_orig.c:2238-2277 and its xmom/ymom/height twins). The centroid
+// differences are formed in FP64; d receives the limited jumps to the edges.
+template <class T>
+__device__ __forceinline__ void de_aux_jumps(double qk, double q0, double q1, double q2, T dx1,
+                                             T dx2, T dy1, T dy2, T inv_area2, const T dxv[3],
+                                             const T dyv[3], T beta, T d[3])
 {
-    const double dq0 = q0 - qk;
-    const double dq1 = q1 - q0;
-    const double dq2 = q2 - q0;
-    double a = dy2 * dq1 - dy1 * dq2;
+    const T dq0 = T(q0 - qk);
+    const T dq1 = T(q1 - q0);
+    const T dq2 = T(q2 - q0);
+    T a = dy2 * dq1 - dy1 * dq2;
     a *= inv_area2;
-    double b = dx1 * dq2 - dx2 * dq1;
+    T b = dx1 * dq2 - dx2 * dq1;
     b *= inv_area2;
-    double d[3];
     d[0] = a * dxv[0] + b * dyv[0];
     d[1] = a * dxv[1] + b * dyv[1];
     d[2] = a * dxv[2] + b * dyv[2];
-    const double qmax = fmax(fmax(dq0, fmax(dq0 + dq1, dq0 + dq2)), 0.0);
-    const double qmin = fmin(fmin(dq0, fmin(dq0 + dq1, dq0 + dq2)), 0.0);
+    const T qmax = fmax(fmax(dq0, fmax(dq0 + dq1, dq0 + dq2)), T(0.0));
+    const T qmin = fmin(fmin(dq0, fmin(dq0 + dq1, dq0 + dq2)), T(0.0));
     de_limit(d, qmin, qmax, beta);
-    out[0] = qk + d[0];
-    out[1] = qk + d[1];
-    out[2] = qk + d[2];
 }
 
-// Gradient toward the single interior neighbour (sw_domain_orig.c:2470-2504)
-__device__ __forceinline__ void de_line_edges(double qk, double q1, double dx2, double dy2,
-                                              const double dxv[3], const double dyv[3],
-                                              double beta, double out[3])
+// Gradient toward the single interior neighbour (sw_domain_orig.c:2470-2504);
+// d receives the limited jumps
+template <class T>
+__device__ __forceinline__ void de_line_jumps(double qk, double q1, T dx2, T dy2, const T dxv[3],
+                                              const T dyv[3], T beta, T d[3])
 {
-    const double dq1 = q1 - qk;
-    const double a = dq1 * dx2;
-    const double b = dq1 * dy2;
-    double d[3];
+    const T dq1 = T(q1 - qk);
+    const T a = dq1 * dx2;
+    const T b = dq1 * dy2;
     d[0] = a * dxv[0] + b * dyv[0];
     d[1] = a * dxv[1] + b * dyv[1];
     d[2] = a * dxv[2] + b * dyv[2];
-    double qmin, qmax;
-    if (dq1 >= 0.0) {
-        qmin = 0.0;
+    T qmin, qmax;
+    if (dq1 >= T(0.0)) {
+        qmin = T(0.0);
         qmax = dq1;
     } else {
         qmin = dq1;
-        qmax = 0.0;
+        qmax = T(0.0);
     }
     de_limit(d, qmin, qmax, beta);
-    out[0] = qk + d[0];
-    out[1] = qk + d[1];
-    out[2] = qk + d[2];
 }
 
 // Central-upwind flux across one edge, low_froude = 0
 // (sw_domain_orig.c:284-505; the rotation is _rotate, :37-57).
-__device__ __forceinline__ void de_flux_central(double ql0, double ql1, double ql2, double qr0,
-                                                double qr1, double qr2, double h_left,
-                                                double h_right, double hle, double hre,
-                                                double n1, double n2, double eps, double ze,
-                                                double g, double ef[3], double &max_speed,
-                                                double &pressure_flux)
+template <class T>
+__device__ __forceinline__ void de_flux_central(T ql0, T ql1, T ql2, T qr0, T qr1, T qr2,
+                                                T h_left, T h_right, T hle, T hre, T n1, T n2,
+                                                T eps, T ze, T g, T ef[3], T &max_speed,
+                                                T &pressure_flux)
 {
-    if (h_left == 0.0 && h_right == 0.0) {
-        ef[0] = ef[1] = ef[2] = 0.0;
-        max_speed = 0.0;
-        pressure_flux = 0.0;
+    if (h_left == T(0.0) && h_right == T(0.0)) {
+        ef[0] = ef[1] = ef[2] = T(0.0);
+        max_speed = T(0.0);
+        pressure_flux = T(0.0);
         return;
     }
     // rotate into the edge frame
-    const double l1 = n1 * ql1 + n2 * ql2;
-    const double l2 = -n2 * ql1 + n1 * ql2;
-    const double r1 = n1 * qr1 + n2 * qr2;
-    const double r2 = -n2 * qr1 + n1 * qr2;
+    const T l1 = n1 * ql1 + n2 * ql2;
+    const T l2 = -n2 * ql1 + n1 * ql2;
+    const T r1 = n1 * qr1 + n2 * qr2;
+    const T r2 = -n2 * qr1 + n1 * qr2;
 
-    double ul, uhl, vhl, ur, uhr, vhr;
+    T ul, uhl, vhl, ur, uhr, vhr;
     uhl = l1;
     vhl = l2;
-    if (hle > 0.0) {
-        const double t = 1.0 / hle;
+    if (hle > T(0.0)) {
+        const T t = T(1.0) / hle;
         ul = uhl * t;
         uhl = h_left * ul;
         vhl = h_left * t * vhl;
     } else {
-        ul = uhl = vhl = 0.0;
+        ul = uhl = vhl = T(0.0);
     }
     uhr = r1;
     vhr = r2;
-    if (hre > 0.0) {
-        const double t = 1.0 / hre;
+    if (hre > T(0.0)) {
+        const T t = T(1.0) / hre;
         ur = uhr * t;
         uhr = h_right * ur;
         vhr = h_right * t * vhr;
     } else {
-        ur = uhr = vhr = 0.0;
+        ur = uhr = vhr = T(0.0);
     }
-    const double cl = sqrt(g * h_left);
-    const double cr = sqrt(g * h_right);
-    double s_max = fmax(ul + cl, ur + cr);
-    if (s_max < 0.0) s_max = 0.0;
-    double s_min = fmin(ul - cl, ur - cr);
-    if (s_min > 0.0) s_min = 0.0;
+    const T cl = sqrt(g * h_left);
+    const T cr = sqrt(g * h_right);
+    T s_max = fmax(ul + cl, ur + cr);
+    if (s_max < T(0.0)) s_max = T(0.0);
+    T s_min = fmin(ul - cl, ur - cr);
+    if (s_min > T(0.0)) s_min = T(0.0);
 
-    const double fl0 = ul * h_left, fl1 = ul * uhl, fl2 = ul * vhl;
-    const double fr0 = ur * h_right, fr1 = ur * uhr, fr2 = ur * vhr;
-    const double denom = s_max - s_min;
+    const T fl0 = ul * h_left, fl1 = ul * uhl, fl2 = ul * vhl;
+    const T fr0 = ur * h_right, fr1 = ur * uhr, fr2 = ur * vhr;
+    const T denom = s_max - s_min;
     if (denom < eps) {
-        ef[0] = ef[1] = ef[2] = 0.0;
-        max_speed = 0.0;
-        pressure_flux = 0.5 * g * 0.5 * (h_left * h_left + h_right * h_right);
+        ef[0] = ef[1] = ef[2] = T(0.0);
+        max_speed = T(0.0);
+        pressure_flux = T(0.5) * g * T(0.5) * (h_left * h_left + h_right * h_right);
         return;
     }
     max_speed = fmax(s_max, -s_min);
-    const double inv = 1.0 / fmax(denom, 1.0e-100);
-    const double sms = s_max * s_min;
-    double e0 = s_max * fl0 - s_min * fr0;
+    const T inv = T(1.0) / fmax(denom, T(1.0e-100));
+    const T sms = s_max * s_min;
+    T e0 = s_max * fl0 - s_min * fr0;
     e0 += sms * (fmax(qr0, ze) - fmax(ql0, ze));
     e0 *= inv;
-    double e1 = s_max * fl1 - s_min * fr1;
-    e1 += 1.0 * sms * (uhr - uhl);
+    T e1 = s_max * fl1 - s_min * fr1;
+    e1 += T(1.0) * sms * (uhr - uhl);
     e1 *= inv;
-    double e2 = s_max * fl2 - s_min * fr2;
-    e2 += 1.0 * sms * (vhr - vhl);
+    T e2 = s_max * fl2 - s_min * fr2;
+    e2 += T(1.0) * sms * (vhr - vhl);
     e2 *= inv;
-    pressure_flux = 0.5 * g * (s_max * h_left * h_left - s_min * h_right * h_right) * inv;
+    pressure_flux = T(0.5) * g * (s_max * h_left * h_left - s_min * h_right * h_right) * inv;
     // rotate back: _rotate(edgeflux, n1, -n2)
     ef[0] = e0;
     ef[1] = n1 * e1 + (-n2) * e2;
     ef[2] = -(-n2) * e1 + n1 * e2;
+}
+
+// The same flux with the stage-diffusion difference supplied by the caller:
+// fmax(qr0, ze) - fmax(ql0, ze) equals h_right - h_left, which the mixed path
+// forms from small differences (used with T = float).
+template <class T>
+__device__ __forceinline__ void de_flux_central_d(T ql1, T ql2, T qr1, T qr2, T h_left, T h_right,
+                                                  T hle, T hre, T n1, T n2, T eps, T diff0, T g,
+                                                  T ef[3], T &max_speed, T &pressure_flux)
+{
+    if (h_left == T(0.0) && h_right == T(0.0)) {
+        ef[0] = ef[1] = ef[2] = T(0.0);
+        max_speed = T(0.0);
+        pressure_flux = T(0.0);
+        return;
+    }
+    const T l1 = n1 * ql1 + n2 * ql2;
+    const T l2 = -n2 * ql1 + n1 * ql2;
+    const T r1 = n1 * qr1 + n2 * qr2;
+    const T r2 = -n2 * qr1 + n1 * qr2;
+    T ul, uhl, vhl, ur, uhr, vhr;
+    if (hle > T(0.0)) {
+        const T t = T(1.0) / hle;
+        ul = l1 * t;
+        uhl = h_left * ul;
+        vhl = h_left * t * l2;
+    } else {
+        ul = uhl = vhl = T(0.0);
+    }
+    if (hre > T(0.0)) {
+        const T t = T(1.0) / hre;
+        ur = r1 * t;
+        uhr = h_right * ur;
+        vhr = h_right * t * r2;
+    } else {
+        ur = uhr = vhr = T(0.0);
+    }
+    const T cl = sqrt(g * h_left);
+    const T cr = sqrt(g * h_right);
+    T s_max = fmax(ul + cl, ur + cr);
+    if (s_max < T(0.0)) s_max = T(0.0);
+    T s_min = fmin(ul - cl, ur - cr);
+    if (s_min > T(0.0)) s_min = T(0.0);
+    const T denom = s_max - s_min;
+    if (denom < eps) {
+        ef[0] = ef[1] = ef[2] = T(0.0);
+        max_speed = T(0.0);
+        pressure_flux = T(0.5) * g * T(0.5) * (h_left * h_left + h_right * h_right);
+        return;
+    }
+    max_speed = fmax(s_max, -s_min);
+    const T inv = T(1.0) / denom;
+    const T sms = s_max * s_min;
+    const T e0 = (s_max * (ul * h_left) - s_min * (ur * h_right) + sms * diff0) * inv;
+    const T e1 = (s_max * (ul * uhl) - s_min * (ur * uhr) + sms * (uhr - uhl)) * inv;
+    const T e2 = (s_max * (ul * vhl) - s_min * (ur * vhr) + sms * (vhr - vhl)) * inv;
+    pressure_flux = T(0.5) * g * (s_max * h_left * h_left - s_min * h_right * h_right) * inv;
+    ef[0] = e0;
+    ef[1] = n1 * e1 - n2 * e2;
+    ef[2] = n2 * e1 + n1 * e2;
 }
 
 // wave64 + LDS block reductions; the combine order is fixed, so results are

@@ -43,15 +43,19 @@ struct anuga_domain {
     std::vector<int> perm;   // perm[new] = old
     std::vector<int> iperm;  // iperm[old] = new
 
-    // static mesh, [3][N] or [N], new order
-    int *nbr = nullptr;
-    double *nx = nullptr, *ny = nullptr, *len = nullptr, *dxv = nullptr, *dyv = nullptr;
-    double *cx = nullptr, *cy = nullptr, *area = nullptr, *radius = nullptr;
+    // static mesh records, internal order
+    int4 *nb = nullptr;
+    double2 *cen = nullptr;
+    void *gx = nullptr, *gf = nullptr;   // DeGeoX64/F64 or DeGeoX32/F32
+    double *area = nullptr;
+    double *nx = nullptr, *ny = nullptr, *len = nullptr, *radius = nullptr;  // FP64 flux geometry
+    bool mixed = false;
+    std::vector<DeGeoF32> gf32;   // host copy of the mixed flux geometry (bed offsets change)
     // quantities
-    double *z = nullptr, *eta = nullptr;
-    double *A[3] = {nullptr, nullptr, nullptr};  // step state / rk2 backup
-    double *B[3] = {nullptr, nullptr, nullptr};  // rk2 substep state
-    double *E[4] = {nullptr, nullptr, nullptr, nullptr};  // we, he, uhe, vhe
+    double *eta = nullptr;
+    DeQ *A = nullptr;   // step state / rk2 backup: {w, uh, vh, z}
+    DeQ *B = nullptr;   // rk2 substep state
+    void *E = nullptr;  // 3N DeEdge4<double> or DeEdge4<float>
     unsigned char *flags = nullptr;
     double *ex[3] = {nullptr, nullptr, nullptr};
     signed char *bkind = nullptr;
@@ -271,64 +275,80 @@ static DeMesh dev_mesh(const anuga_domain *d)
 {
     DeMesh m;
     m.N = d->N;
-    m.nbr = d->nbr;
+    m.nb = d->nb;
+    m.cen = d->cen;
+    m.gx = d->gx;
+    m.gf = d->gf;
+    m.area = d->area;
     m.nx = d->nx;
     m.ny = d->ny;
     m.len = d->len;
-    m.dxv = d->dxv;
-    m.dyv = d->dyv;
-    m.cx = d->cx;
-    m.cy = d->cy;
-    m.area = d->area;
     m.radius = d->radius;
     return m;
 }
 
-static int launch_extrapolate(anuga_domain *d, double *const *S)
+static int launch_extrapolate(anuga_domain *d, DeQ *S)
 {
     timer_begin(d, T_EXTRAP);
-    hipLaunchKernelGGL(de_extrapolate, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, dev_mesh(d),
-                       dev_params(d->P), DeState{S[0], S[1], S[2]}, d->z,
-                       DeEdges{d->E[0], d->E[1], d->E[2], d->E[3]}, d->flags,
-                       d->part + 2 * (size_t)d->nblocks, d->ctl);
+    if (d->mixed)
+        hipLaunchKernelGGL(de_extrapolate<float>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
+                           dev_mesh(d), dev_params(d->P), S, d->E, d->flags,
+                           d->part + 2 * (size_t)d->nblocks, d->ctl);
+    else
+        hipLaunchKernelGGL(de_extrapolate<double>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
+                           dev_mesh(d), dev_params(d->P), S, d->E, d->flags,
+                           d->part + 2 * (size_t)d->nblocks, d->ctl);
     timer_end(d, T_EXTRAP);
     return ANUGA_OK;
 }
 
-static void launch_materialize(anuga_domain *d, double *const *S)
+static void launch_materialize(anuga_domain *d, DeQ *S)
 {
-    hipLaunchKernelGGL(de_materialize, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N, S[0],
-                       S[1], S[2], d->z, d->flags);
+    hipLaunchKernelGGL(de_materialize, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N, S,
+                       d->flags);
+}
+
+template <class T, int SUB>
+static void launch_flux_t(anuga_domain *d, DeQ *S, DeQ *bk)
+{
+    DeBoundary bc{d->bkind, d->bval, d->M};
+    if constexpr (sizeof(T) == 8)
+        hipLaunchKernelGGL((de_flux<double, SUB>), dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
+                           dev_mesh(d), dev_params(d->P), S, d->eta,
+                           (const double *)d->E, bc, d->flags, d->ex[0], d->ex[1],
+                           d->ex[2], bk, d->ctl, d->part, d->part + d->nblocks, d->dneg);
+    else
+        hipLaunchKernelGGL((de_flux_mixed<SUB>), dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
+                           dev_mesh(d), dev_params(d->P), S, d->eta, (const DeEdgeRec32 *)d->E,
+                           bc, d->flags, d->ex[0], d->ex[1], d->ex[2], bk, d->ctl, d->part,
+                           d->part + d->nblocks, d->dneg);
 }
 
 // substep 0: explicit updates; substep 1: advance S with ctl->dt into the backup `bk`
-static void launch_flux(anuga_domain *d, int substep, double *const *S, double *const *bk)
+static void launch_flux(anuga_domain *d, int substep, DeQ *S, DeQ *bk)
 {
-    DeBoundary bc{d->bkind, d->bval, d->M};
-    DeEdges e{d->E[0], d->E[1], d->E[2], d->E[3]};
-    if (substep == 0) {
-        timer_begin(d, T_FLUX0);
-        hipLaunchKernelGGL(de_flux<0>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, dev_mesh(d),
-                           dev_params(d->P), DeState{S[0], S[1], S[2]}, d->z, d->eta, e, bc,
-                           d->flags, d->ex[0], d->ex[1], d->ex[2], nullptr, nullptr, nullptr,
-                           d->ctl, d->part, d->part + d->nblocks, d->dneg);
-        timer_end(d, T_FLUX0);
+    timer_begin(d, substep ? T_FLUX1 : T_FLUX0);
+    if (d->mixed) {
+        if (substep) launch_flux_t<float, 1>(d, S, bk);
+        else launch_flux_t<float, 0>(d, S, bk);
     } else {
-        timer_begin(d, T_FLUX1);
-        hipLaunchKernelGGL(de_flux<1>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, dev_mesh(d),
-                           dev_params(d->P), DeState{S[0], S[1], S[2]}, d->z, d->eta, e, bc,
-                           d->flags, nullptr, nullptr, nullptr, bk[0], bk[1], bk[2], d->ctl,
-                           d->part, d->part + d->nblocks, d->dneg);
-        timer_end(d, T_FLUX1);
+        if (substep) launch_flux_t<double, 1>(d, S, bk);
+        else launch_flux_t<double, 0>(d, S, bk);
     }
+    timer_end(d, substep ? T_FLUX1 : T_FLUX0);
 }
 
-static void launch_update(anuga_domain *d, double *const *S, double *const *out)
+static void launch_update(anuga_domain *d, DeQ *S, DeQ *out)
 {
     timer_begin(d, T_UPDATE);
-    hipLaunchKernelGGL(de_update, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N,
-                       dev_params(d->P), DeState{S[0], S[1], S[2]}, d->z, d->eta, d->flags,
-                       d->ex[0], d->ex[1], d->ex[2], d->ctl, out[0], out[1], out[2], d->dneg);
+    if (d->mixed)
+        hipLaunchKernelGGL(de_update<false>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N,
+                           dev_params(d->P), S, d->eta, d->flags, d->ex[0], d->ex[1], d->ex[2],
+                           d->ctl, out, d->dneg);
+    else
+        hipLaunchKernelGGL(de_update<true>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N,
+                           dev_params(d->P), S, d->eta, d->flags, d->ex[0], d->ex[1], d->ex[2],
+                           d->ctl, out, d->dneg);
     timer_end(d, T_UPDATE);
 }
 
@@ -346,10 +366,10 @@ static void launch_single(anuga_domain *d, void (*k)(DeControl *))
     hipLaunchKernelGGL(k, dim3(1), dim3(64), 0, d->stream, d->ctl);
 }
 
-static void launch_volume(anuga_domain *d, double *const *S)
+static void launch_volume(anuga_domain *d, DeQ *S)
 {
-    hipLaunchKernelGGL(de_volume, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N, S[0], d->z,
-                       d->area, d->part + 2 * (size_t)d->nblocks);
+    hipLaunchKernelGGL(de_volume, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N, S, d->area,
+                       d->part + 2 * (size_t)d->nblocks);
 }
 
 // leave device-controlled mode before host-controlled launches
@@ -435,8 +455,6 @@ int anuga_create(const anuga_mesh *mesh, const anuga_params *params, anuga_domai
     if (!mesh) return fail(nullptr, ANUGA_E_ARG, "mesh is NULL");
     int rc = check_params(nullptr, params);
     if (rc) return rc;
-    if (params->precision != ANUGA_PRECISION_FP64)
-        return fail(nullptr, ANUGA_E_ARG, "mixed precision is not built into this library");
     const int64_t N64 = mesh->number_of_elements, M64 = mesh->boundary_length;
     if (N64 <= 0 || N64 >= (int64_t(1) << 29))
         return fail(nullptr, ANUGA_E_ARG, "number_of_elements must be in [1, 2^29)");
@@ -497,25 +515,27 @@ int anuga_create(const anuga_mesh *mesh, const anuga_params *params, anuga_domai
     } while (0)
 
     DCHK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    d->mixed = params->precision == ANUGA_PRECISION_MIXED;
     const size_t n3 = 3 * (size_t)N;
-    DCHK(dalloc(&d->nbr, n3));
-    DCHK(dalloc(&d->nx, n3));
-    DCHK(dalloc(&d->ny, n3));
-    DCHK(dalloc(&d->len, n3));
-    DCHK(dalloc(&d->dxv, n3));
-    DCHK(dalloc(&d->dyv, n3));
-    DCHK(dalloc(&d->cx, N));
-    DCHK(dalloc(&d->cy, N));
+    DCHK(dalloc(&d->nb, N));
+    DCHK(dalloc(&d->cen, N));
     DCHK(dalloc(&d->area, N));
-    DCHK(dalloc(&d->radius, N));
-    DCHK(dalloc(&d->z, N));
     DCHK(dalloc(&d->eta, N));
-    for (int q = 0; q < 3; q++) {
-        DCHK(dalloc(&d->A[q], N));
-        DCHK(dalloc(&d->B[q], N));
-        DCHK(dalloc(&d->ex[q], N));
+    DCHK(dalloc(&d->A, N));
+    DCHK(dalloc(&d->B, N));
+    for (int q = 0; q < 3; q++) DCHK(dalloc(&d->ex[q], N));
+    if (d->mixed) {
+        DCHK(dalloc((DeGeoX32 **)&d->gx, N));
+        DCHK(dalloc((DeGeoF32 **)&d->gf, N));
+        DCHK(dalloc((DeEdgeRec32 **)&d->E, N));
+    } else {
+        DCHK(dalloc((DeGeoX64 **)&d->gx, N));
+        DCHK(dalloc(&d->nx, n3));
+        DCHK(dalloc(&d->ny, n3));
+        DCHK(dalloc(&d->len, n3));
+        DCHK(dalloc(&d->radius, N));
+        DCHK(dalloc((double **)&d->E, 4 * n3));
     }
-    for (int q = 0; q < 4; q++) DCHK(dalloc(&d->E[q], n3));
     DCHK(dalloc(&d->flags, N));
     DCHK(dalloc(&d->bkind, M));
     DCHK(dalloc(&d->bval, 3 * (size_t)M));
@@ -526,48 +546,74 @@ int anuga_create(const anuga_mesh *mesh, const anuga_params *params, anuga_domai
     DCHK(hipHostMalloc((void **)&d->hneg, sizeof(unsigned long long), hipHostMallocDefault));
 
     {
-        std::vector<int> hn(n3);
-        std::vector<double> a(n3), b(n3), c(n3), dx(n3), dy(n3);
         const double *nrm = mesh->normals, *el = mesh->edgelengths;
         const double *cen = mesh->centroid_coordinates, *em = mesh->edge_coordinates;
+        std::vector<int4> hn(N);
+        std::vector<double2> hc(N);
+        std::vector<double> ha(N);
         for (int p = 0; p < N; p++) {
-            const int k = d->perm[p];
+            const int64_t k = d->perm[p];
+            int v[3];
             for (int i = 0; i < 3; i++) {
-                const int64_t ki = 3 * (int64_t)k + i;
-                const int64_t n = nbr[ki];
-                hn[(size_t)i * N + p] = n >= 0 ? ((d->iperm[n] << 2) | (int)nbe[ki]) : (int)n;
-                a[(size_t)i * N + p] = nrm[2 * ki];
-                b[(size_t)i * N + p] = nrm[2 * ki + 1];
-                c[(size_t)i * N + p] = el[ki];
-                // xv - x and yv - y exactly as sw_domain_orig.c:2130-2135 forms them
-                dx[(size_t)i * N + p] = em[6 * (int64_t)k + 2 * i] - cen[2 * (int64_t)k];
-                dy[(size_t)i * N + p] = em[6 * (int64_t)k + 2 * i + 1] - cen[2 * (int64_t)k + 1];
+                const int64_t n = nbr[3 * k + i];
+                v[i] = n >= 0 ? ((d->iperm[n] << 2) | (int)nbe[3 * k + i]) : (int)n;
             }
+            hn[p] = make_int4(v[0], v[1], v[2], (int)nob[k]);
+            hc[p] = make_double2(cen[2 * k], cen[2 * k + 1]);
+            ha[p] = mesh->areas[k];
         }
-        DCHK(hipMemcpy(d->nbr, hn.data(), n3 * sizeof(int), hipMemcpyHostToDevice));
-        DCHK(hipMemcpy(d->nx, a.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
-        DCHK(hipMemcpy(d->ny, b.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
-        DCHK(hipMemcpy(d->len, c.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
-        DCHK(hipMemcpy(d->dxv, dx.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
-        DCHK(hipMemcpy(d->dyv, dy.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
-        std::vector<double> t(N);
-        for (int p = 0; p < N; p++) t[p] = cen[2 * (int64_t)d->perm[p]];
-        DCHK(hipMemcpy(d->cx, t.data(), N * sizeof(double), hipMemcpyHostToDevice));
-        for (int p = 0; p < N; p++) t[p] = cen[2 * (int64_t)d->perm[p] + 1];
-        DCHK(hipMemcpy(d->cy, t.data(), N * sizeof(double), hipMemcpyHostToDevice));
-        for (int p = 0; p < N; p++) t[p] = mesh->areas[d->perm[p]];
-        DCHK(hipMemcpy(d->area, t.data(), N * sizeof(double), hipMemcpyHostToDevice));
-        for (int p = 0; p < N; p++) t[p] = mesh->radii[d->perm[p]];
-        DCHK(hipMemcpy(d->radius, t.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        DCHK(hipMemcpy(d->nb, hn.data(), N * sizeof(int4), hipMemcpyHostToDevice));
+        DCHK(hipMemcpy(d->cen, hc.data(), N * sizeof(double2), hipMemcpyHostToDevice));
+        DCHK(hipMemcpy(d->area, ha.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        if (!d->mixed) {
+            std::vector<DeGeoX64> gx(N);
+            std::vector<double> a(n3), b(n3), c(n3), r(N);
+            for (int p = 0; p < N; p++) {
+                const int64_t k = d->perm[p];
+                for (int i = 0; i < 3; i++) {
+                    // xv - x and yv - y exactly as sw_domain_orig.c:2130-2135 forms them
+                    gx[p].dxv[i] = em[6 * k + 2 * i] - cen[2 * k];
+                    gx[p].dyv[i] = em[6 * k + 2 * i + 1] - cen[2 * k + 1];
+                    a[(size_t)i * N + p] = nrm[6 * k + 2 * i];
+                    b[(size_t)i * N + p] = nrm[6 * k + 2 * i + 1];
+                    c[(size_t)i * N + p] = el[3 * k + i];
+                }
+                r[p] = mesh->radii[k];
+            }
+            DCHK(hipMemcpy(d->gx, gx.data(), N * sizeof(DeGeoX64), hipMemcpyHostToDevice));
+            DCHK(hipMemcpy(d->nx, a.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+            DCHK(hipMemcpy(d->ny, b.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+            DCHK(hipMemcpy(d->len, c.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+            DCHK(hipMemcpy(d->radius, r.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            std::vector<DeGeoX32> gx(N);
+            std::vector<DeGeoF32> gf(N);
+            for (int p = 0; p < N; p++) {
+                const int64_t k = d->perm[p];
+                for (int i = 0; i < 3; i++) {
+                    gx[p].dxv[i] = (float)(em[6 * k + 2 * i] - cen[2 * k]);
+                    gx[p].dyv[i] = (float)(em[6 * k + 2 * i + 1] - cen[2 * k + 1]);
+                    const int64_t s = sur[3 * k + i];
+                    gx[p].rx[i] = (float)(cen[2 * s] - cen[2 * k]);
+                    gx[p].ry[i] = (float)(cen[2 * s + 1] - cen[2 * k + 1]);
+                    gf[p].nx[i] = (float)nrm[6 * k + 2 * i];
+                    gf[p].ny[i] = (float)nrm[6 * k + 2 * i + 1];
+                    gf[p].len[i] = (float)el[3 * k + i];
+                }
+                gf[p].radius = (float)mesh->radii[k];
+            }
+            for (int p = 0; p < N; p++)
+                for (int i = 0; i < 3; i++) gf[p].dzc[i] = gf[p].pad[i] = 0.f;
+            DCHK(hipMemcpy(d->gx, gx.data(), N * sizeof(DeGeoX32), hipMemcpyHostToDevice));
+            DCHK(hipMemcpy(d->gf, gf.data(), N * sizeof(DeGeoF32), hipMemcpyHostToDevice));
+            d->gf32 = std::move(gf);
+        }
     }
-    DCHK(hipMemset(d->z, 0, N * sizeof(double)));
     DCHK(hipMemset(d->eta, 0, N * sizeof(double)));
-    for (int q = 0; q < 3; q++) {
-        DCHK(hipMemset(d->A[q], 0, N * sizeof(double)));
-        DCHK(hipMemset(d->B[q], 0, N * sizeof(double)));
-        DCHK(hipMemset(d->ex[q], 0, N * sizeof(double)));
-    }
-    for (int q = 0; q < 4; q++) DCHK(hipMemset(d->E[q], 0, n3 * sizeof(double)));
+    DCHK(hipMemset(d->A, 0, N * sizeof(DeQ)));
+    DCHK(hipMemset(d->B, 0, N * sizeof(DeQ)));
+    for (int q = 0; q < 3; q++) DCHK(hipMemset(d->ex[q], 0, N * sizeof(double)));
+    DCHK(hipMemset(d->E, 0, d->mixed ? N * sizeof(DeEdgeRec32) : 4 * n3 * sizeof(double)));
     DCHK(hipMemset(d->flags, DE_F_KEEP, N));
     DCHK(hipMemset(d->bkind, 0, std::max(M, 1)));
     DCHK(hipMemset(d->bval, 0, 3 * (size_t)std::max(M, 1) * sizeof(double)));
@@ -590,10 +636,10 @@ void anuga_destroy(anuga_domain *d)
     if (!d) return;
     if (d->stream) hipStreamSynchronize(d->stream);
     drop_graph(d);
-    void *ptrs[] = {d->nbr, d->nx, d->ny, d->len, d->dxv, d->dyv, d->cx, d->cy, d->area,
-                    d->radius, d->z, d->eta, d->A[0], d->A[1], d->A[2], d->B[0], d->B[1],
-                    d->B[2], d->E[0], d->E[1], d->E[2], d->E[3], d->flags, d->ex[0],
-                    d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->ctl, d->dneg, d->hist};
+    void *ptrs[] = {d->nb, d->cen, d->gx, d->gf, d->area, d->nx, d->ny, d->len, d->radius,
+                    d->eta, d->A, d->B, d->E, d->flags,
+                    d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->ctl, d->dneg,
+                    d->hist};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (d->hctl) hipHostFree(d->hctl);
@@ -620,48 +666,82 @@ int anuga_set_params(anuga_domain *d, const anuga_params *params)
     return ANUGA_OK;
 }
 
-static double *quantity_ptr(anuga_domain *d, int q)
+// component of the DeQ record holding a quantity (-1: not in the record)
+static int record_slot(int q)
 {
-    double *const *cur = d->substate ? d->B : d->A;
     switch (q) {
-    case ANUGA_STAGE: return cur[0];
-    case ANUGA_XMOMENTUM: return cur[1];
-    case ANUGA_YMOMENTUM: return cur[2];
-    case ANUGA_ELEVATION: return d->z;
-    case ANUGA_FRICTION: return d->eta;
-    default: return nullptr;
+    case ANUGA_STAGE: return 0;
+    case ANUGA_XMOMENTUM: return 1;
+    case ANUGA_YMOMENTUM: return 2;
+    case ANUGA_ELEVATION: return 3;
+    default: return -1;
     }
+}
+
+static int download_records(anuga_domain *d, DeQ *src, std::vector<DeQ> &h)
+{
+    h.resize(d->N);
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    HIPCHK(d, hipMemcpy(h.data(), src, d->N * sizeof(DeQ), hipMemcpyDeviceToHost));
+    return ANUGA_OK;
 }
 
 int anuga_set_centroid_values(anuga_domain *d, int quantity, const double *values)
 {
     if (!d || !values) return ANUGA_E_ARG;
-    double *dst = quantity_ptr(d, quantity);
-    if (!dst) return fail(d, ANUGA_E_ARG, "quantity cannot be set");
-    std::vector<double> t(d->N);
-    for (int p = 0; p < d->N; p++) t[p] = values[d->perm[p]];
-    HIPCHK(d, hipStreamSynchronize(d->stream));
-    HIPCHK(d, hipMemcpy(dst, t.data(), d->N * sizeof(double), hipMemcpyHostToDevice));
+    const int N = d->N;
+    if (quantity == ANUGA_FRICTION) {
+        std::vector<double> t(N);
+        for (int p = 0; p < N; p++) t[p] = values[d->perm[p]];
+        HIPCHK(d, hipStreamSynchronize(d->stream));
+        HIPCHK(d, hipMemcpy(d->eta, t.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        return ANUGA_OK;
+    }
+    const int slot = record_slot(quantity);
+    if (slot < 0) return fail(d, ANUGA_E_ARG, "quantity cannot be set");
+    // the bed is part of both state buffers; the others of the current one
+    DeQ *bufs[2] = {d->substate ? d->B : d->A, slot == 3 ? (d->substate ? d->A : d->B) : nullptr};
+    for (DeQ *buf : bufs) {
+        if (!buf) continue;
+        std::vector<DeQ> h;
+        int rc = download_records(d, buf, h);
+        if (rc) return rc;
+        for (int p = 0; p < N; p++) (&h[p].w)[slot] = values[d->perm[p]];
+        HIPCHK(d, hipMemcpy(buf, h.data(), N * sizeof(DeQ), hipMemcpyHostToDevice));
+    }
+    if (slot == 3 && d->mixed) {
+        // bed offsets of the mixed flux's local datum
+        std::vector<int4> nb(N);
+        HIPCHK(d, hipMemcpy(nb.data(), d->nb, N * sizeof(int4), hipMemcpyDeviceToHost));
+        for (int p = 0; p < N; p++) {
+            const int v[3] = {nb[p].x, nb[p].y, nb[p].z};
+            const double zp = values[d->perm[p]];
+            for (int i = 0; i < 3; i++)
+                d->gf32[p].dzc[i] = v[i] >= 0 ? (float)(values[d->perm[v[i] >> 2]] - zp) : 0.f;
+        }
+        HIPCHK(d, hipMemcpy(d->gf, d->gf32.data(), N * sizeof(DeGeoF32), hipMemcpyHostToDevice));
+    }
     return ANUGA_OK;
 }
 
 int anuga_get_centroid_values(anuga_domain *d, int quantity, double *values)
 {
     if (!d || !values) return ANUGA_E_ARG;
-    std::vector<double> t(d->N);
-    HIPCHK(d, hipStreamSynchronize(d->stream));
-    if (quantity == ANUGA_HEIGHT) {
-        std::vector<double> w(d->N), z(d->N);
-        HIPCHK(d, hipMemcpy(w.data(), quantity_ptr(d, ANUGA_STAGE), d->N * sizeof(double),
-                            hipMemcpyDeviceToHost));
-        HIPCHK(d, hipMemcpy(z.data(), d->z, d->N * sizeof(double), hipMemcpyDeviceToHost));
-        for (int p = 0; p < d->N; p++) values[d->perm[p]] = std::fmax(w[p] - z[p], 0.0);
+    const int N = d->N;
+    if (quantity == ANUGA_FRICTION) {
+        std::vector<double> t(N);
+        HIPCHK(d, hipStreamSynchronize(d->stream));
+        HIPCHK(d, hipMemcpy(t.data(), d->eta, N * sizeof(double), hipMemcpyDeviceToHost));
+        for (int p = 0; p < N; p++) values[d->perm[p]] = t[p];
         return ANUGA_OK;
     }
-    double *src = quantity_ptr(d, quantity);
-    if (!src) return fail(d, ANUGA_E_ARG, "unknown quantity");
-    HIPCHK(d, hipMemcpy(t.data(), src, d->N * sizeof(double), hipMemcpyDeviceToHost));
-    for (int p = 0; p < d->N; p++) values[d->perm[p]] = t[p];
+    const int slot = record_slot(quantity);
+    if (slot < 0 && quantity != ANUGA_HEIGHT) return fail(d, ANUGA_E_ARG, "unknown quantity");
+    std::vector<DeQ> h;
+    int rc = download_records(d, d->substate ? d->B : d->A, h);
+    if (rc) return rc;
+    for (int p = 0; p < N; p++)
+        values[d->perm[p]] = slot >= 0 ? (&h[p].w)[slot] : std::fmax(h[p].w - h[p].z, 0.0);
     return ANUGA_OK;
 }
 
@@ -669,27 +749,54 @@ int anuga_get_edge_values(anuga_domain *d, int quantity, double *values)
 {
     if (!d || !values) return ANUGA_E_ARG;
     const size_t n3 = 3 * (size_t)d->N;
-    std::vector<double> a(n3), b;
+    if (quantity != ANUGA_STAGE && quantity != ANUGA_HEIGHT && quantity != ANUGA_XMOMENTUM &&
+        quantity != ANUGA_YMOMENTUM && quantity != ANUGA_ELEVATION)
+        return fail(d, ANUGA_E_ARG, "unknown edge quantity");
     HIPCHK(d, hipStreamSynchronize(d->stream));
-    int src;
-    switch (quantity) {
-    case ANUGA_STAGE: src = 0; break;
-    case ANUGA_HEIGHT: src = 1; break;
-    case ANUGA_XMOMENTUM: src = 2; break;
-    case ANUGA_YMOMENTUM: src = 3; break;
-    case ANUGA_ELEVATION: src = 0; break;
-    default: return fail(d, ANUGA_E_ARG, "unknown edge quantity");
-    }
-    HIPCHK(d, hipMemcpy(a.data(), d->E[src], n3 * sizeof(double), hipMemcpyDeviceToHost));
-    if (quantity == ANUGA_ELEVATION) {
-        b.resize(n3);
-        HIPCHK(d, hipMemcpy(b.data(), d->E[1], n3 * sizeof(double), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < n3; i++) a[i] = a[i] - b[i];  // bed_edge = stage_edge - height_edge
+    std::vector<double> e(4 * n3);
+    if (d->mixed) {
+        // edge = centroid + jump; the centroid state is the one the last
+        // extrapolation saw after it was written back (distribute)
+        std::vector<DeEdgeRec32> r(d->N);
+        std::vector<DeQ> st;
+        HIPCHK(d, hipMemcpy(r.data(), d->E, d->N * sizeof(DeEdgeRec32), hipMemcpyDeviceToHost));
+        int rc = download_records(d, d->substate ? d->B : d->A, st);
+        if (rc) return rc;
+        for (int p = 0; p < d->N; p++)
+            for (int i = 0; i < 3; i++) {
+                const float4 f = r[p].e[i];
+                const double hc = r[p].hc;
+                const double wc = st[p].z + hc;
+                double *o = &e[4 * (3 * (size_t)p + i)];
+                o[0] = wc + f.x;
+                o[1] = hc + f.y;
+                o[2] = f.z;
+                o[3] = f.w;
+            }
+    } else {
+        // four [edge][triangle] arrays -> (triangle, edge, quantity)
+        std::vector<double> soa(4 * n3);
+        HIPCHK(d, hipMemcpy(soa.data(), d->E, 4 * n3 * sizeof(double), hipMemcpyDeviceToHost));
+        for (int p = 0; p < d->N; p++)
+            for (int i = 0; i < 3; i++)
+                for (int qq = 0; qq < 4; qq++)
+                    e[4 * (3 * (size_t)p + i) + qq] = soa[qq * n3 + (size_t)i * d->N + p];
     }
     const int N = d->N;
     for (int p = 0; p < N; p++) {
         const int64_t k = d->perm[p];
-        for (int i = 0; i < 3; i++) values[3 * k + i] = a[(size_t)i * N + p];
+        for (int i = 0; i < 3; i++) {
+            const double *r = &e[4 * (3 * (size_t)p + i)];   // w, h, uh, vh
+            double v;
+            switch (quantity) {
+            case ANUGA_STAGE: v = r[0]; break;
+            case ANUGA_HEIGHT: v = r[1]; break;
+            case ANUGA_XMOMENTUM: v = r[2]; break;
+            case ANUGA_YMOMENTUM: v = r[3]; break;
+            default: v = r[0] - r[1]; break;  // bed_edge = stage_edge - height_edge
+            }
+            values[3 * k + i] = v;
+        }
     }
     return ANUGA_OK;
 }
@@ -720,7 +827,7 @@ int anuga_distribute_to_vertices_and_edges(anuga_domain *d, double *mass_error)
 {
     if (!d) return ANUGA_E_ARG;
     ensure_live(d);
-    double **S = d->substate ? d->B : d->A;
+    DeQ *S = d->substate ? d->B : d->A;
     int rc = launch_extrapolate(d, S);
     if (rc) return rc;
     launch_materialize(d, S);
@@ -890,7 +997,7 @@ int anuga_get_negative_cells(anuga_domain *d, int64_t *total)
 int anuga_get_water_volume(anuga_domain *d, double *volume)
 {
     if (!d || !volume) return ANUGA_E_ARG;
-    double **S = d->substate ? d->B : d->A;
+    DeQ *S = d->substate ? d->B : d->A;
     launch_volume(d, S);
     launch_reduce(d, 3);
     int rc = launch_check(d);

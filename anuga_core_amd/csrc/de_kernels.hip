@@ -1,4 +1,4 @@
-// HIP kernels of the FP64 DE step for gfx950.
+// HIP kernels of the DE step for gfx950 (FP64 and mixed FP32/FP64).
 //
 // One thread per triangle, triangles in locality (RCM) order, 256-thread
 // workgroups remapped XCD-contiguous. Per Euler substep:
@@ -19,21 +19,50 @@
 //                   sw_domain_orig.c:2890-2910).
 //   de_reduce       fixed-order reduction of the per-block partials.
 // No atomics touch the state, so every run is bitwise reproducible.
+//
+// Every kernel is a template on T, the type of the extrapolation,
+// reconstruction and flux arithmetic: double for the FP64 path (bit-comparable
+// with the reference), float for the mixed path. The conserved state, the
+// update, the wet/dry thresholds, the CFL reduction and the volume stay FP64
+// in both.
 #include "de_device.h"
 #include "de_pow.h"
 
-// centroid state; no __restrict__: the Euler update writes it in place
-struct DeState {
-    const double *w;
-    const double *uh;
-    const double *vh;
+
+// ---------------------------------------------------------------------------
+// geometry access, per precision
+// ---------------------------------------------------------------------------
+template <class T> struct DeGeo;
+template <> struct DeGeo<double> {
+    using X = DeGeoX64;
+    using F = DeGeoF64;
 };
+template <> struct DeGeo<float> {
+    using X = DeGeoX32;
+    using F = DeGeoF32;
+};
+
+// the step's centroid state after protect + momentum zeroing, from the flag
+struct DeCentroid {
+    double w, uh, vh, z;
+};
+
+__device__ __forceinline__ DeCentroid de_centroid(const DeQ q, unsigned f)
+{
+    DeCentroid c;
+    c.z = q.z;
+    c.w = (f & DE_F_CLAMP) ? q.z : q.w;
+    c.uh = (f & DE_F_KEEP) ? q.uh : 0.0;
+    c.vh = (f & DE_F_KEEP) ? q.vh : 0.0;
+    return c;
+}
 
 // ---------------------------------------------------------------------------
 // extrapolation
 // ---------------------------------------------------------------------------
+template <class T>
 __global__ void __launch_bounds__(DE_BLOCK)
-de_extrapolate(DeMesh m, DeParams p, DeState s, const double *__restrict__ z, DeEdges e,
+de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict__ E,
                unsigned char *__restrict__ flags, double *__restrict__ part_mass,
                DeControl *ctl)
 {
@@ -46,59 +75,30 @@ de_extrapolate(DeMesh m, DeParams p, DeState s, const double *__restrict__ z, De
 
     if (k < N) {
         const double mah = p.mah;
-        // Loads are issued in three unconditional rounds (own cell, the three
-        // surrogate neighbours with their connectivity, the neighbours'
-        // neighbours' heights) so that each round's gathers overlap.
-        // round 1: own cell
-        int v[3], sur[3];
+        const int4 vk = m.nb[k];
+        const int v[3] = {vk.x, vk.y, vk.z};
+        const int nbound = vk.w;
+        const DeQ qk = q[k];
+        int sur[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) v[i] = m.nbr[i * N + k];
-        const double wk = s.w[k], zk = z[k], uhk = s.uh[k], vhk = s.vh[k];
-        int nbound = 0;
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            sur[i] = v[i] >= 0 ? (v[i] >> 2) : k;
-            nbound += v[i] < 0;
-        }
-        double dxv[3], dyv[3];
+        for (int i = 0; i < 3; i++) sur[i] = v[i] >= 0 ? (v[i] >> 2) : k;
+        // surrogate neighbours: one record each
+        DeQ qs[3];
+        int4 vs[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            dxv[i] = m.dxv[i * N + k];
-            dyv[i] = m.dyv[i * N + k];
+            qs[i] = q[sur[i]];
+            vs[i] = m.nb[sur[i]];
         }
-        // round 2: surrogate neighbours
-        double ws[3], zs[3], uhs[3], vhs[3], xs[3], ys[3];
-        int vs[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const int j = sur[i];
-            ws[i] = s.w[j];
-            zs[i] = z[j];
-            uhs[i] = s.uh[j];
-            vhs[i] = s.vh[j];
-            xs[i] = m.cx[j];
-            ys[i] = m.cy[j];
-#pragma unroll
-            for (int q = 0; q < 3; q++) vs[i][q] = m.nbr[q * N + j];
-        }
-        // round 3: heights of the neighbours' neighbours
-        double h2[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const int t = vs[i][q] >= 0 ? (vs[i][q] >> 2) : sur[i];
-                h2[i][q] = de_height(s.w[t], z[t]);
-            }
-
-        // protect + velocity of triangle k (sw_domain_orig.c:1245-1276, :2014-2064)
-        const bool clamp = de_clamped(wk, zk, mah);
-        DeCell c;
-        c.w = clamp ? zk : wk;
-        c.h = de_height(c.w, zk);
         double hs[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) hs[i] = de_height(ws[i], zs[i]);
+        for (int i = 0; i < 3; i++) hs[i] = de_height(qs[i].w, qs[i].z);
+
+        // protect + velocity of triangle k (sw_domain_orig.c:1245-1276, :2014-2064)
+        const bool clamp = de_clamped(qk.w, qk.z, mah);
+        DeCell c;
+        c.w = clamp ? qk.z : qk.w;
+        c.h = de_height(c.w, qk.z);
         bool dry_k = true;
 #pragma unroll
         for (int i = 0; i < 3; i++) dry_k = dry_k && (v[i] < 0 || hs[i] < mah);
@@ -107,111 +107,174 @@ de_extrapolate(DeMesh m, DeParams p, DeState s, const double *__restrict__ z, De
         c.v = 0.0;
         if (keep) {
             const double inv = 1.0 / c.h;
-            c.u = uhk * inv;
-            c.v = vhk * inv;
+            c.u = qk.uh * inv;
+            c.v = qk.vh * inv;
         }
-        if (clamp) mass = (zk - wk) * m.area[k];
+        if (clamp) mass = (qk.z - qk.w) * m.area[k];
         flags[k] = (clamp ? DE_F_CLAMP : 0u) | (keep ? DE_F_KEEP : 0u);
 
-        // the same for the surrogate neighbours
+        // the same for the surrogate neighbours; the dry-neighbour test of a
+        // wet neighbour needs the heights of its own neighbours
         DeCell nb[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const bool cl = de_clamped(ws[i], zs[i], mah);
-            nb[i].w = cl ? zs[i] : ws[i];
+            const bool cl = de_clamped(qs[i].w, qs[i].z, mah);
+            nb[i].w = cl ? qs[i].z : qs[i].w;
             nb[i].h = hs[i];
-            bool dry = true;
-#pragma unroll
-            for (int q = 0; q < 3; q++) dry = dry && (vs[i][q] < 0 || h2[i][q] < mah);
             nb[i].u = 0.0;
             nb[i].v = 0.0;
-            if (hs[i] > mah && !dry) {
-                const double inv = 1.0 / hs[i];
-                nb[i].u = uhs[i] * inv;
-                nb[i].v = vhs[i] * inv;
+            if (hs[i] > mah) {
+                const int vv[3] = {vs[i].x, vs[i].y, vs[i].z};
+                bool dry = true;
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    if (vv[r] >= 0) {
+                        const DeQ t = q[vv[r] >> 2];
+                        dry = dry && (de_height(t.w, t.z) < mah);
+                    }
+                }
+                if (!dry) {
+                    const double inv = 1.0 / hs[i];
+                    nb[i].u = qs[i].uh * inv;
+                    nb[i].v = qs[i].vh * inv;
+                }
             }
         }
 
-        double ow[3], oh[3], ou[3], ov[3];
+        const typename DeGeo<T>::X gx = static_cast<const typename DeGeo<T>::X *>(m.gx)[k];
+        T dxv[3], dyv[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            dxv[i] = gx.dxv[i];
+            dyv[i] = gx.dyv[i];
+        }
+        // limited jumps from the centroid to the edges, per quantity
+        T jw[3] = {T(0.0), T(0.0), T(0.0)}, jh[3] = {T(0.0), T(0.0), T(0.0)};
+        T ju[3] = {T(0.0), T(0.0), T(0.0)}, jv[3] = {T(0.0), T(0.0), T(0.0)};
         bool constant = (nbound == 3);
+        bool gw = true, gu = true, gv = true;   // false: beta == 0, edge = centroid value
         if (nbound <= 1) {
-            const double dx1 = xs[1] - xs[0], dx2 = xs[2] - xs[0];
-            const double dy1 = ys[1] - ys[0], dy2 = ys[2] - ys[0];
-            const double area2 = dy2 * dx1 - dy1 * dx2;
-            if (area2 <= 0.0) {
+            T dx1, dx2, dy1, dy2;
+            if constexpr (sizeof(T) == 8) {
+                const double2 c0 = m.cen[sur[0]], c1 = m.cen[sur[1]], c2 = m.cen[sur[2]];
+                dx1 = c1.x - c0.x;
+                dx2 = c2.x - c0.x;
+                dy1 = c1.y - c0.y;
+                dy2 = c2.y - c0.y;
+            } else {
+                dx1 = gx.rx[1] - gx.rx[0];
+                dx2 = gx.rx[2] - gx.rx[0];
+                dy1 = gx.ry[1] - gx.ry[0];
+                dy2 = gx.ry[2] - gx.ry[0];
+            }
+            const T area2 = dy2 * dx1 - dy1 * dx2;
+            if (area2 <= T(0.0)) {
                 constant = true;
             } else {
-                const double a_tmp = 0.3, b_tmp = 0.1;
-                const double c_tmp = 1.0 / (a_tmp - b_tmp);
-                const double d_tmp = 1.0 - (c_tmp * a_tmp);
-                const double hc = c.h, h0 = nb[0].h, h1 = nb[1].h, h2_ = nb[2].h;
-                const double hmin = fmin(fmin(h0, fmin(h1, h2_)), hc);
-                const double hmax = fmax(fmax(h0, fmax(h1, h2_)), hc);
-                double hfactor = fmax(0.0, fmin(c_tmp * fmax(hmin, 0.0) / fmax(hc, 1.0e-06) + d_tmp,
-                                                fmin(c_tmp * fmax(hc, 0.0) / fmax(hmax, 1.0e-06) + d_tmp,
-                                                     1.0)));
-                hfactor = fmin(1.2 * fmax(hmin - mah, 0.0) / (fmax(hmin, 0.0) + 1.0 * mah), hfactor);
-                const double inv_area2 = 1.0 / area2;
+                const T a_tmp = T(0.3), b_tmp = T(0.1);
+                const T c_tmp = T(1.0) / (a_tmp - b_tmp);
+                const T d_tmp = T(1.0) - (c_tmp * a_tmp);
+                const T tmah = T(mah);
+                const T hc = T(c.h), h0 = T(nb[0].h), h1 = T(nb[1].h), h2 = T(nb[2].h);
+                const T hmin = fmin(fmin(h0, fmin(h1, h2)), hc);
+                const T hmax = fmax(fmax(h0, fmax(h1, h2)), hc);
+                T hfactor = fmax(T(0.0), fmin(c_tmp * fmax(hmin, T(0.0)) / fmax(hc, T(1.0e-06)) + d_tmp,
+                                              fmin(c_tmp * fmax(hc, T(0.0)) / fmax(hmax, T(1.0e-06)) + d_tmp,
+                                                   T(1.0))));
+                hfactor = fmin(T(1.2) * fmax(hmin - tmah, T(0.0)) / (fmax(hmin, T(0.0)) + T(1.0) * tmah),
+                               hfactor);
+                const T inv_area2 = T(1.0) / area2;
 
-                double beta = p.bwd + (p.bw - p.bwd) * hfactor;
-                if (beta > 0.0) {
-                    de_aux_edges(c.w, nb[0].w, nb[1].w, nb[2].w, dx1, dx2, dy1, dy2, inv_area2,
-                                 dxv, dyv, beta, ow);
-                    de_aux_edges(c.h, nb[0].h, nb[1].h, nb[2].h, dx1, dx2, dy1, dy2, inv_area2,
-                                 dxv, dyv, beta, oh);
-                } else {
-                    ow[0] = ow[1] = ow[2] = c.w;
-                    oh[0] = oh[1] = oh[2] = c.h;
+                T beta = T(p.bwd) + (T(p.bw) - T(p.bwd)) * hfactor;
+                gw = beta > T(0.0);
+                if (gw) {
+                    de_aux_jumps<T>(c.w, nb[0].w, nb[1].w, nb[2].w, dx1, dx2, dy1, dy2, inv_area2,
+                                    dxv, dyv, beta, jw);
+                    de_aux_jumps<T>(c.h, nb[0].h, nb[1].h, nb[2].h, dx1, dx2, dy1, dy2, inv_area2,
+                                    dxv, dyv, beta, jh);
                 }
-                beta = p.buhd + (p.buh - p.buhd) * hfactor;
-                if (beta > 0.0) {
-                    de_aux_edges(c.u, nb[0].u, nb[1].u, nb[2].u, dx1, dx2, dy1, dy2, inv_area2,
-                                 dxv, dyv, beta, ou);
-                } else {
-                    ou[0] = ou[1] = ou[2] = c.u;
-                }
-                beta = p.bvhd + (p.bvh - p.bvhd) * hfactor;
-                if (beta > 0.0) {
-                    de_aux_edges(c.v, nb[0].v, nb[1].v, nb[2].v, dx1, dx2, dy1, dy2, inv_area2,
-                                 dxv, dyv, beta, ov);
-                } else {
-                    ov[0] = ov[1] = ov[2] = c.v;
-                }
+                beta = T(p.buhd) + (T(p.buh) - T(p.buhd)) * hfactor;
+                gu = beta > T(0.0);
+                if (gu)
+                    de_aux_jumps<T>(c.u, nb[0].u, nb[1].u, nb[2].u, dx1, dx2, dy1, dy2, inv_area2,
+                                    dxv, dyv, beta, ju);
+                beta = T(p.bvhd) + (T(p.bvh) - T(p.bvhd)) * hfactor;
+                gv = beta > T(0.0);
+                if (gv)
+                    de_aux_jumps<T>(c.v, nb[0].v, nb[1].v, nb[2].v, dx1, dx2, dy1, dy2, inv_area2,
+                                    dxv, dyv, beta, jv);
             }
         } else if (nbound == 2) {
             // the first edge whose surrogate is not k itself (sw_domain_orig.c:2421-2439)
-            const int i1 = (v[0] >= 0) ? 0 : ((v[1] >= 0) ? 1 : 2);
-            if (v[i1] < 0) {
+            // selects rather than indexing, so the register arrays stay in registers
+            const bool e0 = v[0] >= 0, e1 = v[1] >= 0, e2 = v[2] >= 0;
+            if (!(e0 || e1 || e2)) {
                 atomicOr(err, DE_ERR_NO_NEIGHBOUR);
                 constant = true;
             } else {
-                const double dx1 = xs[i1] - m.cx[k];
-                const double dy1 = ys[i1] - m.cy[k];
-                const double area2 = dx1 * dx1 + dy1 * dy1;
-                double dx2 = 1.0 / area2;
-                const double dy2 = dx2 * dy1;
+                // field-wise selects on named copies: indexing the array here
+                // would place it in private (scratch) memory
+                const DeCell a0 = nb[0], a1 = nb[1], a2 = nb[2];
+                DeCell n1;
+                n1.w = e0 ? a0.w : (e1 ? a1.w : a2.w);
+                n1.h = e0 ? a0.h : (e1 ? a1.h : a2.h);
+                n1.u = e0 ? a0.u : (e1 ? a1.u : a2.u);
+                n1.v = e0 ? a0.v : (e1 ? a1.v : a2.v);
+                T dx1, dy1;
+                if constexpr (sizeof(T) == 8) {
+                    const int s1 = e0 ? sur[0] : (e1 ? sur[1] : sur[2]);
+                    const double2 c1 = m.cen[s1], c0 = m.cen[k];
+                    dx1 = c1.x - c0.x;
+                    dy1 = c1.y - c0.y;
+                } else {
+                    dx1 = e0 ? gx.rx[0] : (e1 ? gx.rx[1] : gx.rx[2]);
+                    dy1 = e0 ? gx.ry[0] : (e1 ? gx.ry[1] : gx.ry[2]);
+                }
+                const T area2 = dx1 * dx1 + dy1 * dy1;
+                T dx2 = T(1.0) / area2;
+                const T dy2 = dx2 * dy1;
                 dx2 *= dx1;
-                de_line_edges(c.w, nb[i1].w, dx2, dy2, dxv, dyv, p.bw, ow);
-                de_line_edges(c.h, nb[i1].h, dx2, dy2, dxv, dyv, p.bw, oh);
-                de_line_edges(c.u, nb[i1].u, dx2, dy2, dxv, dyv, p.bw, ou);
-                de_line_edges(c.v, nb[i1].v, dx2, dy2, dxv, dyv, p.bw, ov);
+                const T bw = T(p.bw);
+                de_line_jumps<T>(c.w, n1.w, dx2, dy2, dxv, dyv, bw, jw);
+                de_line_jumps<T>(c.h, n1.h, dx2, dy2, dxv, dyv, bw, jh);
+                de_line_jumps<T>(c.u, n1.u, dx2, dy2, dxv, dyv, bw, ju);
+                de_line_jumps<T>(c.v, n1.v, dx2, dy2, dxv, dyv, bw, jv);
             }
         }
-        if (constant) {
+        if constexpr (sizeof(T) == 8) {
+            // FP64 edge values: four [edge][triangle] arrays (w, h, uh, vh)
+            double *__restrict__ ow = reinterpret_cast<double *>(E);
+            double *__restrict__ oh = ow + 3 * (size_t)N;
+            double *__restrict__ ouh = ow + 6 * (size_t)N;
+            double *__restrict__ ovh = ow + 9 * (size_t)N;
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                ow[i] = c.w;
-                oh[i] = c.h;
-                ou[i] = c.u;
-                ov[i] = c.v;
+                // edge = centroid + limited jump; the beta == 0 and constant
+                // branches copy the centroid value (sw_domain_orig.c:2278-2283)
+                const double ew = (constant || !gw) ? c.w : c.w + jw[i];
+                const double eh = (constant || !gw) ? c.h : c.h + jh[i];
+                const double eu = (constant || !gu) ? c.u : c.u + ju[i];
+                const double ev = (constant || !gv) ? c.v : c.v + jv[i];
+                const int ki = i * N + k;
+                ow[ki] = ew;
+                oh[ki] = eh;
+                ouh[ki] = eu * eh;
+                ovh[ki] = ev * eh;
             }
-        }
+        } else {
+            DeEdgeRec32 rec;
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            e.we[i * N + k] = ow[i];
-            e.he[i * N + k] = oh[i];
-            e.uhe[i * N + k] = ou[i] * oh[i];
-            e.vhe[i * N + k] = ov[i] * oh[i];
+            for (int i = 0; i < 3; i++) {
+                const float hw = constant ? 0.f : float(jw[i]);
+                const float hh = constant ? 0.f : float(jh[i]);
+                const float eh = float(c.h + double(hh));
+                const float eu = float(c.u + (constant ? 0.0 : double(ju[i])));
+                const float ev = float(c.v + (constant ? 0.0 : double(jv[i])));
+                rec.e[i] = make_float4(hw, hh, eu * eh, ev * eh);
+            }
+            rec.hc = c.h;
+            rec.pad = 0.0;
+            reinterpret_cast<DeEdgeRec32 *>(E)[k] = rec;
         }
     }
 
@@ -230,26 +293,21 @@ de_extrapolate(DeMesh m, DeParams p, DeState s, const double *__restrict__ z, De
 // Write the protect / momentum-zeroing outcome back into the centroid state
 // (what the reference's distribute_to_vertices_and_edges leaves behind).
 __global__ void __launch_bounds__(DE_BLOCK)
-de_materialize(int N, double *__restrict__ w, double *__restrict__ uh, double *__restrict__ vh,
-               const double *__restrict__ z, const unsigned char *__restrict__ flags)
+de_materialize(int N, DeQ *q, const unsigned char *__restrict__ flags)
 {
     const int k = xcd_block() * DE_BLOCK + threadIdx.x;
     if (k >= N) return;
     const unsigned f = flags[k];
-    if (f & DE_F_CLAMP) w[k] = z[k];
+    if (f & DE_F_CLAMP) q[k].w = q[k].z;
     if (!(f & DE_F_KEEP)) {
-        uh[k] = 0.0;
-        vh[k] = 0.0;
+        q[k].uh = 0.0;
+        q[k].vh = 0.0;
     }
 }
 
 // ---------------------------------------------------------------------------
 // friction + update + fix_negative_cells for one triangle
 // ---------------------------------------------------------------------------
-struct DeUpdated {
-    double w, uh, vh;
-};
-
 __device__ __forceinline__ double de_update_one(double q, double ex, double si, double dt,
                                                 int *err)
 {
@@ -265,30 +323,33 @@ __device__ __forceinline__ double de_update_one(double q, double ex, double si, 
     return q / den;
 }
 
-__device__ __forceinline__ DeUpdated de_advance(double w, double uh, double vh, double z,
-                                                double eta, double exw, double exuh,
-                                                double exvh, double dt, const DeParams &p,
-                                                int *err, unsigned long long *nneg)
+// CRPOW: correctly rounded pow(h, 7/3) (the FP64 path); else the math library's
+template <bool CRPOW>
+__device__ __forceinline__ DeQ de_advance(const DeCentroid c, double eta, double exw, double exuh,
+                                          double exvh, double dt, const DeParams &p, int *err,
+                                          unsigned long long *nneg)
 {
     // _manning_friction_flat with eps = minimum_allowed_height (friction.py:64-72)
     double si_uh = 0.0, si_vh = 0.0;
     if (eta > p.mah) {
-        const double h = w - z;
+        const double h = c.w - c.z;
         if (h >= p.mah) {
-            double S = -p.g * eta * eta * sqrt((uh * uh + vh * vh));
-            S /= de_pow_7_3(h);  // correctly rounded pow(h, 7.0/3.0)
-            si_uh += S * uh;
-            si_vh += S * vh;
+            double S = -p.g * eta * eta * sqrt((c.uh * c.uh + c.vh * c.vh));
+            if constexpr (CRPOW) S /= de_pow_7_3(h);
+            else S /= pow(h, 7.0 / 3.0);
+            si_uh += S * c.uh;
+            si_vh += S * c.vh;
         }
     }
-    DeUpdated r;
-    r.w = de_update_one(w, exw, 0.0, dt, err);
-    r.uh = de_update_one(uh, exuh, si_uh, dt, err);
-    r.vh = de_update_one(vh, exvh, si_vh, dt, err);
+    DeQ r;
+    r.w = de_update_one(c.w, exw, 0.0, dt, err);
+    r.uh = de_update_one(c.uh, exuh, si_uh, dt, err);
+    r.vh = de_update_one(c.vh, exvh, si_vh, dt, err);
+    r.z = c.z;
     // _orig_fix_negative_cells (every triangle is a full triangle here)
-    if (r.w - z < 0.0) {
+    if (r.w - c.z < 0.0) {
         atomicAdd(nneg, 1ull);
-        r.w = z;
+        r.w = c.z;
         r.uh = 0.0;
         r.vh = 0.0;
     }
@@ -298,16 +359,14 @@ __device__ __forceinline__ DeUpdated de_advance(double w, double uh, double vh, 
 // ---------------------------------------------------------------------------
 // flux
 // ---------------------------------------------------------------------------
-// SUBSTEP 0: writes explicit updates; SUBSTEP 1: advances with dt and
-// combines with the backup state (rk2 final stage).
-template <int SUBSTEP>
+// SUBSTEP 0: writes explicit updates; SUBSTEP 1: advances with ctl->dt and
+// combines with the backup state bk (rk2 final stage, saxpy 0.5/0.5).
+template <class T, int SUBSTEP>
 __global__ void __launch_bounds__(DE_BLOCK)
-de_flux(DeMesh m, DeParams p, DeState s, const double *__restrict__ z,
-        const double *__restrict__ eta, DeEdges ec, DeBoundary bc,
-        const unsigned char *__restrict__ flags, double *__restrict__ exw,
-        double *__restrict__ exuh, double *__restrict__ exvh, double *__restrict__ bw,
-        double *__restrict__ buh, double *__restrict__ bvh, DeControl *ctl,
-        double *__restrict__ part_dt, double *__restrict__ part_bf,
+de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
+        const T *__restrict__ E, DeBoundary bc, const unsigned char *__restrict__ flags,
+        double *__restrict__ exw, double *__restrict__ exuh, double *__restrict__ exvh, DeQ *bk,
+        DeControl *ctl, double *__restrict__ part_dt, double *__restrict__ part_bf,
         unsigned long long *__restrict__ nneg)
 {
     __shared__ double red_dt[DE_BLOCK / 64];
@@ -320,36 +379,37 @@ de_flux(DeMesh m, DeParams p, DeState s, const double *__restrict__ z,
     double dtmin = 1.0e100, bflux = 0.0;
 
     if (k < N) {
-        const double g = p.g, eps = p.eps;
-        const unsigned f = flags[k];
-        const double zc = z[k];
-        const double wc = (f & DE_F_CLAMP) ? zc : s.w[k];
-        const double uhc = (f & DE_F_KEEP) ? s.uh[k] : 0.0;
-        const double vhc = (f & DE_F_KEEP) ? s.vh[k] : 0.0;
-        const double hc = de_height(wc, zc);
+        const T g = T(p.g), eps = T(p.eps);
+        const DeCentroid c = de_centroid(q[k], flags[k]);
+        const int4 vk = m.nb[k];
+        const int v[3] = {vk.x, vk.y, vk.z};
+        const T *__restrict__ Ew = E;
+        const T *__restrict__ Eh = E + 3 * (size_t)N;
+        const T *__restrict__ Euh = E + 6 * (size_t)N;
+        const T *__restrict__ Evh = E + 9 * (size_t)N;
+        const T hc = T(de_height(c.w, c.z));
+        const T zc = T(c.z);
         double sw = 0.0, sx = 0.0, sy = 0.0;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const int ki = i * N + k;
-            const double ql0 = ec.we[ki], hle = ec.he[ki];
-            const double ql1 = ec.uhe[ki], ql2 = ec.vhe[ki];
-            const double zl = ql0 - hle;
-            const double n1 = m.nx[ki], n2 = m.ny[ki];
-            const int v = m.nbr[ki];
-            double qr0, qr1, qr2, zr, hre;
-            if (v >= 0) {
-                const int nm = (v & 3) * N + (v >> 2);
-                qr0 = ec.we[nm];
-                qr1 = ec.uhe[nm];
-                qr2 = ec.vhe[nm];
-                hre = ec.he[nm];
+            const T ql0 = Ew[ki], hle = Eh[ki], ql1 = Euh[ki], ql2 = Evh[ki];
+            const T zl = ql0 - hle;
+            const T n1 = m.nx[i * N + k], n2 = m.ny[i * N + k];
+            T qr0, qr1, qr2, zr, hre;
+            if (v[i] >= 0) {
+                const int nm = (v[i] & 3) * N + (v[i] >> 2);
+                qr0 = Ew[nm];
+                qr1 = Euh[nm];
+                qr2 = Evh[nm];
+                hre = Eh[nm];
                 zr = qr0 - hre;
             } else {
-                const int j = -v - 1;
+                const int j = -v[i] - 1;
                 const int kind = bc.kind[j];
                 if (kind == 0) {          // reflective
-                    const double r1 = -ql1 * n1 - ql2 * n2;
-                    const double r2 = -ql1 * n2 + ql2 * n1;
+                    const T r1 = -ql1 * n1 - ql2 * n2;
+                    const T r2 = -ql1 * n2 + ql2 * n1;
                     qr0 = ql0;
                     qr1 = n1 * r1 - n2 * r2;
                     qr2 = n2 * r1 + n1 * r2;
@@ -358,37 +418,37 @@ de_flux(DeMesh m, DeParams p, DeState s, const double *__restrict__ z,
                     qr1 = ql1;
                     qr2 = ql2;
                 } else if (kind == 2) {   // Dirichlet
-                    qr0 = bc.val[j];
-                    qr1 = bc.val[bc.M + j];
-                    qr2 = bc.val[2 * bc.M + j];
+                    qr0 = T(bc.val[j]);
+                    qr1 = T(bc.val[bc.M + j]);
+                    qr2 = T(bc.val[2 * bc.M + j]);
                 } else {                  // transmissive from centroid values
-                    qr0 = wc;
-                    qr1 = uhc;
-                    qr2 = vhc;
+                    qr0 = T(c.w);
+                    qr1 = T(c.uh);
+                    qr2 = T(c.vh);
                 }
                 zr = zl;
-                hre = fmax(qr0 - zr, 0.0);
+                hre = fmax(qr0 - zr, T(0.0));
             }
-            const double z_half = fmax(zl, zr);
-            const double h_left = fmax(hle + zl - z_half, 0.0);
-            const double h_right = fmax(hre + zr - z_half, 0.0);
-            double ef[3], speed, pflux;
-            de_flux_central(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1, n2, eps,
-                            z_half, g, ef, speed, pflux);
-            const double length = m.len[ki];
+            const T z_half = fmax(zl, zr);
+            const T h_left = fmax(hle + zl - z_half, T(0.0));
+            const T h_right = fmax(hre + zr - z_half, T(0.0));
+            T ef[3], speed, pflux;
+            de_flux_central<T>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1, n2,
+                               eps, z_half, g, ef, speed, pflux);
+            const T length = m.len[i * N + k];
             ef[0] *= length;
             ef[1] *= length;
             ef[2] *= length;
-            const double pg =
-                length * (-g * 0.5 * (h_left * h_left - hle * hle - (hle + hc) * (zl - zc)) + pflux);
-            sw += -ef[0];
-            sx += -ef[1];
-            sy += -ef[2];
-            if (v < 0) bflux += -ef[0];
-            sx -= n1 * pg;
-            sy -= n2 * pg;
-            if (SUBSTEP == 0 && speed > eps) {
-                const double t = 1.0 / fmax(speed, eps);
+            const T pg = length * (-g * T(0.5) * (h_left * h_left - hle * hle - (hle + hc) * (zl - zc))
+                                   + pflux);
+            sw += double(-ef[0]);
+            sx += double(-ef[1]);
+            sy += double(-ef[2]);
+            if (v[i] < 0) bflux += double(-ef[0]);
+            sx -= double(n1 * pg);
+            sy -= double(n2 * pg);
+            if (SUBSTEP == 0 && double(speed) > p.eps) {
+                const double t = 1.0 / fmax(double(speed), p.eps);
                 dtmin = fmin(dtmin, m.radius[k] * t);
             }
         }
@@ -401,11 +461,15 @@ de_flux(DeMesh m, DeParams p, DeState s, const double *__restrict__ z,
             exuh[k] = sx;
             exvh[k] = sy;
         } else {
-            const DeUpdated r = de_advance(wc, uhc, vhc, zc, eta[k], sw, sx, sy, dt, p, err, nneg);
+            const DeQ r = de_advance<sizeof(T) == 8>(c, eta[k], sw, sx, sy, dt, p, err, nneg);
             // saxpy_centroid_values(0.5, 0.5) with the backup (quantity.c:752-769)
-            bw[k] = 0.5 * r.w + 0.5 * bw[k];
-            buh[k] = 0.5 * r.uh + 0.5 * buh[k];
-            bvh[k] = 0.5 * r.vh + 0.5 * bvh[k];
+            const DeQ b = bk[k];
+            DeQ o;
+            o.w = 0.5 * r.w + 0.5 * b.w;
+            o.uh = 0.5 * r.uh + 0.5 * b.uh;
+            o.vh = 0.5 * r.vh + 0.5 * b.vh;
+            o.z = b.z;
+            bk[k] = o;
         }
     }
 
@@ -429,33 +493,164 @@ de_flux(DeMesh m, DeParams p, DeState s, const double *__restrict__ z,
     }
 }
 
-// friction + update + fix_negative_cells from the explicit updates of substep 0
+// Mixed-precision flux. All FP32 arithmetic is on quantities referred to the
+// own centroid bed z_c (a local datum): edge bed zl = dw - dh, neighbour edge
+// bed zr = (z_n - z_c) + dw' - dh', depths h = h_c + dh formed in FP64 and
+// rounded, and the two terms that are differences of depths,
+//   fmax(wr, z_half) - fmax(wl, z_half) = h_right - h_left
+//   h_left^2 - hle^2 = (zl - z_half)(h_left + hle)   (h_left > 0),
+// evaluated from small differences. Equal to the reference's formulas in real
+// arithmetic (sw_domain_orig.c:879-1004); sums, CFL and update stay FP64.
+template <int SUBSTEP>
 __global__ void __launch_bounds__(DE_BLOCK)
-de_update(int N, DeParams p, DeState s, const double *__restrict__ z,
-          const double *__restrict__ eta, const unsigned char *__restrict__ flags,
-          const double *__restrict__ exw, const double *__restrict__ exuh,
-          const double *__restrict__ exvh, DeControl *ctl, double *ow, double *ouh, double *ovh,
-          unsigned long long *__restrict__ nneg)
+de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
+              const DeEdgeRec32 *__restrict__ E, DeBoundary bc,
+              const unsigned char *__restrict__ flags, double *__restrict__ exw,
+              double *__restrict__ exuh, double *__restrict__ exvh, DeQ *bk, DeControl *ctl,
+              double *__restrict__ part_dt, double *__restrict__ part_bf,
+              unsigned long long *__restrict__ nneg)
+{
+    __shared__ double red_dt[DE_BLOCK / 64];
+    __shared__ double red_bf[DE_BLOCK / 64];
+    if (ctl->done) return;
+    int *err = &ctl->status;
+    const double dt = SUBSTEP ? ctl->dt : 0.0;
+    const int N = m.N;
+    const int k = xcd_block() * DE_BLOCK + threadIdx.x;
+    double dtmin = 1.0e100, bflux = 0.0;
+
+    if (k < N) {
+        const float g = float(p.g), eps = float(p.eps);
+        const DeCentroid c = de_centroid(q[k], flags[k]);
+        const int4 vk = m.nb[k];
+        const int v[3] = {vk.x, vk.y, vk.z};
+        const DeGeoF32 gf = static_cast<const DeGeoF32 *>(m.gf)[k];
+        const DeEdgeRec32 own = E[k];
+        const double hcd = own.hc;
+        const float hc = float(hcd);
+        double sw = 0.0, sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float4 el = own.e[i];                   // dw, dh, uh, vh
+            const float hle = float(hcd + double(el.y));
+            const float zl = el.x - el.y;
+            const float ql1 = el.z, ql2 = el.w;
+            const float n1 = gf.nx[i], n2 = gf.ny[i];
+            float hre, zr, qr1, qr2, dH = 0.f;
+            if (v[i] >= 0) {
+                const DeEdgeRec32 *rn = E + (v[i] >> 2);
+                const float4 er = rn->e[v[i] & 3];
+                const double hcn = rn->hc;
+                hre = float(hcn + double(er.y));
+                zr = gf.dzc[i] + (er.x - er.y);
+                qr1 = er.z;
+                qr2 = er.w;
+                dH = (float(hcn - hcd) + (er.y - el.y)) + (zr - zl);
+            } else {
+                const int j = -v[i] - 1;
+                const int kind = bc.kind[j];
+                float wr;                                   // stage above z_c
+                if (kind == 0) {          // reflective
+                    const float r1 = -ql1 * n1 - ql2 * n2;
+                    const float r2 = -ql1 * n2 + ql2 * n1;
+                    wr = zl + hle;
+                    qr1 = n1 * r1 - n2 * r2;
+                    qr2 = n2 * r1 + n1 * r2;
+                } else if (kind == 1) {   // transmissive
+                    wr = zl + hle;
+                    qr1 = ql1;
+                    qr2 = ql2;
+                } else if (kind == 2) {   // Dirichlet
+                    wr = float(bc.val[j] - c.z);
+                    qr1 = float(bc.val[bc.M + j]);
+                    qr2 = float(bc.val[2 * bc.M + j]);
+                } else {                  // transmissive from centroid values
+                    wr = float(c.w - c.z);
+                    qr1 = float(c.uh);
+                    qr2 = float(c.vh);
+                }
+                zr = zl;
+                hre = (kind <= 1) ? fmax(hle, 0.f) : fmax(wr - zr, 0.f);
+            }
+            const float z_half = fmax(zl, zr);
+            const float hl_raw = hle + (zl - z_half);
+            const float hr_raw = hre + (zr - z_half);
+            const float h_left = fmax(hl_raw, 0.f);
+            const float h_right = fmax(hr_raw, 0.f);
+            const float diff0 = (v[i] >= 0 && hl_raw > 0.f && hr_raw > 0.f) ? dH : h_right - h_left;
+            float ef[3], speed, pflux;
+            de_flux_central_d<float>(ql1, ql2, qr1, qr2, h_left, h_right, hle, hre, n1, n2, eps,
+                                     diff0, g, ef, speed, pflux);
+            const float length = gf.len[i];
+            const float dsq = (hl_raw > 0.f) ? (zl - z_half) * (h_left + hle) : -(hle * hle);
+            const float pg = length * (-g * 0.5f * (dsq - (hle + hc) * zl) + pflux);
+            sw -= double(ef[0] * length);
+            sx -= double(ef[1] * length);
+            sy -= double(ef[2] * length);
+            if (v[i] < 0) bflux -= double(ef[0] * length);
+            sx -= double(n1 * pg);
+            sy -= double(n2 * pg);
+            if (SUBSTEP == 0 && double(speed) > p.eps)
+                dtmin = fmin(dtmin, double(gf.radius) * (1.0 / double(speed)));
+        }
+        const double inv_area = 1.0 / m.area[k];
+        sw *= inv_area;
+        sx *= inv_area;
+        sy *= inv_area;
+        if (SUBSTEP == 0) {
+            exw[k] = sw;
+            exuh[k] = sx;
+            exvh[k] = sy;
+        } else {
+            const DeQ r = de_advance<false>(c, eta[k], sw, sx, sy, dt, p, err, nneg);
+            const DeQ b = bk[k];
+            DeQ o;
+            o.w = 0.5 * r.w + 0.5 * b.w;
+            o.uh = 0.5 * r.uh + 0.5 * b.uh;
+            o.vh = 0.5 * r.vh + 0.5 * b.vh;
+            o.z = b.z;
+            bk[k] = o;
+        }
+    }
+
+    dtmin = de_wave_min(dtmin);
+    bflux = de_wave_sum(bflux);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red_dt[wv] = dtmin;
+        red_bf[wv] = bflux;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a2 = red_dt[0], b2 = red_bf[0];
+        for (int i = 1; i < DE_BLOCK / 64; i++) {
+            a2 = fmin(a2, red_dt[i]);
+            b2 += red_bf[i];
+        }
+        const int lb = xcd_block();
+        part_dt[lb] = a2;
+        part_bf[lb] = b2;
+    }
+}
+
+// friction + update + fix_negative_cells from the explicit updates of substep 0;
+// `out` may alias `q` (Euler, in place: each thread reads then writes its own record)
+template <bool CRPOW>
+__global__ void __launch_bounds__(DE_BLOCK)
+de_update(int N, DeParams p, const DeQ *q, const double *__restrict__ eta,
+          const unsigned char *__restrict__ flags, const double *__restrict__ exw,
+          const double *__restrict__ exuh, const double *__restrict__ exvh, DeControl *ctl,
+          DeQ *out, unsigned long long *__restrict__ nneg)
 {
     if (ctl->done) return;
     const int k = xcd_block() * DE_BLOCK + threadIdx.x;
     if (k >= N) return;
     const double dt = ctl->dt;
     int *err = &ctl->status;
-    const unsigned f = flags[k];
-    const double zc = z[k];
-    const double wc = (f & DE_F_CLAMP) ? zc : s.w[k];
-    const double uhc = (f & DE_F_KEEP) ? s.uh[k] : 0.0;
-    const double vhc = (f & DE_F_KEEP) ? s.vh[k] : 0.0;
-    const DeUpdated r = de_advance(wc, uhc, vhc, zc, eta[k], exw[k], exuh[k], exvh[k], dt, p, err, nneg);
-    ow[k] = r.w;
-    ouh[k] = r.uh;
-    ovh[k] = r.vh;
+    const DeCentroid c = de_centroid(q[k], flags[k]);
+    out[k] = de_advance<CRPOW>(c, eta[k], exw[k], exuh[k], exvh[k], dt, p, err, nneg);
 }
 
-// ---------------------------------------------------------------------------
-// reductions
-// ---------------------------------------------------------------------------
 // Fixed-order reduction of the per-block partials into the control block,
 // one block of 1024 threads. mode 0: flux_dt, bflux0, mass (first substep);
 // 1: bflux1 (second substep); 2: mass (distribute); 3: volume.
@@ -600,13 +795,16 @@ __global__ void de_clear_done(DeControl *c)
 
 // per-block partial of sum(area * max(w - z, 0)) over the current state
 __global__ void __launch_bounds__(DE_BLOCK)
-de_volume(int N, const double *__restrict__ w, const double *__restrict__ z,
-          const double *__restrict__ area, double *__restrict__ part)
+de_volume(int N, const DeQ *__restrict__ q, const double *__restrict__ area,
+          double *__restrict__ part)
 {
     __shared__ double red[DE_BLOCK / 64];
     const int k = xcd_block() * DE_BLOCK + threadIdx.x;
     double v = 0.0;
-    if (k < N) v = area[k] * de_height(w[k], z[k]);
+    if (k < N) {
+        const DeQ r = q[k];
+        v = area[k] * de_height(r.w, r.z);
+    }
     v = de_wave_sum(v);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) red[wv] = v;
@@ -617,12 +815,3 @@ de_volume(int N, const double *__restrict__ w, const double *__restrict__ z,
         part[xcd_block()] = t;
     }
 }
-
-template __global__ void de_flux<0>(DeMesh, DeParams, DeState, const double *, const double *,
-                                    DeEdges, DeBoundary, const unsigned char *, double *,
-                                    double *, double *, double *, double *, double *,
-                                    DeControl *, double *, double *, unsigned long long *);
-template __global__ void de_flux<1>(DeMesh, DeParams, DeState, const double *, const double *,
-                                    DeEdges, DeBoundary, const unsigned char *, double *,
-                                    double *, double *, double *, double *, double *,
-                                    DeControl *, double *, double *, unsigned long long *);

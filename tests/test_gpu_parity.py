@@ -183,3 +183,36 @@ def test_graph_replay_matches_eager(monkeypatch):
         res.append(vals)
     for a, b in zip(*res):
         assert np.array_equal(a[0], b[0]) and a[1] == b[1] and a[2] == b[2]
+
+
+def _error_report(a, b):
+    d = np.abs(a - b)
+    return float(d.sum() / max(np.abs(b).sum(), 1e-300)), float(d.max())
+
+
+@pytest.mark.parametrize("builder,kw,ft", [(S.hills_de1, dict(m=30, n=30), 30.0),
+                                           (S.dam_break_de0, dict(m=30, n=30), 3.0)])
+def test_mixed_precision_error_vs_oracle(builder, kw, ft):
+    """FP32 extrapolation / reconstruction / flux with FP64 state, update, CFL
+    and thresholds: stage relative L1 error <= 1e-5 against the FP64 oracle;
+    volume conserved to FP64 rounding of the same accounting."""
+    print("case mixed %s" % builder.__name__, flush=True)
+    mesh, alg, q, bc = builder(**kw)
+    o = S.setup(OracleDomain(mesh, OracleKernels(gather=True)), alg, q, bc)
+    d = S.setup(Domain(mesh=mesh, precision="mixed"), alg, q, bc)
+    for _ in o.evolve(yieldstep=ft, finaltime=ft):
+        pass
+    for _ in d.evolve(yieldstep=ft, finaltime=ft):
+        pass
+    l1w, mxw = _error_report(d.get_quantity("stage").centroid_values, o.stage_c)
+    l1u, mxu = _error_report(d.get_quantity("xmomentum").centroid_values, o.xmom_c)
+    print("mixed stage L1 %.3e max %.3e | xmom L1 %.3e max %.3e | steps %d vs %d"
+          % (l1w, mxw, l1u, mxu, len(d.dt_history), len(o.dt_history)), flush=True)
+    h = np.maximum(o.stage_c - o.elev_c, 0.0)
+    l1h = float(np.abs(d.get_quantity("stage").centroid_values - o.stage_c).sum() / h.sum())
+    print("mixed stage L1 relative to the water column %.3e" % l1h, flush=True)
+    assert l1h <= 1e-5
+    v0 = float(np.sum(mesh.areas * np.maximum(q["stage"] - q["elevation"], 0)))
+    drift = (d.get_water_volume() - v0 - d.boundary_flux_integral) / v0
+    print("mixed volume drift %.3e" % drift, flush=True)
+    assert abs(drift) < 1e-10
