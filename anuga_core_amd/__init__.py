@@ -11,9 +11,9 @@
 from .boundaries import Dirichlet_boundary, Reflective_boundary, Transmissive_boundary
 from .mesh import Mesh, rectangular, rectangular_cross
 
-__all__ = ["Domain", "Mesh", "rectangular", "rectangular_cross", "rectangular_cross_domain",
-           "rectangular_domain", "Reflective_boundary", "Transmissive_boundary",
-           "Dirichlet_boundary"]
+__all__ = ["Domain", "Ensemble", "Mesh", "rectangular", "rectangular_cross",
+           "rectangular_cross_domain", "rectangular_domain", "Reflective_boundary",
+           "Transmissive_boundary", "Dirichlet_boundary"]
 
 
 def __getattr__(name):
@@ -22,6 +22,9 @@ def __getattr__(name):
     if name == "Domain":
         from .domain import Domain
         return Domain
+    if name == "Ensemble":
+        from .ensemble import Ensemble
+        return Ensemble
     raise AttributeError(name)
 
 

@@ -89,6 +89,19 @@ SIGNATURES = {
     "anuga_synchronize": (C.c_int, [C.c_void_p]),
     "anuga_kernel_timing": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _ip]),
     "anuga_reset_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "anuga_ensemble_create": (C.c_int, [C.c_int64, _ip, _ip, C.POINTER(AnugaParams),
+                                        C.POINTER(C.c_void_p)]),
+    "anuga_ensemble_set_mesh": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(AnugaMesh)]),
+    "anuga_ensemble_set_centroid_values": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, _dp]),
+    "anuga_ensemble_get_centroid_values": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, _dp]),
+    "anuga_ensemble_set_boundary": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int32),
+                                              _dp]),
+    "anuga_ensemble_evolve_steps": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(TimeControl),
+                                              C.POINTER(StepState)]),
+    "anuga_ensemble_distribute_to_vertices_and_edges": (C.c_int, [C.c_void_p, _dp]),
+    "anuga_ensemble_get_water_volumes": (C.c_int, [C.c_void_p, _dp]),
+    "anuga_ensemble_get_boundary_flux_integrals": (C.c_int, [C.c_void_p, _dp]),
+    "anuga_ensemble_size": (C.c_int, [C.c_void_p, _ip]),
 }
 
 _lib = None

@@ -63,7 +63,8 @@ struct alignas(16) DeEdge4 {  // extrapolated values on one edge
 };
 
 struct DeMesh {
-    int N;
+    int N;                            // triangles incl. the padding between scenarios
+    const int *__restrict__ scen;     // [blocks] scenario of each (logical) block; null: one scenario
     const int4 *__restrict__ nb;      // (n << 2) | m per edge, or -(j+1) on boundary edge j;
                                       // .w = number of boundary edges
     const double2 *__restrict__ cen;  // FP64 centroids
@@ -109,7 +110,16 @@ struct DeControl {
     int done;              // 1: finaltime reached, 2: yield time reached (steps become no-ops)
     int status;            // error bits, DE_ERR_*
     int hist_cap, rk2;
+    // the scenario's triangles are [kbeg, kend) and its blocks [bbeg, bend) of
+    // the internal numbering (fixed at creation)
+    int kbeg, kend, bbeg, bend;
 };
+
+// control block of the scenario that owns logical block lb
+__device__ __forceinline__ DeControl *de_ctl(DeControl *ctl, const int *scen, int lb)
+{
+    return scen ? ctl + scen[lb] : ctl;
+}
 
 #define DE_ERR_NO_NEIGHBOUR 1
 #define DE_ERR_SEMI_IMPLICIT 2

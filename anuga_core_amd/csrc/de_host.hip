@@ -1,11 +1,13 @@
 // Host runtime and C ABI of the DE step (see include/anuga_hip.h).
 //
-// The handle owns all device buffers. Triangles are renumbered once at
-// creation (Reverse-Cuthill-McKee on the triangle adjacency) so that neighbour
-// gathers stay within a narrow index window; the permutation is applied on
-// upload and undone on download, so callers only ever see the reference's
-// order. Per-edge and per-triangle arrays are stored structure-of-arrays
-// ([edge][triangle]) for coalesced 64-lane loads.
+// A handle owns all device buffers of one or more scenarios (independent
+// meshes stepped together: an ensemble). Each scenario's triangles are
+// renumbered once at creation (Reverse-Cuthill-McKee on the triangle
+// adjacency) so that neighbour gathers stay within a narrow index window; the
+// permutation is applied on upload and undone on download, so callers only
+// ever see the reference's order. Scenarios sit back to back in one internal
+// numbering, each starting on a workgroup boundary, so that every workgroup
+// belongs to exactly one scenario and reads that scenario's control block.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,11 +39,15 @@ const int kTimerRing = 64;
 }  // namespace
 
 struct anuga_domain {
-    int N = 0, M = 0, nblocks = 0;
+    int N = 0, M = 0, nblocks = 0;   // N: internal triangles incl. padding; M: boundary edges
+    int S = 1;                       // scenarios
     anuga_params P{};
     hipStream_t stream = nullptr;
-    std::vector<int> perm;   // perm[new] = old
-    std::vector<int> iperm;  // iperm[old] = new
+    // per scenario: first internal triangle, triangles, first boundary edge, boundary edges
+    std::vector<int> kbeg, ntri, mbeg, nbnd;
+    std::vector<char> mesh_set, bnd_set;
+    int meshes_missing = 0, boundaries_missing = 0;
+    std::vector<int> perm;   // perm[internal] = caller's index within its scenario (-1: padding)
 
     // static mesh records, internal order
     int4 *nb = nullptr;
@@ -49,31 +55,32 @@ struct anuga_domain {
     void *gx = nullptr, *gf = nullptr;   // DeGeoX64/F64 or DeGeoX32/F32
     double *area = nullptr;
     double *nx = nullptr, *ny = nullptr, *len = nullptr, *radius = nullptr;  // FP64 flux geometry
+    int *scen = nullptr;                 // [nblocks] scenario of each block (S > 1)
     bool mixed = false;
-    std::vector<DeGeoF32> gf32;   // host copy of the mixed flux geometry (bed offsets change)
     // quantities
     double *eta = nullptr;
     DeQ *A = nullptr;   // step state / rk2 backup: {w, uh, vh, z}
     DeQ *B = nullptr;   // rk2 substep state
-    void *E = nullptr;  // 3N DeEdge4<double> or DeEdge4<float>
+    void *E = nullptr;  // FP64: w,h,uh,vh [4][3][N]; mixed: DeEdgeRec32 [N]
     unsigned char *flags = nullptr;
     double *ex[3] = {nullptr, nullptr, nullptr};
     signed char *bkind = nullptr;
     double *bval = nullptr;
     double *part = nullptr;   // [3][nblocks]: dt, bflux, mass
-    DeControl *ctl = nullptr; // device step control
+    DeControl *ctl = nullptr; // [S] device step control
     unsigned long long *dneg = nullptr;
-    DeControl *hctl = nullptr;          // pinned host mirror of *ctl
+    DeControl *hctl = nullptr;          // [S] pinned host mirror of ctl
     unsigned long long *hneg = nullptr; // pinned host mirror of *dneg
-    double *hist = nullptr;             // device timestep history of a step run
+    DeBatchSetup *dsetup = nullptr;     // [S] time control of a step run
+    DeBatchSetup *hsetup = nullptr;     // [S] pinned staging of dsetup
+    double *hist = nullptr;             // device timestep history of a step run (S == 1)
     int64_t hist_cap = 0;
-    bool done_on_device = false;        // a step run may have left ctl->done set
+    bool done_on_device = false;        // a step run may have left done flags set
     hipGraph_t graph = nullptr;         // one captured device-controlled step
     hipGraphExec_t graph_exec = nullptr;
     bool use_graph = true;
 
     int substate = 0;         // 0: at a step boundary (state in A); 1: rk2 substep state in B
-    bool boundary_set = false;
     unsigned long long neg_seen = 0;
     std::string err;
 
@@ -128,8 +135,6 @@ static std::vector<int> rcm_order(int N, const int64_t *neighbours)
         queue.push_back(root);
         level[root] = 0;
         size_t head = 0;
-        int last = root;
-        ecc = 0;
         while (head < queue.size()) {
             int u = queue[head++];
             for (int i = 0; i < 3; i++) {
@@ -148,23 +153,21 @@ static std::vector<int> rcm_order(int N, const int64_t *neighbours)
             if (deg[u] < deg[best] || (deg[u] == deg[best] && u < best)) best = u;
         }
         for (int u : queue) level[u] = -1;
-        last = best;
-        return last;
+        return best;
     };
 
     for (int seed = 0; seed < N; seed++) {
         if (placed[seed]) continue;
-        int root = seed, ecc = 0;
-        int cand = bfs_last_level(root, ecc);
+        int ecc = 0;
+        int cand = bfs_last_level(seed, ecc);
         for (int it = 0; it < 8; it++) {
             int ecc2 = 0;
             int c2 = bfs_last_level(cand, ecc2);
             if (ecc2 <= ecc) break;
-            root = cand;
             cand = c2;
             ecc = ecc2;
         }
-        root = cand;
+        const int root = cand;
         // Cuthill-McKee from root
         size_t head = order.size();
         order.push_back(root);
@@ -275,6 +278,7 @@ static DeMesh dev_mesh(const anuga_domain *d)
 {
     DeMesh m;
     m.N = d->N;
+    m.scen = d->scen;
     m.nb = d->nb;
     m.cen = d->cen;
     m.gx = d->gx;
@@ -287,7 +291,12 @@ static DeMesh dev_mesh(const anuga_domain *d)
     return m;
 }
 
-static int launch_extrapolate(anuga_domain *d, DeQ *S)
+static dim3 scenario_grid(const anuga_domain *d)
+{
+    return dim3((d->S + 63) / 64);
+}
+
+static void launch_extrapolate(anuga_domain *d, DeQ *S)
 {
     timer_begin(d, T_EXTRAP);
     if (d->mixed)
@@ -299,7 +308,6 @@ static int launch_extrapolate(anuga_domain *d, DeQ *S)
                            dev_mesh(d), dev_params(d->P), S, d->E, d->flags,
                            d->part + 2 * (size_t)d->nblocks, d->ctl);
     timer_end(d, T_EXTRAP);
-    return ANUGA_OK;
 }
 
 static void launch_materialize(anuga_domain *d, DeQ *S)
@@ -324,7 +332,7 @@ static void launch_flux_t(anuga_domain *d, DeQ *S, DeQ *bk)
                            d->part + d->nblocks, d->dneg);
 }
 
-// substep 0: explicit updates; substep 1: advance S with ctl->dt into the backup `bk`
+// substep 0: explicit updates; substep 1: advance S with the scenario's dt into the backup `bk`
 static void launch_flux(anuga_domain *d, int substep, DeQ *S, DeQ *bk)
 {
     timer_begin(d, substep ? T_FLUX1 : T_FLUX0);
@@ -342,61 +350,63 @@ static void launch_update(anuga_domain *d, DeQ *S, DeQ *out)
 {
     timer_begin(d, T_UPDATE);
     if (d->mixed)
-        hipLaunchKernelGGL(de_update<false>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N,
-                           dev_params(d->P), S, d->eta, d->flags, d->ex[0], d->ex[1], d->ex[2],
-                           d->ctl, out, d->dneg);
+        hipLaunchKernelGGL(de_update<false>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
+                           d->scen, dev_params(d->P), S, d->eta, d->flags, d->ex[0], d->ex[1],
+                           d->ex[2], d->ctl, out, d->dneg);
     else
-        hipLaunchKernelGGL(de_update<true>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N,
-                           dev_params(d->P), S, d->eta, d->flags, d->ex[0], d->ex[1], d->ex[2],
-                           d->ctl, out, d->dneg);
+        hipLaunchKernelGGL(de_update<true>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
+                           d->scen, dev_params(d->P), S, d->eta, d->flags, d->ex[0], d->ex[1],
+                           d->ex[2], d->ctl, out, d->dneg);
     timer_end(d, T_UPDATE);
 }
 
 static void launch_reduce(anuga_domain *d, int mode)
 {
-    const double *pdt = mode == 0 ? d->part : nullptr;
-    const double *pbf = mode <= 1 ? d->part + d->nblocks : nullptr;
-    const double *pm = (mode == 0 || mode >= 2) ? d->part + 2 * (size_t)d->nblocks : nullptr;
-    hipLaunchKernelGGL(de_reduce, dim3(1), dim3(1024), 0, d->stream, d->nblocks, pdt, pbf, pm,
-                       d->ctl, mode);
+    const double *pdt = d->part, *pbf = d->part + d->nblocks;
+    const double *pm = d->part + 2 * (size_t)d->nblocks;
+    void (*k)(const double *, const double *, const double *, DeControl *) =
+        mode == 0 ? de_reduce<0> : mode == 1 ? de_reduce<1> : mode == 2 ? de_reduce<2> : de_reduce<3>;
+    hipLaunchKernelGGL(k, dim3(d->S), dim3(1024), 0, d->stream, pdt, pbf, pm, d->ctl);
 }
 
-static void launch_single(anuga_domain *d, void (*k)(DeControl *))
+static void launch_control(anuga_domain *d, void (*k)(DeControl *, int))
 {
-    hipLaunchKernelGGL(k, dim3(1), dim3(64), 0, d->stream, d->ctl);
+    hipLaunchKernelGGL(k, scenario_grid(d), dim3(64), 0, d->stream, d->ctl, d->S);
 }
 
 static void launch_volume(anuga_domain *d, DeQ *S)
 {
-    hipLaunchKernelGGL(de_volume, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->N, S, d->area,
-                       d->part + 2 * (size_t)d->nblocks);
+    hipLaunchKernelGGL(de_volume, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, d->scen, d->ctl,
+                       S, d->area, d->part + 2 * (size_t)d->nblocks);
 }
 
 // leave device-controlled mode before host-controlled launches
 static void ensure_live(anuga_domain *d)
 {
     if (d->done_on_device) {
-        launch_single(d, de_clear_done);
+        launch_control(d, de_clear_done);
         d->done_on_device = false;
     }
 }
 
-// One device-controlled step: both substeps, the timestep on the device and
-// the end-of-step bookkeeping. Every kernel returns at once when ctl->done.
+// One device-controlled step of every scenario: both substeps, the timestep
+// on the device and the end-of-step bookkeeping. Every kernel returns at once
+// for a scenario whose done flag is set.
 static void enqueue_device_step(anuga_domain *d)
 {
     const bool rk2 = d->P.timestep_fluxcalls == 2;
     launch_extrapolate(d, d->A);
     launch_flux(d, 0, d->A, nullptr);
     launch_reduce(d, 0);
-    launch_single(d, de_timestep);
+    launch_control(d, de_timestep);
     launch_update(d, d->A, rk2 ? d->B : d->A);
     if (rk2) {
         launch_extrapolate(d, d->B);
         launch_flux(d, 1, d->B, d->A);
         launch_reduce(d, 1);
     }
-    hipLaunchKernelGGL(de_finish_step, dim3(1), dim3(64), 0, d->stream, d->ctl, d->hist);
+    hipLaunchKernelGGL(de_finish_step, scenario_grid(d), dim3(64), 0, d->stream, d->ctl, d->S,
+                       d->S == 1 ? d->hist : nullptr);
 }
 
 static void drop_graph(anuga_domain *d)
@@ -407,18 +417,20 @@ static void drop_graph(anuga_domain *d)
     d->graph = nullptr;
 }
 
-// read the control block and counters; raise a pending device error
+// read the control blocks and counters; raise a pending device error
 static int sync_scalars(anuga_domain *d)
 {
-    HIPCHK(d, hipMemcpyAsync(d->hctl, d->ctl, sizeof(DeControl), hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipMemcpyAsync(d->hctl, d->ctl, d->S * sizeof(DeControl), hipMemcpyDeviceToHost,
+                             d->stream));
     HIPCHK(d, hipMemcpyAsync(d->hneg, d->dneg, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                              d->stream));
     HIPCHK(d, hipStreamSynchronize(d->stream));
-    const int e = d->hctl->status;
+    int e = 0;
+    for (int s = 0; s < d->S; s++) e |= d->hctl[s].status;
     if (e) {
-        hipMemsetAsync(&d->ctl->status, 0, sizeof(int), d->stream);
+        launch_control(d, de_clear_status);
         hipStreamSynchronize(d->stream);
-        d->hctl->status = 0;
+        for (int s = 0; s < d->S; s++) d->hctl[s].status = 0;
         if (e & DE_ERR_SEMI_IMPLICIT)
             return fail(d, ANUGA_E_SEMI_IMPLICIT, "update: division by zero in semi implicit update");
         if (e & DE_ERR_NO_NEIGHBOUR) return fail(d, ANUGA_E_NO_NEIGHBOUR, "Internal neighbour not found");
@@ -439,70 +451,63 @@ static int launch_check(anuga_domain *d)
 }
 
 // ---------------------------------------------------------------------------
-// C ABI
+// creation
 // ---------------------------------------------------------------------------
-extern "C" {
+extern "C" void anuga_destroy(anuga_domain *d);
 
-const char *anuga_last_error(const anuga_domain *d)
+// Allocate the buffers of S scenarios of ntri[s] triangles and nbnd[s]
+// boundary edges; no mesh is uploaded yet. On failure *out stays NULL and
+// the message is the create error.
+static int domain_alloc(const anuga_params *params, int64_t S64, const int64_t *ntri,
+                        const int64_t *nbnd, anuga_domain **out)
 {
-    return d ? d->err.c_str() : g_create_error.c_str();
-}
-
-int anuga_create(const anuga_mesh *mesh, const anuga_params *params, anuga_domain **out)
-{
-    if (!out) return fail(nullptr, ANUGA_E_ARG, "out is NULL");
     *out = nullptr;
-    if (!mesh) return fail(nullptr, ANUGA_E_ARG, "mesh is NULL");
     int rc = check_params(nullptr, params);
     if (rc) return rc;
-    const int64_t N64 = mesh->number_of_elements, M64 = mesh->boundary_length;
-    if (N64 <= 0 || N64 >= (int64_t(1) << 29))
-        return fail(nullptr, ANUGA_E_ARG, "number_of_elements must be in [1, 2^29)");
-    if (M64 < 0 || M64 >= (int64_t(1) << 30)) return fail(nullptr, ANUGA_E_ARG, "bad boundary_length");
-    const int N = (int)N64, M = (int)M64;
-    const int64_t *nbr = mesh->neighbours, *nbe = mesh->neighbour_edges;
-    const int64_t *sur = mesh->surrogate_neighbours, *nob = mesh->number_of_boundaries;
-    if (!nbr || !nbe || !sur || !nob || !mesh->normals || !mesh->edgelengths || !mesh->radii ||
-        !mesh->areas || !mesh->centroid_coordinates || !mesh->edge_coordinates)
-        return fail(nullptr, ANUGA_E_ARG, "a mesh array is NULL");
-
-    // validate connectivity: the device derives surrogate neighbours and the
-    // boundary count from `neighbours`, which requires no internal boundaries
-    for (int k = 0; k < N; k++) {
-        int nb = 0;
-        for (int i = 0; i < 3; i++) {
-            const int64_t ki = 3 * (int64_t)k + i, n = nbr[ki];
-            if (n >= 0) {
-                if (n >= N) return fail(nullptr, ANUGA_E_ARG, "neighbour index out of range");
-                const int64_t m = nbe[ki];
-                if (m < 0 || m > 2 || nbr[3 * n + m] != k)
-                    return fail(nullptr, ANUGA_E_ARG, "inconsistent neighbour_edges");
-                if (sur[ki] != n)
-                    return fail(nullptr, ANUGA_E_ARG, "internal boundary edges are not supported");
-            } else {
-                if (-n - 1 >= M) return fail(nullptr, ANUGA_E_ARG, "boundary index out of range");
-                if (sur[ki] != k)
-                    return fail(nullptr, ANUGA_E_ARG, "internal boundary edges are not supported");
-                nb++;
-            }
-        }
-        if (nob[k] != nb) return fail(nullptr, ANUGA_E_ARG, "number_of_boundaries inconsistent");
+    if (S64 < 1 || S64 >= (int64_t(1) << 24)) return fail(nullptr, ANUGA_E_ARG, "bad scenario count");
+    const int S = (int)S64;
+    int64_t total = 0, mtotal = 0;
+    std::vector<int> kbeg(S), mbeg(S);
+    for (int s = 0; s < S; s++) {
+        if (ntri[s] <= 0 || ntri[s] >= (int64_t(1) << 29))
+            return fail(nullptr, ANUGA_E_ARG, "number_of_elements must be in [1, 2^29)");
+        if (nbnd[s] < 0 || nbnd[s] >= (int64_t(1) << 30))
+            return fail(nullptr, ANUGA_E_ARG, "bad boundary_length");
+        // every scenario starts on a workgroup boundary
+        total = (total + DE_BLOCK - 1) / DE_BLOCK * DE_BLOCK;
+        kbeg[s] = (int)total;
+        mbeg[s] = (int)mtotal;
+        total += ntri[s];
+        mtotal += nbnd[s];
     }
+    if (total >= (int64_t(1) << 29) || mtotal >= (int64_t(1) << 30))
+        return fail(nullptr, ANUGA_E_ARG, "too many triangles for one handle (2^29)");
 
     anuga_domain *d = new anuga_domain();
-    d->N = N;
-    d->M = M;
+    d->S = S;
+    d->N = (int)total;
+    d->M = (int)mtotal;
     d->P = *params;
-    d->nblocks = (N + DE_BLOCK - 1) / DE_BLOCK;
-
-    d->perm.resize(N);
-    d->iperm.resize(N);
-    if (params->reorder) {
-        d->perm = rcm_order(N, nbr);
-    } else {
-        for (int k = 0; k < N; k++) d->perm[k] = k;
+    d->nblocks = (d->N + DE_BLOCK - 1) / DE_BLOCK;
+    d->kbeg = kbeg;
+    d->mbeg = mbeg;
+    d->ntri.resize(S);
+    d->nbnd.resize(S);
+    for (int s = 0; s < S; s++) {
+        d->ntri[s] = (int)ntri[s];
+        d->nbnd[s] = (int)nbnd[s];
     }
-    for (int p = 0; p < N; p++) d->iperm[d->perm[p]] = p;
+    d->mesh_set.assign(S, 0);
+    d->meshes_missing = S;
+    // a scenario without boundary edges needs no boundary conditions
+    d->bnd_set.assign(S, 0);
+    d->boundaries_missing = 0;
+    for (int s = 0; s < S; s++) {
+        d->bnd_set[s] = d->nbnd[s] == 0;
+        if (!d->bnd_set[s]) d->boundaries_missing++;
+    }
+    d->perm.assign(d->N, -1);
+    const int N = d->N, M = d->M;
 
 #define DCHK(call)                                                                       \
     do {                                                                                 \
@@ -540,74 +545,35 @@ int anuga_create(const anuga_mesh *mesh, const anuga_params *params, anuga_domai
     DCHK(dalloc(&d->bkind, M));
     DCHK(dalloc(&d->bval, 3 * (size_t)M));
     DCHK(dalloc(&d->part, 3 * (size_t)d->nblocks));
-    DCHK(dalloc(&d->ctl, 1));
+    DCHK(dalloc(&d->ctl, S));
+    DCHK(dalloc(&d->dsetup, S));
     DCHK(dalloc(&d->dneg, 1));
-    DCHK(hipHostMalloc((void **)&d->hctl, sizeof(DeControl), hipHostMallocDefault));
+    DCHK(hipHostMalloc((void **)&d->hctl, S * sizeof(DeControl), hipHostMallocDefault));
+    DCHK(hipHostMalloc((void **)&d->hsetup, S * sizeof(DeBatchSetup), hipHostMallocDefault));
     DCHK(hipHostMalloc((void **)&d->hneg, sizeof(unsigned long long), hipHostMallocDefault));
+    if (S > 1) {
+        DCHK(dalloc(&d->scen, d->nblocks));
+        std::vector<int> bs(d->nblocks, 0);
+        for (int s = 0; s < S; s++) {
+            const int b0 = d->kbeg[s] / DE_BLOCK, b1 = (d->kbeg[s] + d->ntri[s] + DE_BLOCK - 1) / DE_BLOCK;
+            for (int b = b0; b < b1; b++) bs[b] = s;
+        }
+        DCHK(hipMemcpy(d->scen, bs.data(), d->nblocks * sizeof(int), hipMemcpyHostToDevice));
+    }
 
-    {
-        const double *nrm = mesh->normals, *el = mesh->edgelengths;
-        const double *cen = mesh->centroid_coordinates, *em = mesh->edge_coordinates;
-        std::vector<int4> hn(N);
-        std::vector<double2> hc(N);
-        std::vector<double> ha(N);
-        for (int p = 0; p < N; p++) {
-            const int64_t k = d->perm[p];
-            int v[3];
-            for (int i = 0; i < 3; i++) {
-                const int64_t n = nbr[3 * k + i];
-                v[i] = n >= 0 ? ((d->iperm[n] << 2) | (int)nbe[3 * k + i]) : (int)n;
-            }
-            hn[p] = make_int4(v[0], v[1], v[2], (int)nob[k]);
-            hc[p] = make_double2(cen[2 * k], cen[2 * k + 1]);
-            ha[p] = mesh->areas[k];
-        }
-        DCHK(hipMemcpy(d->nb, hn.data(), N * sizeof(int4), hipMemcpyHostToDevice));
-        DCHK(hipMemcpy(d->cen, hc.data(), N * sizeof(double2), hipMemcpyHostToDevice));
-        DCHK(hipMemcpy(d->area, ha.data(), N * sizeof(double), hipMemcpyHostToDevice));
-        if (!d->mixed) {
-            std::vector<DeGeoX64> gx(N);
-            std::vector<double> a(n3), b(n3), c(n3), r(N);
-            for (int p = 0; p < N; p++) {
-                const int64_t k = d->perm[p];
-                for (int i = 0; i < 3; i++) {
-                    // xv - x and yv - y exactly as sw_domain_orig.c:2130-2135 forms them
-                    gx[p].dxv[i] = em[6 * k + 2 * i] - cen[2 * k];
-                    gx[p].dyv[i] = em[6 * k + 2 * i + 1] - cen[2 * k + 1];
-                    a[(size_t)i * N + p] = nrm[6 * k + 2 * i];
-                    b[(size_t)i * N + p] = nrm[6 * k + 2 * i + 1];
-                    c[(size_t)i * N + p] = el[3 * k + i];
-                }
-                r[p] = mesh->radii[k];
-            }
-            DCHK(hipMemcpy(d->gx, gx.data(), N * sizeof(DeGeoX64), hipMemcpyHostToDevice));
-            DCHK(hipMemcpy(d->nx, a.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
-            DCHK(hipMemcpy(d->ny, b.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
-            DCHK(hipMemcpy(d->len, c.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
-            DCHK(hipMemcpy(d->radius, r.data(), N * sizeof(double), hipMemcpyHostToDevice));
-        } else {
-            std::vector<DeGeoX32> gx(N);
-            std::vector<DeGeoF32> gf(N);
-            for (int p = 0; p < N; p++) {
-                const int64_t k = d->perm[p];
-                for (int i = 0; i < 3; i++) {
-                    gx[p].dxv[i] = (float)(em[6 * k + 2 * i] - cen[2 * k]);
-                    gx[p].dyv[i] = (float)(em[6 * k + 2 * i + 1] - cen[2 * k + 1]);
-                    const int64_t s = sur[3 * k + i];
-                    gx[p].rx[i] = (float)(cen[2 * s] - cen[2 * k]);
-                    gx[p].ry[i] = (float)(cen[2 * s + 1] - cen[2 * k + 1]);
-                    gf[p].nx[i] = (float)nrm[6 * k + 2 * i];
-                    gf[p].ny[i] = (float)nrm[6 * k + 2 * i + 1];
-                    gf[p].len[i] = (float)el[3 * k + i];
-                }
-                gf[p].radius = (float)mesh->radii[k];
-            }
-            for (int p = 0; p < N; p++)
-                for (int i = 0; i < 3; i++) gf[p].dzc[i] = gf[p].pad[i] = 0.f;
-            DCHK(hipMemcpy(d->gx, gx.data(), N * sizeof(DeGeoX32), hipMemcpyHostToDevice));
-            DCHK(hipMemcpy(d->gf, gf.data(), N * sizeof(DeGeoF32), hipMemcpyHostToDevice));
-            d->gf32 = std::move(gf);
-        }
+    // zero state, geometry and partials; the padding triangles between
+    // scenarios stay zero and no kernel computes on them
+    DCHK(hipMemset(d->nb, 0, N * sizeof(int4)));
+    DCHK(hipMemset(d->cen, 0, N * sizeof(double2)));
+    DCHK(hipMemset(d->area, 0, N * sizeof(double)));
+    DCHK(hipMemset(d->gx, 0, N * (d->mixed ? sizeof(DeGeoX32) : sizeof(DeGeoX64))));
+    if (d->mixed) {
+        DCHK(hipMemset(d->gf, 0, N * sizeof(DeGeoF32)));
+    } else {
+        DCHK(hipMemset(d->nx, 0, n3 * sizeof(double)));
+        DCHK(hipMemset(d->ny, 0, n3 * sizeof(double)));
+        DCHK(hipMemset(d->len, 0, n3 * sizeof(double)));
+        DCHK(hipMemset(d->radius, 0, N * sizeof(double)));
     }
     DCHK(hipMemset(d->eta, 0, N * sizeof(double)));
     DCHK(hipMemset(d->A, 0, N * sizeof(DeQ)));
@@ -617,9 +583,18 @@ int anuga_create(const anuga_mesh *mesh, const anuga_params *params, anuga_domai
     DCHK(hipMemset(d->flags, DE_F_KEEP, N));
     DCHK(hipMemset(d->bkind, 0, std::max(M, 1)));
     DCHK(hipMemset(d->bval, 0, 3 * (size_t)std::max(M, 1) * sizeof(double)));
-    DCHK(hipMemset(d->ctl, 0, sizeof(DeControl)));
+    DCHK(hipMemset(d->part, 0, 3 * (size_t)d->nblocks * sizeof(double)));
     DCHK(hipMemset(d->dneg, 0, sizeof(unsigned long long)));
-    std::memset(d->hctl, 0, sizeof(DeControl));
+    std::memset(d->hctl, 0, S * sizeof(DeControl));
+    std::memset(d->hsetup, 0, S * sizeof(DeBatchSetup));
+    for (int s = 0; s < S; s++) {
+        DeControl &c = d->hctl[s];
+        c.kbeg = d->kbeg[s];
+        c.kend = d->kbeg[s] + d->ntri[s];
+        c.bbeg = d->kbeg[s] / DE_BLOCK;
+        c.bend = (c.kend + DE_BLOCK - 1) / DE_BLOCK;
+    }
+    DCHK(hipMemcpy(d->ctl, d->hctl, S * sizeof(DeControl), hipMemcpyHostToDevice));
     *d->hneg = 0;
     {
         const char *g = std::getenv("ANUGA_HIP_GRAPH");
@@ -631,18 +606,413 @@ int anuga_create(const anuga_mesh *mesh, const anuga_params *params, anuga_domai
     return ANUGA_OK;
 }
 
+// Check a mesh of N triangles and M boundary edges (the reference's Mesh
+// arrays) on the host. Failures are reported on d (the create error if NULL).
+static int validate_mesh(anuga_domain *d, const anuga_mesh *mesh, int64_t N, int64_t M)
+{
+    if (!mesh) return fail(d, ANUGA_E_ARG, "mesh is NULL");
+    if (mesh->number_of_elements != N || mesh->boundary_length != M)
+        return fail(d, ANUGA_E_ARG, "mesh size differs from the size given at creation");
+    const int64_t *nbr = mesh->neighbours, *nbe = mesh->neighbour_edges;
+    const int64_t *sur = mesh->surrogate_neighbours, *nob = mesh->number_of_boundaries;
+    if (!nbr || !nbe || !sur || !nob || !mesh->normals || !mesh->edgelengths || !mesh->radii ||
+        !mesh->areas || !mesh->centroid_coordinates || !mesh->edge_coordinates)
+        return fail(d, ANUGA_E_ARG, "a mesh array is NULL");
+
+    // connectivity: the device derives surrogate neighbours and the boundary
+    // count from `neighbours`, which requires no internal boundaries
+    for (int64_t k = 0; k < N; k++) {
+        int nb = 0;
+        for (int i = 0; i < 3; i++) {
+            const int64_t ki = 3 * k + i, n = nbr[ki];
+            if (n >= 0) {
+                if (n >= N) return fail(d, ANUGA_E_ARG, "neighbour index out of range");
+                const int64_t m = nbe[ki];
+                if (m < 0 || m > 2 || nbr[3 * n + m] != k)
+                    return fail(d, ANUGA_E_ARG, "inconsistent neighbour_edges");
+                if (sur[ki] != n)
+                    return fail(d, ANUGA_E_ARG, "internal boundary edges are not supported");
+            } else {
+                if (-n - 1 >= M) return fail(d, ANUGA_E_ARG, "boundary index out of range");
+                if (sur[ki] != k)
+                    return fail(d, ANUGA_E_ARG, "internal boundary edges are not supported");
+                nb++;
+            }
+        }
+        if (nob[k] != nb) return fail(d, ANUGA_E_ARG, "number_of_boundaries inconsistent");
+    }
+    return ANUGA_OK;
+}
+
+// Upload the validated mesh of scenario s (its own triangle and boundary
+// numbering, as the reference's Mesh holds it).
+static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
+{
+    const int N = d->ntri[s], kb = d->kbeg[s], mb = d->mbeg[s];
+    const int64_t *nbr = mesh->neighbours, *nbe = mesh->neighbour_edges;
+    const int64_t *sur = mesh->surrogate_neighbours, *nob = mesh->number_of_boundaries;
+    std::vector<int> perm(N), iperm(N);
+    if (d->P.reorder) perm = rcm_order(N, nbr);
+    else
+        for (int k = 0; k < N; k++) perm[k] = k;
+    for (int p = 0; p < N; p++) iperm[perm[p]] = p;
+
+    const double *nrm = mesh->normals, *el = mesh->edgelengths;
+    const double *cen = mesh->centroid_coordinates, *em = mesh->edge_coordinates;
+    std::vector<int4> hn(N);
+    std::vector<double2> hc(N);
+    std::vector<double> ha(N);
+    for (int p = 0; p < N; p++) {
+        const int64_t k = perm[p];
+        int v[3];
+        for (int i = 0; i < 3; i++) {
+            const int64_t n = nbr[3 * k + i];
+            v[i] = n >= 0 ? (((kb + iperm[n]) << 2) | (int)nbe[3 * k + i]) : (int)(n - mb);
+        }
+        hn[p] = make_int4(v[0], v[1], v[2], (int)nob[k]);
+        hc[p] = make_double2(cen[2 * k], cen[2 * k + 1]);
+        ha[p] = mesh->areas[k];
+    }
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    HIPCHK(d, hipMemcpy(d->nb + kb, hn.data(), N * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(d, hipMemcpy(d->cen + kb, hc.data(), N * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(d, hipMemcpy(d->area + kb, ha.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    if (!d->mixed) {
+        const size_t NN = d->N;
+        std::vector<DeGeoX64> gx(N);
+        std::vector<double> a(3 * (size_t)N), b(3 * (size_t)N), c(3 * (size_t)N), r(N);
+        for (int p = 0; p < N; p++) {
+            const int64_t k = perm[p];
+            for (int i = 0; i < 3; i++) {
+                // xv - x and yv - y exactly as sw_domain_orig.c:2130-2135 forms them
+                gx[p].dxv[i] = em[6 * k + 2 * i] - cen[2 * k];
+                gx[p].dyv[i] = em[6 * k + 2 * i + 1] - cen[2 * k + 1];
+                a[(size_t)i * N + p] = nrm[6 * k + 2 * i];
+                b[(size_t)i * N + p] = nrm[6 * k + 2 * i + 1];
+                c[(size_t)i * N + p] = el[3 * k + i];
+            }
+            r[p] = mesh->radii[k];
+        }
+        HIPCHK(d, hipMemcpy((DeGeoX64 *)d->gx + kb, gx.data(), N * sizeof(DeGeoX64),
+                            hipMemcpyHostToDevice));
+        for (int i = 0; i < 3; i++) {
+            const size_t o = (size_t)i * NN + kb, h = (size_t)i * N;
+            HIPCHK(d, hipMemcpy(d->nx + o, a.data() + h, N * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(d, hipMemcpy(d->ny + o, b.data() + h, N * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(d, hipMemcpy(d->len + o, c.data() + h, N * sizeof(double), hipMemcpyHostToDevice));
+        }
+        HIPCHK(d, hipMemcpy(d->radius + kb, r.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<DeGeoX32> gx(N);
+        std::vector<DeGeoF32> gf(N);
+        for (int p = 0; p < N; p++) {
+            const int64_t k = perm[p];
+            for (int i = 0; i < 3; i++) {
+                gx[p].dxv[i] = (float)(em[6 * k + 2 * i] - cen[2 * k]);
+                gx[p].dyv[i] = (float)(em[6 * k + 2 * i + 1] - cen[2 * k + 1]);
+                const int64_t sn = sur[3 * k + i];
+                gx[p].rx[i] = (float)(cen[2 * sn] - cen[2 * k]);
+                gx[p].ry[i] = (float)(cen[2 * sn + 1] - cen[2 * k + 1]);
+                gf[p].nx[i] = (float)nrm[6 * k + 2 * i];
+                gf[p].ny[i] = (float)nrm[6 * k + 2 * i + 1];
+                gf[p].len[i] = (float)el[3 * k + i];
+                gf[p].dzc[i] = gf[p].pad[i] = 0.f;
+            }
+            gf[p].radius = (float)mesh->radii[k];
+        }
+        HIPCHK(d, hipMemcpy((DeGeoX32 *)d->gx + kb, gx.data(), N * sizeof(DeGeoX32),
+                            hipMemcpyHostToDevice));
+        HIPCHK(d, hipMemcpy((DeGeoF32 *)d->gf + kb, gf.data(), N * sizeof(DeGeoF32),
+                            hipMemcpyHostToDevice));
+    }
+    std::copy(perm.begin(), perm.end(), d->perm.begin() + kb);
+    if (!d->mesh_set[s]) {
+        d->mesh_set[s] = 1;
+        d->meshes_missing--;
+    }
+    return ANUGA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-scenario state access
+// ---------------------------------------------------------------------------
+// component of the DeQ record holding a quantity (-1: not in the record)
+static int record_slot(int q)
+{
+    switch (q) {
+    case ANUGA_STAGE: return 0;
+    case ANUGA_XMOMENTUM: return 1;
+    case ANUGA_YMOMENTUM: return 2;
+    case ANUGA_ELEVATION: return 3;
+    default: return -1;
+    }
+}
+
+static int download_records(anuga_domain *d, const DeQ *src, int kb, int n, std::vector<DeQ> &h)
+{
+    h.resize(n);
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    HIPCHK(d, hipMemcpy(h.data(), src + kb, n * sizeof(DeQ), hipMemcpyDeviceToHost));
+    return ANUGA_OK;
+}
+
+static int scenario_ok(anuga_domain *d, int64_t s)
+{
+    if (s < 0 || s >= d->S) return fail(d, ANUGA_E_ARG, "scenario index out of range");
+    if (!d->mesh_set[s]) return fail(d, ANUGA_E_STATE, "the scenario's mesh is not set");
+    return ANUGA_OK;
+}
+
+static int set_centroids(anuga_domain *d, int s, int quantity, const double *values)
+{
+    if (!values) return fail(d, ANUGA_E_ARG, "values is NULL");
+    const int N = d->ntri[s], kb = d->kbeg[s];
+    const int *perm = d->perm.data() + kb;
+    if (quantity == ANUGA_FRICTION) {
+        std::vector<double> t(N);
+        for (int p = 0; p < N; p++) t[p] = values[perm[p]];
+        HIPCHK(d, hipStreamSynchronize(d->stream));
+        HIPCHK(d, hipMemcpy(d->eta + kb, t.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        return ANUGA_OK;
+    }
+    const int slot = record_slot(quantity);
+    if (slot < 0) return fail(d, ANUGA_E_ARG, "quantity cannot be set");
+    // the bed is part of both state buffers; the others of the current one
+    DeQ *bufs[2] = {d->substate ? d->B : d->A, slot == 3 ? (d->substate ? d->A : d->B) : nullptr};
+    for (DeQ *buf : bufs) {
+        if (!buf) continue;
+        std::vector<DeQ> h;
+        int rc = download_records(d, buf, kb, N, h);
+        if (rc) return rc;
+        for (int p = 0; p < N; p++) (&h[p].w)[slot] = values[perm[p]];
+        HIPCHK(d, hipMemcpy(buf + kb, h.data(), N * sizeof(DeQ), hipMemcpyHostToDevice));
+    }
+    if (slot == 3 && d->mixed) {
+        // bed offsets of the mixed flux's local datum
+        hipLaunchKernelGGL(de_bed_offsets, dim3((N + DE_BLOCK - 1) / DE_BLOCK), dim3(DE_BLOCK), 0,
+                           d->stream, kb, kb + N, d->nb, d->A, (DeGeoF32 *)d->gf);
+        int rc = launch_check(d);
+        if (rc) return rc;
+        HIPCHK(d, hipStreamSynchronize(d->stream));
+    }
+    return ANUGA_OK;
+}
+
+static int get_centroids(anuga_domain *d, int s, int quantity, double *values)
+{
+    if (!values) return fail(d, ANUGA_E_ARG, "values is NULL");
+    const int N = d->ntri[s], kb = d->kbeg[s];
+    const int *perm = d->perm.data() + kb;
+    if (quantity == ANUGA_FRICTION) {
+        std::vector<double> t(N);
+        HIPCHK(d, hipStreamSynchronize(d->stream));
+        HIPCHK(d, hipMemcpy(t.data(), d->eta + kb, N * sizeof(double), hipMemcpyDeviceToHost));
+        for (int p = 0; p < N; p++) values[perm[p]] = t[p];
+        return ANUGA_OK;
+    }
+    const int slot = record_slot(quantity);
+    if (slot < 0 && quantity != ANUGA_HEIGHT) return fail(d, ANUGA_E_ARG, "unknown quantity");
+    std::vector<DeQ> h;
+    int rc = download_records(d, d->substate ? d->B : d->A, kb, N, h);
+    if (rc) return rc;
+    for (int p = 0; p < N; p++)
+        values[perm[p]] = slot >= 0 ? (&h[p].w)[slot] : std::fmax(h[p].w - h[p].z, 0.0);
+    return ANUGA_OK;
+}
+
+static int set_boundary(anuga_domain *d, int s, const int32_t *kind, const double *dirichlet_values)
+{
+    const int M = d->nbnd[s], mb = d->mbeg[s], MT = d->M;
+    if (M > 0 && !kind) return fail(d, ANUGA_E_ARG, "kind is NULL");
+    if (M == 0) return ANUGA_OK;
+    std::vector<signed char> k(M, 0);
+    std::vector<double> v(3 * (size_t)M, 0.0);
+    for (int j = 0; j < M; j++) {
+        if (kind[j] < 0 || kind[j] > 3) return fail(d, ANUGA_E_ARG, "unknown boundary kind");
+        k[j] = (signed char)kind[j];
+        if (kind[j] == ANUGA_BC_DIRICHLET) {
+            if (!dirichlet_values) return fail(d, ANUGA_E_ARG, "Dirichlet values are NULL");
+            for (int q = 0; q < 3; q++) v[(size_t)q * M + j] = dirichlet_values[3 * (size_t)j + q];
+        }
+    }
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    HIPCHK(d, hipMemcpy(d->bkind + mb, k.data(), M, hipMemcpyHostToDevice));
+    for (int q = 0; q < 3; q++)
+        HIPCHK(d, hipMemcpy(d->bval + (size_t)q * MT + mb, v.data() + (size_t)q * M,
+                            M * sizeof(double), hipMemcpyHostToDevice));
+    if (!d->bnd_set[s]) {
+        d->bnd_set[s] = 1;
+        d->boundaries_missing--;
+    }
+    return ANUGA_OK;
+}
+
+static int ready_to_step(anuga_domain *d)
+{
+    if (d->meshes_missing) return fail(d, ANUGA_E_STATE, "a scenario's mesh is not set");
+    if (d->boundaries_missing) return fail(d, ANUGA_E_STATE, "boundary conditions not set");
+    return ANUGA_OK;
+}
+
+static int single(anuga_domain *d)
+{
+    if (d->S != 1)
+        return fail(d, ANUGA_E_STATE, "an ensemble handle: use the anuga_ensemble_* entry points");
+    return ANUGA_OK;
+}
+
+// Up to max_steps device-controlled steps of every scenario with time control
+// tc[s] and state st[s] (see anuga_evolve_steps).
+static int evolve_steps(anuga_domain *d, int64_t max_steps, const anuga_time_control *tc,
+                        anuga_step_state *st, double *dt_history, bool stop_reached)
+{
+    if (d->substate != 0) return fail(d, ANUGA_E_STATE, "evolve_steps called inside an rk2 step");
+    int rc = ready_to_step(d);
+    if (rc) return rc;
+    if (dt_history && max_steps > d->hist_cap) {
+        if (d->hist) hipFree(d->hist);
+        d->hist = nullptr;
+        d->hist_cap = 0;
+        drop_graph(d);
+        HIPCHK(d, dalloc(&d->hist, (size_t)max_steps));
+        d->hist_cap = max_steps;
+    }
+    HIPCHK(d, hipStreamSynchronize(d->stream));   // the staging buffer is free again
+    for (int s = 0; s < d->S; s++) {
+        DeBatchSetup &b = d->hsetup[s];
+        b.t = st[s].relative_time;
+        b.rec_min = st[s].recorded_min_timestep;
+        b.rec_max = st[s].recorded_max_timestep;
+        b.cfl = tc[s].CFL;
+        b.max_dt = tc[s].evolve_max_timestep;
+        b.min_dt = tc[s].evolve_min_timestep;
+        b.finaltime = tc[s].relative_finaltime;
+        b.yieldtime = tc[s].relative_yieldtime;
+        b.fixed_flux_dt = tc[s].fixed_flux_timestep;
+        b.eps = tc[s].epsilon;
+        b.smallsteps = (int)st[s].smallsteps;
+        b.order = (int)st[s].order;
+        b.default_order = (int)tc[s].default_order;
+        b.max_smallsteps = (int)tc[s].max_smallsteps;
+        b.hist_cap = dt_history ? (int)std::min<int64_t>(max_steps, d->hist_cap) : 0;
+        b.rk2 = d->P.timestep_fluxcalls == 2;
+        b.stop_reached = stop_reached ? 1 : 0;
+    }
+    HIPCHK(d, hipMemcpyAsync(d->dsetup, d->hsetup, d->S * sizeof(DeBatchSetup),
+                             hipMemcpyHostToDevice, d->stream));
+    hipLaunchKernelGGL(de_setup_batch, scenario_grid(d), dim3(64), 0, d->stream, d->ctl, d->S,
+                       d->dsetup);
+    d->done_on_device = true;
+
+    const bool graph = d->use_graph && !d->timing;
+    if (graph && !d->graph_exec && max_steps > 0) {
+        HIPCHK(d, hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_device_step(d);
+        hipGraph_t g = nullptr;
+        HIPCHK(d, hipStreamEndCapture(d->stream, &g));
+        d->graph = g;
+        HIPCHK(d, hipGraphInstantiate(&d->graph_exec, d->graph, nullptr, nullptr, 0));
+    }
+    for (int64_t i = 0; i < max_steps; i++) {
+        if (graph) HIPCHK(d, hipGraphLaunch(d->graph_exec, d->stream));
+        else enqueue_device_step(d);
+    }
+    rc = launch_check(d);
+    if (rc) return rc;
+    rc = sync_scalars(d);
+    for (int s = 0; s < d->S; s++) {
+        const DeControl &c = d->hctl[s];
+        st[s].relative_time = c.t;
+        st[s].timestep = c.dt;
+        st[s].flux_timestep = c.flux_dt;
+        st[s].recorded_min_timestep = c.rec_min;
+        st[s].recorded_max_timestep = c.rec_max;
+        st[s].smallsteps = c.smallsteps;
+        st[s].order = c.order;
+        st[s].steps_taken = c.nsteps;
+        st[s].reached = c.done;
+    }
+    if (dt_history && d->hctl[0].nsteps > 0) {
+        HIPCHK(d, hipMemcpy(dt_history, d->hist,
+                            (size_t)std::min<int64_t>(d->hctl[0].nsteps, max_steps) * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    }
+    return rc;
+}
+
+// protect + extrapolation of every scenario, written back to the centroids
+static int distribute(anuga_domain *d)
+{
+    if (d->meshes_missing) return fail(d, ANUGA_E_STATE, "a scenario's mesh is not set");
+    ensure_live(d);
+    DeQ *S = d->substate ? d->B : d->A;
+    launch_extrapolate(d, S);
+    launch_materialize(d, S);
+    launch_reduce(d, 2);
+    int rc = launch_check(d);
+    if (rc) return rc;
+    return sync_scalars(d);
+}
+
+static int volumes(anuga_domain *d)
+{
+    DeQ *S = d->substate ? d->B : d->A;
+    launch_volume(d, S);
+    launch_reduce(d, 3);
+    int rc = launch_check(d);
+    if (rc) return rc;
+    return sync_scalars(d);
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *anuga_last_error(const anuga_domain *d)
+{
+    return d ? d->err.c_str() : g_create_error.c_str();
+}
+
+int anuga_create(const anuga_mesh *mesh, const anuga_params *params, anuga_domain **out)
+{
+    if (!out) return fail(nullptr, ANUGA_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (!mesh) return fail(nullptr, ANUGA_E_ARG, "mesh is NULL");
+    const int64_t n = mesh->number_of_elements, m = mesh->boundary_length;
+    int rc = check_params(nullptr, params);
+    if (rc) return rc;
+    if (n <= 0 || n >= (int64_t(1) << 29))
+        return fail(nullptr, ANUGA_E_ARG, "number_of_elements must be in [1, 2^29)");
+    if (m < 0 || m >= (int64_t(1) << 30)) return fail(nullptr, ANUGA_E_ARG, "bad boundary_length");
+    rc = validate_mesh(nullptr, mesh, n, m);
+    if (rc) return rc;
+    anuga_domain *d = nullptr;
+    rc = domain_alloc(params, 1, &n, &m, &d);
+    if (rc) return rc;
+    rc = upload_mesh(d, 0, mesh);
+    if (rc) {
+        const std::string msg = d->err;
+        anuga_destroy(d);
+        return fail(nullptr, rc, msg);
+    }
+    *out = d;
+    return ANUGA_OK;
+}
+
 void anuga_destroy(anuga_domain *d)
 {
     if (!d) return;
     if (d->stream) hipStreamSynchronize(d->stream);
     drop_graph(d);
     void *ptrs[] = {d->nb, d->cen, d->gx, d->gf, d->area, d->nx, d->ny, d->len, d->radius,
-                    d->eta, d->A, d->B, d->E, d->flags,
-                    d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->ctl, d->dneg,
-                    d->hist};
+                    d->scen, d->eta, d->A, d->B, d->E, d->flags,
+                    d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->ctl, d->dsetup,
+                    d->dneg, d->hist};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (d->hctl) hipHostFree(d->hctl);
+    if (d->hsetup) hipHostFree(d->hsetup);
     if (d->hneg) hipHostFree(d->hneg);
     for (int t = 0; t < T_COUNT; t++) {
         for (auto e : d->timers[t].start) hipEventDestroy(e);
@@ -666,89 +1036,27 @@ int anuga_set_params(anuga_domain *d, const anuga_params *params)
     return ANUGA_OK;
 }
 
-// component of the DeQ record holding a quantity (-1: not in the record)
-static int record_slot(int q)
-{
-    switch (q) {
-    case ANUGA_STAGE: return 0;
-    case ANUGA_XMOMENTUM: return 1;
-    case ANUGA_YMOMENTUM: return 2;
-    case ANUGA_ELEVATION: return 3;
-    default: return -1;
-    }
-}
-
-static int download_records(anuga_domain *d, DeQ *src, std::vector<DeQ> &h)
-{
-    h.resize(d->N);
-    HIPCHK(d, hipStreamSynchronize(d->stream));
-    HIPCHK(d, hipMemcpy(h.data(), src, d->N * sizeof(DeQ), hipMemcpyDeviceToHost));
-    return ANUGA_OK;
-}
-
 int anuga_set_centroid_values(anuga_domain *d, int quantity, const double *values)
 {
     if (!d || !values) return ANUGA_E_ARG;
-    const int N = d->N;
-    if (quantity == ANUGA_FRICTION) {
-        std::vector<double> t(N);
-        for (int p = 0; p < N; p++) t[p] = values[d->perm[p]];
-        HIPCHK(d, hipStreamSynchronize(d->stream));
-        HIPCHK(d, hipMemcpy(d->eta, t.data(), N * sizeof(double), hipMemcpyHostToDevice));
-        return ANUGA_OK;
-    }
-    const int slot = record_slot(quantity);
-    if (slot < 0) return fail(d, ANUGA_E_ARG, "quantity cannot be set");
-    // the bed is part of both state buffers; the others of the current one
-    DeQ *bufs[2] = {d->substate ? d->B : d->A, slot == 3 ? (d->substate ? d->A : d->B) : nullptr};
-    for (DeQ *buf : bufs) {
-        if (!buf) continue;
-        std::vector<DeQ> h;
-        int rc = download_records(d, buf, h);
-        if (rc) return rc;
-        for (int p = 0; p < N; p++) (&h[p].w)[slot] = values[d->perm[p]];
-        HIPCHK(d, hipMemcpy(buf, h.data(), N * sizeof(DeQ), hipMemcpyHostToDevice));
-    }
-    if (slot == 3 && d->mixed) {
-        // bed offsets of the mixed flux's local datum
-        std::vector<int4> nb(N);
-        HIPCHK(d, hipMemcpy(nb.data(), d->nb, N * sizeof(int4), hipMemcpyDeviceToHost));
-        for (int p = 0; p < N; p++) {
-            const int v[3] = {nb[p].x, nb[p].y, nb[p].z};
-            const double zp = values[d->perm[p]];
-            for (int i = 0; i < 3; i++)
-                d->gf32[p].dzc[i] = v[i] >= 0 ? (float)(values[d->perm[v[i] >> 2]] - zp) : 0.f;
-        }
-        HIPCHK(d, hipMemcpy(d->gf, d->gf32.data(), N * sizeof(DeGeoF32), hipMemcpyHostToDevice));
-    }
-    return ANUGA_OK;
+    int rc = single(d);
+    return rc ? rc : set_centroids(d, 0, quantity, values);
 }
 
 int anuga_get_centroid_values(anuga_domain *d, int quantity, double *values)
 {
     if (!d || !values) return ANUGA_E_ARG;
-    const int N = d->N;
-    if (quantity == ANUGA_FRICTION) {
-        std::vector<double> t(N);
-        HIPCHK(d, hipStreamSynchronize(d->stream));
-        HIPCHK(d, hipMemcpy(t.data(), d->eta, N * sizeof(double), hipMemcpyDeviceToHost));
-        for (int p = 0; p < N; p++) values[d->perm[p]] = t[p];
-        return ANUGA_OK;
-    }
-    const int slot = record_slot(quantity);
-    if (slot < 0 && quantity != ANUGA_HEIGHT) return fail(d, ANUGA_E_ARG, "unknown quantity");
-    std::vector<DeQ> h;
-    int rc = download_records(d, d->substate ? d->B : d->A, h);
-    if (rc) return rc;
-    for (int p = 0; p < N; p++)
-        values[d->perm[p]] = slot >= 0 ? (&h[p].w)[slot] : std::fmax(h[p].w - h[p].z, 0.0);
-    return ANUGA_OK;
+    int rc = single(d);
+    return rc ? rc : get_centroids(d, 0, quantity, values);
 }
 
 int anuga_get_edge_values(anuga_domain *d, int quantity, double *values)
 {
     if (!d || !values) return ANUGA_E_ARG;
-    const size_t n3 = 3 * (size_t)d->N;
+    int rc = single(d);
+    if (rc) return rc;
+    const int N = d->N;
+    const size_t n3 = 3 * (size_t)N;
     if (quantity != ANUGA_STAGE && quantity != ANUGA_HEIGHT && quantity != ANUGA_XMOMENTUM &&
         quantity != ANUGA_YMOMENTUM && quantity != ANUGA_ELEVATION)
         return fail(d, ANUGA_E_ARG, "unknown edge quantity");
@@ -757,12 +1065,12 @@ int anuga_get_edge_values(anuga_domain *d, int quantity, double *values)
     if (d->mixed) {
         // edge = centroid + jump; the centroid state is the one the last
         // extrapolation saw after it was written back (distribute)
-        std::vector<DeEdgeRec32> r(d->N);
+        std::vector<DeEdgeRec32> r(N);
         std::vector<DeQ> st;
-        HIPCHK(d, hipMemcpy(r.data(), d->E, d->N * sizeof(DeEdgeRec32), hipMemcpyDeviceToHost));
-        int rc = download_records(d, d->substate ? d->B : d->A, st);
+        HIPCHK(d, hipMemcpy(r.data(), d->E, N * sizeof(DeEdgeRec32), hipMemcpyDeviceToHost));
+        rc = download_records(d, d->substate ? d->B : d->A, 0, N, st);
         if (rc) return rc;
-        for (int p = 0; p < d->N; p++)
+        for (int p = 0; p < N; p++)
             for (int i = 0; i < 3; i++) {
                 const float4 f = r[p].e[i];
                 const double hc = r[p].hc;
@@ -777,12 +1085,11 @@ int anuga_get_edge_values(anuga_domain *d, int quantity, double *values)
         // four [edge][triangle] arrays -> (triangle, edge, quantity)
         std::vector<double> soa(4 * n3);
         HIPCHK(d, hipMemcpy(soa.data(), d->E, 4 * n3 * sizeof(double), hipMemcpyDeviceToHost));
-        for (int p = 0; p < d->N; p++)
+        for (int p = 0; p < N; p++)
             for (int i = 0; i < 3; i++)
                 for (int qq = 0; qq < 4; qq++)
-                    e[4 * (3 * (size_t)p + i) + qq] = soa[qq * n3 + (size_t)i * d->N + p];
+                    e[4 * (3 * (size_t)p + i) + qq] = soa[qq * n3 + (size_t)i * N + p];
     }
-    const int N = d->N;
     for (int p = 0; p < N; p++) {
         const int64_t k = d->perm[p];
         for (int i = 0; i < 3; i++) {
@@ -804,37 +1111,16 @@ int anuga_get_edge_values(anuga_domain *d, int quantity, double *values)
 int anuga_set_boundary(anuga_domain *d, const int32_t *kind, const double *dirichlet_values)
 {
     if (!d) return ANUGA_E_ARG;
-    const int M = d->M;
-    if (M > 0 && !kind) return fail(d, ANUGA_E_ARG, "kind is NULL");
-    std::vector<signed char> k(std::max(M, 1), 0);
-    std::vector<double> v(3 * (size_t)std::max(M, 1), 0.0);
-    for (int j = 0; j < M; j++) {
-        if (kind[j] < 0 || kind[j] > 3) return fail(d, ANUGA_E_ARG, "unknown boundary kind");
-        k[j] = (signed char)kind[j];
-        if (kind[j] == ANUGA_BC_DIRICHLET) {
-            if (!dirichlet_values) return fail(d, ANUGA_E_ARG, "Dirichlet values are NULL");
-            for (int q = 0; q < 3; q++) v[(size_t)q * M + j] = dirichlet_values[3 * (size_t)j + q];
-        }
-    }
-    HIPCHK(d, hipStreamSynchronize(d->stream));
-    HIPCHK(d, hipMemcpy(d->bkind, k.data(), k.size(), hipMemcpyHostToDevice));
-    HIPCHK(d, hipMemcpy(d->bval, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
-    d->boundary_set = true;
-    return ANUGA_OK;
+    int rc = single(d);
+    return rc ? rc : set_boundary(d, 0, kind, dirichlet_values);
 }
 
 int anuga_distribute_to_vertices_and_edges(anuga_domain *d, double *mass_error)
 {
     if (!d) return ANUGA_E_ARG;
-    ensure_live(d);
-    DeQ *S = d->substate ? d->B : d->A;
-    int rc = launch_extrapolate(d, S);
+    int rc = single(d);
     if (rc) return rc;
-    launch_materialize(d, S);
-    launch_reduce(d, 2);
-    rc = launch_check(d);
-    if (rc) return rc;
-    rc = sync_scalars(d);
+    rc = distribute(d);
     if (rc) return rc;
     if (mass_error) *mass_error = d->hctl->mass;
     return ANUGA_OK;
@@ -843,11 +1129,13 @@ int anuga_distribute_to_vertices_and_edges(anuga_domain *d, double *mass_error)
 int anuga_compute_fluxes(anuga_domain *d, double *flux_timestep)
 {
     if (!d) return ANUGA_E_ARG;
-    if (d->substate != 0) return fail(d, ANUGA_E_STATE, "compute_fluxes called inside an rk2 step");
-    if (!d->boundary_set && d->M > 0) return fail(d, ANUGA_E_STATE, "boundary conditions not set");
-    ensure_live(d);
-    int rc = launch_extrapolate(d, d->A);
+    int rc = single(d);
     if (rc) return rc;
+    if (d->substate != 0) return fail(d, ANUGA_E_STATE, "compute_fluxes called inside an rk2 step");
+    rc = ready_to_step(d);
+    if (rc) return rc;
+    ensure_live(d);
+    launch_extrapolate(d, d->A);
     launch_flux(d, 0, d->A, nullptr);
     launch_reduce(d, 0);
     rc = launch_check(d);
@@ -872,13 +1160,15 @@ static int read_negative(anuga_domain *d, int64_t *num_negative)
 int anuga_update_conserved_quantities(anuga_domain *d, double dt, int64_t *num_negative)
 {
     if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
     if (d->substate != 0) return fail(d, ANUGA_E_STATE, "update called twice in a step");
     ensure_live(d);
     const bool rk2 = d->P.timestep_fluxcalls == 2;
     hipLaunchKernelGGL(de_set_dt, dim3(1), dim3(64), 0, d->stream, d->ctl, dt);
     launch_update(d, d->A, rk2 ? d->B : d->A);
     if (!rk2) hipLaunchKernelGGL(de_integrate, dim3(1), dim3(64), 0, d->stream, d->ctl, 0);
-    int rc = launch_check(d);
+    rc = launch_check(d);
     if (rc) return rc;
     if (rk2) d->substate = 1;
     return num_negative ? read_negative(d, num_negative) : ANUGA_OK;
@@ -887,10 +1177,11 @@ int anuga_update_conserved_quantities(anuga_domain *d, double dt, int64_t *num_n
 int anuga_rk2_second_substep(anuga_domain *d, double dt, int64_t *num_negative)
 {
     if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
     if (d->substate != 1) return fail(d, ANUGA_E_STATE, "second substep without a first");
     hipLaunchKernelGGL(de_set_dt, dim3(1), dim3(64), 0, d->stream, d->ctl, dt);
-    int rc = launch_extrapolate(d, d->B);
-    if (rc) return rc;
+    launch_extrapolate(d, d->B);
     launch_flux(d, 1, d->B, d->A);
     launch_reduce(d, 1);
     hipLaunchKernelGGL(de_integrate, dim3(1), dim3(64), 0, d->stream, d->ctl, 1);
@@ -904,74 +1195,16 @@ int anuga_evolve_steps(anuga_domain *d, int64_t max_steps, const anuga_time_cont
                        anuga_step_state *st, double *dt_history)
 {
     if (!d || !tc || !st || max_steps < 0) return ANUGA_E_ARG;
-    if (d->substate != 0) return fail(d, ANUGA_E_STATE, "evolve_steps called inside an rk2 step");
-    if (!d->boundary_set && d->M > 0) return fail(d, ANUGA_E_STATE, "boundary conditions not set");
-    if (dt_history && max_steps > d->hist_cap) {
-        if (d->hist) hipFree(d->hist);
-        d->hist = nullptr;
-        d->hist_cap = 0;
-        drop_graph(d);
-        HIPCHK(d, dalloc(&d->hist, (size_t)max_steps));
-        d->hist_cap = max_steps;
-    }
-    DeBatchSetup b;
-    b.t = st->relative_time;
-    b.rec_min = st->recorded_min_timestep;
-    b.rec_max = st->recorded_max_timestep;
-    b.cfl = tc->CFL;
-    b.max_dt = tc->evolve_max_timestep;
-    b.min_dt = tc->evolve_min_timestep;
-    b.finaltime = tc->relative_finaltime;
-    b.yieldtime = tc->relative_yieldtime;
-    b.fixed_flux_dt = tc->fixed_flux_timestep;
-    b.eps = tc->epsilon;
-    b.smallsteps = (int)st->smallsteps;
-    b.order = (int)st->order;
-    b.default_order = (int)tc->default_order;
-    b.max_smallsteps = (int)tc->max_smallsteps;
-    b.hist_cap = dt_history ? (int)std::min<int64_t>(max_steps, d->hist_cap) : 0;
-    b.rk2 = d->P.timestep_fluxcalls == 2;
-    hipLaunchKernelGGL(de_setup_batch, dim3(1), dim3(64), 0, d->stream, d->ctl, b);
-    d->done_on_device = true;
-
-    const bool graph = d->use_graph && !d->timing;
-    if (graph && !d->graph_exec && max_steps > 0) {
-        HIPCHK(d, hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_device_step(d);
-        hipGraph_t g = nullptr;
-        HIPCHK(d, hipStreamEndCapture(d->stream, &g));
-        d->graph = g;
-        HIPCHK(d, hipGraphInstantiate(&d->graph_exec, d->graph, nullptr, nullptr, 0));
-    }
-    for (int64_t i = 0; i < max_steps; i++) {
-        if (graph) HIPCHK(d, hipGraphLaunch(d->graph_exec, d->stream));
-        else enqueue_device_step(d);
-    }
-    int rc = launch_check(d);
-    if (rc) return rc;
-    rc = sync_scalars(d);
-    const DeControl &c = *d->hctl;
-    st->relative_time = c.t;
-    st->timestep = c.dt;
-    st->flux_timestep = c.flux_dt;
-    st->recorded_min_timestep = c.rec_min;
-    st->recorded_max_timestep = c.rec_max;
-    st->smallsteps = c.smallsteps;
-    st->order = c.order;
-    st->steps_taken = c.nsteps;
-    st->reached = c.done;
-    if (dt_history && c.nsteps > 0) {
-        HIPCHK(d, hipMemcpy(dt_history, d->hist,
-                            (size_t)std::min<int64_t>(c.nsteps, max_steps) * sizeof(double),
-                            hipMemcpyDeviceToHost));
-    }
-    return rc;
+    int rc = single(d);
+    return rc ? rc : evolve_steps(d, max_steps, tc, st, dt_history, false);
 }
 
 int anuga_get_boundary_flux_sums(anuga_domain *d, double *sums)
 {
     if (!d || !sums) return ANUGA_E_ARG;
-    int rc = sync_scalars(d);
+    int rc = single(d);
+    if (rc) return rc;
+    rc = sync_scalars(d);
     if (rc) return rc;
     sums[0] = d->hctl->bflux0;
     sums[1] = d->P.timestep_fluxcalls == 2 ? d->hctl->bflux1 : 0.0;
@@ -981,7 +1214,9 @@ int anuga_get_boundary_flux_sums(anuga_domain *d, double *sums)
 int anuga_get_boundary_flux_integral(anuga_domain *d, double *integral)
 {
     if (!d || !integral) return ANUGA_E_ARG;
-    int rc = sync_scalars(d);
+    int rc = single(d);
+    if (rc) return rc;
+    rc = sync_scalars(d);
     if (rc) return rc;
     *integral = d->hctl->integral;
     return ANUGA_OK;
@@ -997,12 +1232,9 @@ int anuga_get_negative_cells(anuga_domain *d, int64_t *total)
 int anuga_get_water_volume(anuga_domain *d, double *volume)
 {
     if (!d || !volume) return ANUGA_E_ARG;
-    DeQ *S = d->substate ? d->B : d->A;
-    launch_volume(d, S);
-    launch_reduce(d, 3);
-    int rc = launch_check(d);
+    int rc = single(d);
     if (rc) return rc;
-    rc = sync_scalars(d);
+    rc = volumes(d);
     if (rc) return rc;
     *volume = d->hctl->volume;
     return ANUGA_OK;
@@ -1051,6 +1283,97 @@ int anuga_kernel_timing(anuga_domain *d, const char *name, double *avg_ms, int64
         }
     }
     return fail(d, ANUGA_E_ARG, std::string("unknown kernel timer ") + name);
+}
+
+// ---------------------------------------------------------------------------
+// ensembles
+// ---------------------------------------------------------------------------
+int anuga_ensemble_create(int64_t nscenarios, const int64_t *number_of_elements,
+                          const int64_t *boundary_lengths, const anuga_params *params,
+                          anuga_domain **out)
+{
+    if (!out) return fail(nullptr, ANUGA_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (!number_of_elements || !boundary_lengths)
+        return fail(nullptr, ANUGA_E_ARG, "size arrays are NULL");
+    return domain_alloc(params, nscenarios, number_of_elements, boundary_lengths, out);
+}
+
+int anuga_ensemble_set_mesh(anuga_domain *d, int64_t scenario, const anuga_mesh *mesh)
+{
+    if (!d) return ANUGA_E_ARG;
+    if (scenario < 0 || scenario >= d->S) return fail(d, ANUGA_E_ARG, "scenario index out of range");
+    if (d->mesh_set[scenario]) return fail(d, ANUGA_E_STATE, "the scenario's mesh is already set");
+    int rc = validate_mesh(d, mesh, d->ntri[scenario], d->nbnd[scenario]);
+    return rc ? rc : upload_mesh(d, (int)scenario, mesh);
+}
+
+int anuga_ensemble_set_centroid_values(anuga_domain *d, int64_t scenario, int quantity,
+                                       const double *values)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = scenario_ok(d, scenario);
+    return rc ? rc : set_centroids(d, (int)scenario, quantity, values);
+}
+
+int anuga_ensemble_get_centroid_values(anuga_domain *d, int64_t scenario, int quantity,
+                                       double *values)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = scenario_ok(d, scenario);
+    return rc ? rc : get_centroids(d, (int)scenario, quantity, values);
+}
+
+int anuga_ensemble_set_boundary(anuga_domain *d, int64_t scenario, const int32_t *kind,
+                                const double *dirichlet_values)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = scenario_ok(d, scenario);
+    return rc ? rc : set_boundary(d, (int)scenario, kind, dirichlet_values);
+}
+
+int anuga_ensemble_evolve_steps(anuga_domain *d, int64_t max_steps,
+                                const anuga_time_control *tc, anuga_step_state *states)
+{
+    if (!d || !tc || !states || max_steps < 0) return ANUGA_E_ARG;
+    return evolve_steps(d, max_steps, tc, states, nullptr, true);
+}
+
+int anuga_ensemble_distribute_to_vertices_and_edges(anuga_domain *d, double *mass_errors)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = distribute(d);
+    if (rc) return rc;
+    if (mass_errors)
+        for (int s = 0; s < d->S; s++) mass_errors[s] = d->hctl[s].mass;
+    return ANUGA_OK;
+}
+
+int anuga_ensemble_get_water_volumes(anuga_domain *d, double *volumes_out)
+{
+    if (!d || !volumes_out) return ANUGA_E_ARG;
+    int rc = ready_to_step(d);
+    if (rc) return rc;
+    rc = volumes(d);
+    if (rc) return rc;
+    for (int s = 0; s < d->S; s++) volumes_out[s] = d->hctl[s].volume;
+    return ANUGA_OK;
+}
+
+int anuga_ensemble_get_boundary_flux_integrals(anuga_domain *d, double *integrals)
+{
+    if (!d || !integrals) return ANUGA_E_ARG;
+    int rc = sync_scalars(d);
+    if (rc) return rc;
+    for (int s = 0; s < d->S; s++) integrals[s] = d->hctl[s].integral;
+    return ANUGA_OK;
+}
+
+int anuga_ensemble_size(const anuga_domain *d, int64_t *nscenarios)
+{
+    if (!d || !nscenarios) return ANUGA_E_ARG;
+    *nscenarios = d->S;
+    return ANUGA_OK;
 }
 
 }  // extern "C"

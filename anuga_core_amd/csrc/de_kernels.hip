@@ -67,13 +67,15 @@ de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict
                DeControl *ctl)
 {
     __shared__ double red[DE_BLOCK / 64];
-    if (ctl->done) return;
-    int *err = &ctl->status;
+    const int lb = xcd_block();
+    DeControl *cb = de_ctl(ctl, m.scen, lb);
+    if (cb->done) return;
+    int *err = &cb->status;
     const int N = m.N;
-    const int k = xcd_block() * DE_BLOCK + threadIdx.x;
+    const int k = lb * DE_BLOCK + threadIdx.x;
     double mass = 0.0;
 
-    if (k < N) {
+    if (k < cb->kend) {
         const double mah = p.mah;
         const int4 vk = m.nb[k];
         const int v[3] = {vk.x, vk.y, vk.z};
@@ -286,7 +288,7 @@ de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict
     if (threadIdx.x == 0) {
         double t = red[0];
         for (int i = 1; i < DE_BLOCK / 64; i++) t += red[i];
-        part_mass[xcd_block()] = t;
+        part_mass[lb] = t;
     }
 }
 
@@ -371,14 +373,16 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
 {
     __shared__ double red_dt[DE_BLOCK / 64];
     __shared__ double red_bf[DE_BLOCK / 64];
-    if (ctl->done) return;
-    int *err = &ctl->status;
-    const double dt = SUBSTEP ? ctl->dt : 0.0;
+    const int lb = xcd_block();
+    DeControl *cb = de_ctl(ctl, m.scen, lb);
+    if (cb->done) return;
+    int *err = &cb->status;
+    const double dt = SUBSTEP ? cb->dt : 0.0;
     const int N = m.N;
-    const int k = xcd_block() * DE_BLOCK + threadIdx.x;
+    const int k = lb * DE_BLOCK + threadIdx.x;
     double dtmin = 1.0e100, bflux = 0.0;
 
-    if (k < N) {
+    if (k < cb->kend) {
         const T g = T(p.g), eps = T(p.eps);
         const DeCentroid c = de_centroid(q[k], flags[k]);
         const int4 vk = m.nb[k];
@@ -487,7 +491,6 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
             a = fmin(a, red_dt[i]);
             b += red_bf[i];
         }
-        const int lb = xcd_block();
         part_dt[lb] = a;
         part_bf[lb] = b;
     }
@@ -512,14 +515,16 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
 {
     __shared__ double red_dt[DE_BLOCK / 64];
     __shared__ double red_bf[DE_BLOCK / 64];
-    if (ctl->done) return;
-    int *err = &ctl->status;
-    const double dt = SUBSTEP ? ctl->dt : 0.0;
+    const int lb = xcd_block();
+    DeControl *cb = de_ctl(ctl, m.scen, lb);
+    if (cb->done) return;
+    int *err = &cb->status;
+    const double dt = SUBSTEP ? cb->dt : 0.0;
     const int N = m.N;
-    const int k = xcd_block() * DE_BLOCK + threadIdx.x;
+    const int k = lb * DE_BLOCK + threadIdx.x;
     double dtmin = 1.0e100, bflux = 0.0;
 
-    if (k < N) {
+    if (k < cb->kend) {
         const float g = float(p.g), eps = float(p.eps);
         const DeCentroid c = de_centroid(q[k], flags[k]);
         const int4 vk = m.nb[k];
@@ -627,7 +632,6 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
             a2 = fmin(a2, red_dt[i]);
             b2 += red_bf[i];
         }
-        const int lb = xcd_block();
         part_dt[lb] = a2;
         part_bf[lb] = b2;
     }
@@ -637,34 +641,60 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
 // `out` may alias `q` (Euler, in place: each thread reads then writes its own record)
 template <bool CRPOW>
 __global__ void __launch_bounds__(DE_BLOCK)
-de_update(int N, DeParams p, const DeQ *q, const double *__restrict__ eta,
+de_update(const int *__restrict__ scen, DeParams p, const DeQ *q, const double *__restrict__ eta,
           const unsigned char *__restrict__ flags, const double *__restrict__ exw,
           const double *__restrict__ exuh, const double *__restrict__ exvh, DeControl *ctl,
           DeQ *out, unsigned long long *__restrict__ nneg)
 {
-    if (ctl->done) return;
-    const int k = xcd_block() * DE_BLOCK + threadIdx.x;
-    if (k >= N) return;
-    const double dt = ctl->dt;
-    int *err = &ctl->status;
+    const int lb = xcd_block();
+    DeControl *cb = de_ctl(ctl, scen, lb);
+    if (cb->done) return;
+    const int k = lb * DE_BLOCK + threadIdx.x;
+    if (k >= cb->kend) return;
+    const double dt = cb->dt;
+    int *err = &cb->status;
     const DeCentroid c = de_centroid(q[k], flags[k]);
     out[k] = de_advance<CRPOW>(c, eta[k], exw[k], exuh[k], exvh[k], dt, p, err, nneg);
 }
 
-// Fixed-order reduction of the per-block partials into the control block,
-// one block of 1024 threads. mode 0: flux_dt, bflux0, mass (first substep);
-// 1: bflux1 (second substep); 2: mass (distribute); 3: volume.
+// Fixed-order reduction of the per-block partials into the control blocks:
+// workgroup s (1024 threads) reduces the blocks of scenario s. MODE 0:
+// flux_dt, bflux0, mass (first substep); 1: bflux1 (second substep);
+// 2: mass (distribute); 3: volume. Each thread keeps 8 loads in flight (the
+// partials are few and the loop is bound by memory latency, not bandwidth);
+// the order of the additions depends only on the scenario's block count.
+template <int MODE>
 __global__ void __launch_bounds__(1024)
-de_reduce(int nparts, const double *__restrict__ part_dt, const double *__restrict__ part_bf,
-          const double *__restrict__ part_mass, DeControl *ctl, int mode)
+de_reduce(const double *__restrict__ part_dt, const double *__restrict__ part_bf,
+          const double *__restrict__ part_mass, DeControl *ctl)
 {
+    constexpr bool DT = MODE == 0, BF = MODE <= 1, MS = MODE == 0 || MODE >= 2;
+    constexpr int U = 8;
     __shared__ double s0[16], s1[16], s2[16];
-    if (mode < 2 && ctl->done) return;
+    DeControl *cb = ctl + blockIdx.x;
+    if (MODE < 2 && cb->done) return;
+    const int b0 = cb->bbeg, b1 = cb->bend;
     double a = 1.0e100, b = 0.0, c = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += 1024) {
-        if (part_dt) a = fmin(a, part_dt[i]);
-        if (part_bf) b += part_bf[i];
-        if (part_mass) c += part_mass[i];
+    int i = b0 + threadIdx.x;
+    for (; i + (U - 1) * 1024 < b1; i += U * 1024) {
+        double va[U], vb[U], vc[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (DT) va[u] = part_dt[i + u * 1024];
+            if (BF) vb[u] = part_bf[i + u * 1024];
+            if (MS) vc[u] = part_mass[i + u * 1024];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (DT) a = fmin(a, va[u]);
+            if (BF) b += vb[u];
+            if (MS) c += vc[u];
+        }
+    }
+    for (; i < b1; i += 1024) {
+        if (DT) a = fmin(a, part_dt[i]);
+        if (BF) b += part_bf[i];
+        if (MS) c += part_mass[i];
     }
     a = de_wave_min(a);
     b = de_wave_sum(b);
@@ -677,32 +707,40 @@ de_reduce(int nparts, const double *__restrict__ part_dt, const double *__restri
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int i = 1; i < 16; i++) {
-            s0[0] = fmin(s0[0], s0[i]);
-            s1[0] += s1[i];
-            s2[0] += s2[i];
+        for (int k = 1; k < 16; k++) {
+            s0[0] = fmin(s0[0], s0[k]);
+            s1[0] += s1[k];
+            s2[0] += s2[k];
         }
-        if (mode == 0) {
-            ctl->flux_dt = s0[0];
-            ctl->bflux0 = s1[0];
-            ctl->mass = s2[0];
-        } else if (mode == 1) {
-            ctl->bflux1 = s1[0];
-        } else if (mode == 2) {
-            ctl->mass = s2[0];
+        if (MODE == 0) {
+            cb->flux_dt = s0[0];
+            cb->bflux0 = s1[0];
+            cb->mass = s2[0];
+        } else if (MODE == 1) {
+            cb->bflux1 = s1[0];
+        } else if (MODE == 2) {
+            cb->mass = s2[0];
         } else {
-            ctl->volume = s2[0];
+            cb->volume = s2[0];
         }
     }
+}
+
+// one thread per scenario for the control kernels below
+__device__ __forceinline__ DeControl *de_scenario(DeControl *ctl, int S)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    return s < S ? ctl + s : nullptr;
 }
 
 // Generic_Domain.update_timestep (generic_domain.py:2349-2415) on the device:
 // CFL and max-timestep limit, recorded extremes, the small-step counter with
 // its first-order switch and its exception (reported as DE_ERR_SMALL_TIMESTEP),
 // and the clamps to the final time and the next yield time.
-__global__ void de_timestep(DeControl *c)
+__global__ void de_timestep(DeControl *ctl, int S)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || c->done) return;
+    DeControl *c = de_scenario(ctl, S);
+    if (!c || c->done) return;
     const double flux_dt = c->fixed_flux_dt > 0.0 ? c->fixed_flux_dt : c->flux_dt;
     const double a = c->cfl * flux_dt;
     double ts = (c->max_dt < a) ? c->max_dt : a;           // min(CFL * flux_dt, max_dt)
@@ -725,11 +763,13 @@ __global__ void de_timestep(DeControl *c)
 }
 
 // End of a step: boundary flux integral (boundary_flux_integral_operator.py:39-49),
-// relative_time += dt, timestep history, and the yield / final-time test of
-// _evolve_base (generic_domain.py:1855-1895) that turns the next steps into no-ops.
-__global__ void de_finish_step(DeControl *c, double *hist)
+// relative_time += dt, timestep history (one scenario only), and the yield /
+// final-time test of _evolve_base (generic_domain.py:1855-1895) that turns the
+// scenario's next steps into no-ops.
+__global__ void de_finish_step(DeControl *ctl, int S, double *hist)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || c->done) return;
+    DeControl *c = de_scenario(ctl, S);
+    if (!c || c->done) return;
     const double dt = c->dt;
     if (c->rk2) c->integral = c->integral + 0.5 * dt * (c->bflux0 + c->bflux1);
     else c->integral = c->integral + dt * c->bflux0;
@@ -744,7 +784,7 @@ __global__ void de_finish_step(DeControl *c, double *hist)
     }
 }
 
-// host-controlled steps: the host computed dt
+// host-controlled steps (one scenario): the host computed dt
 __global__ void de_set_dt(DeControl *c, double dt)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) c->dt = dt;
@@ -762,11 +802,14 @@ __global__ void de_integrate(DeControl *c, int rk2)
 struct DeBatchSetup {
     double t, rec_min, rec_max, cfl, max_dt, min_dt, finaltime, yieldtime, fixed_flux_dt, eps;
     int smallsteps, order, default_order, max_smallsteps, hist_cap, rk2;
+    int stop_reached;  // a scenario already at its final or yield time takes no step
 };
 
-__global__ void de_setup_batch(DeControl *c, DeBatchSetup b)
+__global__ void de_setup_batch(DeControl *ctl, int S, const DeBatchSetup *__restrict__ setup)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    DeControl *c = de_scenario(ctl, S);
+    if (!c) return;
+    const DeBatchSetup b = setup[c - ctl];
     c->t = b.t;
     c->rec_min = b.rec_min;
     c->rec_max = b.rec_max;
@@ -785,23 +828,38 @@ __global__ void de_setup_batch(DeControl *c, DeBatchSetup b)
     c->rk2 = b.rk2;
     c->nsteps = 0;
     c->done = 0;
+    if (b.stop_reached) {
+        if (c->t >= c->finaltime - c->eps) c->done = 1;
+        else if (c->t >= c->yieldtime) c->done = 2;
+    }
 }
 
 // leave device-controlled mode: steps are live again
-__global__ void de_clear_done(DeControl *c)
+__global__ void de_clear_done(DeControl *ctl, int S)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) c->done = 0;
+    DeControl *c = de_scenario(ctl, S);
+    if (c) c->done = 0;
+}
+
+// clear the error bits of every scenario (a kernel, so that no memset node
+// is needed in code a caller may capture)
+__global__ void de_clear_status(DeControl *ctl, int S)
+{
+    DeControl *c = de_scenario(ctl, S);
+    if (c) c->status = 0;
 }
 
 // per-block partial of sum(area * max(w - z, 0)) over the current state
 __global__ void __launch_bounds__(DE_BLOCK)
-de_volume(int N, const DeQ *__restrict__ q, const double *__restrict__ area,
-          double *__restrict__ part)
+de_volume(const int *__restrict__ scen, const DeControl *ctl, const DeQ *__restrict__ q,
+          const double *__restrict__ area, double *__restrict__ part)
 {
     __shared__ double red[DE_BLOCK / 64];
-    const int k = xcd_block() * DE_BLOCK + threadIdx.x;
+    const int lb = xcd_block();
+    const DeControl *cb = scen ? ctl + scen[lb] : ctl;
+    const int k = lb * DE_BLOCK + threadIdx.x;
     double v = 0.0;
-    if (k < N) {
+    if (k < cb->kend) {
         const DeQ r = q[k];
         v = area[k] * de_height(r.w, r.z);
     }
@@ -812,6 +870,22 @@ de_volume(int N, const DeQ *__restrict__ q, const double *__restrict__ area,
     if (threadIdx.x == 0) {
         double t = red[0];
         for (int i = 1; i < DE_BLOCK / 64; i++) t += red[i];
-        part[xcd_block()] = t;
+        part[lb] = t;
     }
+}
+
+// mixed path: FP32 bed offsets of the neighbours' centroids from the own one
+// (the local datum of the mixed flux), refreshed whenever the bed changes
+__global__ void __launch_bounds__(DE_BLOCK)
+de_bed_offsets(int kbeg, int kend, const int4 *__restrict__ nb, const DeQ *__restrict__ q,
+               DeGeoF32 *gf)
+{
+    const int k = kbeg + blockIdx.x * DE_BLOCK + threadIdx.x;
+    if (k >= kend) return;
+    const int4 v = nb[k];
+    const int vv[3] = {v.x, v.y, v.z};
+    const double z = q[k].z;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        gf[k].dzc[i] = vv[i] >= 0 ? float(q[vv[i] >> 2].z - z) : 0.f;
 }

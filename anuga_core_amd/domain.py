@@ -186,14 +186,19 @@ class Domain:
         rc = getattr(L, name)(self._handle, *args)
         _native.check(self._handle, rc, name)
 
-    def __del__(self):
+    def close(self):
+        """Release the device buffers now (also done when the Domain is
+        garbage-collected)."""
         h = getattr(self, "_handle", None)
         if h is not None and h.value:
-            try:
-                _native.lib().anuga_destroy(h)
-            except Exception:
-                pass
+            _native.lib().anuga_destroy(h)
             self._handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __len__(self):
         return self.number_of_elements

@@ -1,7 +1,7 @@
 """Benchmark: M triangle-steps/s of the DE1 step on a 16M-triangle mesh.
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--precision fp64|mixed]
-                    [--m 2000] [--cpu-steps 3] [--no-cpu]
+                    [--m 2000] [--cpu-steps 20] [--no-cpu] [--no-compare]
 
 Workload (one GPU): rectangular_cross(2000, 2000) over 20 km x 20 km, 16,000,000
 triangles, flow algorithm DE1 (rk2, FP64). Bed: -50 m plus 64 Gaussian hills
@@ -31,24 +31,66 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per triangle (FP64), each input of the unit read once and
-# each output written once:
-#   whole DE1 step: 2 x 232 B per Euler substep + 24 B for reading the
-#   step's initial state in the final saxpy = 488 B;
-#   per kernel (this implementation's split of the substep):
-KERNEL_BYTES_FP64 = {
-    # w,uh,vh,z (32) + neighbours (12) + centroid (16) + edge offsets (48)
-    #   -> 4 edge quantities x 3 edges (96) + flag (1)
-    "de_extrapolate": 205,
-    # edge values (96) + w,z (16) + flag (1) + neighbours (12) + normals (48)
-    #   + lengths (24) + area, radius (16) -> explicit updates (24)
-    "de_flux<0>": 237,
-    # w,uh,vh,z,n (40) + flag (1) + explicit updates (24) -> w,uh,vh (24)
-    "de_update": 89,
-    # de_flux<0> inputs + uh,vh,n (24) + backup state (24) -> state (24)
-    "de_flux<1>": 277,
+# Algorithmic bytes per triangle, each input of the unit read once and each
+# output written once (neighbour gathers are re-reads and do not count).
+#   Whole DE1 step (the reference's data model, int32 indices): 2 x 232 B per
+#   Euler substep + 24 B for reading the step's initial state in the final
+#   saxpy = 488 B (FP64); 344 B with FP32 geometry (mixed).
+#   Per kernel (this implementation's split of a substep), short name ->
+#   (kernel, bytes):
+KERNEL_BYTES = {
+    "fp64": {
+        # w,uh,vh,z (32) + 3 neighbours (12) + centroid (16) + edge offsets (48)
+        #   -> w,h,uh,vh on 3 edges (96) + flag (1)
+        "extrapolate": ("de_extrapolate", 205),
+        # edge values (96) + w,z (16) + flag (1) + neighbours (12) + normals (48)
+        #   + lengths (24) + area, radius (16) -> explicit updates (24)
+        "flux0": ("de_flux<0>", 237),
+        # w,uh,vh,z (32) + n (8) + flag (1) + explicit updates (24) -> w,uh,vh (24)
+        "update": ("de_update", 89),
+        # edge values (96) + w,uh,vh,z (32) + flag (1) + neighbours (12)
+        #   + normals (48) + lengths (24) + area (8) + n (8) + step-initial
+        #   w,uh,vh (24) -> w,uh,vh (24)
+        "flux1": ("de_flux<1>", 277),
+    },
+    "mixed": {
+        # w,uh,vh,z (32) + neighbours (12) + FP32 edge and neighbour offsets (48)
+        #   -> FP32 jumps dw,dh,uh,vh on 3 edges (48) + FP64 depth (8) + flag (1)
+        "extrapolate": ("de_extrapolate", 149),
+        # edge record (56) + w,z (16) + flag (1) + neighbours (12) + FP32 normals,
+        #   lengths, radius, neighbour bed offsets (52) + area (8) -> explicit (24)
+        "flux0": ("de_flux_mixed<0>", 169),
+        "update": ("de_update", 89),
+        # edge record (56) + w,uh,vh,z (32) + flag (1) + neighbours (12) + FP32
+        #   geometry without radius (48) + area (8) + n (8) + step-initial
+        #   w,uh,vh (24) -> w,uh,vh (24)
+        "flux1": ("de_flux_mixed<1>", 213),
+    },
 }
+LAUNCHES_PER_STEP = {"extrapolate": 2, "flux0": 1, "update": 1, "flux1": 1}
 STEP_BYTES = {"fp64": 488, "mixed": 344}
+PROFILE = os.path.join(ROOT, "profiles", "de_kernels_rectangular_cross2000_DE1_%s.txt")
+
+
+def measured_traffic(precision, kernel, m):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
+    this workload (FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy), or None
+    when there is no summary for this mesh size."""
+    path = PROFILE % precision
+    if m != 2000 or not os.path.exists(path):
+        return None
+    header = None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#") or not line.strip():
+                continue
+            cols = line.split()
+            if header is None:
+                header = cols
+                continue
+            if cols[0] == kernel and "hbm_bytes" in header:
+                return float(cols[header.index("hbm_bytes")])
+    return None
 
 
 def build_c1(m=2000, length=20000.0, nhills=64, seed=0):
@@ -103,6 +145,219 @@ def cpu_baseline(mesh, q, bc, steps):
     return nsteps, dt, threads
 
 
+def run_timed(precision, mesh, q, bc, args):
+    """Set up the device domain, run the warm-up, the timed steps and the
+    eager per-kernel timing pass. Returns (domain, result dict)."""
+    from anuga_core_amd.domain import Domain
+    t0 = time.perf_counter()
+    d = Domain(mesh=mesh, precision=precision, reorder=not args.no_reorder)
+    d.set_flow_algorithm("DE1")
+    for name, val in q.items():
+        d.set_quantity(name, val, location="centroids")
+    d.set_boundary(_boundaries(d, bc))
+    d.record_dt_history = False
+    print("%s domain ready in %.1fs" % (precision, time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+    d.distribute_to_vertices_and_edges()
+    v0 = d.get_water_volume()
+    d.evolve_steps(args.warmup)
+    d.synchronize()
+    # timed region: device-controlled steps, one HIP graph per step
+    t1 = time.perf_counter()
+    d.evolve_steps(args.steps)
+    d.synchronize()
+    elapsed = time.perf_counter() - t1
+    print("%s: %d steps in %.3fs" % (precision, args.steps, elapsed), file=sys.stderr, flush=True)
+    # per-kernel durations: HIP events around every launch on the library's
+    # stream, over an eager (non-graph) run of further steps
+    d.reset_kernel_timing(True)
+    d.evolve_steps(args.timing_steps)
+    d.synchronize()
+    kt = {n: d.kernel_timing(n) for n in KERNEL_BYTES[precision]}
+    d.reset_kernel_timing(False)
+    d.distribute_to_vertices_and_edges()
+    v1 = d.get_water_volume()
+    N = mesh.number_of_triangles
+    kernels = {}
+    for short, (ms, n) in kt.items():
+        kname, nbytes = KERNEL_BYTES[precision][short]
+        kernels[kname] = {"avg_ms": round(ms, 5), "launches": n,
+                          "alg_bytes_per_triangle": nbytes,
+                          "alg_GBs": round(nbytes * N / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+    ms_per_step = 1e3 * elapsed / args.steps
+    return d, {
+        "value": N * args.steps / elapsed / 1e6,
+        "ms_per_step": ms_per_step,
+        "kernels": kernels,
+        "volume_drift": (v1 - v0 - d.boundary_flux_integral) / v0,
+        "steps_total": args.warmup + args.steps + args.timing_steps,
+    }
+
+
+def error_report(mixed, ref):
+    """Mixed-precision state against the FP64 device state after the same
+    number of steps: L1 (sum |diff| / sum |ref|) and max |diff| per quantity,
+    and the stage L1 relative to the water column (sum |diff| / sum h)."""
+    out = {}
+    for name in ("stage", "xmomentum", "ymomentum"):
+        a = mixed.get_quantity(name).centroid_values
+        b = ref.get_quantity(name).centroid_values
+        d = np.abs(a - b)
+        out[name] = {"L1": float(d.sum() / np.abs(b).sum()), "max": float(d.max())}
+        if name == "stage":
+            h = np.maximum(b - ref.get_quantity("elevation").centroid_values, 0.0)
+            out["stage_L1_over_water_column"] = float(d.sum() / h.sum())
+    out["time_difference_s"] = mixed.get_time() - ref.get_time()
+    return out
+
+
+def build_ensemble(nscen=512, seed=1, kmin=32, kmax=700, cell=10.0):
+    """Scenario list of the batched workload: k ~ log-uniform[kmin, kmax]
+    (numpy default_rng(seed)), rectangular_cross(k, k) over (cell*k)^2 m
+    (4k^2 triangles), bed -0.005 x plus 8 Gaussian hills (amplitude
+    U(0.5, 3) m, sigma U(0.03, 0.1) L), a hump of U(0.5, 5) m (sigma 0.1 L)
+    at a random centre clipped to the bed, Manning n ~ U(0.01, 0.06),
+    transmissive west, reflective elsewhere. Returns (k, params) per scenario;
+    meshes are built on demand by ensemble_scenario."""
+    rng = np.random.default_rng(seed)
+    ks = np.rint(np.exp(rng.uniform(np.log(kmin), np.log(kmax), nscen))).astype(int)
+    out = []
+    for k in ks:
+        L = cell * k
+        hills = [(rng.uniform(0.5, 3.0), rng.uniform(0.0, L), rng.uniform(0.0, L),
+                  rng.uniform(0.03, 0.1) * L) for _ in range(8)]
+        hump = (rng.uniform(0.5, 5.0), rng.uniform(0.0, L), rng.uniform(0.0, L))
+        out.append((int(k), dict(L=L, hills=hills, hump=hump, n=rng.uniform(0.01, 0.06))))
+    return out
+
+
+def ensemble_scenario(k, prm, mesh=None):
+    """Mesh, centroid quantities and boundary kinds of one batched scenario."""
+    from anuga_core_amd.mesh import Mesh, rectangular_cross
+    L = prm["L"]
+    if mesh is None:
+        mesh = Mesh(*rectangular_cross(k, k, L, L))
+    x = mesh.centroid_coordinates[:, 0]
+    y = mesh.centroid_coordinates[:, 1]
+    z = -0.005 * x
+    for a, cx, cy, s in prm["hills"]:
+        z = z + a * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2.0 * s * s))
+    H, hx, hy = prm["hump"]
+    s = 0.1 * L
+    w = np.maximum(H * np.exp(-((x - hx) ** 2 + (y - hy) ** 2) / (2.0 * s * s)), z)
+    q = dict(elevation=z, stage=w, friction=np.full(x.shape, prm["n"]))
+    bc = {"left": "transmissive", "right": "reflective", "top": "reflective",
+          "bottom": "reflective"}
+    return mesh, q, bc
+
+
+def run_ensemble(args):
+    """The batched workload: every scenario advanced --steps DE1 steps with its
+    own timestep, all scenarios in one device handle."""
+    from anuga_core_amd.ensemble import Ensemble
+    t0 = time.perf_counter()
+    spec = build_ensemble(args.scenarios)
+    order = sorted(range(len(spec)), key=lambda i: spec[i][0])
+    sizes = [(4 * k * k, 4 * k) for k, _ in spec]
+    ens = Ensemble(sizes, flow_algorithm="DE1", precision=args.precision,
+                   reorder=not args.no_reorder)
+    mesh, mesh_k = None, None
+    for j, i in enumerate(order):
+        k, prm = spec[i]
+        if k != mesh_k:
+            mesh, mesh_k = None, k
+        mesh, q, bc = ensemble_scenario(k, prm, mesh)
+        ens.set_scenario(i, mesh, q, bc)
+        if j % 64 == 63:
+            print("  %d scenarios set up (k <= %d) %.1fs" % (j + 1, k, time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
+    del mesh
+    Ntot = int(sum(n for n, _ in sizes))
+    print("ensemble of %d scenarios, %d triangles, set up in %.1fs" %
+          (len(spec), Ntot, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    v0 = ens.get_water_volumes()
+    ens.evolve_steps(args.warmup)
+    ens.synchronize()
+    t1 = time.perf_counter()
+    ens.evolve_steps(args.steps)
+    ens.synchronize()
+    elapsed = time.perf_counter() - t1
+    ens.reset_kernel_timing(True)
+    ens.evolve_steps(args.timing_steps)
+    ens.synchronize()
+    kt = {n: ens.kernel_timing(n) for n in KERNEL_BYTES[args.precision]}
+    ens.reset_kernel_timing(False)
+    v1 = ens.get_water_volumes()
+    bfi = ens.get_boundary_flux_integrals()
+    drift = np.abs(v1 - v0 - bfi) / v0
+    kernels = {}
+    for short, (ms, n) in kt.items():
+        kname, nbytes = KERNEL_BYTES[args.precision][short]
+        kernels[kname] = {"avg_ms": round(ms, 5), "launches": n,
+                          "alg_bytes_per_triangle": nbytes,
+                          "alg_GBs": round(nbytes * Ntot / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+    dom_short = max(KERNEL_BYTES[args.precision],
+                    key=lambda s: kt[s][0] * LAUNCHES_PER_STEP[s])
+    dom, dom_bytes = KERNEL_BYTES[args.precision][dom_short]
+    ach = kernels[dom]["alg_GBs"]
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = Ntot * args.steps / elapsed / 1e6
+    step_gbs = STEP_BYTES[args.precision] * Ntot / (ms_per_step * 1e-3) / 1e9
+    times = ens.get_times()
+    out = {
+        "metric": "M triangle-steps/s, batched DE1 ensemble (per-scenario dt)",
+        "value": round(value, 3), "unit": "M triangle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-flux/f64-state",
+        "data": "synthetic",
+        "config": {"workload": "%d scenarios rectangular_cross(k,k), k log-uniform[32,700] "
+                               "(rng seed 1), 10 m cells, DE1, bed -0.005x + 8 hills, "
+                               "hump U(0.5,5) m, n U(0.01,0.06)" % len(spec),
+                   "scenarios": len(spec), "triangles": Ntot, "precision": args.precision},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_launch": dom_bytes * Ntot},
+        "step_roofline": {"alg_bytes_per_triangle_step": STEP_BYTES[args.precision],
+                          "achieved": round(step_gbs, 1), "unit": "GB/s",
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "kernels": kernels,
+        "simulated_time_s": {"min": float(times.min()), "max": float(times.max())},
+        "max_volume_drift": float(drift.max()),
+    }
+    ens.close()
+    if not args.no_cpu and args.cpu_steps > 0:
+        # bounded sample: the oracle on a few scenarios of the ensemble
+        from oracle.kernels import OracleKernels
+        from oracle.oracle_domain import OracleDomain
+        pick = [order[len(order) // 4], order[len(order) // 2], order[3 * len(order) // 4]]
+        # steps so that the sample is about 3e8 triangle-steps (10-30 s on one host)
+        nsample = sum(4 * spec[i][0] ** 2 for i in pick)
+        csteps = max(args.cpu_steps, int(3.0e8 / nsample))
+        tri_steps, secs = 0, 0.0
+        for i in pick:
+            k, prm = spec[i]
+            mesh, q, bc = ensemble_scenario(k, prm)
+            o = OracleDomain(mesh, OracleKernels(gather=True, pow_mode=0), "DE1")
+            for name, val in q.items():
+                o.set_centroid_values(name, val)
+            o.set_boundary({t: type("B", (), {"kind": kk})() for t, kk in bc.items()})
+            gen = o.evolve(yieldstep=1.0e300, finaltime=1.0e300, max_steps=csteps)
+            next(gen)
+            ts = time.perf_counter()
+            for _ in gen:
+                pass
+            secs += time.perf_counter() - ts
+            tri_steps += mesh.number_of_triangles * len(o.dt_history)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {
+            "value": round(tri_steps / secs / 1e6, 4), "unit": "M triangle-steps/s",
+            "cores": threads, "kind": "port",
+            "sample": "%d DE1 steps of 3 scenarios of the ensemble (k = %s), oracle C "
+                      "restatement, %.1f s" % (csteps, [spec[i][0] for i in pick], secs)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,11 +365,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--m", type=int, default=2000)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--timing-steps", type=int, default=50,
                     help="steps of the eager pass that times each kernel with HIP events")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the second precision (speed and mixed-vs-FP64 error)")
     ap.add_argument("--no-reorder", action="store_true")
+    ap.add_argument("--workload", default="single", choices=["single", "ensemble"],
+                    help="single: the 16M-triangle mesh; ensemble: the batched scenarios")
+    ap.add_argument("--scenarios", type=int, default=512)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run 3 device copies of 1 GiB (a known-traffic kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -125,8 +385,6 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from anuga_core_amd.domain import Domain
-
     if args.calibrate_copy:
         import torch
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
@@ -136,93 +394,79 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if args.workload == "ensemble":
+        run_ensemble(args)
+        return
+
     t0 = time.perf_counter()
     mesh, q, bc = build_c1(args.m)
     N = mesh.number_of_triangles
-    print("mesh built: %d triangles in %.1fs" % (N, time.perf_counter() - t0),
-          file=sys.stderr, flush=True)
-    d = Domain(mesh=mesh, precision=args.precision, reorder=not args.no_reorder)
-    d.set_flow_algorithm("DE1")
-    for name, val in q.items():
-        d.set_quantity(name, val, location="centroids")
-    d.set_boundary(_boundaries(d, bc))
-    d.record_dt_history = False
-    print("device domain ready %.1fs; dry fraction %.3f" %
-          (time.perf_counter() - t0, float(np.mean(q["stage"] - q["elevation"] <= 1e-5))),
+    print("mesh built: %d triangles in %.1fs; dry fraction %.3f" %
+          (N, time.perf_counter() - t0, float(np.mean(q["stage"] - q["elevation"] <= 1e-5))),
           file=sys.stderr, flush=True)
 
-    d.distribute_to_vertices_and_edges()
-    v0 = d.get_water_volume()
-    d.evolve_steps(args.warmup)
-    d.synchronize()
-    # timed region: device-controlled steps, one HIP graph per step
-    t1 = time.perf_counter()
-    d.evolve_steps(args.steps)
-    d.synchronize()
-    elapsed = time.perf_counter() - t1
-    # per-kernel durations: HIP events around every launch on the library's
-    # stream, over an eager (non-graph) run of the same steps
-    d.reset_kernel_timing(True)
-    d.evolve_steps(args.timing_steps)
-    d.synchronize()
-    kt = {n: d.kernel_timing(n) for n in ("extrapolate", "flux0", "update", "flux1")}
-    d.reset_kernel_timing(False)
-    d.distribute_to_vertices_and_edges()
-    v1 = d.get_water_volume()
-    bfi = d.boundary_flux_integral
-
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = N * args.steps / elapsed / 1e6
-    names = {"extrapolate": "de_extrapolate", "flux0": "de_flux<0>", "update": "de_update",
-             "flux1": "de_flux<1>"}
-    per_kernel = {}
-    for short, (ms, n) in kt.items():
-        kname = names[short]
-        b = KERNEL_BYTES_FP64[kname] * N
-        per_kernel[kname] = {"avg_ms": ms, "launches": n,
-                             "alg_GBs": (b / (ms * 1e-3) / 1e9) if ms > 0 else None}
-    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"] * per_kernel[k]["launches"])
-    ach = per_kernel[dom]["alg_GBs"]
-    step_bytes = STEP_BYTES[args.precision] * N
-    step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
+    d, r = run_timed(args.precision, mesh, q, bc, args)
+    kernels = r["kernels"]
+    # dominant kernel: largest total time per step
+    dom_short = max(KERNEL_BYTES[args.precision],
+                    key=lambda s: kernels[KERNEL_BYTES[args.precision][s][0]]["avg_ms"]
+                    * LAUNCHES_PER_STEP[s])
+    dom, dom_bytes = KERNEL_BYTES[args.precision][dom_short]
+    ach = kernels[dom]["alg_GBs"]
+    traffic = measured_traffic(args.precision, dom, args.m)
+    step_gbs = STEP_BYTES[args.precision] * N / (r["ms_per_step"] * 1e-3) / 1e9
 
     out = {
         "metric": "M triangle-steps/s at 16M-tri DE1: FP64 vs FP32-flux (stated err); HBM GB/s",
-        "value": round(value, 3),
+        "value": round(r["value"], 3),
         "unit": "M triangle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(r["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32-flux/f64-state",
         "data": "synthetic",
-        "config": {"workload": "rectangular_cross(%d,%d) 20km DE1, 64 Gaussian hills seed 0, "
-                               "3 m hump, n=0.025, transmissive west" % (args.m, args.m),
+        "config": {"workload": "rectangular_cross(%d,%d) over %.0f km square, DE1, 64 Gaussian "
+                               "hills (rng seed 0), 3 m hump, n=0.025, transmissive west"
+                               % (args.m, args.m, 20.0),
                    "triangles": N, "flow_algorithm": "DE1", "precision": args.precision,
                    "reorder": "rcm" if not args.no_reorder else "none"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                     "alg_bytes_per_triangle": KERNEL_BYTES_FP64[dom]},
+                     "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "alg_bytes_per_launch": dom_bytes * N},
         "step_roofline": {"alg_bytes_per_triangle_step": STEP_BYTES[args.precision],
                           "achieved": round(step_gbs, 1), "unit": "GB/s",
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
-        "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv)
-                        for kk, vv in v.items()} for k, v in per_kernel.items()},
-        "volume_drift": (v1 - v0 - bfi) / v0,
+        "kernels": kernels,
+        "volume_drift": r["volume_drift"],
     }
+
+    if not args.no_compare:
+        other = "mixed" if args.precision == "fp64" else "fp64"
+        d2, r2 = run_timed(other, mesh, q, bc, args)
+        mixed, ref = (d2, d) if other == "mixed" else (d, d2)
+        out[other] = {"value": round(r2["value"], 3), "ms_per_step": round(r2["ms_per_step"], 4),
+                      "kernels": r2["kernels"], "volume_drift": r2["volume_drift"]}
+        rv = {"fp64": r if args.precision == "fp64" else r2,
+              "mixed": r2 if args.precision == "fp64" else r}
+        out["mixed_speedup_vs_fp64"] = round(rv["mixed"]["value"] / rv["fp64"]["value"], 3)
+        out["mixed_error_vs_fp64"] = dict(error_report(mixed, ref),
+                                          steps=r["steps_total"])
+        d2.close()
 
     if not args.no_cpu and args.cpu_steps > 0:
         nsteps, dt, threads = cpu_baseline(mesh, q, bc, args.cpu_steps)
         out["cpu_baseline"] = {
             "value": round(N * nsteps / dt / 1e6, 4), "unit": "M triangle-steps/s",
             "cores": threads, "kind": "port",
-            "sample": "%d DE1 steps of the same 16M-triangle workload, oracle C restatement "
-                      "(OpenMP flux loop), %.1f s" % (nsteps, dt)}
-        out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+            "sample": "%d DE1 steps of the same %d-triangle workload, oracle C restatement "
+                      "(OpenMP per-triangle loops), %.1f s" % (nsteps, N, dt)}
+        out["speedup_vs_cpu"] = round(r["value"] / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
 
 

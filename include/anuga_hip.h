@@ -206,6 +206,45 @@ int anuga_synchronize(anuga_domain *d);
 int anuga_kernel_timing(anuga_domain *d, const char *name, double *avg_ms, int64_t *launches);
 int anuga_reset_kernel_timing(anuga_domain *d, int enable);
 
+/* ---- Ensembles: many independent scenarios stepped together ----------------
+ * The reference runs one Domain per process; an ensemble (flood scenarios,
+ * parameter sweeps) there is a loop over Domain objects, each evolve() a
+ * separate sequence of small launches. An ensemble handle holds S scenarios
+ * (each its own mesh, quantities, boundary conditions and timestep control,
+ * the same algorithm parameters) in one set of device buffers, so that every
+ * kernel launch and every replay of the captured step graph advances all of
+ * them, each with its own dt (update_timestep per scenario). Results per
+ * scenario are bit-identical to a single-scenario handle of the same inputs.
+ * The single-scenario entry points above refuse an ensemble handle of S > 1
+ * (ANUGA_E_STATE). */
+
+/* Allocate S scenarios of number_of_elements[s] triangles and
+ * boundary_lengths[s] boundary edges; meshes follow with anuga_ensemble_set_mesh. */
+int anuga_ensemble_create(int64_t nscenarios, const int64_t *number_of_elements,
+                          const int64_t *boundary_lengths, const anuga_params *params,
+                          anuga_domain **out);
+/* Upload scenario s's mesh (validated as in anuga_create); once per scenario. */
+int anuga_ensemble_set_mesh(anuga_domain *d, int64_t scenario, const anuga_mesh *mesh);
+/* Per-scenario counterparts of anuga_set/get_centroid_values and anuga_set_boundary. */
+int anuga_ensemble_set_centroid_values(anuga_domain *d, int64_t scenario, int quantity,
+                                       const double *values);
+int anuga_ensemble_get_centroid_values(anuga_domain *d, int64_t scenario, int quantity,
+                                       double *values);
+int anuga_ensemble_set_boundary(anuga_domain *d, int64_t scenario, const int32_t *kind,
+                                const double *dirichlet_values);
+/* anuga_evolve_steps for every scenario: tc[s] and states[s] per scenario. A
+ * scenario that reaches its yield or final time stops there while the others
+ * go on; one that is already there at the call takes no step. */
+int anuga_ensemble_evolve_steps(anuga_domain *d, int64_t max_steps,
+                                const anuga_time_control *tc, anuga_step_state *states);
+/* anuga_distribute_to_vertices_and_edges for every scenario; mass_errors (S
+ * values, may be NULL) receives each scenario's protect mass. */
+int anuga_ensemble_distribute_to_vertices_and_edges(anuga_domain *d, double *mass_errors);
+/* Per-scenario water volume (S values) and boundary flux integral (S values). */
+int anuga_ensemble_get_water_volumes(anuga_domain *d, double *volumes);
+int anuga_ensemble_get_boundary_flux_integrals(anuga_domain *d, double *integrals);
+int anuga_ensemble_size(const anuga_domain *d, int64_t *nscenarios);
+
 #ifdef __cplusplus
 }
 #endif

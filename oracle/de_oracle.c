@@ -24,7 +24,7 @@
  * Pinning: every function is compared bit for bit against the reference's own
  * compiled C (oracle/_ref) on seeded states, the flux function against the
  * reference's known-answer tests, and the orchestration against the
- * reference's golden trajectories (tests/test_oracle_*.py).
+ * reference's golden trajectories (tests/test_oracle.py).
  *
  * Build: gcc -O2 -fno-fast-math -ffp-contract=off (no FMA contraction: the
  * reference's default build has none).

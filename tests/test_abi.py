@@ -83,3 +83,18 @@ def test_domain_fails_loudly_without_a_device():
     from anuga_core_amd import rectangular_cross_domain
     with pytest.raises(_native.DeviceError):
         rectangular_cross_domain(3, 3)
+
+
+def test_ensemble_create_rejects_bad_sizes_before_touching_the_device():
+    L = _native.lib()
+    h = C.c_void_p()
+    p = _native.AnugaParams(9.8, 1e-12, 1e-5, 1, 0, 1, 0, 1, 0, 0, 1, 2, 0, 1)
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
+    n = np.array([100, 0, 50], np.int64)
+    m = np.array([10, 4, 6], np.int64)
+    assert L.anuga_ensemble_create(3, ip(n), ip(m), C.byref(p), C.byref(h)) == _native.ANUGA_E_ARG
+    assert b"number_of_elements" in L.anuga_last_error(None)
+    assert L.anuga_ensemble_create(0, ip(n), ip(m), C.byref(p), C.byref(h)) == _native.ANUGA_E_ARG
+    n = np.array([1 << 28, 1 << 28, 10], np.int64)   # too many triangles for one handle
+    assert L.anuga_ensemble_create(3, ip(n), ip(m), C.byref(p), C.byref(h)) == _native.ANUGA_E_ARG
+    assert not h.value

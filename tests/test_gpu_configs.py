@@ -1,0 +1,105 @@
+"""The measurement configurations run end to end on the device (MI355X).
+
+* C0: rectangular_cross(100,100) over 100 m x 100 m (40,000 triangles), DE0,
+  bed -0.01 x, 1 m dam for x < 50 (dry elsewhere), n = 0.03, reflective walls,
+  evolve(yieldstep=1, finaltime=20): device vs oracle at every yield.
+* C2: mixed FP32/FP64 against the FP64 device path and the CPU oracle after
+  1000 DE1 steps on the 64-hill workload scaled to rectangular_cross(150,150)
+  (90,000 triangles, same bed / hump / friction / boundaries per km).
+"""
+
+import numpy as np
+import pytest
+
+from anuga_core_amd.domain import Domain
+from anuga_core_amd.mesh import Mesh, rectangular_cross
+from oracle.kernels import OracleKernels
+from oracle.oracle_domain import OracleDomain
+from tests import scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _c0():
+    p, v, b = rectangular_cross(100, 100, 100.0, 100.0)
+    mesh = Mesh(p, v, b)
+    x = mesh.centroid_coordinates[:, 0]
+    z = -0.01 * x
+    q = dict(elevation=z, stage=np.where(x < 50.0, 1.0, z),
+             friction=np.full(mesh.number_of_triangles, 0.03))
+    bc = {t: "reflective" for t in ("left", "right", "top", "bottom")}
+    return mesh, "DE0", q, bc
+
+
+def test_c0_dam_break_full_run_vs_oracle():
+    print("case C0 40k DE0 to t=20", flush=True)
+    mesh, alg, q, bc = _c0()
+    assert mesh.number_of_triangles == 40000
+    o = S.setup(OracleDomain(mesh, OracleKernels(gather=True)), alg, q, bc)
+    d = S.setup(Domain(mesh=mesh), alg, q, bc)
+    yo = []
+    for t in o.evolve(yieldstep=1.0, finaltime=20.0):
+        yo.append((t, o.stage_c.copy(), o.xmom_c.copy(), o.ymom_c.copy()))
+    yd = []
+    for t in d.evolve(yieldstep=1.0, finaltime=20.0):
+        yd.append((t, d.get_quantity("stage").centroid_values,
+                   d.get_quantity("xmomentum").centroid_values,
+                   d.get_quantity("ymomentum").centroid_values))
+    print("C0 steps %d, yields %d" % (len(d.dt_history), len(yd)), flush=True)
+    assert len(yo) == len(yd) == 21
+    assert d.dt_history == o.dt_history
+    for (to, *qo), (td, *qd) in zip(yo, yd):
+        assert to == td
+        for a, b in zip(qd, qo):
+            assert S.rel_maxnorm(a, b) <= 1e-12
+    vo, vd = o.get_water_volume(), d.get_water_volume()
+    assert abs(vo - vd) <= 1e-12 * vo
+
+
+def _c1_scaled(m=150):
+    import bench
+    return bench.build_c1(m=m, length=20000.0 * m / 2000.0)
+
+
+def _report(a, b):
+    d = np.abs(a - b)
+    return dict(l1=float(d.sum() / max(np.abs(b).sum(), 1e-300)), max=float(d.max()))
+
+
+def test_c2_mixed_error_report():
+    print("case C2 mixed error report", flush=True)
+    mesh, q, bc = _c1_scaled()
+    runs = {}
+    for prec in ("fp64", "mixed"):
+        d = S.setup(Domain(mesh=mesh, precision=prec), "DE1", q, bc)
+        d.evolve_steps(1000)
+        d.distribute_to_vertices_and_edges()
+        runs[prec] = d
+    o = S.setup(OracleDomain(mesh, OracleKernels(gather=True)), "DE1", q, bc)
+    gen = o.evolve(yieldstep=1e300, finaltime=1e300, max_steps=1000)
+    for _ in gen:
+        pass
+    o.distribute_to_vertices_and_edges()
+    h0 = np.maximum(q["stage"] - q["elevation"], 0.0)
+    v0 = float(np.sum(mesh.areas * h0))
+    out = {}
+    f = runs["fp64"]
+    mx = runs["mixed"]
+    for name, oname in (("stage", "stage_c"), ("xmomentum", "xmom_c"), ("ymomentum", "ymom_c")):
+        a = mx.get_quantity(name).centroid_values
+        out[name + "_vs_fp64_gpu"] = _report(a, f.get_quantity(name).centroid_values)
+        out[name + "_vs_cpu_oracle"] = _report(a, getattr(o, oname))
+        out[name + "_fp64_gpu_vs_cpu_oracle"] = _report(f.get_quantity(name).centroid_values,
+                                                        getattr(o, oname))
+    for prec, d in runs.items():
+        out["volume_drift_" + prec] = (d.get_water_volume() - v0 - d.boundary_flux_integral) / v0
+    depth = np.maximum(f.get_quantity("stage").centroid_values
+                       - f.get_quantity("elevation").centroid_values, 0.0)
+    out["stage_L1_over_water_column"] = float(
+        np.abs(mx.get_quantity("stage").centroid_values
+               - f.get_quantity("stage").centroid_values).sum() / depth.sum())
+    for k, v in out.items():
+        print("C2", k, v, flush=True)
+    assert out["stage_fp64_gpu_vs_cpu_oracle"]["max"] <= 1e-12 * np.abs(o.stage_c).max()
+    assert out["stage_L1_over_water_column"] <= 1e-5
+    assert abs(out["volume_drift_mixed"]) <= 1e-9 and abs(out["volume_drift_fp64"]) <= 1e-12

@@ -277,3 +277,37 @@ def test_pow_modes():
         err = abs(Decimal(r) - exact) / Decimal(math.ulp(r))
         assert err <= Decimal("0.5"), (x, float(err))
     lib.oracle_set_pow_mode(0)
+
+
+def test_rk2_step_is_ssp_average_of_two_euler_steps():
+    """DE1's step (generic_domain.py:1974-2051) is SSP-RK2 (Heun): with the
+    first stage's timestep tau for both stages, Q(n+1) = 1/2 Q(n) + 1/2 E(E(Q(n))),
+    E the Euler step that the DE0 golden trajectories pin. A fixed flux
+    timestep makes both runs take the same tau; the oracle's rk2 orchestration
+    must then equal the average of two of its Euler steps, bit for bit."""
+    mesh, _, q, bc = S.dam_break_de0(m=12, n=12)
+    probe = S.setup(OracleDomain(mesh, OracleKernels(gather=True)), "DE1", q, bc)
+    for _ in probe.evolve(yieldstep=1e300, finaltime=1e300, max_steps=1):
+        pass
+    tau = 0.9 * probe.dt_history[0]
+
+    def run(method, steps):
+        o = S.setup(OracleDomain(mesh, OracleKernels(gather=True)), "DE1", q, bc)
+        o.timestepping_method = method
+        o.fixed_flux_timestep = tau
+        gen = o.evolve(yieldstep=1e300, finaltime=1e300, max_steps=steps)
+        next(gen)
+        q0 = [o.stage_c.copy(), o.xmom_c.copy(), o.ymom_c.copy()]
+        for _ in gen:
+            pass
+        assert o.dt_history == [tau] * steps
+        return q0, [o.stage_c.copy(), o.xmom_c.copy(), o.ymom_c.copy()]
+
+    q0, rk2 = run("rk2", 1)
+    _, e2 = run("euler", 2)
+    wet = (q["stage"] - q["elevation"]) > 1e-5
+    assert 0.1 < wet.mean() < 0.9          # a wet/dry case
+    for a, b, c in zip(rk2, e2, q0):
+        assert np.array_equal(a, 0.5 * b + 0.5 * c)
+    # and it is not the plain second Euler stage
+    assert not np.array_equal(rk2[0], e2[0])
