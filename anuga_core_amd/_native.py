@@ -102,6 +102,9 @@ SIGNATURES = {
     "anuga_ensemble_get_water_volumes": (C.c_int, [C.c_void_p, _dp]),
     "anuga_ensemble_get_boundary_flux_integrals": (C.c_int, [C.c_void_p, _dp]),
     "anuga_ensemble_size": (C.c_int, [C.c_void_p, _ip]),
+    "anuga_adjoint_record": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_int64, C.c_int64,
+                                       _ip, C.c_int64, _dp]),
+    "anuga_adjoint_gradient": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
 }
 
 _lib = None

@@ -1,0 +1,860 @@
+// Cell-level math of the FP64 DE step in generic form, for the discrete
+// adjoint.
+//
+// The production kernels are written for speed (packed loads, named copies,
+// precision templates). Here the same arithmetic, in the same operation
+// order, is restated once more over a scalar type T, so that it can run with
+// T = double (the value; bit-identical to the production kernels, checked
+// against the oracle on the CPU) and with T = Dual<K> (forward-mode
+// derivatives along K tangent directions). The adjoint kernels evaluate a
+// cell with dual numbers seeded on one group of inputs at a time and
+// contract the output tangents with the output adjoints: a vector-Jacobian
+// product per cell, exact for the piecewise-smooth function the forward
+// computes (branches are taken on the value part, as the forward takes them).
+//
+// Functions are __host__ __device__ so that tests can run them on the CPU.
+#pragma once
+
+#include <math.h>
+
+#include "de_pow.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define ADJ_HD __host__ __device__ __forceinline__
+#else
+#define ADJ_HD inline
+#endif
+
+// ---------------------------------------------------------------------------
+// forward-mode dual numbers
+// ---------------------------------------------------------------------------
+template <int K>
+struct Dual {
+    double v;
+    double d[K];
+    ADJ_HD Dual() {}
+    ADJ_HD Dual(double x) : v(x)
+    {
+#pragma unroll
+        for (int k = 0; k < K; k++) d[k] = 0.0;
+    }
+};
+
+template <int K>
+ADJ_HD Dual<K> operator+(const Dual<K> &a, const Dual<K> &b)
+{
+    Dual<K> r;
+    r.v = a.v + b.v;
+#pragma unroll
+    for (int k = 0; k < K; k++) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+template <int K>
+ADJ_HD Dual<K> operator-(const Dual<K> &a, const Dual<K> &b)
+{
+    Dual<K> r;
+    r.v = a.v - b.v;
+#pragma unroll
+    for (int k = 0; k < K; k++) r.d[k] = a.d[k] - b.d[k];
+    return r;
+}
+template <int K>
+ADJ_HD Dual<K> operator-(const Dual<K> &a)
+{
+    Dual<K> r;
+    r.v = -a.v;
+#pragma unroll
+    for (int k = 0; k < K; k++) r.d[k] = -a.d[k];
+    return r;
+}
+template <int K>
+ADJ_HD Dual<K> operator*(const Dual<K> &a, const Dual<K> &b)
+{
+    Dual<K> r;
+    r.v = a.v * b.v;
+#pragma unroll
+    for (int k = 0; k < K; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+template <int K>
+ADJ_HD Dual<K> operator/(const Dual<K> &a, const Dual<K> &b)
+{
+    Dual<K> r;
+    r.v = a.v / b.v;
+    const double ib = 1.0 / b.v;
+#pragma unroll
+    for (int k = 0; k < K; k++) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
+    return r;
+}
+template <int K> ADJ_HD Dual<K> operator+(const Dual<K> &a, double b) { return a + Dual<K>(b); }
+template <int K> ADJ_HD Dual<K> operator+(double a, const Dual<K> &b) { return Dual<K>(a) + b; }
+template <int K> ADJ_HD Dual<K> operator-(const Dual<K> &a, double b) { return a - Dual<K>(b); }
+template <int K> ADJ_HD Dual<K> operator-(double a, const Dual<K> &b) { return Dual<K>(a) - b; }
+template <int K> ADJ_HD Dual<K> operator*(const Dual<K> &a, double b) { return a * Dual<K>(b); }
+template <int K> ADJ_HD Dual<K> operator*(double a, const Dual<K> &b) { return Dual<K>(a) * b; }
+template <int K> ADJ_HD Dual<K> operator/(const Dual<K> &a, double b) { return a / Dual<K>(b); }
+template <int K> ADJ_HD Dual<K> operator/(double a, const Dual<K> &b) { return Dual<K>(a) / b; }
+template <int K> ADJ_HD Dual<K> &operator+=(Dual<K> &a, const Dual<K> &b) { return a = a + b; }
+template <int K> ADJ_HD Dual<K> &operator-=(Dual<K> &a, const Dual<K> &b) { return a = a - b; }
+template <int K> ADJ_HD Dual<K> &operator*=(Dual<K> &a, const Dual<K> &b) { return a = a * b; }
+template <int K> ADJ_HD bool operator<(const Dual<K> &a, const Dual<K> &b) { return a.v < b.v; }
+template <int K> ADJ_HD bool operator>(const Dual<K> &a, const Dual<K> &b) { return a.v > b.v; }
+template <int K> ADJ_HD bool operator<=(const Dual<K> &a, const Dual<K> &b) { return a.v <= b.v; }
+template <int K> ADJ_HD bool operator>=(const Dual<K> &a, const Dual<K> &b) { return a.v >= b.v; }
+template <int K> ADJ_HD bool operator==(const Dual<K> &a, const Dual<K> &b) { return a.v == b.v; }
+
+// the value part of a scalar
+ADJ_HD double adj_val(double x) { return x; }
+template <int K> ADJ_HD double adj_val(const Dual<K> &x) { return x.v; }
+
+// max / min: the argument the forward's fmax / fmin returns (ties: the first)
+ADJ_HD double adj_max(double a, double b) { return fmax(a, b); }
+ADJ_HD double adj_min(double a, double b) { return fmin(a, b); }
+template <int K> ADJ_HD Dual<K> adj_max(const Dual<K> &a, const Dual<K> &b)
+{
+    return (a.v >= b.v) ? a : b;
+}
+template <int K> ADJ_HD Dual<K> adj_min(const Dual<K> &a, const Dual<K> &b)
+{
+    return (a.v <= b.v) ? a : b;
+}
+
+ADJ_HD double adj_sqrt(double a) { return sqrt(a); }
+template <int K> ADJ_HD Dual<K> adj_sqrt(const Dual<K> &a)
+{
+    Dual<K> r;
+    r.v = sqrt(a.v);
+    // sqrt is not differentiable at 0; every use here is sqrt(g h) or a
+    // momentum magnitude whose product with the tangent direction vanishes there
+    const double s = r.v > 0.0 ? 0.5 / r.v : 0.0;
+#pragma unroll
+    for (int k = 0; k < K; k++) r.d[k] = a.d[k] * s;
+    return r;
+}
+
+// h^(7/3) with the correctly rounded value of the FP64 path
+ADJ_HD double adj_pow73(double h) { return de_pow_7_3(h); }
+template <int K> ADJ_HD Dual<K> adj_pow73(const Dual<K> &h)
+{
+    Dual<K> r;
+    r.v = de_pow_7_3(h.v);
+    const double s = (7.0 / 3.0) * r.v / h.v;
+#pragma unroll
+    for (int k = 0; k < K; k++) r.d[k] = h.d[k] * s;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// parameters and flags (the same values the production kernels use)
+// ---------------------------------------------------------------------------
+struct AdjParams {
+    double g, eps, mah;
+    double bw, bwd, buh, buhd, bvh, bvhd;
+};
+
+#define ADJ_F_CLAMP 1u
+#define ADJ_F_KEEP 2u
+
+// ---------------------------------------------------------------------------
+// limiter and gradients (sw_domain_orig.c:1306-1333, :2238-2277, :2470-2504)
+// ---------------------------------------------------------------------------
+template <class T>
+ADJ_HD void adj_limit(T d[3], T qmin, T qmax, T beta)
+{
+    T r = T(1000.0), r0 = T(1.0);
+    const T tiny = T(1.0e-100);
+    const T mtiny = T(-1.0e-100);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (d[i] < mtiny) r0 = qmin / d[i];
+        if (d[i] > tiny) r0 = qmax / d[i];
+        r = adj_min(r0, r);
+    }
+    const T phi = adj_min(r * beta, T(1.0));
+    d[0] = d[0] * phi;
+    d[1] = d[1] * phi;
+    d[2] = d[2] * phi;
+}
+
+template <class T>
+ADJ_HD void adj_aux_jumps(T qk, T q0, T q1, T q2, double dx1, double dx2, double dy1, double dy2,
+                          double inv_area2, const double dxv[3], const double dyv[3], T beta,
+                          T d[3])
+{
+    const T dq0 = q0 - qk;
+    const T dq1 = q1 - q0;
+    const T dq2 = q2 - q0;
+    T a = T(dy2) * dq1 - T(dy1) * dq2;
+    a *= T(inv_area2);
+    T b = T(dx1) * dq2 - T(dx2) * dq1;
+    b *= T(inv_area2);
+#pragma unroll
+    for (int i = 0; i < 3; i++) d[i] = a * T(dxv[i]) + b * T(dyv[i]);
+    const T zero = T(0.0);
+    const T qmax = adj_max(adj_max(dq0, adj_max(dq0 + dq1, dq0 + dq2)), zero);
+    const T qmin = adj_min(adj_min(dq0, adj_min(dq0 + dq1, dq0 + dq2)), zero);
+    adj_limit(d, qmin, qmax, beta);
+}
+
+template <class T>
+ADJ_HD void adj_line_jumps(T qk, T q1, double dx2, double dy2, const double dxv[3],
+                           const double dyv[3], T beta, T d[3])
+{
+    const T dq1 = q1 - qk;
+    const T a = dq1 * T(dx2);
+    const T b = dq1 * T(dy2);
+#pragma unroll
+    for (int i = 0; i < 3; i++) d[i] = a * T(dxv[i]) + b * T(dyv[i]);
+    T qmin, qmax;
+    if (dq1 >= T(0.0)) {
+        qmin = T(0.0);
+        qmax = dq1;
+    } else {
+        qmin = dq1;
+        qmax = T(0.0);
+    }
+    adj_limit(d, qmin, qmax, beta);
+}
+
+// ---------------------------------------------------------------------------
+// extrapolation of one triangle (sw_domain_orig.c:1994-2680)
+// ---------------------------------------------------------------------------
+// Inputs: raw state (w, uh, vh, z) of the triangle (slot 0) and of its three
+// surrogate neighbours (slots 1-3; a boundary edge's surrogate is the triangle
+// itself), the flags of the four (protect clamp / momentum kept, which the
+// forward derived from the same values and from the neighbours' neighbours),
+// the number of boundary edges, which edges are interior, and the geometry:
+// edge-midpoint offsets dxv/dyv and the surrogate-centroid differences.
+// Output: stage, height, xmom, ymom on the three edges.
+struct AdjExtrapGeo {
+    double dxv[3], dyv[3];
+    double dx1, dx2, dy1, dy2;   // nbound <= 1: centroid differences sur1-sur0, sur2-sur0
+    double lx, ly;               // nbound == 2: first interior surrogate - own centroid
+    int nbound;
+    bool interior[3];
+};
+
+template <class T>
+ADJ_HD void adj_extrap_cell(const T w[4], const T uh[4], const T vh[4], const T z[4],
+                            const unsigned fl[4], const AdjExtrapGeo &gm, const AdjParams &p,
+                            T ew[3], T eh[3], T euh[3], T evh[3])
+{
+    const T zero = T(0.0);
+    // own centroid after protect and the velocity conversion
+    const T cw = (fl[0] & ADJ_F_CLAMP) ? z[0] : w[0];
+    const T ch = adj_max(cw - z[0], zero);
+    T cu = zero, cv = zero;
+    if (fl[0] & ADJ_F_KEEP) {
+        const T inv = T(1.0) / ch;
+        cu = uh[0] * inv;
+        cv = vh[0] * inv;
+    }
+    // surrogates
+    T nw[3], nh[3], nu[3], nv[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const T hs = adj_max(w[i + 1] - z[i + 1], zero);
+        nw[i] = (fl[i + 1] & ADJ_F_CLAMP) ? z[i + 1] : w[i + 1];
+        nh[i] = hs;
+        nu[i] = zero;
+        nv[i] = zero;
+        if (fl[i + 1] & ADJ_F_KEEP) {
+            const T inv = T(1.0) / hs;
+            nu[i] = uh[i + 1] * inv;
+            nv[i] = vh[i + 1] * inv;
+        }
+    }
+    T jw[3] = {zero, zero, zero}, jh[3] = {zero, zero, zero};
+    T ju[3] = {zero, zero, zero}, jv[3] = {zero, zero, zero};
+    bool constant = (gm.nbound == 3);
+    bool gw = true, gu = true, gv = true;
+    if (gm.nbound <= 1) {
+        const double area2 = gm.dy2 * gm.dx1 - gm.dy1 * gm.dx2;
+        if (area2 <= 0.0) {
+            constant = true;
+        } else {
+            const double a_tmp = 0.3, b_tmp = 0.1;
+            const double c_tmp = 1.0 / (a_tmp - b_tmp);
+            const double d_tmp = 1.0 - (c_tmp * a_tmp);
+            const T hmin = adj_min(adj_min(nh[0], adj_min(nh[1], nh[2])), ch);
+            const T hmax = adj_max(adj_max(nh[0], adj_max(nh[1], nh[2])), ch);
+            T hfactor = adj_max(zero, adj_min(T(c_tmp) * adj_max(hmin, zero) / adj_max(ch, T(1.0e-06)) + T(d_tmp),
+                                              adj_min(T(c_tmp) * adj_max(ch, zero) / adj_max(hmax, T(1.0e-06)) + T(d_tmp),
+                                                      T(1.0))));
+            hfactor = adj_min(T(1.2) * adj_max(hmin - T(p.mah), zero) / (adj_max(hmin, zero) + T(1.0) * T(p.mah)),
+                              hfactor);
+            const double inv_area2 = 1.0 / area2;
+            T beta = T(p.bwd) + (T(p.bw) - T(p.bwd)) * hfactor;
+            gw = beta > zero;
+            if (gw) {
+                adj_aux_jumps<T>(cw, nw[0], nw[1], nw[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2,
+                                 inv_area2, gm.dxv, gm.dyv, beta, jw);
+                adj_aux_jumps<T>(ch, nh[0], nh[1], nh[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2,
+                                 inv_area2, gm.dxv, gm.dyv, beta, jh);
+            }
+            beta = T(p.buhd) + (T(p.buh) - T(p.buhd)) * hfactor;
+            gu = beta > zero;
+            if (gu)
+                adj_aux_jumps<T>(cu, nu[0], nu[1], nu[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2,
+                                 inv_area2, gm.dxv, gm.dyv, beta, ju);
+            beta = T(p.bvhd) + (T(p.bvh) - T(p.bvhd)) * hfactor;
+            gv = beta > zero;
+            if (gv)
+                adj_aux_jumps<T>(cv, nv[0], nv[1], nv[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2,
+                                 inv_area2, gm.dxv, gm.dyv, beta, jv);
+        }
+    } else if (gm.nbound == 2) {
+        const int s = gm.interior[0] ? 0 : (gm.interior[1] ? 1 : 2);
+        if (!(gm.interior[0] || gm.interior[1] || gm.interior[2])) {
+            constant = true;
+        } else {
+            const double area2 = gm.lx * gm.lx + gm.ly * gm.ly;
+            double dx2 = 1.0 / area2;
+            const double dy2 = dx2 * gm.ly;
+            dx2 *= gm.lx;
+            const T bw = T(p.bw);
+            adj_line_jumps<T>(cw, nw[s], dx2, dy2, gm.dxv, gm.dyv, bw, jw);
+            adj_line_jumps<T>(ch, nh[s], dx2, dy2, gm.dxv, gm.dyv, bw, jh);
+            adj_line_jumps<T>(cu, nu[s], dx2, dy2, gm.dxv, gm.dyv, bw, ju);
+            adj_line_jumps<T>(cv, nv[s], dx2, dy2, gm.dxv, gm.dyv, bw, jv);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        ew[i] = (constant || !gw) ? cw : cw + jw[i];
+        eh[i] = (constant || !gw) ? ch : ch + jh[i];
+        const T eu = (constant || !gu) ? cu : cu + ju[i];
+        const T ev = (constant || !gv) ? cv : cv + jv[i];
+        euh[i] = eu * eh[i];
+        evh[i] = ev * eh[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// central-upwind flux across one edge, low_froude = 0 (sw_domain_orig.c:284-505)
+// ---------------------------------------------------------------------------
+template <class T>
+ADJ_HD void adj_flux_central(T ql0, T ql1, T ql2, T qr0, T qr1, T qr2, T h_left, T h_right,
+                             T hle, T hre, double n1, double n2, double eps, T ze, double g,
+                             T ef[3], T &pressure_flux)
+{
+    const T zero = T(0.0);
+    if (h_left == zero && h_right == zero) {
+        ef[0] = ef[1] = ef[2] = zero;
+        pressure_flux = zero;
+        return;
+    }
+    const T l1 = T(n1) * ql1 + T(n2) * ql2;
+    const T l2 = T(-n2) * ql1 + T(n1) * ql2;
+    const T r1 = T(n1) * qr1 + T(n2) * qr2;
+    const T r2 = T(-n2) * qr1 + T(n1) * qr2;
+    T ul, uhl, vhl, ur, uhr, vhr;
+    uhl = l1;
+    vhl = l2;
+    if (hle > zero) {
+        const T t = T(1.0) / hle;
+        ul = uhl * t;
+        uhl = h_left * ul;
+        vhl = h_left * t * vhl;
+    } else {
+        ul = uhl = vhl = zero;
+    }
+    uhr = r1;
+    vhr = r2;
+    if (hre > zero) {
+        const T t = T(1.0) / hre;
+        ur = uhr * t;
+        uhr = h_right * ur;
+        vhr = h_right * t * vhr;
+    } else {
+        ur = uhr = vhr = zero;
+    }
+    const T cl = adj_sqrt(T(g) * h_left);
+    const T cr = adj_sqrt(T(g) * h_right);
+    T s_max = adj_max(ul + cl, ur + cr);
+    if (s_max < zero) s_max = zero;
+    T s_min = adj_min(ul - cl, ur - cr);
+    if (s_min > zero) s_min = zero;
+    const T fl0 = ul * h_left, fl1 = ul * uhl, fl2 = ul * vhl;
+    const T fr0 = ur * h_right, fr1 = ur * uhr, fr2 = ur * vhr;
+    const T denom = s_max - s_min;
+    if (denom < T(eps)) {
+        ef[0] = ef[1] = ef[2] = zero;
+        pressure_flux = T(0.5) * T(g) * T(0.5) * (h_left * h_left + h_right * h_right);
+        return;
+    }
+    const T inv = T(1.0) / adj_max(denom, T(1.0e-100));
+    const T sms = s_max * s_min;
+    T e0 = s_max * fl0 - s_min * fr0;
+    e0 += sms * (adj_max(qr0, ze) - adj_max(ql0, ze));
+    e0 *= inv;
+    T e1 = s_max * fl1 - s_min * fr1;
+    e1 += T(1.0) * sms * (uhr - uhl);
+    e1 *= inv;
+    T e2 = s_max * fl2 - s_min * fr2;
+    e2 += T(1.0) * sms * (vhr - vhl);
+    e2 *= inv;
+    pressure_flux = T(0.5) * T(g) * (s_max * h_left * h_left - s_min * h_right * h_right) * inv;
+    ef[0] = e0;
+    ef[1] = T(n1) * e1 + T(-n2) * e2;
+    ef[2] = -T(-n2) * e1 + T(n1) * e2;
+}
+
+// One edge's contribution to the triangle's explicit update, before the
+// division by the area (sw_domain_orig.c:879-1004 and :1118-1132), as the
+// five terms the forward accumulates in this order: sw += out[0], sx += out[1],
+// sy += out[2], sx -= out[3], sy -= out[4]. Inputs are
+// the triangle's own edge values (left), the neighbour's edge values (right;
+// for a boundary edge derived from the left ones by the boundary kind), and
+// the triangle's centroid depth and bed. kind: -1 interior, else the
+// boundary kind; dir: Dirichlet [stage, xmom, ymom]; cen: the centroid
+// stage / xmom / ymom (centroid-transmissive kind).
+template <class T>
+ADJ_HD void adj_edge_update(T ql0, T hle, T ql1, T ql2, T qr0_in, T hre_in, T qr1_in, T qr2_in,
+                            int kind, const double dir[3], const T cen[3], T hc, T zc, double n1,
+                            double n2, double length, const AdjParams &p, T out[5])
+{
+    const T zero = T(0.0);
+    const T zl = ql0 - hle;
+    T qr0, qr1, qr2, zr, hre;
+    if (kind < 0) {
+        qr0 = qr0_in;
+        qr1 = qr1_in;
+        qr2 = qr2_in;
+        hre = hre_in;
+        zr = qr0 - hre;
+    } else {
+        if (kind == 0) {
+            const T r1 = -ql1 * T(n1) - ql2 * T(n2);
+            const T r2 = -ql1 * T(n2) + ql2 * T(n1);
+            qr0 = ql0;
+            qr1 = T(n1) * r1 - T(n2) * r2;
+            qr2 = T(n2) * r1 + T(n1) * r2;
+        } else if (kind == 1) {
+            qr0 = ql0;
+            qr1 = ql1;
+            qr2 = ql2;
+        } else if (kind == 2) {
+            qr0 = T(dir[0]);
+            qr1 = T(dir[1]);
+            qr2 = T(dir[2]);
+        } else {
+            qr0 = cen[0];
+            qr1 = cen[1];
+            qr2 = cen[2];
+        }
+        zr = zl;
+        hre = adj_max(qr0 - zr, zero);
+    }
+    const T z_half = adj_max(zl, zr);
+    const T h_left = adj_max(hle + zl - z_half, zero);
+    const T h_right = adj_max(hre + zr - z_half, zero);
+    T ef[3], pflux;
+    adj_flux_central<T>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1, n2, p.eps,
+                        z_half, p.g, ef, pflux);
+    const T len = T(length);
+    ef[0] = ef[0] * len;
+    ef[1] = ef[1] * len;
+    ef[2] = ef[2] * len;
+    const T pg = len * (-T(p.g) * T(0.5) * (h_left * h_left - hle * hle - (hle + hc) * (zl - zc)) + pflux);
+    out[0] = -ef[0];
+    out[1] = -ef[1];
+    out[2] = -ef[2];
+    out[3] = T(n1) * pg;
+    out[4] = T(n2) * pg;
+}
+
+// ---------------------------------------------------------------------------
+// friction + update + fix_negative_cells of one triangle
+// (sw_domain_orig.c:2805-2838, quantity.c:772-822, sw_domain_orig.c:2890-2910)
+// ---------------------------------------------------------------------------
+template <class T>
+ADJ_HD T adj_update_one(T q, T ex, T si, double dt)
+{
+    if (q == T(0.0)) si = T(0.0);
+    else si = si / q;
+    q = q + T(dt) * ex;
+    const T den = T(1.0) - T(dt) * si;
+    return q / den;
+}
+
+// c: centroid w, uh, vh, z after protect; ex: explicit updates (already / area)
+template <class T>
+ADJ_HD void adj_advance_cell(T cw, T cuh, T cvh, T cz, T eta, T exw, T exuh, T exvh, double dt,
+                             const AdjParams &p, T out[3])
+{
+    T si_uh = T(0.0), si_vh = T(0.0);
+    if (eta > T(p.mah)) {
+        const T h = cw - cz;
+        if (h >= T(p.mah)) {
+            T S = -T(p.g) * eta * eta * adj_sqrt((cuh * cuh + cvh * cvh));
+            S = S / adj_pow73(h);
+            si_uh = si_uh + S * cuh;
+            si_vh = si_vh + S * cvh;
+        }
+    }
+    T rw = adj_update_one<T>(cw, exw, T(0.0), dt);
+    T ru = adj_update_one<T>(cuh, exuh, si_uh, dt);
+    T rv = adj_update_one<T>(cvh, exvh, si_vh, dt);
+    if (rw - cz < T(0.0)) {
+        rw = cz;
+        ru = T(0.0);
+        rv = T(0.0);
+    }
+    out[0] = rw;
+    out[1] = ru;
+    out[2] = rv;
+}
+
+// ---------------------------------------------------------------------------
+// per-triangle assembly over the library's device layouts
+// ---------------------------------------------------------------------------
+// Layouts (internal triangle order): state records S[4k..4k+3] = w, uh, vh, z;
+// nb[4k+i] = (n << 2) | m for the neighbour n across edge i (its edge m), or
+// -(j+1) for boundary edge j, nb[4k+3] = number of boundary edges; cen[2k..];
+// gx[6k..] = edge-midpoint offsets x0..x2, y0..y2; per-edge arrays [3][N];
+// edge values E = w, h, uh, vh blocks of [3][N]; adjoints of the state [3][N].
+struct AdjView {
+    int N, M;
+    const int *nb;
+    const double *cen, *gx, *nx, *ny, *len, *area, *eta;
+    const signed char *bkind;
+    const double *bval;   // [3][M]
+    AdjParams p;
+    double dt;
+};
+
+// accumulate into an adjoint: several triangles add to the same entry
+ADJ_HD void adj_add(double *a, double v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    atomicAdd(a, v);
+#else
+    *a += v;
+#endif
+}
+
+ADJ_HD void adj_neighbours(const AdjView &m, int k, int v[3], int sur[3])
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        v[i] = m.nb[4 * k + i];
+        sur[i] = v[i] >= 0 ? (v[i] >> 2) : k;
+    }
+}
+
+ADJ_HD AdjExtrapGeo adj_geometry(const AdjView &m, int k, const int v[3], const int sur[3])
+{
+    AdjExtrapGeo g;
+    for (int i = 0; i < 3; i++) {
+        g.dxv[i] = m.gx[6 * k + i];
+        g.dyv[i] = m.gx[6 * k + 3 + i];
+        g.interior[i] = v[i] >= 0;
+    }
+    g.nbound = m.nb[4 * k + 3];
+    g.dx1 = g.dx2 = g.dy1 = g.dy2 = g.lx = g.ly = 0.0;
+    if (g.nbound <= 1) {
+        const double *c0 = m.cen + 2 * sur[0], *c1 = m.cen + 2 * sur[1], *c2 = m.cen + 2 * sur[2];
+        g.dx1 = c1[0] - c0[0];
+        g.dx2 = c2[0] - c0[0];
+        g.dy1 = c1[1] - c0[1];
+        g.dy2 = c2[1] - c0[1];
+    } else if (g.nbound == 2) {
+        const int s1 = v[0] >= 0 ? sur[0] : (v[1] >= 0 ? sur[1] : sur[2]);
+        g.lx = m.cen[2 * s1] - m.cen[2 * k];
+        g.ly = m.cen[2 * s1 + 1] - m.cen[2 * k + 1];
+    }
+    return g;
+}
+
+// protect / momentum flags of triangle k from the state (the production
+// extrapolation's flag byte)
+ADJ_HD unsigned adj_flags(const AdjView &m, const double *S, int k)
+{
+    int v[3], sur[3];
+    adj_neighbours(m, k, v, sur);
+    const double w = S[4 * k], z = S[4 * k + 3], mah = m.p.mah;
+    const double hc = w - z;
+    const bool clamp = (hc < mah * 1.0) && (hc <= 0.0) && (w < z);
+    const double cw = clamp ? z : w;
+    const double ch = fmax(cw - z, 0.0);
+    bool dry = true;
+    for (int i = 0; i < 3; i++)
+        dry = dry && (v[i] < 0 || fmax(S[4 * sur[i]] - S[4 * sur[i] + 3], 0.0) < mah);
+    const bool keep = (ch > mah) && !dry;
+    return (clamp ? ADJ_F_CLAMP : 0u) | (keep ? ADJ_F_KEEP : 0u);
+}
+
+// centroid state after the flags (what the flux and the update read)
+template <class T>
+ADJ_HD void adj_centroid(const T q[4], unsigned f, T c[4])
+{
+    c[0] = (f & ADJ_F_CLAMP) ? q[3] : q[0];
+    c[1] = (f & ADJ_F_KEEP) ? q[1] : T(0.0);
+    c[2] = (f & ADJ_F_KEEP) ? q[2] : T(0.0);
+    c[3] = q[3];
+}
+
+// ---- forward (T = double), for the CPU harness -------------------------------
+ADJ_HD void adj_fwd_extrap(const AdjView &m, const double *S, const unsigned char *fl, int k,
+                           double *E)
+{
+    int v[3], sur[3];
+    adj_neighbours(m, k, v, sur);
+    const AdjExtrapGeo g = adj_geometry(m, k, v, sur);
+    const int id[4] = {k, sur[0], sur[1], sur[2]};
+    double w[4], uh[4], vh[4], z[4];
+    unsigned f[4];
+    for (int j = 0; j < 4; j++) {
+        w[j] = S[4 * id[j]];
+        uh[j] = S[4 * id[j] + 1];
+        vh[j] = S[4 * id[j] + 2];
+        z[j] = S[4 * id[j] + 3];
+        f[j] = fl[id[j]];
+    }
+    double ew[3], eh[3], euh[3], evh[3];
+    adj_extrap_cell<double>(w, uh, vh, z, f, g, m.p, ew, eh, euh, evh);
+    const size_t N = m.N;
+    for (int i = 0; i < 3; i++) {
+        E[i * N + k] = ew[i];
+        E[3 * N + i * N + k] = eh[i];
+        E[6 * N + i * N + k] = euh[i];
+        E[9 * N + i * N + k] = evh[i];
+    }
+}
+
+// inputs of edge i of triangle k: own edge values, the neighbour's, boundary kind
+ADJ_HD void adj_edge_inputs(const AdjView &m, const double *E, int k, int i, int vi, double L[4],
+                            double R[4], int &kind, double dir[3])
+{
+    const size_t N = m.N;
+    const size_t ki = (size_t)i * N + k;
+    L[0] = E[ki];
+    L[1] = E[3 * N + ki];
+    L[2] = E[6 * N + ki];
+    L[3] = E[9 * N + ki];
+    dir[0] = dir[1] = dir[2] = 0.0;
+    if (vi >= 0) {
+        const size_t nm = (size_t)(vi & 3) * N + (vi >> 2);
+        R[0] = E[nm];
+        R[1] = E[3 * N + nm];
+        R[2] = E[6 * N + nm];
+        R[3] = E[9 * N + nm];
+        kind = -1;
+    } else {
+        const int j = -vi - 1;
+        kind = m.bkind[j];
+        R[0] = R[1] = R[2] = R[3] = 0.0;
+        if (kind == 2) {
+            dir[0] = m.bval[j];
+            dir[1] = m.bval[m.M + j];
+            dir[2] = m.bval[2 * m.M + j];
+        }
+    }
+}
+
+// one edge's five update terms from (L, R, C = centroid w, uh, vh, z)
+template <class T>
+ADJ_HD void adj_edge_terms(const AdjView &m, int k, int i, const T L[4], const T R[4],
+                           const T C[4], int kind, const double dir[3], T out[5])
+{
+    const T zero = T(0.0);
+    const T hc = adj_max(C[0] - C[3], zero);
+    const T cen[3] = {C[0], C[1], C[2]};
+    const size_t N = m.N, ki = (size_t)i * N + k;
+    // argument order of the edge function: ql0, hle, ql1, ql2, qr0, hre, qr1, qr2
+    adj_edge_update<T>(L[0], L[1], L[2], L[3], R[0], R[1], R[2], R[3], kind, dir, cen, hc, C[3],
+                       m.nx[ki], m.ny[ki], m.len[ki], m.p, out);
+}
+
+ADJ_HD void adj_fwd_flux(const AdjView &m, const double *S, const unsigned char *fl,
+                         const double *E, int k, double ex[3])
+{
+    int v[3], sur[3];
+    adj_neighbours(m, k, v, sur);
+    double q[4] = {S[4 * k], S[4 * k + 1], S[4 * k + 2], S[4 * k + 3]}, c[4];
+    adj_centroid<double>(q, fl[k], c);
+    double sw = 0.0, sx = 0.0, sy = 0.0;
+    for (int i = 0; i < 3; i++) {
+        double L[4], R[4], dir[3], o[5];
+        int kind;
+        adj_edge_inputs(m, E, k, i, v[i], L, R, kind, dir);
+        // the edge function takes (w, h, uh, vh) in the order ql0, hle, ql1, ql2
+        adj_edge_terms<double>(m, k, i, L, R, c, kind, dir, o);
+        sw += o[0];
+        sx += o[1];
+        sy += o[2];
+        sx -= o[3];
+        sy -= o[4];
+    }
+    const double inv_area = 1.0 / m.area[k];
+    ex[0] = sw * inv_area;
+    ex[1] = sx * inv_area;
+    ex[2] = sy * inv_area;
+}
+
+ADJ_HD void adj_fwd_advance(const AdjView &m, const double *S, const unsigned char *fl,
+                            const double ex[3], int k, double out[3])
+{
+    double q[4] = {S[4 * k], S[4 * k + 1], S[4 * k + 2], S[4 * k + 3]}, c[4];
+    adj_centroid<double>(q, fl[k], c);
+    adj_advance_cell<double>(c[0], c[1], c[2], c[3], m.eta[k], ex[0], ex[1], ex[2], m.dt, m.p, out);
+}
+
+// ---- adjoints ----------------------------------------------------------------------
+// Map an adjoint of the centroid values (w, uh, vh, z after the flags) onto
+// the raw state of triangle k and the bed.
+ADJ_HD void adj_scatter_centroid(unsigned f, const double lc[4], double *lamS, double *zbar,
+                                 int N, int k)
+{
+    const double lw = (f & ADJ_F_CLAMP) ? 0.0 : lc[0];
+    const double lz = lc[3] + ((f & ADJ_F_CLAMP) ? lc[0] : 0.0);
+    const double lu = (f & ADJ_F_KEEP) ? lc[1] : 0.0;
+    const double lv = (f & ADJ_F_KEEP) ? lc[2] : 0.0;
+    if (lw != 0.0) adj_add(lamS + k, lw);
+    if (lu != 0.0) adj_add(lamS + N + k, lu);
+    if (lv != 0.0) adj_add(lamS + 2 * (size_t)N + k, lv);
+    if (lz != 0.0) adj_add(zbar + k, lz);
+}
+
+// Update of triangle k: lam_out (w, uh, vh of the result) -> adjoints of the
+// raw state (lamS, zbar), of the friction coefficient (nbar) and of the
+// explicit updates (lam_ex, stored).
+ADJ_HD void adj_bwd_advance(const AdjView &m, const double *S, const unsigned char *fl,
+                            const double *exw, const double *exuh, const double *exvh,
+                            const double lam_out[3], int k, double *lamS, double *zbar,
+                            double *nbar, double *lam_ex)
+{
+    typedef Dual<4> D;
+    const int N = m.N;
+    double q[4] = {S[4 * k], S[4 * k + 1], S[4 * k + 2], S[4 * k + 3]}, c[4];
+    const unsigned f = fl[k];
+    adj_centroid<double>(q, f, c);
+    const double in[8] = {c[0], c[1], c[2], c[3], m.eta[k], exw[k], exuh[k], exvh[k]};
+    double g[8];
+    for (int pass = 0; pass < 2; pass++) {
+        D x[8];
+        for (int j = 0; j < 8; j++) {
+            x[j] = D(in[j]);
+            if (j / 4 == pass) x[j].d[j % 4] = 1.0;
+        }
+        D o[3];
+        adj_advance_cell<D>(x[0], x[1], x[2], x[3], x[4], x[5], x[6], x[7], m.dt, m.p, o);
+        for (int t = 0; t < 4; t++)
+            g[4 * pass + t] = lam_out[0] * o[0].d[t] + lam_out[1] * o[1].d[t] + lam_out[2] * o[2].d[t];
+    }
+    const double lc[4] = {g[0], g[1], g[2], g[3]};
+    adj_scatter_centroid(f, lc, lamS, zbar, N, k);
+    if (g[4] != 0.0) adj_add(nbar + k, g[4]);
+    lam_ex[k] = g[5];
+    lam_ex[N + k] = g[6];
+    lam_ex[2 * (size_t)N + k] = g[7];
+}
+
+// Flux of triangle k: lam_ex (its explicit update) -> adjoints of its own and
+// its neighbours' edge values (lamE) and of its centroid state.
+ADJ_HD void adj_bwd_flux(const AdjView &m, const double *S, const unsigned char *fl,
+                         const double *E, const double *lam_ex, int k, double *lamE,
+                         double *lamS, double *zbar)
+{
+    typedef Dual<4> D;
+    const int N = m.N;
+    int v[3], sur[3];
+    adj_neighbours(m, k, v, sur);
+    double q[4] = {S[4 * k], S[4 * k + 1], S[4 * k + 2], S[4 * k + 3]}, c[4];
+    const unsigned f = fl[k];
+    adj_centroid<double>(q, f, c);
+    const double inv_area = 1.0 / m.area[k];
+    const double lw = lam_ex[k] * inv_area, lx = lam_ex[N + k] * inv_area;
+    const double ly = lam_ex[2 * (size_t)N + k] * inv_area;
+    const double lo[5] = {lw, lx, ly, -lx, -ly};
+    double lc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < 3; i++) {
+        double L[4], R[4], dir[3];
+        int kind;
+        adj_edge_inputs(m, E, k, i, v[i], L, R, kind, dir);
+        for (int pass = 0; pass < 3; pass++) {
+            if (pass == 1 && kind >= 0) continue;   // no neighbour edge values
+            D Ld[4], Rd[4], Cd[4];
+            for (int j = 0; j < 4; j++) {
+                Ld[j] = D(L[j]);
+                Rd[j] = D(R[j]);
+                Cd[j] = D(c[j]);
+            }
+            D *seed = pass == 0 ? Ld : (pass == 1 ? Rd : Cd);
+            for (int j = 0; j < 4; j++) seed[j].d[j] = 1.0;
+            D o[5];
+            adj_edge_terms<D>(m, k, i, Ld, Rd, Cd, kind, dir, o);
+            double gj[4];
+            for (int t = 0; t < 4; t++) {
+                gj[t] = 0.0;
+                for (int r = 0; r < 5; r++) gj[t] += lo[r] * o[r].d[t];
+            }
+            if (pass == 2) {
+                for (int t = 0; t < 4; t++) lc[t] += gj[t];
+            } else {
+                const size_t slot = pass == 0 ? (size_t)i * N + k
+                                              : (size_t)(v[i] & 3) * N + (v[i] >> 2);
+                for (int t = 0; t < 4; t++)
+                    if (gj[t] != 0.0) adj_add(lamE + (size_t)t * 3 * N + slot, gj[t]);
+            }
+        }
+    }
+    adj_scatter_centroid(f, lc, lamS, zbar, N, k);
+}
+
+// Extrapolation of triangle k: lamE (its edge values) -> adjoints of the raw
+// state of k and its surrogate neighbours (and of their beds).
+ADJ_HD void adj_bwd_extrap(const AdjView &m, const double *S, const unsigned char *fl,
+                           const double *lamE, int k, double *lamS, double *zbar)
+{
+    typedef Dual<4> D;
+    const int N = m.N;
+    int v[3], sur[3];
+    adj_neighbours(m, k, v, sur);
+    const AdjExtrapGeo g = adj_geometry(m, k, v, sur);
+    const int id[4] = {k, sur[0], sur[1], sur[2]};
+    double lo[12];
+    bool any = false;
+    for (int i = 0; i < 3; i++)
+        for (int t = 0; t < 4; t++) {
+            lo[4 * i + t] = lamE[(size_t)t * 3 * N + (size_t)i * N + k];
+            any = any || lo[4 * i + t] != 0.0;
+        }
+    if (!any) return;
+    unsigned f[4];
+    for (int j = 0; j < 4; j++) f[j] = fl[id[j]];
+    // one pass per distinct triangle of the stencil: k, then each interior neighbour
+    for (int pass = 0; pass < 4; pass++) {
+        const int t = id[pass];
+        if (pass > 0 && v[pass - 1] < 0) continue;   // a boundary surrogate is k itself
+        D w[4], uh[4], vh[4], z[4];
+        for (int j = 0; j < 4; j++) {
+            w[j] = D(S[4 * id[j]]);
+            uh[j] = D(S[4 * id[j] + 1]);
+            vh[j] = D(S[4 * id[j] + 2]);
+            z[j] = D(S[4 * id[j] + 3]);
+            if (id[j] == t) {
+                w[j].d[0] = 1.0;
+                uh[j].d[1] = 1.0;
+                vh[j].d[2] = 1.0;
+                z[j].d[3] = 1.0;
+            }
+        }
+        D ew[3], eh[3], euh[3], evh[3];
+        adj_extrap_cell<D>(w, uh, vh, z, f, g, m.p, ew, eh, euh, evh);
+        double gt[4];
+        for (int c = 0; c < 4; c++) {
+            double a = 0.0;
+            for (int i = 0; i < 3; i++)
+                a += lo[4 * i] * ew[i].d[c] + lo[4 * i + 1] * eh[i].d[c] +
+                     lo[4 * i + 2] * euh[i].d[c] + lo[4 * i + 3] * evh[i].d[c];
+            gt[c] = a;
+        }
+        if (gt[0] != 0.0) adj_add(lamS + t, gt[0]);
+        if (gt[1] != 0.0) adj_add(lamS + N + t, gt[1]);
+        if (gt[2] != 0.0) adj_add(lamS + 2 * (size_t)N + t, gt[2]);
+        if (gt[3] != 0.0) adj_add(zbar + t, gt[3]);
+    }
+}

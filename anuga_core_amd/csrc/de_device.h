@@ -9,7 +9,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// threads per workgroup of the per-triangle kernels (a multiple of 64)
+#ifndef DE_BLOCK
 #define DE_BLOCK 256
+#endif
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X. Remap
 // the hardware block id so that each XCD walks one contiguous slice of the

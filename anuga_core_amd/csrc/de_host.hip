@@ -20,6 +20,7 @@
 
 #include "anuga_hip.h"
 #include "de_kernels.hip"
+#include "de_adjoint.hip"
 
 namespace {
 
@@ -86,6 +87,31 @@ struct anuga_domain {
 
     bool timing = false;
     KernelTimer timers[T_COUNT];
+
+    struct AdjRun *adj = nullptr;   // checkpoints of the last anuga_adjoint_record
+};
+
+// Device buffers of a recorded fixed-timestep run and of its adjoint.
+struct AdjRun {
+    int64_t nsteps = 0, every = 0, obs_every = 0, nckpt = 0, nobs = 0;
+    int G = 0;
+    double dt = 0.0;
+    DeQ *ckpt = nullptr;       // [nckpt][N]: state at steps 0, every, 2 every, ...
+    DeQ *seg = nullptr;        // [every][N]: the states of one segment, recomputed
+    DeQ *S1 = nullptr;         // first-stage state of the step being reversed
+    double *E0 = nullptr, *E1 = nullptr, *ex0 = nullptr, *ex1 = nullptr;
+    unsigned char *f0 = nullptr, *f1 = nullptr;
+    double *lam = nullptr, *lamn = nullptr, *lam1 = nullptr, *lamE = nullptr, *lam_ex = nullptr;
+    double *zbar = nullptr, *nbar = nullptr, *rec = nullptr;
+    int *gauges = nullptr;
+    unsigned long long *neg = nullptr;   // fix_negative count of recomputed steps (ignored)
+    ~AdjRun()
+    {
+        void *p[] = {ckpt, seg, S1, E0, E1, ex0, ex1, f0, f1, lam, lamn, lam1, lamE, lam_ex,
+                     zbar, nbar, rec, gauges, neg};
+        for (void *q : p)
+            if (q) hipFree(q);
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -965,6 +991,94 @@ static int volumes(anuga_domain *d)
 }
 
 // ---------------------------------------------------------------------------
+// fixed-timestep runs and their adjoint
+// ---------------------------------------------------------------------------
+static AdjView adj_view(const anuga_domain *d, double dt)
+{
+    AdjView m;
+    m.N = d->N;
+    m.M = d->M;
+    m.nb = reinterpret_cast<const int *>(d->nb);
+    m.cen = reinterpret_cast<const double *>(d->cen);
+    m.gx = reinterpret_cast<const double *>(d->gx);
+    m.nx = d->nx;
+    m.ny = d->ny;
+    m.len = d->len;
+    m.area = d->area;
+    m.eta = d->eta;
+    m.bkind = d->bkind;
+    m.bval = d->bval;
+    const DeParams q = dev_params(d->P);
+    m.p = AdjParams{q.g, q.eps, q.mah, q.bw, q.bwd, q.buh, q.buhd, q.bvh, q.bvhd};
+    m.dt = dt;
+    return m;
+}
+
+// one DE1 step of X in place with the timestep already in ctl->dt (the
+// production kernels; d->B, d->E, d->flags and d->ex serve as scratch)
+static void step_fixed(anuga_domain *d, DeQ *X)
+{
+    launch_extrapolate(d, X);
+    launch_flux(d, 0, X, nullptr);
+    launch_update(d, X, d->B);
+    launch_extrapolate(d, d->B);
+    launch_flux(d, 1, d->B, X);
+}
+
+// the stage quantities of the step from X into the adjoint's buffers
+static void adj_stages(anuga_domain *d, AdjRun *a, const DeQ *X)
+{
+    const dim3 g(d->nblocks), b(DE_BLOCK);
+    const size_t N = d->N;
+    DeBoundary bc{d->bkind, d->bval, d->M};
+    const DeParams p = dev_params(d->P);
+    double *pm = d->part + 2 * (size_t)d->nblocks;
+    hipLaunchKernelGGL(de_extrapolate<double>, g, b, 0, d->stream, dev_mesh(d), p, X, a->E0, a->f0,
+                       pm, d->ctl);
+    hipLaunchKernelGGL((de_flux<double, 0>), g, b, 0, d->stream, dev_mesh(d), p, X, d->eta,
+                       a->E0, bc, a->f0, a->ex0, a->ex0 + N, a->ex0 + 2 * N, nullptr, d->ctl,
+                       d->part, d->part + d->nblocks, a->neg);
+    hipLaunchKernelGGL(de_update<true>, g, b, 0, d->stream, d->scen, p, X, d->eta, a->f0, a->ex0,
+                       a->ex0 + N, a->ex0 + 2 * N, d->ctl, a->S1, a->neg);
+    hipLaunchKernelGGL(de_extrapolate<double>, g, b, 0, d->stream, dev_mesh(d), p, a->S1, a->E1,
+                       a->f1, pm, d->ctl);
+    hipLaunchKernelGGL((de_flux<double, 0>), g, b, 0, d->stream, dev_mesh(d), p, a->S1, d->eta,
+                       a->E1, bc, a->f1, a->ex1, a->ex1 + N, a->ex1 + 2 * N, nullptr, d->ctl,
+                       d->part, d->part + d->nblocks, a->neg);
+}
+
+// lam (adjoint of the step's result, [3][N]) -> adjoint of its start X
+static int adj_step(anuga_domain *d, AdjRun *a, const DeQ *X)
+{
+    const dim3 g(d->nblocks), b(DE_BLOCK);
+    const size_t N = d->N;
+    const AdjView m = adj_view(d, a->dt);
+    const double *Sx = reinterpret_cast<const double *>(X);
+    const double *S1 = reinterpret_cast<const double *>(a->S1);
+    adj_stages(d, a, X);
+    // the final saxpy: half to the step's start directly, half through the stages
+    hipLaunchKernelGGL(adj_k_scale, dim3((3 * N + DE_BLOCK - 1) / DE_BLOCK), b, 0, d->stream,
+                       (int)(3 * N), 0.5, a->lam, a->lamn);
+    HIPCHK(d, hipMemsetAsync(a->lam1, 0, 3 * N * sizeof(double), d->stream));
+    // second stage: S2 = advance(S1)
+    hipLaunchKernelGGL(adj_k_advance, g, b, 0, d->stream, m, S1, a->f1, a->ex1, a->ex1 + N,
+                       a->ex1 + 2 * N, a->lam, 0.5, a->lam1, a->zbar, a->nbar, a->lam_ex);
+    HIPCHK(d, hipMemsetAsync(a->lamE, 0, 12 * N * sizeof(double), d->stream));
+    hipLaunchKernelGGL(adj_k_flux, g, b, 0, d->stream, m, S1, a->f1, a->E1, a->lam_ex, a->lamE,
+                       a->lam1, a->zbar);
+    hipLaunchKernelGGL(adj_k_extrap, g, b, 0, d->stream, m, S1, a->f1, a->lamE, a->lam1, a->zbar);
+    // first stage: S1 = advance(X)
+    hipLaunchKernelGGL(adj_k_advance, g, b, 0, d->stream, m, Sx, a->f0, a->ex0, a->ex0 + N,
+                       a->ex0 + 2 * N, a->lam1, 1.0, a->lamn, a->zbar, a->nbar, a->lam_ex);
+    HIPCHK(d, hipMemsetAsync(a->lamE, 0, 12 * N * sizeof(double), d->stream));
+    hipLaunchKernelGGL(adj_k_flux, g, b, 0, d->stream, m, Sx, a->f0, a->E0, a->lam_ex, a->lamE,
+                       a->lamn, a->zbar);
+    hipLaunchKernelGGL(adj_k_extrap, g, b, 0, d->stream, m, Sx, a->f0, a->lamE, a->lamn, a->zbar);
+    std::swap(a->lam, a->lamn);
+    return launch_check(d);
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -1018,6 +1132,7 @@ void anuga_destroy(anuga_domain *d)
         for (auto e : d->timers[t].start) hipEventDestroy(e);
         for (auto e : d->timers[t].stop) hipEventDestroy(e);
     }
+    delete d->adj;
     if (d->stream) hipStreamDestroy(d->stream);
     delete d;
 }
@@ -1373,6 +1488,153 @@ int anuga_ensemble_size(const anuga_domain *d, int64_t *nscenarios)
 {
     if (!d || !nscenarios) return ANUGA_E_ARG;
     *nscenarios = d->S;
+    return ANUGA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// gradients
+// ---------------------------------------------------------------------------
+int anuga_adjoint_record(anuga_domain *d, int64_t nsteps, double dt, int64_t checkpoint_every,
+                         int64_t ngauges, const int64_t *gauge_triangles, int64_t obs_every,
+                         double *gauge_stage)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    if (d->mixed) return fail(d, ANUGA_E_ARG, "gradients need the FP64 path");
+    if (d->P.timestep_fluxcalls != 2) return fail(d, ANUGA_E_ARG, "gradients need rk2 (DE1)");
+    if (nsteps <= 0 || checkpoint_every <= 0 || obs_every <= 0 || !(dt > 0.0) || ngauges < 0 ||
+        (ngauges > 0 && (!gauge_triangles || !gauge_stage)))
+        return fail(d, ANUGA_E_ARG, "bad run description");
+    if (d->substate != 0) return fail(d, ANUGA_E_STATE, "inside an rk2 step");
+    rc = ready_to_step(d);
+    if (rc) return rc;
+    const int N = d->N, G = (int)ngauges;
+    std::vector<int> iperm(N), g(std::max(G, 1));
+    for (int p = 0; p < N; p++) iperm[d->perm[p]] = p;
+    std::vector<char> seen(N, 0);
+    for (int i = 0; i < G; i++) {
+        const int64_t t = gauge_triangles[i];
+        if (t < 0 || t >= N) return fail(d, ANUGA_E_ARG, "gauge triangle out of range");
+        if (seen[t]) return fail(d, ANUGA_E_ARG, "gauge triangles must be distinct");
+        seen[t] = 1;
+        g[i] = iperm[t];
+    }
+    delete d->adj;
+    d->adj = nullptr;
+    AdjRun *a = new AdjRun();
+    a->nsteps = nsteps;
+    a->every = std::min(checkpoint_every, nsteps);
+    a->obs_every = obs_every;
+    a->nckpt = (nsteps + a->every - 1) / a->every;
+    a->nobs = nsteps / obs_every;
+    a->G = G;
+    a->dt = dt;
+    const size_t n3 = 3 * (size_t)N;
+#define ACHK(call)                                                                  \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            delete a;                                                               \
+            return fail(d, ANUGA_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+        }                                                                           \
+    } while (0)
+    ACHK(dalloc(&a->ckpt, a->nckpt * (size_t)N));
+    ACHK(dalloc(&a->seg, a->every * (size_t)N));
+    ACHK(dalloc(&a->S1, N));
+    ACHK(dalloc(&a->E0, 4 * n3));
+    ACHK(dalloc(&a->E1, 4 * n3));
+    ACHK(dalloc(&a->ex0, n3));
+    ACHK(dalloc(&a->ex1, n3));
+    ACHK(dalloc(&a->f0, N));
+    ACHK(dalloc(&a->f1, N));
+    ACHK(dalloc(&a->lam, n3));
+    ACHK(dalloc(&a->lamn, n3));
+    ACHK(dalloc(&a->lam1, n3));
+    ACHK(dalloc(&a->lamE, 4 * n3));
+    ACHK(dalloc(&a->lam_ex, n3));
+    ACHK(dalloc(&a->zbar, N));
+    ACHK(dalloc(&a->nbar, N));
+    ACHK(dalloc(&a->rec, std::max<int64_t>(a->nobs * G, 1)));
+    ACHK(dalloc(&a->gauges, std::max(G, 1)));
+    ACHK(dalloc(&a->neg, 1));
+    ACHK(hipMemcpy(a->gauges, g.data(), std::max(G, 1) * sizeof(int), hipMemcpyHostToDevice));
+#undef ACHK
+    d->adj = a;
+    ensure_live(d);
+    hipLaunchKernelGGL(de_set_dt, dim3(1), dim3(64), 0, d->stream, d->ctl, dt);
+    HIPCHK(d, hipMemcpyAsync(a->ckpt, d->A, N * sizeof(DeQ), hipMemcpyDeviceToDevice, d->stream));
+    for (int64_t s = 1; s <= nsteps; s++) {
+        step_fixed(d, d->A);
+        if (s % a->every == 0 && s < nsteps)
+            HIPCHK(d, hipMemcpyAsync(a->ckpt + (s / a->every) * (size_t)N, d->A, N * sizeof(DeQ),
+                                     hipMemcpyDeviceToDevice, d->stream));
+        if (G > 0 && s % obs_every == 0)
+            hipLaunchKernelGGL(adj_k_gauges, dim3((G + 63) / 64), dim3(64), 0, d->stream, G,
+                               a->gauges, d->A, a->rec + (s / obs_every - 1) * (size_t)G);
+    }
+    rc = launch_check(d);
+    if (rc) return rc;
+    rc = sync_scalars(d);
+    if (rc) return rc;
+    if (G > 0)
+        HIPCHK(d, hipMemcpy(gauge_stage, a->rec, a->nobs * G * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    return ANUGA_OK;
+}
+
+int anuga_adjoint_gradient(anuga_domain *d, const double *dloss_dstage, double *grad_friction,
+                           double *grad_elevation, double *grad_state0)
+{
+    if (!d) return ANUGA_E_ARG;
+    AdjRun *a = d->adj;
+    if (!a) return fail(d, ANUGA_E_STATE, "no recorded run: call anuga_adjoint_record first");
+    if (a->G > 0 && !dloss_dstage) return fail(d, ANUGA_E_ARG, "dloss_dstage is NULL");
+    const int N = d->N;
+    const size_t n3 = 3 * (size_t)N;
+    if (a->G > 0)
+        HIPCHK(d, hipMemcpy(a->rec, dloss_dstage, a->nobs * a->G * sizeof(double),
+                            hipMemcpyHostToDevice));
+    HIPCHK(d, hipMemsetAsync(a->lam, 0, n3 * sizeof(double), d->stream));
+    HIPCHK(d, hipMemsetAsync(a->zbar, 0, N * sizeof(double), d->stream));
+    HIPCHK(d, hipMemsetAsync(a->nbar, 0, N * sizeof(double), d->stream));
+    ensure_live(d);
+    hipLaunchKernelGGL(de_set_dt, dim3(1), dim3(64), 0, d->stream, d->ctl, a->dt);
+    for (int64_t j = a->nckpt - 1; j >= 0; j--) {
+        const int64_t s0 = j * a->every, s1 = std::min(s0 + a->every, a->nsteps);
+        HIPCHK(d, hipMemcpyAsync(a->seg, a->ckpt + j * (size_t)N, N * sizeof(DeQ),
+                                 hipMemcpyDeviceToDevice, d->stream));
+        for (int64_t t = 1; t < s1 - s0; t++) {
+            DeQ *x = a->seg + t * (size_t)N;
+            HIPCHK(d, hipMemcpyAsync(x, x - N, N * sizeof(DeQ), hipMemcpyDeviceToDevice, d->stream));
+            step_fixed(d, x);
+        }
+        for (int64_t s = s1; s > s0; s--) {
+            if (a->G > 0 && s % a->obs_every == 0)
+                hipLaunchKernelGGL(adj_k_gauge_seed, dim3((a->G + 63) / 64), dim3(64), 0, d->stream,
+                                   a->G, a->gauges, a->rec + (s / a->obs_every - 1) * (size_t)a->G,
+                                   a->lam);
+            int rc = adj_step(d, a, a->seg + (s - 1 - s0) * (size_t)N);
+            if (rc) return rc;
+        }
+    }
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    std::vector<double> h(n3);
+    auto unpermute = [&](const double *src, double *dst) {
+        for (int p = 0; p < N; p++) dst[d->perm[p]] = src[p];
+    };
+    if (grad_friction) {
+        HIPCHK(d, hipMemcpy(h.data(), a->nbar, N * sizeof(double), hipMemcpyDeviceToHost));
+        unpermute(h.data(), grad_friction);
+    }
+    if (grad_elevation) {
+        HIPCHK(d, hipMemcpy(h.data(), a->zbar, N * sizeof(double), hipMemcpyDeviceToHost));
+        unpermute(h.data(), grad_elevation);
+    }
+    if (grad_state0) {
+        HIPCHK(d, hipMemcpy(h.data(), a->lam, n3 * sizeof(double), hipMemcpyDeviceToHost));
+        for (int q = 0; q < 3; q++) unpermute(h.data() + (size_t)q * N, grad_state0 + (size_t)q * N);
+    }
     return ANUGA_OK;
 }
 

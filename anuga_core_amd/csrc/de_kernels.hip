@@ -86,12 +86,8 @@ de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict
         for (int i = 0; i < 3; i++) sur[i] = v[i] >= 0 ? (v[i] >> 2) : k;
         // surrogate neighbours: one record each
         DeQ qs[3];
-        int4 vs[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            qs[i] = q[sur[i]];
-            vs[i] = m.nb[sur[i]];
-        }
+        for (int i = 0; i < 3; i++) qs[i] = q[sur[i]];
         double hs[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) hs[i] = de_height(qs[i].w, qs[i].z);
@@ -115,8 +111,10 @@ de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict
         if (clamp) mass = (qk.z - qk.w) * m.area[k];
         flags[k] = (clamp ? DE_F_CLAMP : 0u) | (keep ? DE_F_KEEP : 0u);
 
-        // the same for the surrogate neighbours; the dry-neighbour test of a
-        // wet neighbour needs the heights of its own neighbours
+        // the same for the surrogate neighbours. A wet neighbour n keeps its
+        // velocity unless all of its own neighbours are dry; k is one of
+        // them, so the heights of n's other neighbours are needed only when
+        // k itself is dry (a boundary edge's surrogate is k: its test is dry_k)
         DeCell nb[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -126,13 +124,16 @@ de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict
             nb[i].u = 0.0;
             nb[i].v = 0.0;
             if (hs[i] > mah) {
-                const int vv[3] = {vs[i].x, vs[i].y, vs[i].z};
-                bool dry = true;
+                bool dry = v[i] < 0 ? dry_k : (c.h < mah);
+                if (v[i] >= 0 && dry) {
+                    const int4 vs = m.nb[sur[i]];
+                    const int vv[3] = {vs.x, vs.y, vs.z};
 #pragma unroll
-                for (int r = 0; r < 3; r++) {
-                    if (vv[r] >= 0) {
-                        const DeQ t = q[vv[r] >> 2];
-                        dry = dry && (de_height(t.w, t.z) < mah);
+                    for (int r = 0; r < 3; r++) {
+                        if (vv[r] >= 0) {
+                            const DeQ t = q[vv[r] >> 2];
+                            dry = dry && (de_height(t.w, t.z) < mah);
+                        }
                     }
                 }
                 if (!dry) {

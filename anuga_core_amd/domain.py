@@ -632,6 +632,34 @@ class Domain:
     def synchronize(self):
         self._call("anuga_synchronize")
 
+    # -- gradients ---------------------------------------------------------------------
+    def adjoint_record(self, nsteps, dt, gauges, obs_every=1, checkpoint_every=100):
+        """Run `nsteps` DE1 steps of fixed timestep `dt` from the current
+        centroid values (set_fixed_flux_timestep semantics with CFL 1; no
+        initial distribute), keeping a checkpoint every `checkpoint_every`
+        steps on the device, and return the stage of the `gauges` triangles
+        after every `obs_every`-th step, shape (nsteps // obs_every, len(gauges)).
+        The domain's state advances to the end of the run."""
+        g = np.ascontiguousarray(gauges, np.int64)
+        nobs = int(nsteps) // int(obs_every)
+        out = np.zeros((max(nobs, 1), len(g)))
+        self._call("anuga_adjoint_record", int(nsteps), C.c_double(dt), int(checkpoint_every),
+                   len(g), g.ctypes.data_as(C.POINTER(C.c_int64)), int(obs_every), _dptr(out))
+        self.relative_time += int(nsteps) * float(dt)
+        self._adjoint_shape = (nobs, len(g))
+        return out[:nobs]
+
+    def adjoint_gradient(self, dloss_dstage):
+        """Discrete adjoint of the last adjoint_record run: given dL/d(stage at
+        the gauges) in the record's shape, return (dL/dfriction, dL/delevation,
+        dL/d(initial stage, xmomentum, ymomentum)) per triangle; the bed
+        derivative is at fixed stage."""
+        N = self.number_of_elements
+        dl = np.ascontiguousarray(dloss_dstage, np.float64).reshape(self._adjoint_shape)
+        gn, gz, gs = np.zeros(N), np.zeros(N), np.zeros((3, N))
+        self._call("anuga_adjoint_gradient", _dptr(dl), _dptr(gn), _dptr(gz), _dptr(gs))
+        return gn, gz, gs
+
     def kernel_timing(self, name):
         ms = C.c_double()
         n = C.c_int64()

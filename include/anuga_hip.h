@@ -245,6 +245,29 @@ int anuga_ensemble_get_water_volumes(anuga_domain *d, double *volumes);
 int anuga_ensemble_get_boundary_flux_integrals(anuga_domain *d, double *integrals);
 int anuga_ensemble_size(const anuga_domain *d, int64_t *nscenarios);
 
+/* ---- Gradients: discrete adjoint of a fixed-timestep DE1 run --------------
+ * The reference has no derivative of the step. These give the exact gradient
+ * of the discrete FP64 DE1 scheme (branches frozen where the forward took them)
+ * for calibration and data assimilation: friction and bed per triangle, and
+ * the initial state, for a loss on the stage at gauge triangles. Single
+ * scenario, FP64, timestep_fluxcalls == 2. */
+
+/* Run nsteps DE1 steps with the fixed timestep dt (set_fixed_flux_timestep
+ * semantics, generic_domain.py:2332-2347, with CFL 1) from the current
+ * centroid state, keeping the state every checkpoint_every steps on the
+ * device; gauge_stage ((nsteps / obs_every) x ngauges) receives the stage of
+ * gauge_triangles (caller's numbering, distinct) after every obs_every-th
+ * step. The handle's state advances to the end of the run. */
+int anuga_adjoint_record(anuga_domain *d, int64_t nsteps, double dt, int64_t checkpoint_every,
+                         int64_t ngauges, const int64_t *gauge_triangles, int64_t obs_every,
+                         double *gauge_stage);
+/* Gradient of a loss L given dL/d(gauge stage) (the record's shape) with respect
+ * to the friction (N), the bed at fixed stage (N) and the initial stage,
+ * xmomentum, ymomentum ((3, N); may be NULL), for the last recorded run. The
+ * run's steps are recomputed segment by segment from the checkpoints. */
+int anuga_adjoint_gradient(anuga_domain *d, const double *dloss_dstage, double *grad_friction,
+                           double *grad_elevation, double *grad_state0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,186 @@
+"""The generic cell math behind the discrete adjoint (anuga_core_amd/csrc/
+de_adjoint.h), driven on the CPU by tests/adjoint_harness.cpp (CPU only):
+
+* its double instantiation is the DE1 step: a fixed-timestep run equals the
+  oracle's (set_fixed_flux_timestep semantics) bit for bit, wet/dry included;
+* its dual-number adjoint is the derivative of that run: gradients of a gauge
+  loss with respect to the friction, the bed and the initial state agree with
+  finite differences along random directions.
+
+The scheme is piecewise smooth: the gradient limiter (sw_domain_orig.c:1306-1333)
+and the wet/dry and upwind choices take min / max of neighbour values, so the
+run has kinks, and a central difference whose step straddles one mixes the two
+one-sided slopes. The adjoint is the derivative of the piece the forward is on.
+Kinks are dense in a run of many steps, so a direction passes when the
+central difference at one of the steps 1e-6 .. 1e-9 agrees to 2e-5 (a smaller
+step straddles fewer kinks; below 1e-9 rounding dominates).
+"""
+
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from anuga_core_amd.mesh import Mesh, rectangular_cross
+from oracle.kernels import OracleKernels
+from oracle.oracle_domain import OracleDomain
+from tests import scenarios as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DE1 = np.array([9.8, 1.0e-12, 1.0e-5, 1.0, 0.0, 1.0, 0.0, 1.0, 0.0])
+
+
+@pytest.fixture(scope="module")
+def harness():
+    tmp = tempfile.mkdtemp()
+    so = os.path.join(tmp, "libharness.so")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-shared", "-fPIC",
+                    "-I" + os.path.join(ROOT, "anuga_core_amd", "csrc"), "-o", so,
+                    os.path.join(ROOT, "tests", "adjoint_harness.cpp")], check=True)
+    return C.CDLL(so)
+
+
+class Layout:
+    """The library's per-triangle layouts, in the mesh's own order."""
+
+    def __init__(self, mesh, q, bc):
+        N = self.N = mesh.number_of_triangles
+        self.M = mesh.boundary_length
+        nbr, ne = mesh.neighbours, mesh.neighbour_edges
+        nb = np.zeros((N, 4), np.int32)
+        for i in range(3):
+            nb[:, i] = np.where(nbr[:, i] >= 0, (nbr[:, i] << 2) | ne[:, i], nbr[:, i])
+        nb[:, 3] = mesh.number_of_boundaries
+        self.nb = nb
+        cen = np.ascontiguousarray(mesh.centroid_coordinates)
+        em = mesh.edge_midpoint_coordinates.reshape(N, 3, 2)
+        self.cen = cen
+        self.gx = np.ascontiguousarray(np.concatenate([em[:, :, 0] - cen[:, 0:1],
+                                                       em[:, :, 1] - cen[:, 1:2]], axis=1))
+        nrm = mesh.normals.reshape(N, 3, 2)
+        self.nx = np.ascontiguousarray(nrm[:, :, 0].T)
+        self.ny = np.ascontiguousarray(nrm[:, :, 1].T)
+        self.len = np.ascontiguousarray(mesh.edgelengths.reshape(N, 3).T)
+        self.area = np.ascontiguousarray(mesh.areas)
+        self.eta = np.ascontiguousarray(q["friction"], np.float64).copy()
+        kinds = {"reflective": 0, "transmissive": 1}
+        self.bkind = np.zeros(max(self.M, 1), np.int8)
+        self.bval = np.zeros(3 * max(self.M, 1))
+        for tag, ids in mesh.tag_boundary_cells.items():
+            self.bkind[np.asarray(ids)] = kinds[bc[tag]]
+        self.S = np.zeros((N, 4))
+        self.S[:, 0] = q["stage"]
+        self.S[:, 3] = q["elevation"]
+
+    def args(self):
+        p = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+        d = C.c_double
+        return [self.N, self.M, p(self.nb, C.c_int), p(self.cen, d), p(self.gx, d), p(self.nx, d),
+                p(self.ny, d), p(self.len, d), p(self.area, d), p(self.eta, d),
+                p(self.bkind, C.c_byte), p(self.bval, d), p(DE1, d)]
+
+
+def _forward(lib, L, S0, dt, nsteps, gauges, obs_every):
+    S = np.ascontiguousarray(S0, np.float64).copy()
+    g = np.ascontiguousarray(gauges, np.int32)
+    rec = np.zeros((max(nsteps // obs_every, 1), len(g)))
+    lib.harness_forward(*L.args(), C.c_double(dt), nsteps,
+                        S.ctypes.data_as(C.POINTER(C.c_double)), len(g),
+                        g.ctypes.data_as(C.POINTER(C.c_int)), obs_every,
+                        rec.ctypes.data_as(C.POINTER(C.c_double)))
+    return S, rec
+
+
+def test_forward_equals_oracle_fixed_timestep(harness):
+    mesh, _, q, bc = S.dam_break_de0(m=10, n=10)
+    tau = 0.05
+    o = S.setup(OracleDomain(mesh, OracleKernels(gather=True)), "DE1", q, bc)
+    o.fixed_flux_timestep = tau
+    gen = o.evolve(yieldstep=1e300, finaltime=1e300, max_steps=25)
+    next(gen)
+    S0 = np.stack([o.stage_c, o.xmom_c, o.ymom_c, o.elev_c], axis=1).copy()
+    for _ in gen:
+        pass
+    assert o.dt_history == [tau] * 25
+    L = Layout(mesh, q, bc)
+    Sh, _ = _forward(harness, L, S0, tau, 25, [], 1)
+    wet = Sh[:, 0] - Sh[:, 3] > 1e-5
+    assert 0.1 < wet.mean() < 0.95
+    assert np.array_equal(Sh[:, 0], o.stage_c)
+    assert np.array_equal(Sh[:, 1], o.xmom_c)
+    assert np.array_equal(Sh[:, 2], o.ymom_c)
+
+
+def _wet_case(m=6, seed=0):
+    """Fully wet (depth >= ~2 m) over a smooth bump, moving water, friction
+    varying per triangle, transmissive west. Small random noise on bed and
+    stage breaks the mesh's mirror symmetries: with exactly equal neighbour
+    values the limiter's max / min sits on a kink, where one-sided derivatives
+    (the adjoint's) and central differences legitimately differ."""
+    p, v, b = rectangular_cross(m, m, 60.0, 60.0)
+    mesh = Mesh(p, v, b)
+    x, y = mesh.centroid_coordinates.T
+    rng = np.random.default_rng(seed)
+    z = -4.0 + 1.5 * np.exp(-((x - 30) ** 2 + (y - 25) ** 2) / 200.0) + 1e-3 * rng.random(len(x))
+    w = 0.4 * np.exp(-((x - 20) ** 2 + (y - 35) ** 2) / 150.0) + 1e-3 * rng.random(len(x))
+    n = 0.03 + 0.01 * rng.random(len(x))
+    q = dict(elevation=z, stage=w, friction=n)
+    bc = {"left": "transmissive", "right": "reflective", "top": "reflective",
+          "bottom": "reflective"}
+    return mesh, q, bc
+
+
+@pytest.mark.parametrize("m,nsteps,obs_every", [(6, 12, 3), (9, 20, 5)])
+def test_adjoint_matches_finite_differences(harness, m, nsteps, obs_every):
+    mesh, q, bc = _wet_case(m)
+    L = Layout(mesh, q, bc)
+    N = L.N
+    rng = np.random.default_rng(1)
+    gauges = rng.choice(N, 6, replace=False).astype(np.int32)
+    dt = 0.2
+    S0 = L.S.copy()
+    x, y = mesh.centroid_coordinates.T       # a smooth initial momentum field
+    S0[:, 1] = 0.3 * np.sin(x / 15.0) * np.cos(y / 20.0) + 1e-4 * rng.random(N)
+    S0[:, 2] = 0.2 * np.cos(x / 25.0) * np.sin(y / 10.0) + 1e-4 * rng.random(N)
+    _, rec = _forward(harness, L, S0, dt, nsteps, gauges, obs_every)
+    obs = rec + 0.01 * rng.standard_normal(rec.shape)
+
+    def loss(eta=None, dz=0.0, dS=0.0):
+        Lp = Layout(mesh, q, bc)
+        if eta is not None:
+            Lp.eta[:] = eta
+        Sp = S0.copy()
+        Sp[:, 3] += dz
+        Sp[:, :3] += dS
+        _, r = _forward(harness, Lp, Sp, dt, nsteps, gauges, obs_every)
+        return float(np.sum((r - obs) ** 2))
+
+    dL = np.ascontiguousarray(2.0 * (rec - obs))
+    nbar, zbar, lam = np.zeros(N), np.zeros(N), np.zeros(3 * N)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    harness.harness_adjoint(*L.args(), C.c_double(dt), nsteps, dp(np.ascontiguousarray(S0)),
+                            len(gauges), gauges.ctypes.data_as(C.POINTER(C.c_int)), obs_every,
+                            dp(dL), dp(nbar), dp(zbar), dp(lam))
+    assert np.any(nbar != 0) and np.any(zbar != 0)
+    for trial in range(4):
+        d = rng.standard_normal(N)
+        _agrees(np.dot(nbar, d), lambda e: loss(eta=L.eta + e * d))
+        d = rng.standard_normal(N)
+        _agrees(np.dot(zbar, d), lambda e: loss(dz=e * d))
+        d = rng.standard_normal((N, 3))
+        ad = np.dot(lam[:N], d[:, 0]) + np.dot(lam[N:2 * N], d[:, 1]) + np.dot(lam[2 * N:], d[:, 2])
+        _agrees(ad, lambda e: loss(dS=e * d))
+
+
+def _agrees(ad, f):
+    """The central difference at one of the steps 1e-6 .. 1e-9 agrees with ad."""
+    diffs = []
+    for eps in (1e-6, 1e-7, 1e-8, 1e-9):
+        fc = (f(eps) - f(-eps)) / (2 * eps)
+        if abs(ad - fc) <= 2e-5 * abs(fc) + 1e-11:
+            return eps
+        diffs.append(fc)
+    raise AssertionError("adjoint %r, central differences %r" % (ad, diffs))
