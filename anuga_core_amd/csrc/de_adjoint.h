@@ -17,14 +17,14 @@
 
 #include <math.h>
 
-#include "de_pow.h"
-
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define ADJ_HD __host__ __device__ __forceinline__
 #else
 #define ADJ_HD inline
 #endif
+
+#include "de_pow.h"
 
 // ---------------------------------------------------------------------------
 // forward-mode dual numbers
@@ -120,6 +120,37 @@ template <int K> ADJ_HD Dual<K> adj_min(const Dual<K> &a, const Dual<K> &b)
     return (a.v <= b.v) ? a : b;
 }
 
+// a ? x : (b ? y : w). For duals the choice is an arithmetic blend with 0/1
+// weights: a select between elements of local arrays compiles to a load from a
+// selected address, which keeps the arrays in scratch memory. The blend's
+// value equals the chosen one for finite inputs (up to the sign of a zero);
+// the double versions, which must reproduce the forward, select exactly.
+ADJ_HD double adj_sel(bool a, double x, double y) { return a ? x : y; }
+template <int K>
+ADJ_HD Dual<K> adj_sel(bool a, const Dual<K> &x, const Dual<K> &y)
+{
+    const double ma = a ? 1.0 : 0.0, mb = 1.0 - ma;
+    Dual<K> r;
+    r.v = ma * x.v + mb * y.v;
+#pragma unroll
+    for (int k = 0; k < K; k++) r.d[k] = ma * x.d[k] + mb * y.d[k];
+    return r;
+}
+ADJ_HD double adj_sel3(bool a, bool b, double x, double y, double w)
+{
+    return a ? x : (b ? y : w);
+}
+template <int K>
+ADJ_HD Dual<K> adj_sel3(bool a, bool b, const Dual<K> &x, const Dual<K> &y, const Dual<K> &w)
+{
+    const double ma = a ? 1.0 : 0.0, mb = (!a && b) ? 1.0 : 0.0, mc = 1.0 - ma - mb;
+    Dual<K> r;
+    r.v = ma * x.v + mb * y.v + mc * w.v;
+#pragma unroll
+    for (int k = 0; k < K; k++) r.d[k] = ma * x.d[k] + mb * y.d[k] + mc * w.d[k];
+    return r;
+}
+
 ADJ_HD double adj_sqrt(double a) { return sqrt(a); }
 template <int K> ADJ_HD Dual<K> adj_sqrt(const Dual<K> &a)
 {
@@ -206,14 +237,9 @@ ADJ_HD void adj_line_jumps(T qk, T q1, double dx2, double dy2, const double dxv[
     const T b = dq1 * T(dy2);
 #pragma unroll
     for (int i = 0; i < 3; i++) d[i] = a * T(dxv[i]) + b * T(dyv[i]);
-    T qmin, qmax;
-    if (dq1 >= T(0.0)) {
-        qmin = T(0.0);
-        qmax = dq1;
-    } else {
-        qmin = dq1;
-        qmax = T(0.0);
-    }
+    const bool up = dq1 >= T(0.0);
+    const T qmin = adj_sel(up, T(0.0), dq1);
+    const T qmax = adj_sel(up, dq1, T(0.0));
     adj_limit(d, qmin, qmax, beta);
 }
 
@@ -305,19 +331,25 @@ ADJ_HD void adj_extrap_cell(const T w[4], const T uh[4], const T vh[4], const T 
                                  inv_area2, gm.dxv, gm.dyv, beta, jv);
         }
     } else if (gm.nbound == 2) {
-        const int s = gm.interior[0] ? 0 : (gm.interior[1] ? 1 : 2);
+        const bool e0 = gm.interior[0], e1 = gm.interior[1];
         if (!(gm.interior[0] || gm.interior[1] || gm.interior[2])) {
             constant = true;
         } else {
+            // the first interior surrogate, selected (not indexed) so that the
+            // arrays stay in registers
+            const T sw = adj_sel3(e0, e1, nw[0], nw[1], nw[2]);
+            const T sh = adj_sel3(e0, e1, nh[0], nh[1], nh[2]);
+            const T su = adj_sel3(e0, e1, nu[0], nu[1], nu[2]);
+            const T sv = adj_sel3(e0, e1, nv[0], nv[1], nv[2]);
             const double area2 = gm.lx * gm.lx + gm.ly * gm.ly;
             double dx2 = 1.0 / area2;
             const double dy2 = dx2 * gm.ly;
             dx2 *= gm.lx;
             const T bw = T(p.bw);
-            adj_line_jumps<T>(cw, nw[s], dx2, dy2, gm.dxv, gm.dyv, bw, jw);
-            adj_line_jumps<T>(ch, nh[s], dx2, dy2, gm.dxv, gm.dyv, bw, jh);
-            adj_line_jumps<T>(cu, nu[s], dx2, dy2, gm.dxv, gm.dyv, bw, ju);
-            adj_line_jumps<T>(cv, nv[s], dx2, dy2, gm.dxv, gm.dyv, bw, jv);
+            adj_line_jumps<T>(cw, sw, dx2, dy2, gm.dxv, gm.dyv, bw, jw);
+            adj_line_jumps<T>(ch, sh, dx2, dy2, gm.dxv, gm.dyv, bw, jh);
+            adj_line_jumps<T>(cu, su, dx2, dy2, gm.dxv, gm.dyv, bw, ju);
+            adj_line_jumps<T>(cv, sv, dx2, dy2, gm.dxv, gm.dyv, bw, jv);
         }
     }
 #pragma unroll
@@ -805,11 +837,14 @@ ADJ_HD void adj_bwd_flux(const AdjView &m, const double *S, const unsigned char 
 }
 
 // Extrapolation of triangle k: lamE (its edge values) -> adjoints of the raw
-// state of k and its surrogate neighbours (and of their beds).
+// state of k and its surrogate neighbours (and of their beds). K tangents per
+// pass (the four components of one stencil triangle in 4 / K passes): a
+// narrower dual keeps the long extrapolation in registers.
+template <int K = 4>
 ADJ_HD void adj_bwd_extrap(const AdjView &m, const double *S, const unsigned char *fl,
                            const double *lamE, int k, double *lamS, double *zbar)
 {
-    typedef Dual<4> D;
+    typedef Dual<K> D;
     const int N = m.N;
     int v[3], sur[3];
     adj_neighbours(m, k, v, sur);
@@ -825,36 +860,46 @@ ADJ_HD void adj_bwd_extrap(const AdjView &m, const double *S, const unsigned cha
     if (!any) return;
     unsigned f[4];
     for (int j = 0; j < 4; j++) f[j] = fl[id[j]];
-    // one pass per distinct triangle of the stencil: k, then each interior neighbour
-    for (int pass = 0; pass < 4; pass++) {
-        const int t = id[pass];
-        if (pass > 0 && v[pass - 1] < 0) continue;   // a boundary surrogate is k itself
+    // passes over the distinct triangles of the stencil (k, then each interior
+    // neighbour) and over groups of K of their four components
+#pragma unroll 1
+    for (int pass = 0; pass < 4 * (4 / K); pass++) {
+        const int slot = pass / (4 / K), c0 = (pass % (4 / K)) * K;
+        const int t = slot == 0 ? k : (slot == 1 ? sur[0] : (slot == 2 ? sur[1] : sur[2]));
+        const int vs = slot == 1 ? v[0] : (slot == 2 ? v[1] : v[2]);
+        if (slot > 0 && vs < 0) continue;   // a boundary surrogate is k itself
+        // seeds by comparison, so that every array index is a constant after
+        // unrolling and the duals stay in registers
         D w[4], uh[4], vh[4], z[4];
+#pragma unroll
         for (int j = 0; j < 4; j++) {
             w[j] = D(S[4 * id[j]]);
             uh[j] = D(S[4 * id[j] + 1]);
             vh[j] = D(S[4 * id[j] + 2]);
             z[j] = D(S[4 * id[j] + 3]);
-            if (id[j] == t) {
-                w[j].d[0] = 1.0;
-                uh[j].d[1] = 1.0;
-                vh[j].d[2] = 1.0;
-                z[j].d[3] = 1.0;
+            const bool mine = id[j] == t;
+#pragma unroll
+            for (int c = 0; c < K; c++) {
+                const int cc = c0 + c;
+                w[j].d[c] = (mine && cc == 0) ? 1.0 : 0.0;
+                uh[j].d[c] = (mine && cc == 1) ? 1.0 : 0.0;
+                vh[j].d[c] = (mine && cc == 2) ? 1.0 : 0.0;
+                z[j].d[c] = (mine && cc == 3) ? 1.0 : 0.0;
             }
         }
         D ew[3], eh[3], euh[3], evh[3];
         adj_extrap_cell<D>(w, uh, vh, z, f, g, m.p, ew, eh, euh, evh);
-        double gt[4];
-        for (int c = 0; c < 4; c++) {
+#pragma unroll
+        for (int c = 0; c < K; c++) {
             double a = 0.0;
+#pragma unroll
             for (int i = 0; i < 3; i++)
                 a += lo[4 * i] * ew[i].d[c] + lo[4 * i + 1] * eh[i].d[c] +
                      lo[4 * i + 2] * euh[i].d[c] + lo[4 * i + 3] * evh[i].d[c];
-            gt[c] = a;
+            const int cc = c0 + c;
+            if (a == 0.0) continue;
+            if (cc < 3) adj_add(lamS + (size_t)cc * N + t, a);
+            else adj_add(zbar + t, a);
         }
-        if (gt[0] != 0.0) adj_add(lamS + t, gt[0]);
-        if (gt[1] != 0.0) adj_add(lamS + N + t, gt[1]);
-        if (gt[2] != 0.0) adj_add(lamS + 2 * (size_t)N + t, gt[2]);
-        if (gt[3] != 0.0) adj_add(zbar + t, gt[3]);
     }
 }

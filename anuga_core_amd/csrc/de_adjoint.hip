@@ -10,6 +10,11 @@
 
 __device__ __forceinline__ int adj_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
+// tangents per pass of the extrapolation's adjoint (register pressure)
+#ifndef ADJ_EXTRAP_K
+#define ADJ_EXTRAP_K 4
+#endif
+
 // adjoint of the update (friction, update, fix_negative) of one stage;
 // lam_out is scaled by `scale` (0.5: the final saxpy of the step)
 __global__ void __launch_bounds__(DE_BLOCK)
@@ -41,7 +46,7 @@ adj_k_extrap(AdjView m, const double *__restrict__ S, const unsigned char *__res
 {
     const int k = adj_tid();
     if (k >= m.N) return;
-    adj_bwd_extrap(m, S, fl, lamE, k, lamS, zbar);
+    adj_bwd_extrap<ADJ_EXTRAP_K>(m, S, fl, lamE, k, lamS, zbar);
 }
 
 // out = a * in over n entries
