@@ -115,11 +115,30 @@ class DeviceError(RuntimeError):
     """A failure reported by the device library."""
 
 
+def _one_hip_runtime():
+    """PyTorch's ROCm wheel bundles its own libamdhip64 / libhsa-runtime64 with
+    the same sonames as /opt/rocm's but loads them by file name. If this
+    library were loaded first, a later `import torch` would map a second HIP
+    and HSA runtime into the process (two clients of the GPU driver with their
+    own queues and address apertures). Importing torch first, when it is
+    installed, makes this library bind to torch's runtime: one per process."""
+    import importlib.util
+    import sys
+    if "torch" in sys.modules:
+        return
+    try:
+        if importlib.util.find_spec("torch") is not None:
+            import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def lib():
     """Load libanuga_hip.so (once). Raises if it is missing or does not load."""
     global _lib
     with _lock:
         if _lib is None:
+            _one_hip_runtime()
             if not os.path.exists(LIB_PATH):
                 raise ImportError(
                     "%s is not built: run `python -m anuga_core_amd.build` (hipcc, gfx950)"

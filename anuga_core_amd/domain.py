@@ -640,6 +640,7 @@ class Domain:
         steps on the device, and return the stage of the `gauges` triangles
         after every `obs_every`-th step, shape (nsteps // obs_every, len(gauges)).
         The domain's state advances to the end of the run."""
+        self._sync_params()
         g = np.ascontiguousarray(gauges, np.int64)
         nobs = int(nsteps) // int(obs_every)
         out = np.zeros((max(nobs, 1), len(g)))

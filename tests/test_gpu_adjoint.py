@@ -14,6 +14,7 @@ import ctypes as C
 
 import numpy as np
 import pytest
+import torch  # noqa: F401  (before the device library: one HIP runtime per process)
 
 from anuga_core_amd.adjoint import GaugeRun
 from anuga_core_amd.domain import Domain
