@@ -79,6 +79,7 @@ SIGNATURES = {
     "anuga_compute_fluxes": (C.c_int, [C.c_void_p, _dp]),
     "anuga_update_conserved_quantities": (C.c_int, [C.c_void_p, C.c_double, _ip]),
     "anuga_rk2_second_substep": (C.c_int, [C.c_void_p, C.c_double, _ip]),
+    "anuga_rk3_later_substeps": (C.c_int, [C.c_void_p, C.c_double, _ip]),
     "anuga_evolve_steps": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(TimeControl),
                                      C.POINTER(StepState), _dp]),
     "anuga_get_boundary_flux_sums": (C.c_int, [C.c_void_p, _dp]),

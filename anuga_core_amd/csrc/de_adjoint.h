@@ -549,6 +549,7 @@ ADJ_HD void adj_advance_cell(T cw, T cuh, T cvh, T cz, T eta, T exw, T exuh, T e
 // edge values E = w, h, uh, vh blocks of [3][N]; adjoints of the state [3][N].
 struct AdjView {
     int N, M;
+    int gx_stride;        // doubles per triangle in gx (the first 6: x0..x2, y0..y2)
     const int *nb;
     const double *cen, *gx, *nx, *ny, *len, *area, *eta;
     const signed char *bkind;
@@ -580,8 +581,8 @@ ADJ_HD AdjExtrapGeo adj_geometry(const AdjView &m, int k, const int v[3], const 
 {
     AdjExtrapGeo g;
     for (int i = 0; i < 3; i++) {
-        g.dxv[i] = m.gx[6 * k + i];
-        g.dyv[i] = m.gx[6 * k + 3 + i];
+        g.dxv[i] = m.gx[(size_t)m.gx_stride * k + i];
+        g.dyv[i] = m.gx[(size_t)m.gx_stride * k + 3 + i];
         g.interior[i] = v[i] >= 0;
     }
     g.nbound = m.nb[4 * k + 3];

@@ -32,8 +32,14 @@ struct alignas(32) DeQ {   // conserved state + bed: one record per triangle
 };
 
 // FP64 geometry (bit-comparable path)
-struct alignas(16) DeGeoX64 {  // extrapolation: edge midpoint - centroid
-    double dxv[3], dyv[3];
+struct alignas(16) DeGeoX64 {  // extrapolation geometry, formed on the host as the kernel would
+    double dxv[3], dyv[3];      // edge midpoint - centroid
+    // the gradient's auxiliary geometry: with at most one boundary edge
+    // dx1, dx2, dy1, dy2 of the surrogate centroids (c1 - c0, c2 - c0), area2
+    // and 1 / area2 (sw_domain_orig.c:2130-2160); with two boundary edges the
+    // line coefficients dx2, dy2 toward the interior neighbour in aux[1], aux[3]
+    // (:2421-2450)
+    double aux[6];
 };
 struct alignas(16) DeGeoF64 {  // flux: outward normals, edge lengths, radius
     double nx[3], ny[3], len[3], radius, pad[2];
@@ -99,7 +105,7 @@ struct DeBoundary {
 // (or a captured graph of one step) needs no host round trip.
 struct DeControl {
     double flux_dt;        // CFL timestep of the last first substep
-    double bflux0, bflux1; // boundary inflow of the two substeps (boundary_flux_sum)
+    double bflux0, bflux1, bflux2; // boundary inflow of the substeps (boundary_flux_sum)
     double mass;           // protect's added mass (last reduction)
     double volume;         // last water volume reduction
     double dt;             // timestep of the current step
@@ -112,7 +118,7 @@ struct DeControl {
     int smallsteps, order, default_order, max_smallsteps;
     int done;              // 1: finaltime reached, 2: yield time reached (steps become no-ops)
     int status;            // error bits, DE_ERR_*
-    int hist_cap, rk2;
+    int hist_cap, method;  // method: timestep_fluxcalls (1 euler, 2 rk2, 3 rk3)
     // the scenario's triangles are [kbeg, kend) and its blocks [bbeg, bend) of
     // the internal numbering (fixed at creation)
     int kbeg, kend, bbeg, bend;

@@ -240,8 +240,8 @@ static int check_params(anuga_domain *d, const anuga_params *p)
         return fail(d, ANUGA_E_ARG, "low_froude != 0 is not supported on the device path");
     if (p->extrapolate_velocity_second_order != 1)
         return fail(d, ANUGA_E_ARG, "extrapolate_velocity_second_order must be 1 (DE algorithms)");
-    if (p->timestep_fluxcalls != 1 && p->timestep_fluxcalls != 2)
-        return fail(d, ANUGA_E_ARG, "timestep_fluxcalls must be 1 (euler) or 2 (rk2)");
+    if (p->timestep_fluxcalls < 1 || p->timestep_fluxcalls > 3)
+        return fail(d, ANUGA_E_ARG, "timestep_fluxcalls must be 1 (euler), 2 (rk2) or 3 (rk3)");
     if (p->precision != ANUGA_PRECISION_FP64 && p->precision != ANUGA_PRECISION_MIXED)
         return fail(d, ANUGA_E_ARG, "unknown precision");
     return ANUGA_OK;
@@ -343,32 +343,40 @@ static void launch_materialize(anuga_domain *d, DeQ *S)
 }
 
 template <class T, int SUB>
-static void launch_flux_t(anuga_domain *d, DeQ *S, DeQ *bk)
+static void launch_flux_t(anuga_domain *d, DeQ *S, DeQ *bk, DeQ *out)
 {
     DeBoundary bc{d->bkind, d->bval, d->M};
     if constexpr (sizeof(T) == 8)
         hipLaunchKernelGGL((de_flux<double, SUB>), dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
                            dev_mesh(d), dev_params(d->P), S, d->eta,
                            (const double *)d->E, bc, d->flags, d->ex[0], d->ex[1],
-                           d->ex[2], bk, d->ctl, d->part, d->part + d->nblocks, d->dneg);
+                           d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks, d->dneg);
     else
         hipLaunchKernelGGL((de_flux_mixed<SUB>), dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
                            dev_mesh(d), dev_params(d->P), S, d->eta, (const DeEdgeRec32 *)d->E,
-                           bc, d->flags, d->ex[0], d->ex[1], d->ex[2], bk, d->ctl, d->part,
+                           bc, d->flags, d->ex[0], d->ex[1], d->ex[2], bk, out, d->ctl, d->part,
                            d->part + d->nblocks, d->dneg);
 }
 
-// substep 0: explicit updates; substep 1: advance S with the scenario's dt into the backup `bk`
-static void launch_flux(anuga_domain *d, int substep, DeQ *S, DeQ *bk)
+template <class T>
+static void launch_flux_p(anuga_domain *d, int substep, DeQ *S, DeQ *bk, DeQ *out)
+{
+    switch (substep) {
+    case 0: launch_flux_t<T, 0>(d, S, bk, out); break;
+    case 1: launch_flux_t<T, 1>(d, S, bk, out); break;
+    case 2: launch_flux_t<T, 2>(d, S, bk, out); break;
+    default: launch_flux_t<T, 3>(d, S, bk, out); break;
+    }
+}
+
+// substep 0: explicit updates of S; substep > 0: advance S with the scenario's
+// dt and combine with the step's initial state bk into out (de_rk_combine)
+static void launch_flux(anuga_domain *d, int substep, DeQ *S, DeQ *bk = nullptr,
+                        DeQ *out = nullptr)
 {
     timer_begin(d, substep ? T_FLUX1 : T_FLUX0);
-    if (d->mixed) {
-        if (substep) launch_flux_t<float, 1>(d, S, bk);
-        else launch_flux_t<float, 0>(d, S, bk);
-    } else {
-        if (substep) launch_flux_t<double, 1>(d, S, bk);
-        else launch_flux_t<double, 0>(d, S, bk);
-    }
+    if (d->mixed) launch_flux_p<float>(d, substep, S, bk, out);
+    else launch_flux_p<double>(d, substep, S, bk, out);
     timer_end(d, substep ? T_FLUX1 : T_FLUX0);
 }
 
@@ -391,7 +399,9 @@ static void launch_reduce(anuga_domain *d, int mode)
     const double *pdt = d->part, *pbf = d->part + d->nblocks;
     const double *pm = d->part + 2 * (size_t)d->nblocks;
     void (*k)(const double *, const double *, const double *, DeControl *) =
-        mode == 0 ? de_reduce<0> : mode == 1 ? de_reduce<1> : mode == 2 ? de_reduce<2> : de_reduce<3>;
+        mode == 0 ? de_reduce<0>
+                  : mode == 1 ? de_reduce<1>
+                              : mode == 2 ? de_reduce<2> : mode == 3 ? de_reduce<3> : de_reduce<4>;
     hipLaunchKernelGGL(k, dim3(d->S), dim3(1024), 0, d->stream, pdt, pbf, pm, d->ctl);
 }
 
@@ -418,19 +428,31 @@ static void ensure_live(anuga_domain *d)
 // One device-controlled step of every scenario: both substeps, the timestep
 // on the device and the end-of-step bookkeeping. Every kernel returns at once
 // for a scenario whose done flag is set.
+static void enqueue_later_substeps(anuga_domain *d)
+{
+    if (d->P.timestep_fluxcalls == 2) {
+        launch_extrapolate(d, d->B);
+        launch_flux(d, 1, d->B, d->A, d->A);
+        launch_reduce(d, 1);
+    } else if (d->P.timestep_fluxcalls == 3) {
+        launch_extrapolate(d, d->B);
+        launch_flux(d, 2, d->B, d->A, d->B);
+        launch_reduce(d, 1);
+        launch_extrapolate(d, d->B);
+        launch_flux(d, 3, d->B, d->A, d->A);
+        launch_reduce(d, 4);
+    }
+}
+
 static void enqueue_device_step(anuga_domain *d)
 {
-    const bool rk2 = d->P.timestep_fluxcalls == 2;
+    const bool rk = d->P.timestep_fluxcalls > 1;
     launch_extrapolate(d, d->A);
-    launch_flux(d, 0, d->A, nullptr);
+    launch_flux(d, 0, d->A);
     launch_reduce(d, 0);
     launch_control(d, de_timestep);
-    launch_update(d, d->A, rk2 ? d->B : d->A);
-    if (rk2) {
-        launch_extrapolate(d, d->B);
-        launch_flux(d, 1, d->B, d->A);
-        launch_reduce(d, 1);
-    }
+    launch_update(d, d->A, rk ? d->B : d->A);
+    enqueue_later_substeps(d);
     hipLaunchKernelGGL(de_finish_step, scenario_grid(d), dim3(64), 0, d->stream, d->ctl, d->S,
                        d->S == 1 ? d->hist : nullptr);
 }
@@ -718,6 +740,37 @@ static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
                 c[(size_t)i * N + p] = el[3 * k + i];
             }
             r[p] = mesh->radii[k];
+            // the gradient's auxiliary geometry, in the kernel's operation order
+            // (this file is compiled without floating-point contraction)
+            double *g = gx[p].aux;
+            for (int j = 0; j < 6; j++) g[j] = 0.0;
+            if (nob[k] <= 1) {
+                int64_t s[3];
+                for (int i = 0; i < 3; i++) s[i] = sur[3 * k + i];
+                const double dx1 = cen[2 * s[1]] - cen[2 * s[0]];
+                const double dx2 = cen[2 * s[2]] - cen[2 * s[0]];
+                const double dy1 = cen[2 * s[1] + 1] - cen[2 * s[0] + 1];
+                const double dy2 = cen[2 * s[2] + 1] - cen[2 * s[0] + 1];
+                const double area2 = dy2 * dx1 - dy1 * dx2;
+                g[0] = dx1;
+                g[1] = dx2;
+                g[2] = dy1;
+                g[3] = dy2;
+                g[4] = area2;
+                g[5] = 1.0 / area2;
+            } else if (nob[k] == 2) {
+                int64_t s1 = -1;
+                for (int i = 0; i < 3 && s1 < 0; i++)
+                    if (nbr[3 * k + i] >= 0) s1 = nbr[3 * k + i];
+                const double dx1 = cen[2 * s1] - cen[2 * k];
+                const double dy1 = cen[2 * s1 + 1] - cen[2 * k + 1];
+                const double area2 = dx1 * dx1 + dy1 * dy1;
+                double dx2 = 1.0 / area2;
+                const double dy2 = dx2 * dy1;
+                dx2 *= dx1;
+                g[1] = dx2;
+                g[3] = dy2;
+            }
         }
         HIPCHK(d, hipMemcpy((DeGeoX64 *)d->gx + kb, gx.data(), N * sizeof(DeGeoX64),
                             hipMemcpyHostToDevice));
@@ -921,7 +974,7 @@ static int evolve_steps(anuga_domain *d, int64_t max_steps, const anuga_time_con
         b.default_order = (int)tc[s].default_order;
         b.max_smallsteps = (int)tc[s].max_smallsteps;
         b.hist_cap = dt_history ? (int)std::min<int64_t>(max_steps, d->hist_cap) : 0;
-        b.rk2 = d->P.timestep_fluxcalls == 2;
+        b.method = (int)d->P.timestep_fluxcalls;
         b.stop_reached = stop_reached ? 1 : 0;
     }
     HIPCHK(d, hipMemcpyAsync(d->dsetup, d->hsetup, d->S * sizeof(DeBatchSetup),
@@ -1001,6 +1054,7 @@ static AdjView adj_view(const anuga_domain *d, double dt)
     m.nb = reinterpret_cast<const int *>(d->nb);
     m.cen = reinterpret_cast<const double *>(d->cen);
     m.gx = reinterpret_cast<const double *>(d->gx);
+    m.gx_stride = sizeof(DeGeoX64) / sizeof(double);
     m.nx = d->nx;
     m.ny = d->ny;
     m.len = d->len;
@@ -1019,10 +1073,10 @@ static AdjView adj_view(const anuga_domain *d, double dt)
 static void step_fixed(anuga_domain *d, DeQ *X)
 {
     launch_extrapolate(d, X);
-    launch_flux(d, 0, X, nullptr);
+    launch_flux(d, 0, X);
     launch_update(d, X, d->B);
     launch_extrapolate(d, d->B);
-    launch_flux(d, 1, d->B, X);
+    launch_flux(d, 1, d->B, X, X);
 }
 
 // the stage quantities of the step from X into the adjoint's buffers
@@ -1036,14 +1090,16 @@ static void adj_stages(anuga_domain *d, AdjRun *a, const DeQ *X)
     hipLaunchKernelGGL(de_extrapolate<double>, g, b, 0, d->stream, dev_mesh(d), p, X, a->E0, a->f0,
                        pm, d->ctl);
     hipLaunchKernelGGL((de_flux<double, 0>), g, b, 0, d->stream, dev_mesh(d), p, X, d->eta,
-                       a->E0, bc, a->f0, a->ex0, a->ex0 + N, a->ex0 + 2 * N, nullptr, d->ctl,
+                       a->E0, bc, a->f0, a->ex0, a->ex0 + N, a->ex0 + 2 * N, nullptr, nullptr,
+                       d->ctl,
                        d->part, d->part + d->nblocks, a->neg);
     hipLaunchKernelGGL(de_update<true>, g, b, 0, d->stream, d->scen, p, X, d->eta, a->f0, a->ex0,
                        a->ex0 + N, a->ex0 + 2 * N, d->ctl, a->S1, a->neg);
     hipLaunchKernelGGL(de_extrapolate<double>, g, b, 0, d->stream, dev_mesh(d), p, a->S1, a->E1,
                        a->f1, pm, d->ctl);
     hipLaunchKernelGGL((de_flux<double, 0>), g, b, 0, d->stream, dev_mesh(d), p, a->S1, d->eta,
-                       a->E1, bc, a->f1, a->ex1, a->ex1 + N, a->ex1 + 2 * N, nullptr, d->ctl,
+                       a->E1, bc, a->f1, a->ex1, a->ex1 + N, a->ex1 + 2 * N, nullptr, nullptr,
+                       d->ctl,
                        d->part, d->part + d->nblocks, a->neg);
 }
 
@@ -1251,7 +1307,7 @@ int anuga_compute_fluxes(anuga_domain *d, double *flux_timestep)
     if (rc) return rc;
     ensure_live(d);
     launch_extrapolate(d, d->A);
-    launch_flux(d, 0, d->A, nullptr);
+    launch_flux(d, 0, d->A);
     launch_reduce(d, 0);
     rc = launch_check(d);
     if (rc) return rc;
@@ -1279,31 +1335,43 @@ int anuga_update_conserved_quantities(anuga_domain *d, double dt, int64_t *num_n
     if (rc) return rc;
     if (d->substate != 0) return fail(d, ANUGA_E_STATE, "update called twice in a step");
     ensure_live(d);
-    const bool rk2 = d->P.timestep_fluxcalls == 2;
+    const bool rk = d->P.timestep_fluxcalls > 1;
     hipLaunchKernelGGL(de_set_dt, dim3(1), dim3(64), 0, d->stream, d->ctl, dt);
-    launch_update(d, d->A, rk2 ? d->B : d->A);
-    if (!rk2) hipLaunchKernelGGL(de_integrate, dim3(1), dim3(64), 0, d->stream, d->ctl, 0);
+    launch_update(d, d->A, rk ? d->B : d->A);
+    if (!rk) hipLaunchKernelGGL(de_integrate, dim3(1), dim3(64), 0, d->stream, d->ctl, 1);
     rc = launch_check(d);
     if (rc) return rc;
-    if (rk2) d->substate = 1;
+    if (rk) d->substate = 1;
+    return num_negative ? read_negative(d, num_negative) : ANUGA_OK;
+}
+
+// the substeps after the first of an rk2 / rk3 step (same dt)
+static int later_substeps(anuga_domain *d, double dt, int64_t *num_negative, int method)
+{
+    int rc = single(d);
+    if (rc) return rc;
+    if (d->P.timestep_fluxcalls != method)
+        return fail(d, ANUGA_E_STATE, "substeps of another timestepping method");
+    if (d->substate != 1) return fail(d, ANUGA_E_STATE, "second substep without a first");
+    hipLaunchKernelGGL(de_set_dt, dim3(1), dim3(64), 0, d->stream, d->ctl, dt);
+    enqueue_later_substeps(d);
+    hipLaunchKernelGGL(de_integrate, dim3(1), dim3(64), 0, d->stream, d->ctl, method);
+    rc = launch_check(d);
+    if (rc) return rc;
+    d->substate = 0;
     return num_negative ? read_negative(d, num_negative) : ANUGA_OK;
 }
 
 int anuga_rk2_second_substep(anuga_domain *d, double dt, int64_t *num_negative)
 {
     if (!d) return ANUGA_E_ARG;
-    int rc = single(d);
-    if (rc) return rc;
-    if (d->substate != 1) return fail(d, ANUGA_E_STATE, "second substep without a first");
-    hipLaunchKernelGGL(de_set_dt, dim3(1), dim3(64), 0, d->stream, d->ctl, dt);
-    launch_extrapolate(d, d->B);
-    launch_flux(d, 1, d->B, d->A);
-    launch_reduce(d, 1);
-    hipLaunchKernelGGL(de_integrate, dim3(1), dim3(64), 0, d->stream, d->ctl, 1);
-    rc = launch_check(d);
-    if (rc) return rc;
-    d->substate = 0;
-    return num_negative ? read_negative(d, num_negative) : ANUGA_OK;
+    return later_substeps(d, dt, num_negative, 2);
+}
+
+int anuga_rk3_later_substeps(anuga_domain *d, double dt, int64_t *num_negative)
+{
+    if (!d) return ANUGA_E_ARG;
+    return later_substeps(d, dt, num_negative, 3);
 }
 
 int anuga_evolve_steps(anuga_domain *d, int64_t max_steps, const anuga_time_control *tc,
@@ -1322,7 +1390,8 @@ int anuga_get_boundary_flux_sums(anuga_domain *d, double *sums)
     rc = sync_scalars(d);
     if (rc) return rc;
     sums[0] = d->hctl->bflux0;
-    sums[1] = d->P.timestep_fluxcalls == 2 ? d->hctl->bflux1 : 0.0;
+    sums[1] = d->P.timestep_fluxcalls >= 2 ? d->hctl->bflux1 : 0.0;
+    sums[2] = d->P.timestep_fluxcalls == 3 ? d->hctl->bflux2 : 0.0;
     return ANUGA_OK;
 }
 

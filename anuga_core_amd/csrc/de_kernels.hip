@@ -11,9 +11,10 @@
 //                   order (bit-identical to the reference's compute-once-and-
 //                   mirror loop, sw_domain_orig.c:765-1150), boundary states
 //                   evaluated in place, CFL timestep and boundary inflow as
-//                   per-block partials. In the second rk2 substep the kernel
-//                   also applies friction, the update, fix_negative_cells and
-//                   the final saxpy, since dt is already known.
+//                   per-block partials. In the later Runge-Kutta substeps the
+//                   kernel also applies friction, the update,
+//                   fix_negative_cells and the substep's combination with the
+//                   step's initial state, since dt is already known.
 //   de_update       friction + update + fix_negative_cells
 //                   (sw_domain_orig.c:2805-2838, quantity.c:772-822,
 //                   sw_domain_orig.c:2890-2910).
@@ -157,20 +158,22 @@ de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict
         bool constant = (nbound == 3);
         bool gw = true, gu = true, gv = true;   // false: beta == 0, edge = centroid value
         if (nbound <= 1) {
-            T dx1, dx2, dy1, dy2;
+            T dx1, dx2, dy1, dy2, area2, inv_area2;
             if constexpr (sizeof(T) == 8) {
-                const double2 c0 = m.cen[sur[0]], c1 = m.cen[sur[1]], c2 = m.cen[sur[2]];
-                dx1 = c1.x - c0.x;
-                dx2 = c2.x - c0.x;
-                dy1 = c1.y - c0.y;
-                dy2 = c2.y - c0.y;
+                dx1 = gx.aux[0];
+                dx2 = gx.aux[1];
+                dy1 = gx.aux[2];
+                dy2 = gx.aux[3];
+                area2 = gx.aux[4];
+                inv_area2 = gx.aux[5];
             } else {
                 dx1 = gx.rx[1] - gx.rx[0];
                 dx2 = gx.rx[2] - gx.rx[0];
                 dy1 = gx.ry[1] - gx.ry[0];
                 dy2 = gx.ry[2] - gx.ry[0];
+                area2 = dy2 * dx1 - dy1 * dx2;
+                inv_area2 = T(1.0) / area2;
             }
-            const T area2 = dy2 * dx1 - dy1 * dx2;
             if (area2 <= T(0.0)) {
                 constant = true;
             } else {
@@ -186,7 +189,6 @@ de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict
                                                    T(1.0))));
                 hfactor = fmin(T(1.2) * fmax(hmin - tmah, T(0.0)) / (fmax(hmin, T(0.0)) + T(1.0) * tmah),
                                hfactor);
-                const T inv_area2 = T(1.0) / area2;
 
                 T beta = T(p.bwd) + (T(p.bw) - T(p.bwd)) * hfactor;
                 gw = beta > T(0.0);
@@ -223,20 +225,18 @@ de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict
                 n1.h = e0 ? a0.h : (e1 ? a1.h : a2.h);
                 n1.u = e0 ? a0.u : (e1 ? a1.u : a2.u);
                 n1.v = e0 ? a0.v : (e1 ? a1.v : a2.v);
-                T dx1, dy1;
+                T dx2, dy2;
                 if constexpr (sizeof(T) == 8) {
-                    const int s1 = e0 ? sur[0] : (e1 ? sur[1] : sur[2]);
-                    const double2 c1 = m.cen[s1], c0 = m.cen[k];
-                    dx1 = c1.x - c0.x;
-                    dy1 = c1.y - c0.y;
+                    dx2 = gx.aux[1];
+                    dy2 = gx.aux[3];
                 } else {
-                    dx1 = e0 ? gx.rx[0] : (e1 ? gx.rx[1] : gx.rx[2]);
-                    dy1 = e0 ? gx.ry[0] : (e1 ? gx.ry[1] : gx.ry[2]);
+                    const T dx1 = e0 ? gx.rx[0] : (e1 ? gx.rx[1] : gx.rx[2]);
+                    const T dy1 = e0 ? gx.ry[0] : (e1 ? gx.ry[1] : gx.ry[2]);
+                    const T area2 = dx1 * dx1 + dy1 * dy1;
+                    dx2 = T(1.0) / area2;
+                    dy2 = dx2 * dy1;
+                    dx2 *= dx1;
                 }
-                const T area2 = dx1 * dx1 + dy1 * dy1;
-                T dx2 = T(1.0) / area2;
-                const T dy2 = dx2 * dy1;
-                dx2 *= dx1;
                 const T bw = T(p.bw);
                 de_line_jumps<T>(c.w, n1.w, dx2, dy2, dxv, dyv, bw, jw);
                 de_line_jumps<T>(c.h, n1.h, dx2, dy2, dxv, dyv, bw, jh);
@@ -359,18 +359,45 @@ __device__ __forceinline__ DeQ de_advance(const DeCentroid c, double eta, double
     return r;
 }
 
+// The Runge-Kutta combination of substep SUBSTEP of the advanced state r with
+// the step's initial state b (saxpy_centroid_values, quantity.c:752-769, with
+// the weights of generic_domain.py:2018-2045 and :2053-2179):
+// 1: rk2 end, 0.5 r + 0.5 b; 2: rk3 middle, 0.25 r + 0.75 b;
+// 3: rk3 end, (2 r + 1 b) / 3.
+template <int SUBSTEP>
+__device__ __forceinline__ DeQ de_rk_combine(const DeQ &r, const DeQ &b)
+{
+    DeQ o;
+    if constexpr (SUBSTEP == 1) {
+        o.w = 0.5 * r.w + 0.5 * b.w;
+        o.uh = 0.5 * r.uh + 0.5 * b.uh;
+        o.vh = 0.5 * r.vh + 0.5 * b.vh;
+    } else if constexpr (SUBSTEP == 2) {
+        o.w = 0.25 * r.w + 0.75 * b.w;
+        o.uh = 0.25 * r.uh + 0.75 * b.uh;
+        o.vh = 0.25 * r.vh + 0.75 * b.vh;
+    } else {
+        o.w = (2.0 * r.w + 1.0 * b.w) / 3.0;
+        o.uh = (2.0 * r.uh + 1.0 * b.uh) / 3.0;
+        o.vh = (2.0 * r.vh + 1.0 * b.vh) / 3.0;
+    }
+    o.z = b.z;
+    return o;
+}
+
 // ---------------------------------------------------------------------------
 // flux
 // ---------------------------------------------------------------------------
-// SUBSTEP 0: writes explicit updates; SUBSTEP 1: advances with ctl->dt and
-// combines with the backup state bk (rk2 final stage, saxpy 0.5/0.5).
+// SUBSTEP 0: writes explicit updates; SUBSTEP > 0: advances with the
+// scenario's dt and writes de_rk_combine<SUBSTEP>(advanced, bk) to out (out
+// may be bk, or q itself: each thread reads its own q and bk before writing).
 template <class T, int SUBSTEP>
 __global__ void __launch_bounds__(DE_BLOCK)
 de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
         const T *__restrict__ E, DeBoundary bc, const unsigned char *__restrict__ flags,
-        double *__restrict__ exw, double *__restrict__ exuh, double *__restrict__ exvh, DeQ *bk,
-        DeControl *ctl, double *__restrict__ part_dt, double *__restrict__ part_bf,
-        unsigned long long *__restrict__ nneg)
+        double *__restrict__ exw, double *__restrict__ exuh, double *__restrict__ exvh,
+        const DeQ *bk, DeQ *out, DeControl *ctl, double *__restrict__ part_dt,
+        double *__restrict__ part_bf, unsigned long long *__restrict__ nneg)
 {
     __shared__ double red_dt[DE_BLOCK / 64];
     __shared__ double red_bf[DE_BLOCK / 64];
@@ -467,14 +494,7 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
             exvh[k] = sy;
         } else {
             const DeQ r = de_advance<sizeof(T) == 8>(c, eta[k], sw, sx, sy, dt, p, err, nneg);
-            // saxpy_centroid_values(0.5, 0.5) with the backup (quantity.c:752-769)
-            const DeQ b = bk[k];
-            DeQ o;
-            o.w = 0.5 * r.w + 0.5 * b.w;
-            o.uh = 0.5 * r.uh + 0.5 * b.uh;
-            o.vh = 0.5 * r.vh + 0.5 * b.vh;
-            o.z = b.z;
-            bk[k] = o;
+            out[k] = de_rk_combine<SUBSTEP>(r, bk[k]);
         }
     }
 
@@ -510,8 +530,8 @@ __global__ void __launch_bounds__(DE_BLOCK)
 de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
               const DeEdgeRec32 *__restrict__ E, DeBoundary bc,
               const unsigned char *__restrict__ flags, double *__restrict__ exw,
-              double *__restrict__ exuh, double *__restrict__ exvh, DeQ *bk, DeControl *ctl,
-              double *__restrict__ part_dt, double *__restrict__ part_bf,
+              double *__restrict__ exuh, double *__restrict__ exvh, const DeQ *bk, DeQ *out,
+              DeControl *ctl, double *__restrict__ part_dt, double *__restrict__ part_bf,
               unsigned long long *__restrict__ nneg)
 {
     __shared__ double red_dt[DE_BLOCK / 64];
@@ -609,13 +629,7 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
             exvh[k] = sy;
         } else {
             const DeQ r = de_advance<false>(c, eta[k], sw, sx, sy, dt, p, err, nneg);
-            const DeQ b = bk[k];
-            DeQ o;
-            o.w = 0.5 * r.w + 0.5 * b.w;
-            o.uh = 0.5 * r.uh + 0.5 * b.uh;
-            o.vh = 0.5 * r.vh + 0.5 * b.vh;
-            o.z = b.z;
-            bk[k] = o;
+            out[k] = de_rk_combine<SUBSTEP>(r, bk[k]);
         }
     }
 
@@ -661,7 +675,7 @@ de_update(const int *__restrict__ scen, DeParams p, const DeQ *q, const double *
 // Fixed-order reduction of the per-block partials into the control blocks:
 // workgroup s (1024 threads) reduces the blocks of scenario s. MODE 0:
 // flux_dt, bflux0, mass (first substep); 1: bflux1 (second substep);
-// 2: mass (distribute); 3: volume. Each thread keeps 8 loads in flight (the
+// 2: mass (distribute); 3: volume; 4: bflux2 (third substep). Each thread keeps 8 loads in flight (the
 // partials are few and the loop is bound by memory latency, not bandwidth);
 // the order of the additions depends only on the scenario's block count.
 template <int MODE>
@@ -669,11 +683,12 @@ __global__ void __launch_bounds__(1024)
 de_reduce(const double *__restrict__ part_dt, const double *__restrict__ part_bf,
           const double *__restrict__ part_mass, DeControl *ctl)
 {
-    constexpr bool DT = MODE == 0, BF = MODE <= 1, MS = MODE == 0 || MODE >= 2;
+    constexpr bool DT = MODE == 0, BF = MODE <= 1 || MODE == 4;
+    constexpr bool MS = MODE == 0 || MODE == 2 || MODE == 3;
     constexpr int U = 8;
     __shared__ double s0[16], s1[16], s2[16];
     DeControl *cb = ctl + blockIdx.x;
-    if (MODE < 2 && cb->done) return;
+    if ((MODE < 2 || MODE == 4) && cb->done) return;
     const int b0 = cb->bbeg, b1 = cb->bend;
     double a = 1.0e100, b = 0.0, c = 0.0;
     int i = b0 + threadIdx.x;
@@ -721,8 +736,10 @@ de_reduce(const double *__restrict__ part_dt, const double *__restrict__ part_bf
             cb->bflux1 = s1[0];
         } else if (MODE == 2) {
             cb->mass = s2[0];
-        } else {
+        } else if (MODE == 3) {
             cb->volume = s2[0];
+        } else {
+            cb->bflux2 = s1[0];
         }
     }
 }
@@ -732,6 +749,15 @@ __device__ __forceinline__ DeControl *de_scenario(DeControl *ctl, int S)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     return s < S ? ctl + s : nullptr;
+}
+
+// boundary_flux_integral_operator.__call__ (boundary_flux_integral_operator.py:46-62)
+__device__ __forceinline__ double de_flux_integral(double integral, double dt, int method,
+                                                   double s0, double s1, double s2)
+{
+    if (method == 1) return integral + dt * s0;
+    if (method == 2) return integral + 0.5 * dt * (s0 + s1);
+    return integral + 1.0 / 6.0 * dt * (s0 + s1 + 4.0 * s2);
 }
 
 // Generic_Domain.update_timestep (generic_domain.py:2349-2415) on the device:
@@ -772,8 +798,7 @@ __global__ void de_finish_step(DeControl *ctl, int S, double *hist)
     DeControl *c = de_scenario(ctl, S);
     if (!c || c->done) return;
     const double dt = c->dt;
-    if (c->rk2) c->integral = c->integral + 0.5 * dt * (c->bflux0 + c->bflux1);
-    else c->integral = c->integral + dt * c->bflux0;
+    c->integral = de_flux_integral(c->integral, dt, c->method, c->bflux0, c->bflux1, c->bflux2);
     c->t = c->t + dt;
     if (hist && c->nsteps < c->hist_cap) hist[c->nsteps] = dt;
     c->nsteps += 1;
@@ -792,17 +817,16 @@ __global__ void de_set_dt(DeControl *c, double dt)
 }
 
 // host-controlled steps: boundary flux integral after the update
-__global__ void de_integrate(DeControl *c, int rk2)
+__global__ void de_integrate(DeControl *c, int method)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (rk2) c->integral = c->integral + 0.5 * c->dt * (c->bflux0 + c->bflux1);
-    else c->integral = c->integral + c->dt * c->bflux0;
+    c->integral = de_flux_integral(c->integral, c->dt, method, c->bflux0, c->bflux1, c->bflux2);
 }
 
 // time-control parameters and state for a run of device-controlled steps
 struct DeBatchSetup {
     double t, rec_min, rec_max, cfl, max_dt, min_dt, finaltime, yieldtime, fixed_flux_dt, eps;
-    int smallsteps, order, default_order, max_smallsteps, hist_cap, rk2;
+    int smallsteps, order, default_order, max_smallsteps, hist_cap, method;
     int stop_reached;  // a scenario already at its final or yield time takes no step
 };
 
@@ -826,7 +850,7 @@ __global__ void de_setup_batch(DeControl *ctl, int S, const DeBatchSetup *__rest
     c->default_order = b.default_order;
     c->max_smallsteps = b.max_smallsteps;
     c->hist_cap = b.hist_cap;
-    c->rk2 = b.rk2;
+    c->method = b.method;
     c->nsteps = 0;
     c->done = 0;
     if (b.stop_reached) {

@@ -204,25 +204,27 @@ class Domain:
         return self.number_of_elements
 
     # -- algorithm presets ---------------------------------------------------------
+    # shallow_water_domain.py:583-895: CFL, timestepping, minimum_allowed_height,
+    # beta (w, uh, vh), beta_dry
+    FLOW_ALGORITHMS = {"DE0": (0.9, "euler", 1.0e-12, 0.5, 0.0),
+                       "DE1": (1.0, "rk2", 1.0e-5, 1.0, 0.0),
+                       "DE2": (1.0, "rk3", 1.0e-5, 1.0, 0.0),
+                       "DE1_7": (1.0, "rk2", 1.0e-12, 0.75, 0.1),
+                       "DE0_7": (0.9, "euler", 1.0e-12, 0.7, 0.1)}
+
     def set_flow_algorithm(self, algorithm="DE0"):
-        """DE0 / DE1 presets (shallow_water_domain.py:583-707)."""
+        """DE presets (shallow_water_domain.py:583-895): DE0, DE1, DE2, DE0_7, DE1_7."""
         algorithm = str(algorithm).replace(".", "_")
-        if algorithm == "DE0":
-            self.CFL = 0.9
-            self.set_timestepping_method("euler")
-            self.minimum_allowed_height = 1.0e-12
-            b = 0.5
-        elif algorithm == "DE1":
-            self.CFL = 1.0
-            self.set_timestepping_method("rk2")
-            self.minimum_allowed_height = 1.0e-5
-            b = 1.0
-        else:
-            raise Exception("Unknown or unsupported flow_algorithm %s; supported: DE0, DE1"
-                            % algorithm)
+        if algorithm not in self.FLOW_ALGORITHMS:
+            raise Exception("Unknown or unsupported flow_algorithm %s; supported: %s"
+                            % (algorithm, ", ".join(self.FLOW_ALGORITHMS)))
+        cfl, method, mah, b, bd = self.FLOW_ALGORITHMS[algorithm]
+        self.CFL = cfl
+        self.set_timestepping_method(method)
+        self.minimum_allowed_height = mah
         self.flow_algorithm = algorithm
         self.beta_w = self.beta_uh = self.beta_vh = b
-        self.beta_w_dry = self.beta_uh_dry = self.beta_vh_dry = 0.0
+        self.beta_w_dry = self.beta_uh_dry = self.beta_vh_dry = bd
         self.default_order = 2
         self.maximum_allowed_speed = 0.0
 
@@ -230,11 +232,11 @@ class Domain:
         return self.flow_algorithm
 
     def set_timestepping_method(self, method):
-        methods = {"euler": 1, "rk2": 2, 1: 1, 2: 2}
+        methods = {"euler": 1, "rk2": 2, "rk3": 3, 1: 1, 2: 2, 3: 3}
         if method not in methods:
             raise Exception("%s is an incorrect timestepping type" % method)
         self._timestep_fluxcalls = methods[method]
-        self.timestepping_method = "euler" if self._timestep_fluxcalls == 1 else "rk2"
+        self.timestepping_method = ("euler", "rk2", "rk3")[self._timestep_fluxcalls - 1]
 
     def get_timestepping_method(self):
         return self.timestepping_method
@@ -458,6 +460,13 @@ class Domain:
         self.relative_time = self.relative_time + self.timestep
         self._call("anuga_rk2_second_substep", C.c_double(self.timestep), None)
 
+    def evolve_one_rk3_step(self, yieldstep, finaltime):
+        """generic_domain.py:2053-2179; the backup is the untouched step state."""
+        self._compute_fluxes()
+        self.update_timestep(yieldstep, finaltime)
+        self._call("anuga_update_conserved_quantities", C.c_double(self.timestep), None)
+        self._call("anuga_rk3_later_substeps", C.c_double(self.timestep), None)
+
     def get_time(self):
         return self.starttime + self.relative_time
 
@@ -533,8 +542,10 @@ class Domain:
         self._distributed = False
         if self._timestep_fluxcalls == 1:
             self.evolve_one_euler_step(yieldstep, self.relative_finaltime)
-        else:
+        elif self._timestep_fluxcalls == 2:
             self.evolve_one_rk2_step(yieldstep, self.relative_finaltime)
+        else:
+            self.evolve_one_rk3_step(yieldstep, self.relative_finaltime)
         self.relative_time = initial + self.timestep
         if self.record_dt_history:
             self.dt_history.append(self.timestep)

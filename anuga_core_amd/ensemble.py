@@ -32,8 +32,10 @@ from .domain import EPSILON, G, MAX_SMALLSTEPS, MAX_TIMESTEP, MIN_TIMESTEP, _QID
 _KINDS = {"reflective": KIND_REFLECTIVE, "transmissive": KIND_TRANSMISSIVE,
           "transmissive_centroid": KIND_TRANSMISSIVE_CENTROID}
 
-# DE presets (shallow_water_domain.py:583-707): CFL, fluxcalls, mah, beta
-_PRESETS = {"DE0": (0.9, 1, 1.0e-12, 0.5), "DE1": (1.0, 2, 1.0e-5, 1.0)}
+# DE presets (shallow_water_domain.py:583-895): CFL, fluxcalls, mah, beta, beta_dry
+_PRESETS = {"DE0": (0.9, 1, 1.0e-12, 0.5, 0.0), "DE1": (1.0, 2, 1.0e-5, 1.0, 0.0),
+            "DE2": (1.0, 3, 1.0e-5, 1.0, 0.0), "DE1_7": (1.0, 2, 1.0e-12, 0.75, 0.1),
+            "DE0_7": (0.9, 1, 1.0e-12, 0.7, 0.1)}
 
 
 def _ip(a):
@@ -61,14 +63,15 @@ class Ensemble:
         self.number_of_triangles = np.array([s[0] for s in sizes], np.int64)
         self.boundary_length = np.array([s[1] for s in sizes], np.int64)
         if flow_algorithm not in _PRESETS:
-            raise Exception("Unknown or unsupported flow_algorithm %s; supported: DE0, DE1"
-                            % flow_algorithm)
+            raise Exception("Unknown or unsupported flow_algorithm %s; supported: %s"
+                            % (flow_algorithm, ", ".join(_PRESETS)))
         self.flow_algorithm = flow_algorithm
-        self.CFL, self._fluxcalls, self.minimum_allowed_height, beta = _PRESETS[flow_algorithm]
+        self.CFL, self._fluxcalls, self.minimum_allowed_height, beta, bdry = \
+            _PRESETS[flow_algorithm]
         self.precision = precision
         prec = {"fp64": _native.PRECISION_FP64, "mixed": _native.PRECISION_MIXED}[precision]
         self._params = _native.AnugaParams(G, EPSILON, self.minimum_allowed_height,
-                                           beta, 0.0, beta, 0.0, beta, 0.0, 0, 1,
+                                           beta, bdry, beta, bdry, beta, bdry, 0, 1,
                                            self._fluxcalls, prec, int(bool(reorder)))
         self.evolve_max_timestep = MAX_TIMESTEP
         self.evolve_min_timestep = MIN_TIMESTEP

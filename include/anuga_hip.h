@@ -85,7 +85,7 @@ typedef struct anuga_params {
     double beta_vh, beta_vh_dry;
     int64_t low_froude;                        /* only 0 (the DE presets) */
     int64_t extrapolate_velocity_second_order; /* only 1 (the DE presets) */
-    int64_t timestep_fluxcalls;                /* 1 = euler (DE0), 2 = rk2 (DE1) */
+    int64_t timestep_fluxcalls;                /* 1 = euler (DE0), 2 = rk2 (DE1), 3 = rk3 (DE2) */
     int64_t precision;                         /* ANUGA_PRECISION_* */
     int64_t reorder;                           /* 1 = Reverse-Cuthill-McKee renumbering */
 } anuga_params;
@@ -138,6 +138,12 @@ int anuga_update_conserved_quantities(anuga_domain *d, double dt, int64_t *num_n
  * (generic_domain.py:2018-2045; quantity.c:752-769). */
 int anuga_rk2_second_substep(anuga_domain *d, double dt, int64_t *num_negative);
 
+/* The second and third rk3 substeps with the same dt (DE2): protect +
+ * extrapolate + fluxes + friction + update, saxpy(0.25, 0.75) with the step's
+ * initial state, again, then saxpy(2.0, 1.0) and the division by 3
+ * (generic_domain.py:2053-2179; quantity.c:752-769). */
+int anuga_rk3_later_substeps(anuga_domain *d, double dt, int64_t *num_negative);
+
 /* Timestep control of Generic_Domain.update_timestep (generic_domain.py:2349-2415). */
 typedef struct anuga_time_control {
     double CFL;
@@ -174,7 +180,7 @@ typedef struct anuga_step_state {
 int anuga_evolve_steps(anuga_domain *d, int64_t max_steps, const anuga_time_control *tc,
                        anuga_step_state *state, double *dt_history);
 
-/* boundary_flux_sum of the last step's substeps (sums[0..1]): the boundary
+/* boundary_flux_sum of the last step's substeps (sums[0..2]): the boundary
  * inflow rate (sw_domain_orig.c:1119-1126). Synchronizes. */
 int anuga_get_boundary_flux_sums(anuga_domain *d, double *sums);
 

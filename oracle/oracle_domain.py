@@ -92,23 +92,25 @@ class OracleDomain:
         self.negative_cells = 0
 
     # -- presets ------------------------------------------------------------
+    # shallow_water_domain.py:583-895: CFL, timestepping, minimum_allowed_height,
+    # beta, beta_dry
+    _PRESETS = {"DE0": (0.9, "euler", 1.0e-12, 0.5, 0.0),
+                "DE1": (1.0, "rk2", 1.0e-5, 1.0, 0.0),
+                "DE2": (1.0, "rk3", 1.0e-5, 1.0, 0.0),
+                "DE1_7": (1.0, "rk2", 1.0e-12, 0.75, 0.1),
+                "DE0_7": (0.9, "euler", 1.0e-12, 0.7, 0.1)}
+
     def set_flow_algorithm(self, algorithm="DE0"):
         algorithm = str(algorithm).replace(".", "_")
-        if algorithm == "DE0":
-            self.CFL = 0.9
-            self.timestepping_method = "euler"
-            self.minimum_allowed_height = 1.0e-12
-            b = 0.5
-        elif algorithm == "DE1":
-            self.CFL = 1.0
-            self.timestepping_method = "rk2"
-            self.minimum_allowed_height = 1.0e-5
-            b = 1.0
-        else:
+        if algorithm not in self._PRESETS:
             raise ValueError("flow algorithm %s is not supported by the oracle" % algorithm)
+        cfl, method, mah, b, bd = self._PRESETS[algorithm]
+        self.CFL = cfl
+        self.timestepping_method = method
+        self.minimum_allowed_height = mah
         self.flow_algorithm = algorithm
         self.beta_w = self.beta_uh = self.beta_vh = b
-        self.beta_w_dry = self.beta_uh_dry = self.beta_vh_dry = 0.0
+        self.beta_w_dry = self.beta_uh_dry = self.beta_vh_dry = bd
         self.default_order = 2
 
     # -- quantities -----------------------------------------------------------
@@ -238,14 +240,46 @@ class OracleDomain:
         self.update_conserved_quantities()
         self.kernels.saxpy(0.5, 0.5)
 
+    def evolve_one_rk3_step(self, yieldstep, finaltime):
+        """generic_domain.py:2053-2179"""
+        self.kernels.backup()
+        initial = self.relative_time
+        self.distribute_to_vertices_and_edges()
+        self.update_boundary()
+        self.compute_fluxes(0)
+        self.compute_forcing_terms()
+        self.update_timestep(yieldstep, finaltime)
+        self.update_conserved_quantities()
+        self.relative_time = self.relative_time + self.timestep
+        self.distribute_to_vertices_and_edges()
+        self.update_boundary()
+        self.compute_fluxes(1)
+        self.compute_forcing_terms()
+        self.update_conserved_quantities()
+        self.kernels.saxpy(0.25, 0.75)
+        self.relative_time = initial + self.timestep * 0.5
+        self.distribute_to_vertices_and_edges()
+        self.update_boundary()
+        self.compute_fluxes(2)
+        self.compute_forcing_terms()
+        self.update_conserved_quantities()
+        self.kernels.saxpy(2.0, 1.0)
+        for q in (self.stage_c, self.xmom_c, self.ymom_c):
+            q[:] = q / 3.0
+        self.relative_time = initial + self.timestep
+
     def _apply_boundary_flux_integral(self):
+        """boundary_flux_integral_operator.py:46-62"""
         dt = self.timestep
+        s = self.boundary_flux_sum
         if self.timestepping_method == "euler":
-            self.boundary_flux_integral = self.boundary_flux_integral + dt * self.boundary_flux_sum[0]
+            self.boundary_flux_integral = self.boundary_flux_integral + dt * s[0]
+        elif self.timestepping_method == "rk2":
+            self.boundary_flux_integral = self.boundary_flux_integral + 0.5 * dt * s[0:2].sum()
         else:
             self.boundary_flux_integral = (self.boundary_flux_integral
-                                           + 0.5 * dt * self.boundary_flux_sum[0:2].sum())
-        self.boundary_flux_sum[:] = 0.0
+                                           + 1.0 / 6.0 * dt * (s[0] + s[1] + 4.0 * s[2]))
+        s[:] = 0.0
 
     # -- evolve ---------------------------------------------------------------------
     def get_time(self):
@@ -277,8 +311,10 @@ class OracleDomain:
             initial = self.relative_time
             if self.timestepping_method == "euler":
                 self.evolve_one_euler_step(yieldstep, self.relative_finaltime)
-            else:
+            elif self.timestepping_method == "rk2":
                 self.evolve_one_rk2_step(yieldstep, self.relative_finaltime)
+            else:
+                self.evolve_one_rk3_step(yieldstep, self.relative_finaltime)
             self._apply_boundary_flux_integral()
             self.relative_time = initial + self.timestep
             self.dt_history.append(self.timestep)

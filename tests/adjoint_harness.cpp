@@ -102,6 +102,7 @@ Run make_run(int N, int M, const int *nb, const double *cen, const double *gx, c
     r.m.nb = nb;
     r.m.cen = cen;
     r.m.gx = gx;
+    r.m.gx_stride = 6;
     r.m.nx = nx;
     r.m.ny = ny;
     r.m.len = len;

@@ -216,3 +216,37 @@ def test_mixed_precision_error_vs_oracle(builder, kw, ft):
     drift = (d.get_water_volume() - v0 - d.boundary_flux_integral) / v0
     print("mixed volume drift %.3e" % drift, flush=True)
     assert abs(drift) < 1e-10
+
+
+def _with_algorithm(builder, alg):
+    def b(**kw):
+        mesh, _, q, bc = builder(**kw)
+        return mesh, alg, q, bc
+    b.__name__ = "%s_%s" % (builder.__name__, alg)
+    return b
+
+
+@pytest.mark.parametrize("alg,builder,kw,ys,ft", [
+    ("DE2", S.hills_de1, dict(m=24, n=24), 5.0, 25.0),
+    ("DE2", S.dam_break_de0, dict(m=30, n=30), 1.0, 3.0),
+    ("DE1_7", S.hills_de1, dict(m=24, n=24), 5.0, 25.0),
+    ("DE0_7", S.dam_break_de0, dict(m=30, n=30), 1.0, 3.0),
+])
+def test_other_de_algorithms_vs_oracle(alg, builder, kw, ys, ft):
+    """DE2 (rk3) and the DE0_7 / DE1_7 presets (beta_dry 0.1) against the oracle
+    (shallow_water_domain.py:709-895), device-controlled steps."""
+    print("case %s %s" % (alg, builder.__name__), flush=True)
+    o, d = _pair(_with_algorithm(builder, alg), **kw)
+    for _ in d.evolve(yieldstep=ys, finaltime=ft):
+        pass
+    for _ in o.evolve(yieldstep=ys, finaltime=ft):
+        pass
+    assert d.dt_history == o.dt_history, "timestep sequences differ"
+    _compare(o, d, alg)
+    bo, bd = o.boundary_flux_integral, d.boundary_flux_integral
+    assert abs(bd - bo) <= TOL * max(abs(bo), 1.0)
+
+
+def test_de2_device_timestep_matches_host_timestep():
+    test_device_timestep_matches_host_timestep(_with_algorithm(S.hills_de1, "DE2"),
+                                               dict(m=20, n=20), 4.0, 20.0)

@@ -311,3 +311,46 @@ def test_rk2_step_is_ssp_average_of_two_euler_steps():
         assert np.array_equal(a, 0.5 * b + 0.5 * c)
     # and it is not the plain second Euler stage
     assert not np.array_equal(rk2[0], e2[0])
+
+
+def test_rk3_step_is_the_reference_combination_of_euler_steps():
+    """DE2's step (generic_domain.py:2053-2179) with the first stage's timestep
+    tau for all stages: Q1 = 0.25 E(E(Qn)) + 0.75 Qn, then
+    Q(n+1) = (2 E(Q1) + Qn) / 3, E the golden-pinned Euler step (a wet/dry case,
+    fixed flux timestep), bit for bit."""
+    mesh, _, q, bc = S.dam_break_de0(m=12, n=12)
+    probe = S.setup(OracleDomain(mesh, OracleKernels(gather=True)), "DE2", q, bc)
+    for _ in probe.evolve(yieldstep=1e300, finaltime=1e300, max_steps=1):
+        pass
+    tau = 0.9 * probe.dt_history[0]
+
+    def fresh(method):
+        o = S.setup(OracleDomain(mesh, OracleKernels(gather=True)), "DE2", q, bc)
+        o.timestepping_method = method
+        o.fixed_flux_timestep = tau
+        return o
+
+    o = fresh("rk3")
+    gen = o.evolve(yieldstep=1e300, finaltime=1e300, max_steps=1)
+    next(gen)
+    q0 = [o.stage_c.copy(), o.xmom_c.copy(), o.ymom_c.copy()]
+    for _ in gen:
+        pass
+    rk3 = [o.stage_c.copy(), o.xmom_c.copy(), o.ymom_c.copy()]
+    # the Euler stages, driven one at a time from the states the formula names
+    e = fresh("euler")
+    gen = e.evolve(yieldstep=1e300, finaltime=1e300, max_steps=2)
+    next(gen)
+    for _ in gen:
+        pass
+    q1 = [0.25 * a + 0.75 * b for a, b in zip([e.stage_c, e.xmom_c, e.ymom_c], q0)]
+    e3 = fresh("euler")
+    for name, v in zip(("stage", "xmomentum", "ymomentum"), q1):
+        e3.set_centroid_values(name, v)
+    e3.evolved_called = True          # no initial distribute: q1 is a raw stage state
+    gen = e3.evolve(yieldstep=1e300, finaltime=1e300, max_steps=1)
+    for _ in gen:
+        pass
+    expect = [(2.0 * a + 1.0 * b) / 3.0 for a, b in zip([e3.stage_c, e3.xmom_c, e3.ymom_c], q0)]
+    for a, b in zip(rk3, expect):
+        assert np.array_equal(a, b)
