@@ -1,21 +1,21 @@
-// Correctly rounded x^(7/3) for FP64 Manning friction, host and device.
+// Correctly rounded x^y for the FP64 path's two library pow calls, host and
+// device: Manning friction's pow(h, 7.0/3.0) (sw_domain_orig.c:2827) and the
+// weir's pow(1 - Q2/Q1, 0.385) (sw_domain_orig.c:710).
 //
-// sw_domain_orig.c:2827 computes S /= pow(h, 7.0/3.0) with the C library's
-// pow. glibc's pow returns the correctly rounded result in all but extremely
-// rare cases, while the GPU math library's pow is only faithful (1 ulp), which
-// breaks bitwise parity after a few steps. This evaluates
-// exp(fl(7/3) * ln(x)) in double-double arithmetic (relative error below
-// 2^-80 before the final rounding), so the rounded result is the correctly
-// rounded one except when the exact value lies within 2^-80 of a rounding
-// boundary.
+// glibc's pow returns the correctly rounded result in all but extremely rare
+// cases, while the GPU math library's pow is only faithful (1 ulp), which
+// breaks bitwise parity after a few steps. This evaluates exp(y * ln(x)) in
+// double-double arithmetic (relative error below 2^-80 before the final
+// rounding), so the rounded result is the correctly rounded one except when
+// the exact value lies within 2^-80 of a rounding boundary.
 //
 // Method (textbook table-driven reductions, e.g. Muller, "Elementary
 // Functions", ch. 6):
 //   x = 2^e * m, m in [1,2); t = m * RINV[i] - 1 (exact, |t| < 2^-7)
 //   ln x = e ln2 - ln RINV[i] + log1p(t)   (cubic part in double-double,
 //                                          t^4.. tail in double)
-//   y = fl(7/3) * ln x = (k * 128 + ... ) : y = (128 q + j) ln2/128 + r,
-//   x^(7/3) = 2^q * 2^(j/128) * exp(r),   |r| <= ln2/256
+//   y ln x = (128 q + j) ln2/128 + r,
+//   x^y = 2^q * 2^(j/128) * exp(r),   |r| <= ln2/256
 // Tables: anuga_core_amd/csrc/gen_pow_tables.py. Valid for normal x > 0 whose
 // result is a normal double (friction uses x = h >= minimum_allowed_height).
 #pragma once
@@ -254,14 +254,14 @@ DE_POW_TABLE double kDePowExp2t[256] = {
     0x1.fa7c1819e90d8p+0, 0x1.74853f3a5931ep-55, 0x1.fd3c22b8f71f1p+0, 0x1.2eb74966579e7p-57,
 };
 
-DE_POW_FN double de_pow_7_3(double x)
+// x^y for a normal x > 0 whose result is a normal double
+DE_POW_FN double de_pow_cr(double x, double y)
 {
     const de_dd kLn2{0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
     const de_dd kNegLn2_128{-0x1.62e42fefa39efp-8, -0x1.abc9e3b39803fp-63};
     const de_dd kThird{0x1.5555555555555p-2, 0x1.5555555555555p-56};
     const de_dd kSixth{0x1.5555555555555p-3, 0x1.5555555555555p-57};
     const double kInvLn2_128 = 0x1.71547652b82fep+7;   // 128 / ln 2
-    const double kY = 7.0 / 3.0;
 
     int e;
     double m = frexp(x, &e);  // m in [0.5, 1)
@@ -285,7 +285,7 @@ DE_POW_FN double de_pow_7_3(double x)
     L = de_dd_add(L, de_dd{kDePowNlogr[2 * i], kDePowNlogr[2 * i + 1]});
     L = de_dd_add(L, l1);
 
-    const de_dd X = de_dd_mul_d(L, kY);
+    const de_dd X = de_dd_mul_d(L, y);
     const double kf = rint(X.hi * kInvLn2_128);
     const de_dd r = de_dd_add(X, de_dd_mul_d(kNegLn2_128, kf));
     const int k = (int)kf;
@@ -307,3 +307,10 @@ DE_POW_FN double de_pow_7_3(double x)
     const de_dd R = de_dd_mul(de_dd{kDePowExp2t[2 * j], kDePowExp2t[2 * j + 1]}, E);
     return ldexp(R.hi + R.lo, q);
 }
+
+// h^(7/3) of Manning friction (sw_domain_orig.c:2827)
+DE_POW_FN double de_pow_7_3(double x) { return de_pow_cr(x, 7.0 / 3.0); }
+
+// x^0.385 of the weir's Villemonte submergence correction
+// (sw_domain_orig.c:710), x = 1 - Q2/Q1 in [0, 1]
+DE_POW_FN double de_pow_0385(double x) { return x == 0.0 ? 0.0 : de_pow_cr(x, 0.385); }
