@@ -75,6 +75,8 @@ SIGNATURES = {
     "anuga_get_centroid_values": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "anuga_get_edge_values": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "anuga_set_boundary": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _dp]),
+    "anuga_set_riverwalls": (C.c_int, [C.c_void_p, _ip, C.c_int64, _dp, _ip, C.c_int64,
+                                       C.c_int64, _dp]),
     "anuga_distribute_to_vertices_and_edges": (C.c_int, [C.c_void_p, _dp]),
     "anuga_compute_fluxes": (C.c_int, [C.c_void_p, _dp]),
     "anuga_update_conserved_quantities": (C.c_int, [C.c_void_p, C.c_double, _ip]),

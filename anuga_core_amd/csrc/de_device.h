@@ -66,6 +66,13 @@ struct alignas(64) DeEdgeRec32 {
     double pad;
 };
 
+// one riverwall half-edge (the layout of anuga/structures/riverwall.py
+// resolved per half-edge): the wall crest elevation and the hydraulic
+// parameters of its wall
+struct alignas(16) DeRiverwall {
+    double elev, qfactor, s1, s2, h1, h2;
+};
+
 template <class T>
 struct alignas(16) DeEdge4 {  // extrapolated values on one edge
     T w, h, uh, vh;
@@ -87,6 +94,10 @@ struct DeMesh {
     const double *__restrict__ ny;
     const double *__restrict__ len;
     const double *__restrict__ radius;
+    // riverwalls (FP64 path): per half-edge, [3][N], the index of its
+    // DeRiverwall record or -1
+    const int *__restrict__ rw;
+    const DeRiverwall *__restrict__ rwd;
 };
 
 struct DeParams {

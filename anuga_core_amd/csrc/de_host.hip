@@ -57,6 +57,9 @@ struct anuga_domain {
     double *area = nullptr;
     double *nx = nullptr, *ny = nullptr, *len = nullptr, *radius = nullptr;  // FP64 flux geometry
     int *scen = nullptr;                 // [nblocks] scenario of each block (S > 1)
+    std::vector<int4> hnb;               // host copy of nb
+    int *rw = nullptr;                   // [3][N] riverwall record per half-edge, -1: none
+    DeRiverwall *rwd = nullptr;          // riverwall records; rw == nullptr: no riverwalls
     bool mixed = false;
     // quantities
     double *eta = nullptr;
@@ -314,6 +317,8 @@ static DeMesh dev_mesh(const anuga_domain *d)
     m.ny = d->ny;
     m.len = d->len;
     m.radius = d->radius;
+    m.rw = d->rw;
+    m.rwd = d->rwd;
     return m;
 }
 
@@ -346,12 +351,20 @@ template <class T, int SUB>
 static void launch_flux_t(anuga_domain *d, DeQ *S, DeQ *bk, DeQ *out)
 {
     DeBoundary bc{d->bkind, d->bval, d->M};
-    if constexpr (sizeof(T) == 8)
-        hipLaunchKernelGGL((de_flux<double, SUB>), dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
-                           dev_mesh(d), dev_params(d->P), S, d->eta,
-                           (const double *)d->E, bc, d->flags, d->ex[0], d->ex[1],
-                           d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks, d->dneg);
-    else
+    if constexpr (sizeof(T) == 8) {
+        if (d->rw)
+            hipLaunchKernelGGL((de_flux<double, SUB, true>), dim3(d->nblocks), dim3(DE_BLOCK), 0,
+                               d->stream, dev_mesh(d), dev_params(d->P), S, d->eta,
+                               (const double *)d->E, bc, d->flags, d->ex[0], d->ex[1],
+                               d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
+                               d->dneg);
+        else
+            hipLaunchKernelGGL((de_flux<double, SUB>), dim3(d->nblocks), dim3(DE_BLOCK), 0,
+                               d->stream, dev_mesh(d), dev_params(d->P), S, d->eta,
+                               (const double *)d->E, bc, d->flags, d->ex[0], d->ex[1],
+                               d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
+                               d->dneg);
+    } else
         hipLaunchKernelGGL((de_flux_mixed<SUB>), dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
                            dev_mesh(d), dev_params(d->P), S, d->eta, (const DeEdgeRec32 *)d->E,
                            bc, d->flags, d->ex[0], d->ex[1], d->ex[2], bk, out, d->ctl, d->part,
@@ -723,6 +736,8 @@ static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
     }
     HIPCHK(d, hipStreamSynchronize(d->stream));
     HIPCHK(d, hipMemcpy(d->nb + kb, hn.data(), N * sizeof(int4), hipMemcpyHostToDevice));
+    if (d->hnb.empty()) d->hnb.assign(d->N, make_int4(-1, -1, -1, 0));
+    std::copy(hn.begin(), hn.end(), d->hnb.begin() + kb);
     HIPCHK(d, hipMemcpy(d->cen + kb, hc.data(), N * sizeof(double2), hipMemcpyHostToDevice));
     HIPCHK(d, hipMemcpy(d->area + kb, ha.data(), N * sizeof(double), hipMemcpyHostToDevice));
     if (!d->mixed) {
@@ -1176,7 +1191,7 @@ void anuga_destroy(anuga_domain *d)
     if (d->stream) hipStreamSynchronize(d->stream);
     drop_graph(d);
     void *ptrs[] = {d->nb, d->cen, d->gx, d->gf, d->area, d->nx, d->ny, d->len, d->radius,
-                    d->scen, d->eta, d->A, d->B, d->E, d->flags,
+                    d->scen, d->rw, d->rwd, d->eta, d->A, d->B, d->E, d->flags,
                     d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->ctl, d->dsetup,
                     d->dneg, d->hist};
     for (void *p : ptrs)
@@ -1284,6 +1299,76 @@ int anuga_set_boundary(anuga_domain *d, const int32_t *kind, const double *diric
     if (!d) return ANUGA_E_ARG;
     int rc = single(d);
     return rc ? rc : set_boundary(d, 0, kind, dirichlet_values);
+}
+
+int anuga_set_riverwalls(anuga_domain *d, const int64_t *edge_flux_type,
+                         int64_t number_of_riverwall_edges, const double *riverwall_elevation,
+                         const int64_t *riverwall_rowIndex, int64_t nrows, int64_t ncols,
+                         const double *hydraulic_properties)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    if (d->substate != 0) return fail(d, ANUGA_E_STATE, "inside a multi-stage step");
+    const int N = d->ntri[0];
+    const int64_t R = number_of_riverwall_edges;
+    if (R < 0 || R > 3 * (int64_t)N) return fail(d, ANUGA_E_ARG, "bad riverwall edge count");
+    if (R > 0) {
+        if (d->mixed) return fail(d, ANUGA_E_ARG, "riverwalls need the FP64 path");
+        if (!edge_flux_type || !riverwall_elevation || !riverwall_rowIndex ||
+            !hydraulic_properties)
+            return fail(d, ANUGA_E_ARG, "a riverwall array is NULL");
+        if (ncols < 5 || nrows < 1)
+            return fail(d, ANUGA_E_ARG, "the hydraulic table needs >= 1 row of >= 5 columns");
+    }
+    // ranks of the flagged half-edges in (triangle, edge) order
+    std::vector<int> rank(3 * (size_t)N, -1);
+    int64_t r = 0;
+    if (R > 0)
+        for (int64_t ki = 0; ki < 3 * (int64_t)N; ki++)
+            if (edge_flux_type[ki] == 1) {
+                if (r == R) return fail(d, ANUGA_E_ARG, "more flagged edges than riverwall edges");
+                rank[ki] = (int)r++;
+            }
+    if (r != R) return fail(d, ANUGA_E_ARG, "fewer flagged edges than riverwall edges");
+    std::vector<DeRiverwall> rec(R);
+    for (int64_t j = 0; j < R; j++) {
+        const int64_t row = riverwall_rowIndex[j];
+        if (row < 0 || row >= nrows) return fail(d, ANUGA_E_ARG, "riverwall_rowIndex out of range");
+        const double *h = hydraulic_properties + row * ncols;
+        rec[j] = DeRiverwall{riverwall_elevation[j], h[0], h[1], h[2], h[3], h[4]};
+    }
+    // An interior edge is computed once, from the half-edge of the triangle
+    // visited first, and mirrored (sw_domain_orig.c:822-840, :1023-1037): it
+    // is a wall iff that half-edge is flagged, with that half-edge's record.
+    // Both half-edges take that record (each triangle evaluates its own edges).
+    const size_t NN = d->N;   // internal stride (incl. padding)
+    std::vector<int> hr(3 * NN, -1);
+    for (int p = 0; p < N; p++) {
+        const int k = d->perm[p];
+        const int4 v4 = d->hnb[p];
+        const int v[3] = {v4.x, v4.y, v4.z};
+        for (int i = 0; i < 3; i++) {
+            int owner = 3 * k + i;
+            if (v[i] >= 0) {
+                const int n = d->perm[v[i] >> 2];
+                if (n < k) owner = 3 * n + (v[i] & 3);
+            }
+            hr[(size_t)i * NN + p] = rank[owner];
+        }
+    }
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    drop_graph(d);
+    if (d->rw) HIPCHK(d, hipFree(d->rw));
+    if (d->rwd) HIPCHK(d, hipFree(d->rwd));
+    d->rw = nullptr;
+    d->rwd = nullptr;
+    if (R == 0) return ANUGA_OK;
+    HIPCHK(d, dalloc(&d->rw, 3 * NN));
+    HIPCHK(d, dalloc(&d->rwd, (size_t)R));
+    HIPCHK(d, hipMemcpy(d->rw, hr.data(), 3 * NN * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(d, hipMemcpy(d->rwd, rec.data(), R * sizeof(DeRiverwall), hipMemcpyHostToDevice));
+    return ANUGA_OK;
 }
 
 int anuga_distribute_to_vertices_and_edges(anuga_domain *d, double *mass_error)
@@ -1571,6 +1656,7 @@ int anuga_adjoint_record(anuga_domain *d, int64_t nsteps, double dt, int64_t che
     int rc = single(d);
     if (rc) return rc;
     if (d->mixed) return fail(d, ANUGA_E_ARG, "gradients need the FP64 path");
+    if (d->rw) return fail(d, ANUGA_E_ARG, "gradients of runs with riverwalls are not supported");
     if (d->P.timestep_fluxcalls != 2) return fail(d, ANUGA_E_ARG, "gradients need rk2 (DE1)");
     if (nsteps <= 0 || checkpoint_every <= 0 || obs_every <= 0 || !(dt > 0.0) || ngauges < 0 ||
         (ngauges > 0 && (!gauge_triangles || !gauge_stage)))

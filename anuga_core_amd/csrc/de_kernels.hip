@@ -386,12 +386,65 @@ __device__ __forceinline__ DeQ de_rk_combine(const DeQ &r, const DeQ &b)
 }
 
 // ---------------------------------------------------------------------------
+// riverwalls
+// ---------------------------------------------------------------------------
+// Blend the edge flux (normal frame, per unit length) towards the
+// Villemonte-corrected weir discharge Qfactor 2/3 H sqrt(2/3 g H) (1 - Q2/Q1)^0.385,
+// returning to the shallow-water flux between submergence s1 and s2 or
+// tailwater-head / weir-height h1 and h2; the momentum fluxes take the mass
+// flux's scaling (at most 10). sw_domain_orig.c:671-763, operation for operation.
+__device__ __forceinline__ void de_weir_adjust(double ef[3], double hl, double hr, double g,
+                                               double weir_height, const DeRiverwall &rw,
+                                               double &speed)
+{
+    const double two3 = 2.0 / 3.0;
+    const double minhd = fmin(hl, hr), maxhd = fmax(hl, hr);
+    const double q1 = rw.qfactor * two3 * maxhd * sqrt(two3 * g * maxhd);
+    const double q2 = rw.qfactor * two3 * minhd * sqrt(two3 * g * minhd);
+    const double q_ratio = q2 / fmax(q1, 1.0e-100);
+    const double h_ratio = minhd / fmax(maxhd, 1.0e-100);
+    const double head_ratio = minhd / fmax(weir_height, 1.0e-100);
+    double q = q1 * de_pow_0385(1.0 - q_ratio);
+    if (hr > hl) q *= -1.0;
+    if ((h_ratio < rw.s2) & (head_ratio < rw.h2)) {
+        const double w1 = fmin(fmax(h_ratio - rw.s1, 0.0) / (rw.s2 - rw.s1), 1.0);
+        const double w2 = fmin(fmax(head_ratio - rw.h1, 0.0) / (rw.h2 - rw.h1), 1.0);
+        const double blended = (q * (1.0 - w1) + w1 * ef[0]) * (1.0 - w2) + w2 * ef[0];
+        double scale = (fabs(ef[0]) > 1.0e-100) ? blended / ef[0] : 0.0;
+        scale = fmax(scale, 0.0);
+        ef[0] = blended;
+        ef[1] *= fmin(scale, 10.0);
+        ef[2] *= fmin(scale, 10.0);
+    }
+    if (fabs(ef[0]) > 0.0) speed = sqrt(g * (maxhd + weir_height)) + fabs(ef[0] / (maxhd + 1.0e-12));
+}
+
+// The weir on a riverwall edge: only where the crest stands above both cells'
+// beds and either side's first-order head over z_half is positive
+// (sw_domain_orig.c:905-968). wn, zn: the neighbour's centroid stage and bed
+// (interior edges); on a boundary edge the right head is hc + zr - z_half.
+__device__ __forceinline__ void de_riverwall_weir(const DeRiverwall &rw, double g, bool interior,
+                                                  double wk, double zc, double hc, double wn,
+                                                  double zn, double zl, double zr,
+                                                  double z_half, double ef[3], double &speed)
+{
+    const double weir_height = fmax(rw.elev - fmin(zl, zr), 0.0);
+    if (!(rw.elev > fmax(zc, zn))) return;
+    const double hl = fmax(wk - z_half, 0.0);
+    const double hr = interior ? fmax(wn - z_half, 0.0) : fmax(hc + zr - z_half, 0.0);
+    if (!((hl > 0.0) || (hr > 0.0))) return;
+    de_weir_adjust(ef, hl, hr, g, weir_height, rw, speed);
+}
+
+// ---------------------------------------------------------------------------
 // flux
 // ---------------------------------------------------------------------------
 // SUBSTEP 0: writes explicit updates; SUBSTEP > 0: advances with the
 // scenario's dt and writes de_rk_combine<SUBSTEP>(advanced, bk) to out (out
 // may be bk, or q itself: each thread reads its own q and bk before writing).
-template <class T, int SUBSTEP>
+// WALLS: the mesh has riverwalls (a separate instantiation, so that the
+// common kernel carries none of their registers).
+template <class T, int SUBSTEP, bool WALLS = false>
 __global__ void __launch_bounds__(DE_BLOCK)
 de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
         const T *__restrict__ E, DeBoundary bc, const unsigned char *__restrict__ flags,
@@ -461,12 +514,32 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
                 zr = zl;
                 hre = fmax(qr0 - zr, T(0.0));
             }
-            const T z_half = fmax(zl, zr);
+            T z_half = fmax(zl, zr);
+            int wall = -1;
+            if constexpr (WALLS) {
+                // the wall is a thin bed step between the two half-edges
+                // (sw_domain_orig.c:880-893)
+                wall = m.rw[i * N + k];
+                if (wall >= 0) z_half = fmax(m.rwd[wall].elev, z_half);
+            }
             const T h_left = fmax(hle + zl - z_half, T(0.0));
             const T h_right = fmax(hre + zr - z_half, T(0.0));
             T ef[3], speed, pflux;
             de_flux_central<T>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1, n2,
                                eps, z_half, g, ef, speed, pflux);
+            if constexpr (WALLS) {
+                if (wall >= 0) {
+                    double wn = 0.0, zn = c.z;
+                    if (v[i] >= 0) {
+                        const int n = v[i] >> 2;
+                        const DeCentroid cn = de_centroid(q[n], flags[n]);
+                        wn = cn.w;
+                        zn = cn.z;
+                    }
+                    de_riverwall_weir(m.rwd[wall], p.g, v[i] >= 0, c.w, c.z, hc, wn, zn, zl, zr,
+                                      z_half, ef, speed);
+                }
+            }
             const T length = m.len[i * N + k];
             ef[0] *= length;
             ef[1] *= length;

@@ -138,6 +138,14 @@ class Domain:
         self._create_device()
         self.set_flow_algorithm("DE0")
 
+        # riverwalls (shallow_water_domain.py:366-379): edge_flux_type 1 on a
+        # wall half-edge; the data lives in riverwallData
+        from .riverwall import RiverWall
+        self.edge_flux_type = np.zeros(3 * N, np.int64)
+        self.number_of_riverwall_edges = 0
+        self.riverwallData = RiverWall(self)
+        self.create_riverwalls = self.riverwallData.create_riverwalls
+
     # -- device handle -----------------------------------------------------------
     def _create_device(self):
         L = _native.lib()
@@ -227,6 +235,25 @@ class Domain:
         self.beta_w_dry = self.beta_uh_dry = self.beta_vh_dry = bd
         self.default_order = 2
         self.maximum_allowed_speed = 0.0
+
+    def get_using_discontinuous_elevation(self):
+        return self.flow_algorithm.startswith("DE")
+
+    def _set_riverwalls(self, edge_flux_type, riverwall_elevation, riverwall_rowIndex,
+                        hydraulic_properties):
+        """Upload riverwall data in the reference's layout (see
+        anuga_core_amd.riverwall)."""
+        eft = np.ascontiguousarray(edge_flux_type, np.int64).ravel()
+        elev = np.ascontiguousarray(riverwall_elevation, np.float64)
+        rows = np.ascontiguousarray(riverwall_rowIndex, np.int64)
+        hp = np.ascontiguousarray(np.atleast_2d(hydraulic_properties), np.float64)
+        if self.precision != "fp64" and len(elev) > 0:
+            raise Exception("riverwalls need precision='fp64'")
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
+        self._call("anuga_set_riverwalls", ip(eft), len(elev), _dptr(elev), ip(rows),
+                   hp.shape[0], hp.shape[1], _dptr(hp))
+        self.edge_flux_type = eft.copy()
+        self.number_of_riverwall_edges = len(elev)
 
     def get_flow_algorithm(self):
         return self.flow_algorithm

@@ -113,6 +113,23 @@ int anuga_get_edge_values(anuga_domain *d, int quantity, double *values);
  * boundary state is evaluated on the device inside the flux kernel. */
 int anuga_set_boundary(anuga_domain *d, const int32_t *kind, const double *dirichlet_values);
 
+/* Riverwalls (FP64 path), in the layout the reference's flux loop reads
+ * (struct domain fields edge_flux_type, riverwall_elevation,
+ * riverwall_rowIndex, riverwall_hydraulic_properties, sw_domain.h, filled by
+ * RiverWall.create_riverwalls, anuga/structures/riverwall.py:382-469):
+ * edge_flux_type (N,3) in the caller's order, 1 on a wall half-edge; the
+ * elevation and hydraulic-table row of each of the number_of_riverwall_edges
+ * flagged half-edges in (triangle, edge) order; the table (nrows, ncols >= 5)
+ * of Qfactor, s1, s2, h1, h2. Replaces the riverwall branch of
+ * _compute_fluxes_central (sw_domain_orig.c:828-837, :880-968) and
+ * adjust_edgeflux_with_weir (:671-763); an interior edge is a wall iff the
+ * half-edge of its lower-numbered triangle is flagged, as in that loop.
+ * number_of_riverwall_edges 0 removes the walls. */
+int anuga_set_riverwalls(anuga_domain *d, const int64_t *edge_flux_type,
+                         int64_t number_of_riverwall_edges, const double *riverwall_elevation,
+                         const int64_t *riverwall_rowIndex, int64_t nrows, int64_t ncols,
+                         const double *hydraulic_properties);
+
 /* protect + second-order edge extrapolation, with the result written back to
  * the centroid state. Replaces protect_new + extrapolate_second_order_edge_sw
  * (sw_domain_orig_ext.pyx:449-475; sw_domain_orig.c:1215-1281, :1994-2680).

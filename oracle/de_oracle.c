@@ -11,9 +11,14 @@
  *   oracle_rotate               anuga/shallow_water/sw_domain_orig.c:37-57
  *   oracle_flux_function_central sw_domain_orig.c:284-505
  *   oracle_compute_fluxes       sw_domain_orig.c:765-1150 (edge computed once,
- *                               mirrored to the neighbour; riverwalls absent)
+ *                               mirrored to the neighbour; riverwall edges
+ *                               counted in loop order as at :828-837, :880-893)
  *   oracle_compute_fluxes_gather same arithmetic, each triangle evaluating its
- *                               own three edges (OpenMP-parallel, deterministic)
+ *                               own three edges (OpenMP-parallel, deterministic;
+ *                               riverwall index from edge_river_wall_counter as
+ *                               in sw_domain_openacc.c:586)
+ *   oracle_adjust_edgeflux_with_weir sw_domain_orig.c:671-763 (with the call
+ *                               site's conditions, :905-968)
  *   oracle_protect              sw_domain_orig.c:1215-1281
  *   oracle_extrapolate          sw_domain_orig.c:1994-2680 (+ :1286-1333)
  *   oracle_manning_friction_flat sw_domain_orig.c:2805-2838
@@ -35,26 +40,29 @@
 #include <string.h>
 
 /*
- * pow(h, 7/3) of Manning friction. Mode 0 calls the C library's pow exactly as
- * the reference does. Mode 1 returns the correctly rounded value: glibc's pow
- * is within 0.52 ulp, so its result is the correctly rounded one unless the
- * exact value is within 0.02 ulp of a rounding boundary; that is tested with
- * an 80-bit powl residual, and such rare cases are settled with 113-bit powq.
+ * The reference's two library pow calls: pow(h, 7/3) of Manning friction and
+ * pow(1 - Q2/Q1, 0.385) of the weir. Mode 0 calls the C library's pow exactly
+ * as the reference does. Mode 1 returns the correctly rounded value: glibc's
+ * pow is within 0.52 ulp, so its result is the correctly rounded one unless
+ * the exact value is within 0.02 ulp of a rounding boundary; that is tested
+ * with an 80-bit powl residual, and such rare cases are settled with 113-bit
+ * powq.
  */
 static int g_pow_mode = 0;
 
 void oracle_set_pow_mode(int mode) { g_pow_mode = mode; }
 
-double oracle_pow_7_3(double h)
+double oracle_pow(double x, double y)
 {
-    const double y = 7.0 / 3.0;
-    double p = pow(h, y);
+    double p = pow(x, y);
     long double e;
-    if (g_pow_mode == 0) return p;
-    e = powl((long double)h, (long double)y) - (long double)p;
+    if (g_pow_mode == 0 || p == 0.0) return p;
+    e = powl((long double)x, (long double)y) - (long double)p;
     if (fabsl(e) < 0.49L * (long double)(nextafter(p, INFINITY) - p)) return p;
-    return (double)powq((__float128)h, (__float128)y);
+    return (double)powq((__float128)x, (__float128)y);
 }
+
+double oracle_pow_7_3(double h) { return oracle_pow(h, 7.0 / 3.0); }
 
 typedef struct {
     int64_t N;                       /* triangles */
@@ -70,6 +78,16 @@ typedef struct {
     const double *areas;             /* (N) */
     const double *centroids;         /* (N,2) */
     const double *edge_midpoints;    /* (N,6) */
+    /* riverwalls (edge_flux_type NULL: none). Flagged half-edges are ranked
+     * in (triangle, edge) order; rank r (0-based) owns riverwall_elevation[r]
+     * and hydraulic-properties row riverwall_rowIndex[r] (columns Qfactor, s1,
+     * s2, h1, h2). edge_river_wall_counter holds rank + 1 (0: not a wall). */
+    const int64_t *edge_flux_type;          /* (N,3) 1 = riverwall */
+    const int64_t *edge_river_wall_counter; /* (N,3) */
+    const double *riverwall_elevation;      /* (R) */
+    const int64_t *riverwall_rowIndex;      /* (R) */
+    int64_t ncol_riverwall_hydraulic_properties;
+    const double *riverwall_hydraulic_properties;
 } oracle_mesh;
 
 typedef struct {
@@ -192,6 +210,67 @@ void oracle_flux_function_central(const double *ql, const double *qr,
     oracle_rotate(edgeflux, n1, -n2);
 }
 
+/*
+ * Blend the edge flux towards the Villemonte-corrected broad-crested weir
+ * discharge Q = Qfactor 2/3 H sqrt(2/3 g H) (1 - Q2/Q1)^0.385, back to the
+ * shallow-water flux as the submergence ratio goes from s1 to s2 or the
+ * tailwater head over the weir height from h1 to h2; momentum fluxes follow
+ * the mass flux's scaling (capped at 10). Sets the edge's wave speed from the
+ * weir head when the mass flux is nonzero.
+ */
+void oracle_adjust_edgeflux_with_weir(double *ef, double h_left, double h_right,
+                                      double g, double weir_height, double Qfactor,
+                                      double s1, double s2, double h1, double h2,
+                                      double *max_speed)
+{
+    const double two3 = 2.0 / 3.0;
+    double minhd = fmin(h_left, h_right);
+    double maxhd = fmax(h_left, h_right);
+    double q1 = Qfactor * two3 * maxhd * sqrt(two3 * g * maxhd);
+    double q2 = Qfactor * two3 * minhd * sqrt(two3 * g * minhd);
+    double q_ratio = q2 / fmax(q1, 1.0e-100);
+    double h_ratio = minhd / fmax(maxhd, 1.0e-100);
+    double head_ratio = minhd / fmax(weir_height, 1.0e-100);
+    double q = q1 * oracle_pow(1.0 - q_ratio, 0.385);
+    if (h_right > h_left) q *= -1.0;
+    if ((h_ratio < s2) & (head_ratio < h2)) {
+        double w1 = fmin(fmax(h_ratio - s1, 0.0) / (s2 - s1), 1.0);
+        double w2 = fmin(fmax(head_ratio - h1, 0.0) / (h2 - h1), 1.0);
+        double blended = (q * (1.0 - w1) + w1 * ef[0]) * (1.0 - w2) + w2 * ef[0];
+        double scale = (fabs(ef[0]) > 1.0e-100) ? blended / ef[0] : 0.0;
+        scale = fmax(scale, 0.0);
+        ef[0] = blended;
+        ef[1] *= fmin(scale, 10.0);
+        ef[2] *= fmin(scale, 10.0);
+    }
+    if (fabs(ef[0]) > 0.0)
+        *max_speed = sqrt(g * (maxhd + weir_height)) + fabs(ef[0] / (maxhd + 1.0e-12));
+}
+
+/*
+ * The weir on riverwall rank r of edge (k, n): only where the wall stands above
+ * both cells' beds and either side's first-order head over z_half is positive
+ * (sw_domain_orig.c:905-968).
+ */
+static void riverwall_weir(const oracle_mesh *m, const oracle_params *p, const oracle_state *s,
+                           int64_t k, int64_t n, int64_t r, double zl, double zr,
+                           double zc, double zc_n, double hc_n, double z_half,
+                           double *ef, double *speed)
+{
+    const double elev = m->riverwall_elevation[r];
+    const double weir_height = fmax(elev - fmin(zl, zr), 0.0);
+    const double *hp;
+    double hl, hr;
+    if (!(elev > fmax(zc, zc_n))) return;
+    hl = fmax(s->w[k] - z_half, 0.0);
+    hr = (n >= 0) ? fmax(s->w[n] - z_half, 0.0) : fmax(hc_n + zr - z_half, 0.0);
+    if (!((hl > 0.0) || (hr > 0.0))) return;
+    hp = m->riverwall_hydraulic_properties +
+         m->riverwall_rowIndex[r] * m->ncol_riverwall_hydraulic_properties;
+    oracle_adjust_edgeflux_with_weir(ef, hl, hr, p->g, weir_height, hp[0], hp[1], hp[2],
+                                     hp[3], hp[4], speed);
+}
+
 /* Right-hand state of edge (k,i): neighbour's edge values or the boundary array. */
 static void right_state(const oracle_mesh *m, const oracle_state *s, int64_t k,
                         int i, double zl, double *qr, double *zr, double *hre)
@@ -228,7 +307,7 @@ double oracle_compute_fluxes(const oracle_mesh *m, const oracle_params *p,
 {
     const int64_t N = m->N;
     double local_timestep = 1.0e+100;
-    int64_t k, n, nm = 0, ki;
+    int64_t k, n, nm = 0, ki, rw_count = 0;
     int i;
 
     memset(s->ex_w, 0, N * sizeof(double));
@@ -241,8 +320,12 @@ double oracle_compute_fluxes(const oracle_mesh *m, const oracle_params *p,
         for (i = 0; i < 3; i++) {
             double ql[3], qr[3], ef[3], zl, zr, hle, hre, hc, zc, hc_n, zc_n;
             double z_half, h_left, h_right, speed, pflux, length, bsw, tmp;
+            int is_rw;
             ki = 3 * k + i;
-            if (done[ki]) continue;
+            if (done[ki]) {
+                if (m->edge_flux_type && m->edge_flux_type[ki] == 1) rw_count++;
+                continue;
+            }
             ql[0] = s->w_e[ki];
             ql[1] = s->uh_e[ki];
             ql[2] = s->vh_e[ki];
@@ -260,12 +343,20 @@ double oracle_compute_fluxes(const oracle_mesh *m, const oracle_params *p,
                 nm = 3 * n + m->neighbour_edges[ki];
             }
             z_half = fmax(zl, zr);
+            is_rw = m->edge_flux_type && m->edge_flux_type[ki] == 1;
+            if (is_rw) {
+                rw_count++;
+                z_half = fmax(m->riverwall_elevation[rw_count - 1], z_half);
+            }
             h_left = fmax(hle + zl - z_half, 0.0);
             h_right = fmax(hre + zr - z_half, 0.0);
             oracle_flux_function_central(ql, qr, h_left, h_right, hle, hre,
                                          m->normals[2 * ki], m->normals[2 * ki + 1],
                                          p->epsilon, z_half, p->g, ef, &speed,
                                          &pflux, p->low_froude);
+            if (is_rw)
+                riverwall_weir(m, p, s, k, n, rw_count - 1, zl, zr, zc, zc_n, hc_n, z_half,
+                               ef, &speed);
             length = m->edgelengths[ki];
             ef[0] *= length;
             ef[1] *= length;
@@ -352,6 +443,7 @@ double oracle_compute_fluxes_gather(const oracle_mesh *m, const oracle_params *p
             int64_t n = m->neighbours[ki];
             double ql[3], qr[3], ef[3], zl, zr, hle, hre, z_half, h_left, h_right;
             double speed, pflux, length, pg, tmp;
+            int64_t rw = m->edge_flux_type ? m->edge_river_wall_counter[ki] - 1 : -1;
             ql[0] = s->w_e[ki];
             ql[1] = s->uh_e[ki];
             ql[2] = s->vh_e[ki];
@@ -359,12 +451,16 @@ double oracle_compute_fluxes_gather(const oracle_mesh *m, const oracle_params *p
             hle = s->h_e[ki];
             right_state(m, s, k, i, zl, qr, &zr, &hre);
             z_half = fmax(zl, zr);
+            if (rw >= 0) z_half = fmax(m->riverwall_elevation[rw], z_half);
             h_left = fmax(hle + zl - z_half, 0.0);
             h_right = fmax(hre + zr - z_half, 0.0);
             oracle_flux_function_central(ql, qr, h_left, h_right, hle, hre,
                                          m->normals[2 * ki], m->normals[2 * ki + 1],
                                          p->epsilon, z_half, p->g, ef, &speed,
                                          &pflux, p->low_froude);
+            if (rw >= 0)
+                riverwall_weir(m, p, s, k, n, rw, zl, zr, zc, (n >= 0) ? s->z[n] : zc,
+                               (n >= 0) ? s->h[n] : hc, z_half, ef, &speed);
             length = m->edgelengths[ki];
             ef[0] *= length;
             ef[1] *= length;

@@ -55,7 +55,11 @@ class _OMesh(C.Structure):
                 ("neighbours", _ip), ("neighbour_edges", _ip), ("surrogate", _ip),
                 ("nboundaries", _ip), ("tri_full_flag", _ip),
                 ("normals", _dp), ("edgelengths", _dp), ("radii", _dp), ("areas", _dp),
-                ("centroids", _dp), ("edge_midpoints", _dp)]
+                ("centroids", _dp), ("edge_midpoints", _dp),
+                ("edge_flux_type", _ip), ("edge_river_wall_counter", _ip),
+                ("riverwall_elevation", _dp), ("riverwall_rowIndex", _ip),
+                ("ncol_riverwall_hydraulic_properties", C.c_int64),
+                ("riverwall_hydraulic_properties", _dp)]
 
 
 class _OParams(C.Structure):
@@ -104,6 +108,9 @@ class OracleKernels:
         lib.oracle_set_pow_mode.argtypes = [C.c_int]
         lib.oracle_pow_7_3.restype = C.c_double
         lib.oracle_pow_7_3.argtypes = [C.c_double]
+        lib.oracle_pow.restype = C.c_double
+        lib.oracle_pow.argtypes = [C.c_double, C.c_double]
+        lib.oracle_adjust_edgeflux_with_weir.argtypes = [_dp] + [C.c_double] * 9 + [_dp]
         lib.oracle_protect.restype = C.c_double
         lib.oracle_extrapolate.restype = C.c_int64
         lib.oracle_compute_fluxes.restype = C.c_double
@@ -130,10 +137,15 @@ class OracleKernels:
         m = d.mesh
         self._keep = [d.neighbours, d.neighbour_edges, d.surrogate_neighbours,
                       d.number_of_boundaries, d.tri_full_flag]
+        rw = d.number_of_riverwall_edges > 0
         self.mesh = _OMesh(d.N, d.M, _i(d.neighbours), _i(d.neighbour_edges),
                            _i(d.surrogate_neighbours), _i(d.number_of_boundaries),
                            _i(d.tri_full_flag), _d(d.normals), _d(d.edgelengths),
-                           _d(d.radii), _d(d.areas), _d(d.centroids), _d(d.edge_midpoints))
+                           _d(d.radii), _d(d.areas), _d(d.centroids), _d(d.edge_midpoints),
+                           _i(d.edge_flux_type) if rw else None,
+                           _i(d.edge_river_wall_counter), _d(d.riverwall_elevation),
+                           _i(d.riverwall_rowIndex), d.ncol_riverwall_hydraulic_properties,
+                           _d(d.riverwall_hydraulic_properties))
         self.state = _OState(**{k: _d(v) for k, v in _state_arrays(d).items()})
         self._edge_flux_work = np.zeros(9 * d.N)
         self._pressuregrad_work = np.zeros(3 * d.N)
@@ -157,6 +169,7 @@ class OracleKernels:
     def compute_fluxes(self, substep, timestep):
         p = self._params()
         d = self.d
+        self.lib.oracle_set_pow_mode(self.pow_mode)     # the weir's pow
         if self.gather:
             return self.lib.oracle_compute_fluxes_gather(
                 C.byref(self.mesh), C.byref(p), C.byref(self.state), substep, timestep,
@@ -215,6 +228,15 @@ class OracleKernels:
         n2 = -normal[1] if direction == -1 else normal[1]
         self.lib.oracle_rotate(_d(r), float(normal[0]), float(n2))
         return r
+
+    def adjust_edgeflux_with_weir(self, edgeflux, h_left, h_right, g, weir_height,
+                                  Qfactor, s1, s2, h1, h2, max_speed):
+        ef = np.array(edgeflux, dtype=np.float64)
+        ms = np.array([max_speed], dtype=np.float64)
+        self.lib.oracle_set_pow_mode(self.pow_mode)
+        self.lib.oracle_adjust_edgeflux_with_weir(_d(ef), h_left, h_right, g, weir_height,
+                                                  Qfactor, s1, s2, h1, h2, _d(ms))
+        return ef, ms[0]
 
 
 # ---------------------------------------------------------------------------
@@ -283,6 +305,8 @@ class RefKernels:
                                               _dp, _dp, _dp, _dp, _dp]
         sw._orig_fix_negative_cells.restype = C.c_int64
         sw._orig_fix_negative_cells.argtypes = [C.c_void_p]
+        sw.adjust_edgeflux_with_weir.restype = C.c_double
+        sw.adjust_edgeflux_with_weir.argtypes = [_dp] + [C.c_double] * 9 + [_dp]
         sw._flux_function_central.restype = C.c_int64
         sw._flux_function_central.argtypes = [
             _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -301,7 +325,6 @@ class RefKernels:
         self.d = d
         N, M = d.N, d.M
         self._buf = dict(
-            edge_flux_type=np.zeros(3 * N, np.int64),
             already_computed_flux=np.zeros(3 * N, np.int64),
             flux_update_frequency=np.ones(3 * N, np.int64),
             update_next_flux=np.ones(3 * N, np.int64),
@@ -309,17 +332,14 @@ class RefKernels:
             edge_flux_work=np.zeros(9 * N), neigh_work=np.zeros(9 * N),
             pressuregrad_work=np.zeros(3 * N),
             allow_timestep_increase=np.ones(1, np.int64),
-            edge_river_wall_counter=np.zeros(3 * N, np.int64),
-            riverwall_elevation=np.zeros(1), riverwall_rowIndex=np.zeros(1, np.int64),
-            riverwall_hydraulic_properties=np.zeros(5),
             bed_boundary_values=np.zeros(max(M, 1)),
         )
         b = self._buf
         D = _RefDomain()
         D.number_of_elements = N
         D.boundary_length = M
-        D.number_of_riverwall_edges = 0
-        D.ncol_riverwall_hydraulic_properties = 5
+        D.number_of_riverwall_edges = d.number_of_riverwall_edges
+        D.ncol_riverwall_hydraulic_properties = d.ncol_riverwall_hydraulic_properties
         D.max_flux_update_frequency = 1
         D.neighbours = _i(d.neighbours)
         D.neighbour_edges = _i(d.neighbour_edges)
@@ -328,7 +348,7 @@ class RefKernels:
         D.edgelengths = _d(d.edgelengths)
         D.radii = _d(d.radii)
         D.areas = _d(d.areas)
-        D.edge_flux_type = _i(b["edge_flux_type"])
+        D.edge_flux_type = _i(d.edge_flux_type)
         D.tri_full_flag = _i(d.tri_full_flag)
         D.already_computed_flux = _i(b["already_computed_flux"])
         D.max_speed = _d(d.max_speed)
@@ -369,10 +389,10 @@ class RefKernels:
         D.y_centroid_work = _d(d.y_centroid_work)
         D.boundary_flux_sum = _d(d.boundary_flux_sum)
         D.allow_timestep_increase = _i(b["allow_timestep_increase"])
-        D.edge_river_wall_counter = _i(b["edge_river_wall_counter"])
-        D.riverwall_elevation = _d(b["riverwall_elevation"])
-        D.riverwall_rowIndex = _i(b["riverwall_rowIndex"])
-        D.riverwall_hydraulic_properties = _d(b["riverwall_hydraulic_properties"])
+        D.edge_river_wall_counter = _i(d.edge_river_wall_counter)
+        D.riverwall_elevation = _d(d.riverwall_elevation)
+        D.riverwall_rowIndex = _i(d.riverwall_rowIndex)
+        D.riverwall_hydraulic_properties = _d(d.riverwall_hydraulic_properties)
         D.stage_semi_implicit_update = _d(d.stage_si)
         D.xmom_semi_implicit_update = _d(d.xmom_si)
         D.ymom_semi_implicit_update = _d(d.ymom_si)
@@ -462,3 +482,11 @@ class RefKernels:
         n2 = -normal[1] if direction == -1 else normal[1]
         self.sw._rotate(_d(r), float(normal[0]), float(n2))
         return r
+
+    def adjust_edgeflux_with_weir(self, edgeflux, h_left, h_right, g, weir_height,
+                                  Qfactor, s1, s2, h1, h2, max_speed):
+        ef = np.array(edgeflux, dtype=np.float64)
+        ms = np.array([max_speed], dtype=np.float64)
+        self.sw.adjust_edgeflux_with_weir(_d(ef), h_left, h_right, g, weir_height, Qfactor,
+                                          s1, s2, h1, h2, _d(ms))
+        return ef, ms[0]

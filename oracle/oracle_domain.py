@@ -18,6 +18,9 @@ Restates (paths relative to the anuga_core tree):
   * update_conserved_quantities shallow_water_domain.py:2089-2163
   * boundary flux integral anuga/operators/boundary_flux_integral_operator.py
   * get_water_volume      shallow_water_domain.py:1626-1653
+  * riverwall edge data   anuga/structures/riverwall.py:382-469 (the layout the
+                          flux loop reads: edge_flux_type, per-rank elevation and
+                          hydraulic row, edge_river_wall_counter)
 Boundary objects are duck-typed: `.kind` in {'reflective', 'transmissive',
 'dirichlet'} and, for Dirichlet, `.dirichlet_values`.
 """
@@ -66,6 +69,7 @@ class OracleDomain:
         self.edge_timestep = z(3 * N) + 1.0e100
         self.boundary_flux_sum = z(3)
         self.boundary_flux_integral = 0.0
+        self._no_riverwalls()
 
         self.g = G
         self.epsilon = EPSILON
@@ -90,6 +94,38 @@ class OracleDomain:
         self.dt_history = []
         self.mass_error = 0.0
         self.negative_cells = 0
+
+    # -- riverwalls -----------------------------------------------------------
+    def _no_riverwalls(self):
+        N = self.N
+        self.number_of_riverwall_edges = 0
+        self.edge_flux_type = np.zeros(3 * N, np.int64)
+        self.edge_river_wall_counter = np.zeros(3 * N, np.int64)
+        self.riverwall_elevation = np.zeros(1)
+        self.riverwall_rowIndex = np.zeros(1, np.int64)
+        self.ncol_riverwall_hydraulic_properties = 5
+        self.riverwall_hydraulic_properties = np.zeros(5)
+
+    def set_riverwalls(self, edge_flux_type, riverwall_elevation, riverwall_rowIndex,
+                       hydraulic_properties):
+        """Riverwall data in the reference's layout: edge_flux_type (N*3, 1 on
+        every half-edge of a wall), elevation and hydraulic-properties row per
+        flagged half-edge in (triangle, edge) order, the hydraulic table
+        (rows of Qfactor, s1, s2, h1, h2)."""
+        eft = np.ascontiguousarray(edge_flux_type, np.int64).ravel()
+        flagged = eft == 1
+        R = int(flagged.sum())
+        assert eft.shape == (3 * self.N,)
+        assert len(riverwall_elevation) == R and len(riverwall_rowIndex) == R
+        hp = np.ascontiguousarray(hydraulic_properties, np.float64)
+        self.number_of_riverwall_edges = R
+        self.edge_flux_type = eft
+        self.edge_river_wall_counter = np.where(flagged, np.cumsum(flagged), 0).astype(np.int64)
+        self.riverwall_elevation = np.ascontiguousarray(riverwall_elevation, np.float64)
+        self.riverwall_rowIndex = np.ascontiguousarray(riverwall_rowIndex, np.int64)
+        self.ncol_riverwall_hydraulic_properties = hp.shape[1]
+        self.riverwall_hydraulic_properties = hp.ravel().copy()
+        self.kernels.bind(self)
 
     # -- presets ------------------------------------------------------------
     # shallow_water_domain.py:583-895: CFL, timestepping, minimum_allowed_height,

@@ -86,3 +86,68 @@ def setup(domain, alg, q, bc, betas=None):
 def rel_maxnorm(a, b):
     den = np.max(np.abs(b))
     return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
+
+
+def riverwall_structure(m=10, n=10, wall_height=2.0, ocean=2.5, land=-999999.0,
+                        qfactor=1.0, alg="DE0"):
+    """The reference's riverwall test set-up (anuga/structures/tests/
+    test_riverwall_structure.py:47-150) on a rectangular_cross mesh of the
+    100 m square (its edges include x = 50): bed -x/150, a wall along x = 50
+    of crest `wall_height`, stage `ocean` for x >= 50 and `land` west of it,
+    friction 0.03, reflective all round."""
+    p, v, b = rectangular_cross(m, n, 100.0, 100.0)
+    mesh = Mesh(p, v, b)
+    x = mesh.centroid_coordinates[:, 0]
+    q = dict(elevation=-x / 150.0, friction=np.full(mesh.number_of_triangles, 0.03),
+             stage=ocean * (x >= 50.0) + land * (x < 50.0))
+    walls = {"centralWall": [[50.0, 0.0, wall_height], [50.0, 100.0, wall_height]]}
+    par = {"centralWall": {"Qfactor": qfactor}}
+    bc = {t: "reflective" for t in ("left", "right", "top", "bottom")}
+    return mesh, alg, q, bc, walls, par
+
+
+def riverwall_levees(m=16, n=12, alg="DE1", seed=3):
+    """Two walls on a sloping 160 m x 120 m valley with hills: a sloped levee
+    along x = 80 (crest 1.2 m to 2.2 m, Qfactor 1.3, s1 0.8) and a low
+    L-shaped wall (crest 0.9 m) from (20, 0) to (20, 60) to (60, 60); a
+    1.5 m pond in the east, 0.4 m of water in the north-west, dry elsewhere;
+    Dirichlet inflow on the right, transmissive left. Walls are overtopped
+    freely and submerged in different places."""
+    p, v, b = rectangular_cross(m, n, 160.0, 120.0)
+    mesh = Mesh(p, v, b)
+    rng = np.random.default_rng(seed)
+    x, y = mesh.centroid_coordinates.T
+    z = 0.004 * x + 0.3 * np.exp(-((x - 50) ** 2 + (y - 90) ** 2) / 300.0) \
+        + 0.01 * rng.random(len(x))
+    w = np.where(x > 80.0, 1.5, np.where((x < 20.0) & (y > 60.0), 0.4, z))
+    w = np.maximum(w, z)
+    q = dict(elevation=z, stage=w, friction=0.02 + 0.02 * rng.random(len(x)))
+    walls = {"levee": [[80.0, 0.0, 1.2], [80.0, 120.0, 2.2]],
+             "lwall": [[20.0, 0.0, 0.9], [20.0, 60.0, 0.9], [60.0, 60.0, 0.9]]}
+    par = {"levee": {"Qfactor": 1.3, "s1": 0.8}}
+    bc = {"left": "transmissive", "right": ("dirichlet", [1.7, 0.0, 0.0]),
+          "top": "reflective", "bottom": "reflective"}
+    return mesh, alg, q, bc, walls, par
+
+
+class _WallMesh:
+    """Minimal domain for RiverWall: records the uploaded arrays."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        self.uploaded = None
+
+    def get_using_discontinuous_elevation(self):
+        return True
+
+    def _set_riverwalls(self, *arrays):
+        self.uploaded = arrays
+
+
+def riverwall_arrays(mesh, walls, par, **kw):
+    """(edge_flux_type, riverwall_elevation, riverwall_rowIndex,
+    hydraulic_properties) of RiverWall.create_riverwalls on this mesh."""
+    from anuga_core_amd.riverwall import RiverWall
+    dom = _WallMesh(mesh)
+    RiverWall(dom).create_riverwalls(walls, par, verbose=False, **kw)
+    return dom.uploaded

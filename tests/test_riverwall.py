@@ -1,0 +1,309 @@
+"""Riverwalls on the CPU: the host set-up (anuga_core_amd/riverwall.py) and the
+oracle's riverwall branch and weir (oracle/de_oracle.c).
+
+* set-up: flagged edges, interpolated crest elevations, hydraulic table and
+  the reference's input errors (anuga/structures/riverwall.py:151-480);
+* weir: closed forms of the weir law (free overflow, Villemonte submergence,
+  blending, sign, wave speed) and, bit for bit, the reference's compiled
+  adjust_edgeflux_with_weir;
+* flux loop: whole runs with walls bit for bit against the reference's
+  compiled _compute_fluxes_central (both oracle loops), including a wall
+  flagged on one side only;
+* the reference's riverwall tests (test_riverwall_structure.py:152-500):
+  no flow below the crest, free overflow and submerged discharge against
+  theory, Qfactor scaling, on the oracle.
+"""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+from anuga_core_amd.mesh import Mesh, rectangular_cross
+from anuga_core_amd.riverwall import RiverWall
+from oracle.kernels import HERE as ORACLE_DIR
+from oracle.kernels import OracleKernels, RefKernels
+from oracle.oracle_domain import G, OracleDomain
+from tests import scenarios as S
+
+HAVE_REF = os.path.exists(os.path.join(ORACLE_DIR, "_ref", "libref_sw.so"))
+need_ref = pytest.mark.skipif(not HAVE_REF, reason="oracle/_ref not built (no reference tree)")
+
+
+# ---------------------------------------------------------------------------
+# set-up
+# ---------------------------------------------------------------------------
+def _square(m=10):
+    p, v, b = rectangular_cross(m, m, 100.0, 100.0)
+    return Mesh(p, v, b)
+
+
+def test_create_riverwalls_flags_interpolates_and_tabulates():
+    mesh = _square()
+    walls = {"w": [[50.0, 0.0, 1.0], [50.0, 100.0, 3.0]]}
+    eft, elev, rows, hp = S.riverwall_arrays(mesh, walls, {"w": {"Qfactor": 0.7}})
+    E = mesh.edge_midpoint_coordinates
+    on = np.abs(E[:, 0] - 50.0) < 1e-9
+    assert np.array_equal(eft == 1, on)
+    assert on.sum() == 20                   # 10 edges of 10 m, both half-edges
+    flagged = np.nonzero(eft)[0]
+    assert np.allclose(elev, 1.0 + 2.0 * E[flagged, 1] / 100.0, rtol=0, atol=1e-14)
+    assert np.array_equal(rows, np.zeros(20))
+    assert np.array_equal(hp, [[0.7, 0.9, 0.95, 1.0, 1.5]])
+
+
+def test_create_riverwalls_later_walls_overwrite_and_defaults():
+    mesh = _square()
+    walls = {"a": [[50.0, 0.0, 1.0], [50.0, 100.0, 1.0]],
+             "b": [[50.0, 40.0, 5.0], [50.0, 60.0, 5.0], [100.0, 60.0, 5.0]]}
+    eft, elev, rows, hp = S.riverwall_arrays(mesh, walls, {"b": {"h1": 2.0, "h2": 3.0}},
+                                             default_riverwallPar={"Qfactor": 1.5})
+    E = mesh.edge_midpoint_coordinates[np.nonzero(eft)[0]]
+    in_b = ((np.abs(E[:, 0] - 50) < 1e-9) & (E[:, 1] > 40) & (E[:, 1] < 60)) | \
+           ((np.abs(E[:, 1] - 60) < 1e-9) & (E[:, 0] > 50))
+    assert np.array_equal(rows, in_b.astype(int))
+    assert np.array_equal(elev, np.where(in_b, 5.0, 1.0))
+    assert np.array_equal(hp, [[1.5, 0.9, 0.95, 1.0, 1.5], [1.5, 0.9, 0.95, 2.0, 3.0]])
+
+
+@pytest.mark.parametrize("par,default,msg", [
+    ({"w": {"s1": 0.95, "s2": 0.9}}, {}, "s1 >= s2"),
+    ({"w": {"h1": 2.0, "h2": 1.0}}, {}, "h1 >= h2"),
+    ({"w": {"s1": -0.1}}, {}, "s1 and s2 must be positive"),
+    ({"x": {"Qfactor": 1.0}}, {}, "no corresponding key"),
+    ({"w": {"Cd": 1.0}}, {}, "not recognised"),
+    ({}, {"Cd": 1.0}, "not recognized"),
+])
+def test_create_riverwalls_rejects_bad_parameters(par, default, msg):
+    walls = {"w": [[50.0, 0.0, 1.0], [50.0, 100.0, 1.0]]}
+    with pytest.raises(Exception, match=msg):
+        S.riverwall_arrays(_square(), walls, par, default_riverwallPar=default)
+
+
+def test_riverwall_connectedness_and_boundary_vertices():
+    mesh = _square()
+    dom = S._WallMesh(mesh)
+    rw = RiverWall(dom)
+    # one continuous wall touching the boundary at both ends
+    rw.create_riverwalls({"w": [[50.0, 0.0, 1.0], [50.0, 100.0, 1.0]]}, verbose=False)
+    info, ok = rw.check_riverwall_connectedness(verbose=True)
+    assert ok and "0 endpoints" in info
+    v1, v2 = rw.get_vertices_corresponding_to_edgeInds(rw.riverwall_edges)
+    V = mesh.vertex_coordinates
+    onb = rw.is_vertex_on_boundary(np.concatenate([v1, v2]))
+    y = V[np.concatenate([v1, v2]), 1]
+    assert np.array_equal(onb == 1, (y == 0.0) | (y == 100.0))
+    # three pieces: four loose ends inside the domain
+    rw.create_riverwalls({"w": [[50.0, 0.0, 1.0], [50.0, 30.0, 1.0]],
+                          "u": [[50.0, 40.0, 1.0], [50.0, 60.0, 1.0]],
+                          "v": [[50.0, 70.0, 1.0], [50.0, 100.0, 1.0]]}, verbose=False)
+    info, ok = rw.check_riverwall_connectedness(verbose=True)
+    assert ok       # each wall on its own is continuous
+    rw.create_riverwalls({"w": [[50.0, 0.0, 1.0], [50.0, 30.0, 1.0], [50.0, 40.0, 1.0],
+                                [50.0, 50.0, 1.0], [50.0, 60.0, 1.0], [50.0, 100.0, 1.0]]},
+                         verbose=False)
+    rw.riverwall_edges = rw.riverwall_edges[np.abs(mesh.edge_midpoint_coordinates[
+        rw.riverwall_edges, 1] - 35.0) > 1.0]       # remove the piece 30..40
+    rw.hydraulic_properties_rowIndex = np.zeros(len(rw.riverwall_edges), np.int64)
+    info, ok = rw.check_riverwall_connectedness(verbose=True)
+    assert ok and "2 endpoints" in info
+    keep = np.abs(mesh.edge_midpoint_coordinates[rw.riverwall_edges, 1] - 55.0) > 1.0
+    rw.riverwall_edges = rw.riverwall_edges[keep]
+    rw.hydraulic_properties_rowIndex = rw.hydraulic_properties_rowIndex[keep]
+    info, ok = rw.check_riverwall_connectedness(verbose=True)
+    assert not ok and "discontinuous" in info
+
+
+def test_export_riverwalls_to_text(tmp_path):
+    mesh = _square()
+    rw = RiverWall(S._WallMesh(mesh))
+    rw.create_riverwalls({"w": [[50.0, 0.0, 1.0], [50.0, 100.0, 3.0]]}, verbose=False,
+                         output_dir=str(tmp_path))
+    lines = (tmp_path / "w.txt").read_text().splitlines()
+    assert lines[1] == str(("Qfactor", "s1", "s2", "h1", "h2"))
+    xyz = np.array([[float(t) for t in ln.split(",")] for ln in lines[5:]])
+    assert xyz.shape == (20, 3)
+    assert np.allclose(xyz[:, 2], 1.0 + 2.0 * xyz[:, 1] / 100.0)
+
+
+# ---------------------------------------------------------------------------
+# the weir
+# ---------------------------------------------------------------------------
+def _weir(k, ef, hl, hr, wh, qf=1.0, s1=0.9, s2=0.95, h1=1.0, h2=1.5, ms=0.0):
+    return k.adjust_edgeflux_with_weir(ef, hl, hr, G, wh, qf, s1, s2, h1, h2, ms)
+
+
+@pytest.mark.parametrize("pow_mode", [0, 1])
+def test_weir_closed_forms(pow_mode):
+    k = OracleKernels(pow_mode=pow_mode)
+    two3 = 2.0 / 3.0
+    # free overflow (dry downstream): Qfactor 2/3 H sqrt(2/3 g H); momentum
+    # fluxes scaled like the mass flux; wave speed from the weir head
+    H, wh = 0.5, 2.0
+    ef0 = np.array([0.3, 0.12, -0.05])
+    ef, ms = _weir(k, ef0, H, 0.0, wh, qf=1.7)
+    q = 1.7 * two3 * H * math.sqrt(two3 * G * H)
+    assert ef[0] == pytest.approx(q, rel=1e-15)
+    assert np.allclose(ef[1:], ef0[1:] * min(q / ef0[0], 10.0), rtol=1e-15)
+    assert ms == pytest.approx(math.sqrt(G * (H + wh)) + q / (H + 1e-12), rel=1e-15)
+    # flow from right to left: the same magnitude, negative
+    ef, _ = _weir(k, -ef0, 0.0, H, wh, qf=1.7)
+    assert ef[0] == pytest.approx(-q, rel=1e-15)
+    # submerged (Villemonte): Q1 (1 - Q2/Q1)^0.385, below the blending range
+    H1, H2 = 0.5, 0.2
+    ef, _ = _weir(k, ef0, H1, H2, 5.0)
+    Q1 = two3 * H1 * math.sqrt(two3 * G * H1)
+    Q2 = two3 * H2 * math.sqrt(two3 * G * H2)
+    assert ef[0] == pytest.approx(Q1 * (1 - Q2 / Q1) ** 0.385, rel=1e-14)
+    # inside the submergence blending range: linear weights on the SW flux
+    H2 = 0.925 * H1                                    # halfway between s1 and s2
+    ef, _ = _weir(k, ef0, H1, H2, 5.0)
+    Q2 = two3 * H2 * math.sqrt(two3 * G * H2)
+    w1 = (H2 / H1 - 0.9) / 0.05
+    assert ef[0] == pytest.approx(Q1 * (1 - Q2 / Q1) ** 0.385 * (1 - w1) + w1 * ef0[0],
+                                  rel=1e-12)
+    # deep over the crest (tailwater head / weir height >= h2): the SW flux
+    ef, _ = _weir(k, ef0, 3.0, 0.0 + 1.6, 1.0)
+    assert np.array_equal(ef, ef0)
+    # equal heads: zero weir discharge, SW flux weight only
+    ef, _ = _weir(k, ef0, 0.4, 0.4, 5.0, s1=10.0, s2=20.0)
+    assert ef[0] == 0.0
+    # the momentum scaling is capped at 10
+    ef, _ = _weir(k, np.array([1e-3, 1.0, 1.0]), 0.5, 0.0, 2.0)
+    assert np.array_equal(ef[1:], [10.0, 10.0])
+
+
+@need_ref
+@pytest.mark.parametrize("seed", [0, 1])
+def test_weir_bitwise_vs_reference_c(seed):
+    rng = np.random.default_rng(seed)
+    ref, ora = RefKernels(), OracleKernels(pow_mode=0)
+    for _ in range(4000):
+        hl, hr = rng.random(2) * rng.choice([1e-3, 1.0, 5.0], 2)
+        if rng.random() < 0.2:
+            hr = 0.0
+        if rng.random() < 0.1:
+            hr = hl * (0.85 + 0.15 * rng.random())
+        wh = rng.choice([0.0, 0.1, 1.0, 3.0]) * rng.random()
+        s1 = rng.random()
+        s2 = s1 + 0.5 * rng.random() + 1e-3
+        h1 = 2 * rng.random()
+        h2 = h1 + rng.random() + 1e-3
+        ef0 = rng.standard_normal(3) * rng.choice([1e-8, 1.0, 100.0])
+        qf = 2 * rng.random()
+        a = _weir(ref, ef0, hl, hr, wh, qf, s1, s2, h1, h2, 0.7)
+        b = _weir(ora, ef0, hl, hr, wh, qf, s1, s2, h1, h2, 0.7)
+        assert np.array_equal(a[0], b[0]) and a[1] == b[1], (hl, hr, wh, qf, s1, s2, h1, h2)
+
+
+# ---------------------------------------------------------------------------
+# the flux loop
+# ---------------------------------------------------------------------------
+def _oracle_run(K, builder, kw, yieldstep, final, flags_fn=None):
+    mesh, alg, q, bc, walls, par = builder(**kw)
+    d = S.setup(OracleDomain(mesh, K), alg, q, bc)
+    eft, elev, rows, hp = S.riverwall_arrays(mesh, walls, par)
+    if flags_fn is not None:
+        eft, elev, rows = flags_fn(mesh, eft, elev, rows)
+    d.set_riverwalls(eft, elev, rows, hp)
+    for _ in d.evolve(yieldstep=yieldstep, finaltime=final):
+        pass
+    return d
+
+
+def one_sided(mesh, eft, elev, rows):
+    """Unflag the east half of every second interior wall edge and raise the
+    crest of the west half of the others by 0.3 m: halves that disagree (the
+    reference computes each interior edge from its lower-numbered triangle)."""
+    eft = eft.copy()
+    elev = elev.copy()
+    rows = rows.copy()
+    cen = mesh.centroid_coordinates
+    flagged = np.nonzero(eft == 1)[0]
+    drop = []
+    for j, ki in enumerate(flagged):
+        k = ki // 3
+        east = cen[k, 0] > mesh.edge_midpoint_coordinates[ki, 0]
+        if east and (j // 2) % 2 == 0 and mesh.neighbours[k, ki % 3] >= 0:
+            drop.append(j)
+        elif not east and (j // 2) % 2 == 1:
+            elev[j] += 0.3
+    keep = np.setdiff1d(np.arange(len(flagged)), drop)
+    eft[flagged[drop]] = 0
+    return eft, elev[keep], rows[keep]
+
+
+RUNS = [(S.riverwall_levees, dict(alg="DE1"), 2.0, 10.0, None),
+        (S.riverwall_levees, dict(alg="DE0"), 2.0, 6.0, None),
+        (S.riverwall_structure, dict(wall_height=20.0, ocean=20.5, land=20.44), 0.5, 2.0, None),
+        (S.riverwall_levees, dict(alg="DE1"), 2.0, 6.0, one_sided)]
+
+
+@need_ref
+@pytest.mark.parametrize("builder,kw,ys,ft,flags", RUNS)
+def test_oracle_riverwalls_bitwise_vs_reference_c(builder, kw, ys, ft, flags):
+    """Runs with walls: the restatement's edge-once loop equals the reference's
+    compiled C bit for bit (libm pow on both sides), every state array and the
+    timestep sequence; the per-triangle loop too where both halves agree."""
+    r = _oracle_run(RefKernels(), builder, kw, ys, ft, flags)
+    loops = [False] if flags else [False, True]
+    for gather in loops:
+        o = _oracle_run(OracleKernels(gather=gather, pow_mode=0), builder, kw, ys, ft, flags)
+        assert r.dt_history == o.dt_history
+        for a in ("stage_c", "xmom_c", "ymom_c", "height_c", "stage_e", "xmom_e", "ymom_e"):
+            assert np.array_equal(getattr(r, a), getattr(o, a)), (a, gather)
+
+
+def _land_volume(d):
+    land = d.mesh.centroid_coordinates[:, 0] < 50.0
+    h = d.stage_c - d.elev_c
+    return float(np.sum((h * (h > 0) * d.areas)[land]))
+
+
+@pytest.mark.parametrize("alg", ["DE0", "DE1"])
+def test_reference_riverwall_physics_on_oracle(alg):
+    """test_riverwall_structure.py:152-500 restated: L = 100 m of wall at
+    x = 50, over 1e-3 s."""
+    K = lambda: OracleKernels(gather=False, pow_mode=1)
+    L, ft = 100.0, 1.0e-3
+    # no flow while the water is below the crest (10 s)
+    d = _oracle_run(K(), S.riverwall_structure, dict(wall_height=-0.2, ocean=-0.3, alg=alg),
+                    10.0, 10.0)
+    assert _land_volume(d) == pytest.approx(0.0, abs=1e-12)
+    # free overflow: Qfactor 2/3 H sqrt(2/3 g H) per unit length
+    for qf in (1.0, 2.0):
+        d = _oracle_run(K(), S.riverwall_structure,
+                        dict(wall_height=2.0, ocean=2.5, qfactor=qf, alg=alg), 1e-4, ft)
+        H = 0.5
+        theory = qf * ft * L * 2.0 / 3.0 * H * (2.0 / 3.0 * G * H) ** 0.5
+        assert _land_volume(d) == pytest.approx(theory, rel=1e-3)
+    # submerged: Q1 (1 - Q2/Q1)^0.385
+    mesh, a, q, bc, walls, par = S.riverwall_structure(wall_height=20.0, ocean=20.5,
+                                                       land=20.44, alg=alg)
+    h0 = q["stage"] - q["elevation"]
+    land = mesh.centroid_coordinates[:, 0] < 50.0
+    v0 = float(np.sum((h0 * (h0 > 0) * mesh.areas)[land]))
+    d = _oracle_run(K(), S.riverwall_structure,
+                    dict(wall_height=20.0, ocean=20.5, land=20.44, alg=alg), 1e-4, ft)
+    Q1 = 2.0 / 3.0 * 0.5 * (2.0 / 3.0 * G * 0.5) ** 0.5
+    Q2 = 2.0 / 3.0 * 0.44 * (2.0 / 3.0 * G * 0.44) ** 0.5
+    assert _land_volume(d) - v0 == pytest.approx(ft * L * Q1 * (1 - Q2 / Q1) ** 0.385,
+                                                 rel=1e-3)
+
+
+def test_riverwall_run_conserves_mass_in_a_closed_basin():
+    """Reflective walls all round: the wall fluxes are antisymmetric, so the
+    volume stays constant to rounding while water overtops the levees."""
+    def closed(**kw):
+        mesh, alg, q, bc, walls, par = S.riverwall_levees(**kw)
+        return mesh, alg, q, {t: "reflective" for t in bc}, walls, par
+    d = _oracle_run(OracleKernels(gather=True), closed, dict(alg="DE1"), 1.0, 1.0)
+    v0 = d.get_water_volume()
+    for _ in d.evolve(yieldstep=2.0, finaltime=9.0):
+        pass
+    assert d.get_water_volume() == pytest.approx(v0, rel=1e-12)
+    # water crossed both walls
+    x, y = d.mesh.centroid_coordinates.T
+    h = d.stage_c - d.elev_c
+    assert h[(x > 20) & (x < 80) & (y < 60)].max() > 1e-3
