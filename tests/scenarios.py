@@ -151,3 +151,24 @@ def riverwall_arrays(mesh, walls, par, **kw):
     dom = _WallMesh(mesh)
     RiverWall(dom).create_riverwalls(walls, par, verbose=False, **kw)
     return dom.uploaded
+
+
+def riverwall_three_walls(alg="DE0"):
+    """The set-up of the reference's cross-backend riverwall test
+    (anuga/shallow_water/tests/test_DE_openmp.py:179-226) on a
+    rectangular_cross mesh of the 20 m x 10 m box (0.5 m cells, so the walls
+    lie on edges): bed -x/10, dry, friction 0.01, Dirichlet stage 0.4 on the
+    left and -2 on the right, reflective top and bottom; walls at x = 5
+    (crest 0.5, y 0..4), x = 15 (crest -0.5, y 0..4), x = 10 (crest 0,
+    y 6..10)."""
+    p, v, b = rectangular_cross(40, 20, 20.0, 10.0)
+    mesh = Mesh(p, v, b)
+    x = mesh.centroid_coordinates[:, 0]
+    N = mesh.number_of_triangles
+    q = dict(elevation=-x / 10.0, friction=np.full(N, 0.01), stage=-x / 10.0)
+    walls = {"wall1": [[5.0, 0.0, 0.5], [5.0, 4.0, 0.5]],
+             "wall2": [[15.0, 0.0, -0.5], [15.0, 4.0, -0.5]],
+             "wall3": [[10.0, 10.0, 0.0], [10.0, 6.0, 0.0]]}
+    bc = {"left": ("dirichlet", [0.4, 0.0, 0.0]), "right": ("dirichlet", [-2.0, 0.0, 0.0]),
+          "top": "reflective", "bottom": "reflective"}
+    return mesh, alg, q, bc, walls, {}

@@ -234,7 +234,9 @@ def one_sided(mesh, eft, elev, rows):
     return eft, elev[keep], rows[keep]
 
 
-RUNS = [(S.riverwall_levees, dict(alg="DE1"), 2.0, 10.0, None),
+RUNS = [(S.riverwall_three_walls, dict(alg="DE0"), 1.0, 4.0, None),
+        (S.riverwall_three_walls, dict(alg="DE1"), 1.0, 4.0, None),
+        (S.riverwall_levees, dict(alg="DE1"), 2.0, 10.0, None),
         (S.riverwall_levees, dict(alg="DE0"), 2.0, 6.0, None),
         (S.riverwall_structure, dict(wall_height=20.0, ocean=20.5, land=20.44), 0.5, 2.0, None),
         (S.riverwall_levees, dict(alg="DE1"), 2.0, 6.0, one_sided)]
