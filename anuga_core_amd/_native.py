@@ -28,6 +28,7 @@ PRECISION_FP64, PRECISION_MIXED = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int64)
+_fp = C.POINTER(C.c_float)
 
 
 class AnugaMesh(C.Structure):
@@ -77,6 +78,12 @@ SIGNATURES = {
     "anuga_set_boundary": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _dp]),
     "anuga_set_riverwalls": (C.c_int, [C.c_void_p, _ip, C.c_int64, _dp, _ip, C.c_int64,
                                        C.c_int64, _dp]),
+    "anuga_set_output_nodes": (C.c_int, [C.c_void_p, C.c_int64, _ip, _ip]),
+    "anuga_get_sww_frame": (C.c_int, [C.c_void_p, C.c_double, _fp, _fp]),
+    "anuga_get_static_output": (C.c_int, [C.c_void_p, C.c_int, _fp, _fp]),
+    "anuga_set_gauges": (C.c_int, [C.c_void_p, C.c_int64, _ip, C.c_int64]),
+    "anuga_sample_gauges": (C.c_int, [C.c_void_p, C.c_double]),
+    "anuga_get_gauge_samples": (C.c_int, [C.c_void_p, C.c_int64, _ip, _dp, _dp]),
     "anuga_distribute_to_vertices_and_edges": (C.c_int, [C.c_void_p, _dp]),
     "anuga_compute_fluxes": (C.c_int, [C.c_void_p, _dp]),
     "anuga_update_conserved_quantities": (C.c_int, [C.c_void_p, C.c_double, _ip]),

@@ -133,6 +133,10 @@ struct DeControl {
     // the scenario's triangles are [kbeg, kend) and its blocks [bbeg, bend) of
     // the internal numbering (fixed at creation)
     int kbeg, kend, bbeg, bend;
+    // gauge recording (one scenario): nsteps at the last recorded row, rows
+    // recorded since the last download, row capacity (0: no gauges)
+    long long gstep;
+    int grows, gcap;
 };
 
 // control block of the scenario that owns logical block lb

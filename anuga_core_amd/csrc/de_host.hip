@@ -60,6 +60,16 @@ struct anuga_domain {
     std::vector<int4> hnb;               // host copy of nb
     int *rw = nullptr;                   // [3][N] riverwall record per half-edge, -1: none
     DeRiverwall *rwd = nullptr;          // riverwall records; rw == nullptr: no riverwalls
+    // output (one scenario): node CSR of the centroid averaging, in the
+    // reference's entry order, with internal triangle ids; caller -> internal map
+    int nodes = 0;
+    int *node_start = nullptr, *node_tri = nullptr, *iperm_dev = nullptr;
+    float *out_f32 = nullptr;            // staging of one frame: [3][nodes] + [3][N]
+    // gauges (one scenario): internal triangle ids, rows of times and states
+    int G = 0, gcap = 0;
+    int64_t grows_host = 0;              // rows on the device not yet downloaded
+    int *gtri = nullptr;
+    double *gtimes = nullptr, *gvals = nullptr;
     bool mixed = false;
     // quantities
     double *eta = nullptr;
@@ -468,6 +478,9 @@ static void enqueue_device_step(anuga_domain *d)
     enqueue_later_substeps(d);
     hipLaunchKernelGGL(de_finish_step, scenario_grid(d), dim3(64), 0, d->stream, d->ctl, d->S,
                        d->S == 1 ? d->hist : nullptr);
+    if (d->G)
+        hipLaunchKernelGGL(de_sample_gauges<false>, dim3(1), dim3(256), 0, d->stream, d->ctl,
+                           d->G, d->gtri, d->A, 0.0, d->gtimes, d->gvals);
 }
 
 static void drop_graph(anuga_domain *d)
@@ -963,6 +976,11 @@ static int evolve_steps(anuga_domain *d, int64_t max_steps, const anuga_time_con
     if (d->substate != 0) return fail(d, ANUGA_E_STATE, "evolve_steps called inside an rk2 step");
     int rc = ready_to_step(d);
     if (rc) return rc;
+    if (d->G) {   // every step records a gauge row: stop where the rows run out
+        if (d->grows_host >= d->gcap)
+            return fail(d, ANUGA_E_STATE, "gauge rows full: download them first");
+        max_steps = std::min<int64_t>(max_steps, d->gcap - d->grows_host);
+    }
     if (dt_history && max_steps > d->hist_cap) {
         if (d->hist) hipFree(d->hist);
         d->hist = nullptr;
@@ -1026,6 +1044,7 @@ static int evolve_steps(anuga_domain *d, int64_t max_steps, const anuga_time_con
         st[s].steps_taken = c.nsteps;
         st[s].reached = c.done;
     }
+    if (d->G) d->grows_host += d->hctl[0].nsteps;
     if (dt_history && d->hctl[0].nsteps > 0) {
         HIPCHK(d, hipMemcpy(dt_history, d->hist,
                             (size_t)std::min<int64_t>(d->hctl[0].nsteps, max_steps) * sizeof(double),
@@ -1191,7 +1210,8 @@ void anuga_destroy(anuga_domain *d)
     if (d->stream) hipStreamSynchronize(d->stream);
     drop_graph(d);
     void *ptrs[] = {d->nb, d->cen, d->gx, d->gf, d->area, d->nx, d->ny, d->len, d->radius,
-                    d->scen, d->rw, d->rwd, d->eta, d->A, d->B, d->E, d->flags,
+                    d->scen, d->rw, d->rwd, d->node_start, d->node_tri, d->iperm_dev,
+                    d->out_f32, d->gtri, d->gtimes, d->gvals, d->eta, d->A, d->B, d->E, d->flags,
                     d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->ctl, d->dsetup,
                     d->dneg, d->hist};
     for (void *p : ptrs)
@@ -1368,6 +1388,219 @@ int anuga_set_riverwalls(anuga_domain *d, const int64_t *edge_flux_type,
     HIPCHK(d, dalloc(&d->rwd, (size_t)R));
     HIPCHK(d, hipMemcpy(d->rw, hr.data(), 3 * NN * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(d, hipMemcpy(d->rwd, rec.data(), R * sizeof(DeRiverwall), hipMemcpyHostToDevice));
+    return ANUGA_OK;
+}
+
+static int ensure_iperm(anuga_domain *d)
+{
+    if (d->iperm_dev) return ANUGA_OK;
+    const int N = d->ntri[0];
+    std::vector<int> ip(N);
+    for (int p = 0; p < N; p++) ip[d->perm[p]] = p;
+    HIPCHK(d, dalloc(&d->iperm_dev, (size_t)N));
+    HIPCHK(d, hipMemcpy(d->iperm_dev, ip.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    return ANUGA_OK;
+}
+
+static int ensure_out_f32(anuga_domain *d)
+{
+    if (d->out_f32) return ANUGA_OK;
+    HIPCHK(d, dalloc(&d->out_f32, 3 * ((size_t)d->nodes + d->ntri[0])));
+    return ANUGA_OK;
+}
+
+int anuga_set_output_nodes(anuga_domain *d, int64_t number_of_nodes,
+                           const int64_t *number_of_triangles_per_node,
+                           const int64_t *vertex_value_indices)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    const int N = d->ntri[0];
+    if (number_of_nodes < 1 || number_of_nodes >= (int64_t(1) << 30))
+        return fail(d, ANUGA_E_ARG, "bad number_of_nodes");
+    if (!number_of_triangles_per_node || !vertex_value_indices)
+        return fail(d, ANUGA_E_ARG, "a node array is NULL");
+    const int nodes = (int)number_of_nodes;
+    std::vector<int> start(nodes + 1);
+    int64_t total = 0;
+    for (int j = 0; j < nodes; j++) {
+        if (number_of_triangles_per_node[j] < 0)
+            return fail(d, ANUGA_E_ARG, "negative number_of_triangles_per_node");
+        start[j] = (int)total;
+        total += number_of_triangles_per_node[j];
+        if (total > 3 * (int64_t)N) break;
+    }
+    if (total != 3 * (int64_t)N)
+        return fail(d, ANUGA_E_ARG, "number_of_triangles_per_node must sum to 3 N");
+    start[nodes] = (int)total;
+    std::vector<int> ip(N), tri(3 * (size_t)N);
+    for (int p = 0; p < N; p++) ip[d->perm[p]] = p;
+    for (int64_t i = 0; i < 3 * (int64_t)N; i++) {
+        const int64_t v = vertex_value_indices[i];
+        if (v < 0 || v >= 3 * (int64_t)N)
+            return fail(d, ANUGA_E_ARG, "vertex_value_indices out of range");
+        tri[i] = ip[v / 3];
+    }
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    void *old[] = {d->node_start, d->node_tri, d->out_f32};
+    for (void *q : old)
+        if (q) HIPCHK(d, hipFree(q));
+    d->node_start = d->node_tri = nullptr;
+    d->out_f32 = nullptr;
+    d->nodes = nodes;
+    HIPCHK(d, dalloc(&d->node_start, (size_t)nodes + 1));
+    HIPCHK(d, dalloc(&d->node_tri, 3 * (size_t)N));
+    HIPCHK(d, hipMemcpy(d->node_start, start.data(), (nodes + 1) * sizeof(int),
+                        hipMemcpyHostToDevice));
+    HIPCHK(d, hipMemcpy(d->node_tri, tri.data(), 3 * (size_t)N * sizeof(int),
+                        hipMemcpyHostToDevice));
+    return ANUGA_OK;
+}
+
+int anuga_get_sww_frame(anuga_domain *d, double minimum_storable_height, float *node_values,
+                        float *centroid_values)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    if (node_values && !d->node_start)
+        return fail(d, ANUGA_E_STATE, "no output nodes: call anuga_set_output_nodes first");
+    const int N = d->ntri[0];
+    if ((rc = ensure_iperm(d)) || (rc = ensure_out_f32(d))) return rc;
+    const DeQ *S = d->substate ? d->B : d->A;
+    float *on = d->out_f32, *oc = d->out_f32 + 3 * (size_t)d->nodes;
+    if (node_values)
+        hipLaunchKernelGGL(de_sww_nodes, dim3((d->nodes + DE_BLOCK - 1) / DE_BLOCK), dim3(DE_BLOCK),
+                           0, d->stream, d->nodes, d->node_start, d->node_tri, S,
+                           minimum_storable_height, on);
+    if (centroid_values)
+        hipLaunchKernelGGL(de_gather_f32, dim3((N + DE_BLOCK - 1) / DE_BLOCK), dim3(DE_BLOCK), 0,
+                           d->stream, N, d->iperm_dev, (const double *)S, 4, 3,
+                           make_int4(0, 1, 2, 0), oc);
+    rc = launch_check(d);
+    if (rc) return rc;
+    if (node_values)
+        HIPCHK(d, hipMemcpyAsync(node_values, on, 3 * (size_t)d->nodes * sizeof(float),
+                                 hipMemcpyDeviceToHost, d->stream));
+    if (centroid_values)
+        HIPCHK(d, hipMemcpyAsync(centroid_values, oc, 3 * (size_t)N * sizeof(float),
+                                 hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    return ANUGA_OK;
+}
+
+int anuga_get_static_output(anuga_domain *d, int quantity, float *node_values,
+                            float *centroid_values)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    if (quantity != ANUGA_ELEVATION && quantity != ANUGA_FRICTION)
+        return fail(d, ANUGA_E_ARG, "static output is elevation or friction");
+    if (node_values && !d->node_start)
+        return fail(d, ANUGA_E_STATE, "no output nodes: call anuga_set_output_nodes first");
+    const int N = d->ntri[0];
+    if ((rc = ensure_iperm(d)) || (rc = ensure_out_f32(d))) return rc;
+    const double *v = quantity == ANUGA_ELEVATION ? (const double *)d->A + 3 : d->eta;
+    const int stride = quantity == ANUGA_ELEVATION ? 4 : 1;
+    float *on = d->out_f32, *oc = d->out_f32 + 3 * (size_t)d->nodes;
+    if (node_values)
+        hipLaunchKernelGGL(de_node_values, dim3((d->nodes + DE_BLOCK - 1) / DE_BLOCK),
+                           dim3(DE_BLOCK), 0, d->stream, d->nodes, d->node_start, d->node_tri, v,
+                           stride, on);
+    if (centroid_values)
+        hipLaunchKernelGGL(de_gather_f32, dim3((N + DE_BLOCK - 1) / DE_BLOCK), dim3(DE_BLOCK), 0,
+                           d->stream, N, d->iperm_dev, v, stride, 1, make_int4(0, 0, 0, 0), oc);
+    rc = launch_check(d);
+    if (rc) return rc;
+    if (node_values)
+        HIPCHK(d, hipMemcpyAsync(node_values, on, (size_t)d->nodes * sizeof(float),
+                                 hipMemcpyDeviceToHost, d->stream));
+    if (centroid_values)
+        HIPCHK(d, hipMemcpyAsync(centroid_values, oc, (size_t)N * sizeof(float),
+                                 hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    return ANUGA_OK;
+}
+
+int anuga_set_gauges(anuga_domain *d, int64_t ngauges, const int64_t *triangles,
+                     int64_t capacity)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    const int N = d->ntri[0];
+    if (ngauges < 0 || ngauges > N) return fail(d, ANUGA_E_ARG, "bad gauge count");
+    if (ngauges > 0 && (!triangles || capacity < 1 || capacity > (int64_t(1) << 24) ||
+                        capacity * ngauges > (int64_t(1) << 31)))
+        return fail(d, ANUGA_E_ARG, "gauge triangles NULL or bad capacity");
+    std::vector<int> ip(N), gt(ngauges);
+    for (int p = 0; p < N; p++) ip[d->perm[p]] = p;
+    for (int64_t g = 0; g < ngauges; g++) {
+        if (triangles[g] < 0 || triangles[g] >= N)
+            return fail(d, ANUGA_E_ARG, "gauge triangle out of range");
+        gt[g] = ip[triangles[g]];
+    }
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    drop_graph(d);
+    void *old[] = {d->gtri, d->gtimes, d->gvals};
+    for (void *q : old)
+        if (q) HIPCHK(d, hipFree(q));
+    d->gtri = nullptr;
+    d->gtimes = d->gvals = nullptr;
+    d->G = d->gcap = 0;
+    d->grows_host = 0;
+    if (ngauges > 0) {
+        HIPCHK(d, dalloc(&d->gtri, (size_t)ngauges));
+        HIPCHK(d, dalloc(&d->gtimes, (size_t)capacity));
+        HIPCHK(d, dalloc(&d->gvals, (size_t)capacity * ngauges * 4));
+        HIPCHK(d, hipMemcpy(d->gtri, gt.data(), ngauges * sizeof(int), hipMemcpyHostToDevice));
+        d->G = (int)ngauges;
+        d->gcap = (int)capacity;
+    }
+    hipLaunchKernelGGL(de_gauge_control, dim3(1), dim3(64), 0, d->stream, d->ctl, 0, d->gcap);
+    return launch_check(d);
+}
+
+int anuga_sample_gauges(anuga_domain *d, double relative_time)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    if (!d->G) return fail(d, ANUGA_E_STATE, "no gauges set");
+    if (d->grows_host >= d->gcap)
+        return fail(d, ANUGA_E_STATE, "gauge rows full: download them first");
+    hipLaunchKernelGGL(de_sample_gauges<true>, dim3(1), dim3(256), 0, d->stream, d->ctl, d->G,
+                       d->gtri, d->substate ? d->B : d->A, relative_time, d->gtimes, d->gvals);
+    d->grows_host++;
+    return launch_check(d);
+}
+
+int anuga_get_gauge_samples(anuga_domain *d, int64_t max_rows, int64_t *nrows, double *times,
+                            double *values)
+{
+    if (!d || !nrows) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    *nrows = 0;
+    if (!d->G) return fail(d, ANUGA_E_STATE, "no gauges set");
+    rc = sync_scalars(d);
+    if (rc) return rc;
+    const int rows = d->hctl[0].grows;
+    if (rows > max_rows) return fail(d, ANUGA_E_ARG, "max_rows is below the rows recorded");
+    if (rows > 0 && (!times || !values)) return fail(d, ANUGA_E_ARG, "times or values is NULL");
+    if (rows > 0) {
+        HIPCHK(d, hipMemcpy(times, d->gtimes, rows * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(d, hipMemcpy(values, d->gvals, (size_t)rows * d->G * 4 * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    }
+    hipLaunchKernelGGL(de_gauge_control, dim3(1), dim3(64), 0, d->stream, d->ctl, 1, 0);
+    rc = launch_check(d);
+    if (rc) return rc;
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    d->grows_host = 0;
+    *nrows = rows;
     return ANUGA_OK;
 }
 

@@ -925,6 +925,7 @@ __global__ void de_setup_batch(DeControl *ctl, int S, const DeBatchSetup *__rest
     c->hist_cap = b.hist_cap;
     c->method = b.method;
     c->nsteps = 0;
+    c->gstep = 0;
     c->done = 0;
     if (b.stop_reached) {
         if (c->t >= c->finaltime - c->eps) c->done = 1;
@@ -986,4 +987,115 @@ de_bed_offsets(int kbeg, int kend, const int4 *__restrict__ nb, const DeQ *__res
 #pragma unroll
     for (int i = 0; i < 3; i++)
         gf[k].dzc[i] = vv[i] >= 0 ? float(q[vv[i] >> 2].z - z) : 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// output: gauge rows and SWW frames
+// ---------------------------------------------------------------------------
+// One row of gauge samples: the centroid state {w, uh, vh, z} of each gauge
+// triangle and the relative time. Device-controlled steps record once per
+// completed step (nsteps moved since the last row); FORCE records now with the
+// host's time t (host-sequenced steps). One workgroup: the row counter moves
+// after every lane has written its gauges.
+template <bool FORCE>
+__global__ void __launch_bounds__(256)
+de_sample_gauges(DeControl *c, int G, const int *__restrict__ tri, const DeQ *__restrict__ S,
+                 double t, double *__restrict__ times, double *__restrict__ vals)
+{
+    const long long ns = c->nsteps;
+    const int row = c->grows;
+    const bool take = (FORCE || ns != c->gstep) && row < c->gcap;
+    if (take)
+        for (int g = threadIdx.x; g < G; g += blockDim.x) {
+            const DeQ q = S[tri[g]];
+            double *o = vals + ((size_t)row * G + g) * 4;
+            o[0] = q.w;
+            o[1] = q.uh;
+            o[2] = q.vh;
+            o[3] = q.z;
+        }
+    __syncthreads();
+    if (take && threadIdx.x == 0) {
+        times[row] = FORCE ? t : c->t;
+        c->grows = row + 1;
+        c->gstep = ns;
+    }
+}
+
+// gauge rows: mode 0 starts recording (cap rows, the current step is not a
+// new one); mode 1 marks the rows downloaded
+__global__ void de_gauge_control(DeControl *c, int mode, int cap)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (mode == 0) {
+        c->gstep = c->nsteps;
+        c->gcap = cap;
+    }
+    c->grows = 0;
+}
+
+// Node values by centroid averaging, _average_centroid_values
+// (abstract_2d_finite_volumes/quantity.c:872-914): the centroid values of
+// the triangles around each node summed in the node's entry order
+// (vertex_value_indices), divided by their count; orphan nodes 0. Stage,
+// momenta and bed together, then the storable-depth selection of
+// SWW_file.store_timestep (anuga/file/sww.py:318-356): where the node depth
+// w - z is below minimum_storable_height the stage is the bed and the
+// momenta 0. Written as float32 [3][nodes] (the file's precision).
+__global__ void __launch_bounds__(DE_BLOCK)
+de_sww_nodes(int nodes, const int *__restrict__ start, const int *__restrict__ tri,
+             const DeQ *__restrict__ S, double min_storable, float *__restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nodes) return;
+    const int b = start[j], e = start[j + 1];
+    double w = 0.0, uh = 0.0, vh = 0.0, z = 0.0;
+    if (e > b) {
+        for (int i = b; i < e; i++) {
+            const DeQ q = S[tri[i]];
+            w += q.w;
+            uh += q.uh;
+            vh += q.vh;
+            z += q.z;
+        }
+        const double k = (double)(e - b);
+        w = w / k;
+        uh = uh / k;
+        vh = vh / k;
+        z = z / k;
+    }
+    const bool storable = w - z >= min_storable;
+    out[j] = (float)(storable ? w : z);
+    out[nodes + j] = storable ? (float)uh : 0.0f;
+    out[2 * (size_t)nodes + j] = storable ? (float)vh : 0.0f;
+}
+
+// node values of one per-triangle quantity v[k * stride] by centroid averaging,
+// as float32
+__global__ void __launch_bounds__(DE_BLOCK)
+de_node_values(int nodes, const int *__restrict__ start, const int *__restrict__ tri,
+               const double *__restrict__ v, int stride, float *__restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nodes) return;
+    const int b = start[j], e = start[j + 1];
+    double a = 0.0;
+    if (e > b) {
+        for (int i = b; i < e; i++) a += v[(size_t)tri[i] * stride];
+        a = a / (double)(e - b);
+    }
+    out[j] = (float)a;
+}
+
+// centroid values in the caller's order as float32: out[q][k] for the
+// quantities at offsets off[q] of a record of `stride` doubles
+__global__ void __launch_bounds__(DE_BLOCK)
+de_gather_f32(int N, const int *__restrict__ iperm, const double *__restrict__ v, int stride,
+              int nq, int4 off, float *__restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const double *r = v + (size_t)iperm[k] * stride;
+    const int o[4] = {off.x, off.y, off.z, off.w};
+    for (int q = 0; q < nq; q++) out[(size_t)q * N + k] = (float)r[o[q]];
 }

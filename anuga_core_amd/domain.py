@@ -18,6 +18,7 @@ stays in HBM and only the CFL timestep (8 bytes) crosses per step.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import warnings
 
 import numpy as np
@@ -83,6 +84,52 @@ class Quantity:
         return float(np.sum(self.domain.mesh.areas * self.centroid_values))
 
 
+MINIMUM_STORABLE_HEIGHT = 1.0e-03    # anuga/config.py minimum_storable_height
+
+
+def _gauge_row(quantities, w, z, uh, vh):
+    """_quantities2csv (anuga/abstract_2d_finite_volumes/gauge.py:30-81)."""
+    import math
+    out = []
+    for q in quantities:
+        if q == "stage":
+            out.append(w)
+        elif q == "elevation":
+            out.append(z)
+        elif q == "xmomentum":
+            out.append(uh)
+        elif q == "ymomentum":
+            out.append(vh)
+        elif q == "depth":
+            out.append(w - z)
+        elif q == "momentum":
+            out.append(math.sqrt(uh ** 2 + vh ** 2))
+        elif q == "speed":
+            if w - z < 0.001:
+                out.append(0.0)
+            elif uh < 1.0e6:
+                out.append(math.sqrt(uh ** 2 + vh ** 2) / (w - z))
+            else:
+                out.append(0.0)
+        elif q == "bearing":
+            out.append(_bearing(uh, vh))
+    return out
+
+
+def _bearing(uh, vh):
+    """calc_bearing (anuga/abstract_2d_finite_volumes/util.py:156-171): the
+    angle in degrees, in [0, 360), from (0, -1) to the momentum vector
+    (numerical_tools.angle); NaN for zero momentum."""
+    import math
+    if uh == 0 and vh == 0:
+        return float("nan")
+    n = math.sqrt(uh * uh + vh * vh)
+    theta = math.acos(max(-1.0, min(1.0, -vh / n)))
+    if uh / n < 0:
+        theta = 2 * math.pi - theta
+    return math.degrees(theta)
+
+
 class Domain:
     """Shallow-water domain whose DE step runs on the GPU.
 
@@ -145,6 +192,22 @@ class Domain:
         self.number_of_riverwall_edges = 0
         self.riverwallData = RiverWall(self)
         self.create_riverwalls = self.riverwallData.create_riverwalls
+
+        # output (shallow_water_domain.py:335-345, anuga/config.py): SWW frames
+        # are prepared on the device at yields; storage is off unless set_store
+        self.store = False
+        self.store_centroids = True
+        self.smooth = True
+        self.minimum_storable_height = MINIMUM_STORABLE_HEIGHT
+        self.quantities_to_be_stored = {"elevation": 1, "friction": 1, "stage": 2,
+                                        "xmomentum": 2, "ymomentum": 2}
+        self.datadir = "."
+        self.simulation_name = "domain"
+        self.output_frequency = 1
+        self.yieldstep_counter = 0
+        self.writer = None
+        self._output_nodes = False
+        self._gauges = None
 
     # -- device handle -----------------------------------------------------------
     def _create_device(self):
@@ -297,15 +360,181 @@ class Domain:
         if self.boundary_map is not None:
             self.set_boundary(self.boundary_map)
 
+    # -- output -----------------------------------------------------------------------
     def set_store(self, flag=True):
-        if flag:
-            warnings.warn("SWW storage is not part of this package; set_store ignored")
+        """Write an SWW file (datadir/name.sww) at every output yield."""
+        self.store = bool(flag)
 
-    def set_name(self, name):
+    def get_store(self):
+        return self.store
+
+    def set_store_centroids(self, flag=True):
+        self.store_centroids = bool(flag)
+
+    def get_store_centroids(self):
+        return self.store_centroids
+
+    def set_store_vertices_uniquely(self, flag=True, reduction=None):
+        if flag:
+            raise Exception("SWW vertex values are stored smoothly (node averages of the "
+                            "centroids); unique vertex storage is not supported")
+        self.smooth = True
+
+    def set_store_vertices_smoothly(self, flag=True, reduction=None):
+        self.set_store_vertices_uniquely(not flag)
+
+    def set_minimum_storable_height(self, minimum_storable_height):
+        self.minimum_storable_height = float(minimum_storable_height)
+
+    def get_minimum_storable_height(self):
+        return self.minimum_storable_height
+
+    def set_quantities_to_be_stored(self, q):
+        """{name: 1 (static) or 2 (every yield)} (shallow_water_domain.py:1520-1570)."""
+        if q is None:
+            self.quantities_to_be_stored = {}
+            self.store = False
+            return
+        if isinstance(q, (list, tuple)):
+            q = {name: 2 for name in q}
+        for name, flag in q.items():
+            if name not in ("stage", "xmomentum", "ymomentum", "elevation", "friction"):
+                raise Exception("Quantity %s is not a quantity of the domain" % name)
+            if flag not in (1, 2):
+                raise Exception("Store flag of %s must be 1 (static) or 2 (dynamic)" % name)
+        self.quantities_to_be_stored = dict(q)
+
+    def set_datadir(self, name):
+        self.datadir = name
+
+    def get_datadir(self):
+        return self.datadir
+
+    def set_name(self, name=None, timestamp=False):
+        """Base name of the SWW file (generic_domain.py:1565-1596)."""
+        if name is None:
+            import __main__
+            main = getattr(__main__, "__file__", None)
+            name = "output_" + os.path.splitext(os.path.basename(main))[0] if main else "domain"
+        name = str(name)
+        if name.endswith(".sww"):
+            name = name[:-4]
+        if timestamp:
+            import time
+            name += time.strftime("_%Y%m%d_%H%M%S")
         self.simulation_name = name
 
     def get_name(self):
-        return getattr(self, "simulation_name", "domain")
+        return self.simulation_name
+
+    def _ensure_output_nodes(self):
+        if self._output_nodes:
+            return
+        from .sww import node_structure
+        m = self.mesh
+        counts, vvi = node_structure(m.triangles, m.number_of_nodes)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
+        self._call("anuga_set_output_nodes", m.number_of_nodes, ip(counts), ip(vvi))
+        self._output_nodes = True
+
+    def _sww_frame(self, minimum_storable_height):
+        """(node values [3][nodes], centroid values [3][N]) of stage,
+        xmomentum, ymomentum as float32, prepared on the device."""
+        self._ensure_output_nodes()
+        nv = np.empty((3, self.mesh.number_of_nodes), np.float32)
+        cv = np.empty((3, self.number_of_elements), np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        self._call("anuga_get_sww_frame", C.c_double(minimum_storable_height), fp(nv), fp(cv))
+        return nv, cv
+
+    def _static_output(self, name):
+        self._ensure_output_nodes()
+        nv = np.empty(self.mesh.number_of_nodes, np.float32)
+        cv = np.empty(self.number_of_elements, np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        self._call("anuga_get_static_output", _QIDS[name], fp(nv), fp(cv))
+        return nv, cv
+
+    def initialise_storage(self):
+        """Create the SWW writer and store the mesh and static quantities."""
+        from .sww import SWW_file
+        if self.writer is not None:
+            self.writer.close()
+        self.writer = SWW_file(self)
+        self.writer.store_connectivity()
+
+    def store_timestep(self):
+        self.writer.store_timestep()
+
+    # -- gauges -------------------------------------------------------------------------
+    def set_gauges(self, points=None, triangles=None, names=None, capacity=4096):
+        """Record the centroid state of the triangles holding `points` (or of
+        `triangles`) after every step, on the device."""
+        if (points is None) == (triangles is None):
+            raise Exception("give either points or triangles")
+        if points is not None:
+            pts = np.atleast_2d(np.asarray(points, np.float64))
+            triangles = self.mesh.find_triangles(pts)
+            if np.any(triangles < 0):
+                raise Exception("gauge point(s) %s fall outside the mesh"
+                                % pts[triangles < 0].tolist())
+        tri = np.ascontiguousarray(triangles, np.int64).ravel()
+        if names is None:
+            names = ["gauge%d" % i for i in range(len(tri))]
+        self._call("anuga_set_gauges", len(tri), tri.ctypes.data_as(C.POINTER(C.c_int64)),
+                   int(capacity))
+        self._gauges = dict(triangles=tri, names=list(names), capacity=int(capacity),
+                            times=[], values=[], pending=0)
+
+    def _drain_gauges(self):
+        g = self._gauges
+        if g is None:
+            return
+        cap, G = g["capacity"], len(g["triangles"])
+        t = np.empty(cap)
+        v = np.empty((cap, G, 4))
+        n = C.c_int64()
+        self._call("anuga_get_gauge_samples", cap, C.byref(n), _dptr(t), _dptr(v))
+        g["pending"] = 0
+        if n.value:
+            g["times"].append(t[:n.value] + self.starttime)
+            g["values"].append(v[:n.value].copy())
+
+    def get_gauge_timeseries(self):
+        """{'time', 'stage', 'xmomentum', 'ymomentum', 'elevation', 'depth'}:
+        arrays (samples, gauges) of every step since set_gauges."""
+        g = self._gauges
+        if g is None:
+            raise Exception("no gauges set")
+        self._drain_gauges()
+        G = len(g["triangles"])
+        t = np.concatenate(g["times"]) if g["times"] else np.empty(0)
+        v = np.concatenate(g["values"]) if g["values"] else np.empty((0, G, 4))
+        g["times"], g["values"] = [t], [v]
+        return dict(time=t, stage=v[:, :, 0], xmomentum=v[:, :, 1], ymomentum=v[:, :, 2],
+                    elevation=v[:, :, 3], depth=v[:, :, 0] - v[:, :, 3])
+
+    def write_gauge_csv(self, directory=".", out_name="gauge_",
+                        quantities=("stage", "depth", "elevation", "xmomentum", "ymomentum")):
+        """One CSV per gauge in the column layout of sww2csv_gauges
+        (anuga/abstract_2d_finite_volumes/gauge.py:30-266): time, hours, then
+        the quantities (stage, elevation, xmomentum, ymomentum, depth,
+        momentum, speed, bearing)."""
+        import csv
+        ts = self.get_gauge_timeseries()
+        paths = []
+        for i, name in enumerate(self._gauges["names"]):
+            path = os.path.join(directory, out_name + name + ".csv")
+            with open(path, "w", newline="") as f:
+                w = csv.writer(f)
+                w.writerow(["time", "hours"] + list(quantities))
+                for r, t in enumerate(ts["time"]):
+                    w.writerow([t, t / 3600.0] + _gauge_row(quantities, ts["stage"][r, i],
+                                                             ts["elevation"][r, i],
+                                                             ts["xmomentum"][r, i],
+                                                             ts["ymomentum"][r, i]))
+            paths.append(path)
+        return paths
 
     # -- quantities -----------------------------------------------------------------
     def set_quantity(self, name, numeric=None, function=None, location="vertices",
@@ -503,9 +732,11 @@ class Domain:
     def set_time(self, time=0.0):
         self.relative_time = float(time) - self.starttime
 
-    def evolve(self, yieldstep=None, finaltime=None, duration=None, skip_initial_step=False):
+    def evolve(self, yieldstep=None, finaltime=None, duration=None, skip_initial_step=False,
+               outputstep=None):
         """Generator over yield times (shallow_water_domain.py:2300-2407 and
-        generic_domain.py:1715-1912 without SWW storage and checkpointing)."""
+        generic_domain.py:1715-1912; SWW storage at every outputstep, no
+        checkpointing)."""
         if self.boundary_map is None:
             raise AssertionError("Boundary tags must be bound to boundary objects before "
                                  "evolving system, e.g. using the method set_boundary.\n"
@@ -513,8 +744,20 @@ class Domain:
                                  % self.get_boundary_tags())
         if not (0 <= self.beta_w <= 2.0):
             raise AssertionError("Attribute self.beta_w must be in the interval [0, 2]")
+        if outputstep is None:
+            outputstep = yieldstep
+        if yieldstep is None:
+            self.output_frequency = 1
+        else:
+            freq = outputstep / yieldstep
+            if not float(freq).is_integer():
+                raise AssertionError("outputstep (%s) should be an integer multiple of "
+                                     "yieldstep (%s)" % (outputstep, yieldstep))
+            self.output_frequency = int(freq)
         self._sync_params()
         self.distribute_to_vertices_and_edges()
+        if self.store and (self.get_relative_time() == 0.0 or not self.evolved_called):
+            self.initialise_storage()
         if self.evolved_called:
             skip_initial_step = True
         self.evolved_called = True
@@ -541,7 +784,7 @@ class Domain:
         self.number_of_steps = 0
         if not skip_initial_step:
             self.distribute_to_vertices_and_edges()
-            yield self.get_time()
+            yield self._yield()
         while True:
             if self.device_timestep:
                 reached = self._device_steps(self._chunk_estimate())
@@ -551,16 +794,24 @@ class Domain:
                 self.relative_time = self.relative_finaltime
                 self.distribute_to_vertices_and_edges()
                 self._check_negative_cells()
-                yield self.get_time()
+                yield self._yield()
                 break
             if reached == 2:
                 self.distribute_to_vertices_and_edges()
                 self._check_negative_cells()
-                yield self.get_time()
+                yield self._yield()
                 self.relative_yieldtime += yieldstep
                 self.recorded_min_timestep = self.evolve_max_timestep
                 self.recorded_max_timestep = self.evolve_min_timestep
                 self.number_of_steps = 0
+
+    def _yield(self):
+        """Store the frame of this yield (every output_frequency-th) and
+        return the time to yield."""
+        if self.store and self.yieldstep_counter % self.output_frequency == 0:
+            self.store_timestep()
+        self.yieldstep_counter += 1
+        return self.get_time()
 
     def _host_step(self, yieldstep):
         """One step with update_timestep on the host; returns 1 / 2 / 0 as the
@@ -577,6 +828,12 @@ class Domain:
         if self.record_dt_history:
             self.dt_history.append(self.timestep)
         self.number_of_steps += 1
+        if self._gauges is not None:
+            g = self._gauges
+            self._call("anuga_sample_gauges", C.c_double(self.relative_time))
+            g["pending"] += 1
+            if g["pending"] == g["capacity"]:
+                self._drain_gauges()
         if self.relative_finaltime is not None and \
                 self.relative_time >= self.relative_finaltime - EPSILON:
             if self.relative_time > self.relative_finaltime:
@@ -626,6 +883,7 @@ class Domain:
         if hist is not None:
             self.dt_history.extend(hist[:n].tolist())
         self._check_negative_cells()
+        self._drain_gauges()
         return st.reached
 
     def evolve_steps(self, nsteps, finaltime=None):

@@ -311,3 +311,20 @@ class Mesh:
 
     def get_edge_midpoint_coordinates(self):
         return self.edge_midpoint_coordinates
+
+    def find_triangles(self, points, tol=1.0e-12):
+        """Index of a triangle containing each point (barycentric test with a
+        relative tolerance; the lowest index on shared edges), -1 outside."""
+        pts = np.atleast_2d(np.asarray(points, np.float64))
+        V = self.vertex_coordinates.reshape(-1, 3, 2)
+        a, b, c = V[:, 0], V[:, 1], V[:, 2]
+        det = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (c[:, 0] - a[:, 0]) * (b[:, 1] - a[:, 1])
+        out = np.full(len(pts), -1, np.int64)
+        for i, (x, y) in enumerate(pts):
+            l1 = ((x - a[:, 0]) * (c[:, 1] - a[:, 1]) - (c[:, 0] - a[:, 0]) * (y - a[:, 1])) / det
+            l2 = ((b[:, 0] - a[:, 0]) * (y - a[:, 1]) - (x - a[:, 0]) * (b[:, 1] - a[:, 1])) / det
+            inside = (l1 >= -tol) & (l2 >= -tol) & (l1 + l2 <= 1.0 + tol)
+            hit = np.nonzero(inside)[0]
+            if len(hit):
+                out[i] = hit[0]
+        return out

@@ -130,6 +130,51 @@ int anuga_set_riverwalls(anuga_domain *d, const int64_t *edge_flux_type,
                          const int64_t *riverwall_rowIndex, int64_t nrows, int64_t ncols,
                          const double *hydraulic_properties);
 
+/* ---- Output at yield points ---------------------------------------------------
+ * Node connectivity of the mesh for SWW output: number_of_triangles_per_node
+ * (nodes) and vertex_value_indices (3N, flat vertex index 3k+i, grouped by
+ * node), the arrays of General_mesh.build_inverted_triangle_structure
+ * (general_mesh.py:790-855). The entry order within a node is kept: it is the
+ * summation order of the node average. */
+int anuga_set_output_nodes(anuga_domain *d, int64_t number_of_nodes,
+                           const int64_t *number_of_triangles_per_node,
+                           const int64_t *vertex_value_indices);
+/* One SWW frame of the current centroid state, computed on the device, as
+ * float32 (the file's precision):
+ * node_values [3][nodes]: stage, xmomentum, ymomentum averaged at the nodes
+ *   from the centroids (Quantity.get_vertex_values with centroid averaging,
+ *   quantity.py:2204-2221, quantity.c:872-914), with stage = bed and momenta
+ *   0 where the node depth is below minimum_storable_height
+ *   (SWW_file.store_timestep, anuga/file/sww.py:318-356);
+ * centroid_values [3][N]: stage_c, xmomentum_c, ymomentum_c in the caller's
+ *   order (store_quantities_centroid, sww.py:1037-1091).
+ * Either pointer may be NULL. */
+int anuga_get_sww_frame(anuga_domain *d, double minimum_storable_height, float *node_values,
+                        float *centroid_values);
+/* The static quantities of the file (quantity ELEVATION or FRICTION): node
+ * averages (nodes) and centroid values (N), float32; either may be NULL
+ * (SWW_file.store_connectivity, sww.py:200-258). */
+int anuga_get_static_output(anuga_domain *d, int quantity, float *node_values,
+                            float *centroid_values);
+
+/* ---- Gauges: centroid time series recorded on the device ------------------------
+ * Record the centroid state of `ngauges` triangles (the caller's numbering)
+ * after every device-controlled step into a device buffer of `capacity` rows,
+ * so that sampling every step costs no host round trip; download with
+ * anuga_get_gauge_samples. The reference samples gauges from the stored SWW
+ * frames (sww2csv_gauges with output_centroids, anuga/abstract_2d_finite_volumes/
+ * gauge.py:84-266: the centroid values of the triangle holding each gauge);
+ * this records them at every step. anuga_evolve_steps stops early when the
+ * rows run out. ngauges 0 removes the gauges. */
+int anuga_set_gauges(anuga_domain *d, int64_t ngauges, const int64_t *triangles,
+                     int64_t capacity);
+/* Record one row now, at relative_time (host-sequenced steps). */
+int anuga_sample_gauges(anuga_domain *d, double relative_time);
+/* Download and clear the recorded rows: *nrows rows (at most max_rows) of
+ * times (relative time after the step) and values [row][gauge][w, uh, vh, z]. */
+int anuga_get_gauge_samples(anuga_domain *d, int64_t max_rows, int64_t *nrows, double *times,
+                            double *values);
+
 /* protect + second-order edge extrapolation, with the result written back to
  * the centroid state. Replaces protect_new + extrapolate_second_order_edge_sw
  * (sw_domain_orig_ext.pyx:449-475; sw_domain_orig.c:1215-1281, :1994-2680).
