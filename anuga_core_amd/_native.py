@@ -36,7 +36,8 @@ class AnugaMesh(C.Structure):
                 ("neighbours", _ip), ("neighbour_edges", _ip),
                 ("surrogate_neighbours", _ip), ("number_of_boundaries", _ip),
                 ("normals", _dp), ("edgelengths", _dp), ("radii", _dp), ("areas", _dp),
-                ("centroid_coordinates", _dp), ("edge_coordinates", _dp)]
+                ("centroid_coordinates", _dp), ("edge_coordinates", _dp),
+                ("vertex_coordinates", _dp)]
 
 
 class AnugaParams(C.Structure):
@@ -47,7 +48,7 @@ class AnugaParams(C.Structure):
                 ("beta_vh", C.c_double), ("beta_vh_dry", C.c_double),
                 ("low_froude", C.c_int64), ("extrapolate_velocity_second_order", C.c_int64),
                 ("timestep_fluxcalls", C.c_int64), ("precision", C.c_int64),
-                ("reorder", C.c_int64)]
+                ("reorder", C.c_int64), ("use_sloped_mannings", C.c_int64)]
 
 
 class TimeControl(C.Structure):

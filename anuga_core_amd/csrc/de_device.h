@@ -98,11 +98,18 @@ struct DeMesh {
     // DeRiverwall record or -1
     const int *__restrict__ rw;
     const DeRiverwall *__restrict__ rwd;
+    // the caller's index of each triangle within its scenario (the
+    // reference's loop order, for the low-Froude sums), and the vertex
+    // coordinates [N][6] (sloped friction); read by the general kernels only
+    const int *__restrict__ caller;
+    const double *__restrict__ vx;
 };
 
 struct DeParams {
     double g, eps, mah;
     double bw, bwd, buh, buhd, bvh, bvhd;
+    int low_froude;   // 0, 1, 2 (sw_domain_orig.c:409-425)
+    int sloped;       // sloped Manning friction (sw_domain_orig.c:2840-2888)
 };
 
 struct DeBoundary {
@@ -244,7 +251,8 @@ template <class T>
 __device__ __forceinline__ void de_flux_central(T ql0, T ql1, T ql2, T qr0, T qr1, T qr2,
                                                 T h_left, T h_right, T hle, T hre, T n1, T n2,
                                                 T eps, T ze, T g, T ef[3], T &max_speed,
-                                                T &pressure_flux)
+                                                T &pressure_flux, int low_froude = 0,
+                                                bool mirrored = false)
 {
     if (h_left == T(0.0) && h_right == T(0.0)) {
         ef[0] = ef[1] = ef[2] = T(0.0);
@@ -281,6 +289,24 @@ __device__ __forceinline__ void de_flux_central(T ql0, T ql1, T ql2, T qr0, T qr
     }
     const T cl = sqrt(g * h_left);
     const T cr = sqrt(g * h_right);
+    // local Froude-number scaling of the momentum diffusion
+    // (sw_domain_orig.c:409-425). The reference evaluates an interior edge
+    // once, from its lower-numbered triangle; `mirrored` marks the other
+    // side, which adds the four squares in that triangle's order.
+    T local_fr = T(1.0);
+    if (low_froude) {
+        const T vl = hle > T(0.0) ? l2 * (T(1.0) / hle) : T(0.0);
+        const T vr = hre > T(0.0) ? r2 * (T(1.0) / hre) : T(0.0);
+        const T sq = mirrored ? ul * ul + ur * ur + vl * vl + vr * vr
+                              : ur * ur + ul * ul + vr * vr + vl * vl;
+        const T den = cl * cl + cr * cr + T(1.0e-10);
+        if (low_froude == 1) {
+            local_fr = sqrt(fmax(T(0.001), fmin(T(1.0), sq / den)));
+        } else {
+            local_fr = sqrt(sq / den);
+            local_fr = sqrt(fmin(T(1.0), T(0.01) + fmax(local_fr - T(0.01), T(0.0))));
+        }
+    }
     T s_max = fmax(ul + cl, ur + cr);
     if (s_max < T(0.0)) s_max = T(0.0);
     T s_min = fmin(ul - cl, ur - cr);
@@ -302,10 +328,10 @@ __device__ __forceinline__ void de_flux_central(T ql0, T ql1, T ql2, T qr0, T qr
     e0 += sms * (fmax(qr0, ze) - fmax(ql0, ze));
     e0 *= inv;
     T e1 = s_max * fl1 - s_min * fr1;
-    e1 += T(1.0) * sms * (uhr - uhl);
+    e1 += local_fr * sms * (uhr - uhl);
     e1 *= inv;
     T e2 = s_max * fl2 - s_min * fr2;
-    e2 += T(1.0) * sms * (vhr - vhl);
+    e2 += local_fr * sms * (vhr - vhl);
     e2 *= inv;
     pressure_flux = T(0.5) * g * (s_max * h_left * h_left - s_min * h_right * h_right) * inv;
     // rotate back: _rotate(edgeflux, n1, -n2)

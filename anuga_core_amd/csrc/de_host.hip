@@ -58,6 +58,8 @@ struct anuga_domain {
     double *nx = nullptr, *ny = nullptr, *len = nullptr, *radius = nullptr;  // FP64 flux geometry
     int *scen = nullptr;                 // [nblocks] scenario of each block (S > 1)
     std::vector<int4> hnb;               // host copy of nb
+    int *caller = nullptr;               // [N] caller's index within the scenario (-1: padding)
+    double *vx = nullptr;                // [N][6] vertex coordinates (sloped friction), or NULL
     int *rw = nullptr;                   // [3][N] riverwall record per half-edge, -1: none
     DeRiverwall *rwd = nullptr;          // riverwall records; rw == nullptr: no riverwalls
     // output (one scenario): node CSR of the centroid averaging, in the
@@ -243,14 +245,22 @@ static DeParams dev_params(const anuga_params &p)
     q.buhd = p.beta_uh_dry;
     q.bvh = p.beta_vh;
     q.bvhd = p.beta_vh_dry;
+    q.low_froude = (int)p.low_froude;
+    q.sloped = p.use_sloped_mannings ? 1 : 0;
     return q;
 }
 
 static int check_params(anuga_domain *d, const anuga_params *p)
 {
     if (!p) return fail(d, ANUGA_E_ARG, "params is NULL");
-    if (p->low_froude != 0)
-        return fail(d, ANUGA_E_ARG, "low_froude != 0 is not supported on the device path");
+    if (p->low_froude < 0 || p->low_froude > 2)
+        return fail(d, ANUGA_E_ARG, "low_froude must be 0, 1 or 2");
+    if (p->use_sloped_mannings != 0 && p->use_sloped_mannings != 1)
+        return fail(d, ANUGA_E_ARG, "use_sloped_mannings must be 0 or 1");
+    if (p->precision == ANUGA_PRECISION_MIXED && (p->low_froude || p->use_sloped_mannings))
+        return fail(d, ANUGA_E_ARG, "low_froude and sloped friction need the FP64 path");
+    if (d && p->use_sloped_mannings && !d->vx)
+        return fail(d, ANUGA_E_ARG, "sloped friction needs the mesh's vertex_coordinates");
     if (p->extrapolate_velocity_second_order != 1)
         return fail(d, ANUGA_E_ARG, "extrapolate_velocity_second_order must be 1 (DE algorithms)");
     if (p->timestep_fluxcalls < 1 || p->timestep_fluxcalls > 3)
@@ -329,6 +339,8 @@ static DeMesh dev_mesh(const anuga_domain *d)
     m.radius = d->radius;
     m.rw = d->rw;
     m.rwd = d->rwd;
+    m.caller = d->caller;
+    m.vx = d->vx;
     return m;
 }
 
@@ -362,7 +374,7 @@ static void launch_flux_t(anuga_domain *d, DeQ *S, DeQ *bk, DeQ *out)
 {
     DeBoundary bc{d->bkind, d->bval, d->M};
     if constexpr (sizeof(T) == 8) {
-        if (d->rw)
+        if (d->rw || d->P.low_froude || d->P.use_sloped_mannings)
             hipLaunchKernelGGL((de_flux<double, SUB, true>), dim3(d->nblocks), dim3(DE_BLOCK), 0,
                                d->stream, dev_mesh(d), dev_params(d->P), S, d->eta,
                                (const double *)d->E, bc, d->flags, d->ex[0], d->ex[1],
@@ -410,6 +422,11 @@ static void launch_update(anuga_domain *d, DeQ *S, DeQ *out)
         hipLaunchKernelGGL(de_update<false>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
                            d->scen, dev_params(d->P), S, d->eta, d->flags, d->ex[0], d->ex[1],
                            d->ex[2], d->ctl, out, d->dneg);
+    else if (d->P.use_sloped_mannings)
+        hipLaunchKernelGGL((de_update<true, true>), dim3(d->nblocks), dim3(DE_BLOCK), 0,
+                           d->stream, d->scen, dev_params(d->P), S, d->eta, d->flags, d->ex[0],
+                           d->ex[1], d->ex[2], d->ctl, out, d->dneg, (const double *)d->E, d->N,
+                           d->vx);
     else
         hipLaunchKernelGGL(de_update<true>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
                            d->scen, dev_params(d->P), S, d->eta, d->flags, d->ex[0], d->ex[1],
@@ -833,6 +850,16 @@ static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
                             hipMemcpyHostToDevice));
     }
     std::copy(perm.begin(), perm.end(), d->perm.begin() + kb);
+    if (!d->caller) HIPCHK(d, dalloc(&d->caller, (size_t)d->N));
+    HIPCHK(d, hipMemcpy(d->caller + kb, perm.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    if (mesh->vertex_coordinates && !d->mixed) {
+        std::vector<double> hv(6 * (size_t)N);
+        for (int p = 0; p < N; p++)
+            for (int j = 0; j < 6; j++) hv[6 * (size_t)p + j] = mesh->vertex_coordinates[6 * (size_t)perm[p] + j];
+        if (!d->vx) HIPCHK(d, dalloc(&d->vx, 6 * (size_t)d->N));
+        HIPCHK(d, hipMemcpy(d->vx + 6 * (size_t)kb, hv.data(), 6 * (size_t)N * sizeof(double),
+                            hipMemcpyHostToDevice));
+    }
     if (!d->mesh_set[s]) {
         d->mesh_set[s] = 1;
         d->meshes_missing--;
@@ -958,6 +985,8 @@ static int ready_to_step(anuga_domain *d)
 {
     if (d->meshes_missing) return fail(d, ANUGA_E_STATE, "a scenario's mesh is not set");
     if (d->boundaries_missing) return fail(d, ANUGA_E_STATE, "boundary conditions not set");
+    if (d->P.use_sloped_mannings && !d->vx)
+        return fail(d, ANUGA_E_ARG, "sloped friction needs the mesh's vertex_coordinates");
     return ANUGA_OK;
 }
 
@@ -1210,7 +1239,7 @@ void anuga_destroy(anuga_domain *d)
     if (d->stream) hipStreamSynchronize(d->stream);
     drop_graph(d);
     void *ptrs[] = {d->nb, d->cen, d->gx, d->gf, d->area, d->nx, d->ny, d->len, d->radius,
-                    d->scen, d->rw, d->rwd, d->node_start, d->node_tri, d->iperm_dev,
+                    d->scen, d->rw, d->rwd, d->caller, d->vx, d->node_start, d->node_tri, d->iperm_dev,
                     d->out_f32, d->gtri, d->gtimes, d->gvals, d->eta, d->A, d->B, d->E, d->flags,
                     d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->ctl, d->dsetup,
                     d->dneg, d->hist};
@@ -1890,6 +1919,8 @@ int anuga_adjoint_record(anuga_domain *d, int64_t nsteps, double dt, int64_t che
     if (rc) return rc;
     if (d->mixed) return fail(d, ANUGA_E_ARG, "gradients need the FP64 path");
     if (d->rw) return fail(d, ANUGA_E_ARG, "gradients of runs with riverwalls are not supported");
+    if (d->P.low_froude || d->P.use_sloped_mannings)
+        return fail(d, ANUGA_E_ARG, "gradients need low_froude 0 and flat friction");
     if (d->P.timestep_fluxcalls != 2) return fail(d, ANUGA_E_ARG, "gradients need rk2 (DE1)");
     if (nsteps <= 0 || checkpoint_every <= 0 || obs_every <= 0 || !(dt > 0.0) || ngauges < 0 ||
         (ngauges > 0 && (!gauge_triangles || !gauge_stage)))

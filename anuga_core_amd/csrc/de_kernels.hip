@@ -327,17 +327,54 @@ __device__ __forceinline__ double de_update_one(double q, double ex, double si, 
 }
 
 // CRPOW: correctly rounded pow(h, 7/3) (the FP64 path); else the math library's
+// bed terms of sloped friction for one triangle
+struct DeSlope {
+    bool on;
+    double zs, zmean;
+};
+
+// The vertex beds of the last extrapolation (edge bed z_e = w_e - h_e, vertex
+// = sum of the two other edges minus the opposite one, sw_domain_orig.c:2660-2668)
+// and their plane's gradient (_gradient, anuga/utilities/util_ext.h:52-92).
+__device__ __forceinline__ DeSlope de_sloped_bed(const double *__restrict__ E, int N, int k,
+                                                 const double *__restrict__ vx)
+{
+    const double *Ew = E, *Eh = E + 3 * (size_t)N;
+    double ze[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) ze[i] = Ew[i * N + k] - Eh[i * N + k];
+    const double z0 = ze[1] + ze[2] - ze[0];
+    const double z1 = ze[0] + ze[2] - ze[1];
+    const double z2 = ze[0] + ze[1] - ze[2];
+    const double *x = vx + 6 * (size_t)k;
+    const double x0 = x[0], y0 = x[1], x1 = x[2], y1 = x[3], x2 = x[4], y2 = x[5];
+    const double det = (y2 - y0) * (x1 - x0) - (y1 - y0) * (x2 - x0);
+    double a = (y2 - y0) * (z1 - z0) - (y1 - y0) * (z2 - z0);
+    a /= det;
+    double b = (x1 - x0) * (z2 - z0) - (x2 - x0) * (z1 - z0);
+    b /= det;
+    DeSlope s;
+    s.on = true;
+    s.zs = sqrt(1.0 + a * a + b * b);
+    s.zmean = (z0 + z1 + z2) * (1.0 / 3.0);
+    return s;
+}
+
 template <bool CRPOW>
 __device__ __forceinline__ DeQ de_advance(const DeCentroid c, double eta, double exw, double exuh,
                                           double exvh, double dt, const DeParams &p, int *err,
-                                          unsigned long long *nneg)
+                                          unsigned long long *nneg,
+                                          const DeSlope sl = DeSlope{false, 0.0, 0.0})
 {
-    // _manning_friction_flat with eps = minimum_allowed_height (friction.py:64-72)
+    // _manning_friction_flat with eps = minimum_allowed_height (friction.py:64-72),
+    // or _manning_friction_sloped (sw_domain_orig.c:2840-2888): depth over
+    // the mean vertex bed and the slope factor sqrt(1 + |grad z|^2)
     double si_uh = 0.0, si_vh = 0.0;
     if (eta > p.mah) {
-        const double h = c.w - c.z;
+        const double h = c.w - (sl.on ? sl.zmean : c.z);
         if (h >= p.mah) {
-            double S = -p.g * eta * eta * sqrt((c.uh * c.uh + c.vh * c.vh));
+            double S = sl.on ? -p.g * eta * eta * sl.zs * sqrt((c.uh * c.uh + c.vh * c.vh))
+                             : -p.g * eta * eta * sqrt((c.uh * c.uh + c.vh * c.vh));
             if constexpr (CRPOW) S /= de_pow_7_3(h);
             else S /= pow(h, 7.0 / 3.0);
             si_uh += S * c.uh;
@@ -442,9 +479,10 @@ __device__ __forceinline__ void de_riverwall_weir(const DeRiverwall &rw, double 
 // SUBSTEP 0: writes explicit updates; SUBSTEP > 0: advances with the
 // scenario's dt and writes de_rk_combine<SUBSTEP>(advanced, bk) to out (out
 // may be bk, or q itself: each thread reads its own q and bk before writing).
-// WALLS: the mesh has riverwalls (a separate instantiation, so that the
-// common kernel carries none of their registers).
-template <class T, int SUBSTEP, bool WALLS = false>
+// GENERAL: the instantiation for riverwalls, low_froude 1/2 and sloped
+// friction (runtime switches), so that the common kernel carries none of
+// their registers.
+template <class T, int SUBSTEP, bool GENERAL = false>
 __global__ void __launch_bounds__(DE_BLOCK)
 de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
         const T *__restrict__ E, DeBoundary bc, const unsigned char *__restrict__ flags,
@@ -516,18 +554,26 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
             }
             T z_half = fmax(zl, zr);
             int wall = -1;
-            if constexpr (WALLS) {
+            bool mirrored = false;
+            if constexpr (GENERAL) {
                 // the wall is a thin bed step between the two half-edges
                 // (sw_domain_orig.c:880-893)
-                wall = m.rw[i * N + k];
-                if (wall >= 0) z_half = fmax(m.rwd[wall].elev, z_half);
+                if (m.rw) {
+                    wall = m.rw[i * N + k];
+                    if (wall >= 0) z_half = fmax(m.rwd[wall].elev, z_half);
+                }
+                if (p.low_froude && v[i] >= 0) mirrored = m.caller[v[i] >> 2] < m.caller[k];
             }
             const T h_left = fmax(hle + zl - z_half, T(0.0));
             const T h_right = fmax(hre + zr - z_half, T(0.0));
             T ef[3], speed, pflux;
-            de_flux_central<T>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1, n2,
-                               eps, z_half, g, ef, speed, pflux);
-            if constexpr (WALLS) {
+            if constexpr (GENERAL)
+                de_flux_central<T>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1,
+                                   n2, eps, z_half, g, ef, speed, pflux, p.low_froude, mirrored);
+            else
+                de_flux_central<T>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1,
+                                   n2, eps, z_half, g, ef, speed, pflux);
+            if constexpr (GENERAL) {
                 if (wall >= 0) {
                     double wn = 0.0, zn = c.z;
                     if (v[i] >= 0) {
@@ -566,7 +612,10 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
             exuh[k] = sx;
             exvh[k] = sy;
         } else {
-            const DeQ r = de_advance<sizeof(T) == 8>(c, eta[k], sw, sx, sy, dt, p, err, nneg);
+            DeSlope sl{false, 0.0, 0.0};
+            if constexpr (GENERAL)
+                if (p.sloped) sl = de_sloped_bed((const double *)E, N, k, m.vx);
+            const DeQ r = de_advance<sizeof(T) == 8>(c, eta[k], sw, sx, sy, dt, p, err, nneg, sl);
             out[k] = de_rk_combine<SUBSTEP>(r, bk[k]);
         }
     }
@@ -727,12 +776,13 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
 
 // friction + update + fix_negative_cells from the explicit updates of substep 0;
 // `out` may alias `q` (Euler, in place: each thread reads then writes its own record)
-template <bool CRPOW>
+template <bool CRPOW, bool SLOPED = false>
 __global__ void __launch_bounds__(DE_BLOCK)
 de_update(const int *__restrict__ scen, DeParams p, const DeQ *q, const double *__restrict__ eta,
           const unsigned char *__restrict__ flags, const double *__restrict__ exw,
           const double *__restrict__ exuh, const double *__restrict__ exvh, DeControl *ctl,
-          DeQ *out, unsigned long long *__restrict__ nneg)
+          DeQ *out, unsigned long long *__restrict__ nneg, const double *__restrict__ E = nullptr,
+          int N = 0, const double *__restrict__ vx = nullptr)
 {
     const int lb = xcd_block();
     DeControl *cb = de_ctl(ctl, scen, lb);
@@ -742,7 +792,9 @@ de_update(const int *__restrict__ scen, DeParams p, const DeQ *q, const double *
     const double dt = cb->dt;
     int *err = &cb->status;
     const DeCentroid c = de_centroid(q[k], flags[k]);
-    out[k] = de_advance<CRPOW>(c, eta[k], exw[k], exuh[k], exvh[k], dt, p, err, nneg);
+    DeSlope sl{false, 0.0, 0.0};
+    if constexpr (SLOPED) sl = de_sloped_bed(E, N, k, vx);
+    out[k] = de_advance<CRPOW>(c, eta[k], exw[k], exuh[k], exvh[k], dt, p, err, nneg, sl);
 }
 
 // Fixed-order reduction of the per-block partials into the control blocks:

@@ -263,6 +263,11 @@ DE_POW_FN double de_pow_cr(double x, double y)
     const de_dd kSixth{0x1.5555555555555p-3, 0x1.5555555555555p-57};
     const double kInvLn2_128 = 0x1.71547652b82fep+7;   // 128 / ln 2
 
+    // outside the method's domain (zero, negative, subnormal, infinite, NaN:
+    // a run that has blown up) the library's pow gives the IEEE special
+    // values; the table index below is only valid for normal x > 0
+    if (!(x >= 0x1p-1022 && x <= 0x1.fffffffffffffp+1023)) return pow(x, y);
+
     int e;
     double m = frexp(x, &e);  // m in [0.5, 1)
     m *= 2.0;

@@ -155,6 +155,7 @@ class Domain:
         self.epsilon = EPSILON
         self.H0 = H0
         self.low_froude = 0
+        self.use_sloped_mannings = False
         self.extrapolate_velocity_second_order = 1
         self.evolve_max_timestep = MAX_TIMESTEP
         self.evolve_min_timestep = MIN_TIMESTEP
@@ -221,13 +222,15 @@ class Domain:
             nrm=np.ascontiguousarray(m.normals), el=np.ascontiguousarray(m.edgelengths),
             rad=np.ascontiguousarray(m.radii), area=np.ascontiguousarray(m.areas),
             cen=np.ascontiguousarray(m.centroid_coordinates),
-            em=np.ascontiguousarray(m.edge_midpoint_coordinates))
+            em=np.ascontiguousarray(m.edge_midpoint_coordinates),
+            vx=np.ascontiguousarray(m.vertex_coordinates))
         k = self._mesh_keep
         ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
         desc = _native.AnugaMesh(self.number_of_elements, self.boundary_length,
                                  ip(k["nb"]), ip(k["nbe"]), ip(k["sur"]), ip(k["nob"]),
                                  _dptr(k["nrm"]), _dptr(k["el"]), _dptr(k["rad"]),
-                                 _dptr(k["area"]), _dptr(k["cen"]), _dptr(k["em"]))
+                                 _dptr(k["area"]), _dptr(k["cen"]), _dptr(k["em"]),
+                                 None if self.precision == "mixed" else _dptr(k["vx"]))
         self._timestep_fluxcalls = 1
         self._params = self._make_params(fp64_defaults=True)
         rc = L.anuga_create(C.byref(desc), C.byref(self._params), C.byref(self._handle))
@@ -246,7 +249,8 @@ class Domain:
                                    self.beta_w, self.beta_w_dry, self.beta_uh, self.beta_uh_dry,
                                    self.beta_vh, self.beta_vh_dry, self.low_froude,
                                    self.extrapolate_velocity_second_order,
-                                   self._timestep_fluxcalls, prec, int(self._reorder))
+                                   self._timestep_fluxcalls, prec, int(self._reorder),
+                                   int(self.use_sloped_mannings))
 
     def _sync_params(self):
         p = self._make_params()
@@ -298,6 +302,17 @@ class Domain:
         self.beta_w_dry = self.beta_uh_dry = self.beta_vh_dry = bd
         self.default_order = 2
         self.maximum_allowed_speed = 0.0
+
+    def set_low_froude(self, low_froude=0):
+        """Low-Froude momentum diffusion scaling 0, 1 or 2
+        (shallow_water_domain.py:1349-1357; FP64 path)."""
+        assert low_froude in [0, 1, 2]
+        self.low_froude = low_froude
+
+    def set_sloped_mannings_function(self, flag=True):
+        """Manning friction over the sloped wetted area
+        (shallow_water_domain.py:1159-1169; FP64 path)."""
+        self.use_sloped_mannings = bool(flag)
 
     def get_using_discontinuous_elevation(self):
         return self.flow_algorithm.startswith("DE")

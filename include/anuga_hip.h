@@ -73,6 +73,7 @@ typedef struct anuga_mesh {
     const double *areas;                   /* (N) */
     const double *centroid_coordinates;    /* (N,2) */
     const double *edge_coordinates;        /* (N,6) edge midpoints */
+    const double *vertex_coordinates;      /* (N,6), or NULL; needed by sloped friction */
 } anuga_mesh;
 
 /* Algorithm parameters; the DE presets are in shallow_water_domain.py:583-707. */
@@ -83,11 +84,13 @@ typedef struct anuga_params {
     double beta_w, beta_w_dry;
     double beta_uh, beta_uh_dry;
     double beta_vh, beta_vh_dry;
-    int64_t low_froude;                        /* only 0 (the DE presets) */
+    int64_t low_froude;                        /* 0 (the DE presets), 1 or 2 (FP64 path) */
     int64_t extrapolate_velocity_second_order; /* only 1 (the DE presets) */
     int64_t timestep_fluxcalls;                /* 1 = euler (DE0), 2 = rk2 (DE1), 3 = rk3 (DE2) */
     int64_t precision;                         /* ANUGA_PRECISION_* */
     int64_t reorder;                           /* 1 = Reverse-Cuthill-McKee renumbering */
+    int64_t use_sloped_mannings;               /* 1 = _manning_friction_sloped (FP64 path;
+                                                  needs vertex_coordinates) */
 } anuga_params;
 
 /* Create a handle: uploads the mesh, allocates all device buffers.

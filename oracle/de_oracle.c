@@ -22,6 +22,8 @@
  *   oracle_protect              sw_domain_orig.c:1215-1281
  *   oracle_extrapolate          sw_domain_orig.c:1994-2680 (+ :1286-1333)
  *   oracle_manning_friction_flat sw_domain_orig.c:2805-2838
+ *   oracle_manning_friction_sloped sw_domain_orig.c:2840-2888 (+ _gradient,
+ *                               anuga/utilities/util_ext.h:52-92)
  *   oracle_update               anuga/abstract_2d_finite_volumes/quantity.c:772-822
  *   oracle_fix_negative_cells   sw_domain_orig.c:2890-2910
  *   oracle_saxpy / oracle_backup quantity.c:735-769
@@ -777,6 +779,39 @@ void oracle_manning_friction_flat(int64_t N, double g, double eps,
             double h = w[k] - z[k];
             if (h >= eps) {
                 double S = -g * eta[k] * eta[k] * sqrt((uh[k] * uh[k] + vh[k] * vh[k]));
+                S /= (g_pow_mode == 0) ? pow(h, seven_thirds) : oracle_pow_7_3(h);
+                xmom_si[k] += S * uh[k];
+                ymom_si[k] += S * vh[k];
+            }
+        }
+    }
+}
+
+/* Manning friction with the bed slope factor sqrt(1 + zx^2 + zy^2) of the
+ * plane through the vertex beds and the depth over their mean. */
+void oracle_manning_friction_sloped(int64_t N, double g, double eps, const double *x,
+                                    const double *w, const double *zv,
+                                    const double *uh, const double *vh,
+                                    const double *eta, double *xmom_si, double *ymom_si)
+{
+    const double one_third = 1.0 / 3.0;
+    const double seven_thirds = 7.0 / 3.0;
+    int64_t k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < N; k++) {
+        if (eta[k] > eps) {
+            const double *v = x + 6 * k;
+            double z0 = zv[3 * k], z1 = zv[3 * k + 1], z2 = zv[3 * k + 2];
+            double det = (v[5] - v[1]) * (v[2] - v[0]) - (v[3] - v[1]) * (v[4] - v[0]);
+            double zx = (v[5] - v[1]) * (z1 - z0) - (v[3] - v[1]) * (z2 - z0);
+            double zy = (v[2] - v[0]) * (z2 - z0) - (v[4] - v[0]) * (z1 - z0);
+            double zs, h;
+            zx /= det;
+            zy /= det;
+            zs = sqrt(1.0 + zx * zx + zy * zy);
+            h = w[k] - (z0 + z1 + z2) * one_third;
+            if (h >= eps) {
+                double S = -g * eta[k] * eta[k] * zs * sqrt((uh[k] * uh[k] + vh[k] * vh[k]));
                 S /= (g_pow_mode == 0) ? pow(h, seven_thirds) : oracle_pow_7_3(h);
                 xmom_si[k] += S * uh[k];
                 ymom_si[k] += S * vh[k];

@@ -124,6 +124,8 @@ class OracleKernels:
         lib.oracle_manning_friction_flat.argtypes = [C.c_int64, C.c_double, C.c_double,
                                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         lib.oracle_update.argtypes = [C.c_int64, C.c_double, _dp, _dp, _dp]
+        lib.oracle_manning_friction_sloped.argtypes = [C.c_int64, C.c_double, C.c_double, _dp,
+                                                       _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         lib.oracle_fix_negative_cells.argtypes = [C.c_int64, _ip, _dp, _dp, _dp, _dp]
         lib.oracle_saxpy.argtypes = [C.c_int64, C.c_double, C.c_double, _dp, _dp]
         lib.oracle_backup.argtypes = [C.c_int64, _dp, _dp]
@@ -182,6 +184,13 @@ class OracleKernels:
     def friction(self):
         d = self.d
         self.lib.oracle_set_pow_mode(self.pow_mode)
+        if d.use_sloped_mannings:
+            self.lib.oracle_manning_friction_sloped(d.N, d.g, d.minimum_allowed_height,
+                                                    _d(d.vertex_coordinates), _d(d.stage_c),
+                                                    _d(d.elev_v), _d(d.xmom_c), _d(d.ymom_c),
+                                                    _d(d.friction_c), _d(d.xmom_si),
+                                                    _d(d.ymom_si))
+            return
         self.lib.oracle_manning_friction_flat(d.N, d.g, d.minimum_allowed_height,
                                               _d(d.stage_c), _d(d.elev_c), _d(d.xmom_c),
                                               _d(d.ymom_c), _d(d.friction_c),
@@ -303,6 +312,9 @@ class RefKernels:
         sw._manning_friction_flat.restype = None
         sw._manning_friction_flat.argtypes = [C.c_double, C.c_double, C.c_int64, _dp, _dp,
                                               _dp, _dp, _dp, _dp, _dp]
+        sw._manning_friction_sloped.restype = None
+        sw._manning_friction_sloped.argtypes = [C.c_double, C.c_double, C.c_int64, _dp, _dp,
+                                                _dp, _dp, _dp, _dp, _dp, _dp]
         sw._orig_fix_negative_cells.restype = C.c_int64
         sw._orig_fix_negative_cells.argtypes = [C.c_void_p]
         sw.adjust_edgeflux_with_weir.restype = C.c_double
@@ -439,6 +451,12 @@ class RefKernels:
 
     def friction(self):
         d = self.d
+        if d.use_sloped_mannings:
+            self.sw._manning_friction_sloped(d.g, d.minimum_allowed_height, d.N,
+                                             _d(d.vertex_coordinates), _d(d.stage_c),
+                                             _d(d.elev_v), _d(d.xmom_c), _d(d.ymom_c),
+                                             _d(d.friction_c), _d(d.xmom_si), _d(d.ymom_si))
+            return
         self.sw._manning_friction_flat(d.g, d.minimum_allowed_height, d.N, _d(d.stage_c),
                                        _d(d.elev_c), _d(d.xmom_c), _d(d.ymom_c),
                                        _d(d.friction_c), _d(d.xmom_si), _d(d.ymom_si))

@@ -75,6 +75,7 @@ class OracleDomain:
         self.epsilon = EPSILON
         self.H0 = 1.0e-3
         self.low_froude = 0
+        self.use_sloped_mannings = False
         self.extrapolate_velocity_second_order = 1
         self.evolve_max_timestep = MAX_TIMESTEP
         self.evolve_min_timestep = MIN_TIMESTEP

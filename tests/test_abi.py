@@ -69,9 +69,14 @@ def test_create_rejects_bad_arguments_before_touching_the_device():
     desc = _native.AnugaMesh(m.number_of_triangles, m.boundary_length, ip(keep[0]), ip(keep[1]),
                              ip(keep[2]), ip(keep[3]), dp(keep[4]), dp(keep[5]), dp(keep[6]),
                              dp(keep[7]), dp(keep[8]), dp(keep[9]))
-    p = _native.AnugaParams(9.8, 1e-12, 1e-5, 1, 0, 1, 0, 1, 0, 1, 1, 2, 0, 1)  # low_froude 1
+    p = _native.AnugaParams(9.8, 1e-12, 1e-5, 1, 0, 1, 0, 1, 0, 3, 1, 2, 0, 1)  # low_froude 3
     assert L.anuga_create(C.byref(desc), C.byref(p), C.byref(h)) == _native.ANUGA_E_ARG
     assert b"low_froude" in L.anuga_last_error(None)
+    p.low_froude = 1
+    p.precision = _native.PRECISION_MIXED     # low_froude needs the FP64 path
+    assert L.anuga_create(C.byref(desc), C.byref(p), C.byref(h)) == _native.ANUGA_E_ARG
+    assert b"FP64" in L.anuga_last_error(None)
+    p.precision = _native.PRECISION_FP64
     p.low_froude = 0
     keep[3][0] += 1   # inconsistent boundary count
     assert L.anuga_create(C.byref(desc), C.byref(p), C.byref(h)) == _native.ANUGA_E_ARG

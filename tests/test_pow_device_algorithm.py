@@ -66,3 +66,16 @@ def test_correct_rounding_against_decimal(pow73):
     for x, r in zip(xs, y):
         exact = (e * Decimal(float(x)).ln()).exp()
         assert abs(Decimal(float(r)) - exact) <= Decimal(math.ulp(float(r))) / 2, x
+
+
+def test_special_values_stay_in_bounds(pow73):
+    """Inputs outside the table method's domain (a run that has blown up:
+    inf, NaN; zero, negative, subnormal) return the IEEE special values of
+    the library pow instead of indexing the tables with a garbage index."""
+    xs = np.array([np.inf, np.nan, 0.0, -0.0, -1.0, -np.inf, 5e-324, 1e-310,
+                   np.finfo(np.float64).max])
+    y = pow73(xs)
+    with np.errstate(all="ignore"):
+        ref = np.power(xs, 7.0 / 3.0)
+    assert np.array_equal(y[:6], ref[:6], equal_nan=True)
+    assert y[6] == 0.0 and y[7] == ref[7] and y[8] == np.inf
