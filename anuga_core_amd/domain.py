@@ -498,8 +498,9 @@ class Domain:
             names = ["gauge%d" % i for i in range(len(tri))]
         self._call("anuga_set_gauges", len(tri), tri.ctypes.data_as(C.POINTER(C.c_int64)),
                    int(capacity))
-        self._gauges = dict(triangles=tri, names=list(names), capacity=int(capacity),
-                            times=[], values=[], pending=0)
+        self._gauges = None if len(tri) == 0 else dict(
+            triangles=tri, names=list(names), capacity=int(capacity), times=[], values=[],
+            pending=0)
 
     def _drain_gauges(self):
         g = self._gauges
