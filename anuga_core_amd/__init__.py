@@ -8,12 +8,19 @@
         print(domain.get_quantity('stage').centroid_values.max())
 """
 
-from .boundaries import Dirichlet_boundary, Reflective_boundary, Transmissive_boundary
+from .boundaries import (Dirichlet_boundary, Reflective_boundary, Time_boundary,
+                         Time_stage_zero_momentum_boundary, Transmissive_boundary,
+                         Transmissive_momentum_set_stage_boundary,
+                         Transmissive_n_momentum_zero_t_momentum_set_stage_boundary,
+                         Transmissive_stage_zero_momentum_boundary)
 from .mesh import Mesh, rectangular, rectangular_cross
 
 __all__ = ["Domain", "Ensemble", "Mesh", "rectangular", "rectangular_cross",
            "rectangular_cross_domain", "rectangular_domain", "Reflective_boundary",
-           "Transmissive_boundary", "Dirichlet_boundary"]
+           "Transmissive_boundary", "Dirichlet_boundary", "Time_boundary",
+           "Time_stage_zero_momentum_boundary", "Transmissive_momentum_set_stage_boundary",
+           "Transmissive_n_momentum_zero_t_momentum_set_stage_boundary",
+           "Transmissive_stage_zero_momentum_boundary"]
 
 
 def __getattr__(name):

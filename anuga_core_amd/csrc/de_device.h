@@ -114,9 +114,19 @@ struct DeParams {
 
 struct DeBoundary {
     const signed char *kind;  // [M]
-    const double *val;        // [3][M] Dirichlet stage, xmom, ymom
+    const double *val;        // [slot][3][M] boundary stage, xmom, ymom
     int M;
+    long long sstride;        // 0: one slot for every substep; 3M: a slot per substep time
 };
+
+// boundary-value slot of a flux substep: its time is t (substep 0), t + dt
+// (rk2's second, rk3's second) or t + dt/2 (rk3's third), the times at which
+// the reference evaluates time-dependent boundaries (generic_domain.py:1974-2179)
+template <int SUBSTEP>
+__device__ __forceinline__ int de_bc_slot()
+{
+    return SUBSTEP == 0 ? 0 : (SUBSTEP == 3 ? 2 : 1);
+}
 
 // Device-resident step control: reduction results, the timestep, the
 // relative time and the update_timestep state, so that a sequence of steps

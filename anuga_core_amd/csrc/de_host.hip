@@ -81,7 +81,9 @@ struct anuga_domain {
     unsigned char *flags = nullptr;
     double *ex[3] = {nullptr, nullptr, nullptr};
     signed char *bkind = nullptr;
-    double *bval = nullptr;
+    double *bval = nullptr;             // [3 slots][3][M]
+    long long bsstride = 0;             // 0, or 3M once per-substep boundary values are set
+    std::vector<signed char> hbkind;    // host copy of bkind
     double *part = nullptr;   // [3][nblocks]: dt, bflux, mass
     DeControl *ctl = nullptr; // [S] device step control
     unsigned long long *dneg = nullptr;
@@ -372,7 +374,7 @@ static void launch_materialize(anuga_domain *d, DeQ *S)
 template <class T, int SUB>
 static void launch_flux_t(anuga_domain *d, DeQ *S, DeQ *bk, DeQ *out)
 {
-    DeBoundary bc{d->bkind, d->bval, d->M};
+    DeBoundary bc{d->bkind, d->bval, d->M, d->bsstride};
     if constexpr (sizeof(T) == 8) {
         if (d->rw || d->P.low_froude || d->P.use_sloped_mannings)
             hipLaunchKernelGGL((de_flux<double, SUB, true>), dim3(d->nblocks), dim3(DE_BLOCK), 0,
@@ -634,7 +636,7 @@ static int domain_alloc(const anuga_params *params, int64_t S64, const int64_t *
     }
     DCHK(dalloc(&d->flags, N));
     DCHK(dalloc(&d->bkind, M));
-    DCHK(dalloc(&d->bval, 3 * (size_t)M));
+    DCHK(dalloc(&d->bval, 9 * (size_t)M));   // three substep slots
     DCHK(dalloc(&d->part, 3 * (size_t)d->nblocks));
     DCHK(dalloc(&d->ctl, S));
     DCHK(dalloc(&d->dsetup, S));
@@ -673,7 +675,7 @@ static int domain_alloc(const anuga_params *params, int64_t S64, const int64_t *
     DCHK(hipMemset(d->E, 0, d->mixed ? N * sizeof(DeEdgeRec32) : 4 * n3 * sizeof(double)));
     DCHK(hipMemset(d->flags, DE_F_KEEP, N));
     DCHK(hipMemset(d->bkind, 0, std::max(M, 1)));
-    DCHK(hipMemset(d->bval, 0, 3 * (size_t)std::max(M, 1) * sizeof(double)));
+    DCHK(hipMemset(d->bval, 0, 9 * (size_t)std::max(M, 1) * sizeof(double)));
     DCHK(hipMemset(d->part, 0, 3 * (size_t)d->nblocks * sizeof(double)));
     DCHK(hipMemset(d->dneg, 0, sizeof(unsigned long long)));
     std::memset(d->hctl, 0, S * sizeof(DeControl));
@@ -962,18 +964,25 @@ static int set_boundary(anuga_domain *d, int s, const int32_t *kind, const doubl
     std::vector<signed char> k(M, 0);
     std::vector<double> v(3 * (size_t)M, 0.0);
     for (int j = 0; j < M; j++) {
-        if (kind[j] < 0 || kind[j] > 3) return fail(d, ANUGA_E_ARG, "unknown boundary kind");
+        if (kind[j] < 0 || kind[j] > 6) return fail(d, ANUGA_E_ARG, "unknown boundary kind");
         k[j] = (signed char)kind[j];
-        if (kind[j] == ANUGA_BC_DIRICHLET) {
-            if (!dirichlet_values) return fail(d, ANUGA_E_ARG, "Dirichlet values are NULL");
+        if (kind[j] == ANUGA_BC_DIRICHLET || kind[j] == ANUGA_BC_SET_STAGE_NORMAL_MOMENTUM ||
+            kind[j] == ANUGA_BC_SET_STAGE_TRANSMISSIVE_MOMENTUM) {
+            if (!dirichlet_values) return fail(d, ANUGA_E_ARG, "boundary values are NULL");
             for (int q = 0; q < 3; q++) v[(size_t)q * M + j] = dirichlet_values[3 * (size_t)j + q];
         }
     }
     HIPCHK(d, hipStreamSynchronize(d->stream));
     HIPCHK(d, hipMemcpy(d->bkind + mb, k.data(), M, hipMemcpyHostToDevice));
-    for (int q = 0; q < 3; q++)
-        HIPCHK(d, hipMemcpy(d->bval + (size_t)q * MT + mb, v.data() + (size_t)q * M,
-                            M * sizeof(double), hipMemcpyHostToDevice));
+    if (d->hbkind.empty()) d->hbkind.assign(std::max(MT, 1), 0);
+    std::copy(k.begin(), k.end(), d->hbkind.begin() + mb);
+    for (int slot = 0; slot < 3; slot++)
+        for (int q = 0; q < 3; q++)
+            HIPCHK(d, hipMemcpy(d->bval + ((size_t)slot * 3 + q) * MT + mb,
+                                v.data() + (size_t)q * M, M * sizeof(double),
+                                hipMemcpyHostToDevice));
+    if (d->bsstride) drop_graph(d);
+    d->bsstride = 0;
     if (!d->bnd_set[s]) {
         d->bnd_set[s] = 1;
         d->boundaries_missing--;
@@ -1005,6 +1014,9 @@ static int evolve_steps(anuga_domain *d, int64_t max_steps, const anuga_time_con
     if (d->substate != 0) return fail(d, ANUGA_E_STATE, "evolve_steps called inside an rk2 step");
     int rc = ready_to_step(d);
     if (rc) return rc;
+    if (d->bsstride)
+        return fail(d, ANUGA_E_STATE, "boundary values set per substep time need host-sequenced "
+                                      "steps (anuga_compute_fluxes / anuga_update_...)");
     if (d->G) {   // every step records a gauge row: stop where the rows run out
         if (d->grows_host >= d->gcap)
             return fail(d, ANUGA_E_STATE, "gauge rows full: download them first");
@@ -1147,7 +1159,7 @@ static void adj_stages(anuga_domain *d, AdjRun *a, const DeQ *X)
 {
     const dim3 g(d->nblocks), b(DE_BLOCK);
     const size_t N = d->N;
-    DeBoundary bc{d->bkind, d->bval, d->M};
+    DeBoundary bc{d->bkind, d->bval, d->M, 0};
     const DeParams p = dev_params(d->P);
     double *pm = d->part + 2 * (size_t)d->nblocks;
     hipLaunchKernelGGL(de_extrapolate<double>, g, b, 0, d->stream, dev_mesh(d), p, X, a->E0, a->f0,
@@ -1348,6 +1360,28 @@ int anuga_set_boundary(anuga_domain *d, const int32_t *kind, const double *diric
     if (!d) return ANUGA_E_ARG;
     int rc = single(d);
     return rc ? rc : set_boundary(d, 0, kind, dirichlet_values);
+}
+
+int anuga_set_boundary_values(anuga_domain *d, int64_t slot, const double *values)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    const int M = d->M;
+    if (slot < 0 || slot > 2) return fail(d, ANUGA_E_ARG, "slot must be 0, 1 or 2");
+    if (M > 0 && !values) return fail(d, ANUGA_E_ARG, "values is NULL");
+    if (M == 0) return ANUGA_OK;
+    std::vector<double> v(3 * (size_t)M);
+    for (int j = 0; j < M; j++)
+        for (int q = 0; q < 3; q++) v[(size_t)q * M + j] = values[3 * (size_t)j + q];
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    HIPCHK(d, hipMemcpy(d->bval + (size_t)slot * 3 * M, v.data(), 3 * (size_t)M * sizeof(double),
+                        hipMemcpyHostToDevice));
+    if (slot > 0 && d->bsstride == 0) {
+        d->bsstride = 3 * (long long)M;
+        drop_graph(d);
+    }
+    return ANUGA_OK;
 }
 
 int anuga_set_riverwalls(anuga_domain *d, const int64_t *edge_flux_type,
@@ -1921,6 +1955,11 @@ int anuga_adjoint_record(anuga_domain *d, int64_t nsteps, double dt, int64_t che
     if (d->rw) return fail(d, ANUGA_E_ARG, "gradients of runs with riverwalls are not supported");
     if (d->P.low_froude || d->P.use_sloped_mannings)
         return fail(d, ANUGA_E_ARG, "gradients need low_froude 0 and flat friction");
+    for (signed char kb : d->hbkind)
+        if (kb > ANUGA_BC_TRANSMISSIVE_CENTROID)
+            return fail(d, ANUGA_E_ARG, "gradients support reflective, transmissive and "
+                                        "Dirichlet boundaries only");
+    if (d->bsstride) return fail(d, ANUGA_E_ARG, "gradients need constant boundary values");
     if (d->P.timestep_fluxcalls != 2) return fail(d, ANUGA_E_ARG, "gradients need rk2 (DE1)");
     if (nsteps <= 0 || checkpoint_every <= 0 || obs_every <= 0 || !(dt > 0.0) || ngauges < 0 ||
         (ngauges > 0 && (!gauge_triangles || !gauge_stage)))

@@ -530,6 +530,7 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
             } else {
                 const int j = -v[i] - 1;
                 const int kind = bc.kind[j];
+                const double *bv = bc.val + (size_t)de_bc_slot<SUBSTEP>() * bc.sstride;
                 if (kind == 0) {          // reflective
                     const T r1 = -ql1 * n1 - ql2 * n2;
                     const T r2 = -ql1 * n2 + ql2 * n1;
@@ -540,14 +541,27 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
                     qr0 = ql0;
                     qr1 = ql1;
                     qr2 = ql2;
-                } else if (kind == 2) {   // Dirichlet
-                    qr0 = T(bc.val[j]);
-                    qr1 = T(bc.val[bc.M + j]);
-                    qr2 = T(bc.val[2 * bc.M + j]);
-                } else {                  // transmissive from centroid values
+                } else if (kind == 2) {   // Dirichlet (constant or time-dependent)
+                    qr0 = T(bv[j]);
+                    qr1 = T(bv[bc.M + j]);
+                    qr2 = T(bv[2 * bc.M + j]);
+                } else if (kind == 3) {   // transmissive from centroid values
                     qr0 = T(c.w);
                     qr1 = T(c.uh);
                     qr2 = T(c.vh);
+                } else if (kind == 4) {   // stage set, normal momentum kept, tangential 0
+                    const T nq = n1 * ql1 + n2 * ql2;
+                    qr0 = T(bv[j]);
+                    qr1 = nq * n1;
+                    qr2 = nq * n2;
+                } else if (kind == 5) {   // transmissive stage, zero momentum
+                    qr0 = ql0;
+                    qr1 = T(0.0);
+                    qr2 = T(0.0);
+                } else {                  // stage set, momentum transmissive
+                    qr0 = T(bv[j]);
+                    qr1 = ql1;
+                    qr2 = ql2;
                 }
                 zr = zl;
                 hre = fmax(qr0 - zr, T(0.0));
@@ -697,6 +711,7 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
             } else {
                 const int j = -v[i] - 1;
                 const int kind = bc.kind[j];
+                const double *bv = bc.val + (size_t)de_bc_slot<SUBSTEP>() * bc.sstride;
                 float wr;                                   // stage above z_c
                 if (kind == 0) {          // reflective
                     const float r1 = -ql1 * n1 - ql2 * n2;
@@ -709,16 +724,29 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
                     qr1 = ql1;
                     qr2 = ql2;
                 } else if (kind == 2) {   // Dirichlet
-                    wr = float(bc.val[j] - c.z);
-                    qr1 = float(bc.val[bc.M + j]);
-                    qr2 = float(bc.val[2 * bc.M + j]);
-                } else {                  // transmissive from centroid values
+                    wr = float(bv[j] - c.z);
+                    qr1 = float(bv[bc.M + j]);
+                    qr2 = float(bv[2 * bc.M + j]);
+                } else if (kind == 3) {   // transmissive from centroid values
                     wr = float(c.w - c.z);
                     qr1 = float(c.uh);
                     qr2 = float(c.vh);
+                } else if (kind == 4) {   // stage set, normal momentum kept, tangential 0
+                    const float nq = n1 * ql1 + n2 * ql2;
+                    wr = float(bv[j] - c.z);
+                    qr1 = nq * n1;
+                    qr2 = nq * n2;
+                } else if (kind == 5) {   // transmissive stage, zero momentum
+                    wr = zl + hle;
+                    qr1 = 0.f;
+                    qr2 = 0.f;
+                } else {                  // stage set, momentum transmissive
+                    wr = float(bv[j] - c.z);
+                    qr1 = ql1;
+                    qr2 = ql2;
                 }
                 zr = zl;
-                hre = (kind <= 1) ? fmax(hle, 0.f) : fmax(wr - zr, 0.f);
+                hre = (kind <= 1 || kind == 5) ? fmax(hle, 0.f) : fmax(wr - zr, 0.f);
             }
             const float z_half = fmax(zl, zr);
             const float hl_raw = hle + (zl - z_half);

@@ -209,6 +209,8 @@ class Domain:
         self.writer = None
         self._output_nodes = False
         self._gauges = None
+        self._time_bc = []
+        self._bc_values = None
 
     # -- device handle -----------------------------------------------------------
     def _create_device(self):
@@ -647,6 +649,7 @@ class Domain:
         M = self.boundary_length
         kind = np.zeros(max(M, 1), np.int32)
         vals = np.zeros((max(M, 1), 3))
+        time_bc = []
         for tag, B in boundary_map.items():
             if B is None:
                 continue
@@ -657,10 +660,25 @@ class Domain:
             if code == 1 and self.centroid_transmissive_bc:
                 code = 3
             kind[ids] = code
-            if code == 2:
+            if getattr(B, "time_dependent", False):
+                time_bc.append((B, ids))
+                vals[ids] = B.values(self.get_time())
+            elif code == 2:
                 vals[ids] = B.dirichlet_values[:3]
         self.boundary_map = dict(boundary_map)
+        self._bc_values = vals
+        self._time_bc = time_bc
         self._call("anuga_set_boundary", kind.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(vals))
+
+    def _update_boundary_slot(self, slot, t):
+        """Time-dependent boundary values at time t for one substep slot
+        (0: the step's start, 1: t + dt, 2: t + dt/2)."""
+        if not self._time_bc:
+            return
+        vals = self._bc_values
+        for B, ids in self._time_bc:
+            vals[ids] = B.values(t)
+        self._call("anuga_set_boundary_values", slot, _dptr(vals))
 
     # -- step -------------------------------------------------------------------------
     def distribute_to_vertices_and_edges(self):
@@ -720,23 +738,30 @@ class Domain:
 
     def evolve_one_euler_step(self, yieldstep, finaltime):
         """generic_domain.py:1914-1972"""
+        self._update_boundary_slot(0, self.get_time())
         self._compute_fluxes()
         self.update_timestep(yieldstep, finaltime)
         self._call("anuga_update_conserved_quantities", C.c_double(self.timestep), None)
 
     def evolve_one_rk2_step(self, yieldstep, finaltime):
         """generic_domain.py:1974-2051; backup is the untouched step state."""
+        self._update_boundary_slot(0, self.get_time())
         self._compute_fluxes()
         self.update_timestep(yieldstep, finaltime)
         self._call("anuga_update_conserved_quantities", C.c_double(self.timestep), None)
         self.relative_time = self.relative_time + self.timestep
+        self._update_boundary_slot(1, self.get_time())
         self._call("anuga_rk2_second_substep", C.c_double(self.timestep), None)
 
     def evolve_one_rk3_step(self, yieldstep, finaltime):
         """generic_domain.py:2053-2179; the backup is the untouched step state."""
+        initial = self.relative_time
+        self._update_boundary_slot(0, self.get_time())
         self._compute_fluxes()
         self.update_timestep(yieldstep, finaltime)
         self._call("anuga_update_conserved_quantities", C.c_double(self.timestep), None)
+        self._update_boundary_slot(1, self.starttime + (initial + self.timestep))
+        self._update_boundary_slot(2, self.starttime + (initial + self.timestep * 0.5))
         self._call("anuga_rk3_later_substeps", C.c_double(self.timestep), None)
 
     def get_time(self):
@@ -802,7 +827,7 @@ class Domain:
             self.distribute_to_vertices_and_edges()
             yield self._yield()
         while True:
-            if self.device_timestep:
+            if self.device_timestep and not self._time_bc:
                 reached = self._device_steps(self._chunk_estimate())
             else:
                 reached = self._host_step(yieldstep)
@@ -916,7 +941,7 @@ class Domain:
         self.recorded_max_timestep = self.evolve_min_timestep
         done = 0
         while done < nsteps:
-            if self.device_timestep:
+            if self.device_timestep and not self._time_bc:
                 before = self.number_of_steps
                 reached = self._device_steps(nsteps - done)
                 done += self.number_of_steps - before

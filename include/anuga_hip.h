@@ -50,6 +50,11 @@ extern "C" {
 #define ANUGA_BC_TRANSMISSIVE 1       /* generic_boundary_conditions.py:173-195 (edge values) */
 #define ANUGA_BC_DIRICHLET 2          /* generic_boundary_conditions.py:221-264 */
 #define ANUGA_BC_TRANSMISSIVE_CENTROID 3 /* transmissive with centroid_transmissive_bc */
+#define ANUGA_BC_SET_STAGE_NORMAL_MOMENTUM 4   /* stage from the values, normal momentum kept,
+                                                  tangential zero (boundaries.py:384-517) */
+#define ANUGA_BC_TRANSMISSIVE_STAGE_ZERO_MOMENTUM 5 /* boundaries.py:520-550 */
+#define ANUGA_BC_SET_STAGE_TRANSMISSIVE_MOMENTUM 6 /* stage from the values, momentum kept
+                                                     (boundaries.py:295-381) */
 
 /* arithmetic of the step */
 #define ANUGA_PRECISION_FP64 0   /* everything FP64, bit-comparable with the reference */
@@ -115,6 +120,15 @@ int anuga_get_edge_values(anuga_domain *d, int quantity, double *values);
  * Replaces Generic_Domain.update_boundary (generic_domain.py:2288-2306): the
  * boundary state is evaluated on the device inside the flux kernel. */
 int anuga_set_boundary(anuga_domain *d, const int32_t *kind, const double *dirichlet_values);
+/* Time-dependent boundary values (M,3) for one substep time: slot 0 at the
+ * step's start t, slot 1 at t + dt (the second substep of rk2 and rk3), slot
+ * 2 at t + dt/2 (rk3's third), the times at which the reference's
+ * update_boundary evaluates Time_boundary & co. (generic_domain.py:1974-2179,
+ * generic_boundary_conditions.py:297-417). Values are read by the kinds that
+ * take them (DIRICHLET, SET_STAGE_*). Once a slot above 0 is set, each
+ * substep reads its own slot, and device-controlled step runs
+ * (anuga_evolve_steps) are refused until anuga_set_boundary is called again. */
+int anuga_set_boundary_values(anuga_domain *d, int64_t slot, const double *values);
 
 /* Riverwalls (FP64 path), in the layout the reference's flux loop reads
  * (struct domain fields edge_flux_type, riverwall_elevation,

@@ -36,6 +36,14 @@ MIN_TIMESTEP = 1.0e-6      # anuga/config.py:153
 MAX_SMALLSTEPS = 50        # anuga/config.py:155
 
 
+def _first(value):
+    """float(value), or float(value[0]) for a vector (boundaries.py:502-506)."""
+    try:
+        return float(value)
+    except TypeError:
+        return float(value[0])
+
+
 class OracleDomain:
     def __init__(self, mesh, kernels, flow_algorithm="DE0"):
         self.mesh = mesh
@@ -209,6 +217,39 @@ class OracleDomain:
                 self.stage_b[ids] = v[0]
                 self.xmom_b[ids] = v[1]
                 self.ymom_b[ids] = v[2]
+            elif B.kind == "time":
+                # Time_boundary.evaluate_segment (generic_boundary_conditions.py:383-417)
+                v = np.asarray(B.function(self.get_time()), np.float64)
+                self.stage_b[ids] = v[0]
+                self.xmom_b[ids] = v[1]
+                self.ymom_b[ids] = v[2]
+            elif B.kind == "time_stage_zero_momentum":
+                # boundaries.py:620-636
+                self.stage_b[ids] = _first(B.function(self.get_time()))
+                self.xmom_b[ids] = 0.0
+                self.ymom_b[ids] = 0.0
+            elif B.kind == "transmissive_n_momentum_zero_t_momentum_set_stage":
+                # boundaries.py:474-517
+                n1 = nrm[vol, 2 * edge]
+                n2 = nrm[vol, 2 * edge + 1]
+                self.stage_b[ids] = _first(B.function(self.get_time()))
+                ndotq = n1 * xe[vol, edge] + n2 * ye[vol, edge]
+                self.xmom_b[ids] = ndotq * n1
+                self.ymom_b[ids] = ndotq * n2
+            elif B.kind == "transmissive_momentum_set_stage":
+                # boundaries.py:344-373 (time rolled over past the function's range)
+                t = self.get_time()
+                if hasattr(B.function, "time"):
+                    while t > B.function.time[-1]:
+                        t -= B.function.time[-1]
+                self.stage_b[ids] = _first(B.function(t))
+                self.xmom_b[ids] = xe[vol, edge]
+                self.ymom_b[ids] = ye[vol, edge]
+            elif B.kind == "transmissive_stage_zero_momentum":
+                # boundaries.py:542-550
+                self.stage_b[ids] = se[vol, edge]
+                self.xmom_b[ids] = 0.0
+                self.ymom_b[ids] = 0.0
             else:
                 raise ValueError("unsupported boundary kind %r" % B.kind)
 
