@@ -80,6 +80,12 @@ SIGNATURES = {
     "anuga_set_riverwalls": (C.c_int, [C.c_void_p, _ip, C.c_int64, _dp, _ip, C.c_int64,
                                        C.c_int64, _dp]),
     "anuga_set_boundary_values": (C.c_int, [C.c_void_p, C.c_int64, _dp]),
+    "anuga_add_rate_operator": (C.c_int, [C.c_void_p, C.c_int64, _ip, _dp, C.c_double,
+                                          C.c_double, _ip]),
+    "anuga_update_rate_operator": (C.c_int, [C.c_void_p, C.c_int64, _dp, C.c_double,
+                                             C.c_double]),
+    "anuga_apply_operators": (C.c_int, [C.c_void_p]),
+    "anuga_get_operator_influx": (C.c_int, [C.c_void_p, C.c_int64, _dp]),
     "anuga_set_output_nodes": (C.c_int, [C.c_void_p, C.c_int64, _ip, _ip]),
     "anuga_get_sww_frame": (C.c_int, [C.c_void_p, C.c_double, _fp, _fp]),
     "anuga_get_static_output": (C.c_int, [C.c_void_p, C.c_int, _fp, _fp]),

@@ -39,6 +39,16 @@ const int kTimerRing = 64;
 
 }  // namespace
 
+// a Rate_operator's device data
+struct RateOp {
+    int n = 0;
+    int *idx = nullptr;      // [n] internal triangles
+    double *rate = nullptr;  // [n] rates, or NULL: the scalar rate0
+    double rate0 = 0.0, factor = 1.0;
+    int careful = 0;         // some rate < 0
+    double *influx = nullptr;
+};
+
 struct anuga_domain {
     int N = 0, M = 0, nblocks = 0;   // N: internal triangles incl. padding; M: boundary edges
     int S = 1;                       // scenarios
@@ -72,6 +82,7 @@ struct anuga_domain {
     int64_t grows_host = 0;              // rows on the device not yet downloaded
     int *gtri = nullptr;
     double *gtimes = nullptr, *gvals = nullptr;
+    std::vector<RateOp> ops;             // fractional-step rate operators, in order
     bool mixed = false;
     // quantities
     double *eta = nullptr;
@@ -486,6 +497,16 @@ static void enqueue_later_substeps(anuga_domain *d)
     }
 }
 
+// fractional-step operators after a step (apply_fractional_steps,
+// generic_domain.py:1849, :2312-2314), in the order they were added
+static void launch_operators(anuga_domain *d)
+{
+    for (const RateOp &o : d->ops)
+        hipLaunchKernelGGL(de_rate, dim3((o.n + DE_BLOCK - 1) / DE_BLOCK), dim3(DE_BLOCK), 0,
+                           d->stream, d->ctl, o.n, o.idx, o.rate, o.rate0, o.factor, o.careful,
+                           d->A, d->area, o.influx);
+}
+
 static void enqueue_device_step(anuga_domain *d)
 {
     const bool rk = d->P.timestep_fluxcalls > 1;
@@ -495,6 +516,7 @@ static void enqueue_device_step(anuga_domain *d)
     launch_control(d, de_timestep);
     launch_update(d, d->A, rk ? d->B : d->A);
     enqueue_later_substeps(d);
+    launch_operators(d);
     hipLaunchKernelGGL(de_finish_step, scenario_grid(d), dim3(64), 0, d->stream, d->ctl, d->S,
                        d->S == 1 ? d->hist : nullptr);
     if (d->G)
@@ -1265,6 +1287,11 @@ void anuga_destroy(anuga_domain *d)
         for (auto e : d->timers[t].stop) hipEventDestroy(e);
     }
     delete d->adj;
+    for (RateOp &o : d->ops) {
+        void *q[] = {o.idx, o.rate, o.influx};
+        for (void *x : q)
+            if (x) hipFree(x);
+    }
     if (d->stream) hipStreamDestroy(d->stream);
     delete d;
 }
@@ -1381,6 +1408,91 @@ int anuga_set_boundary_values(anuga_domain *d, int64_t slot, const double *value
         d->bsstride = 3 * (long long)M;
         drop_graph(d);
     }
+    return ANUGA_OK;
+}
+
+static int upload_rate(anuga_domain *d, RateOp &o, const double *rates, double rate,
+                       double factor)
+{
+    o.rate0 = rate;
+    o.factor = factor;
+    if (rates) {
+        if (!o.rate) HIPCHK(d, dalloc(&o.rate, (size_t)o.n));
+        HIPCHK(d, hipMemcpy(o.rate, rates, o.n * sizeof(double), hipMemcpyHostToDevice));
+        o.careful = 0;
+        for (int i = 0; i < o.n; i++) o.careful |= rates[i] < 0.0;
+    } else {
+        if (o.rate) HIPCHK(d, hipFree(o.rate));
+        o.rate = nullptr;
+        o.careful = rate < 0.0;
+    }
+    return ANUGA_OK;
+}
+
+int anuga_add_rate_operator(anuga_domain *d, int64_t n, const int64_t *indices,
+                            const double *rates, double rate, double factor, int64_t *id)
+{
+    if (!d || !id) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    const int N = d->ntri[0];
+    if (indices && (n < 0 || n > N)) return fail(d, ANUGA_E_ARG, "bad operator triangle count");
+    RateOp o;
+    o.n = indices ? (int)n : N;
+    std::vector<int> ip(N), idx(std::max(o.n, 1));
+    for (int p = 0; p < N; p++) ip[d->perm[p]] = p;
+    for (int i = 0; i < o.n; i++) {
+        const int64_t t = indices ? indices[i] : i;
+        if (t < 0 || t >= N) return fail(d, ANUGA_E_ARG, "operator triangle out of range");
+        idx[i] = ip[t];
+    }
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    HIPCHK(d, dalloc(&o.idx, (size_t)std::max(o.n, 1)));
+    HIPCHK(d, dalloc(&o.influx, 1));
+    HIPCHK(d, hipMemcpy(o.idx, idx.data(), std::max(o.n, 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(d, hipMemset(o.influx, 0, sizeof(double)));
+    d->ops.push_back(o);
+    rc = upload_rate(d, d->ops.back(), rates, rate, factor);
+    if (rc) return rc;
+    drop_graph(d);
+    *id = (int64_t)d->ops.size() - 1;
+    return ANUGA_OK;
+}
+
+int anuga_update_rate_operator(anuga_domain *d, int64_t id, const double *rates, double rate,
+                               double factor)
+{
+    if (!d) return ANUGA_E_ARG;
+    if (id < 0 || id >= (int64_t)d->ops.size()) return fail(d, ANUGA_E_ARG, "no such operator");
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    RateOp &o = d->ops[id];
+    const bool had_array = o.rate != nullptr;
+    const RateOp before = o;
+    int rc = upload_rate(d, o, rates, rate, factor);
+    if (rc) return rc;
+    if (had_array != (o.rate != nullptr) || before.rate0 != o.rate0 ||
+        before.factor != o.factor || before.careful != o.careful)
+        drop_graph(d);
+    return ANUGA_OK;
+}
+
+int anuga_apply_operators(anuga_domain *d)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    if (d->substate != 0) return fail(d, ANUGA_E_STATE, "inside a multi-stage step");
+    ensure_live(d);
+    launch_operators(d);
+    return launch_check(d);
+}
+
+int anuga_get_operator_influx(anuga_domain *d, int64_t id, double *cumulative)
+{
+    if (!d || !cumulative) return ANUGA_E_ARG;
+    if (id < 0 || id >= (int64_t)d->ops.size()) return fail(d, ANUGA_E_ARG, "no such operator");
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    HIPCHK(d, hipMemcpy(cumulative, d->ops[id].influx, sizeof(double), hipMemcpyDeviceToHost));
     return ANUGA_OK;
 }
 

@@ -1179,3 +1179,39 @@ de_gather_f32(int N, const int *__restrict__ iperm, const double *__restrict__ v
     const int o[4] = {off.x, off.y, off.z, off.w};
     for (int q = 0; q < nq; q++) out[(size_t)q * N + k] = (float)r[o[q]];
 }
+
+// ---------------------------------------------------------------------------
+// fractional-step operators
+// ---------------------------------------------------------------------------
+// Rate_operator.__call__ (anuga/operators/rate_operators.py:149-260) on its
+// triangles after a step: local rate = factor * dt * rate; with all rates
+// >= 0 the stage rises by it; otherwise (`careful`) the loss is capped at the
+// depth and the momenta scale with the depth that remains. The inflow
+// (local rate * area) is accumulated for the volume balance (FP64 atomics:
+// a diagnostic, summed in no fixed order).
+__global__ void __launch_bounds__(DE_BLOCK)
+de_rate(const DeControl *ctl, int n, const int *__restrict__ idx, const double *__restrict__ rate,
+        double rate0, double factor, int careful, DeQ *S, const double *__restrict__ area,
+        double *influx)
+{
+    if (ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = idx[i];
+    const double r = rate ? rate[i] : rate0;
+    const double dt = ctl->dt;
+    DeQ q = S[k];
+    double lr = factor * dt * r;
+    if (!careful) {
+        q.w = q.w + lr;
+    } else {
+        const double h = q.w - q.z;
+        lr = fmax(lr, -h);
+        const double lf = lr < 0.0 ? (lr + h) / (h + 1.0e-10) : 1.0;
+        q.w = q.w + lr;
+        q.uh = q.uh * lf;
+        q.vh = q.vh * lf;
+    }
+    S[k] = q;
+    atomicAdd(influx, lr * area[k]);
+}

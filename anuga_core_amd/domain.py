@@ -211,6 +211,7 @@ class Domain:
         self._gauges = None
         self._time_bc = []
         self._bc_values = None
+        self.fractional_step_operators = []
 
     # -- device handle -----------------------------------------------------------
     def _create_device(self):
@@ -763,6 +764,7 @@ class Domain:
         self._update_boundary_slot(1, self.starttime + (initial + self.timestep))
         self._update_boundary_slot(2, self.starttime + (initial + self.timestep * 0.5))
         self._call("anuga_rk3_later_substeps", C.c_double(self.timestep), None)
+        self.relative_time = initial + self.timestep
 
     def get_time(self):
         return self.starttime + self.relative_time
@@ -827,7 +829,7 @@ class Domain:
             self.distribute_to_vertices_and_edges()
             yield self._yield()
         while True:
-            if self.device_timestep and not self._time_bc:
+            if self._device_stepping():
                 reached = self._device_steps(self._chunk_estimate())
             else:
                 reached = self._host_step(yieldstep)
@@ -845,6 +847,20 @@ class Domain:
                 self.recorded_min_timestep = self.evolve_max_timestep
                 self.recorded_max_timestep = self.evolve_min_timestep
                 self.number_of_steps = 0
+
+    def _device_stepping(self):
+        """Runs of steps on the device unless something needs the host every
+        step (time-dependent boundary values or operator rates)."""
+        return (self.device_timestep and not self._time_bc and
+                not any(op.time_dependent for op in self.fractional_step_operators))
+
+    def set_fractional_step_operator(self, operator):
+        if operator not in self.fractional_step_operators:
+            self.fractional_step_operators.append(operator)
+
+    def get_fractional_step_volume_integral(self):
+        """Water added by the fractional-step operators since creation."""
+        return float(sum(op.get_cumulative_influx() for op in self.fractional_step_operators))
 
     def _yield(self):
         """Store the frame of this yield (every output_frequency-th) and
@@ -865,6 +881,14 @@ class Domain:
             self.evolve_one_rk2_step(yieldstep, self.relative_finaltime)
         else:
             self.evolve_one_rk3_step(yieldstep, self.relative_finaltime)
+        if self.fractional_step_operators:
+            # apply_fractional_steps at the step's time as the reference has
+            # it then (start for euler, start + dt for rk2 and rk3)
+            t = self.get_time()
+            for op in self.fractional_step_operators:
+                if op.time_dependent:
+                    op.update(t)
+            self._call("anuga_apply_operators")
         self.relative_time = initial + self.timestep
         if self.record_dt_history:
             self.dt_history.append(self.timestep)
@@ -941,7 +965,7 @@ class Domain:
         self.recorded_max_timestep = self.evolve_min_timestep
         done = 0
         while done < nsteps:
-            if self.device_timestep and not self._time_bc:
+            if self._device_stepping():
                 before = self.number_of_steps
                 reached = self._device_steps(nsteps - done)
                 done += self.number_of_steps - before

@@ -147,6 +147,24 @@ int anuga_set_riverwalls(anuga_domain *d, const int64_t *edge_flux_type,
                          const int64_t *riverwall_rowIndex, int64_t nrows, int64_t ncols,
                          const double *hydraulic_properties);
 
+/* ---- Fractional-step operators ---------------------------------------------------
+ * Rate_operator (anuga/operators/rate_operators.py:149-260): after every step
+ * (apply_fractional_steps, generic_domain.py:1849) the stage of the
+ * operator's triangles (indices in the caller's order, or all when NULL)
+ * rises by factor * dt * rate, with rates[i] per triangle or the scalar
+ * `rate` when rates is NULL; when a rate is negative the loss is capped at
+ * the depth and the momenta scale with the remaining depth. Device-controlled
+ * steps apply the operators inside the step graph; host-sequenced steps call
+ * anuga_apply_operators after each step. *id receives the operator's number. */
+int anuga_add_rate_operator(anuga_domain *d, int64_t n, const int64_t *indices,
+                            const double *rates, double rate, double factor, int64_t *id);
+/* New rates / factor for operator id (e.g. a rate function evaluated at a new time). */
+int anuga_update_rate_operator(anuga_domain *d, int64_t id, const double *rates, double rate,
+                               double factor);
+int anuga_apply_operators(anuga_domain *d);
+/* Cumulative inflow volume sum(local rate * area) of operator id (order of the sum not fixed). */
+int anuga_get_operator_influx(anuga_domain *d, int64_t id, double *cumulative);
+
 /* ---- Output at yield points ---------------------------------------------------
  * Node connectivity of the mesh for SWW output: number_of_triangles_per_node
  * (nodes) and vertex_value_indices (3N, flat vertex index 3k+i, grouped by

@@ -18,6 +18,8 @@ Restates (paths relative to the anuga_core tree):
   * update_conserved_quantities shallow_water_domain.py:2089-2163
   * boundary flux integral anuga/operators/boundary_flux_integral_operator.py
   * get_water_volume      shallow_water_domain.py:1626-1653
+  * Rate_operator         anuga/operators/rate_operators.py:149-260, applied
+                          after every step (generic_domain.py:1849)
   * riverwall edge data   anuga/structures/riverwall.py:382-469 (the layout the
                           flux loop reads: edge_flux_type, per-rank elevation and
                           hydraulic row, edge_river_wall_counter)
@@ -42,6 +44,40 @@ def _first(value):
         return float(value)
     except TypeError:
         return float(value[0])
+
+
+class OracleRateOperator:
+    """Rate_operator.__call__ restated: indices (None: all triangles), rate
+    (number, per-index array, or a function of t), factor (number or a
+    function of t)."""
+
+    def __init__(self, indices=None, rate=0.0, factor=1.0):
+        self.indices = None if indices is None else np.asarray(indices, np.int64)
+        self.rate = rate
+        self.factor = factor
+        self.cumulative_influx = 0.0
+
+    def apply(self, d):
+        t = d.get_time()
+        dt = d.timestep
+        rate = self.rate(t) if callable(self.rate) else self.rate
+        rate = np.asarray(rate, np.float64)
+        factor = float(self.factor(t)) if callable(self.factor) else self.factor
+        idx = slice(None) if self.indices is None else self.indices
+        areas = d.areas[idx]
+        if np.all(rate >= 0.0):
+            local_rates = factor * dt * rate
+            self.cumulative_influx += float(np.sum(local_rates * areas))
+            d.stage_c[idx] = d.stage_c[idx] + local_rates
+        else:
+            heights = d.stage_c[idx] - d.elev_c[idx]
+            local_rates = np.maximum(factor * dt * rate, -heights)
+            local_factors = np.where(local_rates < 0.0,
+                                     (local_rates + heights) / (heights + 1.0e-10), 1.0)
+            self.cumulative_influx += float(np.sum(local_rates * areas))
+            d.stage_c[idx] = d.stage_c[idx] + local_rates
+            d.xmom_c[idx] = d.xmom_c[idx] * local_factors
+            d.ymom_c[idx] = d.ymom_c[idx] * local_factors
 
 
 class OracleDomain:
@@ -77,6 +113,7 @@ class OracleDomain:
         self.edge_timestep = z(3 * N) + 1.0e100
         self.boundary_flux_sum = z(3)
         self.boundary_flux_integral = 0.0
+        self.operators = []
         self._no_riverwalls()
 
         self.g = G
@@ -394,6 +431,8 @@ class OracleDomain:
             else:
                 self.evolve_one_rk3_step(yieldstep, self.relative_finaltime)
             self._apply_boundary_flux_integral()
+            for op in self.operators:
+                op.apply(self)
             self.relative_time = initial + self.timestep
             self.dt_history.append(self.timestep)
             self.number_of_steps += 1

@@ -1,0 +1,67 @@
+"""Rate operators on the device (MI355X) against the oracle's restatement of
+Rate_operator.__call__ (tests/test_operators.py): static rates applied inside
+the captured step (device-controlled runs), time-dependent rates evaluated on
+the host at the reference's times (host-sequenced runs), negative rates.
+Timestep sequences identical, states bit for bit.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from anuga_core_amd.domain import Domain
+from anuga_core_amd.operators import Rate_operator
+from oracle.kernels import OracleKernels
+from oracle.oracle_domain import OracleDomain, OracleRateOperator
+from tests import scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(alg):
+    mesh, _, q, bc = S.dam_break_de0(m=16, n=12)
+    o = S.setup(OracleDomain(mesh, OracleKernels(gather=True)), alg, q, bc)
+    d = S.setup(Domain(mesh=mesh), alg, q, bc)
+    return mesh, o, d
+
+
+def _run_and_compare(o, d, ys, ft):
+    for _ in d.evolve(yieldstep=ys, finaltime=ft):
+        pass
+    for _ in o.evolve(yieldstep=ys, finaltime=ft):
+        pass
+    assert d.dt_history == o.dt_history
+    for name, oname in (("stage", "stage_c"), ("xmomentum", "xmom_c"), ("ymomentum", "ymom_c")):
+        assert np.array_equal(d.get_quantity(name).centroid_values, getattr(o, oname)), name
+
+
+@pytest.mark.parametrize("alg", ["DE0", "DE1", "DE2"])
+def test_static_rain_and_extraction_vs_oracle(alg):
+    print("case static rate operators %s" % alg, flush=True)
+    mesh, o, d = _pair(alg)
+    cen = mesh.centroid_coordinates
+    rain = Rate_operator(d, rate=5.0e-3, factor=0.8,
+                         polygon=[[10, 10], [60, 10], [60, 60], [10, 60]])
+    well = Rate_operator(d, rate=-2.0e-2, center=[25.0, 50.0], radius=12.0)
+    spatial = Rate_operator(d, rate=lambda x, y: 1e-4 * (x / 100.0))
+    o.operators += [OracleRateOperator(rain.indices, 5.0e-3, 0.8),
+                    OracleRateOperator(well.indices, -2.0e-2),
+                    OracleRateOperator(None, 1e-4 * (cen[:, 0] / 100.0))]
+    assert len(rain.indices) > 0 and len(well.indices) > 0
+    assert d._device_stepping()
+    _run_and_compare(o, d, 1.0, 4.0)
+    infl = [op.get_cumulative_influx() for op in (rain, well, spatial)]
+    for a, b in zip(infl, o.operators):
+        assert a == pytest.approx(b.cumulative_influx, rel=1e-12, abs=1e-15)
+
+
+@pytest.mark.parametrize("alg", ["DE0", "DE1"])
+def test_time_dependent_rate_vs_oracle(alg):
+    print("case time-dependent rate operator %s" % alg, flush=True)
+    mesh, o, d = _pair(alg)
+    f = lambda t: 2.0e-3 * (1.0 + math.sin(t))
+    Rate_operator(d, rate=f, indices=np.arange(0, mesh.number_of_triangles, 2))
+    o.operators.append(OracleRateOperator(np.arange(0, mesh.number_of_triangles, 2), f))
+    assert not d._device_stepping()
+    _run_and_compare(o, d, 1.0, 3.0)
