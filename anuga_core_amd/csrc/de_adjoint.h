@@ -151,6 +151,10 @@ ADJ_HD Dual<K> adj_sel3(bool a, bool b, const Dual<K> &x, const Dual<K> &y, cons
     return r;
 }
 
+// the value without derivative
+ADJ_HD double adj_const(double a) { return a; }
+template <int K> ADJ_HD Dual<K> adj_const(const Dual<K> &a) { return Dual<K>(a.v); }
+
 ADJ_HD double adj_sqrt(double a) { return sqrt(a); }
 template <int K> ADJ_HD Dual<K> adj_sqrt(const Dual<K> &a)
 {
@@ -182,6 +186,9 @@ template <int K> ADJ_HD Dual<K> adj_pow73(const Dual<K> &h)
 struct AdjParams {
     double g, eps, mah;
     double bw, bwd, buh, buhd, bvh, bvhd;
+    // limiter jumps below this size are rounding noise: the limiter's factor
+    // is held at its value there (0: exact derivative everywhere)
+    double freeze;
 };
 
 #define ADJ_F_CLAMP 1u
@@ -191,7 +198,7 @@ struct AdjParams {
 // limiter and gradients (sw_domain_orig.c:1306-1333, :2238-2277, :2470-2504)
 // ---------------------------------------------------------------------------
 template <class T>
-ADJ_HD void adj_limit(T d[3], T qmin, T qmax, T beta)
+ADJ_HD void adj_limit(T d[3], T qmin, T qmax, T beta, double freeze = 0.0)
 {
     T r = T(1000.0), r0 = T(1.0);
     const T tiny = T(1.0e-100);
@@ -202,7 +209,14 @@ ADJ_HD void adj_limit(T d[3], T qmin, T qmax, T beta)
         if (d[i] > tiny) r0 = qmax / d[i];
         r = adj_min(r0, r);
     }
-    const T phi = adj_min(r * beta, T(1.0));
+    T phi = adj_min(r * beta, T(1.0));
+    // At jumps of rounding size (still water reached only by rounding noise)
+    // the ratio qmin / d of two noise-sized numbers has derivatives of 1/d,
+    // which the adjoint of many steps amplifies without bound. There the
+    // factor is held at its value; the forward value is unchanged.
+    if (freeze > 0.0 && fabs(adj_val(d[0])) < freeze && fabs(adj_val(d[1])) < freeze &&
+        fabs(adj_val(d[2])) < freeze)
+        phi = adj_const(phi);
     d[0] = d[0] * phi;
     d[1] = d[1] * phi;
     d[2] = d[2] * phi;
@@ -211,7 +225,7 @@ ADJ_HD void adj_limit(T d[3], T qmin, T qmax, T beta)
 template <class T>
 ADJ_HD void adj_aux_jumps(T qk, T q0, T q1, T q2, double dx1, double dx2, double dy1, double dy2,
                           double inv_area2, const double dxv[3], const double dyv[3], T beta,
-                          T d[3])
+                          T d[3], double freeze = 0.0)
 {
     const T dq0 = q0 - qk;
     const T dq1 = q1 - q0;
@@ -225,12 +239,12 @@ ADJ_HD void adj_aux_jumps(T qk, T q0, T q1, T q2, double dx1, double dx2, double
     const T zero = T(0.0);
     const T qmax = adj_max(adj_max(dq0, adj_max(dq0 + dq1, dq0 + dq2)), zero);
     const T qmin = adj_min(adj_min(dq0, adj_min(dq0 + dq1, dq0 + dq2)), zero);
-    adj_limit(d, qmin, qmax, beta);
+    adj_limit(d, qmin, qmax, beta, freeze);
 }
 
 template <class T>
 ADJ_HD void adj_line_jumps(T qk, T q1, double dx2, double dy2, const double dxv[3],
-                           const double dyv[3], T beta, T d[3])
+                           const double dyv[3], T beta, T d[3], double freeze = 0.0)
 {
     const T dq1 = q1 - qk;
     const T a = dq1 * T(dx2);
@@ -240,7 +254,7 @@ ADJ_HD void adj_line_jumps(T qk, T q1, double dx2, double dy2, const double dxv[
     const bool up = dq1 >= T(0.0);
     const T qmin = adj_sel(up, T(0.0), dq1);
     const T qmax = adj_sel(up, dq1, T(0.0));
-    adj_limit(d, qmin, qmax, beta);
+    adj_limit(d, qmin, qmax, beta, freeze);
 }
 
 // ---------------------------------------------------------------------------
@@ -315,20 +329,20 @@ ADJ_HD void adj_extrap_cell(const T w[4], const T uh[4], const T vh[4], const T 
             gw = beta > zero;
             if (gw) {
                 adj_aux_jumps<T>(cw, nw[0], nw[1], nw[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2,
-                                 inv_area2, gm.dxv, gm.dyv, beta, jw);
+                                 inv_area2, gm.dxv, gm.dyv, beta, jw, p.freeze);
                 adj_aux_jumps<T>(ch, nh[0], nh[1], nh[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2,
-                                 inv_area2, gm.dxv, gm.dyv, beta, jh);
+                                 inv_area2, gm.dxv, gm.dyv, beta, jh, p.freeze);
             }
             beta = T(p.buhd) + (T(p.buh) - T(p.buhd)) * hfactor;
             gu = beta > zero;
             if (gu)
                 adj_aux_jumps<T>(cu, nu[0], nu[1], nu[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2,
-                                 inv_area2, gm.dxv, gm.dyv, beta, ju);
+                                 inv_area2, gm.dxv, gm.dyv, beta, ju, p.freeze);
             beta = T(p.bvhd) + (T(p.bvh) - T(p.bvhd)) * hfactor;
             gv = beta > zero;
             if (gv)
                 adj_aux_jumps<T>(cv, nv[0], nv[1], nv[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2,
-                                 inv_area2, gm.dxv, gm.dyv, beta, jv);
+                                 inv_area2, gm.dxv, gm.dyv, beta, jv, p.freeze);
         }
     } else if (gm.nbound == 2) {
         const bool e0 = gm.interior[0], e1 = gm.interior[1];
@@ -346,10 +360,10 @@ ADJ_HD void adj_extrap_cell(const T w[4], const T uh[4], const T vh[4], const T 
             const double dy2 = dx2 * gm.ly;
             dx2 *= gm.lx;
             const T bw = T(p.bw);
-            adj_line_jumps<T>(cw, sw, dx2, dy2, gm.dxv, gm.dyv, bw, jw);
-            adj_line_jumps<T>(ch, sh, dx2, dy2, gm.dxv, gm.dyv, bw, jh);
-            adj_line_jumps<T>(cu, su, dx2, dy2, gm.dxv, gm.dyv, bw, ju);
-            adj_line_jumps<T>(cv, sv, dx2, dy2, gm.dxv, gm.dyv, bw, jv);
+            adj_line_jumps<T>(cw, sw, dx2, dy2, gm.dxv, gm.dyv, bw, jw, p.freeze);
+            adj_line_jumps<T>(ch, sh, dx2, dy2, gm.dxv, gm.dyv, bw, jh, p.freeze);
+            adj_line_jumps<T>(cu, su, dx2, dy2, gm.dxv, gm.dyv, bw, ju, p.freeze);
+            adj_line_jumps<T>(cv, sv, dx2, dy2, gm.dxv, gm.dyv, bw, jv, p.freeze);
         }
     }
 #pragma unroll

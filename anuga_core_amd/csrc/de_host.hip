@@ -1143,6 +1143,10 @@ static int volumes(anuga_domain *d)
 // ---------------------------------------------------------------------------
 // fixed-timestep runs and their adjoint
 // ---------------------------------------------------------------------------
+// limiter jumps below this size (m, m/s) count as rounding noise in the
+// adjoint: the limiter's factor is held there (de_adjoint.h adj_limit)
+const double kAdjLimiterFreeze = 1.0e-10;
+
 static AdjView adj_view(const anuga_domain *d, double dt)
 {
     AdjView m;
@@ -1160,7 +1164,8 @@ static AdjView adj_view(const anuga_domain *d, double dt)
     m.bkind = d->bkind;
     m.bval = d->bval;
     const DeParams q = dev_params(d->P);
-    m.p = AdjParams{q.g, q.eps, q.mah, q.bw, q.bwd, q.buh, q.buhd, q.bvh, q.bvhd};
+    m.p = AdjParams{q.g, q.eps, q.mah, q.bw, q.bwd, q.buh, q.buhd, q.bvh, q.bvhd,
+                    kAdjLimiterFreeze};
     m.dt = dt;
     return m;
 }

@@ -11,6 +11,8 @@
 
 namespace {
 
+double g_freeze = 1.0e-10;   // AdjParams::freeze of the runs (the library's value)
+
 struct Run {
     AdjView m;
     int N;
@@ -110,7 +112,8 @@ Run make_run(int N, int M, const int *nb, const double *cen, const double *gx, c
     r.m.eta = eta;
     r.m.bkind = bkind;
     r.m.bval = bval;
-    r.m.p = AdjParams{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], prm[7], prm[8]};
+    r.m.p = AdjParams{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], prm[7], prm[8],
+                      g_freeze};
     r.m.dt = dt;
     r.resize();
     return r;
@@ -119,6 +122,9 @@ Run make_run(int N, int M, const int *nb, const double *cen, const double *gx, c
 }  // namespace
 
 extern "C" {
+
+// limiter jumps below `freeze` hold the limiter's factor in the adjoint
+void harness_set_freeze(double freeze) { g_freeze = freeze; }
 
 // nsteps fixed-dt DE1 steps of S ([N][4], in place); stage of the gauge
 // triangles after every obs_every-th step into rec ([nsteps / obs_every][G]).

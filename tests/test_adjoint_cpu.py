@@ -184,3 +184,56 @@ def _agrees(ad, f):
             return eps
         diffs.append(fc)
     raise AssertionError("adjoint %r, central differences %r" % (ad, diffs))
+
+
+def test_rounding_noise_in_still_water_is_not_amplified(harness):
+    """A wave entering still water reaches it first as rounding noise. There
+    the limiter's ratio qmin / d of two noise-sized numbers has derivatives
+    of size 1/d, and the exact adjoint of a few hundred steps grows to 1e30
+    and more. The adjoint holds the limiter's factor at jumps below 1e-10
+    (the library's value): the gradient is then bounded and agrees with
+    finite differences of the forward (which the freeze does not change)."""
+    import bench
+    harness.harness_set_freeze.argtypes = [C.c_double]
+    mesh, q, bc, gauges = bench.build_c4(m=16, length=5000.0)
+    L = Layout(mesh, q, bc)
+    N = L.N
+    S0 = np.ascontiguousarray(np.stack([q["stage"], 0 * q["stage"], 0 * q["stage"],
+                                        q["elevation"]], axis=1))
+    g = np.ascontiguousarray(gauges[:16] % N, np.int32)
+    nsteps, dt = 300, 0.1
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    _, rec = _forward(harness, L, S0, dt, nsteps, g, 10)
+    obs = rec + 1e-3 * np.random.default_rng(0).standard_normal(rec.shape)
+    dL = np.ascontiguousarray(2.0 * (rec - obs))
+
+    def grads(freeze):
+        harness.harness_set_freeze(freeze)
+        nbar, zbar, lam = np.zeros(N), np.zeros(N), np.zeros(3 * N)
+        harness.harness_adjoint(*L.args(), C.c_double(dt), nsteps, dp(S0), len(g),
+                                g.ctypes.data_as(C.POINTER(C.c_int)), 10, dp(dL), dp(nbar),
+                                dp(zbar), dp(lam))
+        return nbar, zbar
+    try:
+        _, z_exact = grads(0.0)
+        nbar, zbar = grads(1.0e-10)
+    finally:
+        harness.harness_set_freeze(1.0e-10)
+    assert np.abs(z_exact).max() > 1e10
+    assert np.abs(zbar).max() < 1e-2
+
+    def loss(eta=None, dz=0.0):
+        Lp = Layout(mesh, q, bc)
+        if eta is not None:
+            Lp.eta[:] = eta
+        Sp = S0.copy()
+        Sp[:, 3] += dz
+        _, r = _forward(harness, Lp, Sp, dt, nsteps, g, 10)
+        return float(np.sum((r - obs) ** 2))
+    rng = np.random.default_rng(5)
+    d = rng.standard_normal(N)
+    fd = (loss(eta=L.eta + 1e-6 * d) - loss(eta=L.eta - 1e-6 * d)) / 2e-6
+    assert np.dot(nbar, d) == pytest.approx(fd, rel=1e-4)
+    d = rng.standard_normal(N)
+    fd = (loss(dz=1e-6 * d) - loss(dz=-1e-6 * d)) / 2e-6
+    assert np.dot(zbar, d) == pytest.approx(fd, rel=0.05)
