@@ -85,6 +85,9 @@ SIGNATURES = {
     "anuga_update_rate_operator": (C.c_int, [C.c_void_p, C.c_int64, _dp, C.c_double,
                                              C.c_double]),
     "anuga_apply_operators": (C.c_int, [C.c_void_p]),
+    "anuga_apply_operator": (C.c_int, [C.c_void_p, C.c_int64]),
+    "anuga_get_centroid_subset": (C.c_int, [C.c_void_p, C.c_int64, _ip, _dp]),
+    "anuga_set_centroid_subset": (C.c_int, [C.c_void_p, C.c_int64, _ip, _dp]),
     "anuga_get_operator_influx": (C.c_int, [C.c_void_p, C.c_int64, _dp]),
     "anuga_set_output_nodes": (C.c_int, [C.c_void_p, C.c_int64, _ip, _ip]),
     "anuga_get_sww_frame": (C.c_int, [C.c_void_p, C.c_double, _fp, _fp]),

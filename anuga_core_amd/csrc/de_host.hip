@@ -83,6 +83,10 @@ struct anuga_domain {
     int *gtri = nullptr;
     double *gtimes = nullptr, *gvals = nullptr;
     std::vector<RateOp> ops;             // fractional-step rate operators, in order
+    std::vector<int> hiperm;             // caller's index -> internal (one scenario)
+    int *sub_idx = nullptr;              // staging of a subset access
+    DeQ *sub_val = nullptr;
+    int sub_cap = 0;
     bool mixed = false;
     // quantities
     double *eta = nullptr;
@@ -1279,7 +1283,7 @@ void anuga_destroy(anuga_domain *d)
     drop_graph(d);
     void *ptrs[] = {d->nb, d->cen, d->gx, d->gf, d->area, d->nx, d->ny, d->len, d->radius,
                     d->scen, d->rw, d->rwd, d->caller, d->vx, d->node_start, d->node_tri, d->iperm_dev,
-                    d->out_f32, d->gtri, d->gtimes, d->gvals, d->eta, d->A, d->B, d->E, d->flags,
+                    d->out_f32, d->gtri, d->gtimes, d->gvals, d->sub_idx, d->sub_val, d->eta, d->A, d->B, d->E, d->flags,
                     d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->ctl, d->dsetup,
                     d->dneg, d->hist};
     for (void *p : ptrs)
@@ -1479,6 +1483,91 @@ int anuga_update_rate_operator(anuga_domain *d, int64_t id, const double *rates,
         before.factor != o.factor || before.careful != o.careful)
         drop_graph(d);
     return ANUGA_OK;
+}
+
+// internal ids of n caller triangles in d->sub_idx (staging grown as needed)
+static int stage_subset(anuga_domain *d, int64_t n, const int64_t *indices)
+{
+    const int N = d->ntri[0];
+    if (n < 0 || n > N) return fail(d, ANUGA_E_ARG, "bad subset size");
+    if (n > 0 && !indices) return fail(d, ANUGA_E_ARG, "indices is NULL");
+    if (d->hiperm.empty()) {
+        d->hiperm.resize(N);
+        for (int p = 0; p < N; p++) d->hiperm[d->perm[p]] = p;
+    }
+    std::vector<int> idx(std::max<int64_t>(n, 1));
+    for (int64_t i = 0; i < n; i++) {
+        if (indices[i] < 0 || indices[i] >= N) return fail(d, ANUGA_E_ARG, "index out of range");
+        idx[i] = d->hiperm[indices[i]];
+    }
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    if (n > d->sub_cap) {
+        if (d->sub_idx) HIPCHK(d, hipFree(d->sub_idx));
+        if (d->sub_val) HIPCHK(d, hipFree(d->sub_val));
+        d->sub_idx = nullptr;
+        d->sub_val = nullptr;
+        HIPCHK(d, dalloc(&d->sub_idx, (size_t)n));
+        HIPCHK(d, dalloc(&d->sub_val, (size_t)n));
+        d->sub_cap = (int)n;
+    }
+    if (n > 0)
+        HIPCHK(d, hipMemcpy(d->sub_idx, idx.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    return ANUGA_OK;
+}
+
+int anuga_get_centroid_subset(anuga_domain *d, int64_t n, const int64_t *indices, double *values)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    if (n > 0 && !values) return fail(d, ANUGA_E_ARG, "values is NULL");
+    if ((rc = stage_subset(d, n, indices))) return rc;
+    if (n == 0) return ANUGA_OK;
+    hipLaunchKernelGGL(de_gather_state, dim3((n + DE_BLOCK - 1) / DE_BLOCK), dim3(DE_BLOCK), 0,
+                       d->stream, (int)n, d->sub_idx, d->substate ? d->B : d->A, d->sub_val);
+    rc = launch_check(d);
+    if (rc) return rc;
+    HIPCHK(d, hipMemcpyAsync(values, d->sub_val, n * sizeof(DeQ), hipMemcpyDeviceToHost,
+                             d->stream));
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    return ANUGA_OK;
+}
+
+int anuga_set_centroid_subset(anuga_domain *d, int64_t n, const int64_t *indices,
+                              const double *values)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    if (d->substate != 0) return fail(d, ANUGA_E_STATE, "inside a multi-stage step");
+    if (n > 0 && !values) return fail(d, ANUGA_E_ARG, "values is NULL");
+    if ((rc = stage_subset(d, n, indices))) return rc;
+    if (n == 0) return ANUGA_OK;
+    std::vector<int64_t> sorted(indices, indices + n);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        return fail(d, ANUGA_E_ARG, "indices must be distinct");
+    HIPCHK(d, hipMemcpy(d->sub_val, values, n * sizeof(DeQ), hipMemcpyHostToDevice));
+    ensure_live(d);
+    hipLaunchKernelGGL(de_scatter_state, dim3((n + DE_BLOCK - 1) / DE_BLOCK), dim3(DE_BLOCK), 0,
+                       d->stream, (int)n, d->sub_idx, d->sub_val, d->A);
+    rc = launch_check(d);
+    if (rc) return rc;
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    return ANUGA_OK;
+}
+
+int anuga_apply_operator(anuga_domain *d, int64_t id)
+{
+    if (!d) return ANUGA_E_ARG;
+    if (id < 0 || id >= (int64_t)d->ops.size()) return fail(d, ANUGA_E_ARG, "no such operator");
+    if (d->substate != 0) return fail(d, ANUGA_E_STATE, "inside a multi-stage step");
+    ensure_live(d);
+    const RateOp &o = d->ops[id];
+    hipLaunchKernelGGL(de_rate, dim3((o.n + DE_BLOCK - 1) / DE_BLOCK), dim3(DE_BLOCK), 0,
+                       d->stream, d->ctl, o.n, o.idx, o.rate, o.rate0, o.factor, o.careful, d->A,
+                       d->area, o.influx);
+    return launch_check(d);
 }
 
 int anuga_apply_operators(anuga_domain *d)

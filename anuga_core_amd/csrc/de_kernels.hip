@@ -1215,3 +1215,25 @@ de_rate(const DeControl *ctl, int n, const int *__restrict__ idx, const double *
     S[k] = q;
     atomicAdd(influx, lr * area[k]);
 }
+
+// state of a subset of triangles (internal ids) to [n][4] {w, uh, vh, z}
+__global__ void __launch_bounds__(DE_BLOCK)
+de_gather_state(int n, const int *__restrict__ idx, const DeQ *__restrict__ S, DeQ *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = S[idx[i]];
+}
+
+// w, uh, vh of a subset of triangles from [n][4] (the bed is kept)
+__global__ void __launch_bounds__(DE_BLOCK)
+de_scatter_state(int n, const int *__restrict__ idx, const DeQ *__restrict__ in, DeQ *S)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    DeQ q = S[idx[i]];
+    const DeQ v = in[i];
+    q.w = v.w;
+    q.uh = v.uh;
+    q.vh = v.vh;
+    S[idx[i]] = q;
+}

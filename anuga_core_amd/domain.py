@@ -883,12 +883,15 @@ class Domain:
             self.evolve_one_rk3_step(yieldstep, self.relative_finaltime)
         if self.fractional_step_operators:
             # apply_fractional_steps at the step's time as the reference has
-            # it then (start for euler, start + dt for rk2 and rk3)
+            # it then (start for euler, start + dt for rk2 and rk3), in order
             t = self.get_time()
             for op in self.fractional_step_operators:
+                if getattr(op, "host_side", False):
+                    op.apply(t, self.timestep)
+                    continue
                 if op.time_dependent:
                     op.update(t)
-            self._call("anuga_apply_operators")
+                self._call("anuga_apply_operator", op._id.value)
         self.relative_time = initial + self.timestep
         if self.record_dt_history:
             self.dt_history.append(self.timestep)

@@ -52,6 +52,15 @@ def _inside_polygon(points, polygon):
     return inside
 
 
+def _line_triangles(mesh, line, samples=400):
+    """Triangles crossed by a segment (those holding points sampled along it)."""
+    (x0, y0), (x1, y1) = np.asarray(line, np.float64)
+    t = np.linspace(0.0, 1.0, samples + 1)
+    pts = np.stack([x0 + t * (x1 - x0), y0 + t * (y1 - y0)], axis=1)
+    tri = mesh.find_triangles(pts)
+    return np.unique(tri[tri >= 0])
+
+
 class Rate_operator:
     """Rate_operator(domain, rate=0.0, factor=1.0, region=None, indices=None,
     polygon=None, center=None, radius=None, default_rate=0.0, ...)."""
@@ -134,3 +143,127 @@ class Rate_operator:
     def __repr__(self):
         return "Rate_operator(%s)" % ("all triangles" if self.indices is None
                                       else "%d triangles" % len(self.indices))
+
+
+class Inlet_operator:
+    """Inlet_operator(domain, region, Q=0.0, velocity=None, zero_velocity=False,
+    default=0.0, ...) (anuga/structures/inlet_operator.py): a discharge Q
+    (m^3/s, a number or a function of t) into a region after every step. A
+    positive volume fills the region's lowest stages first to a level
+    surface (Inlet.set_stages_evenly, anuga/structures/inlet.py:192-225); a
+    negative one lowers the region to a uniform depth, or empties it. The
+    momenta follow the region's velocities (or `velocity`, or zero).
+
+    The region is small: each step its state is read from the device, the
+    reference's arithmetic is applied on the host (the same numpy calls, so
+    the same sort order and sums) and written back. A domain with inlets
+    takes host-sequenced steps. `region`: triangle indices, an object with
+    `indices`, or a polygon (triangles whose centroid lies inside)."""
+
+    time_dependent = True
+    host_side = True
+    velocity_protection = 1.0e-6    # anuga/config.py velocity_protection
+
+    def __init__(self, domain, region, Q=0.0, velocity=None, zero_velocity=False, default=0.0,
+                 description=None, label=None, logging=False, verbose=False):
+        self.domain = domain
+        if hasattr(region, "indices"):
+            idx = region.indices
+        else:
+            a = np.asarray(region)
+            if a.ndim == 2 and a.shape == (2, 2) and a.dtype.kind == "f":
+                idx = _line_triangles(domain.mesh, a)
+            elif a.ndim == 2 and a.shape[1] == 2 and a.dtype.kind == "f":
+                idx = np.nonzero(_inside_polygon(domain.mesh.centroid_coordinates, a))[0]
+            else:
+                idx = a
+        self.triangle_indices = np.ascontiguousarray(idx, np.int64)
+        if len(self.triangle_indices) == 0:
+            raise Exception("No triangles have been identified in region")
+        self.areas = domain.mesh.areas.take(self.triangle_indices)
+        self.area = np.sum(self.areas)
+        self.Q = Q
+        if velocity is not None:
+            assert len(velocity) == 2
+        self.velocity = velocity
+        self.zero_velocity = zero_velocity
+        self.default = default
+        self.applied_Q = 0.0
+        self.total_applied_volume = 0.0
+        self.total_requested_volume = 0.0
+        self.volume_integral = 0.0
+        domain.fractional_step_operators.append(self)
+
+    def update_Q(self, t):
+        if callable(self.Q):
+            try:
+                return self.Q(t)
+            except Exception as e:
+                if type(e).__name__ in ("Modeltime_too_early", "Modeltime_too_late"):
+                    return self.default(t) if callable(self.default) else self.default
+                raise
+        return self.Q
+
+    def apply(self, t, timestep):
+        d = self.domain
+        idx = self.triangle_indices
+        n = len(idx)
+        st = np.empty((n, 4))
+        ip = idx.ctypes.data_as(C.POINTER(C.c_int64))
+        d._call("anuga_get_centroid_subset", n, ip, st.ctypes.data_as(C.POINTER(C.c_double)))
+        stages, xmoms, ymoms, elev = st[:, 0].copy(), st[:, 1].copy(), st[:, 2].copy(), st[:, 3]
+        depths = stages - elev
+        current_volume = np.sum(depths * self.areas)
+        assert current_volume >= 0.0
+        Q1 = self.update_Q(t)
+        Q2 = self.update_Q(t + timestep)
+        Q = 0.5 * (Q1 + Q2)
+        volume = Q * timestep
+        self.applied_Q = Q
+        vp = self.velocity_protection
+        u = xmoms * depths / (depths * depths + vp)
+        v = ymoms * depths / (depths * depths + vp)
+        if volume >= 0.0:
+            # set_stages_evenly
+            order = stages.argsort()
+            summed_areas = np.cumsum(self.areas[order])
+            summed_volume = np.zeros_like(self.areas)
+            summed_volume[1:] = np.cumsum(summed_areas[:-1] * np.diff(stages[order]))
+            index = np.nonzero(summed_volume <= volume)[0][-1]
+            depth = (volume - summed_volume[index]) / summed_areas[index]
+            stages[order[0:index + 1]] = stages[order[index]] + depth
+            self.volume_integral += volume
+            self.total_requested_volume += volume
+            depths = stages - elev
+            xmoms, ymoms = self._momenta(depths, u, v)
+        elif current_volume + volume >= 0.0:
+            depth = (current_volume + volume) / self.area
+            stages = elev + depth
+            self.total_requested_volume += volume
+            self.volume_integral += volume
+            depths = stages - elev
+            xmoms, ymoms = self._momenta(depths, u, v)
+        else:
+            stages = elev + 0.0
+            self.total_requested_volume += volume
+            volume = -current_volume
+            self.applied_Q = -current_volume / timestep
+            self.volume_integral -= current_volume
+            xmoms = np.zeros(n)
+            ymoms = np.zeros(n)
+        self.total_applied_volume += volume
+        out = np.ascontiguousarray(np.stack([stages, xmoms, ymoms, elev], axis=1))
+        d._call("anuga_set_centroid_subset", n, ip, out.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def _momenta(self, depths, u, v):
+        if self.zero_velocity:
+            return np.zeros_like(depths), np.zeros_like(depths)
+        if self.velocity is not None:
+            return depths * self.velocity[0], depths * self.velocity[1]
+        return depths * u, depths * v
+
+    def get_cumulative_influx(self):
+        return self.volume_integral
+
+    def __repr__(self):
+        return "Inlet_operator(%d triangles)" % len(self.triangle_indices)

@@ -162,6 +162,17 @@ int anuga_add_rate_operator(anuga_domain *d, int64_t n, const int64_t *indices,
 int anuga_update_rate_operator(anuga_domain *d, int64_t id, const double *rates, double rate,
                                double factor);
 int anuga_apply_operators(anuga_domain *d);
+/* Apply one operator (host-sequenced steps that interleave host-side operators). */
+int anuga_apply_operator(anuga_domain *d, int64_t id);
+
+/* State of a subset of triangles (caller's indices): get gives (n,4)
+ * {stage, xmomentum, ymomentum, elevation}; set takes (n,4) and writes
+ * stage, xmomentum, ymomentum (indices distinct). For host-side operators on
+ * small regions (Inlet_operator, anuga/structures/inlet_operator.py:75-165,
+ * through Inlet's take/put of centroid values, inlet.py:63-190). */
+int anuga_get_centroid_subset(anuga_domain *d, int64_t n, const int64_t *indices, double *values);
+int anuga_set_centroid_subset(anuga_domain *d, int64_t n, const int64_t *indices,
+                              const double *values);
 /* Cumulative inflow volume sum(local rate * area) of operator id (order of the sum not fixed). */
 int anuga_get_operator_influx(anuga_domain *d, int64_t id, double *cumulative);
 

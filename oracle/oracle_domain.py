@@ -20,6 +20,8 @@ Restates (paths relative to the anuga_core tree):
   * get_water_volume      shallow_water_domain.py:1626-1653
   * Rate_operator         anuga/operators/rate_operators.py:149-260, applied
                           after every step (generic_domain.py:1849)
+  * Inlet_operator        anuga/structures/inlet_operator.py:75-183 with
+                          Inlet (anuga/structures/inlet.py:63-237)
   * riverwall edge data   anuga/structures/riverwall.py:382-469 (the layout the
                           flux loop reads: edge_flux_type, per-rank elevation and
                           hydraulic row, edge_river_wall_counter)
@@ -78,6 +80,71 @@ class OracleRateOperator:
             d.stage_c[idx] = d.stage_c[idx] + local_rates
             d.xmom_c[idx] = d.xmom_c[idx] * local_factors
             d.ymom_c[idx] = d.ymom_c[idx] * local_factors
+
+
+class OracleInletOperator:
+    """Inlet_operator.__call__ restated on a set of triangle indices."""
+
+    def __init__(self, indices, Q=0.0, velocity=None, zero_velocity=False, default=0.0):
+        self.idx = np.asarray(indices, np.int64)
+        self.Q = Q
+        self.velocity = velocity
+        self.zero_velocity = zero_velocity
+        self.default = default
+        self.volume_integral = 0.0
+
+    def _Q(self, t):
+        return self.Q(t) if callable(self.Q) else self.Q
+
+    def apply(self, d):
+        idx = self.idx
+        dt = d.timestep
+        t = d.get_time()
+        areas = d.areas.take(idx)
+        area = np.sum(areas)
+        stages = d.stage_c.take(idx)
+        elev = d.elev_c.take(idx)
+        depths = stages - elev
+        current = np.sum(depths * areas)
+        Q = 0.5 * (self._Q(t) + self._Q(t + dt))
+        volume = Q * dt
+        vp = 1.0e-6                        # anuga/config.py velocity_protection
+        u = d.xmom_c.take(idx) * depths / (depths * depths + vp)
+        v = d.ymom_c.take(idx) * depths / (depths * depths + vp)
+
+        def momenta():
+            dep = d.stage_c.take(idx) - d.elev_c.take(idx)
+            if self.velocity is not None:
+                d.xmom_c.put(idx, dep * self.velocity[0])
+                d.ymom_c.put(idx, dep * self.velocity[1])
+            else:
+                d.xmom_c.put(idx, dep * u)
+                d.ymom_c.put(idx, dep * v)
+            if self.zero_velocity:
+                d.xmom_c.put(idx, 0.0)
+                d.ymom_c.put(idx, 0.0)
+
+        if volume >= 0.0:
+            # Inlet.set_stages_evenly (inlet.py:192-225)
+            order = stages.argsort()
+            summed_areas = np.cumsum(areas[order])
+            summed_volume = np.zeros_like(areas)
+            summed_volume[1:] = np.cumsum(summed_areas[:-1] * np.diff(stages[order]))
+            index = np.nonzero(summed_volume <= volume)[0][-1]
+            depth = (volume - summed_volume[index]) / summed_areas[index]
+            stages[order[0:index + 1]] = stages[order[index]] + depth
+            d.stage_c.put(idx, stages)
+            self.volume_integral += volume
+            momenta()
+        elif current + volume >= 0.0:
+            d.stage_c.put(idx, elev + (current + volume) / area)
+            self.volume_integral += volume
+            momenta()
+        else:
+            d.stage_c.put(idx, elev + 0.0)
+            self.volume_integral -= current
+            d.xmom_c.put(idx, 0.0)
+            d.ymom_c.put(idx, 0.0)
 
 
 class OracleDomain:

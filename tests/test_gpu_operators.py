@@ -65,3 +65,23 @@ def test_time_dependent_rate_vs_oracle(alg):
     o.operators.append(OracleRateOperator(np.arange(0, mesh.number_of_triangles, 2), f))
     assert not d._device_stepping()
     _run_and_compare(o, d, 1.0, 3.0)
+
+
+@pytest.mark.parametrize("alg", ["DE0", "DE1"])
+def test_inlets_vs_oracle(alg):
+    """Inlet_operator (host-side arithmetic on the device state of the inlet
+    triangles) next to a static rain operator, against the oracle."""
+    print("case inlets %s" % alg, flush=True)
+    from anuga_core_amd.operators import Inlet_operator, _line_triangles
+    from oracle.oracle_domain import OracleInletOperator
+    mesh, o, d = _pair(alg)
+    l1 = _line_triangles(mesh, [[30.0, 20.0], [40.0, 20.0]])
+    l2 = _line_triangles(mesh, [[70.0, 60.0], [80.0, 60.0]])
+    Inlet_operator(d, [[30.0, 20.0], [40.0, 20.0]], Q=2.0)
+    Rate_operator(d, rate=1e-3, polygon=[[0, 0], [50, 0], [50, 50], [0, 50]])
+    Inlet_operator(d, l2, Q=lambda t: -0.5 + 0.2 * t, velocity=[0.1, 0.0])
+    rain = d.fractional_step_operators[1]
+    o.operators += [OracleInletOperator(l1, 2.0), OracleRateOperator(rain.indices, 1e-3),
+                    OracleInletOperator(l2, lambda t: -0.5 + 0.2 * t, velocity=[0.1, 0.0])]
+    assert not d._device_stepping()
+    _run_and_compare(o, d, 1.0, 4.0)
