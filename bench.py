@@ -358,7 +358,7 @@ def run_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
-def build_c4(m=500, length=5000.0, seed=2):
+def build_c4(m=500, length=5000.0, seed=2, west="transmissive"):
     """The gradient workload: rectangular_cross(m, m) over a 5 km square (10 m
     cells), fully wet (depth >= 5 m): bed -12 m with a 4 m Gaussian bump
     (sigma 800 m), a 0.5 m stage hump (sigma 300 m) off-centre, at rest;
@@ -381,19 +381,24 @@ def build_c4(m=500, length=5000.0, seed=2):
     n_true = 0.03 + 0.01 * field
     gauges = rng.choice(mesh.number_of_triangles, 32, replace=False)
     q = dict(elevation=z, stage=w, friction=n_true)
-    bc = {"left": "transmissive", "right": "reflective", "top": "reflective",
-          "bottom": "reflective"}
+    bc = {"left": west, "right": "reflective", "top": "reflective", "bottom": "reflective"}
     return mesh, q, bc, gauges
 
 
 def run_adjoint(args):
     """Gradient of a gauge misfit with respect to friction and bed by the
     discrete adjoint, checked against central differences of the device
-    forward along random directions."""
+    forward along random directions.
+
+    The default 2000 steps of 0.025 s cover 50 s. A perturbation of the bed
+    of 1e-10 grows by about e every 10 s of simulated time in this wave field
+    (limiter switching), so at 200 s (dt 0.1) any perturbation, however
+    small, changes the gauge record by about 1e-5 and neither central
+    differences nor the derivative of the discrete map are meaningful."""
     from anuga_core_amd.adjoint import GaugeRun
     from anuga_core_amd.domain import Domain
     t0 = time.perf_counter()
-    mesh, q, bc, gauges = build_c4()
+    mesh, q, bc, gauges = build_c4(west=args.adjoint_west)
     N = mesh.number_of_triangles
     d = Domain(mesh=mesh, reorder=not args.no_reorder)
     d.set_flow_algorithm("DE1")
@@ -401,7 +406,7 @@ def run_adjoint(args):
         d.set_quantity(name, val, location="centroids")
     d.set_boundary(_boundaries(d, bc))
     init = dict(stage=q["stage"], xmomentum=np.zeros(N), ymomentum=np.zeros(N))
-    nsteps, dt, obs_every, every = args.steps, 0.1, 10, 100
+    nsteps, dt, obs_every, every = args.steps, args.adjoint_dt, 10, 100
     run = GaugeRun(d, init, nsteps, dt, gauges, obs_every, every)
     print("gradient workload ready: %d triangles in %.1fs" % (N, time.perf_counter() - t0),
           file=sys.stderr, flush=True)
@@ -424,6 +429,7 @@ def run_adjoint(args):
         return float(np.sum((run.forward(n, z) - obs) ** 2))
 
     rng = np.random.default_rng(7)
+    fd_eps = [float(e) for e in args.fd_eps.split(",")]
     checks = []
     for j in range(args.fd_directions):
         which = "friction" if j % 2 == 0 else "elevation"
@@ -431,7 +437,7 @@ def run_adjoint(args):
         dvec /= np.linalg.norm(dvec)
         ad = float(np.dot(gn if which == "friction" else gz, dvec))
         row = {"parameter": which, "adjoint": ad}
-        for eps in (1e-6, 1e-7):
+        for eps in fd_eps:
             if which == "friction":
                 fd = (loss(n0 + eps * dvec, z0) - loss(n0 - eps * dvec, z0)) / (2 * eps)
             else:
@@ -439,9 +445,10 @@ def run_adjoint(args):
             row["fd_%g" % eps] = fd
             row["rel_%g" % eps] = abs(ad - fd) / max(abs(fd), 1e-300)
         checks.append(row)
-        print("direction %d %s adjoint %.9e fd(1e-6) %.9e rel %.2e" %
-              (j, which, ad, row["fd_1e-06"], row["rel_1e-06"]), file=sys.stderr, flush=True)
-    best = [min(r["rel_1e-06"], r["rel_1e-07"]) for r in checks]
+        print("direction %d %s adjoint %.9e fd %s" %
+              (j, which, ad, " ".join("(%g) %.9e" % (e, row["fd_%g" % e]) for e in fd_eps)),
+              file=sys.stderr, flush=True)
+    best = [min(r["rel_%g" % e] for e in fd_eps) for r in checks]
     out = {
         "metric": "seconds for the discrete-adjoint gradient of a 2000-step DE1 gauge loss "
                   "(friction and bed per triangle)",
@@ -450,14 +457,15 @@ def run_adjoint(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "rectangular_cross(500,500) 5 km, DE1 fixed dt %.2f s, fully wet, "
                                "32 gauges every %d steps, true n = 0.03 + 0.01 smooth field "
-                               "(seed 2), start n = 0.03, checkpoint every %d steps"
-                               % (dt, obs_every, every),
+                               "(seed 2), start n = 0.03, checkpoint every %d steps, %s west, "
+                               "reflective elsewhere"
+                               % (dt, obs_every, every, args.adjoint_west),
                    "triangles": N},
         "forward_seconds": round(t_fwd, 3),
         "gradient_over_forward": round(t_adj / t_fwd, 2),
         "loss": loss0,
         "grad_norm": {"friction": float(np.linalg.norm(gn)), "elevation": float(np.linalg.norm(gz))},
-        "fd_check": {"directions": len(checks), "eps": [1e-6, 1e-7],
+        "fd_check": {"directions": len(checks), "eps": fd_eps,
                      "median_rel": float(np.median(best)) if best else None,
                      "max_rel": float(np.max(best)) if best else None,
                      "rows": checks},
@@ -483,6 +491,15 @@ def main():
                     help="single: the 16M-triangle mesh; ensemble: the batched scenarios; "
                          "adjoint: gradients on the 1M-triangle calibration case")
     ap.add_argument("--fd-directions", type=int, default=20)
+    ap.add_argument("--adjoint-west", default="transmissive",
+                    choices=["transmissive", "reflective"],
+                    help="west boundary of the gradient workload (the others are reflective)")
+    ap.add_argument("--adjoint-dt", type=float, default=0.025,
+                    help="fixed timestep (s) of the gradient workload; beyond about 60 s "
+                         "simulated the run is chaotic at rounding level and central "
+                         "differences no longer converge")
+    ap.add_argument("--fd-eps", default="1e-6,1e-7",
+                    help="central-difference steps of the gradient check (comma list)")
     ap.add_argument("--scenarios", type=int, default=512)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run 3 device copies of 1 GiB (a known-traffic kernel for "
