@@ -33,8 +33,8 @@ struct KernelTimer {
     int64_t launches = 0;
 };
 
-const char *kTimerNames[] = {"extrapolate", "flux0", "update", "flux1"};
-enum { T_EXTRAP = 0, T_FLUX0, T_UPDATE, T_FLUX1, T_COUNT };
+const char *kTimerNames[] = {"extrapolate", "flux0", "update", "flux1", "xflux0", "xflux1"};
+enum { T_EXTRAP = 0, T_FLUX0, T_UPDATE, T_FLUX1, T_XFLUX0, T_XFLUX1, T_COUNT };
 const int kTimerRing = 64;
 
 }  // namespace
@@ -70,6 +70,13 @@ struct anuga_domain {
     std::vector<int4> hnb;               // host copy of nb
     int *caller = nullptr;               // [N] caller's index within the scenario (-1: padding)
     double *vx = nullptr;                // [N][6] vertex coordinates (sloped friction), or NULL
+    // compact tiles (reorder 2, one FP64 scenario): per-tile halo half-edges
+    // and per-triangle halo slots for the fused extrapolation + flux kernel;
+    // hcap == 0: no tiles
+    int *tstart = nullptr, *thalo = nullptr;
+    unsigned *tslot = nullptr;
+    int hcap = 0;
+    bool e_stale = false;                // E does not hold the last extrapolation
     int *rw = nullptr;                   // [3][N] riverwall record per half-edge, -1: none
     DeRiverwall *rwd = nullptr;          // riverwall records; rw == nullptr: no riverwalls
     // output (one scenario): node CSR of the centroid averaging, in the
@@ -250,6 +257,53 @@ static std::vector<int> rcm_order(int N, const int64_t *neighbours)
     return order;
 }
 
+// Hilbert-curve index of (x, y) on a 2^order grid (the textbook
+// rotate-and-accumulate form).
+static uint64_t hilbert_index(uint32_t x, uint32_t y, int order)
+{
+    const uint32_t n = 1u << order;
+    uint64_t d = 0;
+    for (uint32_t s = n >> 1; s > 0; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = n - 1 - x;
+                y = n - 1 - y;
+            }
+            std::swap(x, y);
+        }
+    }
+    return d;
+}
+
+// Triangles sorted along a Hilbert curve through their centroids (ties by
+// index): consecutive runs of DE_BLOCK triangles are compact patches whose
+// boundary, the halo the fused kernel extrapolates twice, is short.
+static std::vector<int> tile_order(int N, const double *cen)
+{
+    double x0 = HUGE_VAL, y0 = HUGE_VAL, x1 = -HUGE_VAL, y1 = -HUGE_VAL;
+    for (int k = 0; k < N; k++) {
+        x0 = std::min(x0, cen[2 * k]);
+        x1 = std::max(x1, cen[2 * k]);
+        y0 = std::min(y0, cen[2 * k + 1]);
+        y1 = std::max(y1, cen[2 * k + 1]);
+    }
+    const int order = 16;
+    const double span = std::max(std::max(x1 - x0, y1 - y0), 1e-300);
+    const double scale = ((1 << order) - 1) / span;
+    std::vector<std::pair<uint64_t, int>> key(N);
+    for (int k = 0; k < N; k++) {
+        const double fx = std::min(std::max((cen[2 * k] - x0) * scale, 0.0), (double)((1 << order) - 1));
+        const double fy = std::min(std::max((cen[2 * k + 1] - y0) * scale, 0.0), (double)((1 << order) - 1));
+        key[k] = {hilbert_index((uint32_t)fx, (uint32_t)fy, order), k};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> o(N);
+    for (int k = 0; k < N; k++) o[k] = key[k].second;
+    return o;
+}
+
 static DeParams dev_params(const anuga_params &p)
 {
     DeParams q;
@@ -284,6 +338,7 @@ static int check_params(anuga_domain *d, const anuga_params *p)
         return fail(d, ANUGA_E_ARG, "timestep_fluxcalls must be 1 (euler), 2 (rk2) or 3 (rk3)");
     if (p->precision != ANUGA_PRECISION_FP64 && p->precision != ANUGA_PRECISION_MIXED)
         return fail(d, ANUGA_E_ARG, "unknown precision");
+    if (p->reorder < 0 || p->reorder > 2) return fail(d, ANUGA_E_ARG, "reorder must be 0, 1 or 2");
     return ANUGA_OK;
 }
 
@@ -368,6 +423,7 @@ static dim3 scenario_grid(const anuga_domain *d)
 
 static void launch_extrapolate(anuga_domain *d, DeQ *S)
 {
+    d->e_stale = false;
     timer_begin(d, T_EXTRAP);
     if (d->mixed)
         hipLaunchKernelGGL(de_extrapolate<float>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
@@ -419,6 +475,39 @@ static void launch_flux_p(anuga_domain *d, int substep, DeQ *S, DeQ *bk, DeQ *ou
     case 2: launch_flux_t<T, 2>(d, S, bk, out); break;
     default: launch_flux_t<T, 3>(d, S, bk, out); break;
     }
+}
+
+// The fused extrapolation + flux kernel applies to one FP64 scenario with
+// tiles and the common kernel's options, and to substeps 0 and 1 (rk3's
+// middle substep writes the state that other tiles are extrapolating).
+static bool fused(const anuga_domain *d)
+{
+    return d->hcap > 0 && !d->mixed && d->S == 1 && !d->rw && !d->P.low_froude &&
+           !d->P.use_sloped_mannings;
+}
+
+// extrapolation + flux of substep 0 (explicit updates) or 1 (advance S and
+// combine with bk into out; out must not be S), fused over tiles
+static void launch_xflux(anuga_domain *d, int substep, DeQ *S, DeQ *bk = nullptr,
+                         DeQ *out = nullptr)
+{
+    const int T = substep ? T_XFLUX1 : T_XFLUX0;
+    timer_begin(d, T);
+    DeBoundary bc{d->bkind, d->bval, d->M, d->bsstride};
+    DeTiles tl{d->tstart, d->thalo, d->tslot, d->hcap};
+    const size_t lds = (12 * (size_t)DE_BLOCK + 12 * (size_t)d->hcap) * sizeof(double);
+    if (substep == 0)
+        hipLaunchKernelGGL(de_xflux<0>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
+                           dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags, d->ex[0],
+                           d->ex[1], d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
+                           d->part + 2 * (size_t)d->nblocks, d->dneg);
+    else
+        hipLaunchKernelGGL(de_xflux<1>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
+                           dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags, d->ex[0],
+                           d->ex[1], d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
+                           d->part + 2 * (size_t)d->nblocks, d->dneg);
+    timer_end(d, T);
+    d->e_stale = true;
 }
 
 // substep 0: explicit updates of S; substep > 0: advance S with the scenario's
@@ -488,8 +577,12 @@ static void ensure_live(anuga_domain *d)
 static void enqueue_later_substeps(anuga_domain *d)
 {
     if (d->P.timestep_fluxcalls == 2) {
-        launch_extrapolate(d, d->B);
-        launch_flux(d, 1, d->B, d->A, d->A);
+        if (fused(d)) {
+            launch_xflux(d, 1, d->B, d->A, d->A);
+        } else {
+            launch_extrapolate(d, d->B);
+            launch_flux(d, 1, d->B, d->A, d->A);
+        }
         launch_reduce(d, 1);
     } else if (d->P.timestep_fluxcalls == 3) {
         launch_extrapolate(d, d->B);
@@ -514,8 +607,12 @@ static void launch_operators(anuga_domain *d)
 static void enqueue_device_step(anuga_domain *d)
 {
     const bool rk = d->P.timestep_fluxcalls > 1;
-    launch_extrapolate(d, d->A);
-    launch_flux(d, 0, d->A);
+    if (fused(d)) {
+        launch_xflux(d, 0, d->A);
+    } else {
+        launch_extrapolate(d, d->A);
+        launch_flux(d, 0, d->A);
+    }
     launch_reduce(d, 0);
     launch_control(d, de_timestep);
     launch_update(d, d->A, rk ? d->B : d->A);
@@ -763,6 +860,79 @@ static int validate_mesh(anuga_domain *d, const anuga_mesh *mesh, int64_t N, int
     return ANUGA_OK;
 }
 
+// Halo lists of the tiles of one scenario (internal numbering, nb records hn):
+// for tile b (triangles [b DE_BLOCK, (b + 1) DE_BLOCK)), every interior
+// half-edge whose neighbour lies outside the tile, as the neighbour's packed
+// (n << 2) | e; per triangle, each edge's slot in its tile's list (1023: the
+// neighbour is in the tile). No tiles (hcap 0) if a halo exceeds kMaxHalo
+// (the fused kernel's LDS is 96 B per tile triangle and per halo entry).
+static const int kMaxHalo = 320;
+
+// The tile metadata of triangles in internal order (nb: packed neighbours of
+// one scenario starting at 0). Returns the largest halo, or -1 when a
+// neighbour index is out of range; the caller refuses halos above kMaxHalo.
+static int tile_halos(int N, const int4 *hn, std::vector<int> &start, std::vector<int> &halo,
+                      std::vector<unsigned> &slot)
+{
+    const int nt = (N + DE_BLOCK - 1) / DE_BLOCK;
+    start.assign(nt + 1, 0);
+    halo.clear();
+    halo.reserve((size_t)nt * 96);
+    slot.assign(N, 0);
+    int hcap = 0;
+    for (int b = 0; b < nt; b++) {
+        start[b] = (int)halo.size();
+        const int k0 = b * DE_BLOCK, k1 = std::min(N, k0 + DE_BLOCK);
+        for (int k = k0; k < k1; k++) {
+            const int v[3] = {hn[k].x, hn[k].y, hn[k].z};
+            unsigned sl = 0;
+            for (int i = 0; i < 3; i++) {
+                unsigned si = 1023u;
+                if (v[i] >= 0) {
+                    const int n = v[i] >> 2;
+                    if (n >= N || (v[i] & 3) == 3) return -1;
+                    if (n < k0 || n >= k1) {
+                        si = (unsigned)std::min<size_t>(halo.size() - start[b], 1023);
+                        halo.push_back(v[i]);
+                    }
+                }
+                sl |= si << (10 * i);
+            }
+            slot[k] = sl;
+        }
+        hcap = std::max(hcap, (int)halo.size() - start[b]);
+    }
+    start[nt] = (int)halo.size();
+    return hcap;
+}
+
+static int build_tiles(anuga_domain *d, const std::vector<int4> &hn)
+{
+    d->hcap = 0;
+    for (int **p : {&d->tstart, &d->thalo}) {
+        if (*p) HIPCHK(d, hipFree(*p));
+        *p = nullptr;
+    }
+    if (d->tslot) HIPCHK(d, hipFree(d->tslot));
+    d->tslot = nullptr;
+    const int N = (int)hn.size(), nt = (N + DE_BLOCK - 1) / DE_BLOCK;
+    std::vector<int> start, halo;
+    std::vector<unsigned> slot;
+    const int hcap = tile_halos(N, hn.data(), start, halo, slot);
+    if (hcap < 0) return fail(d, ANUGA_E_ARG, "tiles: neighbour out of range");
+    if (hcap > kMaxHalo) return ANUGA_OK;
+    HIPCHK(d, dalloc(&d->tstart, (size_t)nt + 1));
+    HIPCHK(d, dalloc(&d->thalo, halo.size()));
+    HIPCHK(d, dalloc(&d->tslot, (size_t)N));
+    HIPCHK(d, hipMemcpy(d->tstart, start.data(), (nt + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (!halo.empty())
+        HIPCHK(d, hipMemcpy(d->thalo, halo.data(), halo.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(d, hipMemcpy(d->tslot, slot.data(), N * sizeof(unsigned), hipMemcpyHostToDevice));
+    // the LDS stride: a multiple of 32 doubles, never zero
+    d->hcap = std::max((hcap + 31) / 32 * 32, 32);
+    return ANUGA_OK;
+}
+
 // Upload the validated mesh of scenario s (its own triangle and boundary
 // numbering, as the reference's Mesh holds it).
 static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
@@ -771,7 +941,8 @@ static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
     const int64_t *nbr = mesh->neighbours, *nbe = mesh->neighbour_edges;
     const int64_t *sur = mesh->surrogate_neighbours, *nob = mesh->number_of_boundaries;
     std::vector<int> perm(N), iperm(N);
-    if (d->P.reorder) perm = rcm_order(N, nbr);
+    if (d->P.reorder == 2 && d->S == 1) perm = tile_order(N, mesh->centroid_coordinates);
+    else if (d->P.reorder) perm = rcm_order(N, nbr);
     else
         for (int k = 0; k < N; k++) perm[k] = k;
     for (int p = 0; p < N; p++) iperm[perm[p]] = p;
@@ -887,6 +1058,10 @@ static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
         if (!d->vx) HIPCHK(d, dalloc(&d->vx, 6 * (size_t)d->N));
         HIPCHK(d, hipMemcpy(d->vx + 6 * (size_t)kb, hv.data(), 6 * (size_t)N * sizeof(double),
                             hipMemcpyHostToDevice));
+    }
+    if (d->P.reorder == 2 && d->S == 1 && !d->mixed) {
+        int rc = build_tiles(d, hn);
+        if (rc) return rc;
     }
     if (!d->mesh_set[s]) {
         d->mesh_set[s] = 1;
@@ -1282,7 +1457,7 @@ void anuga_destroy(anuga_domain *d)
     if (d->stream) hipStreamSynchronize(d->stream);
     drop_graph(d);
     void *ptrs[] = {d->nb, d->cen, d->gx, d->gf, d->area, d->nx, d->ny, d->len, d->radius,
-                    d->scen, d->rw, d->rwd, d->caller, d->vx, d->node_start, d->node_tri, d->iperm_dev,
+                    d->scen, d->tstart, d->thalo, d->tslot, d->rw, d->rwd, d->caller, d->vx, d->node_start, d->node_tri, d->iperm_dev,
                     d->out_f32, d->gtri, d->gtimes, d->gvals, d->sub_idx, d->sub_val, d->eta, d->A, d->B, d->E, d->flags,
                     d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->ctl, d->dsetup,
                     d->dneg, d->hist};
@@ -1343,6 +1518,15 @@ int anuga_get_edge_values(anuga_domain *d, int quantity, double *values)
     if (quantity != ANUGA_STAGE && quantity != ANUGA_HEIGHT && quantity != ANUGA_XMOMENTUM &&
         quantity != ANUGA_YMOMENTUM && quantity != ANUGA_ELEVATION)
         return fail(d, ANUGA_E_ARG, "unknown edge quantity");
+    if (d->e_stale) {
+        // the last steps ran the fused kernel, whose edge values stay in LDS:
+        // extrapolate the current state (what distribute_to_vertices_and_edges
+        // computes, without writing protect's outcome back)
+        if (d->meshes_missing) return fail(d, ANUGA_E_STATE, "a scenario's mesh is not set");
+        launch_extrapolate(d, d->substate ? d->B : d->A);
+        rc = launch_check(d);
+        if (rc) return rc;
+    }
     HIPCHK(d, hipStreamSynchronize(d->stream));
     std::vector<double> e(4 * n3);
     if (d->mixed) {
@@ -2018,6 +2202,34 @@ int anuga_rcm_order(int64_t N, const int64_t *neighbours, int64_t *order)
         if (neighbours[i] >= N) return ANUGA_E_ARG;
     const std::vector<int> o = rcm_order((int)N, neighbours);
     for (int64_t i = 0; i < N; i++) order[i] = o[i];
+    return ANUGA_OK;
+}
+
+int anuga_tile_order(int64_t N, const double *centroids, int64_t *order)
+{
+    if (N < 0 || N >= (int64_t(1) << 29) || (N > 0 && (!centroids || !order))) return ANUGA_E_ARG;
+    const std::vector<int> o = tile_order((int)N, centroids);
+    for (int64_t k = 0; k < N; k++) order[k] = o[k];
+    return ANUGA_OK;
+}
+
+int anuga_tile_halos(int64_t N, const int32_t *nb, int64_t *start, int64_t halo_cap,
+                     int32_t *halo, uint32_t *slot, int64_t *nhalo, int64_t *hcap)
+{
+    if (N < 0 || N >= (int64_t(1) << 29) || (N > 0 && (!nb || !slot)) || !start || !nhalo || !hcap)
+        return ANUGA_E_ARG;
+    std::vector<int4> hn(N);
+    for (int64_t k = 0; k < N; k++) hn[k] = make_int4(nb[3 * k], nb[3 * k + 1], nb[3 * k + 2], 0);
+    std::vector<int> st, h;
+    std::vector<unsigned> sl;
+    const int c = tile_halos((int)N, hn.data(), st, h, sl);
+    if (c < 0) return ANUGA_E_ARG;
+    *nhalo = (int64_t)h.size();
+    *hcap = c;
+    if ((int64_t)h.size() > halo_cap || (!halo && !h.empty())) return ANUGA_E_ARG;
+    for (size_t b = 0; b < st.size(); b++) start[b] = st[b];
+    for (size_t q = 0; q < h.size(); q++) halo[q] = h[q];
+    for (int64_t k = 0; k < N; k++) slot[k] = sl[k];
     return ANUGA_OK;
 }
 

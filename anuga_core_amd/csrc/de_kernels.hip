@@ -61,6 +61,219 @@ __device__ __forceinline__ DeCentroid de_centroid(const DeQ q, unsigned f)
 // ---------------------------------------------------------------------------
 // extrapolation
 // ---------------------------------------------------------------------------
+// The extrapolation of one triangle k (sw_domain_orig.c:1215-1281,
+// :1994-2680): its protected centroid cell, the protect / momentum flag,
+// protect's added mass and the limited jumps from the centroid to its three
+// edges. Shared by the extrapolation kernel and the tiled fused kernel, so
+// both produce the same bits.
+template <class T>
+struct DeExtrap {
+    DeCell c;
+    T jw[3], jh[3], ju[3], jv[3];
+    bool constant, gw, gu, gv;
+    unsigned flag;
+    double mass;
+};
+
+template <class T>
+__device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &p,
+                                               const DeQ *__restrict__ q, int k, int *err,
+                                               DeExtrap<T> &x)
+{
+    const double mah = p.mah;
+    const int4 vk = m.nb[k];
+    const int v[3] = {vk.x, vk.y, vk.z};
+    const int nbound = vk.w;
+    const DeQ qk = q[k];
+    int sur[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) sur[i] = v[i] >= 0 ? (v[i] >> 2) : k;
+    // surrogate neighbours: one record each
+    DeQ qs[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) qs[i] = q[sur[i]];
+    double hs[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) hs[i] = de_height(qs[i].w, qs[i].z);
+
+    // protect + velocity of triangle k (sw_domain_orig.c:1245-1276, :2014-2064)
+    const bool clamp = de_clamped(qk.w, qk.z, mah);
+    DeCell &c = x.c;
+    c.w = clamp ? qk.z : qk.w;
+    c.h = de_height(c.w, qk.z);
+    bool dry_k = true;
+#pragma unroll
+    for (int i = 0; i < 3; i++) dry_k = dry_k && (v[i] < 0 || hs[i] < mah);
+    const bool keep = (c.h > mah) && !dry_k;
+    c.u = 0.0;
+    c.v = 0.0;
+    if (keep) {
+        const double inv = 1.0 / c.h;
+        c.u = qk.uh * inv;
+        c.v = qk.vh * inv;
+    }
+    x.mass = clamp ? (qk.z - qk.w) * m.area[k] : 0.0;
+    x.flag = (clamp ? DE_F_CLAMP : 0u) | (keep ? DE_F_KEEP : 0u);
+
+    // the same for the surrogate neighbours. A wet neighbour n keeps its
+    // velocity unless all of its own neighbours are dry; k is one of
+    // them, so the heights of n's other neighbours are needed only when
+    // k itself is dry (a boundary edge's surrogate is k: its test is dry_k)
+    DeCell nb[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const bool cl = de_clamped(qs[i].w, qs[i].z, mah);
+        nb[i].w = cl ? qs[i].z : qs[i].w;
+        nb[i].h = hs[i];
+        nb[i].u = 0.0;
+        nb[i].v = 0.0;
+        if (hs[i] > mah) {
+            bool dry = v[i] < 0 ? dry_k : (c.h < mah);
+            if (v[i] >= 0 && dry) {
+                const int4 vs = m.nb[sur[i]];
+                const int vv[3] = {vs.x, vs.y, vs.z};
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    if (vv[r] >= 0) {
+                        const DeQ t = q[vv[r] >> 2];
+                        dry = dry && (de_height(t.w, t.z) < mah);
+                    }
+                }
+            }
+            if (!dry) {
+                const double inv = 1.0 / hs[i];
+                nb[i].u = qs[i].uh * inv;
+                nb[i].v = qs[i].vh * inv;
+            }
+        }
+    }
+
+    const typename DeGeo<T>::X gx = static_cast<const typename DeGeo<T>::X *>(m.gx)[k];
+    T dxv[3], dyv[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        dxv[i] = gx.dxv[i];
+        dyv[i] = gx.dyv[i];
+    }
+    // limited jumps from the centroid to the edges, per quantity
+    T(&jw)[3] = x.jw, (&jh)[3] = x.jh, (&ju)[3] = x.ju, (&jv)[3] = x.jv;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        jw[i] = T(0.0);
+        jh[i] = T(0.0);
+        ju[i] = T(0.0);
+        jv[i] = T(0.0);
+    }
+    bool &constant = x.constant, &gw = x.gw, &gu = x.gu, &gv = x.gv;
+    constant = (nbound == 3);
+    gw = gu = gv = true;   // false: beta == 0, edge = centroid value
+    if (nbound <= 1) {
+        T dx1, dx2, dy1, dy2, area2, inv_area2;
+        if constexpr (sizeof(T) == 8) {
+            dx1 = gx.aux[0];
+            dx2 = gx.aux[1];
+            dy1 = gx.aux[2];
+            dy2 = gx.aux[3];
+            area2 = gx.aux[4];
+            inv_area2 = gx.aux[5];
+        } else {
+            dx1 = gx.rx[1] - gx.rx[0];
+            dx2 = gx.rx[2] - gx.rx[0];
+            dy1 = gx.ry[1] - gx.ry[0];
+            dy2 = gx.ry[2] - gx.ry[0];
+            area2 = dy2 * dx1 - dy1 * dx2;
+            inv_area2 = T(1.0) / area2;
+        }
+        if (area2 <= T(0.0)) {
+            constant = true;
+        } else {
+            const T a_tmp = T(0.3), b_tmp = T(0.1);
+            const T c_tmp = T(1.0) / (a_tmp - b_tmp);
+            const T d_tmp = T(1.0) - (c_tmp * a_tmp);
+            const T tmah = T(mah);
+            const T hc = T(c.h), h0 = T(nb[0].h), h1 = T(nb[1].h), h2 = T(nb[2].h);
+            const T hmin = fmin(fmin(h0, fmin(h1, h2)), hc);
+            const T hmax = fmax(fmax(h0, fmax(h1, h2)), hc);
+            T hfactor = fmax(T(0.0), fmin(c_tmp * fmax(hmin, T(0.0)) / fmax(hc, T(1.0e-06)) + d_tmp,
+                                          fmin(c_tmp * fmax(hc, T(0.0)) / fmax(hmax, T(1.0e-06)) + d_tmp,
+                                               T(1.0))));
+            hfactor = fmin(T(1.2) * fmax(hmin - tmah, T(0.0)) / (fmax(hmin, T(0.0)) + T(1.0) * tmah),
+                           hfactor);
+
+            T beta = T(p.bwd) + (T(p.bw) - T(p.bwd)) * hfactor;
+            gw = beta > T(0.0);
+            if (gw) {
+                de_aux_jumps<T>(c.w, nb[0].w, nb[1].w, nb[2].w, dx1, dx2, dy1, dy2, inv_area2,
+                                dxv, dyv, beta, jw);
+                de_aux_jumps<T>(c.h, nb[0].h, nb[1].h, nb[2].h, dx1, dx2, dy1, dy2, inv_area2,
+                                dxv, dyv, beta, jh);
+            }
+            beta = T(p.buhd) + (T(p.buh) - T(p.buhd)) * hfactor;
+            gu = beta > T(0.0);
+            if (gu)
+                de_aux_jumps<T>(c.u, nb[0].u, nb[1].u, nb[2].u, dx1, dx2, dy1, dy2, inv_area2,
+                                dxv, dyv, beta, ju);
+            beta = T(p.bvhd) + (T(p.bvh) - T(p.bvhd)) * hfactor;
+            gv = beta > T(0.0);
+            if (gv)
+                de_aux_jumps<T>(c.v, nb[0].v, nb[1].v, nb[2].v, dx1, dx2, dy1, dy2, inv_area2,
+                                dxv, dyv, beta, jv);
+        }
+    } else if (nbound == 2) {
+        // the first edge whose surrogate is not k itself (sw_domain_orig.c:2421-2439)
+        // selects rather than indexing, so the register arrays stay in registers
+        const bool e0 = v[0] >= 0, e1 = v[1] >= 0, e2 = v[2] >= 0;
+        if (!(e0 || e1 || e2)) {
+            atomicOr(err, DE_ERR_NO_NEIGHBOUR);
+            constant = true;
+        } else {
+            // field-wise selects on named copies: indexing the array here
+            // would place it in private (scratch) memory
+            const DeCell a0 = nb[0], a1 = nb[1], a2 = nb[2];
+            DeCell n1;
+            n1.w = e0 ? a0.w : (e1 ? a1.w : a2.w);
+            n1.h = e0 ? a0.h : (e1 ? a1.h : a2.h);
+            n1.u = e0 ? a0.u : (e1 ? a1.u : a2.u);
+            n1.v = e0 ? a0.v : (e1 ? a1.v : a2.v);
+            T dx2, dy2;
+            if constexpr (sizeof(T) == 8) {
+                dx2 = gx.aux[1];
+                dy2 = gx.aux[3];
+            } else {
+                const T dx1 = e0 ? gx.rx[0] : (e1 ? gx.rx[1] : gx.rx[2]);
+                const T dy1 = e0 ? gx.ry[0] : (e1 ? gx.ry[1] : gx.ry[2]);
+                const T area2 = dx1 * dx1 + dy1 * dy1;
+                dx2 = T(1.0) / area2;
+                dy2 = dx2 * dy1;
+                dx2 *= dx1;
+            }
+            const T bw = T(p.bw);
+            de_line_jumps<T>(c.w, n1.w, dx2, dy2, dxv, dyv, bw, jw);
+            de_line_jumps<T>(c.h, n1.h, dx2, dy2, dxv, dyv, bw, jh);
+            de_line_jumps<T>(c.u, n1.u, dx2, dy2, dxv, dyv, bw, ju);
+            de_line_jumps<T>(c.v, n1.v, dx2, dy2, dxv, dyv, bw, jv);
+        }
+    }
+}
+
+// FP64 edge values of edge i (compile-time i, or a select on a run-time one):
+// edge = centroid + limited jump; the beta == 0 and constant branches copy the
+// centroid value (sw_domain_orig.c:2278-2283)
+__device__ __forceinline__ void de_edge64(const DeExtrap<double> &x, int i, double &ew,
+                                          double &eh, double &euh, double &evh)
+{
+    const double jw = i == 0 ? x.jw[0] : (i == 1 ? x.jw[1] : x.jw[2]);
+    const double jh = i == 0 ? x.jh[0] : (i == 1 ? x.jh[1] : x.jh[2]);
+    const double ju = i == 0 ? x.ju[0] : (i == 1 ? x.ju[1] : x.ju[2]);
+    const double jv = i == 0 ? x.jv[0] : (i == 1 ? x.jv[1] : x.jv[2]);
+    ew = (x.constant || !x.gw) ? x.c.w : x.c.w + jw;
+    eh = (x.constant || !x.gw) ? x.c.h : x.c.h + jh;
+    const double eu = (x.constant || !x.gu) ? x.c.u : x.c.u + ju;
+    const double ev = (x.constant || !x.gv) ? x.c.v : x.c.v + jv;
+    euh = eu * eh;
+    evh = ev * eh;
+}
+
 template <class T>
 __global__ void __launch_bounds__(DE_BLOCK)
 de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict__ E,
@@ -77,173 +290,10 @@ de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict
     double mass = 0.0;
 
     if (k < cb->kend) {
-        const double mah = p.mah;
-        const int4 vk = m.nb[k];
-        const int v[3] = {vk.x, vk.y, vk.z};
-        const int nbound = vk.w;
-        const DeQ qk = q[k];
-        int sur[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) sur[i] = v[i] >= 0 ? (v[i] >> 2) : k;
-        // surrogate neighbours: one record each
-        DeQ qs[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) qs[i] = q[sur[i]];
-        double hs[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) hs[i] = de_height(qs[i].w, qs[i].z);
-
-        // protect + velocity of triangle k (sw_domain_orig.c:1245-1276, :2014-2064)
-        const bool clamp = de_clamped(qk.w, qk.z, mah);
-        DeCell c;
-        c.w = clamp ? qk.z : qk.w;
-        c.h = de_height(c.w, qk.z);
-        bool dry_k = true;
-#pragma unroll
-        for (int i = 0; i < 3; i++) dry_k = dry_k && (v[i] < 0 || hs[i] < mah);
-        const bool keep = (c.h > mah) && !dry_k;
-        c.u = 0.0;
-        c.v = 0.0;
-        if (keep) {
-            const double inv = 1.0 / c.h;
-            c.u = qk.uh * inv;
-            c.v = qk.vh * inv;
-        }
-        if (clamp) mass = (qk.z - qk.w) * m.area[k];
-        flags[k] = (clamp ? DE_F_CLAMP : 0u) | (keep ? DE_F_KEEP : 0u);
-
-        // the same for the surrogate neighbours. A wet neighbour n keeps its
-        // velocity unless all of its own neighbours are dry; k is one of
-        // them, so the heights of n's other neighbours are needed only when
-        // k itself is dry (a boundary edge's surrogate is k: its test is dry_k)
-        DeCell nb[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const bool cl = de_clamped(qs[i].w, qs[i].z, mah);
-            nb[i].w = cl ? qs[i].z : qs[i].w;
-            nb[i].h = hs[i];
-            nb[i].u = 0.0;
-            nb[i].v = 0.0;
-            if (hs[i] > mah) {
-                bool dry = v[i] < 0 ? dry_k : (c.h < mah);
-                if (v[i] >= 0 && dry) {
-                    const int4 vs = m.nb[sur[i]];
-                    const int vv[3] = {vs.x, vs.y, vs.z};
-#pragma unroll
-                    for (int r = 0; r < 3; r++) {
-                        if (vv[r] >= 0) {
-                            const DeQ t = q[vv[r] >> 2];
-                            dry = dry && (de_height(t.w, t.z) < mah);
-                        }
-                    }
-                }
-                if (!dry) {
-                    const double inv = 1.0 / hs[i];
-                    nb[i].u = qs[i].uh * inv;
-                    nb[i].v = qs[i].vh * inv;
-                }
-            }
-        }
-
-        const typename DeGeo<T>::X gx = static_cast<const typename DeGeo<T>::X *>(m.gx)[k];
-        T dxv[3], dyv[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            dxv[i] = gx.dxv[i];
-            dyv[i] = gx.dyv[i];
-        }
-        // limited jumps from the centroid to the edges, per quantity
-        T jw[3] = {T(0.0), T(0.0), T(0.0)}, jh[3] = {T(0.0), T(0.0), T(0.0)};
-        T ju[3] = {T(0.0), T(0.0), T(0.0)}, jv[3] = {T(0.0), T(0.0), T(0.0)};
-        bool constant = (nbound == 3);
-        bool gw = true, gu = true, gv = true;   // false: beta == 0, edge = centroid value
-        if (nbound <= 1) {
-            T dx1, dx2, dy1, dy2, area2, inv_area2;
-            if constexpr (sizeof(T) == 8) {
-                dx1 = gx.aux[0];
-                dx2 = gx.aux[1];
-                dy1 = gx.aux[2];
-                dy2 = gx.aux[3];
-                area2 = gx.aux[4];
-                inv_area2 = gx.aux[5];
-            } else {
-                dx1 = gx.rx[1] - gx.rx[0];
-                dx2 = gx.rx[2] - gx.rx[0];
-                dy1 = gx.ry[1] - gx.ry[0];
-                dy2 = gx.ry[2] - gx.ry[0];
-                area2 = dy2 * dx1 - dy1 * dx2;
-                inv_area2 = T(1.0) / area2;
-            }
-            if (area2 <= T(0.0)) {
-                constant = true;
-            } else {
-                const T a_tmp = T(0.3), b_tmp = T(0.1);
-                const T c_tmp = T(1.0) / (a_tmp - b_tmp);
-                const T d_tmp = T(1.0) - (c_tmp * a_tmp);
-                const T tmah = T(mah);
-                const T hc = T(c.h), h0 = T(nb[0].h), h1 = T(nb[1].h), h2 = T(nb[2].h);
-                const T hmin = fmin(fmin(h0, fmin(h1, h2)), hc);
-                const T hmax = fmax(fmax(h0, fmax(h1, h2)), hc);
-                T hfactor = fmax(T(0.0), fmin(c_tmp * fmax(hmin, T(0.0)) / fmax(hc, T(1.0e-06)) + d_tmp,
-                                              fmin(c_tmp * fmax(hc, T(0.0)) / fmax(hmax, T(1.0e-06)) + d_tmp,
-                                                   T(1.0))));
-                hfactor = fmin(T(1.2) * fmax(hmin - tmah, T(0.0)) / (fmax(hmin, T(0.0)) + T(1.0) * tmah),
-                               hfactor);
-
-                T beta = T(p.bwd) + (T(p.bw) - T(p.bwd)) * hfactor;
-                gw = beta > T(0.0);
-                if (gw) {
-                    de_aux_jumps<T>(c.w, nb[0].w, nb[1].w, nb[2].w, dx1, dx2, dy1, dy2, inv_area2,
-                                    dxv, dyv, beta, jw);
-                    de_aux_jumps<T>(c.h, nb[0].h, nb[1].h, nb[2].h, dx1, dx2, dy1, dy2, inv_area2,
-                                    dxv, dyv, beta, jh);
-                }
-                beta = T(p.buhd) + (T(p.buh) - T(p.buhd)) * hfactor;
-                gu = beta > T(0.0);
-                if (gu)
-                    de_aux_jumps<T>(c.u, nb[0].u, nb[1].u, nb[2].u, dx1, dx2, dy1, dy2, inv_area2,
-                                    dxv, dyv, beta, ju);
-                beta = T(p.bvhd) + (T(p.bvh) - T(p.bvhd)) * hfactor;
-                gv = beta > T(0.0);
-                if (gv)
-                    de_aux_jumps<T>(c.v, nb[0].v, nb[1].v, nb[2].v, dx1, dx2, dy1, dy2, inv_area2,
-                                    dxv, dyv, beta, jv);
-            }
-        } else if (nbound == 2) {
-            // the first edge whose surrogate is not k itself (sw_domain_orig.c:2421-2439)
-            // selects rather than indexing, so the register arrays stay in registers
-            const bool e0 = v[0] >= 0, e1 = v[1] >= 0, e2 = v[2] >= 0;
-            if (!(e0 || e1 || e2)) {
-                atomicOr(err, DE_ERR_NO_NEIGHBOUR);
-                constant = true;
-            } else {
-                // field-wise selects on named copies: indexing the array here
-                // would place it in private (scratch) memory
-                const DeCell a0 = nb[0], a1 = nb[1], a2 = nb[2];
-                DeCell n1;
-                n1.w = e0 ? a0.w : (e1 ? a1.w : a2.w);
-                n1.h = e0 ? a0.h : (e1 ? a1.h : a2.h);
-                n1.u = e0 ? a0.u : (e1 ? a1.u : a2.u);
-                n1.v = e0 ? a0.v : (e1 ? a1.v : a2.v);
-                T dx2, dy2;
-                if constexpr (sizeof(T) == 8) {
-                    dx2 = gx.aux[1];
-                    dy2 = gx.aux[3];
-                } else {
-                    const T dx1 = e0 ? gx.rx[0] : (e1 ? gx.rx[1] : gx.rx[2]);
-                    const T dy1 = e0 ? gx.ry[0] : (e1 ? gx.ry[1] : gx.ry[2]);
-                    const T area2 = dx1 * dx1 + dy1 * dy1;
-                    dx2 = T(1.0) / area2;
-                    dy2 = dx2 * dy1;
-                    dx2 *= dx1;
-                }
-                const T bw = T(p.bw);
-                de_line_jumps<T>(c.w, n1.w, dx2, dy2, dxv, dyv, bw, jw);
-                de_line_jumps<T>(c.h, n1.h, dx2, dy2, dxv, dyv, bw, jh);
-                de_line_jumps<T>(c.u, n1.u, dx2, dy2, dxv, dyv, bw, ju);
-                de_line_jumps<T>(c.v, n1.v, dx2, dy2, dxv, dyv, bw, jv);
-            }
-        }
+        DeExtrap<T> x;
+        de_extrap_cell<T>(m, p, q, k, err, x);
+        mass = x.mass;
+        flags[k] = x.flag;
         if constexpr (sizeof(T) == 8) {
             // FP64 edge values: four [edge][triangle] arrays (w, h, uh, vh)
             double *__restrict__ ow = reinterpret_cast<double *>(E);
@@ -252,30 +302,26 @@ de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict
             double *__restrict__ ovh = ow + 9 * (size_t)N;
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                // edge = centroid + limited jump; the beta == 0 and constant
-                // branches copy the centroid value (sw_domain_orig.c:2278-2283)
-                const double ew = (constant || !gw) ? c.w : c.w + jw[i];
-                const double eh = (constant || !gw) ? c.h : c.h + jh[i];
-                const double eu = (constant || !gu) ? c.u : c.u + ju[i];
-                const double ev = (constant || !gv) ? c.v : c.v + jv[i];
+                double ew, eh, euh, evh;
+                de_edge64(x, i, ew, eh, euh, evh);
                 const int ki = i * N + k;
                 ow[ki] = ew;
                 oh[ki] = eh;
-                ouh[ki] = eu * eh;
-                ovh[ki] = ev * eh;
+                ouh[ki] = euh;
+                ovh[ki] = evh;
             }
         } else {
             DeEdgeRec32 rec;
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                const float hw = constant ? 0.f : float(jw[i]);
-                const float hh = constant ? 0.f : float(jh[i]);
-                const float eh = float(c.h + double(hh));
-                const float eu = float(c.u + (constant ? 0.0 : double(ju[i])));
-                const float ev = float(c.v + (constant ? 0.0 : double(jv[i])));
+                const float hw = x.constant ? 0.f : float(x.jw[i]);
+                const float hh = x.constant ? 0.f : float(x.jh[i]);
+                const float eh = float(x.c.h + double(hh));
+                const float eu = float(x.c.u + (x.constant ? 0.0 : double(x.ju[i])));
+                const float ev = float(x.c.v + (x.constant ? 0.0 : double(x.jv[i])));
                 rec.e[i] = make_float4(hw, hh, eu * eh, ev * eh);
             }
-            rec.hc = c.h;
+            rec.hc = x.c.h;
             rec.pad = 0.0;
             reinterpret_cast<DeEdgeRec32 *>(E)[k] = rec;
         }
@@ -476,6 +522,119 @@ __device__ __forceinline__ void de_riverwall_weir(const DeRiverwall &rw, double 
 // ---------------------------------------------------------------------------
 // flux
 // ---------------------------------------------------------------------------
+// One edge's terms of triangle k's flux sums (the body of the reference's
+// edge loop, sw_domain_orig.c:828-1004, for the half-edge (k, i)): own edge
+// values ql0 (stage), hle (depth), ql1, ql2 (momenta); for an interior edge
+// (vi >= 0) the neighbour's at the shared edge, for a boundary edge the
+// boundary state evaluated here. Adds to the sums sw, sx, sy, the boundary
+// inflow and the CFL bound. Shared by the flux kernel and the tiled fused
+// kernel, so both produce the same bits.
+template <class T, int SUBSTEP, bool GENERAL>
+__device__ __forceinline__ void de_edge_flux(const DeMesh &m, const DeParams &p,
+                                             const DeBoundary &bc, const DeQ *q,
+                                             const unsigned char *__restrict__ flags, int N,
+                                             int k, int i, int vi, const DeCentroid &c, T hc,
+                                             T zc, T ql0, T hle, T ql1, T ql2, T qr0, T hre,
+                                             T qr1, T qr2, double &sw, double &sx, double &sy,
+                                             double &bflux, double &dtmin)
+{
+    const T g = T(p.g), eps = T(p.eps);
+    const T zl = ql0 - hle;
+    const T n1 = m.nx[i * N + k], n2 = m.ny[i * N + k];
+    T zr;
+    if (vi >= 0) {
+        zr = qr0 - hre;
+    } else {
+        const int j = -vi - 1;
+        const int kind = bc.kind[j];
+        const double *bv = bc.val + (size_t)de_bc_slot<SUBSTEP>() * bc.sstride;
+        if (kind == 0) {          // reflective
+            const T r1 = -ql1 * n1 - ql2 * n2;
+            const T r2 = -ql1 * n2 + ql2 * n1;
+            qr0 = ql0;
+            qr1 = n1 * r1 - n2 * r2;
+            qr2 = n2 * r1 + n1 * r2;
+        } else if (kind == 1) {   // transmissive
+            qr0 = ql0;
+            qr1 = ql1;
+            qr2 = ql2;
+        } else if (kind == 2) {   // Dirichlet (constant or time-dependent)
+            qr0 = T(bv[j]);
+            qr1 = T(bv[bc.M + j]);
+            qr2 = T(bv[2 * bc.M + j]);
+        } else if (kind == 3) {   // transmissive from centroid values
+            qr0 = T(c.w);
+            qr1 = T(c.uh);
+            qr2 = T(c.vh);
+        } else if (kind == 4) {   // stage set, normal momentum kept, tangential 0
+            const T nq = n1 * ql1 + n2 * ql2;
+            qr0 = T(bv[j]);
+            qr1 = nq * n1;
+            qr2 = nq * n2;
+        } else if (kind == 5) {   // transmissive stage, zero momentum
+            qr0 = ql0;
+            qr1 = T(0.0);
+            qr2 = T(0.0);
+        } else {                  // stage set, momentum transmissive
+            qr0 = T(bv[j]);
+            qr1 = ql1;
+            qr2 = ql2;
+        }
+        zr = zl;
+        hre = fmax(qr0 - zr, T(0.0));
+    }
+    T z_half = fmax(zl, zr);
+    int wall = -1;
+    bool mirrored = false;
+    if constexpr (GENERAL) {
+        // the wall is a thin bed step between the two half-edges
+        // (sw_domain_orig.c:880-893)
+        if (m.rw) {
+            wall = m.rw[i * N + k];
+            if (wall >= 0) z_half = fmax(m.rwd[wall].elev, z_half);
+        }
+        if (p.low_froude && vi >= 0) mirrored = m.caller[vi >> 2] < m.caller[k];
+    }
+    const T h_left = fmax(hle + zl - z_half, T(0.0));
+    const T h_right = fmax(hre + zr - z_half, T(0.0));
+    T ef[3], speed, pflux;
+    if constexpr (GENERAL)
+        de_flux_central<T>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1,
+                           n2, eps, z_half, g, ef, speed, pflux, p.low_froude, mirrored);
+    else
+        de_flux_central<T>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1,
+                           n2, eps, z_half, g, ef, speed, pflux);
+    if constexpr (GENERAL) {
+        if (wall >= 0) {
+            double wn = 0.0, zn = c.z;
+            if (vi >= 0) {
+                const int n = vi >> 2;
+                const DeCentroid cn = de_centroid(q[n], flags[n]);
+                wn = cn.w;
+                zn = cn.z;
+            }
+            de_riverwall_weir(m.rwd[wall], p.g, vi >= 0, c.w, c.z, hc, wn, zn, zl, zr,
+                              z_half, ef, speed);
+        }
+    }
+    const T length = m.len[i * N + k];
+    ef[0] *= length;
+    ef[1] *= length;
+    ef[2] *= length;
+    const T pg = length * (-g * T(0.5) * (h_left * h_left - hle * hle - (hle + hc) * (zl - zc))
+                           + pflux);
+    sw += double(-ef[0]);
+    sx += double(-ef[1]);
+    sy += double(-ef[2]);
+    if (vi < 0) bflux += double(-ef[0]);
+    sx -= double(n1 * pg);
+    sy -= double(n2 * pg);
+    if (SUBSTEP == 0 && double(speed) > p.eps) {
+        const double t = 1.0 / fmax(double(speed), p.eps);
+        dtmin = fmin(dtmin, m.radius[k] * t);
+    }
+}
+
 // SUBSTEP 0: writes explicit updates; SUBSTEP > 0: advances with the
 // scenario's dt and writes de_rk_combine<SUBSTEP>(advanced, bk) to out (out
 // may be bk, or q itself: each thread reads its own q and bk before writing).
@@ -517,105 +676,17 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
         for (int i = 0; i < 3; i++) {
             const int ki = i * N + k;
             const T ql0 = Ew[ki], hle = Eh[ki], ql1 = Euh[ki], ql2 = Evh[ki];
-            const T zl = ql0 - hle;
-            const T n1 = m.nx[i * N + k], n2 = m.ny[i * N + k];
-            T qr0, qr1, qr2, zr, hre;
+            T qr0 = T(0.0), qr1 = T(0.0), qr2 = T(0.0), hre = T(0.0);
             if (v[i] >= 0) {
                 const int nm = (v[i] & 3) * N + (v[i] >> 2);
                 qr0 = Ew[nm];
                 qr1 = Euh[nm];
                 qr2 = Evh[nm];
                 hre = Eh[nm];
-                zr = qr0 - hre;
-            } else {
-                const int j = -v[i] - 1;
-                const int kind = bc.kind[j];
-                const double *bv = bc.val + (size_t)de_bc_slot<SUBSTEP>() * bc.sstride;
-                if (kind == 0) {          // reflective
-                    const T r1 = -ql1 * n1 - ql2 * n2;
-                    const T r2 = -ql1 * n2 + ql2 * n1;
-                    qr0 = ql0;
-                    qr1 = n1 * r1 - n2 * r2;
-                    qr2 = n2 * r1 + n1 * r2;
-                } else if (kind == 1) {   // transmissive
-                    qr0 = ql0;
-                    qr1 = ql1;
-                    qr2 = ql2;
-                } else if (kind == 2) {   // Dirichlet (constant or time-dependent)
-                    qr0 = T(bv[j]);
-                    qr1 = T(bv[bc.M + j]);
-                    qr2 = T(bv[2 * bc.M + j]);
-                } else if (kind == 3) {   // transmissive from centroid values
-                    qr0 = T(c.w);
-                    qr1 = T(c.uh);
-                    qr2 = T(c.vh);
-                } else if (kind == 4) {   // stage set, normal momentum kept, tangential 0
-                    const T nq = n1 * ql1 + n2 * ql2;
-                    qr0 = T(bv[j]);
-                    qr1 = nq * n1;
-                    qr2 = nq * n2;
-                } else if (kind == 5) {   // transmissive stage, zero momentum
-                    qr0 = ql0;
-                    qr1 = T(0.0);
-                    qr2 = T(0.0);
-                } else {                  // stage set, momentum transmissive
-                    qr0 = T(bv[j]);
-                    qr1 = ql1;
-                    qr2 = ql2;
-                }
-                zr = zl;
-                hre = fmax(qr0 - zr, T(0.0));
             }
-            T z_half = fmax(zl, zr);
-            int wall = -1;
-            bool mirrored = false;
-            if constexpr (GENERAL) {
-                // the wall is a thin bed step between the two half-edges
-                // (sw_domain_orig.c:880-893)
-                if (m.rw) {
-                    wall = m.rw[i * N + k];
-                    if (wall >= 0) z_half = fmax(m.rwd[wall].elev, z_half);
-                }
-                if (p.low_froude && v[i] >= 0) mirrored = m.caller[v[i] >> 2] < m.caller[k];
-            }
-            const T h_left = fmax(hle + zl - z_half, T(0.0));
-            const T h_right = fmax(hre + zr - z_half, T(0.0));
-            T ef[3], speed, pflux;
-            if constexpr (GENERAL)
-                de_flux_central<T>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1,
-                                   n2, eps, z_half, g, ef, speed, pflux, p.low_froude, mirrored);
-            else
-                de_flux_central<T>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1,
-                                   n2, eps, z_half, g, ef, speed, pflux);
-            if constexpr (GENERAL) {
-                if (wall >= 0) {
-                    double wn = 0.0, zn = c.z;
-                    if (v[i] >= 0) {
-                        const int n = v[i] >> 2;
-                        const DeCentroid cn = de_centroid(q[n], flags[n]);
-                        wn = cn.w;
-                        zn = cn.z;
-                    }
-                    de_riverwall_weir(m.rwd[wall], p.g, v[i] >= 0, c.w, c.z, hc, wn, zn, zl, zr,
-                                      z_half, ef, speed);
-                }
-            }
-            const T length = m.len[i * N + k];
-            ef[0] *= length;
-            ef[1] *= length;
-            ef[2] *= length;
-            const T pg = length * (-g * T(0.5) * (h_left * h_left - hle * hle - (hle + hc) * (zl - zc))
-                                   + pflux);
-            sw += double(-ef[0]);
-            sx += double(-ef[1]);
-            sy += double(-ef[2]);
-            if (v[i] < 0) bflux += double(-ef[0]);
-            sx -= double(n1 * pg);
-            sy -= double(n2 * pg);
-            if (SUBSTEP == 0 && double(speed) > p.eps) {
-                const double t = 1.0 / fmax(double(speed), p.eps);
-                dtmin = fmin(dtmin, m.radius[k] * t);
-            }
+            de_edge_flux<T, SUBSTEP, GENERAL>(m, p, bc, q, flags, N, k, i, v[i], c, hc, zc, ql0,
+                                              hle, ql1, ql2, qr0, hre, qr1, qr2, sw, sx, sy,
+                                              bflux, dtmin);
         }
         const double inv_area = 1.0 / m.area[k];
         sw *= inv_area;
@@ -650,6 +721,144 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
         }
         part_dt[lb] = a;
         part_bf[lb] = b;
+    }
+}
+
+// Tiled fused extrapolation + flux (FP64, the common kernel's options,
+// SUBSTEP 0 or 1). A workgroup extrapolates its tile's DE_BLOCK triangles and
+// the triangles across its halo half-edges (the neighbour sides of edges that
+// leave the tile) into LDS, then each thread gathers its triangle's three edge fluxes
+// from LDS. The edge values never reach HBM. The arithmetic is
+// de_extrap_cell and de_edge_flux, as in de_extrapolate and de_flux, and the
+// per-block partials (CFL, boundary inflow, protect mass) are formed in the
+// same order, so the results are the same bits. The state read (q) is not
+// written by this kernel (SUBSTEP 1 writes out = the step's initial state).
+template <int SUBSTEP>
+__global__ void __launch_bounds__(DE_BLOCK)
+de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restrict__ eta,
+         DeTiles tl, DeBoundary bc, unsigned char *__restrict__ flags, double *__restrict__ exw,
+         double *__restrict__ exuh, double *__restrict__ exvh, const DeQ *bk, DeQ *out,
+         DeControl *ctl, double *__restrict__ part_dt, double *__restrict__ part_bf,
+         double *__restrict__ part_mass, unsigned long long *__restrict__ nneg)
+{
+    // [4 quantities][3 edges][DE_BLOCK] tile edge values, then [4][3][hcap] halo
+    extern __shared__ double lds[];
+    __shared__ double red_dt[DE_BLOCK / 64], red_bf[DE_BLOCK / 64], red_m[DE_BLOCK / 64];
+    const int lb = xcd_block();
+    DeControl *cb = de_ctl(ctl, m.scen, lb);
+    if (cb->done) return;
+    int *err = &cb->status;
+    const double dt = SUBSTEP ? cb->dt : 0.0;
+    const int N = m.N, t = threadIdx.x;
+    const int k0 = lb * DE_BLOCK, k = k0 + t;
+    const int kend = cb->kend;
+    const int hcap = tl.hcap;
+    double *const hl = lds + 12 * DE_BLOCK;
+    double mass = 0.0, dtmin = 1.0e100, bflux = 0.0;
+    unsigned fk = 0;
+
+    if (k < kend) {
+        DeExtrap<double> x;
+        de_extrap_cell<double>(m, p, q, k, err, x);
+        mass = x.mass;
+        fk = x.flag;
+        flags[k] = (unsigned char)fk;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            double ew, eh, euh, evh;
+            de_edge64(x, i, ew, eh, euh, evh);
+            lds[(0 + i) * DE_BLOCK + t] = ew;
+            lds[(3 + i) * DE_BLOCK + t] = eh;
+            lds[(6 + i) * DE_BLOCK + t] = euh;
+            lds[(9 + i) * DE_BLOCK + t] = evh;
+        }
+    }
+    // halo triangles: all three edges, so that the edge is chosen by the
+    // reader's LDS address (a run-time choice between the registers of x
+    // would place x in scratch)
+    const int h0 = tl.start[lb], nh = tl.start[lb + 1] - h0;
+    for (int j = t; j < nh; j += DE_BLOCK) {
+        DeExtrap<double> x;
+        de_extrap_cell<double>(m, p, q, tl.halo[h0 + j] >> 2, err, x);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            double ew, eh, euh, evh;
+            de_edge64(x, i, ew, eh, euh, evh);
+            hl[(0 + i) * hcap + j] = ew;
+            hl[(3 + i) * hcap + j] = eh;
+            hl[(6 + i) * hcap + j] = euh;
+            hl[(9 + i) * hcap + j] = evh;
+        }
+    }
+    __syncthreads();
+
+    if (k < kend) {
+        const DeCentroid c = de_centroid(q[k], fk);
+        const int4 vk = m.nb[k];
+        const int v[3] = {vk.x, vk.y, vk.z};
+        const unsigned sl = tl.slot[k];
+        const double hc = de_height(c.w, c.z);
+        const double zc = c.z;
+        double sw = 0.0, sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const double ql0 = lds[(0 + i) * DE_BLOCK + t], hle = lds[(3 + i) * DE_BLOCK + t];
+            const double ql1 = lds[(6 + i) * DE_BLOCK + t], ql2 = lds[(9 + i) * DE_BLOCK + t];
+            double qr0 = 0.0, qr1 = 0.0, qr2 = 0.0, hre = 0.0;
+            if (v[i] >= 0) {
+                const unsigned s = (sl >> (10 * i)) & 1023u;
+                const double *src;
+                int stride;
+                if (s == 1023u) {
+                    src = lds + (v[i] & 3) * DE_BLOCK + ((v[i] >> 2) - k0);
+                    stride = 3 * DE_BLOCK;
+                } else {
+                    src = hl + (v[i] & 3) * hcap + s;
+                    stride = 3 * hcap;
+                }
+                qr0 = src[0];
+                hre = src[stride];
+                qr1 = src[2 * stride];
+                qr2 = src[3 * stride];
+            }
+            de_edge_flux<double, SUBSTEP, false>(m, p, bc, q, flags, N, k, i, v[i], c, hc, zc,
+                                                 ql0, hle, ql1, ql2, qr0, hre, qr1, qr2, sw, sx,
+                                                 sy, bflux, dtmin);
+        }
+        const double inv_area = 1.0 / m.area[k];
+        sw *= inv_area;
+        sx *= inv_area;
+        sy *= inv_area;
+        if (SUBSTEP == 0) {
+            exw[k] = sw;
+            exuh[k] = sx;
+            exvh[k] = sy;
+        } else {
+            const DeQ r = de_advance<true>(c, eta[k], sw, sx, sy, dt, p, err, nneg);
+            out[k] = de_rk_combine<SUBSTEP>(r, bk[k]);
+        }
+    }
+
+    dtmin = de_wave_min(dtmin);
+    bflux = de_wave_sum(bflux);
+    mass = de_wave_sum(mass);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red_dt[wv] = dtmin;
+        red_bf[wv] = bflux;
+        red_m[wv] = mass;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = red_dt[0], b = red_bf[0], cm = red_m[0];
+        for (int i = 1; i < DE_BLOCK / 64; i++) {
+            a = fmin(a, red_dt[i]);
+            b += red_bf[i];
+            cm += red_m[i];
+        }
+        part_dt[lb] = a;
+        part_bf[lb] = b;
+        part_mass[lb] = cm;
     }
 }
 

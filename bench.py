@@ -52,6 +52,13 @@ KERNEL_BYTES = {
         #   + normals (48) + lengths (24) + area (8) + n (8) + step-initial
         #   w,uh,vh (24) -> w,uh,vh (24)
         "flux1": ("de_flux<1>", 277),
+        # the tiled fused kernels (Hilbert tiles): extrapolation inputs w,uh,vh,z
+        #   (32) + neighbours (12) + centroid (16) + edge offsets (48) + normals
+        #   (48) + lengths (24) + area, radius (16) -> flag (1) + explicit (24)
+        "xflux0": ("de_xflux<0>", 221),
+        # extrapolation inputs (108) + normals (48) + lengths (24) + area (8)
+        #   + n (8) + step-initial w,uh,vh (24) -> flag (1) + w,uh,vh (24)
+        "xflux1": ("de_xflux<1>", 245),
     },
     "mixed": {
         # w,uh,vh,z (32) + neighbours (12) + FP32 edge and neighbour offsets (48)
@@ -67,7 +74,8 @@ KERNEL_BYTES = {
         "flux1": ("de_flux_mixed<1>", 213),
     },
 }
-LAUNCHES_PER_STEP = {"extrapolate": 2, "flux0": 1, "update": 1, "flux1": 1}
+LAUNCHES_PER_STEP = {"extrapolate": 2, "flux0": 1, "update": 1, "flux1": 1, "xflux0": 1,
+                     "xflux1": 1}
 STEP_BYTES = {"fp64": 488, "mixed": 344}
 PROFILE = os.path.join(ROOT, "profiles", "de_kernels_rectangular_cross2000_DE1_%s.txt")
 
@@ -91,6 +99,10 @@ def measured_traffic(precision, kernel, m):
             if cols[0] == kernel and "hbm_bytes" in header:
                 return float(cols[header.index("hbm_bytes")])
     return None
+
+
+def _reorder(args):
+    return {"tiles": True, "rcm": "rcm", "none": False}["none" if args.no_reorder else args.order]
 
 
 def build_c1(m=2000, length=20000.0, nhills=64, seed=0):
@@ -150,7 +162,7 @@ def run_timed(precision, mesh, q, bc, args):
     eager per-kernel timing pass. Returns (domain, result dict)."""
     from anuga_core_amd.domain import Domain
     t0 = time.perf_counter()
-    d = Domain(mesh=mesh, precision=precision, reorder=not args.no_reorder)
+    d = Domain(mesh=mesh, precision=precision, reorder=_reorder(args))
     d.set_flow_algorithm("DE1")
     for name, val in q.items():
         d.set_quantity(name, val, location="centroids")
@@ -180,6 +192,8 @@ def run_timed(precision, mesh, q, bc, args):
     N = mesh.number_of_triangles
     kernels = {}
     for short, (ms, n) in kt.items():
+        if n == 0:
+            continue   # not run in this configuration (fused or separate kernels)
         kname, nbytes = KERNEL_BYTES[precision][short]
         kernels[kname] = {"avg_ms": round(ms, 5), "launches": n,
                           "alg_bytes_per_triangle": nbytes,
@@ -260,7 +274,7 @@ def run_ensemble(args):
     order = sorted(range(len(spec)), key=lambda i: spec[i][0])
     sizes = [(4 * k * k, 4 * k) for k, _ in spec]
     ens = Ensemble(sizes, flow_algorithm="DE1", precision=args.precision,
-                   reorder=not args.no_reorder)
+                   reorder=_reorder(args))
     mesh, mesh_k = None, None
     for j, i in enumerate(order):
         k, prm = spec[i]
@@ -292,12 +306,13 @@ def run_ensemble(args):
     drift = np.abs(v1 - v0 - bfi) / v0
     kernels = {}
     for short, (ms, n) in kt.items():
+        if n == 0:
+            continue
         kname, nbytes = KERNEL_BYTES[args.precision][short]
         kernels[kname] = {"avg_ms": round(ms, 5), "launches": n,
                           "alg_bytes_per_triangle": nbytes,
                           "alg_GBs": round(nbytes * Ntot / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
-    dom_short = max(KERNEL_BYTES[args.precision],
-                    key=lambda s: kt[s][0] * LAUNCHES_PER_STEP[s])
+    dom_short = max((s for s in kt if kt[s][1] > 0), key=lambda s: kt[s][0] * LAUNCHES_PER_STEP[s])
     dom, dom_bytes = KERNEL_BYTES[args.precision][dom_short]
     ach = kernels[dom]["alg_GBs"]
     ms_per_step = 1e3 * elapsed / args.steps
@@ -400,7 +415,7 @@ def run_adjoint(args):
     t0 = time.perf_counter()
     mesh, q, bc, gauges = build_c4(west=args.adjoint_west)
     N = mesh.number_of_triangles
-    d = Domain(mesh=mesh, reorder=not args.no_reorder)
+    d = Domain(mesh=mesh, reorder=_reorder(args))
     d.set_flow_algorithm("DE1")
     for name, val in q.items():
         d.set_quantity(name, val, location="centroids")
@@ -486,7 +501,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the second precision (speed and mixed-vs-FP64 error)")
-    ap.add_argument("--no-reorder", action="store_true")
+    ap.add_argument("--order", default="tiles", choices=["tiles", "rcm", "none"],
+                    help="internal triangle order: Hilbert tiles (fused FP64 kernel), "
+                         "Reverse Cuthill-McKee, or the mesh's own")
+    ap.add_argument("--no-reorder", action="store_true", help="same as --order none")
     ap.add_argument("--workload", default="single", choices=["single", "ensemble", "adjoint"],
                     help="single: the 16M-triangle mesh; ensemble: the batched scenarios; "
                          "adjoint: gradients on the 1M-triangle calibration case")
@@ -537,8 +555,8 @@ def main():
     d, r = run_timed(args.precision, mesh, q, bc, args)
     kernels = r["kernels"]
     # dominant kernel: largest total time per step
-    dom_short = max(KERNEL_BYTES[args.precision],
-                    key=lambda s: kernels[KERNEL_BYTES[args.precision][s][0]]["avg_ms"]
+    ran = [s for s in KERNEL_BYTES[args.precision] if KERNEL_BYTES[args.precision][s][0] in kernels]
+    dom_short = max(ran, key=lambda s: kernels[KERNEL_BYTES[args.precision][s][0]]["avg_ms"]
                     * LAUNCHES_PER_STEP[s])
     dom, dom_bytes = KERNEL_BYTES[args.precision][dom_short]
     ach = kernels[dom]["alg_GBs"]
@@ -562,7 +580,7 @@ def main():
                                "hills (rng seed 0), 3 m hump, n=0.025, transmissive west"
                                % (args.m, args.m, 20.0),
                    "triangles": N, "flow_algorithm": "DE1", "precision": args.precision,
-                   "reorder": "rcm" if not args.no_reorder else "none"},
+                   "order": args.order},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4),

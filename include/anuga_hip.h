@@ -93,7 +93,11 @@ typedef struct anuga_params {
     int64_t extrapolate_velocity_second_order; /* only 1 (the DE presets) */
     int64_t timestep_fluxcalls;                /* 1 = euler (DE0), 2 = rk2 (DE1), 3 = rk3 (DE2) */
     int64_t precision;                         /* ANUGA_PRECISION_* */
-    int64_t reorder;                           /* 1 = Reverse-Cuthill-McKee renumbering */
+    int64_t reorder;                           /* internal triangle order: 0 = the caller's,
+                                                  1 = Reverse-Cuthill-McKee, 2 = Hilbert-curve
+                                                  tiles (one FP64 scenario: the fused
+                                                  extrapolation + flux kernel; results are
+                                                  the same bits in every order) */
     int64_t use_sloped_mannings;               /* 1 = _manning_friction_sloped (FP64 path;
                                                   needs vertex_coordinates) */
 } anuga_params;
@@ -112,7 +116,9 @@ int anuga_set_params(anuga_domain *d, const anuga_params *params);
 int anuga_set_centroid_values(anuga_domain *d, int quantity, const double *values);
 int anuga_get_centroid_values(anuga_domain *d, int quantity, double *values);
 /* Edge values (N,3) from the last extrapolation; quantity in
- * STAGE, XMOMENTUM, YMOMENTUM, ELEVATION, HEIGHT (Quantity.edge_values). */
+ * STAGE, XMOMENTUM, YMOMENTUM, ELEVATION, HEIGHT (Quantity.edge_values).
+ * After device steps that ran the fused kernel (reorder 2), which keeps edge
+ * values on chip, this extrapolates the current state first. */
 int anuga_get_edge_values(anuga_domain *d, int quantity, double *values);
 
 /* Boundary condition of every boundary edge: kind (M) and, for Dirichlet edges,
@@ -309,6 +315,26 @@ int anuga_get_water_volume(anuga_domain *d, double *volume);
  * params->reorder is set: order[new] = old. Host-only (needs no device).
  * neighbours is (N,3) with boundary edges negative. */
 int anuga_rcm_order(int64_t N, const int64_t *neighbours, int64_t *order);
+
+/* The Hilbert-curve triangle order applied at creation when params->reorder
+ * is 2: triangles sorted by the Hilbert index of their centroid on a 2^16
+ * grid over the bounding box, ties by index; order[new] = old. centroids is
+ * (N,2). Host-only. No counterpart in the reference (its triangles are
+ * processed in the mesh's order); it exists so that runs of 256 consecutive
+ * triangles are compact tiles for the fused extrapolation + flux kernel. */
+int anuga_tile_order(int64_t N, const double *centroids, int64_t *order);
+
+/* The fused kernel's tile metadata, for triangles already in internal order:
+ * nb is (N,3), each interior edge packed as (n << 2) | e (n the neighbour, e
+ * its edge), boundary edges negative. Tile b holds triangles
+ * [256 b, 256 (b + 1)). Writes start (tiles + 1 offsets into halo), halo (the
+ * packed neighbour half-edges that leave each tile, in triangle-then-edge
+ * order; at most halo_cap), slot (N: edge i's halo slot in bits 10i..10i+9,
+ * 1023 when the neighbour is in the tile), *nhalo and *hcap (the largest
+ * halo of a tile; above 320 the library runs the separate kernels instead).
+ * Host-only; no counterpart in the reference. */
+int anuga_tile_halos(int64_t N, const int32_t *nb, int64_t *start, int64_t halo_cap,
+                     int32_t *halo, uint32_t *slot, int64_t *nhalo, int64_t *hcap);
 
 /* Wait for all work on the handle's stream. */
 int anuga_synchronize(anuga_domain *d);

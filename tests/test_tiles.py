@@ -1,0 +1,139 @@
+"""The fused extrapolation + flux kernel's tile layout (CPU): the library's
+Hilbert-curve order and tile metadata, checked by replaying, for every
+interior half-edge, the LDS address the kernel forms (anuga_core_amd/csrc/
+de_kernels.hip, de_xflux) against the triangle and edge that the address
+holds. Calls only host-side entry points of the library."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from anuga_core_amd import _native
+from anuga_core_amd.mesh import Mesh, rectangular_cross
+
+BLOCK = 256          # DE_BLOCK: triangles per tile
+MAX_HALO = 320       # above this the library keeps the separate kernels
+
+
+def _i64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def tile_layout(mesh):
+    """(internal packed neighbours, start, halo, slot, hcap) as the library
+    builds them for this mesh."""
+    lib = _native.lib()
+    N = mesh.number_of_triangles
+    cen = np.ascontiguousarray(mesh.centroid_coordinates, np.float64)
+    order = np.empty(N, np.int64)
+    assert lib.anuga_tile_order(N, cen.ctypes.data_as(C.POINTER(C.c_double)), _i64(order)) == 0
+    assert np.array_equal(np.sort(order), np.arange(N))
+    iperm = np.empty(N, np.int64)
+    iperm[order] = np.arange(N)
+    nbr = mesh.neighbours[order]
+    nbe = mesh.neighbour_edges[order]
+    nb = np.where(nbr >= 0, (iperm[np.maximum(nbr, 0)] << 2) | nbe, -1).astype(np.int32)
+    nb = np.ascontiguousarray(nb)
+    nt = (N + BLOCK - 1) // BLOCK
+    start = np.empty(nt + 1, np.int64)
+    slot = np.empty(N, np.uint32)
+    nhalo, hcap = C.c_int64(), C.c_int64()
+    cap = 3 * N
+    halo = np.empty(cap, np.int32)
+    rc = lib.anuga_tile_halos(N, nb.ctypes.data_as(C.POINTER(C.c_int32)), _i64(start), cap,
+                              halo.ctypes.data_as(C.POINTER(C.c_int32)),
+                              slot.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(nhalo),
+                              C.byref(hcap))
+    assert rc == 0
+    return nb, start, halo[:nhalo.value], slot, hcap.value
+
+
+def replay_lds_addresses(nb, start, halo, slot, hcap):
+    """For every interior half-edge (k, i): the kernel's LDS address of the
+    neighbour's stage value, decoded back to (triangle, edge) through the
+    layout the kernel writes; asserts it is in bounds and is (n, e)."""
+    N = len(nb)
+    stride_h = max((hcap + 31) // 32 * 32, 32)       # the library's rounding
+    lds_len = 12 * BLOCK + 12 * stride_h
+    k = np.repeat(np.arange(N), 3)
+    i = np.tile(np.arange(3), N)
+    v = nb.ravel().astype(np.int64)
+    inter = v >= 0
+    k, i, v = k[inter], i[inter], v[inter]
+    n, e = v >> 2, v & 3
+    b = k // BLOCK
+    k0 = b * BLOCK
+    s = (slot[k].astype(np.int64) >> (10 * i)) & 1023
+    in_tile = s == 1023
+    # kernel: src = lds + e*BLOCK + (n - k0) (tile) or hl + e*hcap + s (halo);
+    # quantity q at src + q*stride with stride 3*BLOCK or 3*hcap
+    addr = np.where(in_tile, e * BLOCK + (n - k0), 12 * BLOCK + e * stride_h + s)
+    stride = np.where(in_tile, 3 * BLOCK, 3 * stride_h)
+    for q in range(4):
+        a = addr + q * stride
+        assert a.min() >= 0 and a.max() < lds_len
+    # decode what the kernel wrote at addr (quantity 0)
+    tile_part = addr < 12 * BLOCK
+    assert np.array_equal(tile_part, in_tile)
+    t_edge = addr[tile_part] // BLOCK
+    t_tri = k0[tile_part] + addr[tile_part] % BLOCK
+    assert np.all(t_tri < N) and np.all(t_tri // BLOCK == b[tile_part])
+    assert np.array_equal(t_tri, n[tile_part]) and np.array_equal(t_edge, e[tile_part])
+    ha = addr[~tile_part] - 12 * BLOCK
+    h_edge = ha // stride_h
+    j = ha % stride_h
+    nh = start[b[~tile_part] + 1] - start[b[~tile_part]]
+    assert np.all(j < nh)
+    entry = halo[start[b[~tile_part]] + j].astype(np.int64)
+    assert np.array_equal(entry >> 2, n[~tile_part]) and np.array_equal(h_edge, e[~tile_part])
+    # the halo holds exactly the half-edges that leave their tile
+    assert len(halo) == int((~in_tile).sum())
+    assert np.all((n[~in_tile] // BLOCK) != b[~in_tile])
+    return int((~in_tile).sum())
+
+
+@pytest.mark.parametrize("m,n", [(200, 200), (37, 53), (1, 1), (8, 8)])
+def test_lds_addresses_of_every_half_edge(m, n):
+    mesh = Mesh(*rectangular_cross(m, n, 1000.0, 700.0))
+    nb, start, halo, slot, hcap = tile_layout(mesh)
+    nhalo = replay_lds_addresses(nb, start, halo, slot, hcap)
+    assert hcap <= MAX_HALO
+    nt = len(start) - 1
+    if m == 200:
+        # compact tiles: about 65 halo edges per 256 triangles on this mesh
+        assert nhalo / nt < 80 and hcap < 160
+    # neighbours are mutual in the internal numbering
+    v = nb.ravel().astype(np.int64)
+    kk = np.repeat(np.arange(len(nb)), 3)[v >= 0]
+    v = v[v >= 0]
+    assert np.array_equal(nb[v >> 2, v & 3] >> 2, kk)
+
+
+def test_tile_order_is_local():
+    """Consecutive triangles in the Hilbert order are close: the mean distance
+    between the centroids of triangles 256 apart stays near a tile's size."""
+    mesh = Mesh(*rectangular_cross(100, 100, 100.0, 100.0))
+    nb, start, halo, slot, hcap = tile_layout(mesh)
+    lib = _native.lib()
+    cen = np.ascontiguousarray(mesh.centroid_coordinates)
+    order = np.empty(len(cen), np.int64)
+    lib.anuga_tile_order(len(cen), cen.ctypes.data_as(C.POINTER(C.c_double)), _i64(order))
+    c = cen[order]
+    step = np.linalg.norm(c[1:] - c[:-1], axis=1)
+    assert np.median(step) < 1.0            # cells are 1 m
+    spread = [np.ptp(c[t:t + BLOCK], axis=0).max() for t in range(0, len(c) - BLOCK, BLOCK)]
+    assert np.median(spread) <= 16.0        # a 256-triangle tile spans about 8 cells
+
+
+def test_tile_halos_rejects_bad_neighbours():
+    lib = _native.lib()
+    nb = np.array([[4 * 5, -1, -1]], np.int32)           # neighbour 5 of a 1-triangle mesh
+    start = np.empty(2, np.int64)
+    halo = np.empty(3, np.int32)
+    slot = np.empty(1, np.uint32)
+    a, b = C.c_int64(), C.c_int64()
+    rc = lib.anuga_tile_halos(1, nb.ctypes.data_as(C.POINTER(C.c_int32)), _i64(start), 3,
+                              halo.ctypes.data_as(C.POINTER(C.c_int32)),
+                              slot.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(a), C.byref(b))
+    assert rc != 0
