@@ -107,11 +107,11 @@ struct DeMesh {
 
 // Compact tiles of DE_BLOCK consecutive triangles (a space-filling-curve
 // order) for the fused extrapolation + flux kernel: per tile, the neighbour
-// half-edges outside it (its halo), and per triangle where each interior
+// triangles outside it (its halo), and per triangle where each interior
 // edge's neighbour value sits in LDS.
 struct DeTiles {
     const int *__restrict__ start;      // [tiles + 1] offsets into halo
-    const int *__restrict__ halo;       // (n << 2) | e of each halo half-edge
+    const int *__restrict__ halo;       // the halo triangles of each tile
     const unsigned *__restrict__ slot;  // [N] 3 x 10 bits: halo slot of edge i, 1023: in the tile
     int hcap;                           // the largest halo (LDS stride)
 };

@@ -861,10 +861,10 @@ static int validate_mesh(anuga_domain *d, const anuga_mesh *mesh, int64_t N, int
 }
 
 // Halo lists of the tiles of one scenario (internal numbering, nb records hn):
-// for tile b (triangles [b DE_BLOCK, (b + 1) DE_BLOCK)), every interior
-// half-edge whose neighbour lies outside the tile, as the neighbour's packed
-// (n << 2) | e; per triangle, each edge's slot in its tile's list (1023: the
-// neighbour is in the tile). No tiles (hcap 0) if a halo exceeds kMaxHalo
+// for tile b (triangles [b DE_BLOCK, (b + 1) DE_BLOCK)), the neighbour
+// triangles outside the tile, each once, in order of first appearance; per
+// triangle, each edge's slot in its tile's list (1023: the neighbour is in
+// the tile). No tiles (hcap 0) if a halo exceeds kMaxHalo
 // (the fused kernel's LDS is 96 B per tile triangle and per halo entry).
 static const int kMaxHalo = 320;
 
@@ -877,8 +877,9 @@ static int tile_halos(int N, const int4 *hn, std::vector<int> &start, std::vecto
     const int nt = (N + DE_BLOCK - 1) / DE_BLOCK;
     start.assign(nt + 1, 0);
     halo.clear();
-    halo.reserve((size_t)nt * 96);
+    halo.reserve((size_t)nt * 64);
     slot.assign(N, 0);
+    std::vector<int> seen_tile(N, -1), seen_slot(N, 0);
     int hcap = 0;
     for (int b = 0; b < nt; b++) {
         start[b] = (int)halo.size();
@@ -892,8 +893,12 @@ static int tile_halos(int N, const int4 *hn, std::vector<int> &start, std::vecto
                     const int n = v[i] >> 2;
                     if (n >= N || (v[i] & 3) == 3) return -1;
                     if (n < k0 || n >= k1) {
-                        si = (unsigned)std::min<size_t>(halo.size() - start[b], 1023);
-                        halo.push_back(v[i]);
+                        if (seen_tile[n] != b) {
+                            seen_tile[n] = b;
+                            seen_slot[n] = (int)(halo.size() - start[b]);
+                            halo.push_back(n);
+                        }
+                        si = (unsigned)std::min(seen_slot[n], 1023);
                     }
                 }
                 sl |= si << (10 * i);

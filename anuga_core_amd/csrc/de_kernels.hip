@@ -726,8 +726,7 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
 
 // Tiled fused extrapolation + flux (FP64, the common kernel's options,
 // SUBSTEP 0 or 1). A workgroup extrapolates its tile's DE_BLOCK triangles and
-// the triangles across its halo half-edges (the neighbour sides of edges that
-// leave the tile) into LDS, then each thread gathers its triangle's three edge fluxes
+// its halo (the neighbour triangles outside the tile, each once) into LDS, then each thread gathers its triangle's three edge fluxes
 // from LDS. The edge values never reach HBM. The arithmetic is
 // de_extrap_cell and de_edge_flux, as in de_extrapolate and de_flux, and the
 // per-block partials (CFL, boundary inflow, protect mass) are formed in the
@@ -779,7 +778,7 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
     const int h0 = tl.start[lb], nh = tl.start[lb + 1] - h0;
     for (int j = t; j < nh; j += DE_BLOCK) {
         DeExtrap<double> x;
-        de_extrap_cell<double>(m, p, q, tl.halo[h0 + j] >> 2, err, x);
+        de_extrap_cell<double>(m, p, q, tl.halo[h0 + j], err, x);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             double ew, eh, euh, evh;

@@ -328,10 +328,11 @@ int anuga_tile_order(int64_t N, const double *centroids, int64_t *order);
  * nb is (N,3), each interior edge packed as (n << 2) | e (n the neighbour, e
  * its edge), boundary edges negative. Tile b holds triangles
  * [256 b, 256 (b + 1)). Writes start (tiles + 1 offsets into halo), halo (the
- * packed neighbour half-edges that leave each tile, in triangle-then-edge
- * order; at most halo_cap), slot (N: edge i's halo slot in bits 10i..10i+9,
- * 1023 when the neighbour is in the tile), *nhalo and *hcap (the largest
- * halo of a tile; above 320 the library runs the separate kernels instead).
+ * neighbour triangles outside each tile, each once, in order of first
+ * appearance over the tile's triangles and edges; at most halo_cap), slot (N:
+ * edge i's halo slot in bits 10i..10i+9, 1023 when the neighbour is in the
+ * tile), *nhalo and *hcap (the largest halo of a tile; above 320 the library
+ * runs the separate kernels instead).
  * Host-only; no counterpart in the reference. */
 int anuga_tile_halos(int64_t N, const int32_t *nb, int64_t *start, int64_t halo_cap,
                      int32_t *halo, uint32_t *slot, int64_t *nhalo, int64_t *hcap);
