@@ -477,13 +477,12 @@ static void launch_flux_p(anuga_domain *d, int substep, DeQ *S, DeQ *bk, DeQ *ou
     }
 }
 
-// The fused extrapolation + flux kernel applies to one FP64 scenario with
-// tiles and the common kernel's options, and to substeps 0 and 1 (rk3's
-// middle substep writes the state that other tiles are extrapolating).
+// The fused extrapolation + flux kernels apply to one scenario with tiles and
+// the common kernel's options (both precisions), and to substeps 0 and 1
+// (rk3's middle substep writes the state that other tiles are extrapolating).
 static bool fused(const anuga_domain *d)
 {
-    return d->hcap > 0 && !d->mixed && d->S == 1 && !d->rw && !d->P.low_froude &&
-           !d->P.use_sloped_mannings;
+    return d->hcap > 0 && d->S == 1 && !d->rw && !d->P.low_froude && !d->P.use_sloped_mannings;
 }
 
 // extrapolation + flux of substep 0 (explicit updates) or 1 (advance S and
@@ -495,6 +494,23 @@ static void launch_xflux(anuga_domain *d, int substep, DeQ *S, DeQ *bk = nullptr
     timer_begin(d, T);
     DeBoundary bc{d->bkind, d->bval, d->M, d->bsstride};
     DeTiles tl{d->tstart, d->thalo, d->tslot, d->hcap};
+    if (d->mixed) {
+        // FP64 centroid depth + 12 FP32 record components per triangle
+        const size_t lds = ((size_t)DE_BLOCK + (size_t)d->hcap) * (sizeof(double) + 12 * sizeof(float));
+        if (substep == 0)
+            hipLaunchKernelGGL(de_xflux_mixed<0>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
+                               dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags,
+                               d->ex[0], d->ex[1], d->ex[2], bk, out, d->ctl, d->part,
+                               d->part + d->nblocks, d->part + 2 * (size_t)d->nblocks, d->dneg);
+        else
+            hipLaunchKernelGGL(de_xflux_mixed<1>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
+                               dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags,
+                               d->ex[0], d->ex[1], d->ex[2], bk, out, d->ctl, d->part,
+                               d->part + d->nblocks, d->part + 2 * (size_t)d->nblocks, d->dneg);
+        timer_end(d, T);
+        d->e_stale = true;
+        return;
+    }
     const size_t lds = (12 * (size_t)DE_BLOCK + 12 * (size_t)d->hcap) * sizeof(double);
     if (substep == 0)
         hipLaunchKernelGGL(de_xflux<0>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
@@ -1064,7 +1080,7 @@ static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
         HIPCHK(d, hipMemcpy(d->vx + 6 * (size_t)kb, hv.data(), 6 * (size_t)N * sizeof(double),
                             hipMemcpyHostToDevice));
     }
-    if (d->P.reorder == 2 && d->S == 1 && !d->mixed) {
+    if (d->P.reorder == 2 && d->S == 1) {
         int rc = build_tiles(d, hn);
         if (rc) return rc;
     }

@@ -274,6 +274,22 @@ __device__ __forceinline__ void de_edge64(const DeExtrap<double> &x, int i, doub
     evh = ev * eh;
 }
 
+// Mixed-path edge record of edge i (compile-time i): FP32 stage and depth
+// jumps and edge momenta, referred to the FP64 centroid depth x.c.h
+__device__ __forceinline__ float4 de_edge32(const DeExtrap<float> &x, int i)
+{
+    const float jw = i == 0 ? x.jw[0] : (i == 1 ? x.jw[1] : x.jw[2]);
+    const float jh = i == 0 ? x.jh[0] : (i == 1 ? x.jh[1] : x.jh[2]);
+    const float ju = i == 0 ? x.ju[0] : (i == 1 ? x.ju[1] : x.ju[2]);
+    const float jv = i == 0 ? x.jv[0] : (i == 1 ? x.jv[1] : x.jv[2]);
+    const float hw = x.constant ? 0.f : jw;
+    const float hh = x.constant ? 0.f : jh;
+    const float eh = float(x.c.h + double(hh));
+    const float eu = float(x.c.u + (x.constant ? 0.0 : double(ju)));
+    const float ev = float(x.c.v + (x.constant ? 0.0 : double(jv)));
+    return make_float4(hw, hh, eu * eh, ev * eh);
+}
+
 template <class T>
 __global__ void __launch_bounds__(DE_BLOCK)
 de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict__ E,
@@ -313,14 +329,7 @@ de_extrapolate(DeMesh m, DeParams p, const DeQ *__restrict__ q, void *__restrict
         } else {
             DeEdgeRec32 rec;
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const float hw = x.constant ? 0.f : float(x.jw[i]);
-                const float hh = x.constant ? 0.f : float(x.jh[i]);
-                const float eh = float(x.c.h + double(hh));
-                const float eu = float(x.c.u + (x.constant ? 0.0 : double(x.ju[i])));
-                const float ev = float(x.c.v + (x.constant ? 0.0 : double(x.jv[i])));
-                rec.e[i] = make_float4(hw, hh, eu * eh, ev * eh);
-            }
+            for (int i = 0; i < 3; i++) rec.e[i] = de_edge32(x, i);
             rec.hc = x.c.h;
             rec.pad = 0.0;
             reinterpret_cast<DeEdgeRec32 *>(E)[k] = rec;
@@ -661,7 +670,6 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
     double dtmin = 1.0e100, bflux = 0.0;
 
     if (k < cb->kend) {
-        const T g = T(p.g), eps = T(p.eps);
         const DeCentroid c = de_centroid(q[k], flags[k]);
         const int4 vk = m.nb[k];
         const int v[3] = {vk.x, vk.y, vk.z};
@@ -861,6 +869,93 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
     }
 }
 
+// One edge's terms of triangle k's sums on the mixed path (the body of the
+// reference's edge loop, sw_domain_orig.c:828-1004, over FP32 differences):
+// own edge record el (dw, dh, uh, vh) and centroid depth hcd; for an interior
+// edge the neighbour's record er at the shared edge and its centroid depth
+// hcn. Shared by de_flux_mixed and the tiled fused kernel.
+template <int SUBSTEP>
+__device__ __forceinline__ void de_edge_flux_mixed(const DeParams &p, const DeBoundary &bc,
+                                                   const DeGeoF32 &gf, const DeCentroid &c,
+                                                   double hcd, float hc, int i, int vi,
+                                                   float4 el, float4 er, double hcn, double &sw,
+                                                   double &sx, double &sy, double &bflux,
+                                                   double &dtmin)
+{
+    const float g = float(p.g), eps = float(p.eps);
+    const float hle = float(hcd + double(el.y));
+    const float zl = el.x - el.y;
+    const float ql1 = el.z, ql2 = el.w;
+    const float n1 = gf.nx[i], n2 = gf.ny[i];
+    float hre, zr, qr1, qr2, dH = 0.f;
+    if (vi >= 0) {
+        hre = float(hcn + double(er.y));
+        zr = gf.dzc[i] + (er.x - er.y);
+        qr1 = er.z;
+        qr2 = er.w;
+        dH = (float(hcn - hcd) + (er.y - el.y)) + (zr - zl);
+    } else {
+        const int j = -vi - 1;
+        const int kind = bc.kind[j];
+        const double *bv = bc.val + (size_t)de_bc_slot<SUBSTEP>() * bc.sstride;
+        float wr;                                   // stage above z_c
+        if (kind == 0) {          // reflective
+            const float r1 = -ql1 * n1 - ql2 * n2;
+            const float r2 = -ql1 * n2 + ql2 * n1;
+            wr = zl + hle;
+            qr1 = n1 * r1 - n2 * r2;
+            qr2 = n2 * r1 + n1 * r2;
+        } else if (kind == 1) {   // transmissive
+            wr = zl + hle;
+            qr1 = ql1;
+            qr2 = ql2;
+        } else if (kind == 2) {   // Dirichlet
+            wr = float(bv[j] - c.z);
+            qr1 = float(bv[bc.M + j]);
+            qr2 = float(bv[2 * bc.M + j]);
+        } else if (kind == 3) {   // transmissive from centroid values
+            wr = float(c.w - c.z);
+            qr1 = float(c.uh);
+            qr2 = float(c.vh);
+        } else if (kind == 4) {   // stage set, normal momentum kept, tangential 0
+            const float nq = n1 * ql1 + n2 * ql2;
+            wr = float(bv[j] - c.z);
+            qr1 = nq * n1;
+            qr2 = nq * n2;
+        } else if (kind == 5) {   // transmissive stage, zero momentum
+            wr = zl + hle;
+            qr1 = 0.f;
+            qr2 = 0.f;
+        } else {                  // stage set, momentum transmissive
+            wr = float(bv[j] - c.z);
+            qr1 = ql1;
+            qr2 = ql2;
+        }
+        zr = zl;
+        hre = (kind <= 1 || kind == 5) ? fmax(hle, 0.f) : fmax(wr - zr, 0.f);
+    }
+    const float z_half = fmax(zl, zr);
+    const float hl_raw = hle + (zl - z_half);
+    const float hr_raw = hre + (zr - z_half);
+    const float h_left = fmax(hl_raw, 0.f);
+    const float h_right = fmax(hr_raw, 0.f);
+    const float diff0 = (vi >= 0 && hl_raw > 0.f && hr_raw > 0.f) ? dH : h_right - h_left;
+    float ef[3], speed, pflux;
+    de_flux_central_d<float>(ql1, ql2, qr1, qr2, h_left, h_right, hle, hre, n1, n2, eps,
+                             diff0, g, ef, speed, pflux);
+    const float length = gf.len[i];
+    const float dsq = (hl_raw > 0.f) ? (zl - z_half) * (h_left + hle) : -(hle * hle);
+    const float pg = length * (-g * 0.5f * (dsq - (hle + hc) * zl) + pflux);
+    sw -= double(ef[0] * length);
+    sx -= double(ef[1] * length);
+    sy -= double(ef[2] * length);
+    if (vi < 0) bflux -= double(ef[0] * length);
+    sx -= double(n1 * pg);
+    sy -= double(n2 * pg);
+    if (SUBSTEP == 0 && double(speed) > p.eps)
+        dtmin = fmin(dtmin, double(gf.radius) * (1.0 / double(speed)));
+}
+
 // Mixed-precision flux. All FP32 arithmetic is on quantities referred to the
 // own centroid bed z_c (a local datum): edge bed zl = dw - dh, neighbour edge
 // bed zr = (z_n - z_c) + dw' - dh', depths h = h_c + dh formed in FP64 and
@@ -890,7 +985,6 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
     double dtmin = 1.0e100, bflux = 0.0;
 
     if (k < cb->kend) {
-        const float g = float(p.g), eps = float(p.eps);
         const DeCentroid c = de_centroid(q[k], flags[k]);
         const int4 vk = m.nb[k];
         const int v[3] = {vk.x, vk.y, vk.z};
@@ -901,81 +995,15 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
         double sw = 0.0, sx = 0.0, sy = 0.0;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            const float4 el = own.e[i];                   // dw, dh, uh, vh
-            const float hle = float(hcd + double(el.y));
-            const float zl = el.x - el.y;
-            const float ql1 = el.z, ql2 = el.w;
-            const float n1 = gf.nx[i], n2 = gf.ny[i];
-            float hre, zr, qr1, qr2, dH = 0.f;
+            float4 er = make_float4(0.f, 0.f, 0.f, 0.f);
+            double hcn = 0.0;
             if (v[i] >= 0) {
                 const DeEdgeRec32 *rn = E + (v[i] >> 2);
-                const float4 er = rn->e[v[i] & 3];
-                const double hcn = rn->hc;
-                hre = float(hcn + double(er.y));
-                zr = gf.dzc[i] + (er.x - er.y);
-                qr1 = er.z;
-                qr2 = er.w;
-                dH = (float(hcn - hcd) + (er.y - el.y)) + (zr - zl);
-            } else {
-                const int j = -v[i] - 1;
-                const int kind = bc.kind[j];
-                const double *bv = bc.val + (size_t)de_bc_slot<SUBSTEP>() * bc.sstride;
-                float wr;                                   // stage above z_c
-                if (kind == 0) {          // reflective
-                    const float r1 = -ql1 * n1 - ql2 * n2;
-                    const float r2 = -ql1 * n2 + ql2 * n1;
-                    wr = zl + hle;
-                    qr1 = n1 * r1 - n2 * r2;
-                    qr2 = n2 * r1 + n1 * r2;
-                } else if (kind == 1) {   // transmissive
-                    wr = zl + hle;
-                    qr1 = ql1;
-                    qr2 = ql2;
-                } else if (kind == 2) {   // Dirichlet
-                    wr = float(bv[j] - c.z);
-                    qr1 = float(bv[bc.M + j]);
-                    qr2 = float(bv[2 * bc.M + j]);
-                } else if (kind == 3) {   // transmissive from centroid values
-                    wr = float(c.w - c.z);
-                    qr1 = float(c.uh);
-                    qr2 = float(c.vh);
-                } else if (kind == 4) {   // stage set, normal momentum kept, tangential 0
-                    const float nq = n1 * ql1 + n2 * ql2;
-                    wr = float(bv[j] - c.z);
-                    qr1 = nq * n1;
-                    qr2 = nq * n2;
-                } else if (kind == 5) {   // transmissive stage, zero momentum
-                    wr = zl + hle;
-                    qr1 = 0.f;
-                    qr2 = 0.f;
-                } else {                  // stage set, momentum transmissive
-                    wr = float(bv[j] - c.z);
-                    qr1 = ql1;
-                    qr2 = ql2;
-                }
-                zr = zl;
-                hre = (kind <= 1 || kind == 5) ? fmax(hle, 0.f) : fmax(wr - zr, 0.f);
+                er = rn->e[v[i] & 3];
+                hcn = rn->hc;
             }
-            const float z_half = fmax(zl, zr);
-            const float hl_raw = hle + (zl - z_half);
-            const float hr_raw = hre + (zr - z_half);
-            const float h_left = fmax(hl_raw, 0.f);
-            const float h_right = fmax(hr_raw, 0.f);
-            const float diff0 = (v[i] >= 0 && hl_raw > 0.f && hr_raw > 0.f) ? dH : h_right - h_left;
-            float ef[3], speed, pflux;
-            de_flux_central_d<float>(ql1, ql2, qr1, qr2, h_left, h_right, hle, hre, n1, n2, eps,
-                                     diff0, g, ef, speed, pflux);
-            const float length = gf.len[i];
-            const float dsq = (hl_raw > 0.f) ? (zl - z_half) * (h_left + hle) : -(hle * hle);
-            const float pg = length * (-g * 0.5f * (dsq - (hle + hc) * zl) + pflux);
-            sw -= double(ef[0] * length);
-            sx -= double(ef[1] * length);
-            sy -= double(ef[2] * length);
-            if (v[i] < 0) bflux -= double(ef[0] * length);
-            sx -= double(n1 * pg);
-            sy -= double(n2 * pg);
-            if (SUBSTEP == 0 && double(speed) > p.eps)
-                dtmin = fmin(dtmin, double(gf.radius) * (1.0 / double(speed)));
+            de_edge_flux_mixed<SUBSTEP>(p, bc, gf, c, hcd, hc, i, v[i], own.e[i], er, hcn, sw, sx,
+                                        sy, bflux, dtmin);
         }
         const double inv_area = 1.0 / m.area[k];
         sw *= inv_area;
@@ -1007,6 +1035,141 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
         }
         part_dt[lb] = a2;
         part_bf[lb] = b2;
+    }
+}
+
+// Tiled fused extrapolation + flux on the mixed path (SUBSTEP 0 or 1): as
+// de_xflux, with the FP32 edge records of de_extrapolate<float> and the FP64
+// centroid depths in LDS, and the edge terms of de_flux_mixed, so the
+// results are the bits of the separate mixed kernels.
+template <int SUBSTEP>
+__global__ void __launch_bounds__(DE_BLOCK)
+de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restrict__ eta,
+               DeTiles tl, DeBoundary bc, unsigned char *__restrict__ flags,
+               double *__restrict__ exw, double *__restrict__ exuh, double *__restrict__ exvh,
+               const DeQ *bk, DeQ *out, DeControl *ctl, double *__restrict__ part_dt,
+               double *__restrict__ part_bf, double *__restrict__ part_mass,
+               unsigned long long *__restrict__ nneg)
+{
+    // centroid depths [DE_BLOCK] tile then [hcap] halo (double); then the
+    // records' components [4][3][DE_BLOCK] tile and [4][3][hcap] halo (float)
+    extern __shared__ double lds[];
+    __shared__ double red_dt[DE_BLOCK / 64], red_bf[DE_BLOCK / 64], red_m[DE_BLOCK / 64];
+    const int lb = xcd_block();
+    DeControl *cb = de_ctl(ctl, m.scen, lb);
+    if (cb->done) return;
+    int *err = &cb->status;
+    const double dt = SUBSTEP ? cb->dt : 0.0;
+    const int t = threadIdx.x;
+    const int k0 = lb * DE_BLOCK, k = k0 + t;
+    const int kend = cb->kend;
+    const int hcap = tl.hcap;
+    double *const hct = lds, *const hch = lds + DE_BLOCK;
+    float *const ft = reinterpret_cast<float *>(lds + DE_BLOCK + hcap);
+    float *const fh = ft + 12 * DE_BLOCK;
+    double mass = 0.0, dtmin = 1.0e100, bflux = 0.0;
+    unsigned fk = 0;
+
+    if (k < kend) {
+        DeExtrap<float> x;
+        de_extrap_cell<float>(m, p, q, k, err, x);
+        mass = x.mass;
+        fk = x.flag;
+        flags[k] = (unsigned char)fk;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float4 r = de_edge32(x, i);
+            ft[(0 + i) * DE_BLOCK + t] = r.x;
+            ft[(3 + i) * DE_BLOCK + t] = r.y;
+            ft[(6 + i) * DE_BLOCK + t] = r.z;
+            ft[(9 + i) * DE_BLOCK + t] = r.w;
+        }
+        hct[t] = x.c.h;
+    }
+    const int h0 = tl.start[lb], nh = tl.start[lb + 1] - h0;
+    for (int j = t; j < nh; j += DE_BLOCK) {
+        DeExtrap<float> x;
+        de_extrap_cell<float>(m, p, q, tl.halo[h0 + j], err, x);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float4 r = de_edge32(x, i);
+            fh[(0 + i) * hcap + j] = r.x;
+            fh[(3 + i) * hcap + j] = r.y;
+            fh[(6 + i) * hcap + j] = r.z;
+            fh[(9 + i) * hcap + j] = r.w;
+        }
+        hch[j] = x.c.h;
+    }
+    __syncthreads();
+
+    if (k < kend) {
+        const DeCentroid c = de_centroid(q[k], fk);
+        const int4 vk = m.nb[k];
+        const int v[3] = {vk.x, vk.y, vk.z};
+        const unsigned sl = tl.slot[k];
+        const DeGeoF32 gf = static_cast<const DeGeoF32 *>(m.gf)[k];
+        const double hcd = hct[t];
+        const float hc = float(hcd);
+        double sw = 0.0, sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float4 el = make_float4(ft[(0 + i) * DE_BLOCK + t], ft[(3 + i) * DE_BLOCK + t],
+                                          ft[(6 + i) * DE_BLOCK + t], ft[(9 + i) * DE_BLOCK + t]);
+            float4 er = make_float4(0.f, 0.f, 0.f, 0.f);
+            double hcn = 0.0;
+            if (v[i] >= 0) {
+                const unsigned s = (sl >> (10 * i)) & 1023u;
+                const float *src;
+                int stride;
+                if (s == 1023u) {
+                    const int tn = (v[i] >> 2) - k0;
+                    src = ft + (v[i] & 3) * DE_BLOCK + tn;
+                    stride = 3 * DE_BLOCK;
+                    hcn = hct[tn];
+                } else {
+                    src = fh + (v[i] & 3) * hcap + s;
+                    stride = 3 * hcap;
+                    hcn = hch[s];
+                }
+                er = make_float4(src[0], src[stride], src[2 * stride], src[3 * stride]);
+            }
+            de_edge_flux_mixed<SUBSTEP>(p, bc, gf, c, hcd, hc, i, v[i], el, er, hcn, sw, sx, sy,
+                                        bflux, dtmin);
+        }
+        const double inv_area = 1.0 / m.area[k];
+        sw *= inv_area;
+        sx *= inv_area;
+        sy *= inv_area;
+        if (SUBSTEP == 0) {
+            exw[k] = sw;
+            exuh[k] = sx;
+            exvh[k] = sy;
+        } else {
+            const DeQ r = de_advance<false>(c, eta[k], sw, sx, sy, dt, p, err, nneg);
+            out[k] = de_rk_combine<SUBSTEP>(r, bk[k]);
+        }
+    }
+
+    dtmin = de_wave_min(dtmin);
+    bflux = de_wave_sum(bflux);
+    mass = de_wave_sum(mass);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red_dt[wv] = dtmin;
+        red_bf[wv] = bflux;
+        red_m[wv] = mass;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = red_dt[0], b = red_bf[0], cm = red_m[0];
+        for (int i = 1; i < DE_BLOCK / 64; i++) {
+            a = fmin(a, red_dt[i]);
+            b += red_bf[i];
+            cm += red_m[i];
+        }
+        part_dt[lb] = a;
+        part_bf[lb] = b;
+        part_mass[lb] = cm;
     }
 }
 

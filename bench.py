@@ -72,6 +72,13 @@ KERNEL_BYTES = {
         #   geometry without radius (48) + area (8) + n (8) + step-initial
         #   w,uh,vh (24) -> w,uh,vh (24)
         "flux1": ("de_flux_mixed<1>", 213),
+        # fused over tiles: w,uh,vh,z (32) + neighbours (12) + FP32 edge and
+        #   neighbour offsets (48) + FP32 normals, lengths, radius, neighbour bed
+        #   offsets (52) + area (8) -> flag (1) + explicit updates (24)
+        "xflux0": ("de_xflux_mixed<0>", 177),
+        # extrapolation inputs (92) + FP32 geometry without radius (48) + area (8)
+        #   + n (8) + step-initial w,uh,vh (24) -> flag (1) + w,uh,vh (24)
+        "xflux1": ("de_xflux_mixed<1>", 205),
     },
 }
 LAUNCHES_PER_STEP = {"extrapolate": 2, "flux0": 1, "update": 1, "flux1": 1, "xflux0": 1,
