@@ -76,6 +76,11 @@ struct anuga_domain {
     int *tstart = nullptr, *thalo = nullptr;
     unsigned *tslot = nullptr;
     int hcap = 0;
+    // per scenario, until every mesh is set: tile offsets, halo (internal
+    // ids), slots and largest halo (-1: no tiles)
+    std::vector<std::vector<int>> ts_start, ts_halo;
+    std::vector<std::vector<unsigned>> ts_slot;
+    std::vector<int> ts_hcap;
     bool e_stale = false;                // E does not hold the last extrapolation
     int *rw = nullptr;                   // [3][N] riverwall record per half-edge, -1: none
     DeRiverwall *rwd = nullptr;          // riverwall records; rw == nullptr: no riverwalls
@@ -477,12 +482,12 @@ static void launch_flux_p(anuga_domain *d, int substep, DeQ *S, DeQ *bk, DeQ *ou
     }
 }
 
-// The fused extrapolation + flux kernels apply to one scenario with tiles and
-// the common kernel's options (both precisions), and to substeps 0 and 1
+// The fused extrapolation + flux kernels apply with tiles and the common
+// kernel's options (both precisions, any number of scenarios), to substeps 0 and 1
 // (rk3's middle substep writes the state that other tiles are extrapolating).
 static bool fused(const anuga_domain *d)
 {
-    return d->hcap > 0 && d->S == 1 && !d->rw && !d->P.low_froude && !d->P.use_sloped_mannings;
+    return d->hcap > 0 && !d->rw && !d->P.low_froude && !d->P.use_sloped_mannings;
 }
 
 // extrapolation + flux of substep 0 (explicit updates) or 1 (advance S and
@@ -927,7 +932,34 @@ static int tile_halos(int N, const int4 *hn, std::vector<int> &start, std::vecto
     return hcap;
 }
 
-static int build_tiles(anuga_domain *d, const std::vector<int4> &hn)
+// The tile metadata of scenario s (its nb records hn, internal ids), kept on
+// the host until every scenario's mesh is set.
+static int scenario_tiles(anuga_domain *d, int s, const std::vector<int4> &hn)
+{
+    const int N = (int)hn.size(), kb = d->kbeg[s];
+    std::vector<int4> loc(hn);
+    for (int4 &r : loc) {
+        int *v[3] = {&r.x, &r.y, &r.z};
+        for (int i = 0; i < 3; i++)
+            if (*v[i] >= 0) *v[i] -= kb << 2;
+    }
+    if (d->ts_hcap.empty()) {
+        d->ts_start.resize(d->S);
+        d->ts_halo.resize(d->S);
+        d->ts_slot.resize(d->S);
+        d->ts_hcap.assign(d->S, -1);
+    }
+    const int hcap = tile_halos(N, loc.data(), d->ts_start[s], d->ts_halo[s], d->ts_slot[s]);
+    if (hcap < 0) return fail(d, ANUGA_E_ARG, "tiles: neighbour out of range");
+    for (int &n : d->ts_halo[s]) n += kb;
+    d->ts_hcap[s] = hcap;
+    return ANUGA_OK;
+}
+
+// Assemble every scenario's tile metadata into the device arrays the fused
+// kernels read (block b's halo at tstart[b]); no tiles (hcap 0) if a scenario
+// has none or a halo exceeds kMaxHalo.
+static int upload_tiles(anuga_domain *d)
 {
     d->hcap = 0;
     for (int **p : {&d->tstart, &d->thalo}) {
@@ -936,20 +968,33 @@ static int build_tiles(anuga_domain *d, const std::vector<int4> &hn)
     }
     if (d->tslot) HIPCHK(d, hipFree(d->tslot));
     d->tslot = nullptr;
-    const int N = (int)hn.size(), nt = (N + DE_BLOCK - 1) / DE_BLOCK;
-    std::vector<int> start, halo;
-    std::vector<unsigned> slot;
-    const int hcap = tile_halos(N, hn.data(), start, halo, slot);
-    if (hcap < 0) return fail(d, ANUGA_E_ARG, "tiles: neighbour out of range");
+    if ((int)d->ts_hcap.size() != d->S) return ANUGA_OK;
+    int hcap = 0;
+    for (int s = 0; s < d->S; s++) {
+        if (d->ts_hcap[s] < 0) return ANUGA_OK;
+        hcap = std::max(hcap, d->ts_hcap[s]);
+    }
     if (hcap > kMaxHalo) return ANUGA_OK;
-    HIPCHK(d, dalloc(&d->tstart, (size_t)nt + 1));
+    std::vector<int> start(d->nblocks + 1, 0), halo;
+    std::vector<unsigned> slot(d->N, 0x3fffffffu);
+    int b = 0;
+    for (int s = 0; s < d->S; s++) {
+        const int b0 = d->kbeg[s] / DE_BLOCK, nt = (int)d->ts_start[s].size() - 1;
+        for (; b < b0; b++) start[b] = (int)halo.size();
+        for (int t = 0; t < nt; t++, b++) start[b] = (int)halo.size() + d->ts_start[s][t];
+        halo.insert(halo.end(), d->ts_halo[s].begin(), d->ts_halo[s].end());
+        std::copy(d->ts_slot[s].begin(), d->ts_slot[s].end(), slot.begin() + d->kbeg[s]);
+    }
+    for (; b <= d->nblocks; b++) start[b] = (int)halo.size();
+    HIPCHK(d, dalloc(&d->tstart, (size_t)d->nblocks + 1));
     HIPCHK(d, dalloc(&d->thalo, halo.size()));
-    HIPCHK(d, dalloc(&d->tslot, (size_t)N));
-    HIPCHK(d, hipMemcpy(d->tstart, start.data(), (nt + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(d, dalloc(&d->tslot, (size_t)d->N));
+    HIPCHK(d, hipMemcpy(d->tstart, start.data(), (d->nblocks + 1) * sizeof(int),
+                        hipMemcpyHostToDevice));
     if (!halo.empty())
         HIPCHK(d, hipMemcpy(d->thalo, halo.data(), halo.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(d, hipMemcpy(d->tslot, slot.data(), N * sizeof(unsigned), hipMemcpyHostToDevice));
-    // the LDS stride: a multiple of 32 doubles, never zero
+    HIPCHK(d, hipMemcpy(d->tslot, slot.data(), d->N * sizeof(unsigned), hipMemcpyHostToDevice));
+    // the LDS stride: a multiple of 32, never zero
     d->hcap = std::max((hcap + 31) / 32 * 32, 32);
     return ANUGA_OK;
 }
@@ -962,7 +1007,7 @@ static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
     const int64_t *nbr = mesh->neighbours, *nbe = mesh->neighbour_edges;
     const int64_t *sur = mesh->surrogate_neighbours, *nob = mesh->number_of_boundaries;
     std::vector<int> perm(N), iperm(N);
-    if (d->P.reorder == 2 && d->S == 1) perm = tile_order(N, mesh->centroid_coordinates);
+    if (d->P.reorder == 2) perm = tile_order(N, mesh->centroid_coordinates);
     else if (d->P.reorder) perm = rcm_order(N, nbr);
     else
         for (int k = 0; k < N; k++) perm[k] = k;
@@ -1080,14 +1125,15 @@ static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
         HIPCHK(d, hipMemcpy(d->vx + 6 * (size_t)kb, hv.data(), 6 * (size_t)N * sizeof(double),
                             hipMemcpyHostToDevice));
     }
-    if (d->P.reorder == 2 && d->S == 1) {
-        int rc = build_tiles(d, hn);
+    if (d->P.reorder == 2) {
+        int rc = scenario_tiles(d, s, hn);
         if (rc) return rc;
     }
     if (!d->mesh_set[s]) {
         d->mesh_set[s] = 1;
         d->meshes_missing--;
     }
+    if (d->P.reorder == 2 && d->meshes_missing == 0) return upload_tiles(d);
     return ANUGA_OK;
 }
 

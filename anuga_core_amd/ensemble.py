@@ -72,7 +72,8 @@ class Ensemble:
         prec = {"fp64": _native.PRECISION_FP64, "mixed": _native.PRECISION_MIXED}[precision]
         self._params = _native.AnugaParams(G, EPSILON, self.minimum_allowed_height,
                                            beta, bdry, beta, bdry, beta, bdry, 0, 1,
-                                           self._fluxcalls, prec, int(bool(reorder)))
+                                           self._fluxcalls, prec,
+                                           {True: 2, "tiles": 2, "rcm": 1, False: 0, None: 0}[reorder])
         self.evolve_max_timestep = MAX_TIMESTEP
         self.evolve_min_timestep = MIN_TIMESTEP
         self.max_smallsteps = MAX_SMALLSTEPS
