@@ -1420,6 +1420,12 @@ static AdjView adj_view(const anuga_domain *d, double dt)
 // production kernels; d->B, d->E, d->flags and d->ex serve as scratch)
 static void step_fixed(anuga_domain *d, DeQ *X)
 {
+    if (fused(d)) {
+        launch_xflux(d, 0, X);
+        launch_update(d, X, d->B);
+        launch_xflux(d, 1, d->B, X, X);
+        return;
+    }
     launch_extrapolate(d, X);
     launch_flux(d, 0, X);
     launch_update(d, X, d->B);
@@ -2144,8 +2150,12 @@ int anuga_compute_fluxes(anuga_domain *d, double *flux_timestep)
     rc = ready_to_step(d);
     if (rc) return rc;
     ensure_live(d);
-    launch_extrapolate(d, d->A);
-    launch_flux(d, 0, d->A);
+    if (fused(d)) {
+        launch_xflux(d, 0, d->A);
+    } else {
+        launch_extrapolate(d, d->A);
+        launch_flux(d, 0, d->A);
+    }
     launch_reduce(d, 0);
     rc = launch_check(d);
     if (rc) return rc;

@@ -498,7 +498,8 @@ def run_adjoint(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 1000 single, 500 ensemble, 2000 adjoint)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--m", type=int, default=2000)
@@ -530,6 +531,8 @@ def main():
                     help="first run 3 device copies of 1 GiB (a known-traffic kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"single": 1000, "ensemble": 500, "adjoint": 2000}[args.workload]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the step shards by spatial domain decomposition, not implemented here)",

@@ -64,7 +64,7 @@ def test_flat_bed_golden():
     _compare(o, d, "flat_bed")
 
 
-@pytest.mark.parametrize("reorder", [True, False])
+@pytest.mark.parametrize("reorder", [True, "rcm", False])
 def test_de0_dam_break_vs_oracle(reorder):
     print("case de0_dam_break reorder=%s" % reorder, flush=True)
     o, d = _pair(S.dam_break_de0, reorder=reorder, m=40, n=40)
@@ -77,9 +77,11 @@ def test_de0_dam_break_vs_oracle(reorder):
     assert abs(d.get_water_volume() - o.get_water_volume()) <= TOL * o.get_water_volume()
 
 
-def test_de1_hills_vs_oracle():
-    print("case de1_hills", flush=True)
-    o, d = _pair(S.hills_de1, m=30, n=30)
+@pytest.mark.parametrize("reorder", [True, "rcm"])
+def test_de1_hills_vs_oracle(reorder):
+    """Hilbert tiles (fused extrapolation + flux) and RCM (separate kernels)."""
+    print("case de1_hills reorder=%s" % reorder, flush=True)
+    o, d = _pair(S.hills_de1, reorder=reorder, m=30, n=30)
     for _ in d.evolve(yieldstep=5.0, finaltime=40.0):
         pass
     for _ in o.evolve(yieldstep=5.0, finaltime=40.0):
@@ -104,6 +106,27 @@ def test_edge_values_after_distribute():
         assert np.array_equal(a, b), name
     for name, oname in (("stage", "stage_c"), ("xmomentum", "xmom_c"), ("ymomentum", "ymom_c")):
         assert np.array_equal(d.get_quantity(name).centroid_values, getattr(o, oname)), name
+
+
+def test_edge_values_after_fused_steps():
+    """The fused kernels keep edge values on chip: asked for after device
+    steps, the library extrapolates the current state, which equals what
+    distribute_to_vertices_and_edges then computes, and what the separate
+    kernels (RCM order) give for the same state."""
+    print("case edge_values after fused steps", flush=True)
+    runs = []
+    for reorder in (True, "rcm"):
+        mesh, alg, q, bc = S.hills_de1(m=20, n=20)
+        d = S.setup(Domain(mesh=mesh, reorder=reorder), alg, q, bc)
+        d.evolve_steps(25)
+        before = {n: d.get_quantity(n).edge_values for n in ("stage", "height", "xmomentum")}
+        d.distribute_to_vertices_and_edges()
+        after = {n: d.get_quantity(n).edge_values for n in ("stage", "height", "xmomentum")}
+        runs.append((before, after))
+    (b_t, a_t), (b_r, a_r) = runs
+    for n in a_t:
+        assert np.array_equal(b_t[n], a_t[n]), n
+        assert np.array_equal(a_t[n], a_r[n]), n
 
 
 def test_run_twice_bitwise():
