@@ -182,7 +182,7 @@ __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &
             dy1 = gx.ry[1] - gx.ry[0];
             dy2 = gx.ry[2] - gx.ry[0];
             area2 = dy2 * dx1 - dy1 * dx2;
-            inv_area2 = T(1.0) / area2;
+            inv_area2 = de_rcp(area2);
         }
         if (area2 <= T(0.0)) {
             constant = true;
@@ -194,10 +194,10 @@ __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &
             const T hc = T(c.h), h0 = T(nb[0].h), h1 = T(nb[1].h), h2 = T(nb[2].h);
             const T hmin = fmin(fmin(h0, fmin(h1, h2)), hc);
             const T hmax = fmax(fmax(h0, fmax(h1, h2)), hc);
-            T hfactor = fmax(T(0.0), fmin(c_tmp * fmax(hmin, T(0.0)) / fmax(hc, T(1.0e-06)) + d_tmp,
-                                          fmin(c_tmp * fmax(hc, T(0.0)) / fmax(hmax, T(1.0e-06)) + d_tmp,
+            T hfactor = fmax(T(0.0), fmin(de_div(c_tmp * fmax(hmin, T(0.0)), fmax(hc, T(1.0e-06))) + d_tmp,
+                                          fmin(de_div(c_tmp * fmax(hc, T(0.0)), fmax(hmax, T(1.0e-06))) + d_tmp,
                                                T(1.0))));
-            hfactor = fmin(T(1.2) * fmax(hmin - tmah, T(0.0)) / (fmax(hmin, T(0.0)) + T(1.0) * tmah),
+            hfactor = fmin(de_div(T(1.2) * fmax(hmin - tmah, T(0.0)), fmax(hmin, T(0.0)) + T(1.0) * tmah),
                            hfactor);
 
             T beta = T(p.bwd) + (T(p.bw) - T(p.bwd)) * hfactor;
@@ -243,7 +243,7 @@ __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &
                 const T dx1 = e0 ? gx.rx[0] : (e1 ? gx.rx[1] : gx.rx[2]);
                 const T dy1 = e0 ? gx.ry[0] : (e1 ? gx.ry[1] : gx.ry[2]);
                 const T area2 = dx1 * dx1 + dy1 * dy1;
-                dx2 = T(1.0) / area2;
+                dx2 = de_rcp(area2);
                 dy2 = dx2 * dy1;
                 dx2 *= dx1;
             }
