@@ -328,3 +328,72 @@ class Mesh:
             if len(hit):
                 out[i] = hit[0]
         return out
+
+
+def graded_square(length, fine_area, coarse_area, fine_box, jitter=0.15, seed=0,
+                  origin=(0.0, 0.0)):
+    """An unstructured mesh of a square: near-equilateral triangles of about
+    `fine_area` m^2 inside `fine_box` = (x0, y0, x1, y1) and `coarse_area` m^2
+    elsewhere, like create_domain_from_regions (anuga/extras.py:59,
+    anuga/pmesh/mesh_interface.py:28) with one interior region and two
+    maximum areas. That builder needs the `triangle` library; this one
+    places jittered hexagonal point sets of the two densities (numpy
+    default_rng(seed), jitter as a fraction of the spacing), points along the
+    four sides, and takes their Delaunay triangulation (scipy). Boundary tags
+    'left', 'right', 'bottom', 'top' as rectangular_cross. Returns (points,
+    triangles, boundary)."""
+    from scipy.spatial import Delaunay
+
+    rng = np.random.default_rng(seed)
+    ox, oy = origin
+    L = float(length)
+
+    def lattice(area, box, keep):
+        a = np.sqrt(4.0 * area / np.sqrt(3.0))       # equilateral side of that area
+        h = a * np.sqrt(3.0) / 2.0
+        x0, y0, x1, y1 = box
+        ys = np.arange(y0 + h / 2, y1, h)
+        rows = []
+        for r, y in enumerate(ys):
+            xs = np.arange(x0 + (a / 2 if r % 2 else 0.0) + a / 4, x1, a)
+            rows.append(np.stack([xs, np.full_like(xs, y)], axis=1))
+        p = np.concatenate(rows) if rows else np.empty((0, 2))
+        p = p + rng.uniform(-jitter, jitter, p.shape) * a
+        return p[keep(p, a)], a
+
+    fx0, fy0, fx1, fy1 = fine_box
+    inside = lambda p, m: ((p[:, 0] > fx0 - m) & (p[:, 0] < fx1 + m) &
+                           (p[:, 1] > fy0 - m) & (p[:, 1] < fy1 + m))
+    fine, af = lattice(fine_area, fine_box, lambda p, a: inside(p, 0.0))
+    coarse, ac = lattice(coarse_area, (0.0, 0.0, L, L),
+                         lambda p, a: ~inside(p, 0.5 * a) & (p[:, 0] > 0.5 * a) &
+                         (p[:, 0] < L - 0.5 * a) & (p[:, 1] > 0.5 * a) & (p[:, 1] < L - 0.5 * a))
+    nb = max(int(np.ceil(L / ac)), 1)
+    t = np.linspace(0.0, L, nb + 1)
+    side = np.concatenate([np.stack([t, np.zeros_like(t)], 1), np.stack([t, np.full_like(t, L)], 1),
+                           np.stack([np.zeros_like(t[1:-1]), t[1:-1]], 1),
+                           np.stack([np.full_like(t[1:-1], L), t[1:-1]], 1)])
+    pts = np.concatenate([side, coarse, fine])
+    tri = Delaunay(pts).simplices.astype(np.int64)
+    # counter-clockwise, as the area formula of Mesh expects
+    P = pts[tri]
+    cross = ((P[:, 1, 0] - P[:, 0, 0]) * (P[:, 2, 1] - P[:, 0, 1]) -
+             (P[:, 1, 1] - P[:, 0, 1]) * (P[:, 2, 0] - P[:, 0, 0]))
+    flip = cross < 0
+    tri[flip] = tri[flip][:, [0, 2, 1]]
+    tri = tri[np.abs(cross) > 1e-9 * L * L / len(tri)]
+    # boundary edges (edge i opposite vertex i) and their sides
+    e = np.concatenate([tri[:, [1, 2]], tri[:, [2, 0]], tri[:, [0, 1]]])
+    key = np.sort(e, axis=1)
+    _, inv, cnt = np.unique(key[:, 0] * len(pts) + key[:, 1], return_inverse=True,
+                            return_counts=True)
+    single = np.nonzero(cnt[inv] == 1)[0]
+    vol, edge = single % len(tri), single // len(tri)
+    mid = 0.5 * (pts[e[single, 0]] + pts[e[single, 1]])
+    tol = 1e-9 * L
+    names = np.where(np.abs(mid[:, 0]) <= tol, "left",
+                     np.where(np.abs(mid[:, 0] - L) <= tol, "right",
+                              np.where(np.abs(mid[:, 1]) <= tol, "bottom", "top")))
+    boundary = {(int(v), int(i)): str(n) for v, i, n in zip(vol, edge, names)}
+    pts = pts + np.array([ox, oy])
+    return pts, tri, boundary

@@ -112,10 +112,17 @@ def _reorder(args):
     return {"tiles": True, "rcm": "rcm", "none": False}["none" if args.no_reorder else args.order]
 
 
-def build_c1(m=2000, length=20000.0, nhills=64, seed=0):
-    """Mesh and initial condition of the 16M-triangle DE1 workload."""
-    from anuga_core_amd.mesh import Mesh, rectangular_cross
-    p, v, b = rectangular_cross(m, m, length, length)
+def build_c1(m=2000, length=20000.0, nhills=64, seed=0, kind="rectangular_cross"):
+    """Mesh and initial condition of the 16M-triangle DE1 workload; with
+    kind='graded', the unstructured variant: about 4M triangles of 50 m^2
+    inside the central 13 km square and 400 m^2 outside (graded_square,
+    seed 0), the same bed, stage, friction and boundaries."""
+    from anuga_core_amd.mesh import Mesh, graded_square, rectangular_cross
+    if kind == "graded":
+        f0, f1 = 0.175 * length, 0.825 * length
+        p, v, b = graded_square(length, 50.0, 400.0, (f0, f0, f1, f1), seed=0)
+    else:
+        p, v, b = rectangular_cross(m, m, length, length)
     mesh = Mesh(p, v, b)
     del p, v
     rng = np.random.default_rng(seed)
@@ -503,6 +510,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--m", type=int, default=2000)
+    ap.add_argument("--mesh", default="rectangular_cross", choices=["rectangular_cross", "graded"],
+                    help="single workload mesh: rectangular_cross(m, m), or the unstructured "
+                         "graded variant (about 4M triangles)")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--timing-steps", type=int, default=50,
                     help="steps of the eager pass that times each kernel with HIP events")
@@ -556,7 +566,7 @@ def main():
         return
 
     t0 = time.perf_counter()
-    mesh, q, bc = build_c1(args.m)
+    mesh, q, bc = build_c1(args.m, kind=args.mesh)
     N = mesh.number_of_triangles
     print("mesh built: %d triangles in %.1fs; dry fraction %.3f" %
           (N, time.perf_counter() - t0, float(np.mean(q["stage"] - q["elevation"] <= 1e-5))),
@@ -570,7 +580,7 @@ def main():
                     * LAUNCHES_PER_STEP[s])
     dom, dom_bytes = KERNEL_BYTES[args.precision][dom_short]
     ach = kernels[dom]["alg_GBs"]
-    traffic = measured_traffic(args.precision, dom, args.m)
+    traffic = measured_traffic(args.precision, dom, args.m) if args.mesh != "graded" else None
     step_gbs = STEP_BYTES[args.precision] * N / (r["ms_per_step"] * 1e-3) / 1e9
 
     out = {
@@ -586,9 +596,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32-flux/f64-state",
         "data": "synthetic",
-        "config": {"workload": "rectangular_cross(%d,%d) over %.0f km square, DE1, 64 Gaussian "
+        "config": {"workload": ("rectangular_cross(%d,%d)" % (args.m, args.m) if args.mesh != "graded"
+                                else "graded Delaunay mesh (50 m^2 in the central 13 km square, "
+                                     "400 m^2 outside, seed 0)")
+                               + " over %.0f km square, DE1, 64 Gaussian "
                                "hills (rng seed 0), 3 m hump, n=0.025, transmissive west"
-                               % (args.m, args.m, 20.0),
+                               % 20.0,
                    "triangles": N, "flow_algorithm": "DE1", "precision": args.precision,
                    "order": args.order},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach,

@@ -46,6 +46,31 @@ def hills_de1(m=30, n=30, length=2000.0, nhills=8, seed=0):
     return mesh, "DE1", q, bc
 
 
+def hills_graded(length=2000.0, fine_area=60.0, coarse_area=500.0, nhills=8, seed=0):
+    """hills_de1 on the unstructured graded mesh (about 20k triangles: fine
+    inside the central square of half the side, coarse outside)."""
+    from anuga_core_amd.mesh import graded_square
+    f0, f1 = 0.25 * length, 0.75 * length
+    p, v, b = graded_square(length, fine_area, coarse_area, (f0, f0, f1, f1), seed=seed)
+    mesh = Mesh(p, v, b)
+    rng = np.random.default_rng(seed)
+    x = mesh.centroid_coordinates[:, 0]
+    y = mesh.centroid_coordinates[:, 1]
+    z = np.full_like(x, -50.0)
+    for _ in range(nhills):
+        a = rng.uniform(5.0, 60.0)
+        s = rng.uniform(0.01, 0.075) * length
+        cx, cy = rng.uniform(0, length, 2)
+        z = z + a * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * s * s))
+    sig = 0.05 * length
+    w = np.maximum(3.0 * np.exp(-((x - length / 2) ** 2 + (y - length / 2) ** 2)
+                                / (2 * sig * sig)), z)
+    q = dict(elevation=z, stage=w, friction=np.full(mesh.number_of_triangles, 0.025))
+    bc = {"left": "transmissive", "right": "reflective", "top": "reflective",
+          "bottom": "reflective"}
+    return mesh, "DE1", q, bc
+
+
 def flat_bed_dirichlet():
     """The reference's test_second_order_flat_bed_onestep set-up
     (test_shallow_water_domain.py:5088-5110)."""

@@ -93,6 +93,22 @@ def test_de1_hills_vs_oracle(reorder):
     assert abs(bd - bo) <= TOL * max(abs(bo), 1.0)
 
 
+@pytest.mark.parametrize("reorder", [True, "rcm"])
+def test_graded_unstructured_mesh_vs_oracle(reorder):
+    """The unstructured (Delaunay, two densities, 2-boundary-edge corner
+    triangles) mesh of the large-mesh variant, scaled down: bit for bit."""
+    print("case graded mesh reorder=%s" % reorder, flush=True)
+    o, d = _pair(S.hills_graded, reorder=reorder)
+    for _ in d.evolve(yieldstep=5.0, finaltime=20.0):
+        pass
+    for _ in o.evolve(yieldstep=5.0, finaltime=20.0):
+        pass
+    print("steps", len(d.dt_history), flush=True)
+    assert d.dt_history == o.dt_history, "timestep sequences differ"
+    for name, oname in (("stage", "stage_c"), ("xmomentum", "xmom_c"), ("ymomentum", "ymom_c")):
+        assert np.array_equal(d.get_quantity(name).centroid_values, getattr(o, oname)), name
+
+
 def test_edge_values_after_distribute():
     print("case edge_values", flush=True)
     o, d = _pair(S.hills_de1, m=20, n=20)
