@@ -82,6 +82,8 @@ SIGNATURES = {
     "anuga_set_boundary_values": (C.c_int, [C.c_void_p, C.c_int64, _dp]),
     "anuga_add_rate_operator": (C.c_int, [C.c_void_p, C.c_int64, _ip, _dp, C.c_double,
                                           C.c_double, _ip]),
+    "anuga_add_set_stage_operator": (C.c_int, [C.c_void_p, C.c_int64, _ip, _dp, C.c_double,
+                                               _ip]),
     "anuga_update_rate_operator": (C.c_int, [C.c_void_p, C.c_int64, _dp, C.c_double,
                                              C.c_double]),
     "anuga_apply_operators": (C.c_int, [C.c_void_p]),

@@ -41,6 +41,7 @@ const int kTimerRing = 64;
 
 // a Rate_operator's device data
 struct RateOp {
+    int kind = 0;            // 0: Rate_operator, 1: Set_stage_operator (rate = stage values)
     int n = 0;
     int *idx = nullptr;      // [n] internal triangles
     double *rate = nullptr;  // [n] rates, or NULL: the scalar rate0
@@ -617,12 +618,21 @@ static void enqueue_later_substeps(anuga_domain *d)
 
 // fractional-step operators after a step (apply_fractional_steps,
 // generic_domain.py:1849, :2312-2314), in the order they were added
+static void launch_operator(anuga_domain *d, const RateOp &o)
+{
+    if (o.n == 0) return;   // an empty region
+    const dim3 g((o.n + DE_BLOCK - 1) / DE_BLOCK), b(DE_BLOCK);
+    if (o.kind == 1)
+        hipLaunchKernelGGL(de_set_stage, g, b, 0, d->stream, d->ctl, o.n, o.idx, o.rate, o.rate0,
+                           d->A);
+    else
+        hipLaunchKernelGGL(de_rate, g, b, 0, d->stream, d->ctl, o.n, o.idx, o.rate, o.rate0,
+                           o.factor, o.careful, d->A, d->area, o.influx);
+}
+
 static void launch_operators(anuga_domain *d)
 {
-    for (const RateOp &o : d->ops)
-        hipLaunchKernelGGL(de_rate, dim3((o.n + DE_BLOCK - 1) / DE_BLOCK), dim3(DE_BLOCK), 0,
-                           d->stream, d->ctl, o.n, o.idx, o.rate, o.rate0, o.factor, o.careful,
-                           d->A, d->area, o.influx);
+    for (const RateOp &o : d->ops) launch_operator(d, o);
 }
 
 static void enqueue_device_step(anuga_domain *d)
@@ -1695,8 +1705,23 @@ static int upload_rate(anuga_domain *d, RateOp &o, const double *rates, double r
     return ANUGA_OK;
 }
 
+static int add_operator(anuga_domain *d, int kind, int64_t n, const int64_t *indices,
+                        const double *rates, double rate, double factor, int64_t *id);
+
 int anuga_add_rate_operator(anuga_domain *d, int64_t n, const int64_t *indices,
                             const double *rates, double rate, double factor, int64_t *id)
+{
+    return add_operator(d, 0, n, indices, rates, rate, factor, id);
+}
+
+int anuga_add_set_stage_operator(anuga_domain *d, int64_t n, const int64_t *indices,
+                                 const double *stages, double stage, int64_t *id)
+{
+    return add_operator(d, 1, n, indices, stages, stage, 1.0, id);
+}
+
+static int add_operator(anuga_domain *d, int kind, int64_t n, const int64_t *indices,
+                        const double *rates, double rate, double factor, int64_t *id)
 {
     if (!d || !id) return ANUGA_E_ARG;
     int rc = single(d);
@@ -1704,6 +1729,7 @@ int anuga_add_rate_operator(anuga_domain *d, int64_t n, const int64_t *indices,
     const int N = d->ntri[0];
     if (indices && (n < 0 || n > N)) return fail(d, ANUGA_E_ARG, "bad operator triangle count");
     RateOp o;
+    o.kind = kind;
     o.n = indices ? (int)n : N;
     std::vector<int> ip(N), idx(std::max(o.n, 1));
     for (int p = 0; p < N; p++) ip[d->perm[p]] = p;
@@ -1820,10 +1846,7 @@ int anuga_apply_operator(anuga_domain *d, int64_t id)
     if (id < 0 || id >= (int64_t)d->ops.size()) return fail(d, ANUGA_E_ARG, "no such operator");
     if (d->substate != 0) return fail(d, ANUGA_E_STATE, "inside a multi-stage step");
     ensure_live(d);
-    const RateOp &o = d->ops[id];
-    hipLaunchKernelGGL(de_rate, dim3((o.n + DE_BLOCK - 1) / DE_BLOCK), dim3(DE_BLOCK), 0,
-                       d->stream, d->ctl, o.n, o.idx, o.rate, o.rate0, o.factor, o.careful, d->A,
-                       d->area, o.influx);
+    launch_operator(d, d->ops[id]);
     return launch_check(d);
 }
 

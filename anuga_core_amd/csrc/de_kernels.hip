@@ -1561,6 +1561,19 @@ de_gather_f32(int N, const int *__restrict__ iperm, const double *__restrict__ v
 // depth and the momenta scale with the depth that remains. The inflow
 // (local rate * area) is accumulated for the volume balance (FP64 atomics:
 // a diagnostic, summed in no fixed order).
+// Set_stage_operator after a step (set_quantity.py:95-131, quantity
+// 'stage'): the stage centroid values of the region become the given values
+// (value[i], or value0 for all); momenta and bed unchanged.
+__global__ void __launch_bounds__(DE_BLOCK)
+de_set_stage(const DeControl *ctl, int n, const int *__restrict__ idx,
+             const double *__restrict__ value, double value0, DeQ *S)
+{
+    if (ctl->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    S[idx[i]].w = value ? value[i] : value0;
+}
+
 __global__ void __launch_bounds__(DE_BLOCK)
 de_rate(const DeControl *ctl, int n, const int *__restrict__ idx, const double *__restrict__ rate,
         double rate0, double factor, int careful, DeQ *S, const double *__restrict__ area,

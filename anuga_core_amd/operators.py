@@ -145,6 +145,94 @@ class Rate_operator:
                                       else "%d triangles" % len(self.indices))
 
 
+def _region_indices(domain, region=None, indices=None, polygon=None, center=None, radius=None,
+                    line=None):
+    """Triangle indices of a region as anuga.Region selects them (None: all)."""
+    mesh = domain.mesh
+    cen = mesh.centroid_coordinates
+    if region is not None:
+        indices = getattr(region, "indices", region)
+    elif polygon is not None:
+        indices = np.nonzero(_inside_polygon(cen, polygon))[0]
+    elif center is not None and radius is not None:
+        c = np.asarray(center, np.float64)
+        indices = np.nonzero(((cen - c) ** 2).sum(axis=1) <= radius * radius)[0]
+    elif line is not None:
+        indices = _line_triangles(mesh, line)
+    return None if indices is None else np.unique(np.asarray(indices, np.int64))
+
+
+class Set_stage_operator:
+    """Set_stage_operator(domain, stage=None, region=None, indices=None,
+    polygon=None, center=None, radius=None, line=None, ...)
+    (anuga/operators/set_stage_operator.py:25-58): after every step the stage
+    centroid values of the region become `stage`, a number or a function of
+    t, of (x, y) or of (x, y, t) at the centroids (Set_quantity.get_value,
+    anuga/operators/set_quantity.py:133-158); momenta are unchanged.
+    `indices` None: every triangle; []: none. Static values are applied on
+    the device inside the captured step; a value that depends on t is
+    evaluated on the host at the domain's time after each step, and the
+    domain then takes host-sequenced steps. stage None: the operator does
+    nothing."""
+
+    def __init__(self, domain, stage=None, region=None, indices=None, polygon=None, center=None,
+                 radius=None, line=None, description=None, label=None, logging=False,
+                 verbose=False):
+        self.domain = domain
+        self.indices = _region_indices(domain, region, indices, polygon, center, radius, line)
+        self.center, self.radius = center, radius
+        cen = domain.mesh.centroid_coordinates
+        sel = slice(None) if self.indices is None else self.indices
+        self._coord = cen[sel]
+        self.stage = stage
+        self.time_dependent = callable(stage) and _npar(stage) in (1, 3)
+        self._id = None
+        if stage is None:
+            return
+        values, scalar = self._values(domain.get_time())
+        n = 0 if self.indices is None else len(self.indices)
+        idx = None if self.indices is None else np.ascontiguousarray(self.indices)
+        self._id = C.c_int64()
+        domain._call("anuga_add_set_stage_operator", n,
+                     None if idx is None else idx.ctypes.data_as(C.POINTER(C.c_int64)),
+                     None if values is None else values.ctypes.data_as(C.POINTER(C.c_double)),
+                     C.c_double(scalar), C.byref(self._id))
+        self._keep = (idx, values)
+        domain.fractional_step_operators.append(self)
+
+    def _values(self, t):
+        """(per-triangle stages or None, scalar stage) at time t."""
+        v = self.stage
+        if v is None:
+            return None, 0.0
+        if callable(v):
+            x, y = self._coord[:, 0], self._coord[:, 1]
+            npar = _npar(v)
+            if npar == 1:
+                return None, float(v(t))
+            r = v(x, y) if npar == 2 else v(x, y, t)
+            return np.ascontiguousarray(np.broadcast_to(r, (len(x),)), np.float64), 0.0
+        return None, float(v)
+
+    def get_stage(self, t=None):
+        return self._values(self.domain.get_time() if t is None else t)
+
+    def update(self, t):
+        """Re-evaluate a time-dependent stage at time t."""
+        values, scalar = self._values(t)
+        self._keep = (self._keep[0], values)
+        self.domain._call("anuga_update_rate_operator", self._id.value,
+                          None if values is None else values.ctypes.data_as(C.POINTER(C.c_double)),
+                          C.c_double(scalar), C.c_double(1.0))
+
+    def get_cumulative_influx(self):
+        return 0.0
+
+    def __repr__(self):
+        return "Set_stage_operator(%s)" % ("all triangles" if self.indices is None
+                                           else "%d triangles" % len(self.indices))
+
+
 class Inlet_operator:
     """Inlet_operator(domain, region, Q=0.0, velocity=None, zero_velocity=False,
     default=0.0, ...) (anuga/structures/inlet_operator.py): a discharge Q

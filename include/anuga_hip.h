@@ -167,6 +167,15 @@ int anuga_add_rate_operator(anuga_domain *d, int64_t n, const int64_t *indices,
 /* New rates / factor for operator id (e.g. a rate function evaluated at a new time). */
 int anuga_update_rate_operator(anuga_domain *d, int64_t id, const double *rates, double rate,
                                double factor);
+
+/* Set_stage_operator (anuga/operators/set_stage_operator.py:25-58, applied
+ * by Set_quantity.__call__, anuga/operators/set_quantity.py:95-131): after
+ * every step the stage centroid values of the n triangles (caller's order;
+ * indices NULL: all triangles) become stages[i], or stage when stages is
+ * NULL. Applied in the order operators were added, rate operators included;
+ * update with anuga_update_rate_operator (rates = stages, factor unused). */
+int anuga_add_set_stage_operator(anuga_domain *d, int64_t n, const int64_t *indices,
+                                 const double *stages, double stage, int64_t *id);
 int anuga_apply_operators(anuga_domain *d);
 /* Apply one operator (host-sequenced steps that interleave host-side operators). */
 int anuga_apply_operator(anuga_domain *d, int64_t id);

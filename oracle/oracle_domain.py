@@ -22,6 +22,8 @@ Restates (paths relative to the anuga_core tree):
                           after every step (generic_domain.py:1849)
   * Inlet_operator        anuga/structures/inlet_operator.py:75-183 with
                           Inlet (anuga/structures/inlet.py:63-237)
+  * Set_stage_operator    anuga/operators/set_stage_operator.py:25-58 applied
+                          by Set_quantity.__call__ (set_quantity.py:95-131)
   * riverwall edge data   anuga/structures/riverwall.py:382-469 (the layout the
                           flux loop reads: edge_flux_type, per-rank elevation and
                           hydraulic row, edge_river_wall_counter)
@@ -80,6 +82,34 @@ class OracleRateOperator:
             d.stage_c[idx] = d.stage_c[idx] + local_rates
             d.xmom_c[idx] = d.xmom_c[idx] * local_factors
             d.ymom_c[idx] = d.ymom_c[idx] * local_factors
+
+
+class OracleSetStageOperator:
+    """Set_quantity.__call__ for 'stage' (anuga/operators/set_quantity.py:
+    95-131, via Set_stage_operator, set_stage_operator.py:25-58) restated:
+    indices (None: all triangles), stage a number or a function of t, of
+    (x, y) or of (x, y, t) at the centroids, evaluated at the domain's time
+    when the operator runs."""
+
+    def __init__(self, indices=None, stage=0.0, centroids=None):
+        self.indices = None if indices is None else np.asarray(indices, np.int64)
+        self.stage = stage
+        self.centroids = centroids
+        self.cumulative_influx = 0.0
+
+    def apply(self, d):
+        t = d.get_time()
+        idx = slice(None) if self.indices is None else self.indices
+        v = self.stage
+        if callable(v):
+            import inspect
+            npar = len(inspect.signature(v).parameters)
+            if npar == 1:
+                v = v(t)
+            else:
+                x, y = self.centroids[idx, 0], self.centroids[idx, 1]
+                v = v(x, y) if npar == 2 else v(x, y, t)
+        d.stage_c[idx] = v
 
 
 class OracleInletOperator:

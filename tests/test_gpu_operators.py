@@ -85,3 +85,34 @@ def test_inlets_vs_oracle(alg):
                     OracleInletOperator(l2, lambda t: -0.5 + 0.2 * t, velocity=[0.1, 0.0])]
     assert not d._device_stepping()
     _run_and_compare(o, d, 1.0, 4.0)
+
+
+@pytest.mark.parametrize("alg", ["DE0", "DE1"])
+def test_set_stage_vs_oracle(alg):
+    """Set_stage_operator: a constant over a polygon and a function of (x, y)
+    over a circle applied inside the captured step, next to rain; then a
+    function of t (host-sequenced). Bit for bit against the oracle."""
+    print("case set stage %s" % alg, flush=True)
+    from anuga_core_amd.operators import Set_stage_operator
+    from oracle.oracle_domain import OracleSetStageOperator
+    mesh, o, d = _pair(alg)
+    cen = mesh.centroid_coordinates
+    pond = Set_stage_operator(d, stage=0.4, polygon=[[60, 20], [90, 20], [90, 50], [60, 50]])
+    g = lambda x, y: 0.2 + 0.001 * y
+    hill = Set_stage_operator(d, stage=g, center=[30.0, 80.0], radius=10.0)
+    rain = Rate_operator(d, rate=1.0e-3)
+    o.operators += [OracleSetStageOperator(pond.indices, 0.4),
+                    OracleSetStageOperator(hill.indices, g, cen),
+                    OracleRateOperator(None, 1.0e-3)]
+    assert len(pond.indices) > 0 and len(hill.indices) > 0
+    assert d._device_stepping()
+    _run_and_compare(o, d, 1.0, 3.0)
+
+    mesh, o, d = _pair(alg)
+    f = lambda t: 0.3 + 0.05 * math.sin(t)
+    tide = Set_stage_operator(d, stage=f, indices=np.arange(0, 40))
+    o.operators.append(OracleSetStageOperator(np.arange(0, 40), f))
+    assert not d._device_stepping()
+    _run_and_compare(o, d, 1.0, 3.0)
+    if alg == "DE1":   # rk2 applies operators at the step's end time
+        assert np.all(d.get_quantity("stage").centroid_values[tide.indices] == f(d.get_time()))

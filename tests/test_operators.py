@@ -79,3 +79,39 @@ def test_inside_polygon():
     assert _inside_polygon(pts, square).tolist() == [True, False, True, True, False]
     tri = [[0, 0], [2, 0], [0, 2]]
     assert _inside_polygon(pts, tri).tolist() == [True, True, True, True, False]
+
+
+def test_set_stage_holds_its_region():
+    """Set_stage_operator on the oracle: after every step the region's stage
+    is the set value (the operator runs after the step), the rest of the
+    basin fills from it, momenta in the region are left as the step made
+    them; a function of t is taken at the domain's time after an rk2 step,
+    which for the last step is the final time."""
+    from oracle.oracle_domain import OracleSetStageOperator
+    o = _basin("DE1", depth=1.0, m=8)
+    p, v, b = rectangular_cross(8, 8, 30.0, 30.0)
+    mesh = Mesh(p, v, b)
+    region = np.nonzero(_inside_polygon(mesh.centroid_coordinates,
+                                        [[0, 0], [10, 0], [10, 10], [0, 10]]))[0]
+    assert len(region) > 0
+    o.operators.append(OracleSetStageOperator(region, 1.5))
+    for _ in o.evolve(yieldstep=1.0, finaltime=3.0):
+        pass
+    assert np.all(o.stage_c[region] == 1.5)
+    outside = np.setdiff1d(np.arange(mesh.number_of_triangles), region)
+    assert o.stage_c[outside].max() > 1.0 + 1e-6          # water spreads from the region
+    assert np.any(o.xmom_c[region] != 0.0)                 # momenta not reset
+    o2 = _basin("DE1", depth=1.0, m=8)
+    f = lambda t: 1.0 + 0.1 * t
+    o2.operators.append(OracleSetStageOperator(region, f))
+    for _ in o2.evolve(yieldstep=1.0, finaltime=2.0):
+        pass
+    assert np.all(o2.stage_c[region] == f(o2.get_time()))
+    assert o2.get_time() == 2.0
+    # f(x, y): evaluated at the region's centroids
+    o3 = _basin("DE0", depth=1.0, m=8)
+    g = lambda x, y: 1.0 + 0.01 * x
+    o3.operators.append(OracleSetStageOperator(region, g, mesh.centroid_coordinates))
+    for _ in o3.evolve(yieldstep=0.5, finaltime=1.0):
+        pass
+    assert np.array_equal(o3.stage_c[region], g(*mesh.centroid_coordinates[region].T))
