@@ -1042,7 +1042,7 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
 // de_xflux, with the FP32 edge records of de_extrapolate<float> and the FP64
 // centroid depths in LDS, and the edge terms of de_flux_mixed, so the
 // results are the bits of the separate mixed kernels. Five waves per SIMD
-// (96 VGPRs, a 20 B spill) run 2-3% faster than four (102 VGPRs).
+// (96 VGPRs, a 12 B spill) run 4% faster than four (100 VGPRs, no spill).
 template <int SUBSTEP>
 __global__ void __launch_bounds__(DE_BLOCK, 5)
 de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restrict__ eta,
