@@ -13,7 +13,8 @@ import os
 import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libanuga_hip.so")
+# ANUGA_HIP_LIBRARY: another build of the library (the autotuner's variants)
+LIB_PATH = os.environ.get("ANUGA_HIP_LIBRARY") or os.path.join(HERE, "libanuga_hip.so")
 
 ANUGA_OK = 0
 ANUGA_E_ARG = -1
@@ -109,6 +110,7 @@ SIGNATURES = {
     "anuga_get_negative_cells": (C.c_int, [C.c_void_p, _ip]),
     "anuga_get_water_volume": (C.c_int, [C.c_void_p, _dp]),
     "anuga_rcm_order": (C.c_int, [C.c_int64, _ip, _ip]),
+    "anuga_build_config": (C.c_int, [_ip, _ip]),
     "anuga_tile_order": (C.c_int, [C.c_int64, _dp, _ip]),
     "anuga_tile_halos": (C.c_int, [C.c_int64, C.POINTER(C.c_int32), _ip, C.c_int64,
                                    C.POINTER(C.c_int32), C.POINTER(C.c_uint32), _ip, _ip]),

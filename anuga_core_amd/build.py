@@ -30,20 +30,41 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
-    tmp = OUT + ".tmp"
+TUNED = os.path.join(HERE, "tuned.json")
+# compile-time parameters the autotuner chooses (anuga_core_amd/autotune.py)
+TUNABLE = {"DE_BLOCK": 256, "DE_MIXED_WAVES": 5}
+
+
+def tuned_config():
+    """The autotuner's recorded choice (tuned.json), else the defaults."""
+    import json
+    cfg = dict(TUNABLE)
+    if os.path.exists(TUNED):
+        with open(TUNED) as f:
+            chosen = json.load(f).get("config", {})
+        cfg.update({k: int(v) for k, v in chosen.items() if k in TUNABLE})
+    return cfg
+
+
+def build(verbose=False, out=None, config=None):
+    """Compile the library (to `out`, default in-tree) with `config` (default:
+    the tuned configuration)."""
+    out = out or OUT
+    cfg = tuned_config() if config is None else dict(TUNABLE, **config)
+    tmp = out + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
            "-ffp-contract=off", "-Wno-unused-value", "-Wno-unused-result",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-I" + os.path.join(ROOT, "include"), "-o", tmp] + SOURCES
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"] + \
+          ["-D%s=%d" % kv for kv in sorted(cfg.items())] + \
+          ["-I" + os.path.join(ROOT, "include"), "-o", tmp] + SOURCES
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building %s" % OUT)
-    os.replace(tmp, OUT)
-    return OUT
+        raise RuntimeError("hipcc failed building %s" % out)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

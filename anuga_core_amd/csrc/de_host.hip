@@ -897,7 +897,9 @@ static int validate_mesh(anuga_domain *d, const anuga_mesh *mesh, int64_t N, int
 // triangle, each edge's slot in its tile's list (1023: the neighbour is in
 // the tile). No tiles (hcap 0) if a halo exceeds kMaxHalo
 // (the fused kernel's LDS is 96 B per tile triangle and per halo entry).
-static const int kMaxHalo = 320;
+// the fused FP64 kernel's LDS (96 B per tile and halo triangle) stays within
+// 64 KB per workgroup
+static const int kMaxHalo = std::min(320, (65536 - 96 * DE_BLOCK - 512) / 96);
 
 // The tile metadata of triangles in internal order (nb: packed neighbours of
 // one scenario starting at 0). Returns the largest halo, or -1 when a
@@ -2302,6 +2304,14 @@ int anuga_rcm_order(int64_t N, const int64_t *neighbours, int64_t *order)
         if (neighbours[i] >= N) return ANUGA_E_ARG;
     const std::vector<int> o = rcm_order((int)N, neighbours);
     for (int64_t i = 0; i < N; i++) order[i] = o[i];
+    return ANUGA_OK;
+}
+
+int anuga_build_config(int64_t *block, int64_t *mixed_waves)
+{
+    if (!block || !mixed_waves) return ANUGA_E_ARG;
+    *block = DE_BLOCK;
+    *mixed_waves = DE_MIXED_WAVES;
     return ANUGA_OK;
 }
 

@@ -1041,10 +1041,14 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
 // Tiled fused extrapolation + flux on the mixed path (SUBSTEP 0 or 1): as
 // de_xflux, with the FP32 edge records of de_extrapolate<float> and the FP64
 // centroid depths in LDS, and the edge terms of de_flux_mixed, so the
-// results are the bits of the separate mixed kernels. Five waves per SIMD
-// (96 VGPRs, a 12 B spill) run 4% faster than four (100 VGPRs, no spill).
+// results are the bits of the separate mixed kernels. DE_MIXED_WAVES: the
+// waves per SIMD the register allocation aims at (the autotuner's choice;
+// 5, with 96 VGPRs and a 12 B spill, ran 4% faster than 4 with 100 VGPRs).
+#ifndef DE_MIXED_WAVES
+#define DE_MIXED_WAVES 5
+#endif
 template <int SUBSTEP>
-__global__ void __launch_bounds__(DE_BLOCK, 5)
+__global__ void __launch_bounds__(DE_BLOCK, DE_MIXED_WAVES)
 de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restrict__ eta,
                DeTiles tl, DeBoundary bc, unsigned char *__restrict__ flags,
                double *__restrict__ exw, double *__restrict__ exuh, double *__restrict__ exvh,

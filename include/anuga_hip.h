@@ -325,6 +325,13 @@ int anuga_get_water_volume(anuga_domain *d, double *volume);
  * neighbours is (N,3) with boundary edges negative. */
 int anuga_rcm_order(int64_t N, const int64_t *neighbours, int64_t *order);
 
+/* The compile-time configuration of this build, chosen by the autotuner
+ * (anuga_core_amd/autotune.py, recorded in anuga_core_amd/tuned.json):
+ * threads per workgroup, which is also the tile size of the fused kernels,
+ * and the waves per SIMD the mixed fused kernel's registers aim at.
+ * Host-only; no counterpart in the reference. */
+int anuga_build_config(int64_t *block, int64_t *mixed_waves);
+
 /* The Hilbert-curve triangle order applied at creation when params->reorder
  * is 2: triangles sorted by the Hilbert index of their centroid on a 2^16
  * grid over the bounding box, ties by index; order[new] = old. centroids is

@@ -103,3 +103,13 @@ def test_ensemble_create_rejects_bad_sizes_before_touching_the_device():
     n = np.array([1 << 28, 1 << 28, 10], np.int64)   # too many triangles for one handle
     assert L.anuga_ensemble_create(3, ip(n), ip(m), C.byref(p), C.byref(h)) == _native.ANUGA_E_ARG
     assert not h.value
+
+
+def test_build_matches_the_tuned_configuration():
+    """The loaded library was compiled with the autotuner's recorded choice
+    (anuga_core_amd/tuned.json), which build() reads."""
+    from anuga_core_amd.build import tuned_config
+    b, w = C.c_int64(), C.c_int64()
+    assert _native.lib().anuga_build_config(C.byref(b), C.byref(w)) == 0
+    cfg = tuned_config()
+    assert (b.value, w.value) == (cfg["DE_BLOCK"], cfg["DE_MIXED_WAVES"])
