@@ -2,10 +2,11 @@
 //
 // A handle owns all device buffers of one or more scenarios (independent
 // meshes stepped together: an ensemble). Each scenario's triangles are
-// renumbered once at creation (Reverse-Cuthill-McKee on the triangle
-// adjacency) so that neighbour gathers stay within a narrow index window; the
-// permutation is applied on upload and undone on download, so callers only
-// ever see the reference's order. Scenarios sit back to back in one internal
+// renumbered once at creation (along a Hilbert curve through the centroids,
+// so that each workgroup's run of triangles is a compact tile for the fused
+// kernels, or Reverse-Cuthill-McKee on the triangle adjacency) so that
+// neighbour gathers stay local; the permutation is applied on upload and
+// undone on download, so callers only ever see the reference's order. Scenarios sit back to back in one internal
 // numbering, each starting on a workgroup boundary, so that every workgroup
 // belongs to exactly one scenario and reads that scenario's control block.
 #include <hip/hip_runtime.h>
