@@ -800,12 +800,15 @@ ADJ_HD void adj_bwd_advance(const AdjView &m, const double *S, const unsigned ch
 }
 
 // Flux of triangle k: lam_ex (its explicit update) -> adjoints of its own and
-// its neighbours' edge values (lamE) and of its centroid state.
+// its neighbours' edge values (lamE) and of its centroid state. KF tangents
+// per pass (the four components of one input set in 4 / KF passes): a
+// narrower dual lowers register pressure; the tangents are the same numbers.
+template <int KF = 4>
 ADJ_HD void adj_bwd_flux(const AdjView &m, const double *S, const unsigned char *fl,
                          const double *E, const double *lam_ex, int k, double *lamE,
                          double *lamS, double *zbar)
 {
-    typedef Dual<4> D;
+    typedef Dual<KF> D;
     const int N = m.N;
     int v[3], sur[3];
     adj_neighbours(m, k, v, sur);
@@ -823,28 +826,30 @@ ADJ_HD void adj_bwd_flux(const AdjView &m, const double *S, const unsigned char 
         adj_edge_inputs(m, E, k, i, v[i], L, R, kind, dir);
         for (int pass = 0; pass < 3; pass++) {
             if (pass == 1 && kind >= 0) continue;   // no neighbour edge values
-            D Ld[4], Rd[4], Cd[4];
-            for (int j = 0; j < 4; j++) {
-                Ld[j] = D(L[j]);
-                Rd[j] = D(R[j]);
-                Cd[j] = D(c[j]);
-            }
-            D *seed = pass == 0 ? Ld : (pass == 1 ? Rd : Cd);
-            for (int j = 0; j < 4; j++) seed[j].d[j] = 1.0;
-            D o[5];
-            adj_edge_terms<D>(m, k, i, Ld, Rd, Cd, kind, dir, o);
-            double gj[4];
-            for (int t = 0; t < 4; t++) {
-                gj[t] = 0.0;
-                for (int r = 0; r < 5; r++) gj[t] += lo[r] * o[r].d[t];
-            }
-            if (pass == 2) {
-                for (int t = 0; t < 4; t++) lc[t] += gj[t];
-            } else {
-                const size_t slot = pass == 0 ? (size_t)i * N + k
-                                              : (size_t)(v[i] & 3) * N + (v[i] >> 2);
-                for (int t = 0; t < 4; t++)
-                    if (gj[t] != 0.0) adj_add(lamE + (size_t)t * 3 * N + slot, gj[t]);
+            for (int s0 = 0; s0 < 4; s0 += KF) {
+                D Ld[4], Rd[4], Cd[4];
+                for (int j = 0; j < 4; j++) {
+                    Ld[j] = D(L[j]);
+                    Rd[j] = D(R[j]);
+                    Cd[j] = D(c[j]);
+                }
+                D *seed = pass == 0 ? Ld : (pass == 1 ? Rd : Cd);
+                for (int j = 0; j < KF; j++) seed[s0 + j].d[j] = 1.0;
+                D o[5];
+                adj_edge_terms<D>(m, k, i, Ld, Rd, Cd, kind, dir, o);
+                double gj[KF];
+                for (int t = 0; t < KF; t++) {
+                    gj[t] = 0.0;
+                    for (int r = 0; r < 5; r++) gj[t] += lo[r] * o[r].d[t];
+                }
+                if (pass == 2) {
+                    for (int t = 0; t < KF; t++) lc[s0 + t] += gj[t];
+                } else {
+                    const size_t slot = pass == 0 ? (size_t)i * N + k
+                                                  : (size_t)(v[i] & 3) * N + (v[i] >> 2);
+                    for (int t = 0; t < KF; t++)
+                        if (gj[t] != 0.0) adj_add(lamE + (size_t)(s0 + t) * 3 * N + slot, gj[t]);
+                }
             }
         }
     }

@@ -10,9 +10,13 @@
 
 __device__ __forceinline__ int adj_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
-// tangents per pass of the extrapolation's adjoint (register pressure)
+// tangents per pass of the extrapolation's and the flux's adjoints (register
+// pressure against repeated primal work)
 #ifndef ADJ_EXTRAP_K
 #define ADJ_EXTRAP_K 4
+#endif
+#ifndef ADJ_FLUX_K
+#define ADJ_FLUX_K 2
 #endif
 
 // adjoint of the update (friction, update, fix_negative) of one stage;
@@ -37,7 +41,7 @@ adj_k_flux(AdjView m, const double *__restrict__ S, const unsigned char *__restr
 {
     const int k = adj_tid();
     if (k >= m.N) return;
-    adj_bwd_flux(m, S, fl, E, lam_ex, k, lamE, lamS, zbar);
+    adj_bwd_flux<ADJ_FLUX_K>(m, S, fl, E, lam_ex, k, lamE, lamS, zbar);
 }
 
 __global__ void __launch_bounds__(DE_BLOCK)
