@@ -106,6 +106,7 @@ struct anuga_domain {
     double *eta = nullptr;
     DeQ *A = nullptr;   // step state / rk2 backup: {w, uh, vh, z}
     DeQ *B = nullptr;   // rk2 substep state
+    DeQ *C = nullptr;   // rk3 middle-substep state of the fused kernels (allocated on demand)
     void *E = nullptr;  // FP64: w,h,uh,vh [4][3][N]; mixed: DeEdgeRec32 [N]
     unsigned char *flags = nullptr;
     double *ex[3] = {nullptr, nullptr, nullptr};
@@ -492,43 +493,40 @@ static bool fused(const anuga_domain *d)
     return d->hcap > 0 && !d->rw && !d->P.low_froude && !d->P.use_sloped_mannings;
 }
 
-// extrapolation + flux of substep 0 (explicit updates) or 1 (advance S and
-// combine with bk into out; out must not be S), fused over tiles
-static void launch_xflux(anuga_domain *d, int substep, DeQ *S, DeQ *bk = nullptr,
-                         DeQ *out = nullptr)
+template <int SUB>
+static void launch_xflux_t(anuga_domain *d, DeQ *S, DeQ *bk, DeQ *out)
 {
-    const int T = substep ? T_XFLUX1 : T_XFLUX0;
-    timer_begin(d, T);
     DeBoundary bc{d->bkind, d->bval, d->M, d->bsstride};
     DeTiles tl{d->tstart, d->thalo, d->tslot, d->hcap};
     if (d->mixed) {
         // FP64 centroid depth + 12 FP32 record components per triangle
         const size_t lds = ((size_t)DE_BLOCK + (size_t)d->hcap) * (sizeof(double) + 12 * sizeof(float));
-        if (substep == 0)
-            hipLaunchKernelGGL(de_xflux_mixed<0>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
-                               dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags,
-                               d->ex[0], d->ex[1], d->ex[2], bk, out, d->ctl, d->part,
-                               d->part + d->nblocks, d->part + 2 * (size_t)d->nblocks, d->dneg);
-        else
-            hipLaunchKernelGGL(de_xflux_mixed<1>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
-                               dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags,
-                               d->ex[0], d->ex[1], d->ex[2], bk, out, d->ctl, d->part,
-                               d->part + d->nblocks, d->part + 2 * (size_t)d->nblocks, d->dneg);
-        timer_end(d, T);
-        d->e_stale = true;
-        return;
+        hipLaunchKernelGGL(de_xflux_mixed<SUB>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
+                           dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags, d->ex[0],
+                           d->ex[1], d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
+                           d->part + 2 * (size_t)d->nblocks, d->dneg);
+    } else {
+        const size_t lds = (12 * (size_t)DE_BLOCK + 12 * (size_t)d->hcap) * sizeof(double);
+        hipLaunchKernelGGL(de_xflux<SUB>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
+                           dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags, d->ex[0],
+                           d->ex[1], d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
+                           d->part + 2 * (size_t)d->nblocks, d->dneg);
     }
-    const size_t lds = (12 * (size_t)DE_BLOCK + 12 * (size_t)d->hcap) * sizeof(double);
-    if (substep == 0)
-        hipLaunchKernelGGL(de_xflux<0>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
-                           dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags, d->ex[0],
-                           d->ex[1], d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
-                           d->part + 2 * (size_t)d->nblocks, d->dneg);
-    else
-        hipLaunchKernelGGL(de_xflux<1>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
-                           dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags, d->ex[0],
-                           d->ex[1], d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
-                           d->part + 2 * (size_t)d->nblocks, d->dneg);
+}
+
+// extrapolation + flux of substep 0 (explicit updates) or 1-3 (advance S and
+// combine with bk into out, de_rk_combine; out must not be S), fused over tiles
+static void launch_xflux(anuga_domain *d, int substep, DeQ *S, DeQ *bk = nullptr,
+                         DeQ *out = nullptr)
+{
+    const int T = substep ? T_XFLUX1 : T_XFLUX0;
+    timer_begin(d, T);
+    switch (substep) {
+    case 0: launch_xflux_t<0>(d, S, bk, out); break;
+    case 1: launch_xflux_t<1>(d, S, bk, out); break;
+    case 2: launch_xflux_t<2>(d, S, bk, out); break;
+    default: launch_xflux_t<3>(d, S, bk, out); break;
+    }
     timer_end(d, T);
     d->e_stale = true;
 }
@@ -608,6 +606,15 @@ static void enqueue_later_substeps(anuga_domain *d)
         }
         launch_reduce(d, 1);
     } else if (d->P.timestep_fluxcalls == 3) {
+        if (fused(d) && d->C) {
+            // the middle substep writes C, not B in place: other tiles are
+            // still extrapolating B
+            launch_xflux(d, 2, d->B, d->A, d->C);
+            launch_reduce(d, 1);
+            launch_xflux(d, 3, d->C, d->A, d->A);
+            launch_reduce(d, 4);
+            return;
+        }
         launch_extrapolate(d, d->B);
         launch_flux(d, 2, d->B, d->A, d->B);
         launch_reduce(d, 1);
@@ -1277,6 +1284,12 @@ static int ready_to_step(anuga_domain *d)
     if (d->boundaries_missing) return fail(d, ANUGA_E_STATE, "boundary conditions not set");
     if (d->P.use_sloped_mannings && !d->vx)
         return fail(d, ANUGA_E_ARG, "sloped friction needs the mesh's vertex_coordinates");
+    // rk3 with the fused kernels: a third state for the middle substep
+    // (allocated here, before any graph capture)
+    if (d->P.timestep_fluxcalls == 3 && fused(d) && !d->C) {
+        HIPCHK(d, hipStreamSynchronize(d->stream));
+        HIPCHK(d, dalloc(&d->C, (size_t)d->N));
+    }
     return ANUGA_OK;
 }
 
@@ -1544,7 +1557,7 @@ void anuga_destroy(anuga_domain *d)
     drop_graph(d);
     void *ptrs[] = {d->nb, d->cen, d->gx, d->gf, d->area, d->nx, d->ny, d->len, d->radius,
                     d->scen, d->tstart, d->thalo, d->tslot, d->rw, d->rwd, d->caller, d->vx, d->node_start, d->node_tri, d->iperm_dev,
-                    d->out_f32, d->gtri, d->gtimes, d->gvals, d->sub_idx, d->sub_val, d->eta, d->A, d->B, d->E, d->flags,
+                    d->out_f32, d->gtri, d->gtimes, d->gvals, d->sub_idx, d->sub_val, d->eta, d->A, d->B, d->C, d->E, d->flags,
                     d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->ctl, d->dsetup,
                     d->dneg, d->hist};
     for (void *p : ptrs)

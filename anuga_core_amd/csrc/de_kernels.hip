@@ -737,14 +737,15 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
     }
 }
 
-// Tiled fused extrapolation + flux (FP64, the common kernel's options,
-// SUBSTEP 0 or 1). A workgroup extrapolates its tile's DE_BLOCK triangles and
+// Tiled fused extrapolation + flux (FP64, the common kernel's options, any
+// SUBSTEP; out never aliases q). A workgroup extrapolates its tile's DE_BLOCK triangles and
 // its halo (the neighbour triangles outside the tile, each once) into LDS, then each thread gathers its triangle's three edge fluxes
 // from LDS. The edge values never reach HBM. The arithmetic is
 // de_extrap_cell and de_edge_flux, as in de_extrapolate and de_flux, and the
 // per-block partials (CFL, boundary inflow, protect mass) are formed in the
-// same order, so the results are the same bits. The state read (q) is not
-// written by this kernel (SUBSTEP 1 writes out = the step's initial state).
+// same order, so the results are the same bits. The state read (q) is never
+// written by this kernel: later substeps write out, the step's initial state
+// (rk2, rk3's last) or a third buffer (rk3's middle).
 template <int SUBSTEP>
 __global__ void __launch_bounds__(DE_BLOCK)
 de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restrict__ eta,
@@ -1043,7 +1044,7 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
     }
 }
 
-// Tiled fused extrapolation + flux on the mixed path (SUBSTEP 0 or 1): as
+// Tiled fused extrapolation + flux on the mixed path (any SUBSTEP): as
 // de_xflux, with the FP32 edge records of de_extrapolate<float> and the FP64
 // centroid depths in LDS, and the edge terms of de_flux_mixed, so the
 // results are the bits of the separate mixed kernels. DE_MIXED_WAVES: the
