@@ -485,12 +485,12 @@ static void launch_flux_p(anuga_domain *d, int substep, DeQ *S, DeQ *bk, DeQ *ou
     }
 }
 
-// The fused extrapolation + flux kernels apply with tiles and the common
-// kernel's options (both precisions, any number of scenarios), to substeps 0 and 1
-// (rk3's middle substep writes the state that other tiles are extrapolating).
+// The fused extrapolation + flux kernels apply with tiles (both precisions,
+// any number of scenarios, riverwalls and low_froude too; not sloped friction,
+// whose update reads the edge values from HBM).
 static bool fused(const anuga_domain *d)
 {
-    return d->hcap > 0 && !d->rw && !d->P.low_froude && !d->P.use_sloped_mannings;
+    return d->hcap > 0 && !d->P.use_sloped_mannings;
 }
 
 template <int SUB>
@@ -505,6 +505,14 @@ static void launch_xflux_t(anuga_domain *d, DeQ *S, DeQ *bk, DeQ *out)
                            dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags, d->ex[0],
                            d->ex[1], d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
                            d->part + 2 * (size_t)d->nblocks, d->dneg);
+    } else if (d->rw || d->P.low_froude) {
+        // the general instantiation: walls, low_froude; the flags in LDS too
+        const size_t lds = (12 * (size_t)DE_BLOCK + 12 * (size_t)d->hcap) * sizeof(double) +
+                           (((size_t)DE_BLOCK + d->hcap + 7) & ~(size_t)7);
+        hipLaunchKernelGGL((de_xflux<SUB, true>), dim3(d->nblocks), dim3(DE_BLOCK), lds,
+                           d->stream, dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags,
+                           d->ex[0], d->ex[1], d->ex[2], bk, out, d->ctl, d->part,
+                           d->part + d->nblocks, d->part + 2 * (size_t)d->nblocks, d->dneg);
     } else {
         const size_t lds = (12 * (size_t)DE_BLOCK + 12 * (size_t)d->hcap) * sizeof(double);
         hipLaunchKernelGGL(de_xflux<SUB>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
@@ -905,9 +913,10 @@ static int validate_mesh(anuga_domain *d, const anuga_mesh *mesh, int64_t N, int
 // triangle, each edge's slot in its tile's list (1023: the neighbour is in
 // the tile). No tiles (hcap 0) if a halo exceeds kMaxHalo
 // (the fused kernel's LDS is 96 B per tile triangle and per halo entry).
-// the fused FP64 kernel's LDS (96 B per tile and halo triangle) stays within
-// 64 KB per workgroup
-static const int kMaxHalo = std::min(320, (65536 - 96 * DE_BLOCK - 512) / 96);
+// the fused FP64 kernel's LDS (96 B per tile and halo triangle, 1 more for
+// the general instantiation's flags, and the static reduction arrays) stays
+// within 64 KB per workgroup
+static const int kMaxHalo = std::min(320, (65536 - 97 * DE_BLOCK - 1024) / 97);
 
 // The tile metadata of triangles in internal order (nb: packed neighbours of
 // one scenario starting at 0). Returns the largest halo, or -1 when a

@@ -542,7 +542,8 @@ __device__ __forceinline__ void de_riverwall_weir(const DeRiverwall &rw, double 
 // (vi >= 0) the neighbour's at the shared edge, for a boundary edge the
 // boundary state evaluated here. Adds to the sums sw, sx, sy, the boundary
 // inflow and the CFL bound. Shared by the flux kernel and the tiled fused
-// kernel, so both produce the same bits.
+// kernel, so both produce the same bits. fnb: where the neighbour's flag is
+// (GENERAL walls; NULL: flags[n]).
 template <class T, int SUBSTEP, bool GENERAL>
 __device__ __forceinline__ void de_edge_flux(const DeMesh &m, const DeParams &p,
                                              const DeBoundary &bc, const DeQ *q,
@@ -550,7 +551,8 @@ __device__ __forceinline__ void de_edge_flux(const DeMesh &m, const DeParams &p,
                                              int k, int i, int vi, const DeCentroid &c, T hc,
                                              T zc, T ql0, T hle, T ql1, T ql2, T qr0, T hre,
                                              T qr1, T qr2, double &sw, double &sx, double &sy,
-                                             double &bflux, double &dtmin)
+                                             double &bflux, double &dtmin,
+                                             const unsigned char *fnb = nullptr)
 {
     const T g = T(p.g), eps = T(p.eps);
     const T zl = ql0 - hle;
@@ -623,7 +625,9 @@ __device__ __forceinline__ void de_edge_flux(const DeMesh &m, const DeParams &p,
             double wn = 0.0, zn = c.z;
             if (vi >= 0) {
                 const int n = vi >> 2;
-                const DeCentroid cn = de_centroid(q[n], flags[n]);
+                // the neighbour's protect flag: from the flag array, or from
+                // the fused kernel's LDS (fnb), where a halo neighbour's is
+                const DeCentroid cn = de_centroid(q[n], fnb ? *fnb : flags[n]);
                 wn = cn.w;
                 zn = cn.z;
             }
@@ -737,8 +741,10 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
     }
 }
 
-// Tiled fused extrapolation + flux (FP64, the common kernel's options, any
-// SUBSTEP; out never aliases q). A workgroup extrapolates its tile's DE_BLOCK triangles and
+// Tiled fused extrapolation + flux (FP64, any SUBSTEP; out never aliases q).
+// GENERAL: riverwalls and low_froude 1/2 (not sloped friction, whose update
+// reads the edge values from HBM), with the tile's and the halo's protect
+// flags in LDS for the walls' neighbour centroids. A workgroup extrapolates its tile's DE_BLOCK triangles and
 // its halo (the neighbour triangles outside the tile, each once) into LDS, then each thread gathers its triangle's three edge fluxes
 // from LDS. The edge values never reach HBM. The arithmetic is
 // de_extrap_cell and de_edge_flux, as in de_extrapolate and de_flux, and the
@@ -746,7 +752,7 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
 // same order, so the results are the same bits. The state read (q) is never
 // written by this kernel: later substeps write out, the step's initial state
 // (rk2, rk3's last) or a third buffer (rk3's middle).
-template <int SUBSTEP>
+template <int SUBSTEP, bool GENERAL = false>
 __global__ void __launch_bounds__(DE_BLOCK)
 de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restrict__ eta,
          DeTiles tl, DeBoundary bc, unsigned char *__restrict__ flags, double *__restrict__ exw,
@@ -767,6 +773,8 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
     const int kend = cb->kend;
     const int hcap = tl.hcap;
     double *const hl = lds + 12 * DE_BLOCK;
+    // GENERAL: the protect flags of the tile's and the halo's triangles
+    unsigned char *const lfl = reinterpret_cast<unsigned char *>(hl + 12 * hcap);
     double mass = 0.0, dtmin = 1.0e100, bflux = 0.0;
     unsigned fk = 0;
 
@@ -776,6 +784,7 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
         mass = x.mass;
         fk = x.flag;
         flags[k] = (unsigned char)fk;
+        if constexpr (GENERAL) lfl[t] = (unsigned char)fk;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             double ew, eh, euh, evh;
@@ -793,6 +802,7 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
     for (int j = t; j < nh; j += DE_BLOCK) {
         DeExtrap<double> x;
         de_extrap_cell<double>(m, p, q, tl.halo[h0 + j], err, x);
+        if constexpr (GENERAL) lfl[DE_BLOCK + j] = (unsigned char)x.flag;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             double ew, eh, euh, evh;
@@ -818,6 +828,7 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
             const double ql0 = lds[(0 + i) * DE_BLOCK + t], hle = lds[(3 + i) * DE_BLOCK + t];
             const double ql1 = lds[(6 + i) * DE_BLOCK + t], ql2 = lds[(9 + i) * DE_BLOCK + t];
             double qr0 = 0.0, qr1 = 0.0, qr2 = 0.0, hre = 0.0;
+            const unsigned char *fnb = nullptr;
             if (v[i] >= 0) {
                 const unsigned s = (sl >> (10 * i)) & 1023u;
                 const double *src;
@@ -825,18 +836,20 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
                 if (s == 1023u) {
                     src = lds + (v[i] & 3) * DE_BLOCK + ((v[i] >> 2) - k0);
                     stride = 3 * DE_BLOCK;
+                    if constexpr (GENERAL) fnb = lfl + ((v[i] >> 2) - k0);
                 } else {
                     src = hl + (v[i] & 3) * hcap + s;
                     stride = 3 * hcap;
+                    if constexpr (GENERAL) fnb = lfl + DE_BLOCK + s;
                 }
                 qr0 = src[0];
                 hre = src[stride];
                 qr1 = src[2 * stride];
                 qr2 = src[3 * stride];
             }
-            de_edge_flux<double, SUBSTEP, false>(m, p, bc, q, flags, N, k, i, v[i], c, hc, zc,
-                                                 ql0, hle, ql1, ql2, qr0, hre, qr1, qr2, sw, sx,
-                                                 sy, bflux, dtmin);
+            de_edge_flux<double, SUBSTEP, GENERAL>(m, p, bc, q, flags, N, k, i, v[i], c, hc, zc,
+                                                   ql0, hle, ql1, ql2, qr0, hre, qr1, qr2, sw,
+                                                   sx, sy, bflux, dtmin, fnb);
         }
         const double inv_area = 1.0 / m.area[k];
         sw *= inv_area;
