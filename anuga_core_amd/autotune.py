@@ -85,6 +85,16 @@ def time_one(precision, m, steps, warmup=20):
     return 1e3 * (time.perf_counter() - t0) / steps, mesh.number_of_triangles
 
 
+def choose(rows):
+    """The block size with the fastest FP64 step, then the wave target with
+    the fastest mixed step at that block size."""
+    fp64 = [r for r in rows if r["precision"] == "fp64"]
+    block = min(fp64, key=lambda r: r["ms_per_step"])["DE_BLOCK"]
+    mixed = [r for r in rows if r["precision"] == "mixed" and r["DE_BLOCK"] == block]
+    waves = min(mixed, key=lambda r: r["ms_per_step"])["DE_MIXED_WAVES"]
+    return {"DE_BLOCK": block, "DE_MIXED_WAVES": waves}
+
+
 def measure(m=1000, steps=200, timeout=300, out_path=None):
     rows = []
     for block, waves in itertools.product(BLOCKS, WAVES):
@@ -106,11 +116,7 @@ def measure(m=1000, steps=200, timeout=300, out_path=None):
                    "ms_per_step": res["ms_per_step"], "triangles": res["triangles"]}
             rows.append(row)
             print(json.dumps(row), flush=True)
-    fp64 = [r for r in rows if r["precision"] == "fp64"]
-    block = min(fp64, key=lambda r: r["ms_per_step"])["DE_BLOCK"]
-    mixed = [r for r in rows if r["precision"] == "mixed" and r["DE_BLOCK"] == block]
-    waves = min(mixed, key=lambda r: r["ms_per_step"])["DE_MIXED_WAVES"]
-    out = {"config": {"DE_BLOCK": block, "DE_MIXED_WAVES": waves},
+    out = {"config": choose(rows),
            "workload": "rectangular_cross(%d,%d) DE1 at 10 m cells, %d timed steps" % (m, m, steps),
            "measured": rows}
     with open(out_path or os.path.join(HERE, "tuned.json"), "w") as f:

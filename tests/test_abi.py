@@ -113,3 +113,19 @@ def test_build_matches_the_tuned_configuration():
     assert _native.lib().anuga_build_config(C.byref(b), C.byref(w)) == 0
     cfg = tuned_config()
     assert (b.value, w.value) == (cfg["DE_BLOCK"], cfg["DE_MIXED_WAVES"])
+
+
+def test_autotuner_choice_rule():
+    """The tuner picks the block by the FP64 time, then the wave target by the
+    mixed time at that block (here the recorded sweep gives 256 and 5)."""
+    import json
+    from anuga_core_amd.autotune import choose
+    rows = [dict(DE_BLOCK=b, DE_MIXED_WAVES=w, precision=p, ms_per_step=t)
+            for b, w, p, t in [(128, 5, "fp64", 0.9), (256, 5, "fp64", 0.8), (512, 5, "fp64", 0.85),
+                               (128, 4, "mixed", 0.5), (256, 4, "mixed", 0.6), (256, 5, "mixed", 0.55),
+                               (512, 4, "mixed", 0.4)]]
+    assert choose(rows) == {"DE_BLOCK": 256, "DE_MIXED_WAVES": 5}
+    tuned = os.path.join(os.path.dirname(_native.__file__), "tuned.json")
+    with open(tuned) as f:
+        rec = json.load(f)
+    assert choose(rec["measured"]) == rec["config"]

@@ -93,6 +93,22 @@ def test_de1_hills_vs_oracle(reorder):
     assert abs(bd - bo) <= TOL * max(abs(bo), 1.0)
 
 
+def test_graded_mesh_mixed_same_bits_in_every_order():
+    """Mixed precision on the unstructured mesh: the fused kernels (tiles) and
+    the separate ones (RCM) give the same state bits."""
+    print("case graded mesh mixed orders", flush=True)
+    runs = []
+    for reorder in (True, "rcm"):
+        mesh, alg, q, bc = S.hills_graded()
+        d = S.setup(Domain(mesh=mesh, precision="mixed", reorder=reorder), alg, q, bc)
+        for _ in d.evolve(yieldstep=5.0, finaltime=10.0):
+            pass
+        runs.append((list(d.dt_history), d.get_quantity("stage").centroid_values,
+                     d.get_quantity("xmomentum").centroid_values))
+    assert runs[0][0] == runs[1][0]
+    assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][2], runs[1][2])
+
+
 @pytest.mark.parametrize("reorder", [True, "rcm"])
 def test_graded_unstructured_mesh_vs_oracle(reorder):
     """The unstructured (Delaunay, two densities, 2-boundary-edge corner
