@@ -486,11 +486,10 @@ static void launch_flux_p(anuga_domain *d, int substep, DeQ *S, DeQ *bk, DeQ *ou
 }
 
 // The fused extrapolation + flux kernels apply with tiles (both precisions,
-// any number of scenarios, riverwalls and low_froude too; not sloped friction,
-// whose update reads the edge values from HBM).
+// any number of scenarios, every option).
 static bool fused(const anuga_domain *d)
 {
-    return d->hcap > 0 && !d->P.use_sloped_mannings;
+    return d->hcap > 0;
 }
 
 template <int SUB>
@@ -505,20 +504,21 @@ static void launch_xflux_t(anuga_domain *d, DeQ *S, DeQ *bk, DeQ *out)
                            dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags, d->ex[0],
                            d->ex[1], d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
                            d->part + 2 * (size_t)d->nblocks, d->dneg);
-    } else if (d->rw || d->P.low_froude) {
-        // the general instantiation: walls, low_froude; the flags in LDS too
+    } else if (d->rw || d->P.low_froude || d->P.use_sloped_mannings) {
+        // the general instantiation: walls, low_froude, sloped friction
         const size_t lds = (12 * (size_t)DE_BLOCK + 12 * (size_t)d->hcap) * sizeof(double) +
                            (((size_t)DE_BLOCK + d->hcap + 7) & ~(size_t)7);
         hipLaunchKernelGGL((de_xflux<SUB, true>), dim3(d->nblocks), dim3(DE_BLOCK), lds,
                            d->stream, dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags,
                            d->ex[0], d->ex[1], d->ex[2], bk, out, d->ctl, d->part,
-                           d->part + d->nblocks, d->part + 2 * (size_t)d->nblocks, d->dneg);
+                           d->part + d->nblocks, d->part + 2 * (size_t)d->nblocks, d->dneg,
+                           (double *)d->E);
     } else {
         const size_t lds = (12 * (size_t)DE_BLOCK + 12 * (size_t)d->hcap) * sizeof(double);
         hipLaunchKernelGGL(de_xflux<SUB>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
                            dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags, d->ex[0],
                            d->ex[1], d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
-                           d->part + 2 * (size_t)d->nblocks, d->dneg);
+                           d->part + 2 * (size_t)d->nblocks, d->dneg, (double *)d->E);
     }
 }
 

@@ -742,9 +742,10 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
 }
 
 // Tiled fused extrapolation + flux (FP64, any SUBSTEP; out never aliases q).
-// GENERAL: riverwalls and low_froude 1/2 (not sloped friction, whose update
-// reads the edge values from HBM), with the tile's and the halo's protect
-// flags in LDS for the walls' neighbour centroids. A workgroup extrapolates its tile's DE_BLOCK triangles and
+// GENERAL: riverwalls, low_froude 1/2 and sloped friction, with the tile's
+// and the halo's protect flags in LDS for the walls' neighbour centroids;
+// with sloped friction each triangle also writes its edge stage and depth to
+// Eg, which the sloped update and the later substeps' friction read. A workgroup extrapolates its tile's DE_BLOCK triangles and
 // its halo (the neighbour triangles outside the tile, each once) into LDS, then each thread gathers its triangle's three edge fluxes
 // from LDS. The edge values never reach HBM. The arithmetic is
 // de_extrap_cell and de_edge_flux, as in de_extrapolate and de_flux, and the
@@ -758,7 +759,8 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
          DeTiles tl, DeBoundary bc, unsigned char *__restrict__ flags, double *__restrict__ exw,
          double *__restrict__ exuh, double *__restrict__ exvh, const DeQ *bk, DeQ *out,
          DeControl *ctl, double *__restrict__ part_dt, double *__restrict__ part_bf,
-         double *__restrict__ part_mass, unsigned long long *__restrict__ nneg)
+         double *__restrict__ part_mass, unsigned long long *__restrict__ nneg,
+         double *__restrict__ Eg)
 {
     // [4 quantities][3 edges][DE_BLOCK] tile edge values, then [4][3][hcap] halo
     extern __shared__ double lds[];
@@ -793,6 +795,14 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
             lds[(3 + i) * DE_BLOCK + t] = eh;
             lds[(6 + i) * DE_BLOCK + t] = euh;
             lds[(9 + i) * DE_BLOCK + t] = evh;
+            if constexpr (GENERAL) {
+                // sloped friction reads the triangle's edge stage and depth
+                // (its vertex beds) from HBM, in the update and below
+                if (p.sloped) {
+                    Eg[(size_t)i * N + k] = ew;
+                    Eg[3 * (size_t)N + (size_t)i * N + k] = eh;
+                }
+            }
         }
     }
     // halo triangles: all three edges, so that the edge is chosen by the
@@ -860,7 +870,10 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
             exuh[k] = sx;
             exvh[k] = sy;
         } else {
-            const DeQ r = de_advance<true>(c, eta[k], sw, sx, sy, dt, p, err, nneg);
+            DeSlope sl{false, 0.0, 0.0};
+            if constexpr (GENERAL)
+                if (p.sloped) sl = de_sloped_bed(Eg, N, k, m.vx);
+            const DeQ r = de_advance<true>(c, eta[k], sw, sx, sy, dt, p, err, nneg, sl);
             out[k] = de_rk_combine<SUBSTEP>(r, bk[k]);
         }
     }
