@@ -6,7 +6,7 @@ fused kernel's occupancy target.
 
 Variants: DE_BLOCK in {128, 256, 512} (threads per workgroup, which is also the
 number of triangles per tile of the fused extrapolation + flux kernels) times
-DE_MIXED_WAVES in {4, 5} (the waves per SIMD the mixed fused kernel's register
+DE_MIXED_WAVES in {4, 5, 6} (the waves per SIMD the mixed fused kernel's register
 allocation aims at). `measure` runs each variant in a child process on a
 DE1 workload of rectangular_cross(m, m) at 10 m cells (bed -50 m plus 8
 hills, a 3 m hump, n = 0.025, transmissive west; numpy default_rng(0)), FP64
@@ -30,7 +30,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 VARIANTS_DIR = os.path.join(HERE, "_variants")
 BLOCKS = (128, 256, 512)
-WAVES = (4, 5)
+WAVES = (4, 5, 6)
 
 
 def _variant_path(block, waves):
@@ -102,7 +102,7 @@ def measure(m=1000, steps=200, timeout=300, out_path=None):
         if not os.path.exists(lib):
             raise RuntimeError("variant %s is not built: run `autotune build` first" % lib)
         for precision in ("fp64", "mixed"):
-            if precision == "fp64" and waves != WAVES[-1]:
+            if precision == "fp64" and waves != 5:
                 continue        # the wave target only changes the mixed kernel
             env = dict(os.environ, ANUGA_HIP_LIBRARY=lib)
             r = subprocess.run([sys.executable, "-m", "anuga_core_amd.autotune", "time",
