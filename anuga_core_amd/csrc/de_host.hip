@@ -115,6 +115,7 @@ struct anuga_domain {
     long long bsstride = 0;             // 0, or 3M once per-substep boundary values are set
     std::vector<signed char> hbkind;    // host copy of bkind
     double *part = nullptr;   // [3][nblocks]: dt, bflux, mass
+    double *chunks = nullptr; // [3][S][DE_RCHUNKS]: the reduction's first stage
     DeControl *ctl = nullptr; // [S] device step control
     unsigned long long *dneg = nullptr;
     DeControl *hctl = nullptr;          // [S] pinned host mirror of ctl
@@ -573,11 +574,16 @@ static void launch_reduce(anuga_domain *d, int mode)
 {
     const double *pdt = d->part, *pbf = d->part + d->nblocks;
     const double *pm = d->part + 2 * (size_t)d->nblocks;
-    void (*k)(const double *, const double *, const double *, DeControl *) =
-        mode == 0 ? de_reduce<0>
-                  : mode == 1 ? de_reduce<1>
-                              : mode == 2 ? de_reduce<2> : mode == 3 ? de_reduce<3> : de_reduce<4>;
-    hipLaunchKernelGGL(k, dim3(d->S), dim3(1024), 0, d->stream, pdt, pbf, pm, d->ctl);
+    typedef void (*Chunks)(const double *, const double *, const double *, const DeControl *,
+                           double *);
+    typedef void (*Final)(const double *, DeControl *);
+    const Chunks kc[5] = {de_reduce_chunks<0>, de_reduce_chunks<1>, de_reduce_chunks<2>,
+                          de_reduce_chunks<3>, de_reduce_chunks<4>};
+    const Final kf[5] = {de_reduce_final<0>, de_reduce_final<1>, de_reduce_final<2>,
+                         de_reduce_final<3>, de_reduce_final<4>};
+    hipLaunchKernelGGL(kc[mode], dim3(DE_RCHUNKS, d->S), dim3(256), 0, d->stream, pdt, pbf, pm,
+                       d->ctl, d->chunks);
+    hipLaunchKernelGGL(kf[mode], dim3(d->S), dim3(64), 0, d->stream, d->chunks, d->ctl);
 }
 
 static void launch_control(anuga_domain *d, void (*k)(DeControl *, int))
@@ -808,6 +814,7 @@ static int domain_alloc(const anuga_params *params, int64_t S64, const int64_t *
     DCHK(dalloc(&d->bkind, M));
     DCHK(dalloc(&d->bval, 9 * (size_t)M));   // three substep slots
     DCHK(dalloc(&d->part, 3 * (size_t)d->nblocks));
+    DCHK(dalloc(&d->chunks, 3 * (size_t)S * DE_RCHUNKS));
     DCHK(dalloc(&d->ctl, S));
     DCHK(dalloc(&d->dsetup, S));
     DCHK(dalloc(&d->dneg, 1));
@@ -1567,7 +1574,7 @@ void anuga_destroy(anuga_domain *d)
     void *ptrs[] = {d->nb, d->cen, d->gx, d->gf, d->area, d->nx, d->ny, d->len, d->radius,
                     d->scen, d->tstart, d->thalo, d->tslot, d->rw, d->rwd, d->caller, d->vx, d->node_start, d->node_tri, d->iperm_dev,
                     d->out_f32, d->gtri, d->gtimes, d->gvals, d->sub_idx, d->sub_val, d->eta, d->A, d->B, d->C, d->E, d->flags,
-                    d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->ctl, d->dsetup,
+                    d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->chunks, d->ctl, d->dsetup,
                     d->dneg, d->hist};
     for (void *p : ptrs)
         if (p) hipFree(p);

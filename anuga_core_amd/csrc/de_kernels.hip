@@ -1233,74 +1233,89 @@ de_update(const int *__restrict__ scen, DeParams p, const DeQ *q, const double *
     out[k] = de_advance<CRPOW>(c, eta[k], exw[k], exuh[k], exvh[k], dt, p, err, nneg, sl);
 }
 
-// Fixed-order reduction of the per-block partials into the control blocks:
-// workgroup s (1024 threads) reduces the blocks of scenario s. MODE 0:
+// Fixed-order reduction of the per-block partials into the control blocks,
+// in two stages so that many CUs read the partials: de_reduce_chunks splits
+// each scenario's blocks into DE_RCHUNKS contiguous chunks, one 256-thread
+// workgroup each; de_reduce_final adds a scenario's chunk results. MODE 0:
 // flux_dt, bflux0, mass (first substep); 1: bflux1 (second substep);
-// 2: mass (distribute); 3: volume; 4: bflux2 (third substep). Each thread keeps 8 loads in flight (the
-// partials are few and the loop is bound by memory latency, not bandwidth);
-// the order of the additions depends only on the scenario's block count.
+// 2: mass (distribute); 3: volume; 4: bflux2 (third substep). The order of
+// the additions depends only on the scenario's block count.
+#define DE_RCHUNKS 32
 template <int MODE>
-__global__ void __launch_bounds__(1024)
-de_reduce(const double *__restrict__ part_dt, const double *__restrict__ part_bf,
-          const double *__restrict__ part_mass, DeControl *ctl)
+__global__ void __launch_bounds__(256)
+de_reduce_chunks(const double *__restrict__ part_dt, const double *__restrict__ part_bf,
+                 const double *__restrict__ part_mass, const DeControl *__restrict__ ctl,
+                 double *__restrict__ chunks)
 {
     constexpr bool DT = MODE == 0, BF = MODE <= 1 || MODE == 4;
     constexpr bool MS = MODE == 0 || MODE == 2 || MODE == 3;
-    constexpr int U = 8;
-    __shared__ double s0[16], s1[16], s2[16];
-    DeControl *cb = ctl + blockIdx.x;
-    if ((MODE < 2 || MODE == 4) && cb->done) return;
-    const int b0 = cb->bbeg, b1 = cb->bend;
-    double a = 1.0e100, b = 0.0, c = 0.0;
-    int i = b0 + threadIdx.x;
-    for (; i + (U - 1) * 1024 < b1; i += U * 1024) {
-        double va[U], vb[U], vc[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            if (DT) va[u] = part_dt[i + u * 1024];
-            if (BF) vb[u] = part_bf[i + u * 1024];
-            if (MS) vc[u] = part_mass[i + u * 1024];
+    __shared__ double s0[4], s1[4], s2[4];
+    const int sc = blockIdx.y, c = blockIdx.x, S = gridDim.y;
+    const DeControl *cb = ctl + sc;
+    double a = 1.0e100, b = 0.0, m = 0.0;
+    if (!((MODE < 2 || MODE == 4) && cb->done)) {
+        const int b0 = cb->bbeg, b1 = cb->bend;
+        const int len = (b1 - b0 + DE_RCHUNKS - 1) / DE_RCHUNKS;
+        const int c0 = b0 + c * len, c1 = min(b1, c0 + len);
+        for (int i = c0 + (int)threadIdx.x; i < c1; i += 256) {
+            if (DT) a = fmin(a, part_dt[i]);
+            if (BF) b += part_bf[i];
+            if (MS) m += part_mass[i];
         }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            if (DT) a = fmin(a, va[u]);
-            if (BF) b += vb[u];
-            if (MS) c += vc[u];
-        }
-    }
-    for (; i < b1; i += 1024) {
-        if (DT) a = fmin(a, part_dt[i]);
-        if (BF) b += part_bf[i];
-        if (MS) c += part_mass[i];
     }
     a = de_wave_min(a);
     b = de_wave_sum(b);
-    c = de_wave_sum(c);
+    m = de_wave_sum(m);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) {
         s0[wv] = a;
         s1[wv] = b;
-        s2[wv] = c;
+        s2[wv] = m;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 1; k < 16; k++) {
+        for (int k = 1; k < 4; k++) {
             s0[0] = fmin(s0[0], s0[k]);
             s1[0] += s1[k];
             s2[0] += s2[k];
         }
+        const size_t o = (size_t)sc * DE_RCHUNKS + c, n = (size_t)S * DE_RCHUNKS;
+        chunks[o] = s0[0];
+        chunks[n + o] = s1[0];
+        chunks[2 * n + o] = s2[0];
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(64)
+de_reduce_final(const double *__restrict__ chunks, DeControl *ctl)
+{
+    const int sc = blockIdx.x, S = gridDim.x, t = threadIdx.x;
+    DeControl *cb = ctl + sc;
+    if ((MODE < 2 || MODE == 4) && cb->done) return;
+    const size_t o = (size_t)sc * DE_RCHUNKS + t, n = (size_t)S * DE_RCHUNKS;
+    double a = 1.0e100, b = 0.0, m = 0.0;
+    if (t < DE_RCHUNKS) {
+        a = chunks[o];
+        b = chunks[n + o];
+        m = chunks[2 * n + o];
+    }
+    a = de_wave_min(a);
+    b = de_wave_sum(b);
+    m = de_wave_sum(m);
+    if (t == 0) {
         if (MODE == 0) {
-            cb->flux_dt = s0[0];
-            cb->bflux0 = s1[0];
-            cb->mass = s2[0];
+            cb->flux_dt = a;
+            cb->bflux0 = b;
+            cb->mass = m;
         } else if (MODE == 1) {
-            cb->bflux1 = s1[0];
+            cb->bflux1 = b;
         } else if (MODE == 2) {
-            cb->mass = s2[0];
+            cb->mass = m;
         } else if (MODE == 3) {
-            cb->volume = s2[0];
+            cb->volume = m;
         } else {
-            cb->bflux2 = s1[0];
+            cb->bflux2 = b;
         }
     }
 }
