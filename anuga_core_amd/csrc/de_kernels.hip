@@ -908,7 +908,8 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
 // hcn. Shared by de_flux_mixed and the tiled fused kernel.
 template <int SUBSTEP>
 __device__ __forceinline__ void de_edge_flux_mixed(const DeParams &p, const DeBoundary &bc,
-                                                   const DeGeoF32 &gf, const DeCentroid &c,
+                                                   const DeGeoF32 *__restrict__ gf,
+                                                   const DeCentroid &c,
                                                    double hcd, float hc, int i, int vi,
                                                    float4 el, float4 er, double hcn, double &sw,
                                                    double &sx, double &sy, double &bflux,
@@ -918,11 +919,11 @@ __device__ __forceinline__ void de_edge_flux_mixed(const DeParams &p, const DeBo
     const float hle = float(hcd + double(el.y));
     const float zl = el.x - el.y;
     const float ql1 = el.z, ql2 = el.w;
-    const float n1 = gf.nx[i], n2 = gf.ny[i];
+    const float n1 = gf->nx[i], n2 = gf->ny[i];
     float hre, zr, qr1, qr2, dH = 0.f;
     if (vi >= 0) {
         hre = float(hcn + double(er.y));
-        zr = gf.dzc[i] + (er.x - er.y);
+        zr = gf->dzc[i] + (er.x - er.y);
         qr1 = er.z;
         qr2 = er.w;
         dH = (float(hcn - hcd) + (er.y - el.y)) + (zr - zl);
@@ -975,7 +976,7 @@ __device__ __forceinline__ void de_edge_flux_mixed(const DeParams &p, const DeBo
     float ef[3], speed, pflux;
     de_flux_central_d<float>(ql1, ql2, qr1, qr2, h_left, h_right, hle, hre, n1, n2, eps,
                              diff0, g, ef, speed, pflux);
-    const float length = gf.len[i];
+    const float length = gf->len[i];
     const float dsq = (hl_raw > 0.f) ? (zl - z_half) * (h_left + hle) : -(hle * hle);
     const float pg = length * (-g * 0.5f * (dsq - (hle + hc) * zl) + pflux);
     sw -= double(ef[0] * length);
@@ -985,7 +986,7 @@ __device__ __forceinline__ void de_edge_flux_mixed(const DeParams &p, const DeBo
     sx -= double(n1 * pg);
     sy -= double(n2 * pg);
     if (SUBSTEP == 0 && double(speed) > p.eps)
-        dtmin = fmin(dtmin, double(gf.radius) * (1.0 / double(speed)));
+        dtmin = fmin(dtmin, double(gf->radius) * (1.0 / double(speed)));
 }
 
 // Mixed-precision flux. All FP32 arithmetic is on quantities referred to the
@@ -1020,7 +1021,7 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
         const DeCentroid c = de_centroid(q[k], flags[k]);
         const int4 vk = m.nb[k];
         const int v[3] = {vk.x, vk.y, vk.z};
-        const DeGeoF32 gf = static_cast<const DeGeoF32 *>(m.gf)[k];
+        const DeGeoF32 *gf = static_cast<const DeGeoF32 *>(m.gf) + k;
         const DeEdgeRec32 own = E[k];
         const double hcd = own.hc;
         const float hc = float(hcd);
@@ -1144,7 +1145,7 @@ de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__
         const int4 vk = m.nb[k];
         const int v[3] = {vk.x, vk.y, vk.z};
         const unsigned sl = tl.slot[k];
-        const DeGeoF32 gf = static_cast<const DeGeoF32 *>(m.gf)[k];
+        const DeGeoF32 *gf = static_cast<const DeGeoF32 *>(m.gf) + k;
         const double hcd = hct[t];
         const float hc = float(hcd);
         double sw = 0.0, sx = 0.0, sy = 0.0;
