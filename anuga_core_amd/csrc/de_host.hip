@@ -554,17 +554,13 @@ static void launch_flux(anuga_domain *d, int substep, DeQ *S, DeQ *bk = nullptr,
 static void launch_update(anuga_domain *d, DeQ *S, DeQ *out)
 {
     timer_begin(d, T_UPDATE);
-    if (d->mixed)
-        hipLaunchKernelGGL(de_update<false>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
-                           d->scen, dev_params(d->P), S, d->eta, d->flags, d->ex[0], d->ex[1],
-                           d->ex[2], d->ctl, out, d->dneg);
-    else if (d->P.use_sloped_mannings)
-        hipLaunchKernelGGL((de_update<true, true>), dim3(d->nblocks), dim3(DE_BLOCK), 0,
+    if (d->P.use_sloped_mannings && !d->mixed)
+        hipLaunchKernelGGL((de_update<true>), dim3(d->nblocks), dim3(DE_BLOCK), 0,
                            d->stream, d->scen, dev_params(d->P), S, d->eta, d->flags, d->ex[0],
                            d->ex[1], d->ex[2], d->ctl, out, d->dneg, (const double *)d->E, d->N,
                            d->vx);
     else
-        hipLaunchKernelGGL(de_update<true>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
+        hipLaunchKernelGGL(de_update<false>, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream,
                            d->scen, dev_params(d->P), S, d->eta, d->flags, d->ex[0], d->ex[1],
                            d->ex[2], d->ctl, out, d->dneg);
     timer_end(d, T_UPDATE);
@@ -1489,7 +1485,7 @@ static void adj_stages(anuga_domain *d, AdjRun *a, const DeQ *X)
                        a->E0, bc, a->f0, a->ex0, a->ex0 + N, a->ex0 + 2 * N, nullptr, nullptr,
                        d->ctl,
                        d->part, d->part + d->nblocks, a->neg);
-    hipLaunchKernelGGL(de_update<true>, g, b, 0, d->stream, d->scen, p, X, d->eta, a->f0, a->ex0,
+    hipLaunchKernelGGL(de_update<false>, g, b, 0, d->stream, d->scen, p, X, d->eta, a->f0, a->ex0,
                        a->ex0 + N, a->ex0 + 2 * N, d->ctl, a->S1, a->neg);
     hipLaunchKernelGGL(de_extrapolate<double>, g, b, 0, d->stream, dev_mesh(d), p, a->S1, a->E1,
                        a->f1, pm, d->ctl);

@@ -386,7 +386,6 @@ __device__ __forceinline__ double de_update_one(double q, double ex, double si, 
     return q / den;
 }
 
-// CRPOW: correctly rounded pow(h, 7/3) (the FP64 path); else the math library's
 // bed terms of sloped friction for one triangle
 struct DeSlope {
     bool on;
@@ -420,7 +419,6 @@ __device__ __forceinline__ DeSlope de_sloped_bed(const double *__restrict__ E, i
     return s;
 }
 
-template <bool CRPOW>
 __device__ __forceinline__ DeQ de_advance(const DeCentroid c, double eta, double exw, double exuh,
                                           double exvh, double dt, const DeParams &p, int *err,
                                           unsigned long long *nneg,
@@ -435,8 +433,7 @@ __device__ __forceinline__ DeQ de_advance(const DeCentroid c, double eta, double
         if (h >= p.mah) {
             double S = sl.on ? -p.g * eta * eta * sl.zs * sqrt((c.uh * c.uh + c.vh * c.vh))
                              : -p.g * eta * eta * sqrt((c.uh * c.uh + c.vh * c.vh));
-            if constexpr (CRPOW) S /= de_pow_7_3(h);
-            else S /= pow(h, 7.0 / 3.0);
+            S /= de_pow_7_3(h);
             si_uh += S * c.uh;
             si_vh += S * c.vh;
         }
@@ -717,7 +714,7 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
             DeSlope sl{false, 0.0, 0.0};
             if constexpr (GENERAL)
                 if (p.sloped) sl = de_sloped_bed((const double *)E, N, k, m.vx);
-            const DeQ r = de_advance<sizeof(T) == 8>(c, eta[k], sw, sx, sy, dt, p, err, nneg, sl);
+            const DeQ r = de_advance(c, eta[k], sw, sx, sy, dt, p, err, nneg, sl);
             out[k] = de_rk_combine<SUBSTEP>(r, bk[k]);
         }
     }
@@ -873,7 +870,7 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
             DeSlope sl{false, 0.0, 0.0};
             if constexpr (GENERAL)
                 if (p.sloped) sl = de_sloped_bed(Eg, N, k, m.vx);
-            const DeQ r = de_advance<true>(c, eta[k], sw, sx, sy, dt, p, err, nneg, sl);
+            const DeQ r = de_advance(c, eta[k], sw, sx, sy, dt, p, err, nneg, sl);
             out[k] = de_rk_combine<SUBSTEP>(r, bk[k]);
         }
     }
@@ -1047,7 +1044,7 @@ de_flux_mixed(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta
             exuh[k] = sx;
             exvh[k] = sy;
         } else {
-            const DeQ r = de_advance<false>(c, eta[k], sw, sx, sy, dt, p, err, nneg);
+            const DeQ r = de_advance(c, eta[k], sw, sx, sy, dt, p, err, nneg);
             out[k] = de_rk_combine<SUBSTEP>(r, bk[k]);
         }
     }
@@ -1145,7 +1142,10 @@ de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__
         const int4 vk = m.nb[k];
         const int v[3] = {vk.x, vk.y, vk.z};
         const unsigned sl = tl.slot[k];
-        const DeGeoF32 *gf = static_cast<const DeGeoF32 *>(m.gf) + k;
+        // the whole record is loaded up front here (the fused kernel's
+        // edge loop runs faster with it in registers than through a pointer)
+        const DeGeoF32 gfr = static_cast<const DeGeoF32 *>(m.gf)[k];
+        const DeGeoF32 *gf = &gfr;
         const double hcd = hct[t];
         const float hc = float(hcd);
         double sw = 0.0, sx = 0.0, sy = 0.0;
@@ -1183,7 +1183,7 @@ de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__
             exuh[k] = sx;
             exvh[k] = sy;
         } else {
-            const DeQ r = de_advance<false>(c, eta[k], sw, sx, sy, dt, p, err, nneg);
+            const DeQ r = de_advance(c, eta[k], sw, sx, sy, dt, p, err, nneg);
             out[k] = de_rk_combine<SUBSTEP>(r, bk[k]);
         }
     }
@@ -1213,7 +1213,7 @@ de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__
 
 // friction + update + fix_negative_cells from the explicit updates of substep 0;
 // `out` may alias `q` (Euler, in place: each thread reads then writes its own record)
-template <bool CRPOW, bool SLOPED = false>
+template <bool SLOPED = false>
 __global__ void __launch_bounds__(DE_BLOCK)
 de_update(const int *__restrict__ scen, DeParams p, const DeQ *q, const double *__restrict__ eta,
           const unsigned char *__restrict__ flags, const double *__restrict__ exw,
@@ -1231,7 +1231,7 @@ de_update(const int *__restrict__ scen, DeParams p, const DeQ *q, const double *
     const DeCentroid c = de_centroid(q[k], flags[k]);
     DeSlope sl{false, 0.0, 0.0};
     if constexpr (SLOPED) sl = de_sloped_bed(E, N, k, vx);
-    out[k] = de_advance<CRPOW>(c, eta[k], exw[k], exuh[k], exvh[k], dt, p, err, nneg, sl);
+    out[k] = de_advance(c, eta[k], exw[k], exuh[k], exvh[k], dt, p, err, nneg, sl);
 }
 
 // Fixed-order reduction of the per-block partials into the control blocks,

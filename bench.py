@@ -156,7 +156,7 @@ def cpu_baseline(mesh, q, bc, steps):
     bounded number of steps of the same workload."""
     from oracle.kernels import OracleKernels
     from oracle.oracle_domain import OracleDomain
-    o = OracleDomain(mesh, OracleKernels(gather=True, pow_mode=0), "DE1")
+    o = OracleDomain(mesh, OracleKernels(gather=True), "DE1")
     for name, val in q.items():
         o.set_centroid_values(name, val)
     o.set_boundary({t: type("B", (), {"kind": k})() for t, k in bc.items()})
@@ -367,7 +367,7 @@ def run_ensemble(args):
         for i in pick:
             k, prm = spec[i]
             mesh, q, bc = ensemble_scenario(k, prm)
-            o = OracleDomain(mesh, OracleKernels(gather=True, pow_mode=0), "DE1")
+            o = OracleDomain(mesh, OracleKernels(gather=True), "DE1")
             for name, val in q.items():
                 o.set_centroid_values(name, val)
             o.set_boundary({t: type("B", (), {"kind": kk})() for t, kk in bc.items()})

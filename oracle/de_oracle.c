@@ -37,32 +37,15 @@
  * reference's default build has none).
  */
 #include <math.h>
-#include <quadmath.h>
 #include <stdint.h>
 #include <string.h>
 
 /*
  * The reference's two library pow calls: pow(h, 7/3) of Manning friction and
- * pow(1 - Q2/Q1, 0.385) of the weir. Mode 0 calls the C library's pow exactly
- * as the reference does. Mode 1 returns the correctly rounded value: glibc's
- * pow is within 0.52 ulp, so its result is the correctly rounded one unless
- * the exact value is within 0.02 ulp of a rounding boundary; that is tested
- * with an 80-bit powl residual, and such rare cases are settled with 113-bit
- * powq.
+ * pow(1 - Q2/Q1, 0.385) of the weir, made exactly as the reference makes them
+ * (the C library's pow; glibc's on this platform).
  */
-static int g_pow_mode = 0;
-
-void oracle_set_pow_mode(int mode) { g_pow_mode = mode; }
-
-double oracle_pow(double x, double y)
-{
-    double p = pow(x, y);
-    long double e;
-    if (g_pow_mode == 0 || p == 0.0) return p;
-    e = powl((long double)x, (long double)y) - (long double)p;
-    if (fabsl(e) < 0.49L * (long double)(nextafter(p, INFINITY) - p)) return p;
-    return (double)powq((__float128)x, (__float128)y);
-}
+double oracle_pow(double x, double y) { return pow(x, y); }
 
 double oracle_pow_7_3(double h) { return oracle_pow(h, 7.0 / 3.0); }
 
@@ -779,7 +762,7 @@ void oracle_manning_friction_flat(int64_t N, double g, double eps,
             double h = w[k] - z[k];
             if (h >= eps) {
                 double S = -g * eta[k] * eta[k] * sqrt((uh[k] * uh[k] + vh[k] * vh[k]));
-                S /= (g_pow_mode == 0) ? pow(h, seven_thirds) : oracle_pow_7_3(h);
+                S /= pow(h, seven_thirds);
                 xmom_si[k] += S * uh[k];
                 ymom_si[k] += S * vh[k];
             }
@@ -812,7 +795,7 @@ void oracle_manning_friction_sloped(int64_t N, double g, double eps, const doubl
             h = w[k] - (z0 + z1 + z2) * one_third;
             if (h >= eps) {
                 double S = -g * eta[k] * eta[k] * zs * sqrt((uh[k] * uh[k] + vh[k] * vh[k]));
-                S /= (g_pow_mode == 0) ? pow(h, seven_thirds) : oracle_pow_7_3(h);
+                S /= pow(h, seven_thirds);
                 xmom_si[k] += S * uh[k];
                 ymom_si[k] += S * vh[k];
             }

@@ -98,14 +98,10 @@ class OracleKernels:
     """Kernels of oracle/de_oracle.c. `gather=True` uses the per-triangle
     (OpenMP) flux loop instead of the edge-once-and-mirror loop."""
 
-    def __init__(self, gather=False, pow_mode=1):
-        """pow_mode 0: the C library's pow, as the reference; 1: correctly
-        rounded pow(h, 7/3) (see oracle_pow_7_3 in de_oracle.c)."""
+    def __init__(self, gather=False):
         lib = _load(os.path.join(HERE, "liboracle.so"))
         self.lib = lib
         self.gather = gather
-        self.pow_mode = pow_mode
-        lib.oracle_set_pow_mode.argtypes = [C.c_int]
         lib.oracle_pow_7_3.restype = C.c_double
         lib.oracle_pow_7_3.argtypes = [C.c_double]
         lib.oracle_pow.restype = C.c_double
@@ -171,7 +167,6 @@ class OracleKernels:
     def compute_fluxes(self, substep, timestep):
         p = self._params()
         d = self.d
-        self.lib.oracle_set_pow_mode(self.pow_mode)     # the weir's pow
         if self.gather:
             return self.lib.oracle_compute_fluxes_gather(
                 C.byref(self.mesh), C.byref(p), C.byref(self.state), substep, timestep,
@@ -183,7 +178,6 @@ class OracleKernels:
 
     def friction(self):
         d = self.d
-        self.lib.oracle_set_pow_mode(self.pow_mode)
         if d.use_sloped_mannings:
             self.lib.oracle_manning_friction_sloped(d.N, d.g, d.minimum_allowed_height,
                                                     _d(d.vertex_coordinates), _d(d.stage_c),
@@ -242,7 +236,6 @@ class OracleKernels:
                                   Qfactor, s1, s2, h1, h2, max_speed):
         ef = np.array(edgeflux, dtype=np.float64)
         ms = np.array([max_speed], dtype=np.float64)
-        self.lib.oracle_set_pow_mode(self.pow_mode)
         self.lib.oracle_adjust_edgeflux_with_weir(_d(ef), h_left, h_right, g, weir_height,
                                                   Qfactor, s1, s2, h1, h2, _d(ms))
         return ef, ms[0]

@@ -45,7 +45,7 @@ def test_options_bitwise_vs_reference_c(builder, kw, ys, ft, low_froude, sloped)
     r = options_run(RefKernels(), builder, kw, ys, ft, low_froude, sloped)
     loops = [False] if low_froude else [False, True]
     for gather in loops:
-        o = options_run(OracleKernels(gather=gather, pow_mode=0), builder, kw, ys, ft,
+        o = options_run(OracleKernels(gather=gather), builder, kw, ys, ft,
                         low_froude, sloped)
         assert r.dt_history == o.dt_history
         for a in ("stage_c", "xmom_c", "ymom_c"):

@@ -2,7 +2,8 @@
 
 * C0: rectangular_cross(100,100) over 100 m x 100 m (40,000 triangles), DE0,
   bed -0.01 x, 1 m dam for x < 50 (dry elsewhere), n = 0.03, reflective walls,
-  evolve(yieldstep=1, finaltime=20): device vs oracle at every yield.
+  evolve(yieldstep=1, finaltime=20): device vs oracle at every yield, bit for
+  bit (the oracle makes the reference's own C library pow calls).
 * C2: mixed FP32/FP64 against the FP64 device path and the CPU oracle after
   1000 DE1 steps on the 64-hill workload scaled to rectangular_cross(150,150)
   (90,000 triangles, same bed / hump / friction / boundaries per km).
@@ -51,7 +52,7 @@ def test_c0_dam_break_full_run_vs_oracle():
     for (to, *qo), (td, *qd) in zip(yo, yd):
         assert to == td
         for a, b in zip(qd, qo):
-            assert S.rel_maxnorm(a, b) <= 1e-12
+            assert np.array_equal(a, b)
     vo, vd = o.get_water_volume(), d.get_water_volume()
     assert abs(vo - vd) <= 1e-12 * vo
 
@@ -100,6 +101,7 @@ def test_c2_mixed_error_report():
                - f.get_quantity("stage").centroid_values).sum() / depth.sum())
     for k, v in out.items():
         print("C2", k, v, flush=True)
-    assert out["stage_fp64_gpu_vs_cpu_oracle"]["max"] <= 1e-12 * np.abs(o.stage_c).max()
+    for name, oname in (("stage", "stage_c"), ("xmomentum", "xmom_c"), ("ymomentum", "ymom_c")):
+        assert np.array_equal(f.get_quantity(name).centroid_values, getattr(o, oname)), name
     assert out["stage_L1_over_water_column"] <= 1e-5
     assert abs(out["volume_drift_mixed"]) <= 1e-9 and abs(out["volume_drift_fp64"]) <= 1e-12

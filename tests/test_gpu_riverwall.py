@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 
 def _pair(builder, kw, flags=None, reorder=True):
     mesh, alg, q, bc, walls, par = builder(**kw)
-    o = S.setup(OracleDomain(mesh, OracleKernels(gather=False, pow_mode=1)), alg, q, bc)
+    o = S.setup(OracleDomain(mesh, OracleKernels(gather=False)), alg, q, bc)
     d = S.setup(Domain(mesh=mesh, reorder=reorder), alg, q, bc)
     eft, elev, rows, hp = S.riverwall_arrays(mesh, walls, par)
     if flags is not None:

@@ -14,7 +14,6 @@
 
 import math
 import os
-from decimal import Decimal, getcontext
 
 import numpy as np
 import pytest
@@ -33,8 +32,8 @@ EPS = 1.0e-12
 
 
 def _kernel_sets():
-    ks = [("oracle", lambda: OracleKernels(pow_mode=0)),
-          ("oracle-gather", lambda: OracleKernels(gather=True, pow_mode=0))]
+    ks = [("oracle", OracleKernels),
+          ("oracle-gather", lambda: OracleKernels(gather=True))]
     if HAVE_REF:
         ks.append(("reference", RefKernels))
     return ks
@@ -101,7 +100,7 @@ def _flat_bed(K):
     return d
 
 
-GOLDEN_KS = KSETS + [("oracle-crpow", lambda: OracleKernels(gather=True, pow_mode=1))]
+GOLDEN_KS = KSETS
 
 
 @pytest.mark.parametrize("name,K", GOLDEN_KS, ids=[k[0] for k in GOLDEN_KS])
@@ -175,7 +174,7 @@ def test_oracle_bitwise_vs_reference_c(builder, kw, yieldstep, final, gather):
     """The restatement equals the reference's compiled C bit for bit, every
     state array and the whole timestep sequence (libm pow on both sides)."""
     res = []
-    for K in (RefKernels(), OracleKernels(gather=gather, pow_mode=0)):
+    for K in (RefKernels(), OracleKernels(gather=gather)):
         mesh, alg, q, bc = builder(**kw)
         d = S.setup(OracleDomain(mesh, K), alg, q, bc)
         for _ in d.evolve(yieldstep=yieldstep, finaltime=final):
@@ -213,10 +212,9 @@ def _four_tri_domain(K):
     return OracleDomain(m, K, "DE0")
 
 
-@pytest.mark.parametrize("pow_mode", [0, 1])
-def test_flat_manning_friction_closed_form(pow_mode):
+def test_flat_manning_friction_closed_form():
     """test_flat_manning_friction (:3276-3356): S = -g eta^2 |uh| / h^(7/3)."""
-    d = _four_tri_domain(OracleKernels(pow_mode=pow_mode))
+    d = _four_tri_domain(OracleKernels())
     d.minimum_allowed_height = 1e-5
     V = d.mesh.vertex_coordinates
     zv = (3 * V[:, 0]).reshape(-1, 3)
@@ -259,24 +257,14 @@ def test_update_closed_forms():
         d.kernels.update(dt)
 
 
-def test_pow_modes():
-    """Mode 0 is the C library's pow; mode 1 is correctly rounded (80-digit decimal)."""
-    k = OracleKernels()
-    lib = k.lib
-    getcontext().prec = 80
-    y = Decimal(7.0 / 3.0)
+def test_oracle_pow_is_the_c_library_pow():
+    """Both pow calls go to the C library's pow, as the reference's C makes them."""
+    lib = OracleKernels().lib
     rng = np.random.default_rng(5)
     xs = np.concatenate([10 ** rng.uniform(-12, 4, 300), rng.uniform(1e-5, 50, 300),
                          [1.0, 2.0, 1e-5, 0.5, 1e-12, 1e4]])
-    lib.oracle_set_pow_mode(0)
     assert all(lib.oracle_pow_7_3(x) == x ** (7.0 / 3.0) for x in xs)
-    lib.oracle_set_pow_mode(1)
-    for x in xs:
-        r = lib.oracle_pow_7_3(float(x))
-        exact = (y * Decimal(float(x)).ln()).exp()
-        err = abs(Decimal(r) - exact) / Decimal(math.ulp(r))
-        assert err <= Decimal("0.5"), (x, float(err))
-    lib.oracle_set_pow_mode(0)
+    assert all(lib.oracle_pow(x, 0.385) == x ** 0.385 for x in xs)
 
 
 def test_rk2_step_is_ssp_average_of_two_euler_steps():

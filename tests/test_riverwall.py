@@ -134,9 +134,8 @@ def _weir(k, ef, hl, hr, wh, qf=1.0, s1=0.9, s2=0.95, h1=1.0, h2=1.5, ms=0.0):
     return k.adjust_edgeflux_with_weir(ef, hl, hr, G, wh, qf, s1, s2, h1, h2, ms)
 
 
-@pytest.mark.parametrize("pow_mode", [0, 1])
-def test_weir_closed_forms(pow_mode):
-    k = OracleKernels(pow_mode=pow_mode)
+def test_weir_closed_forms():
+    k = OracleKernels()
     two3 = 2.0 / 3.0
     # free overflow (dry downstream): Qfactor 2/3 H sqrt(2/3 g H); momentum
     # fluxes scaled like the mass flux; wave speed from the weir head
@@ -178,7 +177,7 @@ def test_weir_closed_forms(pow_mode):
 @pytest.mark.parametrize("seed", [0, 1])
 def test_weir_bitwise_vs_reference_c(seed):
     rng = np.random.default_rng(seed)
-    ref, ora = RefKernels(), OracleKernels(pow_mode=0)
+    ref, ora = RefKernels(), OracleKernels()
     for _ in range(4000):
         hl, hr = rng.random(2) * rng.choice([1e-3, 1.0, 5.0], 2)
         if rng.random() < 0.2:
@@ -251,7 +250,7 @@ def test_oracle_riverwalls_bitwise_vs_reference_c(builder, kw, ys, ft, flags):
     r = _oracle_run(RefKernels(), builder, kw, ys, ft, flags)
     loops = [False] if flags else [False, True]
     for gather in loops:
-        o = _oracle_run(OracleKernels(gather=gather, pow_mode=0), builder, kw, ys, ft, flags)
+        o = _oracle_run(OracleKernels(gather=gather), builder, kw, ys, ft, flags)
         assert r.dt_history == o.dt_history
         for a in ("stage_c", "xmom_c", "ymom_c", "height_c", "stage_e", "xmom_e", "ymom_e"):
             assert np.array_equal(getattr(r, a), getattr(o, a)), (a, gather)
@@ -267,7 +266,7 @@ def _land_volume(d):
 def test_reference_riverwall_physics_on_oracle(alg):
     """test_riverwall_structure.py:152-500 restated: L = 100 m of wall at
     x = 50, over 1e-3 s."""
-    K = lambda: OracleKernels(gather=False, pow_mode=1)
+    K = lambda: OracleKernels(gather=False)
     L, ft = 100.0, 1.0e-3
     # no flow while the water is below the crest (10 s)
     d = _oracle_run(K(), S.riverwall_structure, dict(wall_height=-0.2, ocean=-0.3, alg=alg),
