@@ -9,6 +9,10 @@ library's discrete adjoint (anuga_adjoint_record / anuga_adjoint_gradient).
     loss = ((stage - observed) ** 2).sum()
     loss.backward()                         # n.grad, z.grad per triangle
 
+Each backward differentiates the run of its own forward: when other runs were
+recorded on the domain in between (two terms of one loss, a closure
+optimiser), it records its run again before the adjoint sweep.
+
 The gradient is that of the discrete scheme the forward computes (FP64 path,
 branches of the limiter and of wet/dry taken where the forward took them);
 the bed derivative is at fixed stage. Tensors are float64 on the CPU: the
@@ -38,11 +42,19 @@ class GaugeRun:
         class _Fn(torch.autograd.Function):
             @staticmethod
             def forward(ctx, friction, elevation):
-                return torch.from_numpy(run.forward(friction.detach().cpu().numpy(),
-                                                    elevation.detach().cpu().numpy()))
+                n = friction.detach().cpu().numpy().copy()
+                z = elevation.detach().cpu().numpy().copy()
+                out = torch.from_numpy(run.forward(n, z))
+                # the run this output came from, and its parameters, so that a
+                # backward after other forwards re-records this one
+                ctx.record = run.domain.adjoint_record_serial
+                ctx.params = (n, z)
+                return out
 
             @staticmethod
             def backward(ctx, grad):
+                if run.domain.adjoint_record_serial != ctx.record:
+                    run.forward(*ctx.params)
                 gn, gz, _ = run.domain.adjoint_gradient(grad.detach().cpu().numpy())
                 return torch.from_numpy(gn), torch.from_numpy(gz)
 

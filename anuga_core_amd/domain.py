@@ -186,6 +186,8 @@ class Domain:
         self._last_negative_total = 0
         self.mass_error = 0.0
         self._elev_vertex = np.zeros((N, 3))
+        # bumped by every adjoint_record, so a gradient can tell which run it has
+        self.adjoint_record_serial = 0
 
         self._handle = C.c_void_p()
         self._create_device()
@@ -1017,6 +1019,7 @@ class Domain:
                    len(g), g.ctypes.data_as(C.POINTER(C.c_int64)), int(obs_every), _dptr(out))
         self.relative_time += int(nsteps) * float(dt)
         self._adjoint_shape = (nobs, len(g))
+        self.adjoint_record_serial += 1
         return out[:nobs]
 
     def adjoint_gradient(self, dloss_dstage):

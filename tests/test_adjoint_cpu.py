@@ -19,6 +19,7 @@ step straddles fewer kinks; below 1e-9 rounding dominates).
 import ctypes as C
 import os
 import subprocess
+import sys
 import tempfile
 
 import numpy as np
@@ -140,7 +141,7 @@ def test_adjoint_matches_finite_differences(harness, m, nsteps, obs_every):
     N = L.N
     rng = np.random.default_rng(1)
     gauges = rng.choice(N, 6, replace=False).astype(np.int32)
-    dt = 0.2
+    dt = 0.15    # below the CFL limit of both cases (0.17 for m = 9)
     S0 = L.S.copy()
     x, y = mesh.centroid_coordinates.T       # a smooth initial momentum field
     S0[:, 1] = 0.3 * np.sin(x / 15.0) * np.cos(y / 20.0) + 1e-4 * rng.random(N)
@@ -237,3 +238,68 @@ def test_rounding_noise_in_still_water_is_not_amplified(harness):
     d = rng.standard_normal(N)
     fd = (loss(dz=1e-6 * d) - loss(dz=-1e-6 * d)) / 2e-6
     assert np.dot(zbar, d) == pytest.approx(fd, rel=0.05)
+
+
+ASAN_SCRIPT = r'''
+import ctypes as C, sys
+import numpy as np
+sys.path.insert(0, sys.argv[2])
+from tests.test_adjoint_cpu import Layout, _forward, _wet_case
+lib = C.CDLL(sys.argv[1])
+dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+mesh, q, bc = _wet_case(16)
+L = Layout(mesh, q, bc)
+N = L.N
+rng = np.random.default_rng(4)
+x, y = mesh.centroid_coordinates.T
+S0 = L.S.copy()
+S0[:, 1] = 0.3 * np.sin(x / 15.0) * np.cos(y / 20.0)
+S0[:, 2] = 0.2 * np.cos(x / 25.0) * np.sin(y / 10.0)
+g = rng.choice(N, 12, replace=False).astype(np.int32)
+for dt, nsteps, expect_finite in ((0.08, 60, True), (0.6, 40, False)):
+    S, rec = _forward(lib, L, S0, dt, nsteps, g, 10)
+    assert np.all(np.isfinite(S)) == expect_finite, (dt, "finite" if expect_finite else "blow-up")
+    dL = np.ascontiguousarray(np.nan_to_num(rec))
+    nbar, zbar, lam = np.zeros(N), np.zeros(N), np.zeros(3 * N)
+    lib.harness_adjoint(*L.args(), C.c_double(dt), nsteps, dp(np.ascontiguousarray(S0)), len(g),
+                        g.ctypes.data_as(C.POINTER(C.c_int)), 10, dp(dL), dp(nbar), dp(zbar),
+                        dp(lam))
+xs = np.array([np.nan, np.inf, -np.inf, 0.0, -0.0, -1.0, 5e-324, 1e-310, 1e300, 1e-300, 1.0])
+out = np.empty_like(xs)
+lib.pow73_many(len(xs), dp(xs), dp(out))
+print("asan run ok")
+'''
+
+POW_SHIM = r'''
+#include "de_pow.h"
+extern "C" void pow73_many(long n, const double *x, double *y) {
+    for (long i = 0; i < n; i++) y[i] = de_pow_7_3(x[i]);
+}
+'''
+
+
+def test_adjoint_cell_math_in_bounds_under_asan():
+    """The adjoint's cell math (the device's, compiled for the host) under
+    AddressSanitizer: the 16x16 gradient case at dt 0.08 (below its CFL limit
+    of 0.095) stays finite, a run far above the limit (dt 0.6) blows up to
+    NaN, and both the forward and the adjoint sweep of either stay in bounds,
+    as does pow on NaN, infinite, zero, negative and subnormal depths."""
+    asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True,
+                          text=True).stdout.strip()
+    if not os.path.isabs(asan) or not os.path.exists(asan):
+        pytest.skip("no AddressSanitizer runtime")
+    tmp = tempfile.mkdtemp()
+    shim = os.path.join(tmp, "shim.cpp")
+    open(shim, "w").write(POW_SHIM)
+    so = os.path.join(tmp, "libharness_asan.so")
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-ffp-contract=off", "-shared", "-fPIC",
+                    "-fsanitize=address", "-fno-omit-frame-pointer",
+                    "-I" + os.path.join(ROOT, "anuga_core_amd", "csrc"), "-o", so,
+                    os.path.join(ROOT, "tests", "adjoint_harness.cpp"), shim], check=True)
+    script = os.path.join(tmp, "run.py")
+    open(script, "w").write(ASAN_SCRIPT)
+    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([sys.executable, script, so, ROOT], env=env, capture_output=True,
+                       text=True, timeout=600)
+    assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
+    assert r.returncode == 0 and "asan run ok" in r.stdout, r.stderr[-3000:]

@@ -305,3 +305,17 @@ def test_other_de_algorithms_vs_oracle(alg, builder, kw, ys, ft):
 def test_de2_device_timestep_matches_host_timestep():
     test_device_timestep_matches_host_timestep(_with_algorithm(S.hills_de1, "DE2"),
                                                dict(m=20, n=20), 4.0, 20.0)
+
+
+from tests import golden_cases as GC  # noqa: E402
+
+
+@pytest.mark.parametrize("case", GC.CASES, ids=[c.__name__[5:] for c in GC.CASES])
+def test_golden_multi_yield_on_device(case):
+    """The reference's multi-yield golden trajectories on the device, and the
+    device bit for bit with the oracle on the same case."""
+    print("case golden %s" % case.__name__, flush=True)
+    d = case(lambda mesh: Domain(mesh=mesh))
+    o = case(lambda mesh: OracleDomain(mesh, OracleKernels(gather=True), "DE0"))
+    assert d.dt_history == o.dt_history
+    _compare(o, d, case.__name__)

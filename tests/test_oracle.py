@@ -342,3 +342,14 @@ def test_rk3_step_is_the_reference_combination_of_euler_steps():
     expect = [(2.0 * a + 1.0 * b) / 3.0 for a, b in zip([e3.stage_c, e3.xmom_c, e3.ymom_c], q0)]
     for a, b in zip(rk3, expect):
         assert np.array_equal(a, b)
+
+
+from tests import golden_cases as GC  # noqa: E402
+
+
+@pytest.mark.parametrize("case", GC.CASES, ids=[c.__name__[5:] for c in GC.CASES])
+@pytest.mark.parametrize("name,K", GOLDEN_KS, ids=[k[0] for k in GOLDEN_KS])
+def test_golden_multi_yield(name, K, case):
+    """The reference's multi-yield golden trajectories on the oracle, both of
+    its flux loops and the reference's compiled C."""
+    case(lambda mesh: OracleDomain(mesh, K(), "DE0"))
