@@ -103,6 +103,15 @@ SIGNATURES = {
     "anuga_update_conserved_quantities": (C.c_int, [C.c_void_p, C.c_double, _ip]),
     "anuga_rk2_second_substep": (C.c_int, [C.c_void_p, C.c_double, _ip]),
     "anuga_rk3_later_substeps": (C.c_int, [C.c_void_p, C.c_double, _ip]),
+    "anuga_backup_conserved_quantities": (C.c_int, [C.c_void_p]),
+    "anuga_saxpy_conserved_quantities": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "anuga_divide_conserved_quantities": (C.c_int, [C.c_void_p, C.c_double]),
+    "anuga_update_boundary": (C.c_int, [C.c_void_p]),
+    "anuga_compute_fluxes_ext_central": (C.c_int, [C.c_void_p, C.c_double, _dp]),
+    "anuga_compute_forcing_terms": (C.c_int, [C.c_void_p]),
+    "anuga_update_centroids": (C.c_int, [C.c_void_p, C.c_double]),
+    "anuga_fix_negative_cells": (C.c_int, [C.c_void_p, _ip]),
+    "anuga_boundary_flux_integral_operator": (C.c_int, [C.c_void_p, C.c_double]),
     "anuga_evolve_steps": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(TimeControl),
                                      C.POINTER(StepState), _dp]),
     "anuga_get_boundary_flux_sums": (C.c_int, [C.c_void_p, _dp]),
@@ -114,6 +123,8 @@ SIGNATURES = {
     "anuga_tile_order": (C.c_int, [C.c_int64, _dp, _ip]),
     "anuga_tile_halos": (C.c_int, [C.c_int64, C.POINTER(C.c_int32), _ip, C.c_int64,
                                    C.POINTER(C.c_int32), C.POINTER(C.c_uint32), _ip, _ip]),
+    "anuga_tile_edges": (C.c_int, [C.c_int64, C.POINTER(C.c_int32), _ip, C.c_int64,
+                                   C.POINTER(C.c_uint32), _ip]),
     "anuga_synchronize": (C.c_int, [C.c_void_p]),
     "anuga_kernel_timing": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _ip]),
     "anuga_reset_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),

@@ -114,7 +114,24 @@ struct DeTiles {
     const int *__restrict__ halo;       // the halo triangles of each tile
     const unsigned *__restrict__ slot;  // [N] 3 x 10 bits: halo slot of edge i, 1023: in the tile
     int hcap;                           // the largest halo (LDS stride)
+    // the edges whose flux a tile evaluates once (null: none built): each
+    // in-tile interior edge from its lower-numbered triangle, and each edge
+    // to the halo from the tile's side (DE_TE_* fields)
+    const int *__restrict__ estart;     // [tiles + 1] offsets into edge
+    const unsigned *__restrict__ edge;
 };
+
+// one entry of a tile's edge list: the tile triangle t and its edge i, the
+// neighbour's position (tile triangle or halo slot) and edge j
+#define DE_TE_T(e) ((int)((e) & 1023u))
+#define DE_TE_I(e) ((int)(((e) >> 10) & 3u))
+#define DE_TE_HALO(e) (((e) >> 12) & 1u)
+#define DE_TE_N(e) ((int)(((e) >> 13) & 1023u))
+#define DE_TE_J(e) ((int)(((e) >> 23) & 3u))
+#define DE_TE_MAKE(t, i, halo, n, j) \
+    ((unsigned)(t) | ((unsigned)(i) << 10) | ((unsigned)(halo) << 12) | ((unsigned)(n) << 13) | \
+     ((unsigned)(j) << 23))
+static_assert(DE_BLOCK <= 1024, "tile edge entries hold a 10-bit triangle index");
 
 struct DeParams {
     double g, eps, mah;

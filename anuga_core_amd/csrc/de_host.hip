@@ -80,8 +80,15 @@ struct anuga_domain {
     int hcap = 0;
     // per scenario, until every mesh is set: tile offsets, halo (internal
     // ids), slots and largest halo (-1: no tiles)
-    std::vector<std::vector<int>> ts_start, ts_halo;
-    std::vector<std::vector<unsigned>> ts_slot;
+    std::vector<std::vector<int>> ts_start, ts_halo, ts_estart;
+    std::vector<std::vector<unsigned>> ts_slot, ts_edge;
+    // tile edge lists of the once-per-edge kernel (null: not built); mirror:
+    // every interior edge's normals and lengths mirror exactly (checked per
+    // scenario at upload), which that kernel assumes
+    int *testart = nullptr;
+    unsigned *tedge = nullptr;
+    bool mirror = true;
+    bool edge_once = true;     // ANUGA_HIP_EDGE_ONCE=0: the gather-every-side kernel
     std::vector<int> ts_hcap;
     bool e_stale = false;                // E does not hold the last extrapolation
     int *rw = nullptr;                   // [3][N] riverwall record per half-edge, -1: none
@@ -130,6 +137,13 @@ struct anuga_domain {
     bool use_graph = true;
 
     int substate = 0;         // 0: at a step boundary (state in A); 1: rk2 substep state in B
+    // per-operation calls (the reference's call order): the semi-implicit
+    // momentum updates (allocated on first use), whether B holds a backup of
+    // A, and the flux calls made so far (their count modulo the method's
+    // substeps is the substep, as in _compute_fluxes_central)
+    double *si[2] = {nullptr, nullptr};
+    bool backup_valid = false;
+    int64_t op_flux_calls = 0;
     unsigned long long neg_seen = 0;
     std::string err;
 
@@ -497,7 +511,7 @@ template <int SUB>
 static void launch_xflux_t(anuga_domain *d, DeQ *S, DeQ *bk, DeQ *out)
 {
     DeBoundary bc{d->bkind, d->bval, d->M, d->bsstride};
-    DeTiles tl{d->tstart, d->thalo, d->tslot, d->hcap};
+    DeTiles tl{d->tstart, d->thalo, d->tslot, d->hcap, d->testart, d->tedge};
     if (d->mixed) {
         // FP64 centroid depth + 12 FP32 record components per triangle
         const size_t lds = ((size_t)DE_BLOCK + (size_t)d->hcap) * (sizeof(double) + 12 * sizeof(float));
@@ -514,6 +528,14 @@ static void launch_xflux_t(anuga_domain *d, DeQ *S, DeQ *bk, DeQ *out)
                            d->ex[0], d->ex[1], d->ex[2], bk, out, d->ctl, d->part,
                            d->part + d->nblocks, d->part + 2 * (size_t)d->nblocks, d->dneg,
                            (double *)d->E);
+    } else if (d->mirror && d->tedge && d->edge_once) {
+        // each edge's flux once: tile and halo edge values, then the tile's
+        // centroid depths and beds
+        const size_t lds = (14 * (size_t)DE_BLOCK + 12 * (size_t)d->hcap) * sizeof(double);
+        hipLaunchKernelGGL(de_xflux_e<SUB>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
+                           dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags, d->ex[0],
+                           d->ex[1], d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
+                           d->part + 2 * (size_t)d->nblocks, d->dneg);
     } else {
         const size_t lds = (12 * (size_t)DE_BLOCK + 12 * (size_t)d->hcap) * sizeof(double);
         hipLaunchKernelGGL(de_xflux<SUB>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
@@ -545,6 +567,7 @@ static void launch_xflux(anuga_domain *d, int substep, DeQ *S, DeQ *bk = nullptr
 static void launch_flux(anuga_domain *d, int substep, DeQ *S, DeQ *bk = nullptr,
                         DeQ *out = nullptr)
 {
+    if (out == d->B) d->backup_valid = false;
     timer_begin(d, substep ? T_FLUX1 : T_FLUX0);
     if (d->mixed) launch_flux_p<float>(d, substep, S, bk, out);
     else launch_flux_p<double>(d, substep, S, bk, out);
@@ -553,6 +576,7 @@ static void launch_flux(anuga_domain *d, int substep, DeQ *S, DeQ *bk = nullptr,
 
 static void launch_update(anuga_domain *d, DeQ *S, DeQ *out)
 {
+    if (out == d->B) d->backup_valid = false;
     timer_begin(d, T_UPDATE);
     if (d->P.use_sloped_mannings && !d->mixed)
         hipLaunchKernelGGL((de_update<true>), dim3(d->nblocks), dim3(DE_BLOCK), 0,
@@ -865,6 +889,8 @@ static int domain_alloc(const anuga_params *params, int64_t S64, const int64_t *
     {
         const char *g = std::getenv("ANUGA_HIP_GRAPH");
         d->use_graph = !(g && g[0] == '0');
+        const char *e = std::getenv("ANUGA_HIP_EDGE_ONCE");
+        d->edge_once = !(e && e[0] == '0');
     }
     DCHK(hipDeviceSynchronize());
 #undef DCHK
@@ -924,6 +950,36 @@ static const int kMaxHalo = std::min(320, (65536 - 97 * DE_BLOCK - 1024) / 97);
 // The tile metadata of triangles in internal order (nb: packed neighbours of
 // one scenario starting at 0). Returns the largest halo, or -1 when a
 // neighbour index is out of range; the caller refuses halos above kMaxHalo.
+// The tile edge lists of the once-per-edge fused kernel (de_xflux_e): per
+// tile, each edge between two tile triangles from its lower-numbered side and
+// each edge to the halo from the tile's side, ordered by edge index, then
+// triangle (consecutive entries read consecutive geometry). halo slots as
+// tile_halos assigns them.
+static void tile_edges(int N, const int4 *hn, const std::vector<unsigned> &slot,
+                       std::vector<int> &estart, std::vector<unsigned> &edges)
+{
+    const int nt = (N + DE_BLOCK - 1) / DE_BLOCK;
+    estart.assign(nt + 1, 0);
+    edges.clear();
+    edges.reserve((size_t)N * 5 / 3 + 64);
+    for (int b = 0; b < nt; b++) {
+        estart[b] = (int)edges.size();
+        const int k0 = b * DE_BLOCK, k1 = std::min(N, k0 + DE_BLOCK);
+        for (int i = 0; i < 3; i++)
+            for (int k = k0; k < k1; k++) {
+                const int v = i == 0 ? hn[k].x : (i == 1 ? hn[k].y : hn[k].z);
+                if (v < 0) continue;
+                const int n = v >> 2, j = v & 3;
+                const unsigned s = (slot[k] >> (10 * i)) & 1023u;
+                if (s != 1023u)
+                    edges.push_back(DE_TE_MAKE(k - k0, i, 1, s, j));
+                else if (k < n)
+                    edges.push_back(DE_TE_MAKE(k - k0, i, 0, n - k0, j));
+            }
+    }
+    estart[nt] = (int)edges.size();
+}
+
 static int tile_halos(int N, const int4 *hn, std::vector<int> &start, std::vector<int> &halo,
                       std::vector<unsigned> &slot)
 {
@@ -979,10 +1035,13 @@ static int scenario_tiles(anuga_domain *d, int s, const std::vector<int4> &hn)
         d->ts_start.resize(d->S);
         d->ts_halo.resize(d->S);
         d->ts_slot.resize(d->S);
+        d->ts_estart.resize(d->S);
+        d->ts_edge.resize(d->S);
         d->ts_hcap.assign(d->S, -1);
     }
     const int hcap = tile_halos(N, loc.data(), d->ts_start[s], d->ts_halo[s], d->ts_slot[s]);
     if (hcap < 0) return fail(d, ANUGA_E_ARG, "tiles: neighbour out of range");
+    tile_edges(N, loc.data(), d->ts_slot[s], d->ts_estart[s], d->ts_edge[s]);
     for (int &n : d->ts_halo[s]) n += kb;
     d->ts_hcap[s] = hcap;
     return ANUGA_OK;
@@ -1000,6 +1059,10 @@ static int upload_tiles(anuga_domain *d)
     }
     if (d->tslot) HIPCHK(d, hipFree(d->tslot));
     d->tslot = nullptr;
+    if (d->testart) HIPCHK(d, hipFree(d->testart));
+    if (d->tedge) HIPCHK(d, hipFree(d->tedge));
+    d->testart = nullptr;
+    d->tedge = nullptr;
     if ((int)d->ts_hcap.size() != d->S) return ANUGA_OK;
     int hcap = 0;
     for (int s = 0; s < d->S; s++) {
@@ -1007,17 +1070,34 @@ static int upload_tiles(anuga_domain *d)
         hcap = std::max(hcap, d->ts_hcap[s]);
     }
     if (hcap > kMaxHalo) return ANUGA_OK;
-    std::vector<int> start(d->nblocks + 1, 0), halo;
-    std::vector<unsigned> slot(d->N, 0x3fffffffu);
+    std::vector<int> start(d->nblocks + 1, 0), halo, estart(d->nblocks + 1, 0);
+    std::vector<unsigned> slot(d->N, 0x3fffffffu), edge;
     int b = 0;
     for (int s = 0; s < d->S; s++) {
         const int b0 = d->kbeg[s] / DE_BLOCK, nt = (int)d->ts_start[s].size() - 1;
-        for (; b < b0; b++) start[b] = (int)halo.size();
-        for (int t = 0; t < nt; t++, b++) start[b] = (int)halo.size() + d->ts_start[s][t];
+        for (; b < b0; b++) {
+            start[b] = (int)halo.size();
+            estart[b] = (int)edge.size();
+        }
+        for (int t = 0; t < nt; t++, b++) {
+            start[b] = (int)halo.size() + d->ts_start[s][t];
+            estart[b] = (int)edge.size() + d->ts_estart[s][t];
+        }
         halo.insert(halo.end(), d->ts_halo[s].begin(), d->ts_halo[s].end());
+        edge.insert(edge.end(), d->ts_edge[s].begin(), d->ts_edge[s].end());
         std::copy(d->ts_slot[s].begin(), d->ts_slot[s].end(), slot.begin() + d->kbeg[s]);
     }
-    for (; b <= d->nblocks; b++) start[b] = (int)halo.size();
+    for (; b <= d->nblocks; b++) {
+        start[b] = (int)halo.size();
+        estart[b] = (int)edge.size();
+    }
+    HIPCHK(d, dalloc(&d->testart, (size_t)d->nblocks + 1));
+    HIPCHK(d, dalloc(&d->tedge, std::max<size_t>(edge.size(), 1)));
+    HIPCHK(d, hipMemcpy(d->testart, estart.data(), (d->nblocks + 1) * sizeof(int),
+                        hipMemcpyHostToDevice));
+    if (!edge.empty())
+        HIPCHK(d, hipMemcpy(d->tedge, edge.data(), edge.size() * sizeof(unsigned),
+                            hipMemcpyHostToDevice));
     HIPCHK(d, dalloc(&d->tstart, (size_t)d->nblocks + 1));
     HIPCHK(d, dalloc(&d->thalo, halo.size()));
     HIPCHK(d, dalloc(&d->tslot, (size_t)d->N));
@@ -1033,8 +1113,30 @@ static int upload_tiles(anuga_domain *d)
 
 // Upload the validated mesh of scenario s (its own triangle and boundary
 // numbering, as the reference's Mesh holds it).
+// Whether each interior edge's normal is exactly the negation of its
+// neighbour's and the lengths are equal (true of normals and lengths formed
+// from the two vertices, general_mesh.py:202-232), so that one side's flux
+// gives the other's by negation.
+static bool mesh_mirrors(const anuga_mesh *mesh)
+{
+    const int64_t N = mesh->number_of_elements;
+    const double *nrm = mesh->normals, *el = mesh->edgelengths;
+    for (int64_t k = 0; k < N; k++)
+        for (int i = 0; i < 3; i++) {
+            const int64_t n = mesh->neighbours[3 * k + i];
+            if (n < 0) continue;
+            const int64_t j = mesh->neighbour_edges[3 * k + i];
+            if (!(nrm[6 * n + 2 * j] == -nrm[6 * k + 2 * i]) ||
+                !(nrm[6 * n + 2 * j + 1] == -nrm[6 * k + 2 * i + 1]) ||
+                !(el[3 * n + j] == el[3 * k + i]))
+                return false;
+        }
+    return true;
+}
+
 static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
 {
+    if (!mesh_mirrors(mesh)) d->mirror = false;
     const int N = d->ntri[s], kb = d->kbeg[s], mb = d->mbeg[s];
     const int64_t *nbr = mesh->neighbours, *nbe = mesh->neighbour_edges;
     const int64_t *sur = mesh->surrogate_neighbours, *nob = mesh->number_of_boundaries;
@@ -1571,7 +1673,7 @@ void anuga_destroy(anuga_domain *d)
                     d->scen, d->tstart, d->thalo, d->tslot, d->rw, d->rwd, d->caller, d->vx, d->node_start, d->node_tri, d->iperm_dev,
                     d->out_f32, d->gtri, d->gtimes, d->gvals, d->sub_idx, d->sub_val, d->eta, d->A, d->B, d->C, d->E, d->flags,
                     d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->chunks, d->ctl, d->dsetup,
-                    d->dneg, d->hist};
+                    d->dneg, d->hist, d->si[0], d->si[1], d->testart, d->tedge};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (d->hctl) hipHostFree(d->hctl);
@@ -1601,6 +1703,9 @@ int anuga_set_params(anuga_domain *d, const anuga_params *params)
     if (params->timestep_fluxcalls != d->P.timestep_fluxcalls && d->substate != 0)
         return fail(d, ANUGA_E_STATE, "cannot change the timestepping method inside a step");
     if (std::memcmp(params, &d->P, sizeof(anuga_params)) != 0) drop_graph(d);
+    // a new method restarts the per-operation flux substep count, as the
+    // reference's static base_call does (sw_domain_orig.c:791-794)
+    if (params->timestep_fluxcalls != d->P.timestep_fluxcalls) d->op_flux_calls = 0;
     d->P = *params;
     return ANUGA_OK;
 }
@@ -2273,6 +2378,168 @@ int anuga_rk3_later_substeps(anuga_domain *d, double dt, int64_t *num_negative)
     return later_substeps(d, dt, num_negative, 3);
 }
 
+// ---------------------------------------------------------------------------
+// per-operation calls in the reference's order
+// ---------------------------------------------------------------------------
+static int op_ready(anuga_domain *d)
+{
+    int rc = single(d);
+    if (rc) return rc;
+    if (d->substate != 0)
+        return fail(d, ANUGA_E_STATE, "per-operation call inside a fused rk step");
+    rc = ready_to_step(d);
+    if (rc) return rc;
+    ensure_live(d);
+    return ANUGA_OK;
+}
+
+static int op_kend(const anuga_domain *d)
+{
+    return d->kbeg[0] + d->ntri[0];
+}
+
+static dim3 op_grid(const anuga_domain *d)
+{
+    return dim3((op_kend(d) + DE_BLOCK - 1) / DE_BLOCK);
+}
+
+int anuga_backup_conserved_quantities(anuga_domain *d)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = op_ready(d);
+    if (rc) return rc;
+    hipLaunchKernelGGL(de_backup, op_grid(d), dim3(DE_BLOCK), 0, d->stream, op_kend(d), d->A,
+                       d->B);
+    rc = launch_check(d);
+    if (rc) return rc;
+    d->backup_valid = true;
+    return ANUGA_OK;
+}
+
+int anuga_saxpy_conserved_quantities(anuga_domain *d, double a, double b)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = op_ready(d);
+    if (rc) return rc;
+    if (!d->backup_valid)
+        return fail(d, ANUGA_E_STATE, "saxpy_conserved_quantities without a backup");
+    hipLaunchKernelGGL(de_saxpy, op_grid(d), dim3(DE_BLOCK), 0, d->stream, op_kend(d), a, b,
+                       d->A, d->B);
+    return launch_check(d);
+}
+
+int anuga_divide_conserved_quantities(anuga_domain *d, double divisor)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = op_ready(d);
+    if (rc) return rc;
+    hipLaunchKernelGGL(de_divide, op_grid(d), dim3(DE_BLOCK), 0, d->stream, op_kend(d), divisor,
+                       d->A);
+    return launch_check(d);
+}
+
+int anuga_update_boundary(anuga_domain *d)
+{
+    if (!d) return ANUGA_E_ARG;
+    return op_ready(d);
+}
+
+int anuga_compute_fluxes_ext_central(anuga_domain *d, double timestep, double *flux_timestep)
+{
+    if (!d || !flux_timestep) return ANUGA_E_ARG;
+    int rc = op_ready(d);
+    if (rc) return rc;
+    if (d->e_stale)
+        return fail(d, ANUGA_E_STATE,
+                    "compute_fluxes needs the edge values of distribute_to_vertices_and_edges");
+    const int sub = (int)(d->op_flux_calls % d->P.timestep_fluxcalls);
+    launch_flux(d, 0, d->A);
+    launch_reduce(d, sub == 0 ? 0 : (sub == 1 ? 1 : 4));
+    rc = launch_check(d);
+    if (rc) return rc;
+    rc = sync_scalars(d);
+    if (rc) return rc;
+    if (d->timing) timer_flush(d);
+    d->op_flux_calls++;
+    *flux_timestep = sub == 0 ? d->hctl->flux_dt : timestep;
+    return ANUGA_OK;
+}
+
+int anuga_compute_forcing_terms(anuga_domain *d)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = op_ready(d);
+    if (rc) return rc;
+    const size_t N = (size_t)d->N;
+    for (int i = 0; i < 2; i++)
+        if (!d->si[i]) {
+            HIPCHK(d, hipStreamSynchronize(d->stream));
+            HIPCHK(d, dalloc(&d->si[i], N));
+            HIPCHK(d, hipMemsetAsync(d->si[i], 0, N * sizeof(double), d->stream));
+        }
+    const int kend = op_kend(d);
+    if (d->P.use_sloped_mannings && !d->mixed) {
+        if (d->e_stale)
+            return fail(d, ANUGA_E_STATE, "sloped friction needs the edge values of "
+                                          "distribute_to_vertices_and_edges");
+        hipLaunchKernelGGL(de_friction<true>, op_grid(d), dim3(DE_BLOCK), 0, d->stream, kend,
+                           dev_params(d->P), d->A, d->eta, d->si[0], d->si[1],
+                           (const double *)d->E, d->N, d->vx);
+    } else {
+        hipLaunchKernelGGL(de_friction<false>, op_grid(d), dim3(DE_BLOCK), 0, d->stream, kend,
+                           dev_params(d->P), d->A, d->eta, d->si[0], d->si[1], nullptr, 0,
+                           nullptr);
+    }
+    return launch_check(d);
+}
+
+int anuga_update_centroids(anuga_domain *d, double dt)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = op_ready(d);
+    if (rc) return rc;
+    const size_t N = (size_t)d->N;
+    for (int i = 0; i < 2; i++)
+        if (!d->si[i]) {   // no forcing terms yet: zero semi-implicit updates
+            HIPCHK(d, hipStreamSynchronize(d->stream));
+            HIPCHK(d, dalloc(&d->si[i], N));
+            HIPCHK(d, hipMemsetAsync(d->si[i], 0, N * sizeof(double), d->stream));
+        }
+    hipLaunchKernelGGL(de_set_dt, dim3(1), dim3(64), 0, d->stream, d->ctl, dt);
+    hipLaunchKernelGGL(de_update_centroids, op_grid(d), dim3(DE_BLOCK), 0, d->stream, op_kend(d),
+                       d->ctl, d->A, d->ex[0], d->ex[1], d->ex[2], d->si[0], d->si[1]);
+    d->e_stale = true;
+    rc = launch_check(d);
+    return rc ? rc : sync_scalars(d);
+}
+
+int anuga_fix_negative_cells(anuga_domain *d, int64_t *num_negative)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = op_ready(d);
+    if (rc) return rc;
+    hipLaunchKernelGGL(de_fix_negative, op_grid(d), dim3(DE_BLOCK), 0, d->stream, op_kend(d),
+                       d->A, d->dneg);
+    rc = launch_check(d);
+    if (rc) return rc;
+    int64_t n = 0;
+    rc = read_negative(d, &n);
+    if (rc) return rc;
+    if (num_negative) *num_negative = n;
+    return ANUGA_OK;
+}
+
+int anuga_boundary_flux_integral_operator(anuga_domain *d, double timestep)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = op_ready(d);
+    if (rc) return rc;
+    hipLaunchKernelGGL(de_set_dt, dim3(1), dim3(64), 0, d->stream, d->ctl, timestep);
+    hipLaunchKernelGGL(de_integrate, dim3(1), dim3(64), 0, d->stream, d->ctl,
+                       (int)d->P.timestep_fluxcalls);
+    return launch_check(d);
+}
+
 int anuga_evolve_steps(anuga_domain *d, int64_t max_steps, const anuga_time_control *tc,
                        anuga_step_state *st, double *dt_history)
 {
@@ -2346,6 +2613,24 @@ int anuga_tile_order(int64_t N, const double *centroids, int64_t *order)
     if (N < 0 || N >= (int64_t(1) << 29) || (N > 0 && (!centroids || !order))) return ANUGA_E_ARG;
     const std::vector<int> o = tile_order((int)N, centroids);
     for (int64_t k = 0; k < N; k++) order[k] = o[k];
+    return ANUGA_OK;
+}
+
+int anuga_tile_edges(int64_t N, const int32_t *nb, int64_t *estart, int64_t edge_cap,
+                     uint32_t *edges, int64_t *nedges)
+{
+    if (N < 0 || N >= (int64_t(1) << 29) || (N > 0 && !nb) || !estart || !nedges)
+        return ANUGA_E_ARG;
+    std::vector<int4> hn(N);
+    for (int64_t k = 0; k < N; k++) hn[k] = make_int4(nb[3 * k], nb[3 * k + 1], nb[3 * k + 2], 0);
+    std::vector<int> st, h, est;
+    std::vector<unsigned> sl, e;
+    if (tile_halos((int)N, hn.data(), st, h, sl) < 0) return ANUGA_E_ARG;
+    tile_edges((int)N, hn.data(), sl, est, e);
+    *nedges = (int64_t)e.size();
+    if ((int64_t)e.size() > edge_cap || (!edges && !e.empty())) return ANUGA_E_ARG;
+    for (size_t b = 0; b < est.size(); b++) estart[b] = est[b];
+    for (size_t q = 0; q < e.size(); q++) edges[q] = e[q];
     return ANUGA_OK;
 }
 

@@ -78,6 +78,7 @@ struct DeExtrap {
     bool constant, gw, gu, gv;
     unsigned flag;
     double mass;
+    double z;   // the centroid bed
 };
 
 template <class T>
@@ -90,6 +91,7 @@ __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &
     const int v[3] = {vk.x, vk.y, vk.z};
     const int nbound = vk.w;
     const DeQ qk = q[k];
+    x.z = qk.z;
     int sur[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) sur[i] = v[i] >= 0 ? (v[i] >> 2) : k;
@@ -419,24 +421,32 @@ __device__ __forceinline__ DeSlope de_sloped_bed(const double *__restrict__ E, i
     return s;
 }
 
+// The semi-implicit Manning coefficient S of one triangle:
+// _manning_friction_flat with eps = minimum_allowed_height (friction.py:64-72),
+// or _manning_friction_sloped (sw_domain_orig.c:2840-2888): depth over the
+// mean vertex bed and the slope factor sqrt(1 + |grad z|^2). False where the
+// reference adds nothing (n <= eps or a depth below eps).
+__device__ __forceinline__ bool de_manning_S(const DeCentroid &c, double eta, const DeParams &p,
+                                             const DeSlope &sl, double &S)
+{
+    if (!(eta > p.mah)) return false;
+    const double h = c.w - (sl.on ? sl.zmean : c.z);
+    if (!(h >= p.mah)) return false;
+    S = sl.on ? -p.g * eta * eta * sl.zs * sqrt((c.uh * c.uh + c.vh * c.vh))
+              : -p.g * eta * eta * sqrt((c.uh * c.uh + c.vh * c.vh));
+    S /= de_pow_7_3(h);
+    return true;
+}
+
 __device__ __forceinline__ DeQ de_advance(const DeCentroid c, double eta, double exw, double exuh,
                                           double exvh, double dt, const DeParams &p, int *err,
                                           unsigned long long *nneg,
                                           const DeSlope sl = DeSlope{false, 0.0, 0.0})
 {
-    // _manning_friction_flat with eps = minimum_allowed_height (friction.py:64-72),
-    // or _manning_friction_sloped (sw_domain_orig.c:2840-2888): depth over
-    // the mean vertex bed and the slope factor sqrt(1 + |grad z|^2)
-    double si_uh = 0.0, si_vh = 0.0;
-    if (eta > p.mah) {
-        const double h = c.w - (sl.on ? sl.zmean : c.z);
-        if (h >= p.mah) {
-            double S = sl.on ? -p.g * eta * eta * sl.zs * sqrt((c.uh * c.uh + c.vh * c.vh))
-                             : -p.g * eta * eta * sqrt((c.uh * c.uh + c.vh * c.vh));
-            S /= de_pow_7_3(h);
-            si_uh += S * c.uh;
-            si_vh += S * c.vh;
-        }
+    double si_uh = 0.0, si_vh = 0.0, S;
+    if (de_manning_S(c, eta, p, sl, S)) {
+        si_uh += S * c.uh;
+        si_vh += S * c.vh;
     }
     DeQ r;
     r.w = de_update_one(c.w, exw, 0.0, dt, err);
@@ -898,6 +908,215 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
     }
 }
 
+// The fused FP64 kernel with each edge's flux evaluated once (the common
+// options: no riverwalls, low_froude 0, flat friction; the mesh's normals and
+// lengths mirror exactly across every interior edge). Phase 1 is de_xflux's.
+// Phase 2a walks the tile's edge list: an edge between two tile triangles
+// from its lower-numbered side, which also forms the other side's pressure
+// term (the reference's compute-once-and-mirror loop, sw_domain_orig.c:
+// 996-1009), and an edge to the halo from the tile's side; the results
+// overwrite the LDS edge values that only this entry reads. Phase 2b gathers
+// each triangle's three edges in local order as de_xflux does: the same
+// operations on the same values, so the same bits (the other side's flux is
+// exactly the negation, its pressure flux and wave speed identical);
+// boundary edges are evaluated there, from the untouched edge values.
+template <int SUBSTEP>
+__global__ void __launch_bounds__(DE_BLOCK)
+de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restrict__ eta,
+           DeTiles tl, DeBoundary bc, unsigned char *__restrict__ flags, double *__restrict__ exw,
+           double *__restrict__ exuh, double *__restrict__ exvh, const DeQ *bk, DeQ *out,
+           DeControl *ctl, double *__restrict__ part_dt, double *__restrict__ part_bf,
+           double *__restrict__ part_mass, unsigned long long *__restrict__ nneg)
+{
+    // [4 quantities][3 edges][DE_BLOCK] tile edge values, [4][3][hcap] halo,
+    // then [2][DE_BLOCK] the tile's centroid depths and beds
+    extern __shared__ double lds[];
+    __shared__ double red_dt[DE_BLOCK / 64], red_bf[DE_BLOCK / 64], red_m[DE_BLOCK / 64];
+    const int lb = xcd_block();
+    DeControl *cb = de_ctl(ctl, m.scen, lb);
+    if (cb->done) return;
+    int *err = &cb->status;
+    const double dt = SUBSTEP ? cb->dt : 0.0;
+    const int N = m.N, t = threadIdx.x;
+    const int k0 = lb * DE_BLOCK, k = k0 + t;
+    const int kend = cb->kend;
+    const int hcap = tl.hcap;
+    double *const hl = lds + 12 * DE_BLOCK;
+    double *const hz = hl + 12 * hcap;
+    double mass = 0.0, dtmin = 1.0e100, bflux = 0.0;
+    unsigned fk = 0;
+    constexpr int Q = 3 * DE_BLOCK;   // stride between quantities of the tile
+
+    if (k < kend) {
+        DeExtrap<double> x;
+        de_extrap_cell<double>(m, p, q, k, err, x);
+        mass = x.mass;
+        fk = x.flag;
+        flags[k] = (unsigned char)fk;
+        hz[t] = x.c.h;
+        hz[DE_BLOCK + t] = x.z;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            double ew, eh, euh, evh;
+            de_edge64(x, i, ew, eh, euh, evh);
+            lds[(0 + i) * DE_BLOCK + t] = ew;
+            lds[(3 + i) * DE_BLOCK + t] = eh;
+            lds[(6 + i) * DE_BLOCK + t] = euh;
+            lds[(9 + i) * DE_BLOCK + t] = evh;
+        }
+    }
+    const int h0 = tl.start[lb], nh = tl.start[lb + 1] - h0;
+    for (int j = t; j < nh; j += DE_BLOCK) {
+        DeExtrap<double> x;
+        de_extrap_cell<double>(m, p, q, tl.halo[h0 + j], err, x);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            double ew, eh, euh, evh;
+            de_edge64(x, i, ew, eh, euh, evh);
+            hl[(0 + i) * hcap + j] = ew;
+            hl[(3 + i) * hcap + j] = eh;
+            hl[(6 + i) * hcap + j] = euh;
+            hl[(9 + i) * hcap + j] = evh;
+        }
+    }
+    __syncthreads();
+
+    // phase 2a: each listed edge once
+    {
+        const double g = p.g, eps = p.eps;
+        const int e0 = tl.estart[lb], ne = tl.estart[lb + 1] - e0;
+        for (int e = t; e < ne; e += DE_BLOCK) {
+            const unsigned en = tl.edge[e0 + e];
+            const int ta = DE_TE_T(en), ia = DE_TE_I(en), nb = DE_TE_N(en), jb = DE_TE_J(en);
+            const bool halo = DE_TE_HALO(en);
+            const int ka = k0 + ta;
+            double *const A = lds + ia * DE_BLOCK + ta;
+            double *const R = halo ? hl + jb * hcap + nb : lds + jb * DE_BLOCK + nb;
+            const int rs = halo ? 3 * hcap : Q;
+            const double ql0 = A[0], hle = A[Q], ql1 = A[2 * Q], ql2 = A[3 * Q];
+            const double qr0 = R[0], hre = R[rs], qr1 = R[2 * rs], qr2 = R[3 * rs];
+            const double n1 = m.nx[ia * N + ka], n2 = m.ny[ia * N + ka];
+            const double length = m.len[ia * N + ka];
+            const double zl = ql0 - hle, zr = qr0 - hre;
+            const double z_half = fmax(zl, zr);
+            const double h_left = fmax(hle + zl - z_half, 0.0);
+            const double h_right = fmax(hre + zr - z_half, 0.0);
+            double ef[3], speed, pflux;
+            de_flux_central<double>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1,
+                                    n2, eps, z_half, g, ef, speed, pflux);
+            ef[0] *= length;
+            ef[1] *= length;
+            ef[2] *= length;
+            const double hca = hz[ta], zca = hz[DE_BLOCK + ta];
+            const double pga = length * (-g * 0.5 * (h_left * h_left - hle * hle -
+                                                     (hle + hca) * (zl - zca)) + pflux);
+            A[0] = -ef[0];
+            A[Q] = -ef[1];
+            A[2 * Q] = -ef[2];
+            A[3 * Q] = n1 * pga;
+            R[0] = n2 * pga;
+            double tt = 0.0;
+            const bool cfl = SUBSTEP == 0 && speed > eps;
+            if (cfl) {
+                tt = 1.0 / fmax(speed, eps);
+                dtmin = fmin(dtmin, m.radius[ka] * tt);
+            }
+            if (!halo) {
+                // the other side: its own depths and bed, the same flux
+                const double hcb = hz[nb], zcb = hz[DE_BLOCK + nb];
+                const double pgb = length * (-g * 0.5 * (h_right * h_right - hre * hre -
+                                                         (hre + hcb) * (zr - zcb)) + pflux);
+                R[rs] = n1 * pgb;
+                R[2 * rs] = n2 * pgb;
+                if (cfl) dtmin = fmin(dtmin, m.radius[k0 + nb] * tt);
+            }
+        }
+    }
+    __syncthreads();
+
+    // phase 2b: each triangle's three edges in local order
+    if (k < kend) {
+        const DeCentroid c = de_centroid(q[k], fk);
+        const int4 vk = m.nb[k];
+        const int v[3] = {vk.x, vk.y, vk.z};
+        const unsigned sl = tl.slot[k];
+        double sw = 0.0, sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            double *const A = lds + i * DE_BLOCK + t;
+            if (v[i] < 0) {
+                const double hc = de_height(c.w, c.z);
+                de_edge_flux<double, SUBSTEP, false>(m, p, bc, q, flags, N, k, i, v[i], c, hc,
+                                                     c.z, A[0], A[Q], A[2 * Q], A[3 * Q], 0.0,
+                                                     0.0, 0.0, 0.0, sw, sx, sy, bflux, dtmin);
+                continue;
+            }
+            const int n = v[i] >> 2, j = v[i] & 3;
+            const unsigned s = (sl >> (10 * i)) & 1023u;
+            double a0, a1, a2, b1, b2;
+            if (s != 1023u) {           // to the halo: this side's entry
+                a0 = A[0];
+                a1 = A[Q];
+                a2 = A[2 * Q];
+                b1 = A[3 * Q];
+                b2 = hl[j * hcap + s];
+            } else if (k < n) {         // evaluated from this side
+                a0 = A[0];
+                a1 = A[Q];
+                a2 = A[2 * Q];
+                b1 = A[3 * Q];
+                b2 = lds[j * DE_BLOCK + (n - k0)];
+            } else {                    // evaluated from the neighbour's side
+                const double *O = lds + j * DE_BLOCK + (n - k0);
+                a0 = -O[0];
+                a1 = -O[Q];
+                a2 = -O[2 * Q];
+                b1 = -A[Q];
+                b2 = -A[2 * Q];
+            }
+            sw += a0;
+            sx += a1;
+            sy += a2;
+            sx -= b1;
+            sy -= b2;
+        }
+        const double inv_area = 1.0 / m.area[k];
+        sw *= inv_area;
+        sx *= inv_area;
+        sy *= inv_area;
+        if (SUBSTEP == 0) {
+            exw[k] = sw;
+            exuh[k] = sx;
+            exvh[k] = sy;
+        } else {
+            const DeQ r = de_advance(c, eta[k], sw, sx, sy, dt, p, err, nneg);
+            out[k] = de_rk_combine<SUBSTEP>(r, bk[k]);
+        }
+    }
+
+    dtmin = de_wave_min(dtmin);
+    bflux = de_wave_sum(bflux);
+    mass = de_wave_sum(mass);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red_dt[wv] = dtmin;
+        red_bf[wv] = bflux;
+        red_m[wv] = mass;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = red_dt[0], b = red_bf[0], cm = red_m[0];
+        for (int i = 1; i < DE_BLOCK / 64; i++) {
+            a = fmin(a, red_dt[i]);
+            b += red_bf[i];
+            cm += red_m[i];
+        }
+        part_dt[lb] = a;
+        part_bf[lb] = b;
+        part_mass[lb] = cm;
+    }
+}
+
 // One edge's terms of triangle k's sums on the mixed path (the body of the
 // reference's edge loop, sw_domain_orig.c:828-1004, over FP32 differences):
 // own edge record el (dw, dh, uh, vh) and centroid depth hcd; for an interior
@@ -1232,6 +1451,108 @@ de_update(const int *__restrict__ scen, DeParams p, const DeQ *q, const double *
     DeSlope sl{false, 0.0, 0.0};
     if constexpr (SLOPED) sl = de_sloped_bed(E, N, k, vx);
     out[k] = de_advance(c, eta[k], exw[k], exuh[k], exvh[k], dt, p, err, nneg, sl);
+}
+
+// ---------------------------------------------------------------------------
+// per-operation kernels: one kernel per call the reference's Domain makes in
+// evolve_one_{euler,rk2,rk3}_step (generic_domain.py:1914-2179), over the
+// centroid state q as distribute_to_vertices_and_edges left it. One
+// scenario, triangles [0, kend).
+// ---------------------------------------------------------------------------
+// backup_centroid_values of stage, xmomentum, ymomentum (quantity.c:735-749)
+__global__ void __launch_bounds__(DE_BLOCK)
+de_backup(int kend, const DeQ *__restrict__ q, DeQ *__restrict__ bk)
+{
+    const int k = blockIdx.x * DE_BLOCK + threadIdx.x;
+    if (k < kend) bk[k] = q[k];
+}
+
+// saxpy_centroid_values (quantity.c:752-769): q = a q + b backup
+__global__ void __launch_bounds__(DE_BLOCK)
+de_saxpy(int kend, double a, double b, DeQ *__restrict__ q, const DeQ *__restrict__ bk)
+{
+    const int k = blockIdx.x * DE_BLOCK + threadIdx.x;
+    if (k >= kend) return;
+    DeQ r = q[k];
+    const DeQ s = bk[k];
+    r.w = a * r.w + b * s.w;
+    r.uh = a * r.uh + b * s.uh;
+    r.vh = a * r.vh + b * s.vh;
+    q[k] = r;
+}
+
+// rk3's closing division of the conserved centroids (generic_domain.py:2167-2170)
+__global__ void __launch_bounds__(DE_BLOCK)
+de_divide(int kend, double divisor, DeQ *__restrict__ q)
+{
+    const int k = blockIdx.x * DE_BLOCK + threadIdx.x;
+    if (k >= kend) return;
+    DeQ r = q[k];
+    r.w = r.w / divisor;
+    r.uh = r.uh / divisor;
+    r.vh = r.vh / divisor;
+    q[k] = r;
+}
+
+// compute_forcing_terms: Manning friction added to the momenta's
+// semi-implicit updates (manning_friction_implicit, friction.py:22-76). The
+// sloped form reads the vertex beds of the last extrapolation (E).
+template <bool SLOPED>
+__global__ void __launch_bounds__(DE_BLOCK)
+de_friction(int kend, DeParams p, const DeQ *__restrict__ q, const double *__restrict__ eta,
+            double *__restrict__ si_uh, double *__restrict__ si_vh,
+            const double *__restrict__ E = nullptr, int N = 0,
+            const double *__restrict__ vx = nullptr)
+{
+    const int k = blockIdx.x * DE_BLOCK + threadIdx.x;
+    if (k >= kend) return;
+    const DeQ r = q[k];
+    const DeCentroid c{r.w, r.uh, r.vh, r.z};
+    DeSlope sl{false, 0.0, 0.0};
+    if constexpr (SLOPED) sl = de_sloped_bed(E, N, k, vx);
+    double S;
+    if (de_manning_S(c, eta[k], p, sl, S)) {
+        si_uh[k] += S * c.uh;
+        si_vh[k] += S * c.vh;
+    }
+}
+
+// Quantity.update of stage, xmomentum and ymomentum (quantity.c:772-822): the
+// explicit and semi-implicit updates with timestep dt; the semi-implicit
+// updates are reset. Stage has no semi-implicit term.
+__global__ void __launch_bounds__(DE_BLOCK)
+de_update_centroids(int kend, DeControl *ctl, DeQ *__restrict__ q,
+                    const double *__restrict__ exw, const double *__restrict__ exuh,
+                    const double *__restrict__ exvh, double *__restrict__ si_uh,
+                    double *__restrict__ si_vh)
+{
+    const int k = blockIdx.x * DE_BLOCK + threadIdx.x;
+    if (k >= kend) return;
+    const double dt = ctl->dt;
+    int *err = &ctl->status;
+    DeQ r = q[k];
+    r.w = de_update_one(r.w, exw[k], 0.0, dt, err);
+    r.uh = de_update_one(r.uh, exuh[k], si_uh[k], dt, err);
+    r.vh = de_update_one(r.vh, exvh[k], si_vh[k], dt, err);
+    q[k] = r;
+    si_uh[k] = 0.0;
+    si_vh[k] = 0.0;
+}
+
+// _orig_fix_negative_cells (sw_domain_orig.c:2890-2910); every triangle is full
+__global__ void __launch_bounds__(DE_BLOCK)
+de_fix_negative(int kend, DeQ *__restrict__ q, unsigned long long *__restrict__ nneg)
+{
+    const int k = blockIdx.x * DE_BLOCK + threadIdx.x;
+    if (k >= kend) return;
+    DeQ r = q[k];
+    if (r.w - r.z < 0.0) {
+        atomicAdd(nneg, 1ull);
+        r.w = r.z;
+        r.uh = 0.0;
+        r.vh = 0.0;
+        q[k] = r;
+    }
 }
 
 // Fixed-order reduction of the per-block partials into the control blocks,

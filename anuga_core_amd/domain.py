@@ -166,6 +166,7 @@ class Domain:
         self.evolve_min_timestep = MIN_TIMESTEP
         self.max_smallsteps = MAX_SMALLSTEPS
         self.fixed_flux_timestep = None
+        self.max_flux_update_frequency = 1
         self.centroid_transmissive_bc = False
         self.boundary_map = None
         self.starttime = float(starttime)
@@ -182,6 +183,9 @@ class Domain:
         # host synchronisation each; False: one library call per substep with
         # update_timestep on the host (the reference's sequencing, for checks)
         self.device_timestep = True
+        # "fused": each step is a few fused kernels; "per_operation": each step
+        # makes the reference's own calls in its order, one library call each
+        self.step_mode = "fused"
         self._distributed = False
         self._last_negative_total = 0
         self.mass_error = 0.0
@@ -345,6 +349,25 @@ class Domain:
 
     def get_flow_algorithm(self):
         return self.flow_algorithm
+
+    def set_local_extrapolation_and_flux_updating(self, nlevels=8):
+        """shallow_water_domain.py:1193-1215 with the reference's checks and
+        exceptions. Local flux updating itself (max_flux_update_frequency > 1,
+        sw_domain_orig.c:509-669) is not built on the device: it is refused
+        here, and evolve refuses a max_flux_update_frequency set directly,
+        rather than evolving with every flux updated every step."""
+        mfuf = 2 ** nlevels
+        if mfuf != 1:
+            if self.timestepping_method != "euler":
+                raise Exception("Local extrapolation and flux updating only supported with "
+                                "euler timestepping")
+            if not self.get_using_discontinuous_elevation():
+                raise Exception("Local extrapolation and flux updating only supported for "
+                                "discontinuous flow algorithms")
+            raise NotImplementedError("local extrapolation and flux updating "
+                                      "(max_flux_update_frequency %d) is not implemented on "
+                                      "the device" % mfuf)
+        self.max_flux_update_frequency = mfuf
 
     def set_timestepping_method(self, method):
         methods = {"euler": 1, "rk2": 2, "rk3": 3, 1: 1, 2: 2, 3: 3}
@@ -700,7 +723,113 @@ class Domain:
         self._distributed = True
 
     def update_boundary(self):
-        """Boundary states are evaluated inside the flux kernel; nothing to do."""
+        """Boundary states are evaluated inside the flux kernel from the edge
+        values. In per-operation steps the time-dependent boundary values of
+        the current time go to the slot that compute_fluxes reads
+        (generic_domain.py:2288-2306)."""
+        if self.step_mode == "per_operation":
+            self._update_boundary_slot(0, self.get_time())
+            self._call("anuga_update_boundary")
+
+    # -- the reference's per-operation calls --------------------------------------------
+    def set_step_mode(self, mode="fused"):
+        """'fused' (default): a step is a few fused kernels, and with nothing
+        time-dependent on the host, runs of steps keep update_timestep on the
+        device. 'per_operation': every step makes the calls of
+        evolve_one_{euler,rk2,rk3}_step (generic_domain.py:1914-2179) in the
+        reference's order, one library call each (backup, distribute,
+        update_boundary, compute_fluxes, compute_forcing_terms,
+        update_timestep, update_conserved_quantities, saxpy). Both give the
+        same bits."""
+        if mode not in ("fused", "per_operation"):
+            raise ValueError("step mode must be 'fused' or 'per_operation'")
+        self.step_mode = mode
+
+    def backup_conserved_quantities(self):
+        """generic_domain.py:2187-2191"""
+        self._call("anuga_backup_conserved_quantities")
+
+    def saxpy_conserved_quantities(self, a, b):
+        """generic_domain.py:2193-2199"""
+        self._call("anuga_saxpy_conserved_quantities", C.c_double(a), C.c_double(b))
+
+    def compute_fluxes(self):
+        """shallow_water_domain.py:1828-1879: compute_fluxes_ext_central over
+        the edge values of the last distribute_to_vertices_and_edges."""
+        ft = C.c_double()
+        self._call("anuga_compute_fluxes_ext_central", C.c_double(self.evolve_max_timestep),
+                   C.byref(ft))
+        self.flux_timestep = ft.value
+
+    def compute_forcing_terms(self):
+        """generic_domain.py:2417-2427 with the DE forcing term, Manning
+        friction (friction.py:22-76)."""
+        self._call("anuga_compute_forcing_terms")
+
+    def update_conserved_quantities(self):
+        """shallow_water_domain.py:2089-2163: Stage/Xmom/Ymom.update, then
+        fix_negative_cells with the reference's warning."""
+        self._call("anuga_update_centroids", C.c_double(self.timestep))
+        n = C.c_int64()
+        self._call("anuga_fix_negative_cells", C.byref(n))
+        if n.value > 0:
+            warnings.warn("Negative cells being set to zero depth, possible loss of "
+                          "conservation. \nConsider using "
+                          "domain.report_water_volume_statistics() to check the extent "
+                          "of the problem")
+
+    def _per_operation_euler(self, yieldstep, finaltime):
+        """generic_domain.py:1914-1972"""
+        self.distribute_to_vertices_and_edges()
+        self.update_boundary()
+        self.compute_fluxes()
+        self.compute_forcing_terms()
+        self.update_timestep(yieldstep, finaltime)
+        self.update_conserved_quantities()
+
+    def _per_operation_rk2(self, yieldstep, finaltime):
+        """generic_domain.py:1974-2051"""
+        self.backup_conserved_quantities()
+        self.distribute_to_vertices_and_edges()
+        self.update_boundary()
+        self.compute_fluxes()
+        self.compute_forcing_terms()
+        self.update_timestep(yieldstep, finaltime)
+        self.update_conserved_quantities()
+        self.set_relative_time(self.get_relative_time() + self.timestep)
+        self.distribute_to_vertices_and_edges()
+        self.update_boundary()
+        self.compute_fluxes()
+        self.compute_forcing_terms()
+        self.update_conserved_quantities()
+        self.saxpy_conserved_quantities(0.5, 0.5)
+
+    def _per_operation_rk3(self, yieldstep, finaltime):
+        """generic_domain.py:2053-2179"""
+        self.backup_conserved_quantities()
+        initial_time = self.get_relative_time()
+        self.distribute_to_vertices_and_edges()
+        self.update_boundary()
+        self.compute_fluxes()
+        self.compute_forcing_terms()
+        self.update_timestep(yieldstep, finaltime)
+        self.update_conserved_quantities()
+        self.set_relative_time(self.relative_time + self.timestep)
+        self.distribute_to_vertices_and_edges()
+        self.update_boundary()
+        self.compute_fluxes()
+        self.compute_forcing_terms()
+        self.update_conserved_quantities()
+        self.saxpy_conserved_quantities(0.25, 0.75)
+        self.set_relative_time(initial_time + self.timestep * 0.5)
+        self.distribute_to_vertices_and_edges()
+        self.update_boundary()
+        self.compute_fluxes()
+        self.compute_forcing_terms()
+        self.update_conserved_quantities()
+        self.saxpy_conserved_quantities(2.0, 1.0)
+        self._call("anuga_divide_conserved_quantities", C.c_double(3.0))
+        self.set_relative_time(initial_time + self.timestep)
 
     def update_timestep(self, yieldstep, finaltime):
         """generic_domain.py:2349-2415"""
@@ -746,6 +875,8 @@ class Domain:
 
     def evolve_one_euler_step(self, yieldstep, finaltime):
         """generic_domain.py:1914-1972"""
+        if self.step_mode == "per_operation":
+            return self._per_operation_euler(yieldstep, finaltime)
         self._update_boundary_slot(0, self.get_time())
         self._compute_fluxes()
         self.update_timestep(yieldstep, finaltime)
@@ -753,6 +884,8 @@ class Domain:
 
     def evolve_one_rk2_step(self, yieldstep, finaltime):
         """generic_domain.py:1974-2051; backup is the untouched step state."""
+        if self.step_mode == "per_operation":
+            return self._per_operation_rk2(yieldstep, finaltime)
         self._update_boundary_slot(0, self.get_time())
         self._compute_fluxes()
         self.update_timestep(yieldstep, finaltime)
@@ -763,6 +896,8 @@ class Domain:
 
     def evolve_one_rk3_step(self, yieldstep, finaltime):
         """generic_domain.py:2053-2179; the backup is the untouched step state."""
+        if self.step_mode == "per_operation":
+            return self._per_operation_rk3(yieldstep, finaltime)
         initial = self.relative_time
         self._update_boundary_slot(0, self.get_time())
         self._compute_fluxes()
@@ -779,6 +914,10 @@ class Domain:
     def get_relative_time(self):
         return self.relative_time
 
+    def set_relative_time(self, time=0.0):
+        """generic_domain.py set_relative_time: time since starttime."""
+        self.relative_time = float(time)
+
     def set_time(self, time=0.0):
         self.relative_time = float(time) - self.starttime
 
@@ -794,6 +933,10 @@ class Domain:
                                  % self.get_boundary_tags())
         if not (0 <= self.beta_w <= 2.0):
             raise AssertionError("Attribute self.beta_w must be in the interval [0, 2]")
+        if self.max_flux_update_frequency != 1:
+            raise NotImplementedError("local extrapolation and flux updating "
+                                      "(max_flux_update_frequency %s) is not implemented on "
+                                      "the device" % self.max_flux_update_frequency)
         if outputstep is None:
             outputstep = yieldstep
         if yieldstep is None:
@@ -858,7 +1001,7 @@ class Domain:
     def _device_stepping(self):
         """Runs of steps on the device unless something needs the host every
         step (time-dependent boundary values or operator rates)."""
-        return (self.device_timestep and not self._time_bc and
+        return (self.device_timestep and self.step_mode == "fused" and not self._time_bc and
                 not any(op.time_dependent for op in self.fractional_step_operators))
 
     def set_fractional_step_operator(self, operator):
@@ -888,6 +1031,11 @@ class Domain:
             self.evolve_one_rk2_step(yieldstep, self.relative_finaltime)
         else:
             self.evolve_one_rk3_step(yieldstep, self.relative_finaltime)
+        if self.step_mode == "per_operation":
+            # the boundary flux integral operator, the first fractional step
+            # (shallow_water_domain.py:386-387; the fused steps integrate in
+            # their update kernels)
+            self._call("anuga_boundary_flux_integral_operator", C.c_double(self.timestep))
         if self.fractional_step_operators:
             # apply_fractional_steps at the step's time as the reference has
             # it then (start for euler, start + dt for rk2 and rk3), in order

@@ -267,6 +267,67 @@ int anuga_rk2_second_substep(anuga_domain *d, double dt, int64_t *num_negative);
  * (generic_domain.py:2053-2179; quantity.c:752-769). */
 int anuga_rk3_later_substeps(anuga_domain *d, double dt, int64_t *num_negative);
 
+/* ---- Per-operation calls, in the reference's own call order ------------------
+ * One entry point per call that evolve_one_euler_step / evolve_one_rk2_step /
+ * evolve_one_rk3_step make (generic_domain.py:1914-2179), for a binding that
+ * keeps the reference's orchestration and replaces only the C extension calls:
+ *   backup_conserved_quantities; distribute_to_vertices_and_edges (above);
+ *   update_boundary; compute_fluxes; compute_forcing_terms; (update_timestep
+ *   on the host); update_conserved_quantities = anuga_update_centroids +
+ *   anuga_fix_negative_cells; saxpy_conserved_quantities; rk3's division.
+ * The state stays in HBM. They are refused (ANUGA_E_STATE) between
+ * anuga_update_conserved_quantities and the fused later substeps. */
+
+/* backup_conserved_quantities: copies stage, xmomentum, ymomentum
+ * (generic_domain.py:2187-2191 -> _backup_centroid_values, quantity.c:735-749). */
+int anuga_backup_conserved_quantities(anuga_domain *d);
+
+/* saxpy_conserved_quantities(a, b): q = a q + b backup for the three conserved
+ * quantities (generic_domain.py:2193-2199 -> _saxpy_centroid_values,
+ * quantity.c:752-769). ANUGA_E_STATE without an earlier backup. */
+int anuga_saxpy_conserved_quantities(anuga_domain *d, double a, double b);
+
+/* Q.centroid_values / divisor for the three conserved quantities: rk3's
+ * closing division (generic_domain.py:2167-2170). */
+int anuga_divide_conserved_quantities(anuga_domain *d, double divisor);
+
+/* update_boundary (generic_domain.py:2288-2306): boundary states are formed
+ * inside the flux kernel from the edge values, so this only checks that the
+ * handle can step. Time-dependent boundary values are uploaded to slot 0 with
+ * anuga_set_boundary_values before each anuga_compute_fluxes_ext_central. */
+int anuga_update_boundary(anuga_domain *d);
+
+/* compute_fluxes_ext_central(domain, timestep) (sw_domain_orig_ext.pyx:419-432
+ * -> _compute_fluxes_central, sw_domain_orig.c:765-1150) over the edge values
+ * of the last distribute_to_vertices_and_edges: explicit updates, the
+ * substep's boundary_flux_sum, and flux_timestep = the CFL timestep on the
+ * first substep of a step, else `timestep` unchanged. The substep is the count
+ * of calls modulo the method's substeps, as in the reference (restarted when
+ * the method changes). ANUGA_E_STATE when the edge values are stale. */
+int anuga_compute_fluxes_ext_central(anuga_domain *d, double timestep, double *flux_timestep);
+
+/* compute_forcing_terms: Manning friction (flat, or sloped with
+ * use_sloped_mannings) added to the momenta's semi-implicit updates
+ * (friction.py:22-76 -> _manning_friction_flat / _sloped,
+ * sw_domain_orig.c:2805-2888). */
+int anuga_compute_forcing_terms(anuga_domain *d);
+
+/* Stage.update(dt), Xmom.update(dt), Ymom.update(dt)
+ * (shallow_water_domain.py:2114-2118 -> _update, quantity.c:772-822): explicit
+ * and semi-implicit updates; the semi-implicit updates are reset.
+ * ANUGA_E_SEMI_IMPLICIT for a denominator <= 0. */
+int anuga_update_centroids(anuga_domain *d, double dt);
+
+/* fix_negative_cells (sw_domain_orig_ext.pyx:538-553 -> _orig_fix_negative_cells,
+ * sw_domain_orig.c:2890-2910); num_negative (may be NULL) receives the count. */
+int anuga_fix_negative_cells(anuga_domain *d, int64_t *num_negative);
+
+/* boundary_flux_integral_operator.__call__ after a step of `timestep`
+ * (anuga/operators/boundary_flux_integral_operator.py:39-49), from the
+ * boundary_flux_sum of the step's substeps; read with
+ * anuga_get_boundary_flux_integral. */
+int anuga_boundary_flux_integral_operator(anuga_domain *d, double timestep);
+
 /* Timestep control of Generic_Domain.update_timestep (generic_domain.py:2349-2415). */
 typedef struct anuga_time_control {
     double CFL;
@@ -352,6 +413,18 @@ int anuga_tile_order(int64_t N, const double *centroids, int64_t *order);
  * Host-only; no counterpart in the reference. */
 int anuga_tile_halos(int64_t N, const int32_t *nb, int64_t *start, int64_t halo_cap,
                      int32_t *halo, uint32_t *slot, int64_t *nhalo, int64_t *hcap);
+
+/* The edge lists of the fused kernel that evaluates each edge's flux once,
+ * for the same tiles and packed neighbours as anuga_tile_halos: per tile
+ * (estart: tiles + 1 offsets), each edge between two tile triangles from its
+ * lower-numbered side and each edge to the halo from the tile's side, ordered
+ * by edge index then triangle; an entry packs the triangle's offset in the
+ * tile (bits 0-9), its edge (10-11), a halo flag (12), the neighbour's tile
+ * offset or halo slot (13-22) and the neighbour's edge (23-24). At most
+ * edge_cap entries; *nedges receives the count.
+ * Host-only; no counterpart in the reference. */
+int anuga_tile_edges(int64_t N, const int32_t *nb, int64_t *estart, int64_t edge_cap,
+                     uint32_t *edges, int64_t *nedges);
 
 /* Wait for all work on the handle's stream. */
 int anuga_synchronize(anuga_domain *d);

@@ -176,6 +176,23 @@ def test_adjoint_matches_finite_differences(harness, m, nsteps, obs_every):
         _agrees(ad, lambda e: loss(dS=e * d))
 
 
+def smooth_direction(mesh, rng, modes=6):
+    """A random direction that is smooth on the mesh's scale: a sum of a few
+    random low-order sine modes over the domain's extent. A white-noise bed
+    perturbation roughens the bed from one triangle to the next, so every
+    small step switches some limiter branch and the central differences
+    wander by 1e-5..1e-4 (relative) down to steps of 1e-9, where rounding
+    takes over; along smooth directions they settle to 1e-6."""
+    x, y = mesh.centroid_coordinates.T
+    lx, ly = x.max() - x.min(), y.max() - y.min()
+    d = np.zeros(len(x))
+    for _ in range(modes):
+        kx, ky = rng.integers(1, 4, 2)
+        px, py = rng.uniform(0.0, 2.0 * np.pi, 2)
+        d += rng.standard_normal() * np.sin(kx * np.pi * x / lx + px) * np.sin(ky * np.pi * y / ly + py)
+    return d
+
+
 def _agrees(ad, f):
     """The central difference at one of the steps 1e-6 .. 1e-9 agrees with ad."""
     diffs = []

@@ -123,3 +123,51 @@ def test_ensemble_volume_and_independence():
     alone.evolve_steps(100)
     assert np.array_equal(alone.get_quantity(0, "stage"), big.get_quantity(1, "stage"))
     assert alone.get_time(0) == big.get_time(1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_reduced_batched_workload_vs_oracle(precision):
+    """The batched workload's generator at reduced size: 64 scenarios, k
+    log-uniform in [8, 40] (seed 1; 256 to 6400 triangles each), DE1, one
+    handle, 120 steps with per-scenario timesteps. A sample of 12 scenarios
+    (the smallest, the largest and 10 drawn at random) is run on the oracle:
+    FP64 gives the oracle's bits and its time (the sum of the scenario's own
+    timesteps); mixed stays within the stated error (stage L1 over the water
+    column <= 1e-5) and its time within 1e-4 relative."""
+    import bench
+    print("ensemble reduced batched workload vs oracle", precision, flush=True)
+    from oracle.kernels import OracleKernels
+    from oracle.oracle_domain import OracleDomain
+    spec = bench.build_ensemble(64, seed=1, kmin=8, kmax=40)
+    scen = [bench.ensemble_scenario(k, prm) for k, prm in spec]
+    ks = np.array([k for k, _ in spec])
+    assert ks.min() < 12 and ks.max() > 30
+    ens = Ensemble([m for m, _, _ in scen], flow_algorithm="DE1", precision=precision)
+    for s, (mesh, q, bc) in enumerate(scen):
+        ens.set_scenario(s, mesh, q, bc)
+    nsteps = 120
+    ens.evolve_steps(nsteps)
+    assert list(ens.number_of_steps) == [nsteps] * len(scen)
+    rng = np.random.default_rng(11)
+    sample = sorted({int(np.argmin(ks)), int(np.argmax(ks))} |
+                    set(rng.choice(len(scen), 10, replace=False).tolist()))
+    for s in sample:
+        mesh, q, bc = scen[s]
+        o = S.setup(OracleDomain(mesh, OracleKernels(gather=True)), "DE1", q, bc)
+        for _ in o.evolve(yieldstep=1e300, finaltime=1e300, max_steps=nsteps):
+            pass
+        assert len(o.dt_history) == nsteps
+        t_oracle = o.get_time()
+        if precision == "fp64":
+            assert ens.get_time(s) == t_oracle, (s, ens.get_time(s), t_oracle)
+            for name, oname in (("stage", "stage_c"), ("xmomentum", "xmom_c"),
+                                ("ymomentum", "ymom_c")):
+                assert np.array_equal(ens.get_quantity(s, name), getattr(o, oname)), (s, name)
+        else:
+            w = ens.get_quantity(s, "stage")
+            h = np.maximum(o.stage_c - o.elev_c, 0.0)
+            l1 = np.sum(np.abs(w - o.stage_c) * mesh.areas) / max(np.sum(h * mesh.areas), 1e-30)
+            print("scenario %d k=%d stage L1/water column %.2e" % (s, ks[s], l1), flush=True)
+            assert l1 <= 1e-5, (s, l1)
+            assert abs(ens.get_time(s) - t_oracle) <= 1e-4 * t_oracle
+    ens.close()

@@ -353,3 +353,40 @@ def test_golden_multi_yield(name, K, case):
     """The reference's multi-yield golden trajectories on the oracle, both of
     its flux loops and the reference's compiled C."""
     case(lambda mesh: OracleDomain(mesh, K(), "DE0"))
+
+
+def _uniform_flow(K, kind, alg="DE1", m=8):
+    """Uniform 1 m deep water moving at 0.5 m/s east over a flat bed, every
+    boundary of one kind, 20 steps of alg."""
+    from anuga_core_amd.mesh import rectangular_cross
+    mesh = Mesh(*rectangular_cross(m, m, 40.0, 40.0))
+    N = mesh.number_of_triangles
+    q = dict(elevation=np.zeros(N), stage=np.ones(N), xmomentum=np.full(N, 0.5),
+             ymomentum=np.zeros(N), friction=np.zeros(N))
+    d = S.setup(OracleDomain(mesh, K()), alg, q, {t: kind for t in ("left", "right", "top",
+                                                                   "bottom")})
+    for _ in d.evolve(yieldstep=1e300, finaltime=1e300, max_steps=20):
+        pass
+    return d
+
+
+@pytest.mark.parametrize("alg", ["DE0", "DE1"])
+@pytest.mark.parametrize("name,K", KSETS, ids=[k[0] for k in KSETS])
+def test_transmissive_boundary_keeps_uniform_flow(name, K, alg):
+    """Known answer of the transmissive boundary
+    (generic_boundary_conditions.py:173-195, the boundary state is the inside
+    edge state): a uniform flow leaves and enters through it undisturbed, so it
+    stays uniform to rounding, with the CFL timestep of the uniform state
+    (0.5 + sqrt(9.8 * 1)) and no net boundary inflow. Reflective walls on the
+    same flow make waves, which shows the check is sensitive to the kind."""
+    d = _uniform_flow(K, "transmissive", alg)
+    assert np.max(np.abs(d.stage_c - 1.0)) < 1e-12
+    assert np.max(np.abs(d.xmom_c - 0.5)) < 1e-12
+    assert np.max(np.abs(d.ymom_c)) < 1e-12
+    cfl = {"DE0": 0.9, "DE1": 1.0}[alg]
+    radius = np.min(d.mesh.radii)
+    assert d.dt_history[-1] == pytest.approx(cfl * radius / (0.5 + math.sqrt(G * 1.0)),
+                                             rel=1e-12)
+    assert abs(d.boundary_flux_integral) < 1e-9
+    r = _uniform_flow(K, "reflective", alg)
+    assert np.max(np.abs(r.xmom_c - 0.5)) > 1e-2

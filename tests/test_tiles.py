@@ -150,3 +150,130 @@ def test_tile_halos_rejects_bad_neighbours():
                               halo.ctypes.data_as(C.POINTER(C.c_int32)),
                               slot.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(a), C.byref(b))
     assert rc != 0
+
+
+def tile_edge_lists(nb):
+    lib = _native.lib()
+    N = len(nb)
+    nt = (N + BLOCK - 1) // BLOCK
+    estart = np.empty(nt + 1, np.int64)
+    cap = 3 * N + 1
+    edges = np.empty(cap, np.uint32)
+    ne = C.c_int64()
+    rc = lib.anuga_tile_edges(N, nb.ctypes.data_as(C.POINTER(C.c_int32)), _i64(estart), cap,
+                              edges.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(ne))
+    assert rc == 0
+    return estart, edges[:ne.value]
+
+
+def replay_edge_once(nb, start, halo, slot, hcap, estart, edges):
+    """The once-per-edge kernel's LDS dataflow (de_xflux_e) replayed with
+    tokens: phase 1 writes each edge value as (k, edge, quantity); phase 2a
+    must read, for each listed edge, its own and its neighbour's values, and
+    its writes must not touch a slot another entry reads or writes; phase 2b
+    must read, for every half-edge, the terms its own edge produced, through
+    the roles (this side, other side, halo, boundary) the kernel decodes.
+    Returns the number of entries."""
+    N = len(nb)
+    hs = max((hcap + 31) // 32 * 32, 32)
+    Q = 3 * BLOCK
+    size = 14 * BLOCK + 12 * hs
+    nt = len(start) - 1
+    listed = set()
+    for b in range(nt):
+        k0, k1 = b * BLOCK, min(N, (b + 1) * BLOCK)
+        lds = {}
+        for k in range(k0, k1):
+            for i in range(3):
+                for q in range(4):
+                    lds[(3 * q + i) * BLOCK + (k - k0)] = ("val", k, i, q)
+        hb = halo[start[b]:start[b + 1]]
+        for s, h in enumerate(hb):
+            for i in range(3):
+                for q in range(4):
+                    lds[12 * BLOCK + (3 * q + i) * hs + s] = ("val", int(h), i, q)
+        touched = set()
+        for en in edges[estart[b]:estart[b + 1]]:
+            en = int(en)
+            t, i, isha, nn, j = en & 1023, (en >> 10) & 3, (en >> 12) & 1, (en >> 13) & 1023, \
+                (en >> 23) & 3
+            k = k0 + t
+            assert k < k1
+            v = int(nb[k, i])
+            assert v >= 0 and (v & 3) == j
+            n = v >> 2
+            A = i * BLOCK + t
+            if isha:
+                assert nn < len(hb) and int(hb[nn]) == n
+                R, rs = 12 * BLOCK + j * hs + nn, 3 * hs
+            else:
+                assert k0 <= n < k1 and n - k0 == nn and k < n
+                R, rs = j * BLOCK + nn, Q
+            addrs = [A + q * Q for q in range(4)] + [R + q * rs for q in range(4)]
+            for a in addrs:
+                assert 0 <= a < 12 * BLOCK + 12 * hs
+                assert a not in touched, "two entries share an LDS slot"
+            for q in range(4):
+                assert lds[A + q * Q] == ("val", k, i, q)
+                assert lds[R + q * rs] == ("val", n, j, q)
+            touched.update(addrs)
+            assert (k, i) not in listed
+            listed.add((k, i))
+            for q, name in enumerate(("a0", "a1", "a2", "b1")):
+                lds[A + q * Q] = (name, k, i)
+            lds[R] = ("b2", k, i)
+            if not isha:
+                lds[R + rs] = ("c1", k, i)
+                lds[R + 2 * rs] = ("c2", k, i)
+        # phase 2b
+        for k in range(k0, k1):
+            t = k - k0
+            for i in range(3):
+                v = int(nb[k, i])
+                A = i * BLOCK + t
+                if v < 0:
+                    assert all(lds[A + q * Q] == ("val", k, i, q) for q in range(4))
+                    continue
+                n, j = v >> 2, v & 3
+                s = (int(slot[k]) >> (10 * i)) & 1023
+                if s != 1023:
+                    got = [lds[A + q * Q] for q in range(4)] + [lds[12 * BLOCK + j * hs + s]]
+                    want = [(x, k, i) for x in ("a0", "a1", "a2", "b1", "b2")]
+                elif k < n:
+                    got = [lds[A + q * Q] for q in range(4)] + [lds[j * BLOCK + (n - k0)]]
+                    want = [(x, k, i) for x in ("a0", "a1", "a2", "b1", "b2")]
+                else:
+                    O = j * BLOCK + (n - k0)
+                    got = [lds[O + q * Q] for q in range(3)] + [lds[A + Q], lds[A + 2 * Q]]
+                    want = [(x, n, j) for x in ("a0", "a1", "a2", "c1", "c2")]
+                assert got == want, (k, i, got, want)
+    # every half-edge with an interior neighbour is listed once from one side
+    for k in range(N):
+        for i in range(3):
+            v = int(nb[k, i])
+            if v < 0:
+                continue
+            n = v >> 2
+            same_tile = n // BLOCK == k // BLOCK
+            assert ((k, i) in listed) == (not same_tile or k < n)
+    return len(edges)
+
+
+@pytest.mark.parametrize("m,n", [(40, 40), (37, 53), (1, 1), (8, 8)])
+def test_edge_once_lds_dataflow(m, n):
+    mesh = Mesh(*rectangular_cross(m, n, 1000.0, 700.0))
+    nb, start, halo, slot, hcap = tile_layout(mesh)
+    estart, edges = tile_edge_lists(nb)
+    ne = replay_edge_once(nb, start, halo, slot, hcap, estart, edges)
+    N = len(nb)
+    if m == 40:
+        # about 1.6 evaluated edges per triangle instead of 3
+        assert 1.4 < ne / N < 1.8
+
+
+def test_edge_once_dataflow_on_unstructured_mesh():
+    from anuga_core_amd.mesh import graded_square
+    mesh = Mesh(*graded_square(600.0, 40.0, 300.0, (150.0, 150.0, 450.0, 450.0), seed=1))
+    nb, start, halo, slot, hcap = tile_layout(mesh)
+    estart, edges = tile_edge_lists(nb)
+    replay_edge_once(nb, start, halo, slot, hcap, estart, edges)
