@@ -118,6 +118,8 @@ SIGNATURES = {
     "anuga_get_boundary_flux_integral": (C.c_int, [C.c_void_p, _dp]),
     "anuga_get_negative_cells": (C.c_int, [C.c_void_p, _ip]),
     "anuga_get_water_volume": (C.c_int, [C.c_void_p, _dp]),
+    "anuga_water_volume_noalloc": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p]),
     "anuga_rcm_order": (C.c_int, [C.c_int64, _ip, _ip]),
     "anuga_build_config": (C.c_int, [_ip, _ip]),
     "anuga_tile_order": (C.c_int, [C.c_int64, _dp, _ip]),

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "anuga_hip.h"
@@ -26,6 +27,23 @@
 namespace {
 
 thread_local std::string g_create_error;
+
+// ---------------------------------------------------------------------------
+// the device scratch of anuga_water_volume_noalloc: a control block, the
+// reduction's chunk results and one partial per block of at most
+// kNoAllocMaxN triangles. One copy per GPU, alive until the process ends;
+// host code uses the address hipGetSymbolAddress gives on each device, never
+// the array's name.
+// ---------------------------------------------------------------------------
+static const int64_t kNoAllocMaxN = int64_t(1) << 27;
+static const int kNoAllocCtl = 32;                        // doubles for the DeControl
+static const int kNoAllocChunks = 3 * DE_RCHUNKS;         // the first stage's results
+static const int64_t kNoAllocDoubles =
+    kNoAllocCtl + kNoAllocChunks + (kNoAllocMaxN + DE_BLOCK - 1) / DE_BLOCK;
+static_assert(sizeof(DeControl) <= kNoAllocCtl * sizeof(double), "control block size");
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[256];                        // per device, from hipGetSymbolAddress
 
 struct KernelTimer {
     std::vector<hipEvent_t> start, stop;
@@ -2587,6 +2605,55 @@ int anuga_get_water_volume(anuga_domain *d, double *volume)
     rc = volumes(d);
     if (rc) return rc;
     *volume = d->hctl->volume;
+    return ANUGA_OK;
+}
+
+// the current device's scratch address (looked up once per device)
+static int scratch_address(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(nullptr, ANUGA_E_HIP, hipGetErrorString(e));
+    if (dev < 0 || dev >= 256) return fail(nullptr, ANUGA_E_HIP, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(nullptr, ANUGA_E_HIP, std::string("scratch address: ") +
+                                                  hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return ANUGA_OK;
+}
+
+int anuga_water_volume_noalloc(int64_t N, const double *stage, const double *elevation,
+                               const double *area, double *volume, void *stream)
+{
+    if (N <= 0 || !stage || !elevation || !area || !volume)
+        return fail(nullptr, ANUGA_E_ARG, "water_volume_noalloc: bad arguments");
+    if (N > kNoAllocMaxN)
+        return fail(nullptr, ANUGA_E_ARG, "water_volume_noalloc: at most 2^27 triangles");
+    double *sc = nullptr;
+    int rc = scratch_address(&sc);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    DeControl *ctl = reinterpret_cast<DeControl *>(sc);
+    double *chunks = sc + kNoAllocCtl, *part = chunks + kNoAllocChunks;
+    const int n = (int)N, nb = (int)((N + DE_BLOCK - 1) / DE_BLOCK);
+    // the reduction of anuga_get_water_volume (de_volume, MODE 3), over the
+    // caller's arrays
+    hipLaunchKernelGGL(de_control_init, dim3(1), dim3(64), 0, st, ctl, n, nb);
+    hipLaunchKernelGGL(de_volume_arrays, dim3(nb), dim3(DE_BLOCK), 0, st, n, stage, elevation,
+                       area, part);
+    hipLaunchKernelGGL(de_reduce_chunks<3>, dim3(DE_RCHUNKS, 1), dim3(256), 0, st, part, part,
+                       part, ctl, chunks);
+    hipLaunchKernelGGL(de_reduce_final<3>, dim3(1), dim3(64), 0, st, chunks, ctl);
+    hipLaunchKernelGGL(de_copy_volume, dim3(1), dim3(64), 0, st, ctl, volume);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(nullptr, ANUGA_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return ANUGA_OK;
 }
 

@@ -1773,20 +1773,11 @@ __global__ void de_clear_status(DeControl *ctl, int S)
     if (c) c->status = 0;
 }
 
-// per-block partial of sum(area * max(w - z, 0)) over the current state
-__global__ void __launch_bounds__(DE_BLOCK)
-de_volume(const int *__restrict__ scen, const DeControl *ctl, const DeQ *__restrict__ q,
-          const double *__restrict__ area, double *__restrict__ part)
+// the per-block partial of a per-thread value: wave sums, then the block's
+// waves in order (the fixed order every volume reduction uses)
+__device__ __forceinline__ void de_block_sum(double v, double *__restrict__ part, int lb)
 {
     __shared__ double red[DE_BLOCK / 64];
-    const int lb = xcd_block();
-    const DeControl *cb = scen ? ctl + scen[lb] : ctl;
-    const int k = lb * DE_BLOCK + threadIdx.x;
-    double v = 0.0;
-    if (k < cb->kend) {
-        const DeQ r = q[k];
-        v = area[k] * de_height(r.w, r.z);
-    }
     v = de_wave_sum(v);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) red[wv] = v;
@@ -1796,6 +1787,52 @@ de_volume(const int *__restrict__ scen, const DeControl *ctl, const DeQ *__restr
         for (int i = 1; i < DE_BLOCK / 64; i++) t += red[i];
         part[lb] = t;
     }
+}
+
+// per-block partial of sum(area * max(w - z, 0)) over the current state
+__global__ void __launch_bounds__(DE_BLOCK)
+de_volume(const int *__restrict__ scen, const DeControl *ctl, const DeQ *__restrict__ q,
+          const double *__restrict__ area, double *__restrict__ part)
+{
+    const int lb = xcd_block();
+    const DeControl *cb = scen ? ctl + scen[lb] : ctl;
+    const int k = lb * DE_BLOCK + threadIdx.x;
+    double v = 0.0;
+    if (k < cb->kend) {
+        const DeQ r = q[k];
+        v = area[k] * de_height(r.w, r.z);
+    }
+    de_block_sum(v, part, lb);
+}
+
+// the same partials over separate stage, bed and area arrays of N triangles
+__global__ void __launch_bounds__(DE_BLOCK)
+de_volume_arrays(int N, const double *__restrict__ w, const double *__restrict__ z,
+                 const double *__restrict__ area, double *__restrict__ part)
+{
+    const int lb = xcd_block();
+    const int k = lb * DE_BLOCK + threadIdx.x;
+    double v = 0.0;
+    if (k < N) v = area[k] * de_height(w[k], z[k]);
+    de_block_sum(v, part, lb);
+}
+
+// a one-scenario control block over triangles [0, N) in nblocks blocks
+__global__ void de_control_init(DeControl *c, int N, int nblocks)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c->kbeg = 0;
+    c->kend = N;
+    c->bbeg = 0;
+    c->bend = nblocks;
+    c->done = 0;
+    c->status = 0;
+    c->volume = 0.0;
+}
+
+__global__ void de_copy_volume(const DeControl *c, double *out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *out = c->volume;
 }
 
 // mixed path: FP32 bed offsets of the neighbours' centroids from the own one

@@ -381,6 +381,20 @@ int anuga_get_negative_cells(anuga_domain *d, int64_t *total);
  * (get_water_volume, shallow_water_domain.py:1626-1653), fixed-order FP64 reduction. */
 int anuga_get_water_volume(anuga_domain *d, double *volume);
 
+/* No counterpart in the reference: a variant of anuga_get_water_volume for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). sum(area * max(stage -
+ * elevation, 0)) over N triangles of caller-provided device arrays, reduced
+ * in anuga_get_water_volume's fixed order (the same bits for the same arrays
+ * in the same order), written to the device double *volume, enqueued on
+ * `stream` (a hipStream_t; NULL: the null stream) without waiting. Allocates
+ * nothing: its partial sums live in a static device array of the library,
+ * one per GPU (4 MiB + 1 KiB, held until the process ends), so N is at most
+ * 2^27 (ANUGA_E_ARG beyond) and calls that overlap on one GPU race on it.
+ * Errors: anuga_last_error(NULL). */
+int anuga_water_volume_noalloc(int64_t N, const double *stage, const double *elevation,
+                               const double *area, double *volume, void *stream);
+
 /* The Reverse-Cuthill-McKee triangle order applied at creation when
  * params->reorder is set: order[new] = old. Host-only (needs no device).
  * neighbours is (N,3) with boundary edges negative. */
