@@ -125,7 +125,7 @@ SIGNATURES = {
     "anuga_tile_order": (C.c_int, [C.c_int64, _dp, _ip]),
     "anuga_tile_halos": (C.c_int, [C.c_int64, C.POINTER(C.c_int32), _ip, C.c_int64,
                                    C.POINTER(C.c_int32), C.POINTER(C.c_uint32), _ip, _ip]),
-    "anuga_tile_edges": (C.c_int, [C.c_int64, C.POINTER(C.c_int32), _ip, C.c_int64,
+    "anuga_tile_edges": (C.c_int, [C.c_int64, C.POINTER(C.c_int32), _ip, _ip, C.c_int64,
                                    C.POINTER(C.c_uint32), _ip]),
     "anuga_synchronize": (C.c_int, [C.c_void_p]),
     "anuga_kernel_timing": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _ip]),

@@ -103,6 +103,11 @@ struct DeMesh {
     // coordinates [N][6] (sloped friction); read by the general kernels only
     const int *__restrict__ caller;
     const double *__restrict__ vx;
+    // vertex geometry (FP64 path, where it reproduces the mesh's arrays): per
+    // triangle its three vertices' node indices, and the node coordinates;
+    // null: the stored geometry records
+    const int4 *__restrict__ tv;
+    const double2 *__restrict__ node;
 };
 
 // Compact tiles of DE_BLOCK consecutive triangles (a space-filling-curve
@@ -115,9 +120,10 @@ struct DeTiles {
     const unsigned *__restrict__ slot;  // [N] 3 x 10 bits: halo slot of edge i, 1023: in the tile
     int hcap;                           // the largest halo (LDS stride)
     // the edges whose flux a tile evaluates once (null: none built): each
-    // in-tile interior edge from its lower-numbered triangle, and each edge
+    // in-tile interior edge from its lower-numbered triangle, then each edge
     // to the halo from the tile's side (DE_TE_* fields)
     const int *__restrict__ estart;     // [tiles + 1] offsets into edge
+    const int *__restrict__ esplit;     // [tiles] the first edge to the halo
     const unsigned *__restrict__ edge;
 };
 

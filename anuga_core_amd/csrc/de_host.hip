@@ -18,6 +18,7 @@
 #include <cstring>
 #include <string>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "anuga_hip.h"
@@ -98,15 +99,22 @@ struct anuga_domain {
     int hcap = 0;
     // per scenario, until every mesh is set: tile offsets, halo (internal
     // ids), slots and largest halo (-1: no tiles)
-    std::vector<std::vector<int>> ts_start, ts_halo, ts_estart;
+    std::vector<std::vector<int>> ts_start, ts_halo, ts_estart, ts_esplit;
     std::vector<std::vector<unsigned>> ts_slot, ts_edge;
     // tile edge lists of the once-per-edge kernel (null: not built); mirror:
     // every interior edge's normals and lengths mirror exactly (checked per
     // scenario at upload), which that kernel assumes
-    int *testart = nullptr;
+    int *testart = nullptr, *tesplit = nullptr;
     unsigned *tedge = nullptr;
     bool mirror = true;
     bool edge_once = true;     // ANUGA_HIP_EDGE_ONCE=0: the gather-every-side kernel
+    // vertex geometry: the fused FP64 kernel forms edge offsets, normals and
+    // lengths from the vertices (tv, node) instead of reading the stored
+    // records, where that reproduces every scenario's arrays bit for bit
+    // (ANUGA_HIP_VERTEX_GEOMETRY=0: the stored records)
+    int4 *tv = nullptr;
+    double2 *node = nullptr;
+    bool vgeom = true;
     std::vector<int> ts_hcap;
     bool e_stale = false;                // E does not hold the last extrapolation
     int *rw = nullptr;                   // [3][N] riverwall record per half-edge, -1: none
@@ -454,6 +462,8 @@ static DeMesh dev_mesh(const anuga_domain *d)
     m.rwd = d->rwd;
     m.caller = d->caller;
     m.vx = d->vx;
+    m.tv = d->vgeom ? d->tv : nullptr;
+    m.node = d->vgeom ? d->node : nullptr;
     return m;
 }
 
@@ -529,7 +539,7 @@ template <int SUB>
 static void launch_xflux_t(anuga_domain *d, DeQ *S, DeQ *bk, DeQ *out)
 {
     DeBoundary bc{d->bkind, d->bval, d->M, d->bsstride};
-    DeTiles tl{d->tstart, d->thalo, d->tslot, d->hcap, d->testart, d->tedge};
+    DeTiles tl{d->tstart, d->thalo, d->tslot, d->hcap, d->testart, d->tesplit, d->tedge};
     if (d->mixed) {
         // FP64 centroid depth + 12 FP32 record components per triangle
         const size_t lds = ((size_t)DE_BLOCK + (size_t)d->hcap) * (sizeof(double) + 12 * sizeof(float));
@@ -548,9 +558,10 @@ static void launch_xflux_t(anuga_domain *d, DeQ *S, DeQ *bk, DeQ *out)
                            (double *)d->E);
     } else if (d->mirror && d->tedge && d->edge_once) {
         // each edge's flux once: tile and halo edge values, then the tile's
-        // centroid depths and beds
+        // centroid depths and beds; geometry from the vertices where it can be
         const size_t lds = (14 * (size_t)DE_BLOCK + 12 * (size_t)d->hcap) * sizeof(double);
-        hipLaunchKernelGGL(de_xflux_e<SUB>, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
+        auto kern = (d->vgeom && d->tv) ? de_xflux_e<SUB, true> : de_xflux_e<SUB, false>;
+        hipLaunchKernelGGL(kern, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
                            dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags, d->ex[0],
                            d->ex[1], d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
                            d->part + 2 * (size_t)d->nblocks, d->dneg);
@@ -909,6 +920,8 @@ static int domain_alloc(const anuga_params *params, int64_t S64, const int64_t *
         d->use_graph = !(g && g[0] == '0');
         const char *e = std::getenv("ANUGA_HIP_EDGE_ONCE");
         d->edge_once = !(e && e[0] == '0');
+        const char *vg = std::getenv("ANUGA_HIP_VERTEX_GEOMETRY");
+        d->vgeom = !(vg && vg[0] == '0') && !d->mixed;
     }
     DCHK(hipDeviceSynchronize());
 #undef DCHK
@@ -974,26 +987,32 @@ static const int kMaxHalo = std::min(320, (65536 - 97 * DE_BLOCK - 1024) / 97);
 // triangle (consecutive entries read consecutive geometry). halo slots as
 // tile_halos assigns them.
 static void tile_edges(int N, const int4 *hn, const std::vector<unsigned> &slot,
-                       std::vector<int> &estart, std::vector<unsigned> &edges)
+                       std::vector<int> &estart, std::vector<int> &esplit,
+                       std::vector<unsigned> &edges)
 {
     const int nt = (N + DE_BLOCK - 1) / DE_BLOCK;
     estart.assign(nt + 1, 0);
+    esplit.assign(nt, 0);
     edges.clear();
     edges.reserve((size_t)N * 5 / 3 + 64);
     for (int b = 0; b < nt; b++) {
         estart[b] = (int)edges.size();
         const int k0 = b * DE_BLOCK, k1 = std::min(N, k0 + DE_BLOCK);
-        for (int i = 0; i < 3; i++)
-            for (int k = k0; k < k1; k++) {
-                const int v = i == 0 ? hn[k].x : (i == 1 ? hn[k].y : hn[k].z);
-                if (v < 0) continue;
-                const int n = v >> 2, j = v & 3;
-                const unsigned s = (slot[k] >> (10 * i)) & 1023u;
-                if (s != 1023u)
-                    edges.push_back(DE_TE_MAKE(k - k0, i, 1, s, j));
-                else if (k < n)
-                    edges.push_back(DE_TE_MAKE(k - k0, i, 0, n - k0, j));
-            }
+        // the edges inside the tile first, then the edges to the halo
+        for (int halo = 0; halo < 2; halo++) {
+            if (halo) esplit[b] = (int)edges.size();
+            for (int i = 0; i < 3; i++)
+                for (int k = k0; k < k1; k++) {
+                    const int v = i == 0 ? hn[k].x : (i == 1 ? hn[k].y : hn[k].z);
+                    if (v < 0) continue;
+                    const int n = v >> 2, j = v & 3;
+                    const unsigned s = (slot[k] >> (10 * i)) & 1023u;
+                    if (halo && s != 1023u)
+                        edges.push_back(DE_TE_MAKE(k - k0, i, 1, s, j));
+                    else if (!halo && s == 1023u && k < n)
+                        edges.push_back(DE_TE_MAKE(k - k0, i, 0, n - k0, j));
+                }
+        }
     }
     estart[nt] = (int)edges.size();
 }
@@ -1054,12 +1073,13 @@ static int scenario_tiles(anuga_domain *d, int s, const std::vector<int4> &hn)
         d->ts_halo.resize(d->S);
         d->ts_slot.resize(d->S);
         d->ts_estart.resize(d->S);
+        d->ts_esplit.resize(d->S);
         d->ts_edge.resize(d->S);
         d->ts_hcap.assign(d->S, -1);
     }
     const int hcap = tile_halos(N, loc.data(), d->ts_start[s], d->ts_halo[s], d->ts_slot[s]);
     if (hcap < 0) return fail(d, ANUGA_E_ARG, "tiles: neighbour out of range");
-    tile_edges(N, loc.data(), d->ts_slot[s], d->ts_estart[s], d->ts_edge[s]);
+    tile_edges(N, loc.data(), d->ts_slot[s], d->ts_estart[s], d->ts_esplit[s], d->ts_edge[s]);
     for (int &n : d->ts_halo[s]) n += kb;
     d->ts_hcap[s] = hcap;
     return ANUGA_OK;
@@ -1078,8 +1098,9 @@ static int upload_tiles(anuga_domain *d)
     if (d->tslot) HIPCHK(d, hipFree(d->tslot));
     d->tslot = nullptr;
     if (d->testart) HIPCHK(d, hipFree(d->testart));
+    if (d->tesplit) HIPCHK(d, hipFree(d->tesplit));
     if (d->tedge) HIPCHK(d, hipFree(d->tedge));
-    d->testart = nullptr;
+    d->testart = d->tesplit = nullptr;
     d->tedge = nullptr;
     if ((int)d->ts_hcap.size() != d->S) return ANUGA_OK;
     int hcap = 0;
@@ -1088,18 +1109,20 @@ static int upload_tiles(anuga_domain *d)
         hcap = std::max(hcap, d->ts_hcap[s]);
     }
     if (hcap > kMaxHalo) return ANUGA_OK;
-    std::vector<int> start(d->nblocks + 1, 0), halo, estart(d->nblocks + 1, 0);
+    std::vector<int> start(d->nblocks + 1, 0), halo, estart(d->nblocks + 1, 0),
+        esplit(d->nblocks, 0);
     std::vector<unsigned> slot(d->N, 0x3fffffffu), edge;
     int b = 0;
     for (int s = 0; s < d->S; s++) {
         const int b0 = d->kbeg[s] / DE_BLOCK, nt = (int)d->ts_start[s].size() - 1;
         for (; b < b0; b++) {
             start[b] = (int)halo.size();
-            estart[b] = (int)edge.size();
+            estart[b] = esplit[b] = (int)edge.size();
         }
         for (int t = 0; t < nt; t++, b++) {
             start[b] = (int)halo.size() + d->ts_start[s][t];
             estart[b] = (int)edge.size() + d->ts_estart[s][t];
+            esplit[b] = (int)edge.size() + d->ts_esplit[s][t];
         }
         halo.insert(halo.end(), d->ts_halo[s].begin(), d->ts_halo[s].end());
         edge.insert(edge.end(), d->ts_edge[s].begin(), d->ts_edge[s].end());
@@ -1108,8 +1131,12 @@ static int upload_tiles(anuga_domain *d)
     for (; b <= d->nblocks; b++) {
         start[b] = (int)halo.size();
         estart[b] = (int)edge.size();
+        if (b < d->nblocks) esplit[b] = (int)edge.size();
     }
     HIPCHK(d, dalloc(&d->testart, (size_t)d->nblocks + 1));
+    HIPCHK(d, dalloc(&d->tesplit, (size_t)d->nblocks));
+    HIPCHK(d, hipMemcpy(d->tesplit, esplit.data(), d->nblocks * sizeof(int),
+                        hipMemcpyHostToDevice));
     HIPCHK(d, dalloc(&d->tedge, std::max<size_t>(edge.size(), 1)));
     HIPCHK(d, hipMemcpy(d->testart, estart.data(), (d->nblocks + 1) * sizeof(int),
                         hipMemcpyHostToDevice));
@@ -1150,6 +1177,82 @@ static bool mesh_mirrors(const anuga_mesh *mesh)
                 return false;
         }
     return true;
+}
+
+// Whether the mesh's edge midpoints, normals and lengths are the ones
+// general_mesh.py forms from its vertex coordinates (:202-232, :599-626),
+// bit for bit, so that the kernels may form them from the vertices.
+static bool geometry_from_vertices(const anuga_mesh *mesh)
+{
+    const double *V = mesh->vertex_coordinates;
+    if (!V) return false;
+    const double *nrm = mesh->normals, *el = mesh->edgelengths, *em = mesh->edge_coordinates;
+    for (int64_t k = 0; k < mesh->number_of_elements; k++) {
+        const double *v = V + 6 * k;
+        for (int i = 0; i < 3; i++) {
+            const int a = (i + 1) % 3, b = (i + 2) % 3;
+            const double mx = 0.5 * (v[2 * a] + v[2 * b]), my = 0.5 * (v[2 * a + 1] + v[2 * b + 1]);
+            if (!(mx == em[6 * k + 2 * i]) || !(my == em[6 * k + 2 * i + 1])) return false;
+            const double xn = v[2 * b] - v[2 * a], yn = v[2 * b + 1] - v[2 * a + 1];
+            const double l = std::sqrt(xn * xn + yn * yn);
+            if (!(l == el[3 * k + i]) || !(yn / l == nrm[6 * k + 2 * i]) ||
+                !(-(xn / l) == nrm[6 * k + 2 * i + 1]))
+                return false;
+        }
+    }
+    return true;
+}
+
+// Nodes and per-triangle node indices of scenario s (internal order): the
+// vertices with identical coordinates are one node, numbered by first use, so
+// a tile's nodes are close together. Scenario s owns node slots
+// [3 kbeg, 3 (kbeg + ntri)). Turns the vertex geometry off (for every
+// scenario) when this mesh's arrays are not formed from its vertices.
+static int upload_vertex_geometry(anuga_domain *d, int s, const anuga_mesh *mesh,
+                                  const std::vector<int> &perm)
+{
+    if (!geometry_from_vertices(mesh)) {
+        d->vgeom = false;
+        return ANUGA_OK;
+    }
+    const int N = d->ntri[s], kb = d->kbeg[s];
+    const double *V = mesh->vertex_coordinates;
+    struct Key {
+        uint64_t x, y;
+        bool operator==(const Key &o) const { return x == o.x && y == o.y; }
+    };
+    struct Hash {
+        size_t operator()(const Key &k) const { return std::hash<uint64_t>()(k.x * 0x9E3779B97F4A7C15ull ^ k.y); }
+    };
+    std::unordered_map<Key, int, Hash> id;
+    id.reserve((size_t)N * 2);
+    std::vector<double2> nodes;
+    nodes.reserve((size_t)N);
+    std::vector<int4> tv(N);
+    const int base = 3 * kb;
+    for (int p = 0; p < N; p++) {
+        const double *v = V + 6 * (size_t)perm[p];
+        int t[3];
+        for (int i = 0; i < 3; i++) {
+            Key key;
+            std::memcpy(&key.x, v + 2 * i, 8);
+            std::memcpy(&key.y, v + 2 * i + 1, 8);
+            auto it = id.find(key);
+            if (it == id.end()) {
+                it = id.emplace(key, (int)nodes.size()).first;
+                nodes.push_back(make_double2(v[2 * i], v[2 * i + 1]));
+            }
+            t[i] = base + it->second;
+        }
+        tv[p] = make_int4(t[0], t[1], t[2], 0);
+    }
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    if (!d->tv) HIPCHK(d, dalloc(&d->tv, (size_t)d->N));
+    if (!d->node) HIPCHK(d, dalloc(&d->node, 3 * (size_t)d->N));
+    HIPCHK(d, hipMemcpy(d->tv + kb, tv.data(), N * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(d, hipMemcpy(d->node + base, nodes.data(), nodes.size() * sizeof(double2),
+                        hipMemcpyHostToDevice));
+    return ANUGA_OK;
 }
 
 static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
@@ -1276,6 +1379,10 @@ static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
         if (!d->vx) HIPCHK(d, dalloc(&d->vx, 6 * (size_t)d->N));
         HIPCHK(d, hipMemcpy(d->vx + 6 * (size_t)kb, hv.data(), 6 * (size_t)N * sizeof(double),
                             hipMemcpyHostToDevice));
+    }
+    if (d->vgeom) {
+        int rc = upload_vertex_geometry(d, s, mesh, perm);
+        if (rc) return rc;
     }
     if (d->P.reorder == 2) {
         int rc = scenario_tiles(d, s, hn);
@@ -1691,7 +1798,8 @@ void anuga_destroy(anuga_domain *d)
                     d->scen, d->tstart, d->thalo, d->tslot, d->rw, d->rwd, d->caller, d->vx, d->node_start, d->node_tri, d->iperm_dev,
                     d->out_f32, d->gtri, d->gtimes, d->gvals, d->sub_idx, d->sub_val, d->eta, d->A, d->B, d->C, d->E, d->flags,
                     d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->chunks, d->ctl, d->dsetup,
-                    d->dneg, d->hist, d->si[0], d->si[1], d->testart, d->tedge};
+                    d->dneg, d->hist, d->si[0], d->si[1], d->testart, d->tesplit, d->tedge, d->tv,
+                    d->node};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (d->hctl) hipHostFree(d->hctl);
@@ -2683,20 +2791,21 @@ int anuga_tile_order(int64_t N, const double *centroids, int64_t *order)
     return ANUGA_OK;
 }
 
-int anuga_tile_edges(int64_t N, const int32_t *nb, int64_t *estart, int64_t edge_cap,
-                     uint32_t *edges, int64_t *nedges)
+int anuga_tile_edges(int64_t N, const int32_t *nb, int64_t *estart, int64_t *esplit,
+                     int64_t edge_cap, uint32_t *edges, int64_t *nedges)
 {
-    if (N < 0 || N >= (int64_t(1) << 29) || (N > 0 && !nb) || !estart || !nedges)
+    if (N < 0 || N >= (int64_t(1) << 29) || (N > 0 && !nb) || !estart || !esplit || !nedges)
         return ANUGA_E_ARG;
     std::vector<int4> hn(N);
     for (int64_t k = 0; k < N; k++) hn[k] = make_int4(nb[3 * k], nb[3 * k + 1], nb[3 * k + 2], 0);
-    std::vector<int> st, h, est;
+    std::vector<int> st, h, est, esp;
     std::vector<unsigned> sl, e;
     if (tile_halos((int)N, hn.data(), st, h, sl) < 0) return ANUGA_E_ARG;
-    tile_edges((int)N, hn.data(), sl, est, e);
+    tile_edges((int)N, hn.data(), sl, est, esp, e);
     *nedges = (int64_t)e.size();
     if ((int64_t)e.size() > edge_cap || (!edges && !e.empty())) return ANUGA_E_ARG;
     for (size_t b = 0; b < est.size(); b++) estart[b] = est[b];
+    for (size_t b = 0; b < esp.size(); b++) esplit[b] = esp[b];
     for (size_t q = 0; q < e.size(); q++) edges[q] = e[q];
     return ANUGA_OK;
 }

@@ -81,7 +81,41 @@ struct DeExtrap {
     double z;   // the centroid bed
 };
 
-template <class T>
+// The extrapolation geometry of triangle k from its vertices, as
+// general_mesh.py forms the arrays the reference's extrapolation reads
+// (:599-626 edge midpoints 0.5 (Va + Vb); sw_domain_orig.c:2112-2135 their
+// offsets from the stored centroid).
+__device__ __forceinline__ void de_vertex_offsets(const DeMesh &m, int k, double dxv[3],
+                                                  double dyv[3])
+{
+    const int4 t = m.tv[k];
+    const double2 v0 = m.node[t.x], v1 = m.node[t.y], v2 = m.node[t.z];
+    const double2 c = m.cen[k];
+    dxv[0] = 0.5 * (v1.x + v2.x) - c.x;
+    dyv[0] = 0.5 * (v1.y + v2.y) - c.y;
+    dxv[1] = 0.5 * (v2.x + v0.x) - c.x;
+    dyv[1] = 0.5 * (v2.y + v0.y) - c.y;
+    dxv[2] = 0.5 * (v0.x + v1.x) - c.x;
+    dyv[2] = 0.5 * (v0.y + v1.y) - c.y;
+}
+
+// Outward normal and length of edge i of triangle k from its vertices
+// (general_mesh.py:202-232: the edge opposite vertex i, (x, y) =
+// V[i+2] - V[i+1], l = sqrt(x^2 + y^2), normal (y / l, -(x / l))).
+__device__ __forceinline__ void de_vertex_normal(const DeMesh &m, int k, int i, double &n1,
+                                                 double &n2, double &len)
+{
+    const int4 t = m.tv[k];
+    const int a = i == 0 ? t.y : (i == 1 ? t.z : t.x);
+    const int b = i == 0 ? t.z : (i == 1 ? t.x : t.y);
+    const double2 va = m.node[a], vb = m.node[b];
+    const double xn = vb.x - va.x, yn = vb.y - va.y;
+    len = sqrt(xn * xn + yn * yn);
+    n1 = yn / len;
+    n2 = -(xn / len);
+}
+
+template <class T, bool VG = false>
 __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &p,
                                                const DeQ *__restrict__ q, int k, int *err,
                                                DeExtrap<T> &x)
@@ -155,12 +189,17 @@ __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &
         }
     }
 
-    const typename DeGeo<T>::X gx = static_cast<const typename DeGeo<T>::X *>(m.gx)[k];
+    typename DeGeo<T>::X gx;
     T dxv[3], dyv[3];
+    if constexpr (VG) {
+        de_vertex_offsets(m, k, dxv, dyv);
+    } else {
+        gx = static_cast<const typename DeGeo<T>::X *>(m.gx)[k];
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        dxv[i] = gx.dxv[i];
-        dyv[i] = gx.dyv[i];
+        for (int i = 0; i < 3; i++) {
+            dxv[i] = gx.dxv[i];
+            dyv[i] = gx.dyv[i];
+        }
     }
     // limited jumps from the centroid to the edges, per quantity
     T(&jw)[3] = x.jw, (&jh)[3] = x.jh, (&ju)[3] = x.ju, (&jv)[3] = x.jv;
@@ -176,7 +215,16 @@ __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &
     gw = gu = gv = true;   // false: beta == 0, edge = centroid value
     if (nbound <= 1) {
         T dx1, dx2, dy1, dy2, area2, inv_area2;
-        if constexpr (sizeof(T) == 8) {
+        if constexpr (VG) {
+            // the surrogate centroids' triangle (sw_domain_orig.c:2130-2160)
+            const double2 c0 = m.cen[sur[0]], c1 = m.cen[sur[1]], c2 = m.cen[sur[2]];
+            dx1 = c1.x - c0.x;
+            dx2 = c2.x - c0.x;
+            dy1 = c1.y - c0.y;
+            dy2 = c2.y - c0.y;
+            area2 = dy2 * dx1 - dy1 * dx2;
+            inv_area2 = 1.0 / area2;
+        } else if constexpr (sizeof(T) == 8) {
             dx1 = gx.aux[0];
             dx2 = gx.aux[1];
             dy1 = gx.aux[2];
@@ -243,7 +291,16 @@ __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &
             n1.u = e0 ? a0.u : (e1 ? a1.u : a2.u);
             n1.v = e0 ? a0.v : (e1 ? a1.v : a2.v);
             T dx2, dy2;
-            if constexpr (sizeof(T) == 8) {
+            if constexpr (VG) {
+                // toward the one interior neighbour (sw_domain_orig.c:2421-2450)
+                const int s1 = e0 ? sur[0] : (e1 ? sur[1] : sur[2]);
+                const double2 cs = m.cen[s1], ck = m.cen[k];
+                const double ddx = cs.x - ck.x, ddy = cs.y - ck.y;
+                const double a2 = ddx * ddx + ddy * ddy;
+                dx2 = 1.0 / a2;
+                dy2 = dx2 * ddy;
+                dx2 *= ddx;
+            } else if constexpr (sizeof(T) == 8) {
                 dx2 = gx.aux[1];
                 dy2 = gx.aux[3];
             } else {
@@ -908,19 +965,86 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
     }
 }
 
+// One listed edge of a tile (de_xflux_e): its flux from the tile triangle's
+// side (HALO: the neighbour is in the halo; otherwise in the tile, and its
+// side's pressure term and CFL bound are formed here too), written over the
+// LDS edge values that only this entry reads.
+template <int SUBSTEP, bool VG, bool HALO>
+__device__ __forceinline__ void de_tile_edge(const DeMesh &m, const DeParams &p, unsigned en,
+                                             int k0, double *lds, double *hl,
+                                             const double *hz, int hcap, double &dtmin)
+{
+    constexpr int Q = 3 * DE_BLOCK;
+    const double g = p.g, eps = p.eps;
+    const int N = m.N;
+    const int ta = DE_TE_T(en), ia = DE_TE_I(en), nb = DE_TE_N(en), jb = DE_TE_J(en);
+    const int ka = k0 + ta;
+    double *const A = lds + ia * DE_BLOCK + ta;
+    double *const R = HALO ? hl + jb * hcap + nb : lds + jb * DE_BLOCK + nb;
+    const int rs = HALO ? 3 * hcap : Q;
+    const double ql0 = A[0], hle = A[Q], ql1 = A[2 * Q], ql2 = A[3 * Q];
+    const double qr0 = R[0], hre = R[rs], qr1 = R[2 * rs], qr2 = R[3 * rs];
+    double n1, n2, length;
+    if constexpr (VG) {
+        de_vertex_normal(m, ka, ia, n1, n2, length);
+    } else {
+        n1 = m.nx[ia * N + ka];
+        n2 = m.ny[ia * N + ka];
+        length = m.len[ia * N + ka];
+    }
+    const double zl = ql0 - hle, zr = qr0 - hre;
+    const double z_half = fmax(zl, zr);
+    const double h_left = fmax(hle + zl - z_half, 0.0);
+    const double h_right = fmax(hre + zr - z_half, 0.0);
+    double ef[3], speed, pflux;
+    de_flux_central<double>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1, n2,
+                            eps, z_half, g, ef, speed, pflux);
+    ef[0] *= length;
+    ef[1] *= length;
+    ef[2] *= length;
+    const double hca = hz[ta], zca = hz[DE_BLOCK + ta];
+    const double pga = length * (-g * 0.5 * (h_left * h_left - hle * hle -
+                                             (hle + hca) * (zl - zca)) + pflux);
+    A[0] = -ef[0];
+    A[Q] = -ef[1];
+    A[2 * Q] = -ef[2];
+    A[3 * Q] = n1 * pga;
+    R[0] = n2 * pga;
+    double tt = 0.0;
+    const bool cfl = SUBSTEP == 0 && speed > eps;
+    if (cfl) {
+        tt = 1.0 / fmax(speed, eps);
+        dtmin = fmin(dtmin, m.radius[ka] * tt);
+    }
+    if constexpr (!HALO) {
+        // the other side: its own depths and bed, the same flux
+        const double hcb = hz[nb], zcb = hz[DE_BLOCK + nb];
+        const double pgb = length * (-g * 0.5 * (h_right * h_right - hre * hre -
+                                                 (hre + hcb) * (zr - zcb)) + pflux);
+        R[rs] = n1 * pgb;
+        R[2 * rs] = n2 * pgb;
+        if (cfl) dtmin = fmin(dtmin, m.radius[k0 + nb] * tt);
+    }
+}
+
 // The fused FP64 kernel with each edge's flux evaluated once (the common
 // options: no riverwalls, low_froude 0, flat friction; the mesh's normals and
-// lengths mirror exactly across every interior edge). Phase 1 is de_xflux's.
-// Phase 2a walks the tile's edge list: an edge between two tile triangles
-// from its lower-numbered side, which also forms the other side's pressure
-// term (the reference's compute-once-and-mirror loop, sw_domain_orig.c:
-// 996-1009), and an edge to the halo from the tile's side; the results
-// overwrite the LDS edge values that only this entry reads. Phase 2b gathers
-// each triangle's three edges in local order as de_xflux does: the same
-// operations on the same values, so the same bits (the other side's flux is
-// exactly the negation, its pressure flux and wave speed identical);
-// boundary edges are evaluated there, from the untouched edge values.
-template <int SUBSTEP>
+// lengths mirror exactly across every interior edge). VG: the geometry is
+// formed from the vertices. Phases, with a barrier after each:
+//  1. every thread extrapolates its triangle into LDS (de_extrap_cell, as
+//     de_xflux);
+//  2. the first waves extrapolate the halo while the others evaluate the
+//     edges inside the tile, each from its lower-numbered side, which also
+//     forms the other side's pressure term (the reference's compute-once-and-
+//     mirror loop, sw_domain_orig.c:996-1009);
+//  3. the edges to the halo, from the tile's side;
+//  4. each thread gathers its triangle's three edges in local order as
+//     de_xflux does: the same operations on the same values, so the same bits
+//     (the other side's flux is exactly the negation, its pressure flux and
+//     wave speed identical); boundary edges are evaluated here, from the
+//     untouched edge values.
+// Each listed edge's results overwrite the LDS edge values only it reads.
+template <int SUBSTEP, bool VG = false>
 __global__ void __launch_bounds__(DE_BLOCK)
 de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restrict__ eta,
            DeTiles tl, DeBoundary bc, unsigned char *__restrict__ flags, double *__restrict__ exw,
@@ -928,6 +1052,7 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
            DeControl *ctl, double *__restrict__ part_dt, double *__restrict__ part_bf,
            double *__restrict__ part_mass, unsigned long long *__restrict__ nneg)
 {
+    static_assert(DE_BLOCK >= 128, "the halo and the tile's edges share the workgroup's waves");
     // [4 quantities][3 edges][DE_BLOCK] tile edge values, [4][3][hcap] halo,
     // then [2][DE_BLOCK] the tile's centroid depths and beds
     extern __shared__ double lds[];
@@ -946,10 +1071,24 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
     double mass = 0.0, dtmin = 1.0e100, bflux = 0.0;
     unsigned fk = 0;
     constexpr int Q = 3 * DE_BLOCK;   // stride between quantities of the tile
+    // phase 4's own inputs, loaded now so that their latency hides behind
+    // the extrapolation and the edge list
+    double area_k = 1.0, eta_k = 0.0;
+    DeQ bk_k{0.0, 0.0, 0.0, 0.0};
+    unsigned sl_k = 0;
+    if (k < kend) {
+        area_k = m.area[k];
+        sl_k = tl.slot[k];
+        if (SUBSTEP) {
+            eta_k = eta[k];
+            bk_k = bk[k];
+        }
+    }
 
+    // phase 1
     if (k < kend) {
         DeExtrap<double> x;
-        de_extrap_cell<double>(m, p, q, k, err, x);
+        de_extrap_cell<double, VG>(m, p, q, k, err, x);
         mass = x.mass;
         fk = x.flag;
         flags[k] = (unsigned char)fk;
@@ -965,81 +1104,44 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
             lds[(9 + i) * DE_BLOCK + t] = evh;
         }
     }
+    __syncthreads();
+
+    // phase 2: hw waves on the halo (at least one wave left for the tile's
+    // edges), the rest on the edges inside the tile
     const int h0 = tl.start[lb], nh = tl.start[lb + 1] - h0;
-    for (int j = t; j < nh; j += DE_BLOCK) {
-        DeExtrap<double> x;
-        de_extrap_cell<double>(m, p, q, tl.halo[h0 + j], err, x);
+    const int e0 = tl.estart[lb], es = tl.esplit[lb], e1 = tl.estart[lb + 1];
+    const int hw = min((nh + 63) >> 6, DE_BLOCK / 64 - 1);
+    if (t < 64 * hw) {
+        for (int j = t; j < nh; j += 64 * hw) {
+            DeExtrap<double> x;
+            de_extrap_cell<double, VG>(m, p, q, tl.halo[h0 + j], err, x);
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            double ew, eh, euh, evh;
-            de_edge64(x, i, ew, eh, euh, evh);
-            hl[(0 + i) * hcap + j] = ew;
-            hl[(3 + i) * hcap + j] = eh;
-            hl[(6 + i) * hcap + j] = euh;
-            hl[(9 + i) * hcap + j] = evh;
+            for (int i = 0; i < 3; i++) {
+                double ew, eh, euh, evh;
+                de_edge64(x, i, ew, eh, euh, evh);
+                hl[(0 + i) * hcap + j] = ew;
+                hl[(3 + i) * hcap + j] = eh;
+                hl[(6 + i) * hcap + j] = euh;
+                hl[(9 + i) * hcap + j] = evh;
+            }
         }
+    } else {
+        for (int e = e0 + t - 64 * hw; e < es; e += DE_BLOCK - 64 * hw)
+            de_tile_edge<SUBSTEP, VG, false>(m, p, tl.edge[e], k0, lds, hl, hz, hcap, dtmin);
     }
     __syncthreads();
 
-    // phase 2a: each listed edge once
-    {
-        const double g = p.g, eps = p.eps;
-        const int e0 = tl.estart[lb], ne = tl.estart[lb + 1] - e0;
-        for (int e = t; e < ne; e += DE_BLOCK) {
-            const unsigned en = tl.edge[e0 + e];
-            const int ta = DE_TE_T(en), ia = DE_TE_I(en), nb = DE_TE_N(en), jb = DE_TE_J(en);
-            const bool halo = DE_TE_HALO(en);
-            const int ka = k0 + ta;
-            double *const A = lds + ia * DE_BLOCK + ta;
-            double *const R = halo ? hl + jb * hcap + nb : lds + jb * DE_BLOCK + nb;
-            const int rs = halo ? 3 * hcap : Q;
-            const double ql0 = A[0], hle = A[Q], ql1 = A[2 * Q], ql2 = A[3 * Q];
-            const double qr0 = R[0], hre = R[rs], qr1 = R[2 * rs], qr2 = R[3 * rs];
-            const double n1 = m.nx[ia * N + ka], n2 = m.ny[ia * N + ka];
-            const double length = m.len[ia * N + ka];
-            const double zl = ql0 - hle, zr = qr0 - hre;
-            const double z_half = fmax(zl, zr);
-            const double h_left = fmax(hle + zl - z_half, 0.0);
-            const double h_right = fmax(hre + zr - z_half, 0.0);
-            double ef[3], speed, pflux;
-            de_flux_central<double>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1,
-                                    n2, eps, z_half, g, ef, speed, pflux);
-            ef[0] *= length;
-            ef[1] *= length;
-            ef[2] *= length;
-            const double hca = hz[ta], zca = hz[DE_BLOCK + ta];
-            const double pga = length * (-g * 0.5 * (h_left * h_left - hle * hle -
-                                                     (hle + hca) * (zl - zca)) + pflux);
-            A[0] = -ef[0];
-            A[Q] = -ef[1];
-            A[2 * Q] = -ef[2];
-            A[3 * Q] = n1 * pga;
-            R[0] = n2 * pga;
-            double tt = 0.0;
-            const bool cfl = SUBSTEP == 0 && speed > eps;
-            if (cfl) {
-                tt = 1.0 / fmax(speed, eps);
-                dtmin = fmin(dtmin, m.radius[ka] * tt);
-            }
-            if (!halo) {
-                // the other side: its own depths and bed, the same flux
-                const double hcb = hz[nb], zcb = hz[DE_BLOCK + nb];
-                const double pgb = length * (-g * 0.5 * (h_right * h_right - hre * hre -
-                                                         (hre + hcb) * (zr - zcb)) + pflux);
-                R[rs] = n1 * pgb;
-                R[2 * rs] = n2 * pgb;
-                if (cfl) dtmin = fmin(dtmin, m.radius[k0 + nb] * tt);
-            }
-        }
-    }
+    // phase 3: the edges to the halo
+    for (int e = es + t; e < e1; e += DE_BLOCK)
+        de_tile_edge<SUBSTEP, VG, true>(m, p, tl.edge[e], k0, lds, hl, hz, hcap, dtmin);
     __syncthreads();
 
-    // phase 2b: each triangle's three edges in local order
+    // phase 4: each triangle's three edges in local order
     if (k < kend) {
         const DeCentroid c = de_centroid(q[k], fk);
         const int4 vk = m.nb[k];
         const int v[3] = {vk.x, vk.y, vk.z};
-        const unsigned sl = tl.slot[k];
+        const unsigned sl = sl_k;
         double sw = 0.0, sx = 0.0, sy = 0.0;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -1080,7 +1182,7 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
             sx -= b1;
             sy -= b2;
         }
-        const double inv_area = 1.0 / m.area[k];
+        const double inv_area = 1.0 / area_k;
         sw *= inv_area;
         sx *= inv_area;
         sy *= inv_area;
@@ -1089,8 +1191,8 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
             exuh[k] = sx;
             exvh[k] = sy;
         } else {
-            const DeQ r = de_advance(c, eta[k], sw, sx, sy, dt, p, err, nneg);
-            out[k] = de_rk_combine<SUBSTEP>(r, bk[k]);
+            const DeQ r = de_advance(c, eta_k, sw, sx, sy, dt, p, err, nneg);
+            out[k] = de_rk_combine<SUBSTEP>(r, bk_k);
         }
     }
 

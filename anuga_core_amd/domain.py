@@ -116,6 +116,14 @@ def _gauge_row(quantities, w, z, uh, vh):
     return out
 
 
+def _boundary_values(B, t, ids):
+    """[stage, xmomentum, ymomentum] of a time-dependent boundary on its
+    edges ids at time t: per edge (File/Field boundaries) or one row."""
+    if hasattr(B, "values_at"):
+        return B.values_at(t, ids)
+    return B.values(t)
+
+
 def _bearing(uh, vh):
     """calc_bearing (anuga/abstract_2d_finite_volumes/util.py:156-171): the
     angle in degrees, in [0, 360), from (0, -1) to the momentum vector
@@ -693,7 +701,7 @@ class Domain:
             kind[ids] = code
             if getattr(B, "time_dependent", False):
                 time_bc.append((B, ids))
-                vals[ids] = B.values(self.get_time())
+                vals[ids] = _boundary_values(B, self.get_time(), ids)
             elif code == 2:
                 vals[ids] = B.dirichlet_values[:3]
         self.boundary_map = dict(boundary_map)
@@ -708,7 +716,7 @@ class Domain:
             return
         vals = self._bc_values
         for B, ids in self._time_bc:
-            vals[ids] = B.values(t)
+            vals[ids] = _boundary_values(B, t, ids)
         self._call("anuga_set_boundary_values", slot, _dptr(vals))
 
     # -- step -------------------------------------------------------------------------
@@ -920,6 +928,17 @@ class Domain:
 
     def set_time(self, time=0.0):
         self.relative_time = float(time) - self.starttime
+
+    def get_starttime(self):
+        return self.starttime
+
+    def set_starttime(self, time):
+        """generic_domain.py:1607-1615"""
+        if self.evolved_called:
+            raise Exception("Can't change simulation start time once evolve has been called")
+        self.starttime = float(time)
+        # starttime is now the origin of relative_time
+        self.relative_time = 0.0
 
     def evolve(self, yieldstep=None, finaltime=None, duration=None, skip_initial_step=False,
                outputstep=None):

@@ -431,14 +431,14 @@ int anuga_tile_halos(int64_t N, const int32_t *nb, int64_t *start, int64_t halo_
 /* The edge lists of the fused kernel that evaluates each edge's flux once,
  * for the same tiles and packed neighbours as anuga_tile_halos: per tile
  * (estart: tiles + 1 offsets), each edge between two tile triangles from its
- * lower-numbered side and each edge to the halo from the tile's side, ordered
- * by edge index then triangle; an entry packs the triangle's offset in the
- * tile (bits 0-9), its edge (10-11), a halo flag (12), the neighbour's tile
- * offset or halo slot (13-22) and the neighbour's edge (23-24). At most
- * edge_cap entries; *nedges receives the count.
- * Host-only; no counterpart in the reference. */
-int anuga_tile_edges(int64_t N, const int32_t *nb, int64_t *estart, int64_t edge_cap,
-                     uint32_t *edges, int64_t *nedges);
+ * lower-numbered side, then (from esplit[tile]) each edge to the halo from
+ * the tile's side, each group ordered by edge index then triangle; an entry
+ * packs the triangle's offset in the tile (bits 0-9), its edge (10-11), a
+ * halo flag (12), the neighbour's tile offset or halo slot (13-22) and the
+ * neighbour's edge (23-24). At most edge_cap entries; *nedges receives the
+ * count. Host-only; no counterpart in the reference. */
+int anuga_tile_edges(int64_t N, const int32_t *nb, int64_t *estart, int64_t *esplit,
+                     int64_t edge_cap, uint32_t *edges, int64_t *nedges);
 
 /* Wait for all work on the handle's stream. */
 int anuga_synchronize(anuga_domain *d);

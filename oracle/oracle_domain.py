@@ -384,6 +384,13 @@ class OracleDomain:
                 self.stage_b[ids] = se[vol, edge]
                 self.xmom_b[ids] = 0.0
                 self.ymom_b[ids] = 0.0
+            elif B.kind in ("file", "field"):
+                # File_boundary / Field_boundary: per-edge values at the
+                # domain's time (generic_boundary_conditions.py:646-680)
+                v = B.values_at(self.get_time(), ids)
+                self.stage_b[ids] = v[:, 0]
+                self.xmom_b[ids] = v[:, 1]
+                self.ymom_b[ids] = v[:, 2]
             else:
                 raise ValueError("unsupported boundary kind %r" % B.kind)
 
@@ -496,6 +503,14 @@ class OracleDomain:
     # -- evolve ---------------------------------------------------------------------
     def get_time(self):
         return self.starttime + self.relative_time
+
+    def set_starttime(self, time):
+        """generic_domain.py:1607-1615"""
+        if self.evolved_called:
+            raise Exception("Can't change simulation start time once evolve has been called")
+        self.starttime = float(time)
+        # starttime is now the origin of relative_time
+        self.relative_time = 0.0
 
     def evolve(self, yieldstep=None, finaltime=None, max_steps=None):
         """Generator over yield times; `max_steps` stops after that many steps

@@ -157,13 +157,21 @@ def tile_edge_lists(nb):
     N = len(nb)
     nt = (N + BLOCK - 1) // BLOCK
     estart = np.empty(nt + 1, np.int64)
+    esplit = np.empty(nt, np.int64)
     cap = 3 * N + 1
     edges = np.empty(cap, np.uint32)
     ne = C.c_int64()
-    rc = lib.anuga_tile_edges(N, nb.ctypes.data_as(C.POINTER(C.c_int32)), _i64(estart), cap,
-                              edges.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(ne))
+    rc = lib.anuga_tile_edges(N, nb.ctypes.data_as(C.POINTER(C.c_int32)), _i64(estart),
+                              _i64(esplit), cap, edges.ctypes.data_as(C.POINTER(C.c_uint32)),
+                              C.byref(ne))
     assert rc == 0
-    return estart, edges[:ne.value]
+    edges = edges[:ne.value]
+    # each tile's edges inside the tile come first, then its edges to the halo
+    for b in range(nt):
+        assert estart[b] <= esplit[b] <= estart[b + 1]
+        halo = (edges[estart[b]:estart[b + 1]] >> 12) & 1
+        assert np.all(halo[:esplit[b] - estart[b]] == 0) and np.all(halo[esplit[b] - estart[b]:] == 1)
+    return estart, edges
 
 
 def replay_edge_once(nb, start, halo, slot, hcap, estart, edges):
@@ -213,6 +221,9 @@ def replay_edge_once(nb, start, halo, slot, hcap, estart, edges):
             for a in addrs:
                 assert 0 <= a < 12 * BLOCK + 12 * hs
                 assert a not in touched, "two entries share an LDS slot"
+                # an edge inside the tile touches only the tile's values: it runs
+                # while the halo's are being written
+                assert isha or a < 12 * BLOCK
             for q in range(4):
                 assert lds[A + q * Q] == ("val", k, i, q)
                 assert lds[R + q * rs] == ("val", n, j, q)
