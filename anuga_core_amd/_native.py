@@ -88,6 +88,11 @@ SIGNATURES = {
     "anuga_update_rate_operator": (C.c_int, [C.c_void_p, C.c_int64, _dp, C.c_double,
                                              C.c_double]),
     "anuga_apply_operators": (C.c_int, [C.c_void_p]),
+    "anuga_add_inlet_operator": (C.c_int, [C.c_void_p, C.c_int64, _ip, _dp, C.c_double,
+                                           C.c_int64, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, _ip]),
+    "anuga_update_inlet_operator": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_double]),
+    "anuga_get_inlet_operator_stats": (C.c_int, [C.c_void_p, C.c_int64, _dp]),
     "anuga_apply_operator": (C.c_int, [C.c_void_p, C.c_int64]),
     "anuga_get_centroid_subset": (C.c_int, [C.c_void_p, C.c_int64, _ip, _dp]),
     "anuga_set_centroid_subset": (C.c_int, [C.c_void_p, C.c_int64, _ip, _dp]),

@@ -61,13 +61,18 @@ const int kTimerRing = 64;
 
 // a Rate_operator's device data
 struct RateOp {
-    int kind = 0;            // 0: Rate_operator, 1: Set_stage_operator (rate = stage values)
+    int kind = 0;            // 0: Rate_operator, 1: Set_stage_operator (rate = stage values),
+                             // 2: Inlet_operator (rate = areas, influx = its 4 statistics)
     int n = 0;
     int *idx = nullptr;      // [n] internal triangles
     double *rate = nullptr;  // [n] rates, or NULL: the scalar rate0
     double rate0 = 0.0, factor = 1.0;
     int careful = 0;         // some rate < 0
     double *influx = nullptr;
+    // inlets: np.sum(areas), Q at the operator's time and a timestep later,
+    // velocity mode (0 region, 1 given, 2 zero) and the given velocity
+    double total_area = 0.0, Q1 = 0.0, Q2 = 0.0, u = 0.0, v = 0.0;
+    int vmode = 0;
 };
 
 struct anuga_domain {
@@ -693,7 +698,11 @@ static void launch_operator(anuga_domain *d, const RateOp &o)
 {
     if (o.n == 0) return;   // an empty region
     const dim3 g((o.n + DE_BLOCK - 1) / DE_BLOCK), b(DE_BLOCK);
-    if (o.kind == 1)
+    if (o.kind == 2) {
+        DeInlet in{o.n, o.idx, o.rate, o.total_area, o.Q1, o.Q2, o.vmode, o.u, o.v, o.influx};
+        hipLaunchKernelGGL(de_inlet, dim3(1), dim3(DE_INLET_THREADS), 0, d->stream, d->ctl, in,
+                           d->A);
+    } else if (o.kind == 1)
         hipLaunchKernelGGL(de_set_stage, g, b, 0, d->stream, d->ctl, o.n, o.idx, o.rate, o.rate0,
                            d->A);
     else
@@ -2010,11 +2019,75 @@ static int add_operator(anuga_domain *d, int kind, int64_t n, const int64_t *ind
     return ANUGA_OK;
 }
 
+int anuga_add_inlet_operator(anuga_domain *d, int64_t n, const int64_t *indices,
+                             const double *areas, double total_area, int64_t velocity_mode,
+                             double u, double v, double Q1, double Q2, int64_t *id)
+{
+    if (!d || !id || !indices || !areas || n < 1) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    if (n > DE_INLET_MAX)
+        return fail(d, ANUGA_E_ARG, "inlet regions on the device hold at most 2048 triangles");
+    if (velocity_mode < 0 || velocity_mode > 2) return fail(d, ANUGA_E_ARG, "bad velocity mode");
+    const int N = d->ntri[0];
+    RateOp o;
+    o.kind = 2;
+    o.n = (int)n;
+    std::vector<int> ip(N), idx(n);
+    for (int p = 0; p < N; p++) ip[d->perm[p]] = p;
+    for (int64_t i = 0; i < n; i++) {
+        if (indices[i] < 0 || indices[i] >= N)
+            return fail(d, ANUGA_E_ARG, "operator triangle out of range");
+        idx[i] = ip[indices[i]];
+    }
+    o.total_area = total_area;
+    o.Q1 = Q1;
+    o.Q2 = Q2;
+    o.vmode = (int)velocity_mode;
+    o.u = u;
+    o.v = v;
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    HIPCHK(d, dalloc(&o.idx, (size_t)n));
+    HIPCHK(d, dalloc(&o.rate, (size_t)n));
+    HIPCHK(d, dalloc(&o.influx, 4));
+    HIPCHK(d, hipMemcpy(o.idx, idx.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(d, hipMemcpy(o.rate, areas, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(d, hipMemset(o.influx, 0, 4 * sizeof(double)));
+    d->ops.push_back(o);
+    drop_graph(d);
+    *id = (int64_t)d->ops.size() - 1;
+    return ANUGA_OK;
+}
+
+int anuga_update_inlet_operator(anuga_domain *d, int64_t id, double Q1, double Q2)
+{
+    if (!d) return ANUGA_E_ARG;
+    if (id < 0 || id >= (int64_t)d->ops.size() || d->ops[id].kind != 2)
+        return fail(d, ANUGA_E_ARG, "no such inlet operator");
+    if (d->ops[id].Q1 != Q1 || d->ops[id].Q2 != Q2) drop_graph(d);
+    d->ops[id].Q1 = Q1;
+    d->ops[id].Q2 = Q2;
+    return ANUGA_OK;
+}
+
+int anuga_get_inlet_operator_stats(anuga_domain *d, int64_t id, double *stats)
+{
+    if (!d || !stats) return ANUGA_E_ARG;
+    if (id < 0 || id >= (int64_t)d->ops.size() || d->ops[id].kind != 2)
+        return fail(d, ANUGA_E_ARG, "no such inlet operator");
+    HIPCHK(d, hipMemcpyAsync(stats, d->ops[id].influx, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                             d->stream));
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    return ANUGA_OK;
+}
+
 int anuga_update_rate_operator(anuga_domain *d, int64_t id, const double *rates, double rate,
                                double factor)
 {
     if (!d) return ANUGA_E_ARG;
     if (id < 0 || id >= (int64_t)d->ops.size()) return fail(d, ANUGA_E_ARG, "no such operator");
+    if (d->ops[id].kind == 2)
+        return fail(d, ANUGA_E_ARG, "an inlet operator: use anuga_update_inlet_operator");
     HIPCHK(d, hipStreamSynchronize(d->stream));
     RateOp &o = d->ops[id];
     const bool had_array = o.rate != nullptr;

@@ -2113,6 +2113,185 @@ de_rate(const DeControl *ctl, int n, const int *__restrict__ idx, const double *
     atomicAdd(influx, lr * area[k]);
 }
 
+// ---------------------------------------------------------------------------
+// Inlet_operator (anuga/structures/inlet_operator.py:75-183) on the device:
+// one workgroup levels the region with the reference's numpy arithmetic
+// ---------------------------------------------------------------------------
+#define DE_INLET_MAX 2048
+#define DE_INLET_THREADS 1024
+
+// numpy's sum of a contiguous float64 array (pairwise_sum of
+// numpy/_core/src/umath/loops_utils.h.src): fewer than 8 in order from 0.0;
+// up to 128 in eight interleaved partial sums, combined as a tree, then the
+// rest in order; above 128 the halves at a multiple of 8, added.
+__device__ __noinline__ double de_np_leaf(const double *x, int n)
+{
+    if (n < 8) {
+        double r = 0.0;
+        for (int i = 0; i < n; i++) r += x[i];
+        return r;
+    }
+    double r[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) r[j] = x[j];
+    int i = 8;
+    for (; i < n - (n % 8); i += 8) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) r[j] += x[i + j];
+    }
+    double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; i++) res += x[i];
+    return res;
+}
+
+template <int D>
+__device__ double de_np_sum(const double *x, int n)
+{
+    if (n <= 128) return de_np_leaf(x, n);
+    int n2 = n / 2;
+    n2 -= n2 % 8;
+    return de_np_sum<D - 1>(x, n2) + de_np_sum<D - 1>(x + n2, n - n2);
+}
+template <>
+__device__ double de_np_sum<0>(const double *x, int n)
+{
+    return de_np_leaf(x, n);
+}
+
+// Inlet parameters: the discharge over the step is 0.5 (Q1 + Q2) (Q at the
+// operator's time and one timestep later); vmode 0: momenta from the
+// region's velocities, 1: from (u, v), 2: zero. stats: cumulative volume,
+// total applied volume, total requested volume, applied discharge.
+struct DeInlet {
+    int n;
+    const int *idx;       // [n] internal triangles, in the operator's order
+    const double *area;   // [n]
+    double total_area;    // np.sum(area)
+    double Q1, Q2;
+    int vmode;
+    double u, v;
+    double *stats;        // [4]
+};
+
+__global__ void __launch_bounds__(DE_INLET_THREADS)
+de_inlet(const DeControl *ctl, DeInlet in, DeQ *S)
+{
+    __shared__ double st[DE_INLET_MAX], el[DE_INLET_MAX], buf[DE_INLET_MAX];
+    __shared__ int order[DE_INLET_MAX];
+    __shared__ double level;
+    __shared__ int how, last;
+    if (ctl->done) return;
+    const int n = in.n, t = threadIdx.x;
+    const double vp = 1.0e-6;   // anuga/config.py velocity_protection
+    for (int i = t; i < n; i += DE_INLET_THREADS) {
+        const DeQ q = S[in.idx[i]];
+        st[i] = q.w;
+        el[i] = q.z;
+        buf[i] = (q.w - q.z) * in.area[i];
+    }
+    __syncthreads();
+    const double dt = ctl->dt;
+    // the step's volume (inlet_operator.py:86-96); the case is uniform
+    const double Q = 0.5 * (in.Q1 + in.Q2);
+    const double volume = Q * dt;
+    if (t == 0) {
+        const double cv = de_np_sum<6>(buf, n);   // np.sum(depths * areas)
+        how = volume >= 0.0 ? 0 : (cv + volume >= 0.0 ? 1 : 2);
+        level = cv;
+        if (how == 0) {
+            in.stats[0] += volume;     // fractional_step_volume_integral
+            in.stats[2] += volume;     // total_requested_volume
+            in.stats[1] += volume;     // total_applied_volume
+            in.stats[3] = Q;
+        } else if (how == 1) {
+            in.stats[2] += volume;
+            in.stats[0] += volume;
+            in.stats[1] += volume;
+            in.stats[3] = Q;
+        } else {
+            in.stats[2] += volume;
+            in.stats[0] -= cv;
+            in.stats[1] += -cv;
+            in.stats[3] = -cv / dt;
+        }
+    }
+    __syncthreads();
+    if (how == 0) {
+        // set_stages_evenly (inlet.py:192-225): stages.argsort(), ties by
+        // position; then the cumulative areas and volumes in that order
+        int np2 = 1;
+        while (np2 < n) np2 <<= 1;
+        for (int i = t; i < np2; i += DE_INLET_THREADS) {
+            order[i] = i < n ? i : -1;
+            if (i < n) buf[i] = in.area[i];
+        }
+        __syncthreads();
+        for (int size = 2; size <= np2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int i = t; i < np2; i += DE_INLET_THREADS) {
+                    const int j = i ^ stride;
+                    if (j > i) {
+                        const int a = order[i], b = order[j];
+                        // -1 (padding) sorts last
+                        const bool gt = a < 0 ? b >= 0
+                                      : (b >= 0 && (st[a] > st[b] || (st[a] == st[b] && a > b)));
+                        const bool up = (i & size) == 0;
+                        if (gt == up) {
+                            order[i] = b;
+                            order[j] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (t == 0) {
+            double sa = buf[order[0]], sv = 0.0, sa_at = sa, sv_at = 0.0;
+            int index = 0;
+            for (int i = 1; i < n; i++) {
+                const double nsv = sv + sa * (st[order[i]] - st[order[i - 1]]);
+                sa = sa + buf[order[i]];
+                sv = nsv;
+                if (!(sv <= volume)) break;
+                index = i;
+                sa_at = sa;
+                sv_at = sv;
+            }
+            const double depth = (volume - sv_at) / sa_at;
+            level = st[order[index]] + depth;
+            last = index;
+        }
+        __syncthreads();
+        for (int i = t; i <= last; i += DE_INLET_THREADS) st[order[i]] = level;
+    } else if (how == 1) {
+        const double depth = (level + volume) / in.total_area;
+        for (int i = t; i < n; i += DE_INLET_THREADS) st[i] = el[i] + depth;
+    } else {
+        for (int i = t; i < n; i += DE_INLET_THREADS) st[i] = el[i] + 0.0;
+    }
+    __syncthreads();
+    // momenta from the velocities before the change (inlet.py:122-128)
+    for (int i = t; i < n; i += DE_INLET_THREADS) {
+        const int k = in.idx[i];
+        DeQ q = S[k];
+        const double d0 = q.w - q.z, d1 = st[i] - el[i];
+        if (how == 2 || in.vmode == 2) {
+            q.uh = 0.0;
+            q.vh = 0.0;
+        } else if (in.vmode == 1) {
+            q.uh = d1 * in.u;
+            q.vh = d1 * in.v;
+        } else {
+            const double u = q.uh * d0 / (d0 * d0 + vp);
+            const double v = q.vh * d0 / (d0 * d0 + vp);
+            q.uh = d1 * u;
+            q.vh = d1 * v;
+        }
+        q.w = st[i];
+        S[k] = q;
+    }
+}
+
 // state of a subset of triangles (internal ids) to [n][4] {w, uh, vh, z}
 __global__ void __launch_bounds__(DE_BLOCK)
 de_gather_state(int n, const int *__restrict__ idx, const DeQ *__restrict__ S, DeQ *__restrict__ out)

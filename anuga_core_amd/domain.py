@@ -1064,7 +1064,7 @@ class Domain:
                     op.apply(t, self.timestep)
                     continue
                 if op.time_dependent:
-                    op.update(t)
+                    op.update(t, self.timestep)
                 self._call("anuga_apply_operator", op._id.value)
         self.relative_time = initial + self.timestep
         if self.record_dt_history:

@@ -127,7 +127,7 @@ class Rate_operator:
         f = self.factor
         return float(f(t)) if callable(f) else float(f)
 
-    def update(self, t):
+    def update(self, t, timestep=None):
         """Re-evaluate a time-dependent rate or factor at time t."""
         rates, scalar = self._rate_values(t)
         self._keep = (self._keep[0], rates)
@@ -217,7 +217,7 @@ class Set_stage_operator:
     def get_stage(self, t=None):
         return self._values(self.domain.get_time() if t is None else t)
 
-    def update(self, t):
+    def update(self, t, timestep=None):
         """Re-evaluate a time-dependent stage at time t."""
         values, scalar = self._values(t)
         self._keep = (self._keep[0], values)
@@ -242,18 +242,23 @@ class Inlet_operator:
     negative one lowers the region to a uniform depth, or empties it. The
     momenta follow the region's velocities (or `velocity`, or zero).
 
-    The region is small: each step its state is read from the device, the
-    reference's arithmetic is applied on the host (the same numpy calls, so
-    the same sort order and sums) and written back. A domain with inlets
-    takes host-sequenced steps. `region`: triangle indices, an object with
-    `indices`, or a polygon (triangles whose centroid lies inside)."""
+    Regions of up to 2048 triangles are levelled on the device, one
+    workgroup per inlet after every step (anuga_add_inlet_operator), with
+    numpy's summation order and the stages sorted with equal stages in
+    region order: the same bits as the reference's numpy wherever equal
+    stages have equal areas (otherwise the cumulative area may round in
+    another order). A constant Q keeps runs of steps on the device; a Q(t)
+    is evaluated on the host each step (two scalars uploaded). Larger
+    regions (or device=False) apply the reference's arithmetic on the host
+    to the region's state, read and written back each step. `region`:
+    triangle indices, an object with `indices`, a line, or a polygon
+    (triangles whose centroid lies inside)."""
 
-    time_dependent = True
-    host_side = True
     velocity_protection = 1.0e-6    # anuga/config.py velocity_protection
+    DEVICE_MAX = 2048
 
     def __init__(self, domain, region, Q=0.0, velocity=None, zero_velocity=False, default=0.0,
-                 description=None, label=None, logging=False, verbose=False):
+                 description=None, label=None, logging=False, verbose=False, device=True):
         self.domain = domain
         if hasattr(region, "indices"):
             idx = region.indices
@@ -280,7 +285,34 @@ class Inlet_operator:
         self.total_applied_volume = 0.0
         self.total_requested_volume = 0.0
         self.volume_integral = 0.0
+        self.host_side = not (device and len(self.triangle_indices) <= self.DEVICE_MAX)
+        self.time_dependent = self.host_side or callable(Q)
+        if not self.host_side:
+            vmode = 2 if zero_velocity else (1 if velocity is not None else 0)
+            u, v = (float(velocity[0]), float(velocity[1])) if velocity is not None else (0.0, 0.0)
+            q0 = 0.0 if callable(Q) else float(Q)
+            self._id = C.c_int64()
+            areas = np.ascontiguousarray(self.areas, np.float64)
+            domain._call("anuga_add_inlet_operator", len(self.triangle_indices),
+                         self.triangle_indices.ctypes.data_as(C.POINTER(C.c_int64)),
+                         areas.ctypes.data_as(C.POINTER(C.c_double)), C.c_double(self.area),
+                         vmode, C.c_double(u), C.c_double(v), C.c_double(q0), C.c_double(q0),
+                         C.byref(self._id))
         domain.fractional_step_operators.append(self)
+
+    def update(self, t, timestep=0.0):
+        """Q at the operator's time and a timestep later, for the device."""
+        domain = self.domain
+        Q1 = self.update_Q(t)
+        Q2 = self.update_Q(t + timestep)
+        domain._call("anuga_update_inlet_operator", self._id.value, C.c_double(Q1),
+                     C.c_double(Q2))
+
+    def _stats(self):
+        st = np.zeros(4)
+        self.domain._call("anuga_get_inlet_operator_stats", self._id.value,
+                          st.ctypes.data_as(C.POINTER(C.c_double)))
+        return st
 
     def update_Q(self, t):
         if callable(self.Q):
@@ -351,7 +383,23 @@ class Inlet_operator:
         return depths * u, depths * v
 
     def get_cumulative_influx(self):
+        if not self.host_side:
+            st = self._stats()
+            (self.volume_integral, self.total_applied_volume, self.total_requested_volume,
+             self.applied_Q) = (float(x) for x in st)
         return self.volume_integral
+
+    def get_total_applied_volume(self):
+        self.get_cumulative_influx()
+        return self.total_applied_volume
+
+    def get_total_requested_volume(self):
+        self.get_cumulative_influx()
+        return self.total_requested_volume
+
+    def get_applied_Q(self):
+        self.get_cumulative_influx()
+        return self.applied_Q
 
     def __repr__(self):
         return "Inlet_operator(%d triangles)" % len(self.triangle_indices)

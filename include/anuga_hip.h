@@ -177,6 +177,25 @@ int anuga_update_rate_operator(anuga_domain *d, int64_t id, const double *rates,
 int anuga_add_set_stage_operator(anuga_domain *d, int64_t n, const int64_t *indices,
                                  const double *stages, double stage, int64_t *id);
 int anuga_apply_operators(anuga_domain *d);
+
+/* Inlet_operator (anuga/structures/inlet_operator.py:75-183) on the device: a
+ * discharge 0.5 (Q1 + Q2) over each step into n <= 2048 triangles (indices in
+ * the caller's numbering, in the operator's order; areas and
+ * total_area = np.sum(areas) in that order). A positive volume fills the
+ * lowest stages first to a level surface (Inlet.set_stages_evenly,
+ * anuga/structures/inlet.py:192-225; stages sorted ascending, equal stages by
+ * position), a negative one lowers the region to a uniform depth or empties
+ * it; momenta from the region's velocities (velocity_mode 0), from (u, v)
+ * (1) or zero (2). The step's dt is the device's. Applied after every step,
+ * with the other operators, in the order added. */
+int anuga_add_inlet_operator(anuga_domain *d, int64_t n, const int64_t *indices,
+                             const double *areas, double total_area, int64_t velocity_mode,
+                             double u, double v, double Q1, double Q2, int64_t *id);
+/* Q at the operator's time and one timestep later, for the next application. */
+int anuga_update_inlet_operator(anuga_domain *d, int64_t id, double Q1, double Q2);
+/* stats[4]: cumulative volume (fractional_step_volume_integral's share), total
+ * applied volume, total requested volume, last applied discharge. Synchronizes. */
+int anuga_get_inlet_operator_stats(anuga_domain *d, int64_t id, double *stats);
 /* Apply one operator (host-sequenced steps that interleave host-side operators). */
 int anuga_apply_operator(anuga_domain *d, int64_t id);
 

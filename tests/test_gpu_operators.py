@@ -69,8 +69,8 @@ def test_time_dependent_rate_vs_oracle(alg):
 
 @pytest.mark.parametrize("alg", ["DE0", "DE1"])
 def test_inlets_vs_oracle(alg):
-    """Inlet_operator (host-side arithmetic on the device state of the inlet
-    triangles) next to a static rain operator, against the oracle."""
+    """Inlet_operator (levelled on the device) next to a static rain
+    operator, against the oracle's numpy restatement."""
     print("case inlets %s" % alg, flush=True)
     from anuga_core_amd.operators import Inlet_operator, _line_triangles
     from oracle.oracle_domain import OracleInletOperator
@@ -116,3 +116,50 @@ def test_set_stage_vs_oracle(alg):
     _run_and_compare(o, d, 1.0, 3.0)
     if alg == "DE1":   # rk2 applies operators at the step's end time
         assert np.all(d.get_quantity("stage").centroid_values[tide.indices] == f(d.get_time()))
+
+
+@pytest.mark.parametrize("alg", ["DE0", "DE1"])
+def test_device_inlets_match_host_inlets(alg):
+    """The device's inlet levelling (one workgroup per inlet, after every
+    step) against the same operators applied on the host to the region's
+    state read back (the reference's numpy arithmetic): the same bits on
+    this equal-area mesh, for a filling, a draining and a time-dependent
+    inlet with each velocity mode; a constant-Q domain keeps runs of steps
+    on the device."""
+    print("case device inlets %s" % alg, flush=True)
+    from anuga_core_amd.operators import Inlet_operator, _line_triangles
+    runs = []
+    for device in (True, False):
+        mesh, _, q, bc = S.dam_break_de0(m=16, n=12)
+        d = S.setup(Domain(mesh=mesh), alg, q, bc)
+        a = Inlet_operator(d, [[30.0, 20.0], [40.0, 20.0]], Q=2.0, device=device)
+        b = Inlet_operator(d, _line_triangles(mesh, [[10.0, 10.0], [25.0, 12.0]]), Q=-0.3,
+                           zero_velocity=True, device=device)
+        c = Inlet_operator(d, _line_triangles(mesh, [[45.0, 5.0], [45.0, 15.0]]),
+                           Q=lambda t: 0.5 + 0.2 * t, velocity=[0.1, -0.05], device=device)
+        if device:
+            assert not a.host_side and not a.time_dependent and c.time_dependent
+        for _ in d.evolve(yieldstep=1.0, finaltime=4.0):
+            pass
+        runs.append((list(d.dt_history), [d.get_quantity(n).centroid_values.copy() for n in
+                                          ("stage", "xmomentum", "ymomentum")],
+                     [op.get_cumulative_influx() for op in (a, b, c)]))
+        d.close()
+    assert runs[0][0] == runs[1][0]
+    for x, y in zip(runs[0][1], runs[1][1]):
+        assert np.array_equal(x, y)
+    assert runs[0][2] == runs[1][2]
+
+
+def test_constant_inlet_runs_on_the_device():
+    print("case device inlet device-stepping", flush=True)
+    from anuga_core_amd.operators import Inlet_operator
+    mesh, _, q, bc = S.dam_break_de0(m=16, n=12)
+    d = S.setup(Domain(mesh=mesh), "DE1", q, bc)
+    v0 = d.get_water_volume()
+    op = Inlet_operator(d, [[30.0, 20.0], [40.0, 20.0]], Q=2.0)
+    assert d._device_stepping()
+    d.evolve_steps(40)
+    added = op.get_cumulative_influx()
+    assert added == pytest.approx(2.0 * d.get_time(), rel=1e-12)
+    assert d.get_water_volume() - v0 == pytest.approx(added, rel=1e-9)
