@@ -238,7 +238,9 @@ __device__ __forceinline__ float de_rcp(float a) { return __builtin_amdgcn_rcpf(
 __device__ __forceinline__ double de_sqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ float de_sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 
-// limit_gradient (sw_domain_orig.c:1306-1333) applied to provisional jumps d
+// limit_gradient (sw_domain_orig.c:1306-1333) applied to provisional jumps d.
+// The two ratios exclude each other: the numerator is selected first, so a
+// wave whose lanes take both signs divides once per edge, not twice.
 template <class T>
 __device__ __forceinline__ void de_limit(T d[3], T qmin, T qmax, T beta)
 {
@@ -246,8 +248,8 @@ __device__ __forceinline__ void de_limit(T d[3], T qmin, T qmax, T beta)
     const T tiny = T(1.0e-100);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        if (d[i] < -tiny) r0 = de_div(qmin, d[i]);
-        if (d[i] > tiny) r0 = de_div(qmax, d[i]);
+        const T num = d[i] < -tiny ? qmin : qmax;
+        if (d[i] < -tiny || d[i] > tiny) r0 = de_div(num, d[i]);
         r = fmin(r0, r);
     }
     const T phi = fmin(r * beta, T(1.0));
