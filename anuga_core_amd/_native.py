@@ -117,6 +117,8 @@ SIGNATURES = {
     "anuga_update_centroids": (C.c_int, [C.c_void_p, C.c_double]),
     "anuga_fix_negative_cells": (C.c_int, [C.c_void_p, _ip]),
     "anuga_boundary_flux_integral_operator": (C.c_int, [C.c_void_p, C.c_double]),
+    "anuga_set_max_flux_update_frequency": (C.c_int, [C.c_void_p, C.c_int64]),
+    "anuga_compute_flux_update_frequency": (C.c_int, [C.c_void_p, C.c_double]),
     "anuga_evolve_steps": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(TimeControl),
                                      C.POINTER(StepState), _dp]),
     "anuga_get_boundary_flux_sums": (C.c_int, [C.c_void_p, _dp]),

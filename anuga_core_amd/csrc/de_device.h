@@ -127,6 +127,20 @@ struct DeTiles {
     const unsigned *__restrict__ edge;
 };
 
+// Local flux updating (max_flux_update_frequency > 1): per half-edge state,
+// [3][N] in the internal order (half-edge (k, i) at i * N + k). e0..e2: the
+// stored stage and momentum fluxes into k (length included), pg: the stored
+// pressure term; et: the edge timestep radius / max(speed, eps); fuf: the
+// update frequency (a power of two); acf: the flux call that last evaluated
+// the edge, its sign flipped by the frequency update (the reference's
+// already_computed_flux); unf: evaluate on the next call; tmin [N]: each
+// triangle's smallest frequency.
+struct DeLts {
+    double *e0, *e1, *e2, *pg, *et;
+    int *fuf, *acf, *tmin;
+    unsigned char *unf;
+};
+
 // one entry of a tile's edge list: the tile triangle t and its edge i, the
 // neighbour's position (tile triangle or halo slot) and edge j
 #define DE_TE_T(e) ((int)((e) & 1023u))

@@ -175,6 +175,16 @@ struct anuga_domain {
     double *si[2] = {nullptr, nullptr};
     bool backup_valid = false;
     int64_t op_flux_calls = 0;
+    // local flux updating (max_flux_update_frequency lts_M > 1): per-half-edge
+    // state on the device; the reference's static counters on the host (the
+    // flux call number, the step count modulo M from -1, the carried CFL
+    // timestep) and its allow_timestep_increase flag
+    int lts_M = 1;
+    int lts_call = 0;
+    long long lts_cyc = -1;
+    double lts_dt = 0.0;
+    int lts_allow = 1;
+    DeLts lts{};
     unsigned long long neg_seen = 0;
     std::string err;
 
@@ -1559,6 +1569,9 @@ static int evolve_steps(anuga_domain *d, int64_t max_steps, const anuga_time_con
     if (d->bsstride)
         return fail(d, ANUGA_E_STATE, "boundary values set per substep time need host-sequenced "
                                       "steps (anuga_compute_fluxes / anuga_update_...)");
+    if (d->lts_M > 1)
+        return fail(d, ANUGA_E_STATE, "local flux updating needs host-sequenced steps "
+                                      "(anuga_compute_flux_update_frequency after each timestep)");
     if (d->G) {   // every step records a gauge row: stop where the rows run out
         if (d->grows_host >= d->gcap)
             return fail(d, ANUGA_E_STATE, "gauge rows full: download them first");
@@ -1808,7 +1821,8 @@ void anuga_destroy(anuga_domain *d)
                     d->out_f32, d->gtri, d->gtimes, d->gvals, d->sub_idx, d->sub_val, d->eta, d->A, d->B, d->C, d->E, d->flags,
                     d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->chunks, d->ctl, d->dsetup,
                     d->dneg, d->hist, d->si[0], d->si[1], d->testart, d->tesplit, d->tedge, d->tv,
-                    d->node};
+                    d->node, d->lts.e0, d->lts.e1, d->lts.e2, d->lts.pg, d->lts.et, d->lts.fuf,
+                    d->lts.acf, d->lts.tmin, d->lts.unf};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (d->hctl) hipHostFree(d->hctl);
@@ -1837,6 +1851,12 @@ int anuga_set_params(anuga_domain *d, const anuga_params *params)
         return fail(d, ANUGA_E_ARG, "precision and reorder are fixed at creation");
     if (params->timestep_fluxcalls != d->P.timestep_fluxcalls && d->substate != 0)
         return fail(d, ANUGA_E_STATE, "cannot change the timestepping method inside a step");
+    if (d->lts_M > 1 && params->timestep_fluxcalls != 1)
+        return fail(d, ANUGA_E_STATE,
+                    "Local extrapolation and flux updating only supported with euler timestepping");
+    if (d->lts_M > 1 && (params->low_froude || params->use_sloped_mannings))
+        return fail(d, ANUGA_E_STATE, "local flux updating runs without low_froude or sloped "
+                                      "friction");
     if (std::memcmp(params, &d->P, sizeof(anuga_params)) != 0) drop_graph(d);
     // a new method restarts the per-operation flux substep count, as the
     // reference's static base_call does (sw_domain_orig.c:791-794)
@@ -2211,6 +2231,8 @@ int anuga_set_riverwalls(anuga_domain *d, const int64_t *edge_flux_type,
     int rc = single(d);
     if (rc) return rc;
     if (d->substate != 0) return fail(d, ANUGA_E_STATE, "inside a multi-stage step");
+    if (d->lts_M > 1 && number_of_riverwall_edges > 0)
+        return fail(d, ANUGA_E_STATE, "local flux updating runs without riverwalls");
     const int N = d->ntri[0];
     const int64_t R = number_of_riverwall_edges;
     if (R < 0 || R > 3 * (int64_t)N) return fail(d, ANUGA_E_ARG, "bad riverwall edge count");
@@ -2496,6 +2518,9 @@ int anuga_distribute_to_vertices_and_edges(anuga_domain *d, double *mass_error)
     return ANUGA_OK;
 }
 
+static int lts_fluxes(anuga_domain *d, double *flux_timestep);
+static int op_kend(const anuga_domain *d);
+
 int anuga_compute_fluxes(anuga_domain *d, double *flux_timestep)
 {
     if (!d) return ANUGA_E_ARG;
@@ -2505,6 +2530,12 @@ int anuga_compute_fluxes(anuga_domain *d, double *flux_timestep)
     rc = ready_to_step(d);
     if (rc) return rc;
     ensure_live(d);
+    if (d->lts_M > 1) {
+        launch_extrapolate(d, d->A);
+        rc = lts_fluxes(d, flux_timestep);
+        if (d->timing) timer_flush(d);
+        return rc;
+    }
     if (fused(d)) {
         launch_xflux(d, 0, d->A);
     } else {
@@ -2575,6 +2606,94 @@ int anuga_rk3_later_substeps(anuga_domain *d, double dt, int64_t *num_negative)
 {
     if (!d) return ANUGA_E_ARG;
     return later_substeps(d, dt, num_negative, 3);
+}
+
+// ---------------------------------------------------------------------------
+// local flux updating (sw_domain_orig.c:509-669 and the skip at :829)
+// ---------------------------------------------------------------------------
+// The flux call of a domain with local flux updating (euler: substep 0):
+// the edges due this call, the stored terms of the others; the CFL timestep
+// is the reference's static one, restarted from 1e100 only when the step
+// count allows the timestep to grow (:813-817).
+static int lts_fluxes(anuga_domain *d, double *flux_timestep)
+{
+    if (d->lts_call == INT32_MAX)
+        return fail(d, ANUGA_E_STATE, "local flux updating: too many flux calls");
+    d->lts_call++;
+    DeBoundary bc{d->bkind, d->bval, d->M, d->bsstride};
+    hipLaunchKernelGGL(de_flux_lts, dim3(d->nblocks), dim3(DE_BLOCK), 0, d->stream, dev_mesh(d),
+                       dev_params(d->P), d->A, (const double *)d->E, bc, d->flags, d->ex[0],
+                       d->ex[1], d->ex[2], d->lts, d->lts_call, d->ctl, d->part,
+                       d->part + d->nblocks);
+    launch_reduce(d, 0);
+    int rc = launch_check(d);
+    if (rc) return rc;
+    rc = sync_scalars(d);
+    if (rc) return rc;
+    const double fresh = d->hctl->flux_dt;
+    d->lts_dt = d->lts_allow == 1 ? fresh : fmin(d->lts_dt, fresh);
+    if (flux_timestep) *flux_timestep = d->lts_dt;
+    return ANUGA_OK;
+}
+
+int anuga_set_max_flux_update_frequency(anuga_domain *d, int64_t max_flux_update_frequency)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    const int64_t M = max_flux_update_frequency;
+    if (M < 1 || M > (int64_t(1) << 30) || (M & (M - 1)) != 0)
+        return fail(d, ANUGA_E_ARG, "max_flux_update_frequency must be a power of 2 (at most 2^30)");
+    if (M == 1) {
+        d->lts_M = 1;
+        return ANUGA_OK;
+    }
+    if (d->P.timestep_fluxcalls != 1)
+        return fail(d, ANUGA_E_STATE,
+                    "Local extrapolation and flux updating only supported with euler timestepping");
+    if (d->mixed || d->rw || d->P.low_froude || d->P.use_sloped_mannings)
+        return fail(d, ANUGA_E_STATE, "local flux updating runs the FP64 kernels without "
+                                      "riverwalls, low_froude or sloped friction");
+    const size_t n3 = 3 * (size_t)d->N;
+    HIPCHK(d, hipStreamSynchronize(d->stream));
+    if (!d->lts.e0) {
+        HIPCHK(d, dalloc(&d->lts.e0, n3));
+        HIPCHK(d, dalloc(&d->lts.e1, n3));
+        HIPCHK(d, dalloc(&d->lts.e2, n3));
+        HIPCHK(d, dalloc(&d->lts.pg, n3));
+        HIPCHK(d, dalloc(&d->lts.et, n3));
+        HIPCHK(d, dalloc(&d->lts.fuf, n3));
+        HIPCHK(d, dalloc(&d->lts.acf, n3));
+        HIPCHK(d, dalloc(&d->lts.tmin, (size_t)d->N));
+        HIPCHK(d, dalloc(&d->lts.unf, n3));
+    }
+    hipLaunchKernelGGL(de_lts_init, dim3((unsigned)((n3 + DE_BLOCK - 1) / DE_BLOCK)),
+                       dim3(DE_BLOCK), 0, d->stream, (int)n3, d->lts);
+    d->lts_M = (int)M;
+    d->lts_call = 0;
+    d->lts_cyc = -1;
+    d->lts_dt = 0.0;
+    d->lts_allow = 1;
+    return launch_check(d);
+}
+
+int anuga_compute_flux_update_frequency(anuga_domain *d, double timestep)
+{
+    if (!d) return ANUGA_E_ARG;
+    int rc = single(d);
+    if (rc) return rc;
+    if (d->lts_M == 1) return ANUGA_OK;
+    ensure_live(d);
+    d->lts_cyc++;
+    if (d->lts_cyc == d->lts_M) d->lts_cyc = 0;
+    const int kend = op_kend(d);
+    const dim3 grid((kend + DE_BLOCK - 1) / DE_BLOCK);
+    hipLaunchKernelGGL(de_lts_part1, grid, dim3(DE_BLOCK), 0, d->stream, kend, d->N, d->nb,
+                       d->lts, d->lts_cyc, d->lts_M, timestep);
+    hipLaunchKernelGGL(de_lts_part2, grid, dim3(DE_BLOCK), 0, d->stream, kend, d->N, d->nb,
+                       d->lts, d->lts_cyc + 1);
+    d->lts_allow = d->lts_cyc + 1 == d->lts_M ? 1 : 0;
+    return launch_check(d);
 }
 
 // ---------------------------------------------------------------------------
@@ -2651,6 +2770,13 @@ int anuga_compute_fluxes_ext_central(anuga_domain *d, double timestep, double *f
     if (d->e_stale)
         return fail(d, ANUGA_E_STATE,
                     "compute_fluxes needs the edge values of distribute_to_vertices_and_edges");
+    if (d->lts_M > 1) {
+        rc = lts_fluxes(d, flux_timestep);
+        if (rc) return rc;
+        if (d->timing) timer_flush(d);
+        d->op_flux_calls++;
+        return ANUGA_OK;
+    }
     const int sub = (int)(d->op_flux_calls % d->P.timestep_fluxcalls);
     launch_flux(d, 0, d->A);
     launch_reduce(d, sub == 0 ? 0 : (sub == 1 ? 1 : 4));

@@ -609,14 +609,13 @@ __device__ __forceinline__ void de_riverwall_weir(const DeRiverwall &rw, double 
 // kernel, so both produce the same bits. fnb: where the neighbour's flag is
 // (GENERAL walls; NULL: flags[n]).
 template <class T, int SUBSTEP, bool GENERAL>
-__device__ __forceinline__ void de_edge_flux(const DeMesh &m, const DeParams &p,
-                                             const DeBoundary &bc, const DeQ *q,
-                                             const unsigned char *__restrict__ flags, int N,
-                                             int k, int i, int vi, const DeCentroid &c, T hc,
-                                             T zc, T ql0, T hle, T ql1, T ql2, T qr0, T hre,
-                                             T qr1, T qr2, double &sw, double &sx, double &sy,
-                                             double &bflux, double &dtmin,
-                                             const unsigned char *fnb = nullptr)
+__device__ __forceinline__ void de_edge_terms(const DeMesh &m, const DeParams &p,
+                                              const DeBoundary &bc, const DeQ *q,
+                                              const unsigned char *__restrict__ flags, int N,
+                                              int k, int i, int vi, const DeCentroid &c, T hc,
+                                              T zc, T ql0, T hle, T ql1, T ql2, T qr0, T hre,
+                                              T qr1, T qr2, T ef[3], T &pg, T &speed,
+                                              const unsigned char *fnb = nullptr)
 {
     const T g = T(p.g), eps = T(p.eps);
     const T zl = ql0 - hle;
@@ -677,7 +676,7 @@ __device__ __forceinline__ void de_edge_flux(const DeMesh &m, const DeParams &p,
     }
     const T h_left = fmax(hle + zl - z_half, T(0.0));
     const T h_right = fmax(hre + zr - z_half, T(0.0));
-    T ef[3], speed, pflux;
+    T pflux;
     if constexpr (GENERAL)
         de_flux_central<T>(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1,
                            n2, eps, z_half, g, ef, speed, pflux, p.low_froude, mirrored);
@@ -703,8 +702,26 @@ __device__ __forceinline__ void de_edge_flux(const DeMesh &m, const DeParams &p,
     ef[0] *= length;
     ef[1] *= length;
     ef[2] *= length;
-    const T pg = length * (-g * T(0.5) * (h_left * h_left - hle * hle - (hle + hc) * (zl - zc))
-                           + pflux);
+    pg = length * (-g * T(0.5) * (h_left * h_left - hle * hle - (hle + hc) * (zl - zc)) + pflux);
+}
+
+// de_edge_terms added to triangle k's sums (sw_domain_orig.c:1107-1132, the
+// edge's flux, then its pressure terms along the normal), the boundary inflow
+// and the CFL bound
+template <class T, int SUBSTEP, bool GENERAL>
+__device__ __forceinline__ void de_edge_flux(const DeMesh &m, const DeParams &p,
+                                             const DeBoundary &bc, const DeQ *q,
+                                             const unsigned char *__restrict__ flags, int N,
+                                             int k, int i, int vi, const DeCentroid &c, T hc,
+                                             T zc, T ql0, T hle, T ql1, T ql2, T qr0, T hre,
+                                             T qr1, T qr2, double &sw, double &sx, double &sy,
+                                             double &bflux, double &dtmin,
+                                             const unsigned char *fnb = nullptr)
+{
+    T ef[3], pg, speed;
+    de_edge_terms<T, SUBSTEP, GENERAL>(m, p, bc, q, flags, N, k, i, vi, c, hc, zc, ql0, hle, ql1,
+                                       ql2, qr0, hre, qr1, qr2, ef, pg, speed, fnb);
+    const T n1 = m.nx[i * N + k], n2 = m.ny[i * N + k];
     sw += double(-ef[0]);
     sx += double(-ef[1]);
     sy += double(-ef[2]);
@@ -803,6 +820,211 @@ de_flux(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ eta,
         part_dt[lb] = a;
         part_bf[lb] = b;
     }
+}
+
+// ---------------------------------------------------------------------------
+// local flux updating (euler steps, FP64, the common options)
+// ---------------------------------------------------------------------------
+// compute_fluxes with local flux updating (sw_domain_orig.c:765-1150 with the
+// skip test at :829): each triangle evaluates those of its half-edges whose
+// update flag is set (both halves of an edge carry the same flag, and each
+// half's terms are the bits the reference mirrors to it), stores their terms,
+// edge timesteps and the call number, and sums the stored terms of all three
+// edges, fresh or not (:1107-1139). The CFL partials cover the evaluated
+// edges; the inflow partials every boundary edge's stored flux.
+__global__ void __launch_bounds__(DE_BLOCK)
+de_flux_lts(DeMesh m, DeParams p, const DeQ *q, const double *__restrict__ E, DeBoundary bc,
+            const unsigned char *__restrict__ flags, double *__restrict__ exw,
+            double *__restrict__ exuh, double *__restrict__ exvh, DeLts lt, int call,
+            DeControl *ctl, double *__restrict__ part_dt, double *__restrict__ part_bf)
+{
+    __shared__ double red_dt[DE_BLOCK / 64];
+    __shared__ double red_bf[DE_BLOCK / 64];
+    const int lb = xcd_block();
+    DeControl *cb = de_ctl(ctl, m.scen, lb);
+    if (cb->done) return;
+    const int N = m.N;
+    const int k = lb * DE_BLOCK + threadIdx.x;
+    double dtmin = 1.0e100, bflux = 0.0;
+
+    if (k < cb->kend) {
+        const DeCentroid c = de_centroid(q[k], flags[k]);
+        const int4 vk = m.nb[k];
+        const int v[3] = {vk.x, vk.y, vk.z};
+        const double *__restrict__ Ew = E;
+        const double *__restrict__ Eh = E + 3 * (size_t)N;
+        const double *__restrict__ Euh = E + 6 * (size_t)N;
+        const double *__restrict__ Evh = E + 9 * (size_t)N;
+        const double hc = de_height(c.w, c.z);
+        const double zc = c.z;
+        double sw = 0.0, sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int ki = i * N + k;
+            double e0, e1, e2, pg;
+            if (lt.unf[ki]) {
+                const double ql0 = Ew[ki], hle = Eh[ki], ql1 = Euh[ki], ql2 = Evh[ki];
+                double qr0 = 0.0, qr1 = 0.0, qr2 = 0.0, hre = 0.0;
+                if (v[i] >= 0) {
+                    const int nm = (v[i] & 3) * N + (v[i] >> 2);
+                    qr0 = Ew[nm];
+                    qr1 = Euh[nm];
+                    qr2 = Evh[nm];
+                    hre = Eh[nm];
+                }
+                double ef[3], speed;
+                de_edge_terms<double, 0, false>(m, p, bc, q, flags, N, k, i, v[i], c, hc, zc,
+                                                ql0, hle, ql1, ql2, qr0, hre, qr1, qr2, ef, pg,
+                                                speed);
+                e0 = -ef[0];
+                e1 = -ef[1];
+                e2 = -ef[2];
+                const double t = 1.0 / fmax(speed, p.eps);
+                const double et = m.radius[k] * t;
+                lt.e0[ki] = e0;
+                lt.e1[ki] = e1;
+                lt.e2[ki] = e2;
+                lt.pg[ki] = pg;
+                lt.et[ki] = et;
+                lt.acf[ki] = call;
+                if (speed > p.eps) dtmin = fmin(dtmin, et);
+            } else {
+                e0 = lt.e0[ki];
+                e1 = lt.e1[ki];
+                e2 = lt.e2[ki];
+                pg = lt.pg[ki];
+            }
+            const double n1 = m.nx[ki], n2 = m.ny[ki];
+            sw += e0;
+            sx += e1;
+            sy += e2;
+            if (v[i] < 0) bflux += e0;
+            sx -= n1 * pg;
+            sy -= n2 * pg;
+        }
+        const double inv_area = 1.0 / m.area[k];
+        exw[k] = sw * inv_area;
+        exuh[k] = sx * inv_area;
+        exvh[k] = sy * inv_area;
+    }
+
+    dtmin = de_wave_min(dtmin);
+    bflux = de_wave_sum(bflux);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red_dt[wv] = dtmin;
+        red_bf[wv] = bflux;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = red_dt[0], b = red_bf[0];
+        for (int i = 1; i < DE_BLOCK / 64; i++) {
+            a = fmin(a, red_dt[i]);
+            b += red_bf[i];
+        }
+        part_dt[lb] = a;
+        part_bf[lb] = b;
+    }
+}
+
+// The update frequency derived from an edge pair's timesteps
+// (sw_domain_orig.c:573-600): int(min(et / dt, M)) of each side, the smaller,
+// clamped to [1, M] and rounded down to a power of two.
+__device__ __forceinline__ int de_lts_frequency(double et_a, double et_b, bool pair, int M,
+                                                double timestep)
+{
+    const double notSoFast = 1.0;
+    long long f = (long long)fmin((et_a / timestep) * notSoFast, M * 1.0);
+    if (pair) f = (long long)fmin((double)(long long)fmin(et_b / timestep * notSoFast, M * 1.0),
+                                  (double)f);
+    if (f < 1) f = 1;
+    if (f > M) f = M;
+    for (long long ii = 2; ii < M; ii *= 2)
+        if (f > ii && f < 2 * ii) f = ii;
+    return (int)f;
+}
+
+// _compute_flux_update_frequency, part 1 (sw_domain_orig.c:540-608), for
+// triangle k's half-edges; cyc is the step count modulo M. The two halves of
+// an edge hold the same frequency and already_computed_flux, so each half
+// replays the reference's two visits of the pair on its own copy: with a
+// positive call number the first visit derives the frequency and flips the
+// sign, and a second visit, made only if the new frequency divides cyc,
+// flips it back; with a negative one the first visit flips and the second
+// flips back and derives it; 0 (never evaluated) changes nothing. A boundary
+// edge has one visit: it flips the sign and derives the frequency if the sign
+// was positive. Then part 2's first loop (:618-628): tmin[k], the smallest of
+// the three.
+__global__ void __launch_bounds__(DE_BLOCK)
+de_lts_part1(int kend, int N, const int4 *__restrict__ nb, DeLts lt, long long cyc, int M,
+             double timestep)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= kend) return;
+    const int4 vk = nb[k];
+    const int v[3] = {vk.x, vk.y, vk.z};
+    int fmin3 = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int ki = i * N + k;
+        int f = lt.fuf[ki];
+        if ((cyc & (long long)(f - 1)) == 0) {
+            int a = lt.acf[ki];
+            bool derive;
+            if (v[i] >= 0) {
+                derive = a != 0;
+            } else {
+                derive = a > 0;
+                a = -a;
+            }
+            if (derive) {
+                const double etb = v[i] >= 0 ? lt.et[(v[i] & 3) * N + (v[i] >> 2)] : 0.0;
+                f = de_lts_frequency(lt.et[ki], etb, v[i] >= 0, M, timestep);
+                if (v[i] >= 0 && a > 0 && (cyc & (long long)(f - 1)) != 0) a = -a;
+                lt.fuf[ki] = f;
+            }
+            lt.acf[ki] = a;
+        }
+        fmin3 = i == 0 ? f : min(fmin3, f);
+    }
+    lt.tmin[k] = fmin3;
+}
+
+// part 2's second loop (:636-656): each half-edge takes the smaller of its
+// two triangles' minima, and is evaluated on the next call when its frequency
+// divides the next step count.
+__global__ void __launch_bounds__(DE_BLOCK)
+de_lts_part2(int kend, int N, const int4 *__restrict__ nb, DeLts lt, long long cyc_next)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= kend) return;
+    const int4 vk = nb[k];
+    const int v[3] = {vk.x, vk.y, vk.z};
+    const int tk = lt.tmin[k];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int ki = i * N + k;
+        const int f = v[i] >= 0 ? min(tk, lt.tmin[v[i] >> 2]) : tk;
+        lt.fuf[ki] = f;
+        lt.unf[ki] = (cyc_next & (long long)(f - 1)) == 0 ? 1 : 0;
+    }
+}
+
+// initial state: nothing stored, every edge evaluated, frequency 1, edge
+// timesteps 1e100 (shallow_water_domain.py:404-426)
+__global__ void __launch_bounds__(DE_BLOCK)
+de_lts_init(int n3, DeLts lt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    lt.e0[i] = 0.0;
+    lt.e1[i] = 0.0;
+    lt.e2[i] = 0.0;
+    lt.pg[i] = 0.0;
+    lt.et[i] = 1.0e100;
+    lt.fuf[i] = 1;
+    lt.acf[i] = 0;
+    lt.unf[i] = 1;
 }
 
 // Tiled fused extrapolation + flux (FP64, any SUBSTEP; out never aliases q).
@@ -966,10 +1188,11 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
 }
 
 // One listed edge of a tile (de_xflux_e): its flux from the tile triangle's
-// side (HALO: the neighbour is in the halo; otherwise in the tile, and its
-// side's pressure term and CFL bound are formed here too), written over the
-// LDS edge values that only this entry reads.
-template <int SUBSTEP, bool VG, bool HALO>
+// side (the entry's halo bit: the neighbour is in the halo; otherwise in the
+// tile, and its side's pressure term and CFL bound are formed here too),
+// written over the LDS edge values that only this entry reads. The halo bit
+// is read at run time, so a wave can take edges of both kinds in one pass.
+template <int SUBSTEP, bool VG>
 __device__ __forceinline__ void de_tile_edge(const DeMesh &m, const DeParams &p, unsigned en,
                                              int k0, double *lds, double *hl,
                                              const double *hz, int hcap, double &dtmin)
@@ -978,10 +1201,11 @@ __device__ __forceinline__ void de_tile_edge(const DeMesh &m, const DeParams &p,
     const double g = p.g, eps = p.eps;
     const int N = m.N;
     const int ta = DE_TE_T(en), ia = DE_TE_I(en), nb = DE_TE_N(en), jb = DE_TE_J(en);
+    const bool halo = DE_TE_HALO(en);
     const int ka = k0 + ta;
     double *const A = lds + ia * DE_BLOCK + ta;
-    double *const R = HALO ? hl + jb * hcap + nb : lds + jb * DE_BLOCK + nb;
-    const int rs = HALO ? 3 * hcap : Q;
+    double *const R = halo ? hl + jb * hcap + nb : lds + jb * DE_BLOCK + nb;
+    const int rs = halo ? 3 * hcap : Q;
     const double ql0 = A[0], hle = A[Q], ql1 = A[2 * Q], ql2 = A[3 * Q];
     const double qr0 = R[0], hre = R[rs], qr1 = R[2 * rs], qr2 = R[3 * rs];
     double n1, n2, length;
@@ -1016,7 +1240,7 @@ __device__ __forceinline__ void de_tile_edge(const DeMesh &m, const DeParams &p,
         tt = 1.0 / fmax(speed, eps);
         dtmin = fmin(dtmin, m.radius[ka] * tt);
     }
-    if constexpr (!HALO) {
+    if (!halo) {
         // the other side: its own depths and bed, the same flux
         const double hcb = hz[nb], zcb = hz[DE_BLOCK + nb];
         const double pgb = length * (-g * 0.5 * (h_right * h_right - hre * hre -
@@ -1033,19 +1257,21 @@ __device__ __forceinline__ void de_tile_edge(const DeMesh &m, const DeParams &p,
 // formed from the vertices. Phases, with a barrier after each:
 //  1. every thread extrapolates its triangle into LDS (de_extrap_cell, as
 //     de_xflux);
-//  2. the first waves extrapolate the halo while the others evaluate the
-//     edges inside the tile, each from its lower-numbered side, which also
-//     forms the other side's pressure term (the reference's compute-once-and-
-//     mirror loop, sw_domain_orig.c:996-1009);
-//  3. the edges to the halo, from the tile's side;
+//  2. the first waves extrapolate the halo while each thread of the others
+//     evaluates one edge inside the tile, from its lower-numbered side, which
+//     also forms the other side's pressure term (the reference's compute-
+//     once-and-mirror loop, sw_domain_orig.c:996-1009);
+//  3. the tile's remaining edges and the edges to the halo (from the tile's
+//     side) in one pass over all the waves;
 //  4. each thread gathers its triangle's three edges in local order as
 //     de_xflux does: the same operations on the same values, so the same bits
 //     (the other side's flux is exactly the negation, its pressure flux and
 //     wave speed identical); boundary edges are evaluated here, from the
 //     untouched edge values.
 // Each listed edge's results overwrite the LDS edge values only it reads.
+// Four waves per SIMD (at most 128 VGPRs), which the LDS allows too.
 template <int SUBSTEP, bool VG = false>
-__global__ void __launch_bounds__(DE_BLOCK)
+__global__ void __launch_bounds__(DE_BLOCK, 4)
 de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restrict__ eta,
            DeTiles tl, DeBoundary bc, unsigned char *__restrict__ flags, double *__restrict__ exw,
            double *__restrict__ exuh, double *__restrict__ exvh, const DeQ *bk, DeQ *out,
@@ -1126,14 +1352,14 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
             }
         }
     } else {
-        for (int e = e0 + t - 64 * hw; e < es; e += DE_BLOCK - 64 * hw)
-            de_tile_edge<SUBSTEP, VG, false>(m, p, tl.edge[e], k0, lds, hl, hz, hcap, dtmin);
+        const int e = e0 + t - 64 * hw;
+        if (e < es) de_tile_edge<SUBSTEP, VG>(m, p, tl.edge[e], k0, lds, hl, hz, hcap, dtmin);
     }
     __syncthreads();
 
-    // phase 3: the edges to the halo
-    for (int e = es + t; e < e1; e += DE_BLOCK)
-        de_tile_edge<SUBSTEP, VG, true>(m, p, tl.edge[e], k0, lds, hl, hz, hcap, dtmin);
+    // phase 3: the tile's remaining edges, then the edges to the halo
+    for (int e = min(es, e0 + DE_BLOCK - 64 * hw) + t; e < e1; e += DE_BLOCK)
+        de_tile_edge<SUBSTEP, VG>(m, p, tl.edge[e], k0, lds, hl, hz, hcap, dtmin);
     __syncthreads();
 
     // phase 4: each triangle's three edges in local order

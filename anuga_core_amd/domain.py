@@ -202,6 +202,7 @@ class Domain:
         self.adjoint_record_serial = 0
 
         self._handle = C.c_void_p()
+        self._lts_on_device = 1
         self._create_device()
         self.set_flow_algorithm("DE0")
 
@@ -275,8 +276,21 @@ class Domain:
                                    int(self.use_sloped_mannings))
 
     def _sync_params(self):
+        """The parameters, and max_flux_update_frequency (also when set
+        directly, as the reference's C reads it from the domain). Only euler
+        steps call compute_flux_update_frequency (generic_domain.py:1960-1963),
+        so with rk2 / rk3 the reference's edge state keeps its initial values
+        and every flux is evaluated: the library gets 1 then. A change
+        restarts the edge state."""
         p = self._make_params()
+        M = int(self.max_flux_update_frequency) if self._timestep_fluxcalls == 1 else 1
+        if M == 1 and self._lts_on_device != 1:
+            self._call("anuga_set_max_flux_update_frequency", 1)
+            self._lts_on_device = 1
         self._call("anuga_set_params", C.byref(p))
+        if M != self._lts_on_device:
+            self._call("anuga_set_max_flux_update_frequency", M)
+            self._lts_on_device = M
 
     def _call(self, name, *args):
         L = _native.lib()
@@ -360,22 +374,22 @@ class Domain:
 
     def set_local_extrapolation_and_flux_updating(self, nlevels=8):
         """shallow_water_domain.py:1193-1215 with the reference's checks and
-        exceptions. Local flux updating itself (max_flux_update_frequency > 1,
-        sw_domain_orig.c:509-669) is not built on the device: it is refused
-        here, and evolve refuses a max_flux_update_frequency set directly,
-        rather than evolving with every flux updated every step."""
-        mfuf = 2 ** nlevels
-        if mfuf != 1:
+        exceptions: each edge's flux is then re-evaluated every 1, 2, 4, ...
+        2**nlevels steps by its edge timestep (sw_domain_orig.c:509-669), on
+        the device, with host-sequenced euler steps."""
+        self.max_flux_update_frequency = 2 ** nlevels
+        if self.max_flux_update_frequency != 1:
             if self.timestepping_method != "euler":
                 raise Exception("Local extrapolation and flux updating only supported with "
                                 "euler timestepping")
             if not self.get_using_discontinuous_elevation():
                 raise Exception("Local extrapolation and flux updating only supported for "
                                 "discontinuous flow algorithms")
-            raise NotImplementedError("local extrapolation and flux updating "
-                                      "(max_flux_update_frequency %d) is not implemented on "
-                                      "the device" % mfuf)
-        self.max_flux_update_frequency = mfuf
+        self._sync_params()
+
+    def compute_flux_update_frequency(self):
+        """shallow_water_domain.py:2718-2745 with the step's timestep."""
+        self._call("anuga_compute_flux_update_frequency", C.c_double(self.timestep))
 
     def set_timestepping_method(self, method):
         methods = {"euler": 1, "rk2": 2, "rk3": 3, 1: 1, 2: 2, 3: 3}
@@ -793,6 +807,8 @@ class Domain:
         self.compute_fluxes()
         self.compute_forcing_terms()
         self.update_timestep(yieldstep, finaltime)
+        if self.max_flux_update_frequency != 1:
+            self.compute_flux_update_frequency()
         self.update_conserved_quantities()
 
     def _per_operation_rk2(self, yieldstep, finaltime):
@@ -888,6 +904,8 @@ class Domain:
         self._update_boundary_slot(0, self.get_time())
         self._compute_fluxes()
         self.update_timestep(yieldstep, finaltime)
+        if self.max_flux_update_frequency != 1:
+            self.compute_flux_update_frequency()
         self._call("anuga_update_conserved_quantities", C.c_double(self.timestep), None)
 
     def evolve_one_rk2_step(self, yieldstep, finaltime):
@@ -952,10 +970,6 @@ class Domain:
                                  % self.get_boundary_tags())
         if not (0 <= self.beta_w <= 2.0):
             raise AssertionError("Attribute self.beta_w must be in the interval [0, 2]")
-        if self.max_flux_update_frequency != 1:
-            raise NotImplementedError("local extrapolation and flux updating "
-                                      "(max_flux_update_frequency %s) is not implemented on "
-                                      "the device" % self.max_flux_update_frequency)
         if outputstep is None:
             outputstep = yieldstep
         if yieldstep is None:
@@ -1021,6 +1035,7 @@ class Domain:
         """Runs of steps on the device unless something needs the host every
         step (time-dependent boundary values or operator rates)."""
         return (self.device_timestep and self.step_mode == "fused" and not self._time_bc and
+                (self.max_flux_update_frequency == 1 or self._timestep_fluxcalls != 1) and
                 not any(op.time_dependent for op in self.fractional_step_operators))
 
     def set_fractional_step_operator(self, operator):

@@ -347,6 +347,28 @@ int anuga_fix_negative_cells(anuga_domain *d, int64_t *num_negative);
  * anuga_get_boundary_flux_integral. */
 int anuga_boundary_flux_integral_operator(anuga_domain *d, double timestep);
 
+/* Local extrapolation and flux updating, euler steps only
+ * (set_local_extrapolation_and_flux_updating sets max_flux_update_frequency =
+ * 2**nlevels, shallow_water_domain.py:1193-1215). With it above 1,
+ * anuga_compute_fluxes and anuga_compute_fluxes_ext_central evaluate only the
+ * edges due this step, sum the stored terms of the others, and keep the CFL
+ * timestep from growing except every max_flux_update_frequency steps
+ * (_compute_fluxes_central, sw_domain_orig.c:779-829). A power of two up to
+ * 2^30; 1 switches it off. Setting it restarts the edge state and counters.
+ * FP64, without riverwalls, low_froude or sloped friction; while it is on,
+ * anuga_evolve_steps is refused (host-sequenced steps only) and so are rk2 /
+ * rk3 (ANUGA_E_STATE). */
+int anuga_set_max_flux_update_frequency(anuga_domain *d, int64_t max_flux_update_frequency);
+
+/* compute_flux_update_frequency(domain, timestep) (shallow_water_domain.py:2718-2745
+ * -> _compute_flux_update_frequency, sw_domain_orig.c:509-669), called after
+ * update_timestep with the step's timestep (generic_domain.py:1960-1963):
+ * each edge's update frequency from its edge timesteps, and the edges that
+ * the next flux call evaluates. The reference keeps the step count in a
+ * process-wide static; here the handle holds it. A no-op when
+ * max_flux_update_frequency is 1. */
+int anuga_compute_flux_update_frequency(anuga_domain *d, double timestep);
+
 /* Timestep control of Generic_Domain.update_timestep (generic_domain.py:2349-2415). */
 typedef struct anuga_time_control {
     double CFL;

@@ -13,6 +13,9 @@
  *   oracle_compute_fluxes       sw_domain_orig.c:765-1150 (edge computed once,
  *                               mirrored to the neighbour; riverwall edges
  *                               counted in loop order as at :828-837, :880-893)
+ *   oracle_compute_fluxes_lts   the same loop with local flux updating's skip
+ *                               test and static timestep (:784-829)
+ *   oracle_compute_flux_update_frequency sw_domain_orig.c:509-669
  *   oracle_compute_fluxes_gather same arithmetic, each triangle evaluating its
  *                               own three edges (OpenMP-parallel, deterministic;
  *                               riverwall index from edge_river_wall_counter as
@@ -285,13 +288,24 @@ static void right_state(const oracle_mesh *m, const oracle_state *s, int64_t k,
  * Returns the CFL timestep on substep 0, else `timestep`.
  * boundary_flux_sum[substep] accumulates the boundary inflow.
  */
-double oracle_compute_fluxes(const oracle_mesh *m, const oracle_params *p,
-                             oracle_state *s, int64_t substep, double timestep,
-                             double *boundary_flux_sum, double *edge_flux_work,
-                             double *pressuregrad_work, int64_t *done)
+/*
+ * The edge loop and the sums of _compute_fluxes_central. An edge is skipped
+ * when done[ki] == call (its mirror was set earlier in this call) or, with
+ * update_next_flux given, when update_next_flux[ki] != 1 (:829); skipped
+ * edges keep their stored edge_flux_work / pressuregrad_work / edge_timestep,
+ * which the sums read like fresh ones. *local_timestep is the reference's
+ * static local_timestep (:779): reset to 1e100 when `reset`, else lowered
+ * from its last value (:813-817).
+ */
+static double compute_fluxes_core(const oracle_mesh *m, const oracle_params *p,
+                                  oracle_state *s, int64_t substep, double timestep,
+                                  double *boundary_flux_sum, double *edge_flux_work,
+                                  double *pressuregrad_work, int64_t *done, int64_t call,
+                                  const int64_t *update_next_flux, double *local_ts,
+                                  int reset)
 {
     const int64_t N = m->N;
-    double local_timestep = 1.0e+100;
+    double local_timestep = reset ? 1.0e+100 : *local_ts;
     int64_t k, n, nm = 0, ki, rw_count = 0;
     int i;
 
@@ -299,7 +313,6 @@ double oracle_compute_fluxes(const oracle_mesh *m, const oracle_params *p,
     memset(s->ex_uh, 0, N * sizeof(double));
     memset(s->ex_vh, 0, N * sizeof(double));
     memset(s->max_speed, 0, N * sizeof(double));
-    memset(done, 0, 3 * N * sizeof(int64_t));
 
     for (k = 0; k < N; k++) {
         for (i = 0; i < 3; i++) {
@@ -307,7 +320,7 @@ double oracle_compute_fluxes(const oracle_mesh *m, const oracle_params *p,
             double z_half, h_left, h_right, speed, pflux, length, bsw, tmp;
             int is_rw;
             ki = 3 * k + i;
-            if (done[ki]) {
+            if (done[ki] == call || (update_next_flux && update_next_flux[ki] != 1)) {
                 if (m->edge_flux_type && m->edge_flux_type[ki] == 1) rw_count++;
                 continue;
             }
@@ -351,14 +364,14 @@ double oracle_compute_fluxes(const oracle_mesh *m, const oracle_params *p,
             edge_flux_work[3 * ki + 2] = -ef[2];
             bsw = length * (-p->g * 0.5 * (h_left * h_left - hle * hle - (hle + hc) * (zl - zc)) + pflux);
             pressuregrad_work[ki] = bsw;
-            done[ki] = 1;
+            done[ki] = call;
             if (n >= 0) {
                 edge_flux_work[3 * nm + 0] = ef[0];
                 edge_flux_work[3 * nm + 1] = ef[1];
                 edge_flux_work[3 * nm + 2] = ef[2];
                 bsw = length * (-p->g * 0.5 * (h_right * h_right - hre * hre - (hre + hc_n) * (zr - zc_n)) + pflux);
                 pressuregrad_work[nm] = bsw;
-                done[nm] = 1;
+                done[nm] = call;
             }
             if (substep == 0) {
                 tmp = 1.0 / fmax(speed, p->epsilon);
@@ -399,8 +412,137 @@ double oracle_compute_fluxes(const oracle_mesh *m, const oracle_params *p,
         s->ex_uh[k] *= inv_area;
         s->ex_vh[k] *= inv_area;
     }
+    *local_ts = local_timestep;
     if (substep == 0) timestep = local_timestep;
     return timestep;
+}
+
+double oracle_compute_fluxes(const oracle_mesh *m, const oracle_params *p,
+                             oracle_state *s, int64_t substep, double timestep,
+                             double *boundary_flux_sum, double *edge_flux_work,
+                             double *pressuregrad_work, int64_t *done)
+{
+    double lt;
+    memset(done, 0, 3 * m->N * sizeof(int64_t));
+    return compute_fluxes_core(m, p, s, substep, timestep, boundary_flux_sum, edge_flux_work,
+                               pressuregrad_work, done, 1, NULL, &lt, 1);
+}
+
+/*
+ * Local flux updating (max_flux_update_frequency > 1; the reference allows
+ * it with euler steps only, shallow_water_domain.py:1193-1215). lts holds the
+ * reference's per-domain arrays and its static counters.
+ */
+typedef struct {
+    int64_t max_flux_update_frequency;
+    int64_t call;                    /* _compute_fluxes_central's static call */
+    int64_t cyclic_number_of_steps;  /* _compute_flux_update_frequency's static, from -1 */
+    double local_timestep;           /* _compute_fluxes_central's static local_timestep */
+    int64_t allow_timestep_increase;
+    int64_t *already_computed_flux;  /* (3N) */
+    int64_t *flux_update_frequency;  /* (3N), from 1 */
+    int64_t *update_next_flux;       /* (3N), from 1 */
+    int64_t *update_extrapolation;   /* (N) */
+    double *edge_flux_work;          /* (9N) */
+    double *pressuregrad_work;       /* (3N) */
+} oracle_lts;
+
+/* compute_fluxes of a domain with local flux updating (substep 0 of euler):
+ * call++ (sw_domain_orig.c:789), then the edge loop above with the skip test
+ * and the static timestep reset only when allow_timestep_increase is 1. */
+double oracle_compute_fluxes_lts(const oracle_mesh *m, const oracle_params *p,
+                                 oracle_state *s, double timestep,
+                                 double *boundary_flux_sum, oracle_lts *l)
+{
+    l->call++;
+    return compute_fluxes_core(m, p, s, 0, timestep, boundary_flux_sum, l->edge_flux_work,
+                               l->pressuregrad_work, l->already_computed_flux, l->call,
+                               l->update_next_flux, &l->local_timestep,
+                               l->allow_timestep_increase == 1);
+}
+
+static int64_t mod_pow2(int64_t n, int64_t d)
+{
+    return (int64_t)((uint64_t)n & ((uint64_t)d - 1u));
+}
+
+/* _compute_flux_update_frequency (sw_domain_orig.c:509-669), in its loop
+ * order: part 1 re-derives each edge pair's frequency from its edge
+ * timesteps after a flux update (the sign of already_computed_flux marks the
+ * pair's first visit), part 2 takes the minimum over each triangle and then
+ * over each edge pair and sets update_next_flux for the next step. */
+void oracle_compute_flux_update_frequency(const oracle_mesh *m, const oracle_state *s,
+                                          double timestep, oracle_lts *l)
+{
+    const int64_t N = m->N, M = l->max_flux_update_frequency;
+    const double notSoFast = 1.0;
+    int64_t k, i, k3, ki, mm, n, nm = 0, ii, ii2, fuf;
+    int64_t *acf = l->already_computed_flux, *ff = l->flux_update_frequency;
+    if (M == 1) return;
+    l->cyclic_number_of_steps++;
+    if (l->cyclic_number_of_steps == M) l->cyclic_number_of_steps = 0;
+    for (k = 0; k < N; k++) {
+        for (i = 0; i < 3; i++) {
+            ki = 3 * k + i;
+            if (mod_pow2(l->cyclic_number_of_steps, ff[ki]) != 0) continue;
+            n = m->neighbours[ki];
+            if (n >= 0) {
+                mm = m->neighbour_edges[ki];
+                nm = 3 * n + mm;
+            }
+            if (acf[ki] > 0) {
+                acf[ki] *= -1;
+                if (n >= 0) acf[nm] *= -1;
+            } else {
+                acf[ki] *= -1;
+                if (n >= 0) acf[nm] *= -1;
+                continue;
+            }
+            fuf = (int64_t)fmin((s->edge_timestep[ki] / timestep) * notSoFast, M * 1.);
+            if (n >= 0)
+                fuf = (int64_t)fmin((double)(int64_t)fmin(s->edge_timestep[nm] / timestep * notSoFast,
+                                                          M * 1.),
+                                    (double)fuf);
+            if (fuf < 1) fuf = 1;
+            if (fuf > M) fuf = M;
+            ii = 2;
+            while (ii < M) {
+                ii2 = 2 * ii;
+                if ((fuf > ii) && (fuf < ii2)) {
+                    fuf = ii;
+                    continue;
+                }
+                ii = ii2;
+            }
+            ff[ki] = fuf;
+            if (n >= 0) ff[nm] = fuf;
+        }
+    }
+    for (k = 0; k < N; k++) {
+        k3 = 3 * k;
+        ii = (int64_t)fmin((double)ff[k3], fmin((double)ff[k3 + 1], (double)ff[k3 + 2]));
+        ff[k3] = (int64_t)fmin((double)ii, (double)ff[k3]);
+        ff[k3 + 1] = (int64_t)fmin((double)ii, (double)ff[k3 + 1]);
+        ff[k3 + 2] = (int64_t)fmin((double)ii, (double)ff[k3 + 2]);
+    }
+    for (k = 0; k < N; k++) {
+        l->update_extrapolation[k] = 0;
+        for (i = 0; i < 3; i++) {
+            ki = 3 * k + i;
+            n = m->neighbours[ki];
+            if (n >= 0) {
+                nm = 3 * n + m->neighbour_edges[ki];
+                ff[ki] = (int64_t)fmin((double)ff[ki], (double)ff[nm]);
+            }
+            if (mod_pow2(l->cyclic_number_of_steps + 1, ff[ki]) == 0) {
+                l->update_next_flux[ki] = 1;
+                l->update_extrapolation[k] = 1;
+            } else {
+                l->update_next_flux[ki] = 0;
+            }
+        }
+    }
+    l->allow_timestep_increase = (l->cyclic_number_of_steps + 1 == M) ? 1 : 0;
 }
 
 /*

@@ -94,9 +94,19 @@ def _state_arrays(d):
                 max_speed=d.max_speed, edge_timestep=d.edge_timestep)
 
 
+class _OLts(C.Structure):
+    _fields_ = [("max_flux_update_frequency", C.c_int64), ("call", C.c_int64),
+                ("cyclic_number_of_steps", C.c_int64), ("local_timestep", C.c_double),
+                ("allow_timestep_increase", C.c_int64),
+                ("already_computed_flux", _ip), ("flux_update_frequency", _ip),
+                ("update_next_flux", _ip), ("update_extrapolation", _ip),
+                ("edge_flux_work", _dp), ("pressuregrad_work", _dp)]
+
+
 class OracleKernels:
     """Kernels of oracle/de_oracle.c. `gather=True` uses the per-triangle
-    (OpenMP) flux loop instead of the edge-once-and-mirror loop."""
+    (OpenMP) flux loop instead of the edge-once-and-mirror loop; with local
+    flux updating on, the flux call is the edge-once loop with its skip test."""
 
     def __init__(self, gather=False):
         lib = _load(os.path.join(HERE, "liboracle.so"))
@@ -129,6 +139,34 @@ class OracleKernels:
             _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
             C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, C.c_int64]
         lib.oracle_rotate.argtypes = [_dp, C.c_double, C.c_double]
+        lib.oracle_compute_fluxes_lts.restype = C.c_double
+        lib.oracle_compute_fluxes_lts.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_double, _dp, C.c_void_p]
+        lib.oracle_compute_flux_update_frequency.argtypes = [C.c_void_p, C.c_void_p,
+                                                             C.c_double, C.c_void_p]
+        self.lts = None
+
+    def set_max_flux_update_frequency(self, M):
+        """Local flux updating with the reference's initial arrays
+        (shallow_water_domain.py:404-426, generic_domain.py:386) and fresh
+        static counters; M == 1 switches it off."""
+        if M == 1:
+            self.lts = None
+            return
+        N = self.d.N
+        a = dict(acf=np.zeros(3 * N, np.int64), fuf=np.ones(3 * N, np.int64),
+                 unf=np.ones(3 * N, np.int64), uex=np.ones(N, np.int64),
+                 efw=np.zeros(9 * N), pgw=np.zeros(3 * N))
+        self._lts_arrays = a
+        self.d.edge_timestep[:] = 1.0e100
+        self.lts = _OLts(M, 0, -1, 0.0, 1, _i(a["acf"]), _i(a["fuf"]), _i(a["unf"]),
+                         _i(a["uex"]), _d(a["efw"]), _d(a["pgw"]))
+
+    def compute_flux_update_frequency(self, timestep):
+        if self.lts is None:
+            return
+        self.lib.oracle_compute_flux_update_frequency(C.byref(self.mesh), C.byref(self.state),
+                                                      timestep, C.byref(self.lts))
 
     def bind(self, d):
         self.d = d
@@ -167,6 +205,10 @@ class OracleKernels:
     def compute_fluxes(self, substep, timestep):
         p = self._params()
         d = self.d
+        if self.lts is not None:
+            return self.lib.oracle_compute_fluxes_lts(
+                C.byref(self.mesh), C.byref(p), C.byref(self.state), timestep,
+                _d(d.boundary_flux_sum), C.byref(self.lts))
         if self.gather:
             return self.lib.oracle_compute_fluxes_gather(
                 C.byref(self.mesh), C.byref(p), C.byref(self.state), substep, timestep,
@@ -287,13 +329,26 @@ class _RefDomain(C.Structure):
 
 
 class RefKernels:
-    """Kernels of the reference's compiled sw_domain_orig.c and quantity.c."""
+    """Kernels of the reference's compiled sw_domain_orig.c and quantity.c.
+    `fresh=True` loads a private copy of libref_sw.so, so that its static
+    counters (the flux call number, the local-updating step count and
+    timestep, sw_domain_orig.c:523, :779-786) start from their initial values."""
 
-    # the library's static call counters are process-wide, so is this
+    # the shared library's static call counters are process-wide, so is this
     _last_fluxcalls = None
 
-    def __init__(self):
-        sw = _load(os.path.join(HERE, "_ref", "libref_sw.so"))
+    def __init__(self, fresh=False):
+        path = os.path.join(HERE, "_ref", "libref_sw.so")
+        if fresh:
+            import shutil
+            import tempfile
+            _load(path)     # the build check and its message
+            tmp = tempfile.mkdtemp(prefix="ref_sw_")
+            copy = os.path.join(tmp, "libref_sw_%d.so" % id(self))
+            shutil.copyfile(path, copy)
+            path = copy
+            self._last_fluxcalls = None     # this instance's own counters
+        sw = _load(path)
         q = _load(os.path.join(HERE, "_ref", "libref_quantity.so"))
         self.sw, self.q = sw, q
         sw._protect_new.restype = C.c_double
@@ -325,6 +380,9 @@ class RefKernels:
         q._saxpy_centroid_values.argtypes = [C.c_int64, C.c_double, C.c_double, _dp, _dp]
         q._backup_centroid_values.restype = C.c_int64
         q._backup_centroid_values.argtypes = [C.c_int64, _dp, _dp]
+        sw._compute_flux_update_frequency.restype = C.c_int64
+        sw._compute_flux_update_frequency.argtypes = [C.c_void_p, C.c_double]
+        self.lts_M = 1
 
     def bind(self, d):
         self.d = d
@@ -345,7 +403,7 @@ class RefKernels:
         D.boundary_length = M
         D.number_of_riverwall_edges = d.number_of_riverwall_edges
         D.ncol_riverwall_hydraulic_properties = d.ncol_riverwall_hydraulic_properties
-        D.max_flux_update_frequency = 1
+        D.max_flux_update_frequency = self.lts_M
         D.neighbours = _i(d.neighbours)
         D.neighbour_edges = _i(d.neighbour_edges)
         D.surrogate_neighbours = _i(d.surrogate_neighbours)
@@ -433,14 +491,31 @@ class RefKernels:
         # keeps the previous value, so (call - base_call) % fluxcalls == 1.
         self._params()
         if substep == 0:
-            fc = 3 if RefKernels._last_fluxcalls == 2 else 2
+            fc = 3 if self._last_fluxcalls == 2 else 2
         else:
-            if RefKernels._last_fluxcalls is None:
+            if self._last_fluxcalls is None:
                 raise RuntimeError("substep 1 requested before substep 0")
-            fc = RefKernels._last_fluxcalls
-        RefKernels._last_fluxcalls = fc
+            fc = self._last_fluxcalls
+        self._set_last_fluxcalls(fc)
         self.D.timestep_fluxcalls = fc
         return self.sw._compute_fluxes_central(C.byref(self.D), timestep)
+
+    def _set_last_fluxcalls(self, fc):
+        if "_last_fluxcalls" in self.__dict__:
+            self._last_fluxcalls = fc
+        else:
+            RefKernels._last_fluxcalls = fc
+
+    def set_max_flux_update_frequency(self, M):
+        """Local flux updating: the reference's arrays as bind() set them
+        (their initial values) and max_flux_update_frequency = M. The step
+        count is the library's static, so use a fresh instance per run."""
+        self.lts_M = M
+        self.d.edge_timestep[:] = 1.0e100
+        self.bind(self.d)
+
+    def compute_flux_update_frequency(self, timestep):
+        self.sw._compute_flux_update_frequency(C.byref(self.D), timestep)
 
     def friction(self):
         d = self.d

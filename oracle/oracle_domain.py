@@ -10,6 +10,10 @@ Restates (paths relative to the anuga_core tree):
   * evolve / _evolve_base shallow_water_domain.py:2300-2407,
                           anuga/abstract_2d_finite_volumes/generic_domain.py:1715-1912
   * euler / rk2 steps     generic_domain.py:1914-2051
+  * local flux updating   set_local_extrapolation_and_flux_updating
+                          (shallow_water_domain.py:1193-1215) and the
+                          compute_flux_update_frequency call of the euler step
+                          (generic_domain.py:1960-1963)
   * update_timestep       generic_domain.py:2349-2415
   * update_boundary       generic_domain.py:2288-2306 with
                           Reflective_boundary.evaluate_segment (boundaries.py:235-292),
@@ -224,6 +228,7 @@ class OracleDomain:
         self.max_smallsteps = MAX_SMALLSTEPS
         self.fixed_flux_timestep = None
         self.boundary_map = None
+        self.max_flux_update_frequency = 1
         self.kernels = kernels
         kernels.bind(self)
         self.set_flow_algorithm(flow_algorithm)
@@ -436,12 +441,24 @@ class OracleDomain:
         self.timestep = timestep
 
     def evolve_one_euler_step(self, yieldstep, finaltime):
+        """generic_domain.py:1914-1972"""
         self.distribute_to_vertices_and_edges()
         self.update_boundary()
         self.compute_fluxes(0)
         self.compute_forcing_terms()
         self.update_timestep(yieldstep, finaltime)
+        if self.max_flux_update_frequency != 1:
+            self.kernels.compute_flux_update_frequency(self.timestep)
         self.update_conserved_quantities()
+
+    def set_local_extrapolation_and_flux_updating(self, nlevels=8):
+        """shallow_water_domain.py:1193-1215"""
+        self.max_flux_update_frequency = 2 ** nlevels
+        if self.max_flux_update_frequency != 1:
+            if self.timestepping_method != "euler":
+                raise Exception("Local extrapolation and flux updating only supported with "
+                                "euler timestepping")
+        self.kernels.set_max_flux_update_frequency(self.max_flux_update_frequency)
 
     def evolve_one_rk2_step(self, yieldstep, finaltime):
         self.kernels.backup()

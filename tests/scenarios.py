@@ -83,6 +83,28 @@ def flat_bed_dirichlet():
     return mesh, "DE0", q, bc
 
 
+def inflow_over_slope(m=10, n=10, graded=False, fine_area=50.0):
+    """The set-up of the reference's local-flux-updating test
+    (test_local_extrapolation_and_flux_updating.py:22-59): a 100 m square,
+    bed -x/150, initially dry, n = 0.03, a Dirichlet [0, 0, 0] east boundary
+    letting water in, reflective elsewhere. The reference meshes it with
+    `triangle` (maximum area 200 m^2), which is absent here: a rectangular_cross
+    mesh, or the unstructured graded_square mesh of about that area."""
+    if graded:
+        from anuga_core_amd.mesh import graded_square
+        p, v, b = graded_square(100.0, fine_area, 2.0 * fine_area, (50.0, 0.0, 100.0, 100.0),
+                                seed=2)
+    else:
+        p, v, b = rectangular_cross(m, n, 100.0, 100.0)
+    mesh = Mesh(p, v, b)
+    x = mesh.centroid_coordinates[:, 0]
+    z = -x / 150.0
+    q = dict(elevation=z, stage=z.copy(), friction=np.full(mesh.number_of_triangles, 0.03))
+    bc = {"left": "reflective", "right": ("dirichlet", [0.0, 0.0, 0.0]), "top": "reflective",
+          "bottom": "reflective"}
+    return mesh, "DE0", q, bc
+
+
 def make_boundary(spec, domain):
     if isinstance(spec, tuple) and spec[0] == "dirichlet":
         return Dirichlet_boundary(spec[1])

@@ -76,10 +76,10 @@ def test_options_refused_on_mixed_and_gradients():
         d.adjoint_record(4, 0.05, [0, 1], 1)
 
 
-def test_local_flux_updating_is_refused():
+def test_local_flux_updating_checks():
     """set_local_extrapolation_and_flux_updating keeps the reference's checks
-    (shallow_water_domain.py:1193-1215); local flux updating itself is not
-    built, so it is refused instead of being ignored."""
+    (shallow_water_domain.py:1193-1215); the library refuses the options its
+    local-updating kernels do not carry instead of ignoring them."""
     from anuga_core_amd.domain import Domain
     from tests import scenarios as S
     mesh, _, q, bc = S.dam_break_de0(m=6, n=6)
@@ -87,10 +87,11 @@ def test_local_flux_updating_is_refused():
     with pytest.raises(Exception, match="only supported with euler"):
         d.set_local_extrapolation_and_flux_updating(nlevels=3)
     d.set_flow_algorithm("DE0")
-    with pytest.raises(NotImplementedError):
-        d.set_local_extrapolation_and_flux_updating(nlevels=3)
+    d.set_local_extrapolation_and_flux_updating(nlevels=3)
+    for _ in d.evolve(yieldstep=0.1, finaltime=0.2):
+        pass
     d.set_local_extrapolation_and_flux_updating(nlevels=0)     # 2**0 == 1: every step
-    d.max_flux_update_frequency = 4
-    with pytest.raises(NotImplementedError):
-        for _ in d.evolve(yieldstep=0.1, finaltime=0.2):
-            pass
+    d.close()
+    d = S.setup(Domain(mesh=mesh, precision="mixed"), "DE0", q, bc)
+    with pytest.raises(Exception, match="FP64"):
+        d.set_local_extrapolation_and_flux_updating(nlevels=3)

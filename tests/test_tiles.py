@@ -201,7 +201,16 @@ def replay_edge_once(nb, start, halo, slot, hcap, estart, edges):
                 for q in range(4):
                     lds[12 * BLOCK + (3 * q + i) * hs + s] = ("val", int(h), i, q)
         touched = set()
-        for en in edges[estart[b]:estart[b + 1]]:
+        # the kernel's split of the list: in phase 2 each thread of the waves
+        # not on the halo takes one entry inside the tile, phase 3 the rest
+        e0, e1 = int(estart[b]), int(estart[b + 1])
+        es = e0 + int(np.sum(((edges[e0:e1] >> 12) & 1) == 0))
+        hw = min((len(hb) + 63) // 64, BLOCK // 64 - 1)
+        p2 = list(range(e0, min(es, e0 + BLOCK - 64 * hw)))
+        p3 = list(range(min(es, e0 + BLOCK - 64 * hw), e1))
+        assert p2 + p3 == list(range(e0, e1))
+        assert all(((int(edges[e]) >> 12) & 1) == 0 for e in p2)
+        for en in edges[e0:e1]:
             en = int(en)
             t, i, isha, nn, j = en & 1023, (en >> 10) & 3, (en >> 12) & 1, (en >> 13) & 1023, \
                 (en >> 23) & 3
