@@ -856,13 +856,18 @@ ADJ_HD void adj_bwd_flux(const AdjView &m, const double *S, const unsigned char 
     adj_scatter_centroid(f, lc, lamS, zbar, N, k);
 }
 
-// Extrapolation of triangle k: lamE (its edge values) -> adjoints of the raw
-// state of k and its surrogate neighbours (and of their beds). K tangents per
-// pass (the four components of one stencil triangle in 4 / K passes): a
-// narrower dual keeps the long extrapolation in registers.
+// Extrapolation of triangle k: lamE (its edge values) -> the adjoints of the
+// raw state of k and of its surrogate neighbours (and of their beds), stored
+// per stencil slot: X[(4 slot + c) N + k], slot 0 for k itself (a boundary
+// edge's surrogate is k and counts there), slot 1 + i for the neighbour
+// across edge i, c over w, uh, vh, z. adj_gather_extrap then sums each
+// triangle's contributions in a fixed order, so the result does not depend on
+// the order in which triangles run. K tangents per pass (the four components
+// of one stencil triangle in 4 / K passes): a narrower dual keeps the long
+// extrapolation in registers.
 template <int K = 4>
 ADJ_HD void adj_bwd_extrap(const AdjView &m, const double *S, const unsigned char *fl,
-                           const double *lamE, int k, double *lamS, double *zbar)
+                           const double *lamE, int k, double *X)
 {
     typedef Dual<K> D;
     const int N = m.N;
@@ -877,6 +882,7 @@ ADJ_HD void adj_bwd_extrap(const AdjView &m, const double *S, const unsigned cha
             lo[4 * i + t] = lamE[(size_t)t * 3 * N + (size_t)i * N + k];
             any = any || lo[4 * i + t] != 0.0;
         }
+    for (int j = 0; j < 16; j++) X[(size_t)j * N + k] = 0.0;
     if (!any) return;
     unsigned f[4];
     for (int j = 0; j < 4; j++) f[j] = fl[id[j]];
@@ -916,10 +922,26 @@ ADJ_HD void adj_bwd_extrap(const AdjView &m, const double *S, const unsigned cha
             for (int i = 0; i < 3; i++)
                 a += lo[4 * i] * ew[i].d[c] + lo[4 * i + 1] * eh[i].d[c] +
                      lo[4 * i + 2] * euh[i].d[c] + lo[4 * i + 3] * evh[i].d[c];
-            const int cc = c0 + c;
-            if (a == 0.0) continue;
-            if (cc < 3) adj_add(lamS + (size_t)cc * N + t, a);
-            else adj_add(zbar + t, a);
+            X[(size_t)(4 * slot + c0 + c) * N + k] = a;
         }
+    }
+}
+
+// The extrapolation adjoints that reach triangle t (adj_bwd_extrap's X): its
+// own stencil's, then those of its neighbours across edges 0, 1, 2, added to
+// the state adjoint lamS and the bed adjoint zbar.
+ADJ_HD void adj_gather_extrap(const AdjView &m, const double *X, int t, double *lamS,
+                              double *zbar)
+{
+    const int N = m.N;
+    int v[3], sur[3];
+    adj_neighbours(m, t, v, sur);
+    for (int c = 0; c < 4; c++) {
+        double a = X[(size_t)c * N + t];
+        for (int i = 0; i < 3; i++)
+            if (v[i] >= 0) a += X[(size_t)(4 * (1 + (v[i] & 3)) + c) * N + (v[i] >> 2)];
+        if (a == 0.0) continue;
+        if (c < 3) lamS[(size_t)c * N + t] += a;
+        else zbar[t] += a;
     }
 }

@@ -1,11 +1,16 @@
 // Discrete adjoint of the FP64 DE1 step with a fixed timestep, on the device.
 //
 // One thread per triangle, as the forward. Each kernel applies one
-// vector-Jacobian product of de_adjoint.h to its triangle and adds the
-// results into the adjoints of the triangles it read (atomic FP64 adds: the
-// adjoint of a gather is a scatter). The forward quantities each product
-// needs (flags, edge values, explicit updates, the stage state) come from the
-// production kernels, recomputed step by step from checkpoints.
+// vector-Jacobian product of de_adjoint.h to its triangle. The adjoint of a
+// gather is a scatter; it is kept deterministic: the update and flux kernels
+// add only into their own triangle's entries, except the flux's adjoint of a
+// neighbour's edge value, an entry that receives exactly two contributions
+// into a zeroed array (a + b in either order); the extrapolation stores its
+// contributions per stencil slot and a second kernel sums each triangle's in
+// a fixed order. Gradients are therefore the same bits on every run. The
+// forward quantities each product needs (flags, edge values, explicit
+// updates, the stage state) come from the production kernels, recomputed
+// step by step from checkpoints.
 #include "de_adjoint.h"
 
 __device__ __forceinline__ int adj_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -46,11 +51,21 @@ adj_k_flux(AdjView m, const double *__restrict__ S, const unsigned char *__restr
 
 __global__ void __launch_bounds__(DE_BLOCK)
 adj_k_extrap(AdjView m, const double *__restrict__ S, const unsigned char *__restrict__ fl,
-             const double *__restrict__ lamE, double *lamS, double *zbar)
+             const double *__restrict__ lamE, double *__restrict__ X)
 {
     const int k = adj_tid();
     if (k >= m.N) return;
-    adj_bwd_extrap<ADJ_EXTRAP_K>(m, S, fl, lamE, k, lamS, zbar);
+    adj_bwd_extrap<ADJ_EXTRAP_K>(m, S, fl, lamE, k, X);
+}
+
+// the extrapolation adjoints stored per stencil slot, summed per triangle in
+// a fixed order (no two threads add to one entry)
+__global__ void __launch_bounds__(DE_BLOCK)
+adj_k_extrap_gather(AdjView m, const double *__restrict__ X, double *lamS, double *zbar)
+{
+    const int t = adj_tid();
+    if (t >= m.N) return;
+    adj_gather_extrap(m, X, t, lamS, zbar);
 }
 
 // out = a * in over n entries

@@ -205,12 +205,13 @@ struct AdjRun {
     double *E0 = nullptr, *E1 = nullptr, *ex0 = nullptr, *ex1 = nullptr;
     unsigned char *f0 = nullptr, *f1 = nullptr;
     double *lam = nullptr, *lamn = nullptr, *lam1 = nullptr, *lamE = nullptr, *lam_ex = nullptr;
+    double *lamX = nullptr;    // [16][N] the extrapolation adjoint per stencil slot
     double *zbar = nullptr, *nbar = nullptr, *rec = nullptr;
     int *gauges = nullptr;
     unsigned long long *neg = nullptr;   // fix_negative count of recomputed steps (ignored)
     ~AdjRun()
     {
-        void *p[] = {ckpt, seg, S1, E0, E1, ex0, ex1, f0, f1, lam, lamn, lam1, lamE, lam_ex,
+        void *p[] = {ckpt, seg, S1, E0, E1, ex0, ex1, f0, f1, lam, lamn, lam1, lamE, lam_ex, lamX,
                      zbar, nbar, rec, gauges, neg};
         for (void *q : p)
             if (q) hipFree(q);
@@ -1763,14 +1764,16 @@ static int adj_step(anuga_domain *d, AdjRun *a, const DeQ *X)
     HIPCHK(d, hipMemsetAsync(a->lamE, 0, 12 * N * sizeof(double), d->stream));
     hipLaunchKernelGGL(adj_k_flux, g, b, 0, d->stream, m, S1, a->f1, a->E1, a->lam_ex, a->lamE,
                        a->lam1, a->zbar);
-    hipLaunchKernelGGL(adj_k_extrap, g, b, 0, d->stream, m, S1, a->f1, a->lamE, a->lam1, a->zbar);
+    hipLaunchKernelGGL(adj_k_extrap, g, b, 0, d->stream, m, S1, a->f1, a->lamE, a->lamX);
+    hipLaunchKernelGGL(adj_k_extrap_gather, g, b, 0, d->stream, m, a->lamX, a->lam1, a->zbar);
     // first stage: S1 = advance(X)
     hipLaunchKernelGGL(adj_k_advance, g, b, 0, d->stream, m, Sx, a->f0, a->ex0, a->ex0 + N,
                        a->ex0 + 2 * N, a->lam1, 1.0, a->lamn, a->zbar, a->nbar, a->lam_ex);
     HIPCHK(d, hipMemsetAsync(a->lamE, 0, 12 * N * sizeof(double), d->stream));
     hipLaunchKernelGGL(adj_k_flux, g, b, 0, d->stream, m, Sx, a->f0, a->E0, a->lam_ex, a->lamE,
                        a->lamn, a->zbar);
-    hipLaunchKernelGGL(adj_k_extrap, g, b, 0, d->stream, m, Sx, a->f0, a->lamE, a->lamn, a->zbar);
+    hipLaunchKernelGGL(adj_k_extrap, g, b, 0, d->stream, m, Sx, a->f0, a->lamE, a->lamX);
+    hipLaunchKernelGGL(adj_k_extrap_gather, g, b, 0, d->stream, m, a->lamX, a->lamn, a->zbar);
     std::swap(a->lam, a->lamn);
     return launch_check(d);
 }
@@ -3224,6 +3227,7 @@ int anuga_adjoint_record(anuga_domain *d, int64_t nsteps, double dt, int64_t che
     ACHK(dalloc(&a->lamn, n3));
     ACHK(dalloc(&a->lam1, n3));
     ACHK(dalloc(&a->lamE, 4 * n3));
+    ACHK(dalloc(&a->lamX, 16 * (size_t)N));
     ACHK(dalloc(&a->lam_ex, n3));
     ACHK(dalloc(&a->zbar, N));
     ACHK(dalloc(&a->nbar, N));

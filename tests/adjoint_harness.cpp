@@ -75,7 +75,9 @@ struct Run {
         }
         for (int k = 0; k < N; k++)
             adj_bwd_flux(m, S, f.data(), E.data(), lam_ex.data(), k, lamE.data(), lamS, zbar);
-        for (int k = 0; k < N; k++) adj_bwd_extrap(m, S, f.data(), lamE.data(), k, lamS, zbar);
+        std::vector<double> X(16 * (size_t)N);
+        for (int k = 0; k < N; k++) adj_bwd_extrap(m, S, f.data(), lamE.data(), k, X.data());
+        for (int k = 0; k < N; k++) adj_gather_extrap(m, X.data(), k, lamS, zbar);
     }
     // lam (of the step's result, [3][N]) -> lam (of its start), given the start S
     void step_adjoint(const double *S, double *lam, double *zbar, double *nbar)
@@ -191,8 +193,11 @@ double harness_part(int N, int M, const int *nb, const double *cen, const double
         std::vector<double> Eo(12 * N);
         for (int k = 0; k < N; k++) adj_fwd_extrap(r.m, S, f.data(), k, Eo.data());
         for (int i = 0; i < 12 * N; i++) val += wt[i] * Eo[i];
-        if (lamS)
-            for (int k = 0; k < N; k++) adj_bwd_extrap(r.m, S, f.data(), wt, k, lamS, zbar);
+        if (lamS) {
+            std::vector<double> X(16 * (size_t)N);
+            for (int k = 0; k < N; k++) adj_bwd_extrap(r.m, S, f.data(), wt, k, X.data());
+            for (int k = 0; k < N; k++) adj_gather_extrap(r.m, X.data(), k, lamS, zbar);
+        }
     } else {
         for (int k = 0; k < N; k++) {
             double e[3];

@@ -3,7 +3,8 @@ DE1 run (anuga_adjoint_record / anuga_adjoint_gradient) against
 
 * the CPU driver of the same cell math (tests/adjoint_harness.cpp, itself
   checked against the oracle and finite differences): the recorded gauge
-  series bit for bit, the gradients to the rounding of the atomic sums;
+  series and the gradients bit for bit (the device sums every adjoint in a
+  fixed order), and the same bits on a second run;
 * central finite differences of the device forward and of the CPU oracle
   along random directions (steps 1e-6 .. 1e-9; see test_adjoint_cpu.py for
   why a step may straddle a kink of the limiter);
@@ -80,8 +81,14 @@ def test_gpu_adjoint_matches_cpu_driver(harness):
     assert np.array_equal(rec, hrec)
     assert np.array_equal(d.get_quantity("stage").centroid_values, hS[:, 0])
     for a, b in ((gn, hn), (gz, hz), (gs, hs)):
-        assert np.allclose(a, b, rtol=1e-9, atol=1e-12 * np.abs(b).max())
+        assert np.array_equal(a, b)
     assert np.abs(gn).max() > 0 and np.abs(gz).max() > 0
+    # a second run: the same bits
+    d.close()
+    d = _domain(mesh, q, bc, init)
+    d.adjoint_record(nsteps, dt, gauges, obs_every, checkpoint_every=5)
+    for a, b in zip(d.adjoint_gradient(dL), (gn, gz, gs)):
+        assert np.array_equal(a, b)
 
 
 def _cfl_limit(mesh, q, bc, init):
