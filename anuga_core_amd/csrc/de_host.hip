@@ -992,11 +992,12 @@ static int validate_mesh(anuga_domain *d, const anuga_mesh *mesh, int64_t N, int
 // triangles outside the tile, each once, in order of first appearance; per
 // triangle, each edge's slot in its tile's list (1023: the neighbour is in
 // the tile). No tiles (hcap 0) if a halo exceeds kMaxHalo
-// (the fused kernel's LDS is 96 B per tile triangle and per halo entry).
-// the fused FP64 kernel's LDS (96 B per tile and halo triangle, 1 more for
-// the general instantiation's flags, and the static reduction arrays) stays
-// within 64 KB per workgroup
-static const int kMaxHalo = std::min(320, (65536 - 97 * DE_BLOCK - 1024) / 97);
+// so that every fused kernel's LDS stays within 64 KB per workgroup: 96 B
+// per halo entry and up to 113 B per tile triangle (the once-per-edge
+// kernel's edge values, centroid depths and beds; the general
+// instantiation's flag byte), the static reduction arrays, and the halo
+// stride rounded up to a multiple of 32.
+static const int kMaxHalo = std::min(320, (65536 - 113 * DE_BLOCK - 1024) / 97 / 32 * 32);
 
 // The tile metadata of triangles in internal order (nb: packed neighbours of
 // one scenario starting at 0). Returns the largest halo, or -1 when a
