@@ -32,10 +32,10 @@ def hipcc():
 
 TUNED = os.path.join(HERE, "tuned.json")
 # compile-time parameters the autotuner chooses (anuga_core_amd/autotune.py)
-# (ADJ_*: tangents per pass of the adjoint's flux and extrapolation products,
-# measured by hand: flux 2 (220 VGPRs, 2 waves per SIMD) ran 3% faster than 4
-# (256 VGPRs + 118 AGPRs, 1 wave); extrapolation 2 or 1 ran 12-19% slower)
-TUNABLE = {"DE_BLOCK": 256, "DE_MIXED_WAVES": 5, "ADJ_FLUX_K": 2, "ADJ_EXTRAP_K": 4}
+# (ADJ_FLUX_K: tangents per pass of the adjoint's flux product, measured by
+# hand: 2 (220 VGPRs, 2 waves per SIMD) ran 3% faster than 4 (256 VGPRs + 118
+# AGPRs, 1 wave))
+TUNABLE = {"DE_BLOCK": 256, "DE_MIXED_WAVES": 5, "ADJ_FLUX_K": 2}
 
 
 def tuned_config():

@@ -856,6 +856,388 @@ ADJ_HD void adj_bwd_flux(const AdjView &m, const double *S, const unsigned char 
     adj_scatter_centroid(f, lc, lamS, zbar, N, k);
 }
 
+// ---- reverse-mode products of the extrapolation ----------------------------
+// The same derivatives as the dual-number instantiation of adj_extrap_cell
+// (the branches the forward takes; min / max pass the derivative of the
+// argument they return, ties to the first; the limiter factor held where
+// it is frozen), accumulated from the outputs back to the inputs in one
+// sweep instead of one forward pass per group of input tangents.
+
+// adj_limit's reverse: g (adjoint of the limited jumps, in) -> gd (of the
+// unlimited jumps d), gqmin, gqmax, gbeta (added to).
+ADJ_HD void adj_rev_limit(const double d[3], double qmin, double qmax, double beta,
+                          double freeze, const double g[3], double gd[3], double &gqmin,
+                          double &gqmax, double &gbeta)
+{
+    const double tiny = 1.0e-100;
+    double r = 1000.0, r0 = 1.0;
+    int rsrc = -1, r0src = -1, rnum = 0, r0num = 0;
+    for (int i = 0; i < 3; i++) {
+        if (d[i] < -tiny) {
+            r0 = qmin / d[i];
+            r0src = i;
+            r0num = 1;
+        }
+        if (d[i] > tiny) {
+            r0 = qmax / d[i];
+            r0src = i;
+            r0num = 2;
+        }
+        if (r0 <= r) {
+            r = r0;
+            rsrc = r0src;
+            rnum = r0num;
+        }
+    }
+    const double rb = r * beta;
+    const bool live = rb <= 1.0 &&
+                      !(freeze > 0.0 && fabs(d[0]) < freeze && fabs(d[1]) < freeze &&
+                        fabs(d[2]) < freeze);
+    const double phi = rb <= 1.0 ? rb : 1.0;
+    double gphi = 0.0;
+    for (int i = 0; i < 3; i++) {
+        gd[i] = g[i] * phi;
+        gphi += g[i] * d[i];
+    }
+    if (!live) return;
+    const double gr = gphi * beta;
+    gbeta += gphi * r;
+    if (rsrc < 0) return;
+    const double ib = 1.0 / d[rsrc];
+    const double gnum = gr * ib;
+    gd[rsrc] += -(r * gr) * ib;
+    if (rnum == 1) gqmin += gnum;
+    else gqmax += gnum;
+}
+
+// adj_aux_jumps' reverse: g (adjoint of d) -> gq[4] (qk, q0, q1, q2), gbeta
+// (added to)
+ADJ_HD void adj_rev_aux(double qk, double q0, double q1, double q2, double dx1, double dx2,
+                        double dy1, double dy2, double inv_area2, const double dxv[3],
+                        const double dyv[3], double beta, double freeze, const double g[3],
+                        double gq[4], double &gbeta)
+{
+    const double dq0 = q0 - qk, dq1 = q1 - q0, dq2 = q2 - q0;
+    double a = dy2 * dq1 - dy1 * dq2;
+    a *= inv_area2;
+    double b = dx1 * dq2 - dx2 * dq1;
+    b *= inv_area2;
+    double d[3];
+    for (int i = 0; i < 3; i++) d[i] = a * dxv[i] + b * dyv[i];
+    const double s1 = dq0 + dq1, s2 = dq0 + dq2;
+    // qmax = max(max(dq0, max(s1, s2)), 0), each max passing its first on ties
+    const int mx_in = s1 >= s2 ? 1 : 2;
+    const double mxv = mx_in == 1 ? s1 : s2;
+    const int mx_o = dq0 >= mxv ? 0 : mx_in;
+    const double mxo = mx_o == 0 ? dq0 : mxv;
+    const bool mx_live = mxo >= 0.0;
+    const double qmax = mx_live ? mxo : 0.0;
+    const int mn_in = s1 <= s2 ? 1 : 2;
+    const double mnv = mn_in == 1 ? s1 : s2;
+    const int mn_o = dq0 <= mnv ? 0 : mn_in;
+    const double mno = mn_o == 0 ? dq0 : mnv;
+    const bool mn_live = mno <= 0.0;
+    const double qmin = mn_live ? mno : 0.0;
+    double gd[3], gqmin = 0.0, gqmax = 0.0;
+    adj_rev_limit(d, qmin, qmax, beta, freeze, g, gd, gqmin, gqmax, gbeta);
+    double gdq0 = 0.0, gs1 = 0.0, gs2 = 0.0;
+    if (mx_live) {
+        if (mx_o == 0) gdq0 += gqmax;
+        else if (mx_o == 1) gs1 += gqmax;
+        else gs2 += gqmax;
+    }
+    if (mn_live) {
+        if (mn_o == 0) gdq0 += gqmin;
+        else if (mn_o == 1) gs1 += gqmin;
+        else gs2 += gqmin;
+    }
+    double ga = 0.0, gb = 0.0;
+    for (int i = 0; i < 3; i++) {
+        ga += gd[i] * dxv[i];
+        gb += gd[i] * dyv[i];
+    }
+    ga *= inv_area2;
+    gb *= inv_area2;
+    const double gdq1 = ga * dy2 - gb * dx2 + gs1;
+    const double gdq2 = gb * dx1 - ga * dy1 + gs2;
+    gdq0 += gs1 + gs2;
+    gq[0] -= gdq0;
+    gq[1] += gdq0 - gdq1 - gdq2;
+    gq[2] += gdq1;
+    gq[3] += gdq2;
+}
+
+// adj_line_jumps' reverse: g (adjoint of d) -> gqk, gq1 (added to); beta is a
+// constant there
+ADJ_HD void adj_rev_line(double qk, double q1, double dx2, double dy2, const double dxv[3],
+                         const double dyv[3], double beta, double freeze, const double g[3],
+                         double &gqk, double &gq1)
+{
+    const double dq1 = q1 - qk;
+    const double a = dq1 * dx2, b = dq1 * dy2;
+    double d[3];
+    for (int i = 0; i < 3; i++) d[i] = a * dxv[i] + b * dyv[i];
+    const bool up = dq1 >= 0.0;
+    const double qmin = up ? 0.0 : dq1, qmax = up ? dq1 : 0.0;
+    double gd[3], gqmin = 0.0, gqmax = 0.0, gbeta = 0.0;
+    adj_rev_limit(d, qmin, qmax, beta, freeze, g, gd, gqmin, gqmax, gbeta);
+    double gdq1 = up ? gqmax : gqmin;
+    for (int i = 0; i < 3; i++) gdq1 += (gd[i] * dxv[i]) * dx2 + (gd[i] * dyv[i]) * dy2;
+    gq1 += gdq1;
+    gqk -= gdq1;
+}
+
+// The vector-Jacobian product of adj_extrap_cell: lo[4 i + t] (adjoints of
+// ew, eh, euh, evh on edge i) -> gs[4 j + c] (adjoints of w, uh, vh, z of
+// stencil slot j).
+ADJ_HD void adj_extrap_vjp(const double w[4], const double uh[4], const double vh[4],
+                           const double z[4], const unsigned fl[4], const AdjExtrapGeo &gm,
+                           const AdjParams &p, const double lo[12], double gs[16])
+{
+    for (int j = 0; j < 16; j++) gs[j] = 0.0;
+    // forward values: the cells
+    const bool clamp0 = fl[0] & ADJ_F_CLAMP, keep0 = fl[0] & ADJ_F_KEEP;
+    const double cw = clamp0 ? z[0] : w[0];
+    const bool ch_live = cw - z[0] >= 0.0;
+    const double ch = ch_live ? cw - z[0] : 0.0;
+    double cu = 0.0, cv = 0.0, cinv = 0.0;
+    if (keep0) {
+        cinv = 1.0 / ch;
+        cu = uh[0] * cinv;
+        cv = vh[0] * cinv;
+    }
+    double nw[3], nh[3], nu[3], nv[3], ninv[3];
+    bool nh_live[3];
+    for (int i = 0; i < 3; i++) {
+        nh_live[i] = w[i + 1] - z[i + 1] >= 0.0;
+        nh[i] = nh_live[i] ? w[i + 1] - z[i + 1] : 0.0;
+        nw[i] = (fl[i + 1] & ADJ_F_CLAMP) ? z[i + 1] : w[i + 1];
+        nu[i] = nv[i] = ninv[i] = 0.0;
+        if (fl[i + 1] & ADJ_F_KEEP) {
+            ninv[i] = 1.0 / nh[i];
+            nu[i] = uh[i + 1] * ninv[i];
+            nv[i] = vh[i + 1] * ninv[i];
+        }
+    }
+    // forward values: the jumps (limited), as adj_extrap_cell forms them
+    double jw[3] = {0.0, 0.0, 0.0}, jh[3] = {0.0, 0.0, 0.0};
+    double ju[3] = {0.0, 0.0, 0.0}, jv[3] = {0.0, 0.0, 0.0};
+    bool constant = gm.nbound == 3, gw = true, gu = true, gv = true;
+    bool aux = false, line = false;
+    double area2 = 0.0, inv_area2 = 0.0, hfactor = 0.0, bw_ = 0.0, bu_ = 0.0, bv_ = 0.0;
+    double ldx2 = 0.0, ldy2 = 0.0;
+    int lsel = 0;
+    const double c_tmp = 1.0 / (0.3 - 0.1), d_tmp = 1.0 - (c_tmp * 0.3);
+    // hfactor's selections
+    int hmin_src = 0, hmax_src = 0;
+    double hmin = 0.0, hmax = 0.0;
+    bool t1_live = false, t2_live = false, t3_live = false, t4_live = false, m1_live = false;
+    bool m2_a = false, hf1_live = false, t5_live = false, t6_live = false, c_sel = false;
+    double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0, rA = 0.0, rB = 0.0, m1 = 0.0, m2 = 0.0;
+    double t5 = 0.0, t6 = 0.0, C = 0.0, hf1 = 0.0;
+    if (gm.nbound <= 1) {
+        area2 = gm.dy2 * gm.dx1 - gm.dy1 * gm.dx2;
+        if (area2 <= 0.0) {
+            constant = true;
+        } else {
+            aux = true;
+            // hmin = min(min(nh0, min(nh1, nh2)), ch); hmax likewise
+            {
+                const int ia = nh[1] <= nh[2] ? 1 : 2;
+                const int ib = nh[0] <= nh[ia] ? 0 : ia;
+                hmin_src = nh[ib] <= ch ? ib : 3;
+                hmin = hmin_src == 3 ? ch : nh[ib];
+                const int ja = nh[1] >= nh[2] ? 1 : 2;
+                const int jb = nh[0] >= nh[ja] ? 0 : ja;
+                hmax_src = nh[jb] >= ch ? jb : 3;
+                hmax = hmax_src == 3 ? ch : nh[jb];
+            }
+            t1_live = hmin >= 0.0;
+            t1 = t1_live ? hmin : 0.0;
+            t2_live = ch >= 1.0e-06;
+            t2 = t2_live ? ch : 1.0e-06;
+            rA = c_tmp * t1 / t2;
+            const double A = rA + d_tmp;
+            t3_live = ch >= 0.0;
+            t3 = t3_live ? ch : 0.0;
+            t4_live = hmax >= 1.0e-06;
+            t4 = t4_live ? hmax : 1.0e-06;
+            rB = c_tmp * t3 / t4;
+            const double B = rB + d_tmp;
+            m1_live = B <= 1.0;
+            m1 = m1_live ? B : 1.0;
+            m2_a = A <= m1;
+            m2 = m2_a ? A : m1;
+            hf1_live = !(0.0 >= m2);
+            hf1 = hf1_live ? m2 : 0.0;
+            t5_live = hmin - p.mah >= 0.0;
+            t5 = t5_live ? hmin - p.mah : 0.0;
+            t6_live = hmin >= 0.0;
+            t6 = (t6_live ? hmin : 0.0) + 1.0 * p.mah;
+            C = 1.2 * t5 / t6;
+            c_sel = C <= hf1;
+            hfactor = c_sel ? C : hf1;
+            inv_area2 = 1.0 / area2;
+            bw_ = p.bwd + (p.bw - p.bwd) * hfactor;
+            gw = bw_ > 0.0;
+            if (gw) {
+                adj_aux_jumps<double>(cw, nw[0], nw[1], nw[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2,
+                                      inv_area2, gm.dxv, gm.dyv, bw_, jw, p.freeze);
+                adj_aux_jumps<double>(ch, nh[0], nh[1], nh[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2,
+                                      inv_area2, gm.dxv, gm.dyv, bw_, jh, p.freeze);
+            }
+            bu_ = p.buhd + (p.buh - p.buhd) * hfactor;
+            gu = bu_ > 0.0;
+            if (gu)
+                adj_aux_jumps<double>(cu, nu[0], nu[1], nu[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2,
+                                      inv_area2, gm.dxv, gm.dyv, bu_, ju, p.freeze);
+            bv_ = p.bvhd + (p.bvh - p.bvhd) * hfactor;
+            gv = bv_ > 0.0;
+            if (gv)
+                adj_aux_jumps<double>(cv, nv[0], nv[1], nv[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2,
+                                      inv_area2, gm.dxv, gm.dyv, bv_, jv, p.freeze);
+        }
+    } else if (gm.nbound == 2) {
+        if (!(gm.interior[0] || gm.interior[1] || gm.interior[2])) {
+            constant = true;
+        } else {
+            line = true;
+            lsel = gm.interior[0] ? 0 : (gm.interior[1] ? 1 : 2);
+            const double la2 = gm.lx * gm.lx + gm.ly * gm.ly;
+            ldx2 = 1.0 / la2;
+            ldy2 = ldx2 * gm.ly;
+            ldx2 *= gm.lx;
+            adj_line_jumps<double>(cw, nw[lsel], ldx2, ldy2, gm.dxv, gm.dyv, p.bw, jw, p.freeze);
+            adj_line_jumps<double>(ch, nh[lsel], ldx2, ldy2, gm.dxv, gm.dyv, p.bw, jh, p.freeze);
+            adj_line_jumps<double>(cu, nu[lsel], ldx2, ldy2, gm.dxv, gm.dyv, p.bw, ju, p.freeze);
+            adj_line_jumps<double>(cv, nv[lsel], ldx2, ldy2, gm.dxv, gm.dyv, p.bw, jv, p.freeze);
+        }
+    }
+    // reverse: the edge values
+    const bool kw = !(constant || !gw), ku = !(constant || !gu), kv = !(constant || !gv);
+    double gcw = 0.0, gch = 0.0, gcu = 0.0, gcv = 0.0;
+    double gjw[3], gjh[3], gju[3], gjv[3];
+    for (int i = 0; i < 3; i++) {
+        const double eh = kw ? ch + jh[i] : ch;
+        const double eu = ku ? cu + ju[i] : cu;
+        const double ev = kv ? cv + jv[i] : cv;
+        const double lw = lo[4 * i], lh = lo[4 * i + 1], lu = lo[4 * i + 2], lv = lo[4 * i + 3];
+        const double geh = lh + lu * eu + lv * ev;
+        const double geu = lu * eh, gev = lv * eh;
+        gcw += lw;
+        gch += geh;
+        gcu += geu;
+        gcv += gev;
+        gjw[i] = kw ? lw : 0.0;
+        gjh[i] = kw ? geh : 0.0;
+        gju[i] = ku ? geu : 0.0;
+        gjv[i] = kv ? gev : 0.0;
+    }
+    // reverse: the jumps
+    double gnw[3] = {0.0, 0.0, 0.0}, gnh[3] = {0.0, 0.0, 0.0};
+    double gnu[3] = {0.0, 0.0, 0.0}, gnv[3] = {0.0, 0.0, 0.0};
+    if (aux) {
+        double ghf = 0.0;
+        if (gw) {
+            double gq[4] = {0.0, 0.0, 0.0, 0.0}, gb = 0.0;
+            adj_rev_aux(cw, nw[0], nw[1], nw[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2, inv_area2,
+                        gm.dxv, gm.dyv, bw_, p.freeze, gjw, gq, gb);
+            gcw += gq[0];
+            for (int i = 0; i < 3; i++) gnw[i] += gq[i + 1];
+            double gq2[4] = {0.0, 0.0, 0.0, 0.0};
+            adj_rev_aux(ch, nh[0], nh[1], nh[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2, inv_area2,
+                        gm.dxv, gm.dyv, bw_, p.freeze, gjh, gq2, gb);
+            gch += gq2[0];
+            for (int i = 0; i < 3; i++) gnh[i] += gq2[i + 1];
+            ghf += gb * (p.bw - p.bwd);
+        }
+        if (gu) {
+            double gq[4] = {0.0, 0.0, 0.0, 0.0}, gb = 0.0;
+            adj_rev_aux(cu, nu[0], nu[1], nu[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2, inv_area2,
+                        gm.dxv, gm.dyv, bu_, p.freeze, gju, gq, gb);
+            gcu += gq[0];
+            for (int i = 0; i < 3; i++) gnu[i] += gq[i + 1];
+            ghf += gb * (p.buh - p.buhd);
+        }
+        if (gv) {
+            double gq[4] = {0.0, 0.0, 0.0, 0.0}, gb = 0.0;
+            adj_rev_aux(cv, nv[0], nv[1], nv[2], gm.dx1, gm.dx2, gm.dy1, gm.dy2, inv_area2,
+                        gm.dxv, gm.dyv, bv_, p.freeze, gjv, gq, gb);
+            gcv += gq[0];
+            for (int i = 0; i < 3; i++) gnv[i] += gq[i + 1];
+            ghf += gb * (p.bvh - p.bvhd);
+        }
+        // hfactor -> ch, hmin, hmax
+        double ghmin = 0.0, ghmax = 0.0;
+        const double gC = c_sel ? ghf : 0.0, ghf1 = c_sel ? 0.0 : ghf;
+        // C = 1.2 t5 / t6
+        const double gt5 = gC * 1.2 / t6;
+        const double gt6 = -(gC * C) / t6;
+        if (t5_live) ghmin += gt5;
+        if (t6_live) ghmin += gt6;
+        const double gm2 = hf1_live ? ghf1 : 0.0;
+        const double gA = m2_a ? gm2 : 0.0, gm1 = m2_a ? 0.0 : gm2;
+        const double gB = m1_live ? gm1 : 0.0;
+        // A = c t1 / t2 + d, B = c t3 / t4 + d
+        const double gt1 = gA * c_tmp / t2, gt2 = -(gA * rA) / t2;
+        const double gt3 = gB * c_tmp / t4, gt4 = -(gB * rB) / t4;
+        if (t1_live) ghmin += gt1;
+        if (t2_live) gch += gt2;
+        if (t3_live) gch += gt3;
+        if (t4_live) ghmax += gt4;
+        if (hmin_src == 3) gch += ghmin;
+        else gnh[hmin_src] += ghmin;
+        if (hmax_src == 3) gch += ghmax;
+        else gnh[hmax_src] += ghmax;
+    } else if (line) {
+        double gqk = 0.0, gq1 = 0.0;
+        adj_rev_line(cw, nw[lsel], ldx2, ldy2, gm.dxv, gm.dyv, p.bw, p.freeze, gjw, gqk, gq1);
+        gcw += gqk;
+        gnw[lsel] += gq1;
+        gqk = gq1 = 0.0;
+        adj_rev_line(ch, nh[lsel], ldx2, ldy2, gm.dxv, gm.dyv, p.bw, p.freeze, gjh, gqk, gq1);
+        gch += gqk;
+        gnh[lsel] += gq1;
+        gqk = gq1 = 0.0;
+        adj_rev_line(cu, nu[lsel], ldx2, ldy2, gm.dxv, gm.dyv, p.bw, p.freeze, gju, gqk, gq1);
+        gcu += gqk;
+        gnu[lsel] += gq1;
+        gqk = gq1 = 0.0;
+        adj_rev_line(cv, nv[lsel], ldx2, ldy2, gm.dxv, gm.dyv, p.bw, p.freeze, gjv, gqk, gq1);
+        gcv += gqk;
+        gnv[lsel] += gq1;
+    }
+    // reverse: the cells -> the raw state of each slot
+    if (keep0) {
+        gs[1] += gcu * cinv;
+        gs[2] += gcv * cinv;
+        const double ginv = gcu * uh[0] + gcv * vh[0];
+        gch += -(ginv * cinv) * cinv;
+    }
+    if (ch_live) {
+        gcw += gch;
+        gs[3] -= gch;
+    }
+    if (clamp0) gs[3] += gcw;
+    else gs[0] += gcw;
+    for (int i = 0; i < 3; i++) {
+        const int j = 4 * (i + 1);
+        double gh = gnh[i];
+        if (fl[i + 1] & ADJ_F_KEEP) {
+            gs[j + 1] += gnu[i] * ninv[i];
+            gs[j + 2] += gnv[i] * ninv[i];
+            const double ginv = gnu[i] * uh[i + 1] + gnv[i] * vh[i + 1];
+            gh += -(ginv * ninv[i]) * ninv[i];
+        }
+        if (nh_live[i]) {
+            gs[j] += gh;
+            gs[j + 3] -= gh;
+        }
+        if (fl[i + 1] & ADJ_F_CLAMP) gs[j + 3] += gnw[i];
+        else gs[j] += gnw[i];
+    }
+}
+
 // Extrapolation of triangle k: lamE (its edge values) -> the adjoints of the
 // raw state of k and of its surrogate neighbours (and of their beds), stored
 // per stencil slot: X[(4 slot + c) N + k], slot 0 for k itself (a boundary
@@ -925,6 +1307,48 @@ ADJ_HD void adj_bwd_extrap(const AdjView &m, const double *S, const unsigned cha
             X[(size_t)(4 * slot + c0 + c) * N + k] = a;
         }
     }
+}
+
+// adj_bwd_extrap by reverse mode: the same stores into X from one sweep of
+// adj_extrap_vjp.
+ADJ_HD void adj_bwd_extrap_rev(const AdjView &m, const double *S, const unsigned char *fl,
+                               const double *lamE, int k, double *X)
+{
+    const int N = m.N;
+    int v[3], sur[3];
+    adj_neighbours(m, k, v, sur);
+    double lo[12];
+    bool any = false;
+    for (int i = 0; i < 3; i++)
+        for (int t = 0; t < 4; t++) {
+            lo[4 * i + t] = lamE[(size_t)t * 3 * N + (size_t)i * N + k];
+            any = any || lo[4 * i + t] != 0.0;
+        }
+    double out[16];
+    for (int j = 0; j < 16; j++) out[j] = 0.0;
+    if (any) {
+        const AdjExtrapGeo g = adj_geometry(m, k, v, sur);
+        const int id[4] = {k, sur[0], sur[1], sur[2]};
+        double w[4], uh[4], vh[4], z[4];
+        unsigned f[4];
+        for (int j = 0; j < 4; j++) {
+            w[j] = S[4 * id[j]];
+            uh[j] = S[4 * id[j] + 1];
+            vh[j] = S[4 * id[j] + 2];
+            z[j] = S[4 * id[j] + 3];
+            f[j] = fl[id[j]];
+        }
+        double gs[16];
+        adj_extrap_vjp(w, uh, vh, z, f, g, m.p, lo, gs);
+        // a boundary edge's surrogate is k itself: its slot's share is k's
+        for (int c = 0; c < 4; c++) out[c] = gs[c];
+        for (int i = 0; i < 3; i++)
+            for (int c = 0; c < 4; c++) {
+                if (v[i] >= 0) out[4 * (i + 1) + c] = gs[4 * (i + 1) + c];
+                else out[c] += gs[4 * (i + 1) + c];
+            }
+    }
+    for (int j = 0; j < 16; j++) X[(size_t)j * N + k] = out[j];
 }
 
 // The extrapolation adjoints that reach triangle t (adj_bwd_extrap's X): its

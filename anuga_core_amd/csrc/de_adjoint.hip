@@ -15,11 +15,8 @@
 
 __device__ __forceinline__ int adj_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
-// tangents per pass of the extrapolation's and the flux's adjoints (register
-// pressure against repeated primal work)
-#ifndef ADJ_EXTRAP_K
-#define ADJ_EXTRAP_K 4
-#endif
+// tangents per pass of the flux's adjoint (register pressure against
+// repeated primal work); the extrapolation's adjoint is a reverse sweep
 #ifndef ADJ_FLUX_K
 #define ADJ_FLUX_K 2
 #endif
@@ -55,7 +52,7 @@ adj_k_extrap(AdjView m, const double *__restrict__ S, const unsigned char *__res
 {
     const int k = adj_tid();
     if (k >= m.N) return;
-    adj_bwd_extrap<ADJ_EXTRAP_K>(m, S, fl, lamE, k, X);
+    adj_bwd_extrap_rev(m, S, fl, lamE, k, X);
 }
 
 // the extrapolation adjoints stored per stencil slot, summed per triangle in

@@ -76,7 +76,7 @@ struct Run {
         for (int k = 0; k < N; k++)
             adj_bwd_flux(m, S, f.data(), E.data(), lam_ex.data(), k, lamE.data(), lamS, zbar);
         std::vector<double> X(16 * (size_t)N);
-        for (int k = 0; k < N; k++) adj_bwd_extrap(m, S, f.data(), lamE.data(), k, X.data());
+        for (int k = 0; k < N; k++) adj_bwd_extrap_rev(m, S, f.data(), lamE.data(), k, X.data());
         for (int k = 0; k < N; k++) adj_gather_extrap(m, X.data(), k, lamS, zbar);
     }
     // lam (of the step's result, [3][N]) -> lam (of its start), given the start S
@@ -167,6 +167,24 @@ void harness_adjoint(int N, int M, const int *nb, const double *cen, const doubl
         if (G > 0 && s % obs_every == 0)
             for (int g = 0; g < G; g++) lam[gauges[g]] += dL[(s / obs_every - 1) * G + g];
         r.step_adjoint(states[s - 1].data(), lam, zbar, nbar);
+    }
+}
+
+// the extrapolation's vector-Jacobian product per stencil slot (X, [16][N])
+// for the edge-value adjoints wt ([4][3][N]), by forward-mode duals and by
+// the reverse sweep
+void harness_extrap_products(int N, int M, const int *nb, const double *cen, const double *gx,
+                             const double *nx, const double *ny, const double *len,
+                             const double *area, const double *eta, const signed char *bkind,
+                             const double *bval, const double *prm, const double *S,
+                             const double *wt, double *X_dual, double *X_rev)
+{
+    Run r = make_run(N, M, nb, cen, gx, nx, ny, len, area, eta, bkind, bval, prm, 0.1);
+    std::vector<unsigned char> f(N);
+    for (int k = 0; k < N; k++) f[k] = (unsigned char)adj_flags(r.m, S, k);
+    for (int k = 0; k < N; k++) {
+        adj_bwd_extrap(r.m, S, f.data(), wt, k, X_dual);
+        adj_bwd_extrap_rev(r.m, S, f.data(), wt, k, X_rev);
     }
 }
 
