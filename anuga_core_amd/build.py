@@ -32,10 +32,7 @@ def hipcc():
 
 TUNED = os.path.join(HERE, "tuned.json")
 # compile-time parameters the autotuner chooses (anuga_core_amd/autotune.py)
-# (ADJ_FLUX_K: tangents per pass of the adjoint's flux product, measured by
-# hand: 2 (220 VGPRs, 2 waves per SIMD) ran 3% faster than 4 (256 VGPRs + 118
-# AGPRs, 1 wave))
-TUNABLE = {"DE_BLOCK": 256, "DE_MIXED_WAVES": 5, "ADJ_FLUX_K": 2}
+TUNABLE = {"DE_BLOCK": 256, "DE_MIXED_WAVES": 5}
 
 
 def tuned_config():

@@ -1238,6 +1238,309 @@ ADJ_HD void adj_extrap_vjp(const double w[4], const double uh[4], const double v
     }
 }
 
+// ---- reverse-mode product of one edge's update terms -------------------------
+// adj_flux_central's reverse: gef (adjoints of the flux), gpf (of the pressure
+// flux) -> adjoints of its inputs, added to g[11] in the order ql0, ql1, ql2,
+// qr0, qr1, qr2, h_left, h_right, hle, hre, ze.
+ADJ_HD void adj_rev_flux_central(double ql0, double ql1, double ql2, double qr0, double qr1,
+                                 double qr2, double h_left, double h_right, double hle,
+                                 double hre, double n1, double n2, double eps, double ze,
+                                 double g, const double gef[3], double gpf, double gq[11])
+{
+    if (h_left == 0.0 && h_right == 0.0) return;
+    const double l1 = n1 * ql1 + n2 * ql2, l2 = (-n2) * ql1 + n1 * ql2;
+    const double r1 = n1 * qr1 + n2 * qr2, r2 = (-n2) * qr1 + n1 * qr2;
+    const bool lp = hle > 0.0, rp = hre > 0.0;
+    double tl = 0.0, ul = 0.0, uhl = 0.0, vhl = 0.0, tr = 0.0, ur = 0.0, uhr = 0.0, vhr = 0.0;
+    if (lp) {
+        tl = 1.0 / hle;
+        ul = l1 * tl;
+        uhl = h_left * ul;
+        vhl = h_left * tl * l2;
+    }
+    if (rp) {
+        tr = 1.0 / hre;
+        ur = r1 * tr;
+        uhr = h_right * ur;
+        vhr = h_right * tr * r2;
+    }
+    const double cl = sqrt(g * h_left), cr = sqrt(g * h_right);
+    const double a1 = ul + cl, a2 = ur + cr, b1 = ul - cl, b2 = ur - cr;
+    const bool smax_l = a1 >= a2, smin_l = b1 <= b2;
+    double s_max = smax_l ? a1 : a2, s_min = smin_l ? b1 : b2;
+    const bool smax_live = !(s_max < 0.0), smin_live = !(s_min > 0.0);
+    if (!smax_live) s_max = 0.0;
+    if (!smin_live) s_min = 0.0;
+    double gul = 0.0, guhl = 0.0, gvhl = 0.0, gur = 0.0, guhr = 0.0, gvhr = 0.0;
+    double gsmax = 0.0, gsmin = 0.0, ghl = 0.0, ghr = 0.0;
+    const double denom = s_max - s_min;
+    if (denom < eps) {
+        // ef = 0; pflux = 0.5 g 0.5 (hl^2 + hr^2)
+        const double c = 0.5 * g * 0.5;
+        gq[6] += gpf * c * (h_left + h_left);
+        gq[7] += gpf * c * (h_right + h_right);
+        return;
+    }
+    const double fl0 = ul * h_left, fl1 = ul * uhl, fl2 = ul * vhl;
+    const double fr0 = ur * h_right, fr1 = ur * uhr, fr2 = ur * vhr;
+    const double inv = 1.0 / denom;
+    const double sms = s_max * s_min;
+    const bool mr_q = qr0 >= ze, ml_q = ql0 >= ze;
+    const double mr = mr_q ? qr0 : ze, ml = ml_q ? ql0 : ze;
+    const double E0 = s_max * fl0 - s_min * fr0 + sms * (mr - ml);
+    const double E1 = s_max * fl1 - s_min * fr1 + 1.0 * sms * (uhr - uhl);
+    const double E2 = s_max * fl2 - s_min * fr2 + 1.0 * sms * (vhr - vhl);
+    const double Q = s_max * h_left * h_left - s_min * h_right * h_right;
+    const double P = 0.5 * g * Q;
+    // ef0 = e0, ef1 = n1 e1 - n2 e2, ef2 = n2 e1 + n1 e2
+    const double ge0 = gef[0], ge1 = n1 * gef[1] + n2 * gef[2];
+    const double ge2 = (-n2) * gef[1] + n1 * gef[2];
+    const double gE0 = ge0 * inv, gE1 = ge1 * inv, gE2 = ge2 * inv, gP = gpf * inv;
+    const double ginv = ge0 * E0 + ge1 * E1 + ge2 * E2 + gpf * P;
+    const double gden = -(ginv * inv) * inv;
+    gsmax += gden;
+    gsmin -= gden;
+    double gsms = 0.0;
+    gsmax += gE0 * fl0 + gE1 * fl1 + gE2 * fl2;
+    gsmin -= gE0 * fr0 + gE1 * fr1 + gE2 * fr2;
+    const double gfl0 = gE0 * s_max, gfl1 = gE1 * s_max, gfl2 = gE2 * s_max;
+    const double gfr0 = -gE0 * s_min, gfr1 = -gE1 * s_min, gfr2 = -gE2 * s_min;
+    gsms += gE0 * (mr - ml) + gE1 * (uhr - uhl) + gE2 * (vhr - vhl);
+    const double gm = gE0 * sms;
+    if (mr_q) gq[3] += gm;
+    else gq[10] += gm;
+    if (ml_q) gq[0] -= gm;
+    else gq[10] -= gm;
+    guhr += gE1 * sms;
+    guhl -= gE1 * sms;
+    gvhr += gE2 * sms;
+    gvhl -= gE2 * sms;
+    const double gQ = gP * (0.5 * g);
+    gsmax += gQ * (h_left * h_left);
+    gsmin -= gQ * (h_right * h_right);
+    ghl += gQ * (s_max * h_left + s_max * h_left);
+    ghr -= gQ * (s_min * h_right + s_min * h_right);
+    gsmax += gsms * s_min;
+    gsmin += gsms * s_max;
+    gul += gfl0 * h_left + gfl1 * uhl + gfl2 * vhl;
+    ghl += gfl0 * ul;
+    guhl += gfl1 * ul;
+    gvhl += gfl2 * ul;
+    gur += gfr0 * h_right + gfr1 * uhr + gfr2 * vhr;
+    ghr += gfr0 * ur;
+    guhr += gfr1 * ur;
+    gvhr += gfr2 * ur;
+    double gcl = 0.0, gcr = 0.0;
+    if (smax_live) {
+        if (smax_l) {
+            gul += gsmax;
+            gcl += gsmax;
+        } else {
+            gur += gsmax;
+            gcr += gsmax;
+        }
+    }
+    if (smin_live) {
+        if (smin_l) {
+            gul += gsmin;
+            gcl -= gsmin;
+        } else {
+            gur += gsmin;
+            gcr -= gsmin;
+        }
+    }
+    if (cl > 0.0) ghl += gcl * (0.5 / cl) * g;
+    if (cr > 0.0) ghr += gcr * (0.5 / cr) * g;
+    double gl1 = 0.0, gl2 = 0.0, gr1 = 0.0, gr2 = 0.0;
+    if (lp) {
+        const double ght = gvhl * l2;
+        gl2 += gvhl * (h_left * tl);
+        ghl += ght * tl;
+        double gtl = ght * h_left;
+        ghl += guhl * ul;
+        gul += guhl * h_left;
+        gl1 += gul * tl;
+        gtl += gul * l1;
+        gq[8] += -(gtl * tl) / hle;
+    }
+    if (rp) {
+        const double ght = gvhr * r2;
+        gr2 += gvhr * (h_right * tr);
+        ghr += ght * tr;
+        double gtr = ght * h_right;
+        ghr += guhr * ur;
+        gur += guhr * h_right;
+        gr1 += gur * tr;
+        gtr += gur * r1;
+        gq[9] += -(gtr * tr) / hre;
+    }
+    gq[1] += gl1 * n1 + gl2 * (-n2);
+    gq[2] += gl1 * n2 + gl2 * n1;
+    gq[4] += gr1 * n1 + gr2 * (-n2);
+    gq[5] += gr1 * n2 + gr2 * n1;
+    gq[6] += ghl;
+    gq[7] += ghr;
+}
+
+// The vector-Jacobian product of adj_edge_terms: go[5] (adjoints of the five
+// update terms) -> gL, gR, gC (adjoints of the own and neighbour edge values
+// w, h, uh, vh and of the centroid w, uh, vh, z), added to. The same
+// derivatives as the dual-number instantiation.
+ADJ_HD void adj_edge_vjp(const AdjView &m, int k, int i, const double L[4], const double R[4],
+                         const double C[4], int kind, const double dir[3], const double go[5],
+                         double gL[4], double gR[4], double gC[4])
+{
+    const size_t N = m.N, ki = (size_t)i * N + k;
+    const double n1 = m.nx[ki], n2 = m.ny[ki], len = m.len[ki];
+    const AdjParams &p = m.p;
+    const bool hc_live = C[0] - C[3] >= 0.0;
+    const double hc = hc_live ? C[0] - C[3] : 0.0, zc = C[3];
+    const double ql0 = L[0], hle = L[1], ql1 = L[2], ql2 = L[3];
+    const double zl = ql0 - hle;
+    double qr0, qr1, qr2, zr, hre;
+    double r1 = 0.0, r2 = 0.0;
+    bool hre_live = true;
+    if (kind < 0) {
+        qr0 = R[0];
+        qr1 = R[2];
+        qr2 = R[3];
+        hre = R[1];
+        zr = qr0 - hre;
+    } else {
+        if (kind == 0) {
+            r1 = -ql1 * n1 - ql2 * n2;
+            r2 = -ql1 * n2 + ql2 * n1;
+            qr0 = ql0;
+            qr1 = n1 * r1 - n2 * r2;
+            qr2 = n2 * r1 + n1 * r2;
+        } else if (kind == 1) {
+            qr0 = ql0;
+            qr1 = ql1;
+            qr2 = ql2;
+        } else if (kind == 2) {
+            qr0 = dir[0];
+            qr1 = dir[1];
+            qr2 = dir[2];
+        } else {
+            qr0 = C[0];
+            qr1 = C[1];
+            qr2 = C[2];
+        }
+        zr = zl;
+        hre_live = qr0 - zr >= 0.0;
+        hre = hre_live ? qr0 - zr : 0.0;
+    }
+    const bool zh_l = zl >= zr;
+    const double z_half = zh_l ? zl : zr;
+    const double hl_in = hle + zl - z_half, hr_in = hre + zr - z_half;
+    const bool hl_live = hl_in >= 0.0, hr_live = hr_in >= 0.0;
+    const double h_left = hl_live ? hl_in : 0.0, h_right = hr_live ? hr_in : 0.0;
+    // forward pressure flux (needed by nothing below but its adjoint's path)
+    // out = -ef len (3), n1 pg, n2 pg; pg = len (K X + pflux), K = -g 0.5
+    const double gpg = go[3] * n1 + go[4] * n2;
+    const double gef[3] = {-go[0] * len, -go[1] * len, -go[2] * len};
+    const double K = -p.g * 0.5;
+    const double gX = (gpg * len) * K, gpf = gpg * len;
+    double gq[11] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    adj_rev_flux_central(ql0, ql1, ql2, qr0, qr1, qr2, h_left, h_right, hle, hre, n1, n2, p.eps,
+                         z_half, p.g, gef, gpf, gq);
+    // X = h_left^2 - hle^2 - (hle + hc)(zl - zc)
+    double ghl = gq[6] + gX * (h_left + h_left);
+    double ghle = gq[8] - gX * (hle + hle) - gX * (zl - zc);
+    const double ghc = -gX * (zl - zc);
+    double gzl = -gX * (hle + hc);
+    const double gzc = gX * (hle + hc);
+    double ghr = gq[7], ghre = gq[9], gzh = gq[10], gzr = 0.0;
+    double gql0 = gq[0], gql1 = gq[1], gql2 = gq[2], gqr0 = gq[3], gqr1 = gq[4], gqr2 = gq[5];
+    if (hr_live) {
+        ghre += ghr;
+        gzr += ghr;
+        gzh -= ghr;
+    }
+    if (hl_live) {
+        ghle += ghl;
+        gzl += ghl;
+        gzh -= ghl;
+    }
+    if (zh_l) gzl += gzh;
+    else gzr += gzh;
+    if (kind < 0) {
+        gR[0] += gqr0 + gzr;
+        gR[1] += ghre - gzr;
+        gR[2] += gqr1;
+        gR[3] += gqr2;
+    } else {
+        if (hre_live) {
+            gqr0 += ghre;
+            gzr -= ghre;
+        }
+        gzl += gzr;
+        if (kind == 0) {
+            gql0 += gqr0;
+            const double gr1 = gqr1 * n1 + gqr2 * n2;
+            const double gr2 = -gqr1 * n2 + gqr2 * n1;
+            gql1 += -gr1 * n1 - gr2 * n2;
+            gql2 += -gr1 * n2 + gr2 * n1;
+        } else if (kind == 1) {
+            gql0 += gqr0;
+            gql1 += gqr1;
+            gql2 += gqr2;
+        } else if (kind != 2) {
+            gC[0] += gqr0;
+            gC[1] += gqr1;
+            gC[2] += gqr2;
+        }
+    }
+    gql0 += gzl;
+    ghle -= gzl;
+    gL[0] += gql0;
+    gL[1] += ghle;
+    gL[2] += gql1;
+    gL[3] += gql2;
+    if (hc_live) {
+        gC[0] += ghc;
+        gC[3] -= ghc;
+    }
+    gC[3] += gzc;
+}
+
+// adj_bwd_flux by reverse mode: the same additions from one sweep of
+// adj_edge_vjp per edge.
+ADJ_HD void adj_bwd_flux_rev(const AdjView &m, const double *S, const unsigned char *fl,
+                             const double *E, const double *lam_ex, int k, double *lamE,
+                             double *lamS, double *zbar)
+{
+    const int N = m.N;
+    int v[3], sur[3];
+    adj_neighbours(m, k, v, sur);
+    double q[4] = {S[4 * k], S[4 * k + 1], S[4 * k + 2], S[4 * k + 3]}, c[4];
+    const unsigned f = fl[k];
+    adj_centroid<double>(q, f, c);
+    const double inv_area = 1.0 / m.area[k];
+    const double lw = lam_ex[k] * inv_area, lx = lam_ex[N + k] * inv_area;
+    const double ly = lam_ex[2 * (size_t)N + k] * inv_area;
+    const double lo[5] = {lw, lx, ly, -lx, -ly};
+    double lc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < 3; i++) {
+        double L[4], R[4], dir[3];
+        int kind;
+        adj_edge_inputs(m, E, k, i, v[i], L, R, kind, dir);
+        double gL[4] = {0.0, 0.0, 0.0, 0.0}, gR[4] = {0.0, 0.0, 0.0, 0.0};
+        double gC[4] = {0.0, 0.0, 0.0, 0.0};
+        adj_edge_vjp(m, k, i, L, R, c, kind, dir, lo, gL, gR, gC);
+        const size_t own = (size_t)i * N + k;
+        for (int t = 0; t < 4; t++)
+            if (gL[t] != 0.0) adj_add(lamE + (size_t)t * 3 * N + own, gL[t]);
+        if (kind < 0) {
+            const size_t nbs = (size_t)(v[i] & 3) * N + (v[i] >> 2);
+            for (int t = 0; t < 4; t++)
+                if (gR[t] != 0.0) adj_add(lamE + (size_t)t * 3 * N + nbs, gR[t]);
+        }
+        for (int t = 0; t < 4; t++) lc[t] += gC[t];
+    }
+    adj_scatter_centroid(f, lc, lamS, zbar, N, k);
+}
+
 // Extrapolation of triangle k: lamE (its edge values) -> the adjoints of the
 // raw state of k and of its surrogate neighbours (and of their beds), stored
 // per stencil slot: X[(4 slot + c) N + k], slot 0 for k itself (a boundary

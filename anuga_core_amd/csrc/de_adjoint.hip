@@ -15,12 +15,6 @@
 
 __device__ __forceinline__ int adj_tid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
-// tangents per pass of the flux's adjoint (register pressure against
-// repeated primal work); the extrapolation's adjoint is a reverse sweep
-#ifndef ADJ_FLUX_K
-#define ADJ_FLUX_K 2
-#endif
-
 // adjoint of the update (friction, update, fix_negative) of one stage;
 // lam_out is scaled by `scale` (0.5: the final saxpy of the step)
 __global__ void __launch_bounds__(DE_BLOCK)
@@ -43,7 +37,7 @@ adj_k_flux(AdjView m, const double *__restrict__ S, const unsigned char *__restr
 {
     const int k = adj_tid();
     if (k >= m.N) return;
-    adj_bwd_flux<ADJ_FLUX_K>(m, S, fl, E, lam_ex, k, lamE, lamS, zbar);
+    adj_bwd_flux_rev(m, S, fl, E, lam_ex, k, lamE, lamS, zbar);
 }
 
 __global__ void __launch_bounds__(DE_BLOCK)

@@ -74,7 +74,7 @@ struct Run {
                             lamS, zbar, nbar, lam_ex.data());
         }
         for (int k = 0; k < N; k++)
-            adj_bwd_flux(m, S, f.data(), E.data(), lam_ex.data(), k, lamE.data(), lamS, zbar);
+            adj_bwd_flux_rev(m, S, f.data(), E.data(), lam_ex.data(), k, lamE.data(), lamS, zbar);
         std::vector<double> X(16 * (size_t)N);
         for (int k = 0; k < N; k++) adj_bwd_extrap_rev(m, S, f.data(), lamE.data(), k, X.data());
         for (int k = 0; k < N; k++) adj_gather_extrap(m, X.data(), k, lamS, zbar);
@@ -188,6 +188,29 @@ void harness_extrap_products(int N, int M, const int *nb, const double *cen, con
     }
 }
 
+
+// the flux's vector-Jacobian product both ways on the edge values of S:
+// wt [3][N] weights the explicit updates; out_* receive lamE [4][3][N] then
+// lamS [3][N] then zbar [N] (16 N values each)
+void harness_flux_products(int N, int M, const int *nb, const double *cen, const double *gx,
+                           const double *nx, const double *ny, const double *len,
+                           const double *area, const double *eta, const signed char *bkind,
+                           const double *bval, const double *prm, const double *S,
+                           const double *wt, double *out_dual, double *out_rev)
+{
+    Run r = make_run(N, M, nb, cen, gx, nx, ny, len, area, eta, bkind, bval, prm, 0.1);
+    std::vector<unsigned char> f(N);
+    for (int k = 0; k < N; k++) f[k] = (unsigned char)adj_flags(r.m, S, k);
+    std::vector<double> E(12 * (size_t)N);
+    for (int k = 0; k < N; k++) adj_fwd_extrap(r.m, S, f.data(), k, E.data());
+    const size_t n = (size_t)N;
+    for (int k = 0; k < N; k++) {
+        adj_bwd_flux(r.m, S, f.data(), E.data(), wt, k, out_dual, out_dual + 12 * n,
+                     out_dual + 15 * n);
+        adj_bwd_flux_rev(r.m, S, f.data(), E.data(), wt, k, out_rev, out_rev + 12 * n,
+                         out_rev + 15 * n);
+    }
+}
 }  // extern "C"
 
 extern "C" {
@@ -223,7 +246,8 @@ double harness_part(int N, int M, const int *nb, const double *cen, const double
             val += wt[k] * e[0] + wt[N + k] * e[1] + wt[2 * N + k] * e[2];
         }
         if (lamS)
-            for (int k = 0; k < N; k++) adj_bwd_flux(r.m, S, f.data(), E, wt, k, lamE, lamS, zbar);
+            for (int k = 0; k < N; k++)
+                adj_bwd_flux_rev(r.m, S, f.data(), E, wt, k, lamE, lamS, zbar);
     }
     return val;
 }

@@ -67,11 +67,20 @@ class Layout:
         self.len = np.ascontiguousarray(mesh.edgelengths.reshape(N, 3).T)
         self.area = np.ascontiguousarray(mesh.areas)
         self.eta = np.ascontiguousarray(q["friction"], np.float64).copy()
-        kinds = {"reflective": 0, "transmissive": 1}
-        self.bkind = np.zeros(max(self.M, 1), np.int8)
-        self.bval = np.zeros(3 * max(self.M, 1))
+        # boundary kinds of the edge function: 0 reflective, 1 transmissive,
+        # 2 Dirichlet ([stage, xmom, ymom] in bval), 3 centroid-transmissive
+        kinds = {"reflective": 0, "transmissive": 1, "transmissive_centroid": 3}
+        Mb = max(self.M, 1)
+        self.bkind = np.zeros(Mb, np.int8)
+        self.bval = np.zeros(3 * Mb)
         for tag, ids in mesh.tag_boundary_cells.items():
-            self.bkind[np.asarray(ids)] = kinds[bc[tag]]
+            ids = np.asarray(ids)
+            if isinstance(bc[tag], tuple):
+                self.bkind[ids] = 2
+                for c in range(3):
+                    self.bval[c * Mb + ids] = bc[tag][1][c]
+            else:
+                self.bkind[ids] = kinds[bc[tag]]
         self.S = np.zeros((N, 4))
         self.S[:, 0] = q["stage"]
         self.S[:, 3] = q["elevation"]
@@ -361,3 +370,43 @@ def test_reverse_extrapolation_product_equals_dual(harness, case):
     assert np.max(np.abs(Xr - Xd)) <= 1e-11 * scale, np.max(np.abs(Xr - Xd)) / scale
     # every slot that carries a value in one carries it in the other
     assert np.array_equal(Xd != 0.0, Xr != 0.0) or np.max(np.abs(Xr - Xd)) == 0.0
+
+
+@pytest.mark.parametrize("case", ["wet", "dam_break", "boundaries"])
+def test_reverse_flux_product_equals_dual(harness, case):
+    """The flux's hand-written reverse sweep (adj_edge_vjp) gives the
+    forward-mode dual product's adjoints of the edge values, the centroid
+    state and the bed to rounding: on moving wet water, on a wet/dry dam break
+    (dry edges, the zero-depth early return, clamped wave speeds), and on a
+    mesh whose boundary edges are reflective, transmissive, Dirichlet and
+    centroid-transmissive."""
+    if case == "wet":
+        mesh, q, bc = _wet_case(9)
+        steps, dt = 8, 0.1
+    elif case == "boundaries":
+        mesh = Mesh(*rectangular(8, 6, 40.0, 30.0))
+        x, y = mesh.centroid_coordinates.T
+        z = -0.02 * x + 0.01 * np.sin(y)
+        q = dict(elevation=z, stage=np.where(x < 20, 1.0, z),
+                 xmomentum=np.where(x < 20, 0.1, 0.0),
+                 friction=np.full(mesh.number_of_triangles, 0.03))
+        bc = {"left": ("dirichlet", [0.8, 0.05, 0.0]), "right": "transmissive",
+              "top": "reflective", "bottom": "transmissive_centroid"}
+        steps, dt = 20, 0.02
+    else:
+        mesh, _, q, bc = S.dam_break_de0(m=12, n=10)
+        steps, dt = 15, 0.02
+    L = Layout(mesh, q, bc)
+    Sx, _ = _forward(harness, L, L.S, dt, steps, [], 1)
+    N = L.N
+    wt = np.random.default_rng(6).standard_normal(3 * N)
+    Od, Or = np.zeros(16 * N), np.zeros(16 * N)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    S_ = np.ascontiguousarray(Sx)
+    harness.harness_flux_products(*L.args(), dp(S_), dp(wt), dp(Od), dp(Or))
+    # lamE, lamS, zbar, each to rounding of its own scale
+    for a, b in ((0, 12 * N), (12 * N, 15 * N), (15 * N, 16 * N)):
+        d, r = Od[a:b], Or[a:b]
+        scale = np.abs(d).max()
+        assert scale > 0
+        assert np.max(np.abs(r - d)) <= 1e-11 * scale, np.max(np.abs(r - d)) / scale
