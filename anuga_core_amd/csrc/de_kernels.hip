@@ -1310,6 +1310,17 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
             bk_k = bk[k];
         }
     }
+    // the edge-list entries of phases 2 and 3, likewise: hw waves take the
+    // halo in phase 2 (at least one wave left for the tile's edges), each
+    // thread of the others one edge inside the tile
+    const int h0 = tl.start[lb], nh = tl.start[lb + 1] - h0;
+    const int e0 = tl.estart[lb], es = tl.esplit[lb], e1 = tl.estart[lb + 1];
+    const int hw = min((nh + 63) >> 6, DE_BLOCK / 64 - 1);
+    const int e2 = e0 + t - 64 * hw, e3 = min(es, e0 + DE_BLOCK - 64 * hw) + t;
+    const bool has2 = t >= 64 * hw && e2 < es;
+    // phase 2's item: a tile edge, or the first halo triangle of a halo thread
+    const unsigned en2 = has2 ? tl.edge[e2] : (t < nh && t < 64 * hw ? (unsigned)tl.halo[h0 + t] : 0u);
+    const unsigned en3 = e3 < e1 ? tl.edge[e3] : 0u;
 
     // phase 1
     if (k < kend) {
@@ -1332,15 +1343,11 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
     }
     __syncthreads();
 
-    // phase 2: hw waves on the halo (at least one wave left for the tile's
-    // edges), the rest on the edges inside the tile
-    const int h0 = tl.start[lb], nh = tl.start[lb + 1] - h0;
-    const int e0 = tl.estart[lb], es = tl.esplit[lb], e1 = tl.estart[lb + 1];
-    const int hw = min((nh + 63) >> 6, DE_BLOCK / 64 - 1);
+    // phase 2: the halo's extrapolation, and edges inside the tile
     if (t < 64 * hw) {
         for (int j = t; j < nh; j += 64 * hw) {
             DeExtrap<double> x;
-            de_extrap_cell<double, VG>(m, p, q, tl.halo[h0 + j], err, x);
+            de_extrap_cell<double, VG>(m, p, q, j == t ? (int)en2 : tl.halo[h0 + j], err, x);
 #pragma unroll
             for (int i = 0; i < 3; i++) {
                 double ew, eh, euh, evh;
@@ -1351,15 +1358,17 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
                 hl[(9 + i) * hcap + j] = evh;
             }
         }
-    } else {
-        const int e = e0 + t - 64 * hw;
-        if (e < es) de_tile_edge<SUBSTEP, VG>(m, p, tl.edge[e], k0, lds, hl, hz, hcap, dtmin);
+    } else if (has2) {
+        de_tile_edge<SUBSTEP, VG>(m, p, en2, k0, lds, hl, hz, hcap, dtmin);
     }
     __syncthreads();
 
     // phase 3: the tile's remaining edges, then the edges to the halo
-    for (int e = min(es, e0 + DE_BLOCK - 64 * hw) + t; e < e1; e += DE_BLOCK)
-        de_tile_edge<SUBSTEP, VG>(m, p, tl.edge[e], k0, lds, hl, hz, hcap, dtmin);
+    if (e3 < e1) {
+        de_tile_edge<SUBSTEP, VG>(m, p, en3, k0, lds, hl, hz, hcap, dtmin);
+        for (int e = e3 + DE_BLOCK; e < e1; e += DE_BLOCK)
+            de_tile_edge<SUBSTEP, VG>(m, p, tl.edge[e], k0, lds, hl, hz, hcap, dtmin);
+    }
     __syncthreads();
 
     // phase 4: each triangle's three edges in local order
@@ -1651,6 +1660,10 @@ de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__
     float *const fh = ft + 12 * DE_BLOCK;
     double mass = 0.0, dtmin = 1.0e100, bflux = 0.0;
     unsigned fk = 0;
+    // a halo thread's first halo triangle, loaded now so that its latency
+    // hides behind the tile's extrapolation
+    const int h0 = tl.start[lb], nh = tl.start[lb + 1] - h0;
+    const int kh0 = t < nh ? tl.halo[h0 + t] : 0;
 
     if (k < kend) {
         DeExtrap<float> x;
@@ -1668,10 +1681,9 @@ de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__
         }
         hct[t] = x.c.h;
     }
-    const int h0 = tl.start[lb], nh = tl.start[lb + 1] - h0;
     for (int j = t; j < nh; j += DE_BLOCK) {
         DeExtrap<float> x;
-        de_extrap_cell<float>(m, p, q, tl.halo[h0 + j], err, x);
+        de_extrap_cell<float>(m, p, q, j == t ? kh0 : tl.halo[h0 + j], err, x);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const float4 r = de_edge32(x, i);
