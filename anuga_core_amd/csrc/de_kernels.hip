@@ -85,10 +85,9 @@ struct DeExtrap {
 // general_mesh.py forms the arrays the reference's extrapolation reads
 // (:599-626 edge midpoints 0.5 (Va + Vb); sw_domain_orig.c:2112-2135 their
 // offsets from the stored centroid).
-__device__ __forceinline__ void de_vertex_offsets(const DeMesh &m, int k, double dxv[3],
+__device__ __forceinline__ void de_vertex_offsets(const DeMesh &m, int k, int4 t, double dxv[3],
                                                   double dyv[3])
 {
-    const int4 t = m.tv[k];
     const double2 v0 = m.node[t.x], v1 = m.node[t.y], v2 = m.node[t.z];
     const double2 c = m.cen[k];
     dxv[0] = 0.5 * (v1.x + v2.x) - c.x;
@@ -115,16 +114,16 @@ __device__ __forceinline__ void de_vertex_normal(const DeMesh &m, int k, int i, 
     n2 = -(xn / len);
 }
 
+// vk, qk, tvk: triangle k's neighbour record, state and vertex record (VG),
+// loaded by the caller
 template <class T, bool VG = false>
 __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &p,
-                                               const DeQ *__restrict__ q, int k, int *err,
-                                               DeExtrap<T> &x)
+                                               const DeQ *__restrict__ q, int k, int4 vk,
+                                               DeQ qk, int4 tvk, int *err, DeExtrap<T> &x)
 {
     const double mah = p.mah;
-    const int4 vk = m.nb[k];
     const int v[3] = {vk.x, vk.y, vk.z};
     const int nbound = vk.w;
-    const DeQ qk = q[k];
     x.z = qk.z;
     int sur[3];
 #pragma unroll
@@ -192,7 +191,7 @@ __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &
     typename DeGeo<T>::X gx;
     T dxv[3], dyv[3];
     if constexpr (VG) {
-        de_vertex_offsets(m, k, dxv, dyv);
+        de_vertex_offsets(m, k, tvk, dxv, dyv);
     } else {
         gx = static_cast<const typename DeGeo<T>::X *>(m.gx)[k];
 #pragma unroll
@@ -318,6 +317,16 @@ __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &
             de_line_jumps<T>(c.v, n1.v, dx2, dy2, dxv, dyv, bw, jv);
         }
     }
+}
+
+// the same, loading triangle k's records
+template <class T, bool VG = false>
+__device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &p,
+                                               const DeQ *__restrict__ q, int k, int *err,
+                                               DeExtrap<T> &x)
+{
+    const int4 tvk = VG ? m.tv[k] : make_int4(0, 0, 0, 0);
+    de_extrap_cell<T, VG>(m, p, q, k, m.nb[k], q[k], tvk, err, x);
 }
 
 // FP64 edge values of edge i (compile-time i, or a select on a run-time one):
@@ -1284,25 +1293,23 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
     extern __shared__ double lds[];
     __shared__ double red_dt[DE_BLOCK / 64], red_bf[DE_BLOCK / 64], red_m[DE_BLOCK / 64];
     const int lb = xcd_block();
-    DeControl *cb = de_ctl(ctl, m.scen, lb);
-    if (cb->done) return;
-    int *err = &cb->status;
-    const double dt = SUBSTEP ? cb->dt : 0.0;
     const int N = m.N, t = threadIdx.x;
     const int k0 = lb * DE_BLOCK, k = k0 + t;
-    const int kend = cb->kend;
-    const int hcap = tl.hcap;
-    double *const hl = lds + 12 * DE_BLOCK;
-    double *const hz = hl + 12 * hcap;
-    double mass = 0.0, dtmin = 1.0e100, bflux = 0.0;
-    unsigned fk = 0;
-    constexpr int Q = 3 * DE_BLOCK;   // stride between quantities of the tile
+    // phase 1's own records, requested before the control block is read
+    // (bounded by the arrays' length, not yet by the scenario's end)
+    int4 vk1 = make_int4(-1, -1, -1, 3), tv1 = make_int4(0, 0, 0, 0);
+    DeQ qk1{0.0, 0.0, 0.0, 0.0};
+    if (k < N) {
+        vk1 = m.nb[k];
+        qk1 = q[k];
+        if (VG) tv1 = m.tv[k];
+    }
     // phase 4's own inputs, loaded now so that their latency hides behind
     // the extrapolation and the edge list
     double area_k = 1.0, eta_k = 0.0;
     DeQ bk_k{0.0, 0.0, 0.0, 0.0};
     unsigned sl_k = 0;
-    if (k < kend) {
+    if (k < N) {
         area_k = m.area[k];
         sl_k = tl.slot[k];
         if (SUBSTEP) {
@@ -1322,10 +1329,21 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
     const unsigned en2 = has2 ? tl.edge[e2] : (t < nh && t < 64 * hw ? (unsigned)tl.halo[h0 + t] : 0u);
     const unsigned en3 = e3 < e1 ? tl.edge[e3] : 0u;
 
+    DeControl *cb = de_ctl(ctl, m.scen, lb);
+    if (cb->done) return;
+    int *err = &cb->status;
+    const double dt = SUBSTEP ? cb->dt : 0.0;
+    const int kend = cb->kend;
+    const int hcap = tl.hcap;
+    double *const hl = lds + 12 * DE_BLOCK;
+    double *const hz = hl + 12 * hcap;
+    double mass = 0.0, dtmin = 1.0e100, bflux = 0.0;
+    unsigned fk = 0;
+    constexpr int Q = 3 * DE_BLOCK;   // stride between quantities of the tile
     // phase 1
     if (k < kend) {
         DeExtrap<double> x;
-        de_extrap_cell<double, VG>(m, p, q, k, err, x);
+        de_extrap_cell<double, VG>(m, p, q, k, vk1, qk1, tv1, err, x);
         mass = x.mass;
         fk = x.flag;
         flags[k] = (unsigned char)fk;
@@ -1647,12 +1665,20 @@ de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__
     extern __shared__ double lds[];
     __shared__ double red_dt[DE_BLOCK / 64], red_bf[DE_BLOCK / 64], red_m[DE_BLOCK / 64];
     const int lb = xcd_block();
+    const int t = threadIdx.x;
+    const int k0 = lb * DE_BLOCK, k = k0 + t;
+    // the extrapolation's own records, requested before the control block is
+    // read (bounded by the arrays' length, not yet by the scenario's end)
+    int4 vk1 = make_int4(-1, -1, -1, 3);
+    DeQ qk1{0.0, 0.0, 0.0, 0.0};
+    if (k < m.N) {
+        vk1 = m.nb[k];
+        qk1 = q[k];
+    }
     DeControl *cb = de_ctl(ctl, m.scen, lb);
     if (cb->done) return;
     int *err = &cb->status;
     const double dt = SUBSTEP ? cb->dt : 0.0;
-    const int t = threadIdx.x;
-    const int k0 = lb * DE_BLOCK, k = k0 + t;
     const int kend = cb->kend;
     const int hcap = tl.hcap;
     double *const hct = lds, *const hch = lds + DE_BLOCK;
@@ -1667,7 +1693,7 @@ de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__
 
     if (k < kend) {
         DeExtrap<float> x;
-        de_extrap_cell<float>(m, p, q, k, err, x);
+        de_extrap_cell<float>(m, p, q, k, vk1, qk1, make_int4(0, 0, 0, 0), err, x);
         mass = x.mass;
         fk = x.flag;
         flags[k] = (unsigned char)fk;
