@@ -101,10 +101,9 @@ __device__ __forceinline__ void de_vertex_offsets(const DeMesh &m, int k, int4 t
 // Outward normal and length of edge i of triangle k from its vertices
 // (general_mesh.py:202-232: the edge opposite vertex i, (x, y) =
 // V[i+2] - V[i+1], l = sqrt(x^2 + y^2), normal (y / l, -(x / l))).
-__device__ __forceinline__ void de_vertex_normal(const DeMesh &m, int k, int i, double &n1,
+__device__ __forceinline__ void de_vertex_normal(const DeMesh &m, int4 t, int i, double &n1,
                                                  double &n2, double &len)
 {
-    const int4 t = m.tv[k];
     const int a = i == 0 ? t.y : (i == 1 ? t.z : t.x);
     const int b = i == 0 ? t.z : (i == 1 ? t.x : t.y);
     const double2 va = m.node[a], vb = m.node[b];
@@ -1196,6 +1195,46 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
     }
 }
 
+// The workgroup's partial results: its smallest CFL timestep, boundary flux
+// and protect mass (substep 0), or its boundary flux alone (the later
+// substeps' reductions read nothing else). A sum whose lanes are all zero is
+// zero without the exchange.
+__device__ __forceinline__ double de_wave_sum_nz(double x)
+{
+    return __ballot(x != 0.0) ? de_wave_sum(x) : 0.0;
+}
+
+template <int SUBSTEP>
+__device__ __forceinline__ void de_block_partials(double dtmin, double bflux, double mass,
+                                                  double *red_dt, double *red_bf, double *red_m,
+                                                  double *part_dt, double *part_bf,
+                                                  double *part_mass, int lb)
+{
+    if (SUBSTEP == 0) dtmin = de_wave_min(dtmin);
+    bflux = de_wave_sum_nz(bflux);
+    if (SUBSTEP == 0) mass = de_wave_sum_nz(mass);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red_dt[wv] = dtmin;
+        red_bf[wv] = bflux;
+        red_m[wv] = mass;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = red_dt[0], b = red_bf[0], cm = red_m[0];
+        for (int i = 1; i < DE_BLOCK / 64; i++) {
+            a = fmin(a, red_dt[i]);
+            b += red_bf[i];
+            cm += red_m[i];
+        }
+        if (SUBSTEP == 0) {
+            part_dt[lb] = a;
+            part_mass[lb] = cm;
+        }
+        part_bf[lb] = b;
+    }
+}
+
 // One listed edge of a tile (de_xflux_e): its flux from the tile triangle's
 // side (the entry's halo bit: the neighbour is in the halo; otherwise in the
 // tile, and its side's pressure term and CFL bound are formed here too),
@@ -1203,7 +1242,7 @@ de_xflux(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__restri
 // is read at run time, so a wave can take edges of both kinds in one pass.
 template <int SUBSTEP, bool VG>
 __device__ __forceinline__ void de_tile_edge(const DeMesh &m, const DeParams &p, unsigned en,
-                                             int k0, double *lds, double *hl,
+                                             int4 tva, int k0, double *lds, double *hl,
                                              const double *hz, int hcap, double &dtmin)
 {
     constexpr int Q = 3 * DE_BLOCK;
@@ -1219,7 +1258,7 @@ __device__ __forceinline__ void de_tile_edge(const DeMesh &m, const DeParams &p,
     const double qr0 = R[0], hre = R[rs], qr1 = R[2 * rs], qr2 = R[3 * rs];
     double n1, n2, length;
     if constexpr (VG) {
-        de_vertex_normal(m, ka, ia, n1, n2, length);
+        de_vertex_normal(m, tva, ia, n1, n2, length);
     } else {
         n1 = m.nx[ia * N + ka];
         n2 = m.ny[ia * N + ka];
@@ -1359,13 +1398,25 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
             lds[(9 + i) * DE_BLOCK + t] = evh;
         }
     }
+    // phase 2's records, requested before the barrier: a halo thread's
+    // first triangle's neighbours, an edge's vertices
+    const bool hal2 = t < nh && t < 64 * hw;
+    int4 rec2 = make_int4(-1, -1, -1, 3);
+    if (hal2) {
+        rec2 = m.nb[en2];
+    } else if (VG && has2) {
+        rec2 = m.tv[k0 + DE_TE_T(en2)];
+    }
     __syncthreads();
 
     // phase 2: the halo's extrapolation, and edges inside the tile
     if (t < 64 * hw) {
         for (int j = t; j < nh; j += 64 * hw) {
             DeExtrap<double> x;
-            de_extrap_cell<double, VG>(m, p, q, j == t ? (int)en2 : tl.halo[h0 + j], err, x);
+            const int kh = j == t ? (int)en2 : tl.halo[h0 + j];
+            if (j != t) rec2 = m.nb[kh];   // a further halo triangle
+            const int4 tvh = VG ? m.tv[kh] : make_int4(0, 0, 0, 0);
+            de_extrap_cell<double, VG>(m, p, q, kh, rec2, q[kh], tvh, err, x);
 #pragma unroll
             for (int i = 0; i < 3; i++) {
                 double ew, eh, euh, evh;
@@ -1377,22 +1428,35 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
             }
         }
     } else if (has2) {
-        de_tile_edge<SUBSTEP, VG>(m, p, en2, k0, lds, hl, hz, hcap, dtmin);
+        de_tile_edge<SUBSTEP, VG>(m, p, en2, rec2, k0, lds, hl, hz, hcap, dtmin);
     }
+    // phase 3's first edge's vertices, requested before the barrier
+    const int4 tv3 = VG && e3 < e1 ? m.tv[k0 + DE_TE_T(en3)] : make_int4(0, 0, 0, 0);
     __syncthreads();
 
     // phase 3: the tile's remaining edges, then the edges to the halo
     if (e3 < e1) {
-        de_tile_edge<SUBSTEP, VG>(m, p, en3, k0, lds, hl, hz, hcap, dtmin);
-        for (int e = e3 + DE_BLOCK; e < e1; e += DE_BLOCK)
-            de_tile_edge<SUBSTEP, VG>(m, p, tl.edge[e], k0, lds, hl, hz, hcap, dtmin);
+        de_tile_edge<SUBSTEP, VG>(m, p, en3, tv3, k0, lds, hl, hz, hcap, dtmin);
+        for (int e = e3 + DE_BLOCK; e < e1; e += DE_BLOCK) {
+            const unsigned en = tl.edge[e];
+            const int4 tva = VG ? m.tv[k0 + DE_TE_T(en)] : make_int4(0, 0, 0, 0);
+            de_tile_edge<SUBSTEP, VG>(m, p, en, tva, k0, lds, hl, hz, hcap, dtmin);
+        }
+    }
+    // phase 4's records, requested before the barrier (substep 0: the
+    // later substeps' update needs the registers)
+    DeQ qk4{0.0, 0.0, 0.0, 0.0};
+    int4 vk4 = make_int4(-1, -1, -1, 3);
+    if (SUBSTEP == 0 && k < kend) {
+        qk4 = q[k];
+        vk4 = m.nb[k];
     }
     __syncthreads();
 
     // phase 4: each triangle's three edges in local order
     if (k < kend) {
-        const DeCentroid c = de_centroid(q[k], fk);
-        const int4 vk = m.nb[k];
+        const DeCentroid c = de_centroid(SUBSTEP == 0 ? qk4 : q[k], fk);
+        const int4 vk = SUBSTEP == 0 ? vk4 : m.nb[k];
         const int v[3] = {vk.x, vk.y, vk.z};
         const unsigned sl = sl_k;
         double sw = 0.0, sx = 0.0, sy = 0.0;
@@ -1449,27 +1513,8 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
         }
     }
 
-    dtmin = de_wave_min(dtmin);
-    bflux = de_wave_sum(bflux);
-    mass = de_wave_sum(mass);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) {
-        red_dt[wv] = dtmin;
-        red_bf[wv] = bflux;
-        red_m[wv] = mass;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = red_dt[0], b = red_bf[0], cm = red_m[0];
-        for (int i = 1; i < DE_BLOCK / 64; i++) {
-            a = fmin(a, red_dt[i]);
-            b += red_bf[i];
-            cm += red_m[i];
-        }
-        part_dt[lb] = a;
-        part_bf[lb] = b;
-        part_mass[lb] = cm;
-    }
+    de_block_partials<SUBSTEP>(dtmin, bflux, mass, red_dt, red_bf, red_m, part_dt, part_bf,
+                               part_mass, lb);
 }
 
 // One edge's terms of triangle k's sums on the mixed path (the body of the
@@ -1720,16 +1765,26 @@ de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__
         }
         hch[j] = x.c.h;
     }
+    // the flux's own records, requested before the barrier; the whole
+    // geometry record is held in registers (the edge loop runs faster with
+    // it there than through a pointer)
+    DeQ qk2{0.0, 0.0, 0.0, 0.0};
+    int4 vk2 = make_int4(-1, -1, -1, 3);
+    unsigned sl2 = 0;
+    DeGeoF32 gfr{};
+    if (k < kend) {
+        qk2 = q[k];
+        vk2 = m.nb[k];
+        sl2 = tl.slot[k];
+        gfr = static_cast<const DeGeoF32 *>(m.gf)[k];
+    }
     __syncthreads();
 
     if (k < kend) {
-        const DeCentroid c = de_centroid(q[k], fk);
-        const int4 vk = m.nb[k];
+        const DeCentroid c = de_centroid(qk2, fk);
+        const int4 vk = vk2;
         const int v[3] = {vk.x, vk.y, vk.z};
-        const unsigned sl = tl.slot[k];
-        // the whole record is loaded up front here (the fused kernel's
-        // edge loop runs faster with it in registers than through a pointer)
-        const DeGeoF32 gfr = static_cast<const DeGeoF32 *>(m.gf)[k];
+        const unsigned sl = sl2;
         const DeGeoF32 *gf = &gfr;
         const double hcd = hct[t];
         const float hc = float(hcd);
@@ -1773,27 +1828,8 @@ de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__
         }
     }
 
-    dtmin = de_wave_min(dtmin);
-    bflux = de_wave_sum(bflux);
-    mass = de_wave_sum(mass);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) {
-        red_dt[wv] = dtmin;
-        red_bf[wv] = bflux;
-        red_m[wv] = mass;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = red_dt[0], b = red_bf[0], cm = red_m[0];
-        for (int i = 1; i < DE_BLOCK / 64; i++) {
-            a = fmin(a, red_dt[i]);
-            b += red_bf[i];
-            cm += red_m[i];
-        }
-        part_dt[lb] = a;
-        part_bf[lb] = b;
-        part_mass[lb] = cm;
-    }
+    de_block_partials<SUBSTEP>(dtmin, bflux, mass, red_dt, red_bf, red_m, part_dt, part_bf,
+                               part_mass, lb);
 }
 
 // friction + update + fix_negative_cells from the explicit updates of substep 0;
