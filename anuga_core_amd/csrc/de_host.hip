@@ -2,9 +2,9 @@
 //
 // A handle owns all device buffers of one or more scenarios (independent
 // meshes stepped together: an ensemble). Each scenario's triangles are
-// renumbered once at creation (along a Hilbert curve through the centroids,
-// so that each workgroup's run of triangles is a compact tile for the fused
-// kernels, or Reverse-Cuthill-McKee on the triangle adjacency) so that
+// renumbered once at creation (recursive coordinate bisection of the
+// centroids, so that each workgroup's run of triangles is a compact tile for
+// the fused kernels, or Reverse-Cuthill-McKee on the triangle adjacency) so that
 // neighbour gathers stay local; the permutation is applied on upload and
 // undone on download, so callers only ever see the reference's order. Scenarios sit back to back in one internal
 // numbering, each starting on a workgroup boundary, so that every workgroup
@@ -322,50 +322,47 @@ static std::vector<int> rcm_order(int N, const int64_t *neighbours)
     return order;
 }
 
-// Hilbert-curve index of (x, y) on a 2^order grid (the textbook
-// rotate-and-accumulate form).
-static uint64_t hilbert_index(uint32_t x, uint32_t y, int order)
+// Recursive coordinate bisection of ids[lo, hi) into runs of DE_BLOCK
+// triangles: split across the longer side of the centroids' bounding box so
+// that the first part holds half of the runs (whole runs), recurse on both
+// parts. Ties in the coordinate go by triangle index; each finished run is in
+// increasing triangle index.
+static void tile_bisect(std::vector<int> &ids, size_t lo, size_t hi, const double *cen)
 {
-    const uint32_t n = 1u << order;
-    uint64_t d = 0;
-    for (uint32_t s = n >> 1; s > 0; s >>= 1) {
-        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
-        d += (uint64_t)s * s * ((3u * rx) ^ ry);
-        if (ry == 0) {
-            if (rx == 1) {
-                x = n - 1 - x;
-                y = n - 1 - y;
-            }
-            std::swap(x, y);
-        }
+    const size_t n = hi - lo;
+    if (n <= (size_t)DE_BLOCK) {
+        std::sort(ids.begin() + lo, ids.begin() + hi);
+        return;
     }
-    return d;
-}
-
-// Triangles sorted along a Hilbert curve through their centroids (ties by
-// index): consecutive runs of DE_BLOCK triangles are compact patches whose
-// boundary, the halo the fused kernel extrapolates twice, is short.
-static std::vector<int> tile_order(int N, const double *cen)
-{
     double x0 = HUGE_VAL, y0 = HUGE_VAL, x1 = -HUGE_VAL, y1 = -HUGE_VAL;
-    for (int k = 0; k < N; k++) {
+    for (size_t a = lo; a < hi; a++) {
+        const int k = ids[a];
         x0 = std::min(x0, cen[2 * k]);
         x1 = std::max(x1, cen[2 * k]);
         y0 = std::min(y0, cen[2 * k + 1]);
         y1 = std::max(y1, cen[2 * k + 1]);
     }
-    const int order = 16;
-    const double span = std::max(std::max(x1 - x0, y1 - y0), 1e-300);
-    const double scale = ((1 << order) - 1) / span;
-    std::vector<std::pair<uint64_t, int>> key(N);
-    for (int k = 0; k < N; k++) {
-        const double fx = std::min(std::max((cen[2 * k] - x0) * scale, 0.0), (double)((1 << order) - 1));
-        const double fy = std::min(std::max((cen[2 * k + 1] - y0) * scale, 0.0), (double)((1 << order) - 1));
-        key[k] = {hilbert_index((uint32_t)fx, (uint32_t)fy, order), k};
-    }
-    std::sort(key.begin(), key.end());
+    const int dim = (x1 - x0 >= y1 - y0) ? 0 : 1;
+    const size_t runs = (n + DE_BLOCK - 1) / DE_BLOCK;
+    const size_t nl = runs / 2 * (size_t)DE_BLOCK;
+    std::nth_element(ids.begin() + lo, ids.begin() + lo + nl, ids.begin() + hi,
+                     [cen, dim](int a, int b) {
+                         const double ca = cen[2 * a + dim], cb = cen[2 * b + dim];
+                         return ca < cb || (ca == cb && a < b);
+                     });
+    tile_bisect(ids, lo, lo + nl, cen);
+    tile_bisect(ids, lo + nl, hi, cen);
+}
+
+// Triangles in tile order: consecutive runs of DE_BLOCK triangles are compact
+// patches (recursive coordinate bisection of the centroids), so the halo the
+// fused kernel extrapolates twice is short, and consecutive runs are
+// neighbouring patches.
+static std::vector<int> tile_order(int N, const double *cen)
+{
     std::vector<int> o(N);
-    for (int k = 0; k < N; k++) o[k] = key[k].second;
+    for (int k = 0; k < N; k++) o[k] = k;
+    tile_bisect(o, 0, (size_t)N, cen);
     return o;
 }
 

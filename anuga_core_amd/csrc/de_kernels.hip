@@ -1,6 +1,6 @@
 // HIP kernels of the DE step for gfx950 (FP64 and mixed FP32/FP64).
 //
-// One thread per triangle, triangles in a locality order (Hilbert tiles or
+// One thread per triangle, triangles in a locality order (bisection tiles or
 // RCM), DE_BLOCK-thread workgroups remapped XCD-contiguous. Per Euler substep:
 //   de_extrapolate  protect + velocity + dry-neighbour zeroing + limited
 //                   second-order edge values (sw_domain_orig.c:1215-1281,
@@ -19,8 +19,8 @@
 //                   (sw_domain_orig.c:2805-2838, quantity.c:772-822,
 //                   sw_domain_orig.c:2890-2910).
 //   de_reduce       fixed-order reduction of the per-block partials.
-//   de_xflux,       extrapolation and flux fused over compact tiles (a
-//   de_xflux_mixed  Hilbert-curve order): a workgroup extrapolates its
+//   de_xflux,       extrapolation and flux fused over compact tiles (the
+//   de_xflux_mixed  bisection order): a workgroup extrapolates its
 //                   triangles and their halo into LDS and gathers the fluxes
 //                   from there, with the same device functions as the
 //                   separate kernels, so the same bits.

@@ -144,8 +144,8 @@ class Domain:
     Domain(coordinates, vertices, boundary) builds the mesh like anuga.Domain;
     `mesh=` accepts a prebuilt anuga_core_amd.mesh.Mesh. `precision` is 'fp64'
     (bit-comparable with the reference) or 'mixed'; `reorder` picks the
-    library's internal triangle order: True or 'tiles' (Hilbert-curve tiles,
-    which let FP64 steps fuse extrapolation and flux), 'rcm' (Reverse
+    library's internal triangle order: True or 'tiles' (compact tiles from a
+    recursive bisection, which let FP64 steps fuse extrapolation and flux), 'rcm' (Reverse
     Cuthill-McKee) or False (the mesh's order). Results are the same bits in
     every order.
     """
@@ -160,7 +160,7 @@ class Domain:
         self.boundary_length = mesh.boundary_length
         self.verbose = verbose
         self.precision = precision
-        # True / "tiles": Hilbert-curve tiles (the fused FP64 kernel); "rcm":
+        # True / "tiles": bisection tiles (the fused FP64 kernel); "rcm":
         # Reverse Cuthill-McKee; False: the mesh's own order
         self._reorder = {True: 2, "tiles": 2, "rcm": 1, False: 0, None: 0}[reorder]
 

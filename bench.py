@@ -52,7 +52,7 @@ KERNEL_BYTES = {
         #   + normals (48) + lengths (24) + area (8) + n (8) + step-initial
         #   w,uh,vh (24) -> w,uh,vh (24)
         "flux1": ("de_flux<1>", 277),
-        # the tiled fused kernels (Hilbert tiles): extrapolation inputs w,uh,vh,z
+        # the tiled fused kernels (compact tiles): extrapolation inputs w,uh,vh,z
         #   (32) + neighbours (12) + centroid (16) + edge offsets (48) + normals
         #   (48) + lengths (24) + area, radius (16) -> flag (1) + explicit (24)
         "xflux0": ("de_xflux<0>", 221),
@@ -553,7 +553,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the second precision (speed and mixed-vs-FP64 error)")
     ap.add_argument("--order", default="tiles", choices=["tiles", "rcm", "none"],
-                    help="internal triangle order: Hilbert tiles (fused FP64 kernel), "
+                    help="internal triangle order: compact tiles (fused FP64 kernel), "
                          "Reverse Cuthill-McKee, or the mesh's own")
     ap.add_argument("--no-reorder", action="store_true", help="same as --order none")
     ap.add_argument("--workload", default="single", choices=["single", "ensemble", "adjoint"],

@@ -94,7 +94,7 @@ typedef struct anuga_params {
     int64_t timestep_fluxcalls;                /* 1 = euler (DE0), 2 = rk2 (DE1), 3 = rk3 (DE2) */
     int64_t precision;                         /* ANUGA_PRECISION_* */
     int64_t reorder;                           /* internal triangle order: 0 = the caller's,
-                                                  1 = Reverse-Cuthill-McKee, 2 = Hilbert-curve
+                                                  1 = Reverse-Cuthill-McKee, 2 = compact
                                                   tiles (one FP64 scenario: the fused
                                                   extrapolation + flux kernel; results are
                                                   the same bits in every order) */
@@ -448,12 +448,14 @@ int anuga_rcm_order(int64_t N, const int64_t *neighbours, int64_t *order);
  * Host-only; no counterpart in the reference. */
 int anuga_build_config(int64_t *block, int64_t *mixed_waves);
 
-/* The Hilbert-curve triangle order applied at creation when params->reorder
- * is 2: triangles sorted by the Hilbert index of their centroid on a 2^16
- * grid over the bounding box, ties by index; order[new] = old. centroids is
- * (N,2). Host-only. No counterpart in the reference (its triangles are
- * processed in the mesh's order); it exists so that runs of 256 consecutive
- * triangles are compact tiles for the fused extrapolation + flux kernel. */
+/* The tile order applied at creation when params->reorder is 2: recursive
+ * coordinate bisection of the centroids into runs of the workgroup size (the
+ * longer side of each part's bounding box is cut so that the first part holds
+ * half of the runs; ties by index), each run in increasing triangle index;
+ * order[new] = old. centroids is (N,2). Host-only. No counterpart in the
+ * reference (its triangles are processed in the mesh's order); it exists so
+ * that runs of 256 consecutive triangles are compact tiles, with short halos,
+ * for the fused extrapolation + flux kernel. */
 int anuga_tile_order(int64_t N, const double *centroids, int64_t *order);
 
 /* The fused kernel's tile metadata, for triangles already in internal order:

@@ -46,7 +46,7 @@ def _ensemble(scen, alg, precision="fp64"):
 
 
 def _domain(mesh, q, bc, alg, precision="fp64", reorder=True):
-    # the ensemble's internal order (Hilbert tiles per scenario) by default, so
+    # the ensemble's internal order (bisection tiles per scenario) by default, so
     # that per-block partial sums (the boundary flux integral) add in the same
     # order
     return S.setup(Domain(mesh=mesh, precision=precision, reorder=reorder), alg, q, bc)

@@ -79,7 +79,7 @@ def test_de0_dam_break_vs_oracle(reorder):
 
 @pytest.mark.parametrize("reorder", [True, "rcm"])
 def test_de1_hills_vs_oracle(reorder):
-    """Hilbert tiles (fused extrapolation + flux) and RCM (separate kernels)."""
+    """Bisection tiles (fused extrapolation + flux) and RCM (separate kernels)."""
     print("case de1_hills reorder=%s" % reorder, flush=True)
     o, d = _pair(S.hills_de1, reorder=reorder, m=30, n=30)
     for _ in d.evolve(yieldstep=5.0, finaltime=40.0):

@@ -1,5 +1,5 @@
 """The fused extrapolation + flux kernel's tile layout (CPU): the library's
-Hilbert-curve order and tile metadata, checked by replaying, for every
+tile order (recursive coordinate bisection) and tile metadata, checked by replaying, for every
 interior half-edge, the LDS address the kernel forms (anuga_core_amd/csrc/
 de_kernels.hip, de_xflux) against the triangle and edge that the address
 holds. Calls only host-side entry points of the library."""
@@ -124,8 +124,8 @@ def test_lds_addresses_of_every_half_edge(m, n):
 
 
 def test_tile_order_is_local():
-    """Consecutive triangles in the Hilbert order are close: the mean distance
-    between the centroids of triangles 256 apart stays near a tile's size."""
+    """Consecutive triangles in the tile order are close, and each run of 256
+    is a compact patch of about 8 x 8 cells."""
     mesh = Mesh(*rectangular_cross(100, 100, 100.0, 100.0))
     nb, start, halo, slot, hcap = tile_layout(mesh)
     lib = _native.lib()
@@ -137,6 +137,28 @@ def test_tile_order_is_local():
     assert np.median(step) < 1.0            # cells are 1 m
     spread = [np.ptp(c[t:t + BLOCK], axis=0).max() for t in range(0, len(c) - BLOCK, BLOCK)]
     assert np.median(spread) <= 16.0 * (BLOCK / 256) ** 0.5   # 256 triangles: about 8 cells
+
+
+def test_tile_halos_are_short():
+    """The bisection's runs of 256 triangles are compact: on a 200 x 200
+    cross mesh the mean halo (neighbours outside the tile) is under 50
+    triangles, about the boundary of an 8 x 8-cell patch cut through cells,
+    and no tile needs a second halo wave (at most 64). The halo is the work
+    the fused kernel repeats per tile."""
+    if BLOCK != 256:
+        pytest.skip("sized for 256-triangle tiles")
+    mesh = Mesh(*rectangular_cross(200, 200, 200.0, 200.0))
+    nb, start, halo, slot, hcap = tile_layout(mesh)
+    nh = np.diff(start)
+    assert nh.mean() < 50.0, nh.mean()
+    assert nh.max() <= 64, nh.max()
+    # every tile's halo is exactly the outside neighbours of its triangles
+    for t in range(0, len(nh), 37):
+        k0, k1 = t * BLOCK, min((t + 1) * BLOCK, len(nb))
+        v = nb[k0:k1].ravel()
+        n = v[v >= 0] >> 2
+        outside = np.unique(n[(n < k0) | (n >= k1)])
+        assert np.array_equal(np.sort(halo[start[t]:start[t + 1]]), outside)
 
 
 def test_tile_halos_rejects_bad_neighbours():
