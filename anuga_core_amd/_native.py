@@ -129,7 +129,7 @@ SIGNATURES = {
                                              C.c_void_p, C.c_void_p]),
     "anuga_rcm_order": (C.c_int, [C.c_int64, _ip, _ip]),
     "anuga_build_config": (C.c_int, [_ip, _ip]),
-    "anuga_tile_order": (C.c_int, [C.c_int64, _dp, _ip]),
+    "anuga_tile_order": (C.c_int, [C.c_int64, _dp, _ip, _ip]),
     "anuga_tile_halos": (C.c_int, [C.c_int64, C.POINTER(C.c_int32), _ip, C.c_int64,
                                    C.POINTER(C.c_int32), C.POINTER(C.c_uint32), _ip, _ip]),
     "anuga_tile_edges": (C.c_int, [C.c_int64, C.POINTER(C.c_int32), _ip, _ip, C.c_int64,

@@ -12,12 +12,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <memory>
 #include <string>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -323,46 +327,143 @@ static std::vector<int> rcm_order(int N, const int64_t *neighbours)
 }
 
 // Recursive coordinate bisection of ids[lo, hi) into runs of DE_BLOCK
-// triangles: split across the longer side of the centroids' bounding box so
-// that the first part holds half of the runs (whole runs), recurse on both
-// parts. Ties in the coordinate go by triangle index; each finished run is in
-// increasing triangle index.
-static void tile_bisect(std::vector<int> &ids, size_t lo, size_t hi, const double *cen)
+// triangles. Each cut is across x or y and leaves a whole number of runs on
+// its first side, between 40% and 60% of them (and within 64 runs of the
+// middle); of those cuts (both axes) the
+// one that separates the fewest adjacent pairs of the part is taken (then the
+// one nearest the middle, then x), so that the runs are compact patches whose
+// boundaries follow the mesh's lines. Ties in a coordinate go by triangle
+// index; each finished run is in increasing triangle index. nbr: (N,3)
+// neighbours (negative: boundary), or null for cuts at the middle run. Parts
+// of more than 4096 runs are cut at the middle run across their longer side
+// (the search's passes over such a part cost more than they find).
+// Parts are cut on several threads at once (disjoint ranges of ids and tmp).
+// side[k] = 4 * (lo / DE_BLOCK + 1) + s marks triangle k as in the part
+// starting at lo (a whole number of runs), s = 1 on the cut's first side and
+// 2 on its second; a part reads only the marks that carry its own lo, so the
+// marks of parts cut at the same time never mix.
+struct TileCut {
+    const double *cen;
+    const int64_t *nbr;
+    std::vector<int> tmp;
+    std::unique_ptr<std::atomic<int>[]> side;
+};
+
+static void tile_bisect(TileCut &tc, std::vector<int> &ids, size_t lo, size_t hi, int depth)
 {
     const size_t n = hi - lo;
     if (n <= (size_t)DE_BLOCK) {
         std::sort(ids.begin() + lo, ids.begin() + hi);
         return;
     }
-    double x0 = HUGE_VAL, y0 = HUGE_VAL, x1 = -HUGE_VAL, y1 = -HUGE_VAL;
-    for (size_t a = lo; a < hi; a++) {
-        const int k = ids[a];
-        x0 = std::min(x0, cen[2 * k]);
-        x1 = std::max(x1, cen[2 * k]);
-        y0 = std::min(y0, cen[2 * k + 1]);
-        y1 = std::max(y1, cen[2 * k + 1]);
-    }
-    const int dim = (x1 - x0 >= y1 - y0) ? 0 : 1;
+    const double *cen = tc.cen;
     const size_t runs = (n + DE_BLOCK - 1) / DE_BLOCK;
-    const size_t nl = runs / 2 * (size_t)DE_BLOCK;
-    std::nth_element(ids.begin() + lo, ids.begin() + lo + nl, ids.begin() + hi,
-                     [cen, dim](int a, int b) {
-                         const double ca = cen[2 * a + dim], cb = cen[2 * b + dim];
-                         return ca < cb || (ca == cb && a < b);
-                     });
-    tile_bisect(ids, lo, lo + nl, cen);
-    tile_bisect(ids, lo + nl, hi, cen);
+    auto less = [cen](int dim) {
+        return [cen, dim](int a, int b) {
+            const double ca = cen[2 * a + dim], cb = cen[2 * b + dim];
+            return ca < cb || (ca == cb && a < b);
+        };
+    };
+    int best_dim = 0;
+    size_t best_runs = runs / 2;
+    if (tc.nbr && runs <= 4096) {
+        // candidate cuts: 40% to 60% of the runs, at most 64 runs either side
+        // of the middle (a large part finds a cut along the mesh's lines
+        // within that; the window's sort stays small)
+        const size_t mid = runs / 2;
+        const size_t r0 = std::max<size_t>(
+            {(size_t)1, (size_t)std::floor(0.4 * (double)runs), mid > 64 ? mid - 64 : (size_t)0});
+        const size_t r1 = std::min<size_t>(
+            {runs - 1, (size_t)std::ceil(0.6 * (double)runs), mid + 64});
+        long long best_cut = -1;
+        double best_off = 0.0;
+        for (int dim = 0; dim < 2 && r0 <= r1; dim++) {
+            int *a = tc.tmp.data() + lo;
+            std::copy(ids.begin() + lo, ids.begin() + hi, a);
+            const size_t pl = r0 * DE_BLOCK, ph = std::min(n, r1 * (size_t)DE_BLOCK);
+            std::nth_element(a, a + pl, a + n, less(dim));
+            std::nth_element(a + pl, a + ph, a + n, less(dim));
+            std::sort(a + pl, a + ph, less(dim));
+            const int tag = 4 * ((int)(lo / DE_BLOCK) + 1);
+            auto mark = [&tc, tag](int64_t v) {
+                const int m = tc.side[v].load(std::memory_order_relaxed);
+                return (m & ~3) == tag ? (m & 3) : 0;
+            };
+            for (size_t i = 0; i < n; i++)
+                tc.side[a[i]].store(tag + (i < pl ? 1 : 2), std::memory_order_relaxed);
+            long long cut = 0;
+            for (size_t i = 0; i < pl; i++)
+                for (int j = 0; j < 3; j++) {
+                    const int64_t v = tc.nbr[3 * (size_t)a[i] + j];
+                    if (v >= 0 && mark(v) == 2) cut++;
+                }
+            for (size_t i = pl; i <= ph; i++) {
+                if ((i % DE_BLOCK) == 0) {
+                    const size_t r = i / DE_BLOCK;
+                    const double off = std::fabs((double)r - 0.5 * (double)runs);
+                    if (best_cut < 0 || cut < best_cut || (cut == best_cut && off < best_off)) {
+                        best_cut = cut;
+                        best_off = off;
+                        best_dim = dim;
+                        best_runs = r;
+                    }
+                }
+                if (i == ph) break;
+                const int e = a[i];   // moves to the first side
+                for (int j = 0; j < 3; j++) {
+                    const int64_t v = tc.nbr[3 * (size_t)e + j];
+                    if (v < 0 || v == e) continue;
+                    const int sv = mark(v);
+                    if (sv == 1) cut--;
+                    else if (sv == 2) cut++;
+                }
+                tc.side[e].store(tag + 1, std::memory_order_relaxed);
+            }
+            for (size_t i = 0; i < n; i++) tc.side[a[i]].store(0, std::memory_order_relaxed);
+        }
+    } else {
+        double x0 = HUGE_VAL, y0 = HUGE_VAL, x1 = -HUGE_VAL, y1 = -HUGE_VAL;
+        for (size_t i = lo; i < hi; i++) {
+            const int k = ids[i];
+            x0 = std::min(x0, cen[2 * k]);
+            x1 = std::max(x1, cen[2 * k]);
+            y0 = std::min(y0, cen[2 * k + 1]);
+            y1 = std::max(y1, cen[2 * k + 1]);
+        }
+        best_dim = (x1 - x0 >= y1 - y0) ? 0 : 1;
+    }
+    const size_t nl = best_runs * (size_t)DE_BLOCK;
+    std::nth_element(ids.begin() + lo, ids.begin() + lo + nl, ids.begin() + hi, less(best_dim));
+    if (depth < 4 && n > ((size_t)1 << 16)) {
+        // the first part on its own thread (16 at most); on one thread when
+        // none can be started
+        std::thread first;
+        try {
+            first = std::thread(tile_bisect, std::ref(tc), std::ref(ids), lo, lo + nl, depth + 1);
+        } catch (...) {
+            tile_bisect(tc, ids, lo, lo + nl, depth + 1);
+        }
+        tile_bisect(tc, ids, lo + nl, hi, depth + 1);
+        if (first.joinable()) first.join();
+    } else {
+        tile_bisect(tc, ids, lo, lo + nl, depth + 1);
+        tile_bisect(tc, ids, lo + nl, hi, depth + 1);
+    }
 }
 
 // Triangles in tile order: consecutive runs of DE_BLOCK triangles are compact
-// patches (recursive coordinate bisection of the centroids), so the halo the
-// fused kernel extrapolates twice is short, and consecutive runs are
-// neighbouring patches.
-static std::vector<int> tile_order(int N, const double *cen)
+// patches (the bisection above), so the halo the fused kernel extrapolates
+// twice is short, and consecutive runs are neighbouring patches.
+static std::vector<int> tile_order(int N, const double *cen, const int64_t *nbr)
 {
     std::vector<int> o(N);
     for (int k = 0; k < N; k++) o[k] = k;
-    tile_bisect(o, 0, (size_t)N, cen);
+    TileCut tc{cen, nbr, std::vector<int>(nbr ? N : 0), nullptr};
+    if (nbr) {
+        tc.side.reset(new std::atomic<int>[N]);
+        for (int k = 0; k < N; k++) tc.side[k].store(0, std::memory_order_relaxed);
+    }
+    tile_bisect(tc, o, 0, (size_t)N, 0);
     return o;
 }
 
@@ -1280,7 +1381,7 @@ static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
     const int64_t *nbr = mesh->neighbours, *nbe = mesh->neighbour_edges;
     const int64_t *sur = mesh->surrogate_neighbours, *nob = mesh->number_of_boundaries;
     std::vector<int> perm(N), iperm(N);
-    if (d->P.reorder == 2) perm = tile_order(N, mesh->centroid_coordinates);
+    if (d->P.reorder == 2) perm = tile_order(N, mesh->centroid_coordinates, mesh->neighbours);
     else if (d->P.reorder) perm = rcm_order(N, nbr);
     else
         for (int k = 0; k < N; k++) perm[k] = k;
@@ -2983,10 +3084,14 @@ int anuga_build_config(int64_t *block, int64_t *mixed_waves)
     return ANUGA_OK;
 }
 
-int anuga_tile_order(int64_t N, const double *centroids, int64_t *order)
+int anuga_tile_order(int64_t N, const double *centroids, const int64_t *neighbours,
+                     int64_t *order)
 {
     if (N < 0 || N >= (int64_t(1) << 29) || (N > 0 && (!centroids || !order))) return ANUGA_E_ARG;
-    const std::vector<int> o = tile_order((int)N, centroids);
+    if (neighbours)
+        for (int64_t i = 0; i < 3 * N; i++)
+            if (neighbours[i] >= N) return ANUGA_E_ARG;
+    const std::vector<int> o = tile_order((int)N, centroids, neighbours);
     for (int64_t k = 0; k < N; k++) order[k] = o[k];
     return ANUGA_OK;
 }

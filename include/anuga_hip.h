@@ -449,14 +449,19 @@ int anuga_rcm_order(int64_t N, const int64_t *neighbours, int64_t *order);
 int anuga_build_config(int64_t *block, int64_t *mixed_waves);
 
 /* The tile order applied at creation when params->reorder is 2: recursive
- * coordinate bisection of the centroids into runs of the workgroup size (the
- * longer side of each part's bounding box is cut so that the first part holds
- * half of the runs; ties by index), each run in increasing triangle index;
- * order[new] = old. centroids is (N,2). Host-only. No counterpart in the
- * reference (its triangles are processed in the mesh's order); it exists so
- * that runs of 256 consecutive triangles are compact tiles, with short halos,
- * for the fused extrapolation + flux kernel. */
-int anuga_tile_order(int64_t N, const double *centroids, int64_t *order);
+ * coordinate bisection of the centroids into runs of the workgroup size. Each
+ * cut is across x or y and leaves a whole number of runs on its first side,
+ * 40% to 60% of them and within 64 runs of the middle; with neighbours (N,3; negative on boundary edges) the
+ * cut separating the fewest adjacent pairs is taken (then the one nearest the
+ * middle, then x); without them, and for parts of more than 4096 runs, the
+ * longer side is cut at the middle run.
+ * Ties by index; each run in increasing triangle index; order[new] = old.
+ * centroids is (N,2). Host-only. No counterpart in the reference (its
+ * triangles are processed in the mesh's order); it exists so that runs of
+ * 256 consecutive triangles are compact tiles, with short halos, for the
+ * fused extrapolation + flux kernel. */
+int anuga_tile_order(int64_t N, const double *centroids, const int64_t *neighbours,
+                     int64_t *order);
 
 /* The fused kernel's tile metadata, for triangles already in internal order:
  * nb is (N,3), each interior edge packed as (n << 2) | e (n the neighbour, e
