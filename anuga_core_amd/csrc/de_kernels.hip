@@ -1343,22 +1343,10 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
         qk1 = q[k];
         if (VG) tv1 = m.tv[k];
     }
-    // phase 4's own inputs, loaded now so that their latency hides behind
-    // the extrapolation and the edge list
-    double area_k = 1.0, eta_k = 0.0;
-    DeQ bk_k{0.0, 0.0, 0.0, 0.0};
-    unsigned sl_k = 0;
-    if (k < N) {
-        area_k = m.area[k];
-        sl_k = tl.slot[k];
-        if (SUBSTEP) {
-            eta_k = eta[k];
-            bk_k = bk[k];
-        }
-    }
-    // the edge-list entries of phases 2 and 3, likewise: hw waves take the
-    // halo in phase 2 (at least one wave left for the tile's edges), each
-    // thread of the others one edge inside the tile
+    // the edge-list entries of phases 2 and 3, loaded now so that their
+    // latency hides behind the extrapolation: hw waves take the halo in
+    // phase 2 (at least one wave left for the tile's edges), each thread of
+    // the others one edge inside the tile
     const int h0 = tl.start[lb], nh = tl.start[lb + 1] - h0;
     const int e0 = tl.estart[lb], es = tl.esplit[lb], e1 = tl.estart[lb + 1];
     const int hw = min((nh + 63) >> 6, DE_BLOCK / 64 - 1);
@@ -1443,20 +1431,28 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
             de_tile_edge<SUBSTEP, VG>(m, p, en, tva, k0, lds, hl, hz, hcap, dtmin);
         }
     }
-    // phase 4's records, requested before the barrier (substep 0: the
-    // later substeps' update needs the registers)
-    DeQ qk4{0.0, 0.0, 0.0, 0.0};
+    // phase 4's inputs, requested before the barrier (held from the
+    // kernel's start they cost registers in every phase)
+    DeQ qk4{0.0, 0.0, 0.0, 0.0}, bk_k{0.0, 0.0, 0.0, 0.0};
     int4 vk4 = make_int4(-1, -1, -1, 3);
-    if (SUBSTEP == 0 && k < kend) {
+    double eta_k = 0.0, area_k = 1.0;
+    unsigned sl_k = 0;
+    if (k < kend) {
         qk4 = q[k];
         vk4 = m.nb[k];
+        area_k = m.area[k];
+        sl_k = tl.slot[k];
+        if (SUBSTEP) {
+            eta_k = eta[k];
+            bk_k = bk[k];
+        }
     }
     __syncthreads();
 
     // phase 4: each triangle's three edges in local order
     if (k < kend) {
-        const DeCentroid c = de_centroid(SUBSTEP == 0 ? qk4 : q[k], fk);
-        const int4 vk = SUBSTEP == 0 ? vk4 : m.nb[k];
+        const DeCentroid c = de_centroid(qk4, fk);
+        const int4 vk = vk4;
         const int v[3] = {vk.x, vk.y, vk.z};
         const unsigned sl = sl_k;
         double sw = 0.0, sx = 0.0, sy = 0.0;
