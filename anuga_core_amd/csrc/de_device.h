@@ -99,15 +99,11 @@ struct DeMesh {
     const int *__restrict__ rw;
     const DeRiverwall *__restrict__ rwd;
     // the caller's index of each triangle within its scenario (the
-    // reference's loop order, for the low-Froude sums), and the vertex
-    // coordinates [N][6] (sloped friction); read by the general kernels only
+    // reference's loop order, for the low-Froude sums)
     const int *__restrict__ caller;
+    // the vertex coordinates [N][3] (x, y): sloped friction, and the fused
+    // FP64 kernel's geometry where it reproduces the mesh's arrays
     const double *__restrict__ vx;
-    // vertex geometry (FP64 path, where it reproduces the mesh's arrays): per
-    // triangle its three vertices' node indices, and the node coordinates;
-    // null: the stored geometry records
-    const int4 *__restrict__ tv;
-    const double2 *__restrict__ node;
 };
 
 // Compact tiles of DE_BLOCK consecutive triangles (a space-filling-curve

@@ -118,11 +118,9 @@ struct anuga_domain {
     bool mirror = true;
     bool edge_once = true;     // ANUGA_HIP_EDGE_ONCE=0: the gather-every-side kernel
     // vertex geometry: the fused FP64 kernel forms edge offsets, normals and
-    // lengths from the vertices (tv, node) instead of reading the stored
-    // records, where that reproduces every scenario's arrays bit for bit
-    // (ANUGA_HIP_VERTEX_GEOMETRY=0: the stored records)
-    int4 *tv = nullptr;
-    double2 *node = nullptr;
+    // lengths from each triangle's vertex coordinates (vx) instead of reading
+    // the stored records, where that reproduces every scenario's arrays bit
+    // for bit (ANUGA_HIP_VERTEX_GEOMETRY=0: the stored records)
     bool vgeom = true;
     std::vector<int> ts_hcap;
     bool e_stale = false;                // E does not hold the last extrapolation
@@ -576,8 +574,6 @@ static DeMesh dev_mesh(const anuga_domain *d)
     m.rwd = d->rwd;
     m.caller = d->caller;
     m.vx = d->vx;
-    m.tv = d->vgeom ? d->tv : nullptr;
-    m.node = d->vgeom ? d->node : nullptr;
     return m;
 }
 
@@ -674,7 +670,7 @@ static void launch_xflux_t(anuga_domain *d, DeQ *S, DeQ *bk, DeQ *out)
         // each edge's flux once: tile and halo edge values, then the tile's
         // centroid depths and beds; geometry from the vertices where it can be
         const size_t lds = (14 * (size_t)DE_BLOCK + 12 * (size_t)d->hcap) * sizeof(double);
-        auto kern = (d->vgeom && d->tv) ? de_xflux_e<SUB, true> : de_xflux_e<SUB, false>;
+        auto kern = (d->vgeom && d->vx) ? de_xflux_e<SUB, true> : de_xflux_e<SUB, false>;
         hipLaunchKernelGGL(kern, dim3(d->nblocks), dim3(DE_BLOCK), lds, d->stream,
                            dev_mesh(d), dev_params(d->P), S, d->eta, tl, bc, d->flags, d->ex[0],
                            d->ex[1], d->ex[2], bk, out, d->ctl, d->part, d->part + d->nblocks,
@@ -1322,55 +1318,15 @@ static bool geometry_from_vertices(const anuga_mesh *mesh)
     return true;
 }
 
-// Nodes and per-triangle node indices of scenario s (internal order): the
-// vertices with identical coordinates are one node, numbered by first use, so
-// a tile's nodes are close together. Scenario s owns node slots
-// [3 kbeg, 3 (kbeg + ntri)). Turns the vertex geometry off (for every
-// scenario) when this mesh's arrays are not formed from its vertices.
+// The vertex geometry stays on only while every scenario's arrays are
+// formed from its vertices (the kernels then read the per-triangle vertex
+// coordinates, vx).
 static int upload_vertex_geometry(anuga_domain *d, int s, const anuga_mesh *mesh,
                                   const std::vector<int> &perm)
 {
-    if (!geometry_from_vertices(mesh)) {
-        d->vgeom = false;
-        return ANUGA_OK;
-    }
-    const int N = d->ntri[s], kb = d->kbeg[s];
-    const double *V = mesh->vertex_coordinates;
-    struct Key {
-        uint64_t x, y;
-        bool operator==(const Key &o) const { return x == o.x && y == o.y; }
-    };
-    struct Hash {
-        size_t operator()(const Key &k) const { return std::hash<uint64_t>()(k.x * 0x9E3779B97F4A7C15ull ^ k.y); }
-    };
-    std::unordered_map<Key, int, Hash> id;
-    id.reserve((size_t)N * 2);
-    std::vector<double2> nodes;
-    nodes.reserve((size_t)N);
-    std::vector<int4> tv(N);
-    const int base = 3 * kb;
-    for (int p = 0; p < N; p++) {
-        const double *v = V + 6 * (size_t)perm[p];
-        int t[3];
-        for (int i = 0; i < 3; i++) {
-            Key key;
-            std::memcpy(&key.x, v + 2 * i, 8);
-            std::memcpy(&key.y, v + 2 * i + 1, 8);
-            auto it = id.find(key);
-            if (it == id.end()) {
-                it = id.emplace(key, (int)nodes.size()).first;
-                nodes.push_back(make_double2(v[2 * i], v[2 * i + 1]));
-            }
-            t[i] = base + it->second;
-        }
-        tv[p] = make_int4(t[0], t[1], t[2], 0);
-    }
-    HIPCHK(d, hipStreamSynchronize(d->stream));
-    if (!d->tv) HIPCHK(d, dalloc(&d->tv, (size_t)d->N));
-    if (!d->node) HIPCHK(d, dalloc(&d->node, 3 * (size_t)d->N));
-    HIPCHK(d, hipMemcpy(d->tv + kb, tv.data(), N * sizeof(int4), hipMemcpyHostToDevice));
-    HIPCHK(d, hipMemcpy(d->node + base, nodes.data(), nodes.size() * sizeof(double2),
-                        hipMemcpyHostToDevice));
+    (void)s;
+    (void)perm;
+    if (!geometry_from_vertices(mesh)) d->vgeom = false;
     return ANUGA_OK;
 }
 
@@ -1922,8 +1878,8 @@ void anuga_destroy(anuga_domain *d)
                     d->scen, d->tstart, d->thalo, d->tslot, d->rw, d->rwd, d->caller, d->vx, d->node_start, d->node_tri, d->iperm_dev,
                     d->out_f32, d->gtri, d->gtimes, d->gvals, d->sub_idx, d->sub_val, d->eta, d->A, d->B, d->C, d->E, d->flags,
                     d->ex[0], d->ex[1], d->ex[2], d->bkind, d->bval, d->part, d->chunks, d->ctl, d->dsetup,
-                    d->dneg, d->hist, d->si[0], d->si[1], d->testart, d->tesplit, d->tedge, d->tv,
-                    d->node, d->lts.e0, d->lts.e1, d->lts.e2, d->lts.pg, d->lts.et, d->lts.fuf,
+                    d->dneg, d->hist, d->si[0], d->si[1], d->testart, d->tesplit, d->tedge,
+                    d->lts.e0, d->lts.e1, d->lts.e2, d->lts.pg, d->lts.et, d->lts.fuf,
                     d->lts.acf, d->lts.tmin, d->lts.unf};
     for (void *p : ptrs)
         if (p) hipFree(p);

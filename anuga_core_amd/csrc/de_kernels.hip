@@ -85,10 +85,22 @@ struct DeExtrap {
 // general_mesh.py forms the arrays the reference's extrapolation reads
 // (:599-626 edge midpoints 0.5 (Va + Vb); sw_domain_orig.c:2112-2135 their
 // offsets from the stored centroid).
-__device__ __forceinline__ void de_vertex_offsets(const DeMesh &m, int k, int4 t, double dxv[3],
-                                                  double dyv[3])
+// The three vertices of a triangle (its row of the per-triangle vertex
+// coordinates, the same doubles as the nodes they come from).
+struct DeVerts {
+    double2 v0, v1, v2;
+};
+
+__device__ __forceinline__ DeVerts de_load_verts(const DeMesh &m, int k)
 {
-    const double2 v0 = m.node[t.x], v1 = m.node[t.y], v2 = m.node[t.z];
+    const double2 *r = reinterpret_cast<const double2 *>(m.vx) + 3 * (size_t)k;
+    return DeVerts{r[0], r[1], r[2]};
+}
+
+__device__ __forceinline__ void de_vertex_offsets(const DeMesh &m, int k, const DeVerts &w,
+                                                  double dxv[3], double dyv[3])
+{
+    const double2 v0 = w.v0, v1 = w.v1, v2 = w.v2;
     const double2 c = m.cen[k];
     dxv[0] = 0.5 * (v1.x + v2.x) - c.x;
     dyv[0] = 0.5 * (v1.y + v2.y) - c.y;
@@ -101,24 +113,25 @@ __device__ __forceinline__ void de_vertex_offsets(const DeMesh &m, int k, int4 t
 // Outward normal and length of edge i of triangle k from its vertices
 // (general_mesh.py:202-232: the edge opposite vertex i, (x, y) =
 // V[i+2] - V[i+1], l = sqrt(x^2 + y^2), normal (y / l, -(x / l))).
-__device__ __forceinline__ void de_vertex_normal(const DeMesh &m, int4 t, int i, double &n1,
+__device__ __forceinline__ void de_vertex_normal(const DeMesh &m, int k, int i, double &n1,
                                                  double &n2, double &len)
 {
-    const int a = i == 0 ? t.y : (i == 1 ? t.z : t.x);
-    const int b = i == 0 ? t.z : (i == 1 ? t.x : t.y);
-    const double2 va = m.node[a], vb = m.node[b];
+    const double2 *r = reinterpret_cast<const double2 *>(m.vx) + 3 * (size_t)k;
+    const int a = i == 2 ? 0 : i + 1, b = i == 0 ? 2 : i - 1;
+    const double2 va = r[a], vb = r[b];
     const double xn = vb.x - va.x, yn = vb.y - va.y;
     len = sqrt(xn * xn + yn * yn);
     n1 = yn / len;
     n2 = -(xn / len);
 }
 
-// vk, qk, tvk: triangle k's neighbour record, state and vertex record (VG),
+// vk, qk, tvk: triangle k's neighbour record, state and vertices (VG),
 // loaded by the caller
 template <class T, bool VG = false>
 __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &p,
                                                const DeQ *__restrict__ q, int k, int4 vk,
-                                               DeQ qk, int4 tvk, int *err, DeExtrap<T> &x)
+                                               DeQ qk, const DeVerts &tvk, int *err,
+                                               DeExtrap<T> &x)
 {
     const double mah = p.mah;
     const int v[3] = {vk.x, vk.y, vk.z};
@@ -324,7 +337,7 @@ __device__ __forceinline__ void de_extrap_cell(const DeMesh &m, const DeParams &
                                                const DeQ *__restrict__ q, int k, int *err,
                                                DeExtrap<T> &x)
 {
-    const int4 tvk = VG ? m.tv[k] : make_int4(0, 0, 0, 0);
+    const DeVerts tvk = VG ? de_load_verts(m, k) : DeVerts{};
     de_extrap_cell<T, VG>(m, p, q, k, m.nb[k], q[k], tvk, err, x);
 }
 
@@ -1242,7 +1255,7 @@ __device__ __forceinline__ void de_block_partials(double dtmin, double bflux, do
 // is read at run time, so a wave can take edges of both kinds in one pass.
 template <int SUBSTEP, bool VG>
 __device__ __forceinline__ void de_tile_edge(const DeMesh &m, const DeParams &p, unsigned en,
-                                             int4 tva, int k0, double *lds, double *hl,
+                                             int k0, double *lds, double *hl,
                                              const double *hz, int hcap, double &dtmin)
 {
     constexpr int Q = 3 * DE_BLOCK;
@@ -1258,7 +1271,7 @@ __device__ __forceinline__ void de_tile_edge(const DeMesh &m, const DeParams &p,
     const double qr0 = R[0], hre = R[rs], qr1 = R[2 * rs], qr2 = R[3 * rs];
     double n1, n2, length;
     if constexpr (VG) {
-        de_vertex_normal(m, tva, ia, n1, n2, length);
+        de_vertex_normal(m, ka, ia, n1, n2, length);
     } else {
         n1 = m.nx[ia * N + ka];
         n2 = m.ny[ia * N + ka];
@@ -1336,12 +1349,13 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
     const int k0 = lb * DE_BLOCK, k = k0 + t;
     // phase 1's own records, requested before the control block is read
     // (bounded by the arrays' length, not yet by the scenario's end)
-    int4 vk1 = make_int4(-1, -1, -1, 3), tv1 = make_int4(0, 0, 0, 0);
+    int4 vk1 = make_int4(-1, -1, -1, 3);
+    DeVerts tv1{};
     DeQ qk1{0.0, 0.0, 0.0, 0.0};
     if (k < N) {
         vk1 = m.nb[k];
         qk1 = q[k];
-        if (VG) tv1 = m.tv[k];
+        if (VG) tv1 = de_load_verts(m, k);
     }
     // the edge-list entries of phases 2 and 3, loaded now so that their
     // latency hides behind the extrapolation: hw waves take the halo in
@@ -1387,14 +1401,10 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
         }
     }
     // phase 2's records, requested before the barrier: a halo thread's
-    // first triangle's neighbours, an edge's vertices
+    // first triangle's neighbours
     const bool hal2 = t < nh && t < 64 * hw;
     int4 rec2 = make_int4(-1, -1, -1, 3);
-    if (hal2) {
-        rec2 = m.nb[en2];
-    } else if (VG && has2) {
-        rec2 = m.tv[k0 + DE_TE_T(en2)];
-    }
+    if (hal2) rec2 = m.nb[en2];
     __syncthreads();
 
     // phase 2: the halo's extrapolation, and edges inside the tile
@@ -1403,7 +1413,7 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
             DeExtrap<double> x;
             const int kh = j == t ? (int)en2 : tl.halo[h0 + j];
             if (j != t) rec2 = m.nb[kh];   // a further halo triangle
-            const int4 tvh = VG ? m.tv[kh] : make_int4(0, 0, 0, 0);
+            const DeVerts tvh = VG ? de_load_verts(m, kh) : DeVerts{};
             de_extrap_cell<double, VG>(m, p, q, kh, rec2, q[kh], tvh, err, x);
 #pragma unroll
             for (int i = 0; i < 3; i++) {
@@ -1416,20 +1426,15 @@ de_xflux_e(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__rest
             }
         }
     } else if (has2) {
-        de_tile_edge<SUBSTEP, VG>(m, p, en2, rec2, k0, lds, hl, hz, hcap, dtmin);
+        de_tile_edge<SUBSTEP, VG>(m, p, en2, k0, lds, hl, hz, hcap, dtmin);
     }
-    // phase 3's first edge's vertices, requested before the barrier
-    const int4 tv3 = VG && e3 < e1 ? m.tv[k0 + DE_TE_T(en3)] : make_int4(0, 0, 0, 0);
     __syncthreads();
 
     // phase 3: the tile's remaining edges, then the edges to the halo
     if (e3 < e1) {
-        de_tile_edge<SUBSTEP, VG>(m, p, en3, tv3, k0, lds, hl, hz, hcap, dtmin);
-        for (int e = e3 + DE_BLOCK; e < e1; e += DE_BLOCK) {
-            const unsigned en = tl.edge[e];
-            const int4 tva = VG ? m.tv[k0 + DE_TE_T(en)] : make_int4(0, 0, 0, 0);
-            de_tile_edge<SUBSTEP, VG>(m, p, en, tva, k0, lds, hl, hz, hcap, dtmin);
-        }
+        de_tile_edge<SUBSTEP, VG>(m, p, en3, k0, lds, hl, hz, hcap, dtmin);
+        for (int e = e3 + DE_BLOCK; e < e1; e += DE_BLOCK)
+            de_tile_edge<SUBSTEP, VG>(m, p, tl.edge[e], k0, lds, hl, hz, hcap, dtmin);
     }
     // phase 4's inputs, requested before the barrier (held from the
     // kernel's start they cost registers in every phase)
@@ -1734,7 +1739,7 @@ de_xflux_mixed(DeMesh m, DeParams p, const DeQ *__restrict__ q, const double *__
 
     if (k < kend) {
         DeExtrap<float> x;
-        de_extrap_cell<float>(m, p, q, k, vk1, qk1, make_int4(0, 0, 0, 0), err, x);
+        de_extrap_cell<float>(m, p, q, k, vk1, qk1, DeVerts{}, err, x);
         mass = x.mass;
         fk = x.flag;
         flags[k] = (unsigned char)fk;
