@@ -85,14 +85,20 @@ LAUNCHES_PER_STEP = {"extrapolate": 2, "flux0": 1, "update": 1, "flux1": 1, "xfl
                      "xflux1": 1}
 STEP_BYTES = {"fp64": 488, "mixed": 344}
 PROFILE = os.path.join(ROOT, "profiles", "de_kernels_rectangular_cross2000_DE1_%s.txt")
+PROFILE_GRADED = os.path.join(ROOT, "profiles", "de_kernels_graded_square4M_DE1_%s.txt")
 
 
-def measured_traffic(precision, kernel, m):
+def measured_traffic(precision, kernel, m, kind="rectangular_cross"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
     this workload (FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy), or None
-    when there is no summary for this mesh size."""
-    path = PROFILE % precision
-    if m != 2000 or not os.path.exists(path):
+    when there is no summary for this mesh and kernel."""
+    if kind == "graded":
+        path = PROFILE_GRADED % precision
+    elif m == 2000:
+        path = PROFILE % precision
+    else:
+        return None
+    if not os.path.exists(path):
         return None
     header = None
     with open(path) as f:
@@ -613,7 +619,7 @@ def main():
     dom_short = max(ran, key=lambda s: kernels[label[s]]["avg_ms"] * LAUNCHES_PER_STEP[s])
     dom, dom_bytes = label[dom_short], KERNEL_BYTES[args.precision][dom_short][1]
     ach = kernels[dom]["alg_GBs"]
-    traffic = measured_traffic(args.precision, dom, args.m) if args.mesh != "graded" else None
+    traffic = measured_traffic(args.precision, dom, args.m, args.mesh)
     step_gbs = STEP_BYTES[args.precision] * N / (r["ms_per_step"] * 1e-3) / 1e9
 
     out = {
