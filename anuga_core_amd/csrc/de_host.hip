@@ -1321,13 +1321,9 @@ static bool geometry_from_vertices(const anuga_mesh *mesh)
 // The vertex geometry stays on only while every scenario's arrays are
 // formed from its vertices (the kernels then read the per-triangle vertex
 // coordinates, vx).
-static int upload_vertex_geometry(anuga_domain *d, int s, const anuga_mesh *mesh,
-                                  const std::vector<int> &perm)
+static void check_vertex_geometry(anuga_domain *d, const anuga_mesh *mesh)
 {
-    (void)s;
-    (void)perm;
     if (!geometry_from_vertices(mesh)) d->vgeom = false;
-    return ANUGA_OK;
 }
 
 static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
@@ -1455,10 +1451,7 @@ static int upload_mesh(anuga_domain *d, int s, const anuga_mesh *mesh)
         HIPCHK(d, hipMemcpy(d->vx + 6 * (size_t)kb, hv.data(), 6 * (size_t)N * sizeof(double),
                             hipMemcpyHostToDevice));
     }
-    if (d->vgeom) {
-        int rc = upload_vertex_geometry(d, s, mesh, perm);
-        if (rc) return rc;
-    }
+    if (d->vgeom) check_vertex_geometry(d, mesh);
     if (d->P.reorder == 2) {
         int rc = scenario_tiles(d, s, hn);
         if (rc) return rc;
